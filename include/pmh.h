@@ -1,0 +1,200 @@
+/*
+ * pmh.h -- C ABI of the MI355X (gfx950) execution backend for polars-metal's
+ * relational hot path: predicate filter, stream compaction, hash / u128
+ * radix-sort group-by, fused sum/count/len/min/max/mean.
+ *
+ * Every entry point below replaces one function of the reference's PyO3
+ * module `polars_metal._native` (crates/polars-metal-core/src/lib.rs:57-83);
+ * the comment on each names the reference function it stands in for.  Plans
+ * travel as JSON text with exactly the keys of the reference's Python dicts.
+ *
+ * Conventions
+ *   - Status: every function returns PMH_OK (0) or a PMH_E_* code; the
+ *     message is in pmh_last_error() (thread-local).  The codes map onto the
+ *     reference's Python exception classes (see pmh_status).
+ *   - Buffers: Arrow little-endian value buffers and LSB-first validity
+ *     bitmaps.  A NULL validity pointer means "all valid".  Inputs may live
+ *     in host memory or in device memory (pmh_column.location).
+ *   - Output validity bitmaps are padded to a multiple of 4 bytes, minimum 4
+ *     (crates/polars-metal-core/src/udf/common.rs:74-84).
+ *   - Group order in results is unspecified; callers sort by key
+ *     (crates/polars-metal-core/src/udf/groupby/mod.rs:720-746).
+ *   - Row counts must be below 2^31.
+ *   - Calls are serialised per process; no entry point may be called from two
+ *     host threads at once on the same device.
+ */
+#ifndef PMH_H
+#define PMH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  Mapping to the reference's exceptions:
+ *   PMH_E_VALUE    -> ValueError        (plan parse, bad dtype/op, groupby/mod.rs:755-756)
+ *   PMH_E_KEY      -> KeyError          (missing column, groupby/mod.rs:809-814)
+ *   PMH_E_NOTIMPL  -> NotImplementedError (unsupported key dtype, groupby/mod.rs:952-954)
+ *   PMH_E_RUNTIME  -> RuntimeError      (compaction failure, udf/compact.rs:97-101)
+ *   PMH_E_COMPUTE  -> polars.exceptions.ComputeError (device failure, error.rs:19-33) */
+typedef enum {
+    PMH_OK = 0,
+    PMH_E_VALUE = 1,
+    PMH_E_KEY = 2,
+    PMH_E_NOTIMPL = 3,
+    PMH_E_RUNTIME = 4,
+    PMH_E_COMPUTE = 5
+} pmh_status;
+
+/* Wire dtype tags (crates/polars-metal-core/src/plan/mod.rs:78-101).
+ * Utf8 key columns arrive as u32 dictionary codes. */
+typedef enum {
+    PMH_I64 = 0, PMH_F64 = 1, PMH_BOOL = 2, PMH_I32 = 3, PMH_F32 = 4, PMH_I8 = 5,
+    PMH_I16 = 6, PMH_U8 = 7, PMH_U16 = 8, PMH_U32 = 9, PMH_U64 = 10, PMH_UTF8 = 11
+} pmh_dtype;
+
+/* Comparison operators (crates/polars-metal-kernels/src/cmp.rs CompareOp). */
+typedef enum { PMH_EQ = 0, PMH_NE = 1, PMH_LT = 2, PMH_LE = 3, PMH_GT = 4, PMH_GE = 5 } pmh_cmp_op;
+
+typedef enum { PMH_HOST = 0, PMH_DEVICE = 1 } pmh_location;
+
+/* One input column: the C form of the reference's
+ * `(name, dtype_tag, data: bytes, valid: bytes)` tuple
+ * (crates/polars-metal-core/src/udf/groupby/mod.rs:720-746). */
+typedef struct pmh_column {
+    const char *name;
+    int32_t tag;        /* pmh_dtype */
+    int32_t location;   /* pmh_location of data and valid */
+    const void *data;
+    uint64_t data_len;  /* bytes */
+    const uint8_t *valid;
+    uint64_t valid_len; /* bytes; ignored when valid == NULL */
+} pmh_column;
+
+/* One output column: the C form of the reference's output tuple
+ * `(name, tag, data, valid)` (groupby/mod.rs:1109-1143). */
+typedef struct pmh_out_column {
+    const char *name;
+    int32_t tag;
+    int32_t location;
+    void *data;
+    uint64_t data_len;
+    uint8_t *valid;
+    uint64_t valid_len;
+} pmh_out_column;
+
+/* Group-by path selection recorded for each call (no reference
+ * counterpart: the reference logs nothing, SURVEY of _callback.py:22-125). */
+typedef enum { PMH_METHOD_HASH = 0, PMH_METHOD_SORT = 1 } pmh_method;
+
+typedef struct pmh_exec_stats {
+    int32_t method;           /* pmh_method that produced the result */
+    int32_t n_passes;         /* hash passes run (1 + resizes) */
+    uint64_t initial_capacity;/* first hash-table capacity tried */
+    uint64_t final_capacity;  /* capacity of the pass that succeeded */
+    uint64_t n_groups;
+    uint64_t n_selected;      /* rows that passed the predicate */
+    double device_ms;         /* device time of the call (HIP events) */
+} pmh_exec_stats;
+
+typedef struct pmh_result {
+    uint64_t n_rows;          /* rows in every output column */
+    uint32_t n_columns;
+    pmh_out_column *columns;
+    pmh_exec_stats stats;
+} pmh_result;
+
+/* Output placement flags for the fused entry. */
+#define PMH_OUT_HOST 0u
+/* Keep outputs in device memory owned by the library; they stay valid until
+ * the next call on the device or pmh_result_free. */
+#define PMH_OUT_DEVICE 1u
+
+/* Last error message of the calling thread ("" if none). */
+const char *pmh_last_error(void);
+
+/* Replaces _native.version_string (crates/polars-metal-core/src/lib.rs:31-35). */
+const char *pmh_version_string(void);
+
+/* Replaces _native.device_name (crates/polars-metal-core/src/lib.rs:36-40).
+ * Writes the device name into buf (NUL-terminated). */
+int pmh_device_name(char *buf, size_t buf_len);
+
+/* Replaces _native.execute_groupby
+ * (crates/polars-metal-core/src/udf/groupby/mod.rs:747-1144).
+ * plan_json: {"keys": [[name, tag], ...], "aggs": [{"kind": "Simple"|"Length",
+ * "input_col": ..., "op": "Sum|Mean|Count|Min|Max|Len", "output_alias": ...}],
+ * "options"?: {...}}.  Outputs: keys in `keys` order, then aggs in `aggs`
+ * order (mod.rs:1109-1143). */
+int pmh_execute_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
+                        uint32_t n_columns, pmh_result **out);
+
+/* Fused filter + group-by.  No single reference function: it replaces the
+ * pair _udf._filter_via_polars (python/polars_metal/_udf.py:550-597, Polars
+ * CPU compaction) + _native.execute_groupby (groupby/mod.rs:747) on the
+ * GroupBy(Filter(Scan)) shape.  plan_json is the execute_groupby plan plus an
+ * optional "predicate" in the reference's PredicateAst dict form
+ * (crates/polars-metal-core/src/udf/predicate.rs:92-200).
+ * flags: PMH_OUT_HOST or PMH_OUT_DEVICE. */
+int pmh_execute_filter_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
+                               uint32_t n_columns, uint32_t flags, pmh_result **out);
+
+/* Replaces _native.execute_filter_compact
+ * (crates/polars-metal-core/src/udf/compact.rs:46-147): keep = pred_data AND
+ * pred_valid (bit-packed, host memory); every column (I64/F64/Bool/I32/F32/
+ * I16/I8/U8/U16/U32/U64) is compacted stably.  Result column i holds column i
+ * compacted; result->n_rows is n_out. */
+int pmh_execute_filter_compact(const uint8_t *pred_data, const uint8_t *pred_valid, uint64_t n_rows,
+                               const pmh_column *columns, uint32_t n_columns, pmh_result **out);
+
+/* Replace _native.cmp_{i64,f64}_col_scalar / _col_col
+ * (crates/polars-metal-core/src/udf/compare.rs:23-145).  Host buffers.
+ * Outputs are bit-packed, ceil(n/8) bytes rounded up to 4 (min 4); out_data
+ * is valid AND (lhs op rhs); float compares use Polars' TotalOrd (NaN equal
+ * to NaN and above every number). */
+int pmh_cmp_i64_col_scalar(const int64_t *lhs, const uint8_t *lhs_valid, int64_t rhs, int op,
+                           uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid);
+int pmh_cmp_f64_col_scalar(const double *lhs, const uint8_t *lhs_valid, double rhs, int op,
+                           uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid);
+int pmh_cmp_i64_col_col(const int64_t *lhs, const uint8_t *lhs_valid, const int64_t *rhs,
+                        const uint8_t *rhs_valid, int op, uint64_t n_rows, uint8_t *out_data,
+                        uint8_t *out_valid);
+int pmh_cmp_f64_col_col(const double *lhs, const uint8_t *lhs_valid, const double *rhs,
+                        const uint8_t *rhs_valid, int op, uint64_t n_rows, uint8_t *out_data,
+                        uint8_t *out_valid);
+
+/* Replace _native.bool_and_dispatch / bool_or_dispatch
+ * (crates/polars-metal-core/src/udf/logical.rs:26,44): Kleene AND / OR. */
+int pmh_bool_and(const uint8_t *ld, const uint8_t *lv, const uint8_t *rd, const uint8_t *rv,
+                 uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid);
+int pmh_bool_or(const uint8_t *ld, const uint8_t *lv, const uint8_t *rd, const uint8_t *rv,
+                uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid);
+
+/* Replaces _native.compute_lifting_plan
+ * (crates/polars-metal-core/src/router_udf.rs:22-40).  Host-only; no device
+ * needed.  *out_json receives {"<Kind>#<seq>": "gpu_lift"|"cpu_leave"|
+ * "fallback:<reason>"}; free it with pmh_string_free. */
+int pmh_compute_lifting_plan(const char *plan_json, char **out_json);
+
+/* Replaces _native.warmup_common_fused_signatures
+ * (crates/polars-metal-core/src/udf/groupby/mod.rs:187-201): initialises the
+ * device, stream and scratch arena; returns the number of kernels warmed. */
+int pmh_warmup_common_fused_signatures(void);
+
+void pmh_result_free(pmh_result *r);
+void pmh_string_free(char *s);
+
+/* Device-time instrumentation (no reference counterpart; the reference's
+ * UDF returns empty timings, python/polars_metal/_udf.py:85-87).  When
+ * enabled, every kernel launch is bracketed by HIP events on the launch
+ * stream; pmh_kernel_times returns, per kernel name, the number of launches
+ * and the summed milliseconds as JSON {"name": [launches, ms], ...}. */
+int pmh_set_profiling(int enabled);
+int pmh_kernel_times(char **out_json);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMH_H */

@@ -1,0 +1,189 @@
+// pmh_abi.cpp -- the extern "C" entry points declared in include/pmh.h.
+// Each converts library exceptions into a status code plus a thread-local
+// message, the C form of the reference's PyErr mapping
+// (crates/polars-metal-core/src/error.rs:19-33).
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <sstream>
+
+#include "pmh_internal.h"
+
+namespace pmh {
+const char *last_error();
+}
+
+using namespace pmh;
+
+namespace {
+std::mutex g_call_mutex;
+
+template <class F>
+int guarded(F &&f) {
+    std::lock_guard<std::mutex> lk(g_call_mutex);
+    try {
+        set_last_error("");
+        f();
+        return PMH_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("polars_metal: host allocation failed");
+        return PMH_E_COMPUTE;
+    } catch (const std::exception &e) {
+        set_last_error(std::string("polars_metal: ") + e.what());
+        return PMH_E_RUNTIME;
+    }
+}
+
+char *dup_string(const std::string &s) {
+    char *p = (char *)malloc(s.size() + 1);
+    memcpy(p, s.c_str(), s.size() + 1);
+    return p;
+}
+}  // namespace
+
+extern "C" {
+
+const char *pmh_last_error(void) { return last_error(); }
+
+const char *pmh_version_string(void) { return "0.1.0-mi355x"; }
+
+int pmh_device_name(char *buf, size_t buf_len) {
+    return guarded([&] {
+        if (!buf || buf_len == 0) fail(PMH_E_VALUE, "polars_metal: device_name needs a buffer");
+        Runtime &r = rt();
+        std::string nm = r.prop.name;
+        if (r.prop.gcnArchName[0]) nm += std::string(" (") + r.prop.gcnArchName + ")";
+        strncpy(buf, nm.c_str(), buf_len - 1);
+        buf[buf_len - 1] = 0;
+    });
+}
+
+int pmh_execute_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns, uint32_t n_columns,
+                        pmh_result **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        *out = nullptr;
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        GroupByPlan plan = parse_groupby_plan(plan_json, false);
+        *out = run_filter_groupby(plan, n_rows, columns, n_columns, PMH_OUT_HOST);
+    });
+}
+
+int pmh_execute_filter_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
+                               uint32_t n_columns, uint32_t flags, pmh_result **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        *out = nullptr;
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        if (flags & ~PMH_OUT_DEVICE) fail(PMH_E_VALUE, "polars_metal: unknown flags");
+        GroupByPlan plan = parse_groupby_plan(plan_json, true);
+        *out = run_filter_groupby(plan, n_rows, columns, n_columns, flags);
+    });
+}
+
+int pmh_execute_filter_compact(const uint8_t *pred_data, const uint8_t *pred_valid, uint64_t n_rows,
+                               const pmh_column *columns, uint32_t n_columns, pmh_result **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        *out = nullptr;
+        *out = run_filter_compact(pred_data, pred_valid, n_rows, columns, n_columns);
+    });
+}
+
+int pmh_cmp_i64_col_scalar(const int64_t *lhs, const uint8_t *lhs_valid, int64_t rhs, int op, uint64_t n_rows,
+                           uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_cmp(0, 0, lhs, lhs_valid, nullptr, nullptr, rhs, 0.0, op, n_rows, out_data, out_valid); });
+}
+
+int pmh_cmp_f64_col_scalar(const double *lhs, const uint8_t *lhs_valid, double rhs, int op, uint64_t n_rows,
+                           uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_cmp(1, 0, lhs, lhs_valid, nullptr, nullptr, 0, rhs, op, n_rows, out_data, out_valid); });
+}
+
+int pmh_cmp_i64_col_col(const int64_t *lhs, const uint8_t *lhs_valid, const int64_t *rhs, const uint8_t *rhs_valid,
+                        int op, uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_cmp(0, 1, lhs, lhs_valid, rhs, rhs_valid, 0, 0.0, op, n_rows, out_data, out_valid); });
+}
+
+int pmh_cmp_f64_col_col(const double *lhs, const uint8_t *lhs_valid, const double *rhs, const uint8_t *rhs_valid,
+                        int op, uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_cmp(1, 1, lhs, lhs_valid, rhs, rhs_valid, 0, 0.0, op, n_rows, out_data, out_valid); });
+}
+
+int pmh_bool_and(const uint8_t *ld, const uint8_t *lv, const uint8_t *rd, const uint8_t *rv, uint64_t n_rows,
+                 uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_bool(0, ld, lv, rd, rv, n_rows, out_data, out_valid); });
+}
+
+int pmh_bool_or(const uint8_t *ld, const uint8_t *lv, const uint8_t *rd, const uint8_t *rv, uint64_t n_rows,
+                uint8_t *out_data, uint8_t *out_valid) {
+    return guarded([&] { run_bool(1, ld, lv, rd, rv, n_rows, out_data, out_valid); });
+}
+
+int pmh_compute_lifting_plan(const char *plan_json, char **out_json) {
+    return guarded([&] {
+        if (!out_json) fail(PMH_E_VALUE, "polars_metal: out_json is NULL");
+        *out_json = nullptr;
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan is NULL");
+        *out_json = dup_string(compute_lifting_plan_json(plan_json));
+    });
+}
+
+int pmh_warmup_common_fused_signatures(void) {
+    int n = 0;
+    int st = guarded([&] {
+        Runtime &r = rt();
+        // touch the arena so the first query does not pay for it
+        slab("gb:launch", 16384);
+        PMH_HIP(hipStreamSynchronize(r.stream));
+        n = 1;
+    });
+    return st == PMH_OK ? n : -st;
+}
+
+void pmh_result_free(pmh_result *r) {
+    if (!r) return;
+    for (uint32_t i = 0; i < r->n_columns; ++i) {
+        pmh_out_column &c = r->columns[i];
+        free((void *)c.name);
+        if (c.location == PMH_HOST) {
+            free(c.data);
+            free(c.valid);
+        }
+    }
+    free(r->columns);
+    free(r);
+}
+
+void pmh_string_free(char *s) { free(s); }
+
+int pmh_set_profiling(int enabled) {
+    return guarded([&] {
+        Runtime &r = rt();
+        r.profiling = enabled != 0;
+        if (!enabled) r.ktimes.clear();
+    });
+}
+
+int pmh_kernel_times(char **out_json) {
+    return guarded([&] {
+        if (!out_json) fail(PMH_E_VALUE, "polars_metal: out_json is NULL");
+        Runtime &r = rt();
+        collect_kernel_times();
+        std::ostringstream os;
+        os << "{";
+        bool first = true;
+        for (auto &kv : r.ktimes) {
+            if (!first) os << ", ";
+            first = false;
+            os << "\"" << kv.first << "\": [" << kv.second.first << ", " << kv.second.second << "]";
+        }
+        os << "}";
+        *out_json = dup_string(os.str());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,345 @@
+// pmh_device.h -- device helpers shared by the gfx950 kernels: column loads,
+// the stack-free predicate evaluator, key lanes, hashing, the exact f64
+// accumulator and the state-merge "sinks" for LDS and global tables.
+#pragma once
+
+#include "pmh_internal.h"
+
+namespace pmh {
+
+constexpr uint64_t kEmpty = ~0ull;
+
+// Every per-group state array merges with one of these operators.
+enum MergeOp : int32_t { MO_ADD32 = 0, MO_ADD64, MO_ADDF64, MO_MIN32, MO_MAX32, MO_MIN64, MO_MAX64, MO_OR32 };
+
+__host__ __device__ inline int mo_size(int op) {
+    return (op == MO_ADD32 || op == MO_MIN32 || op == MO_MAX32 || op == MO_OR32) ? 4 : 8;
+}
+__host__ __device__ inline uint64_t mo_identity(int op) {
+    switch (op) {
+    case MO_MIN32: return 0x7FFFFFFFull;
+    case MO_MAX32: return 0x80000000ull;
+    case MO_MIN64: return 0x7FFFFFFFFFFFFFFFull;
+    case MO_MAX64: return 0x8000000000000000ull;
+    default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column loads (the tag is wave-uniform, so the switch does not diverge)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool dvalid(const uint8_t *v, uint64_t r) {
+    return v ? ((v[r >> 3] >> (r & 7)) & 1) : true;
+}
+
+__device__ __forceinline__ int64_t dload_i64(const DevCol &c, uint64_t r) {
+    switch (c.tag) {
+    case PMH_I64: case PMH_U64: return ((const int64_t *)c.data)[r];
+    case PMH_I32: return ((const int32_t *)c.data)[r];
+    case PMH_U32: case PMH_UTF8: return ((const uint32_t *)c.data)[r];
+    case PMH_I16: return ((const int16_t *)c.data)[r];
+    case PMH_U16: return ((const uint16_t *)c.data)[r];
+    case PMH_I8: return ((const int8_t *)c.data)[r];
+    case PMH_U8: return ((const uint8_t *)c.data)[r];
+    case PMH_BOOL: return (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
+    case PMH_F64: return (int64_t)((const double *)c.data)[r];
+    case PMH_F32: return (int64_t)((const float *)c.data)[r];
+    default: return 0;
+    }
+}
+
+__device__ __forceinline__ double dload_f64(const DevCol &c, uint64_t r) {
+    if (c.tag == PMH_F64) return ((const double *)c.data)[r];
+    if (c.tag == PMH_F32) return (double)((const float *)c.data)[r];
+    return (double)dload_i64(c, r);
+}
+
+// Raw data bits, zero-extended: the key encoder's view of a value
+// (crates/polars-metal-kernels/src/groupby.rs:226-292).
+__device__ __forceinline__ uint64_t dload_bits(const DevCol &c, uint64_t r) {
+    switch (c.tag) {
+    case PMH_I64: case PMH_U64: case PMH_F64: return ((const uint64_t *)c.data)[r];
+    case PMH_I32: case PMH_U32: case PMH_F32: case PMH_UTF8: return ((const uint32_t *)c.data)[r];
+    case PMH_I16: case PMH_U16: return ((const uint16_t *)c.data)[r];
+    case PMH_I8: case PMH_U8: return ((const uint8_t *)c.data)[r];
+    case PMH_BOOL: return (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
+    default: return 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// predicate: node k's (data, valid) pair lives in bit k of two masks
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tot_cmp(double a, double b) {
+    // Polars TotalOrd: NaN == NaN, NaN above every number, -0.0 == +0.0.
+    bool an = a != a, bn = b != b;
+    if (an | bn) return (int)an - (int)bn;
+    return (a < b) ? -1 : ((a > b) ? 1 : 0);
+}
+
+__device__ __forceinline__ bool apply_cmp(int op, int c) {
+    switch (op) {
+    case PMH_EQ: return c == 0;
+    case PMH_NE: return c != 0;
+    case PMH_LT: return c < 0;
+    case PMH_LE: return c <= 0;
+    case PMH_GT: return c > 0;
+    default: return c >= 0;
+    }
+}
+
+__device__ __forceinline__ bool eval_pred(const QueryDesc *__restrict__ q, uint64_t r) {
+    const int n = q->n_pred;
+    if (n == 0) return true;
+    uint32_t D = 0, V = 0;
+    for (int k = 0; k < n; ++k) {
+        const PredNode &nd = q->pred[k];
+        bool d = false, v = true;
+        switch (nd.kind) {
+        case PK_COLBOOL: {
+            const DevCol &c = q->cols[nd.lhs];
+            v = dvalid(c.valid, r);
+            d = (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
+            break;
+        }
+        case PK_CONST: d = nd.cval != 0; break;
+        case PK_CMP: {
+            bool lv = true, rv = true;
+            int c;
+            if (nd.is_f64) {
+                double a = nd.lf, b = nd.rf;
+                if (nd.lkind == OK_COL) { const DevCol &cc = q->cols[nd.lhs]; lv = dvalid(cc.valid, r); a = dload_f64(cc, r); }
+                if (nd.rkind == OK_COL) { const DevCol &cc = q->cols[nd.rhs]; rv = dvalid(cc.valid, r); b = dload_f64(cc, r); }
+                c = tot_cmp(a, b);
+            } else {
+                int64_t a = nd.li, b = nd.ri;
+                if (nd.lkind == OK_COL) { const DevCol &cc = q->cols[nd.lhs]; lv = dvalid(cc.valid, r); a = dload_i64(cc, r); }
+                if (nd.rkind == OK_COL) { const DevCol &cc = q->cols[nd.rhs]; rv = dvalid(cc.valid, r); b = dload_i64(cc, r); }
+                c = (a < b) ? -1 : ((a > b) ? 1 : 0);
+            }
+            v = lv && rv;
+            d = v && apply_cmp(nd.op, c);
+            break;
+        }
+        default: {  // PK_AND / PK_OR, Kleene (shaders/logical_bool.metal:1-31)
+            const bool dom = nd.kind == PK_OR;
+            const bool ld = (D >> nd.lhs) & 1, lvv = (V >> nd.lhs) & 1;
+            const bool rd = (D >> nd.rhs) & 1, rvv = (V >> nd.rhs) & 1;
+            const bool lk = lvv && ld == dom, rk = rvv && rd == dom;
+            if (lk || rk) { v = true; d = dom; }
+            else if (lvv && rvv) { v = true; d = !dom; }
+            else { v = false; d = false; }
+            break;
+        }
+        }
+        D |= (uint32_t)d << k;
+        V |= (uint32_t)v << k;
+    }
+    return ((D & V) >> (n - 1)) & 1;
+}
+
+// ---------------------------------------------------------------------------
+// keys
+// ---------------------------------------------------------------------------
+struct Key {
+    uint64_t lo, hi;
+    int special;  // 0: table key; 1: the all-ones lane; 2: null (KM_RAW64)
+};
+
+__device__ __forceinline__ void or_shift128(uint64_t &lo, uint64_t &hi, uint64_t v, int sh) {
+    if (sh < 64) {
+        lo |= v << sh;
+        if (sh > 0) hi |= v >> (64 - sh);
+    } else {
+        hi |= v << (sh - 64);
+    }
+}
+
+__device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, uint64_t r) {
+    Key k{0, 0, 0};
+    const int mode = q->key_mode;
+    if (mode == KM_NONE) return k;
+    if (mode == KM_RAW64) {
+        const DevCol &c = q->cols[q->keys[0].col];
+        if (!dvalid(c.valid, r)) { k.special = 2; return k; }
+        k.lo = dload_bits(c, r);
+        if (k.lo == kEmpty) k.special = 1;
+        return k;
+    }
+    for (int i = 0; i < q->n_keys; ++i) {
+        const KeyField &f = q->keys[i];
+        const DevCol &c = q->cols[f.col];
+        if (!dvalid(c.valid, r)) or_shift128(k.lo, k.hi, 1, f.null_off);
+        else or_shift128(k.lo, k.hi, dload_bits(c, r), f.data_off);
+    }
+    if (q->allones_possible && k.lo == kEmpty && (mode == KM_LANE64 || k.hi == kEmpty)) k.special = 1;
+    return k;
+}
+
+__device__ __forceinline__ uint64_t hash_key(uint64_t lo, uint64_t hi) {
+    uint64_t x = lo ^ (hi * 0x9E3779B97F4A7C15ull);
+    x ^= x >> 31;
+    x *= 0xd6e8feb86659fd93ull;
+    x ^= x >> 32;
+    x *= 0xd6e8feb86659fd93ull;
+    x ^= x >> 32;
+    return x;
+}
+
+// f64 -> int64 whose signed order is the IEEE order of non-NaN doubles
+// (-0.0 sorts just below +0.0).
+__device__ __forceinline__ int64_t f64_order_key(double x) {
+    int64_t b = __double_as_longlong(x);
+    return b >= 0 ? b : (b ^ 0x7FFFFFFFFFFFFFFFll);
+}
+__device__ __forceinline__ double f64_from_order_key(int64_t k) {
+    int64_t b = k >= 0 ? k : (k ^ 0x7FFFFFFFFFFFFFFFll);
+    return __longlong_as_double(b);
+}
+
+// ---------------------------------------------------------------------------
+// state sinks
+// ---------------------------------------------------------------------------
+// Global table: state arrays of nslots entries each.
+struct GSink {
+    void *const *arr;
+    uint64_t nslots;
+    __device__ __forceinline__ void add32(int a, uint64_t s, uint32_t v) { atomicAdd((uint32_t *)arr[a] + s, v); }
+    __device__ __forceinline__ void add64(int a, uint64_t s, int64_t v) {
+        atomicAdd((unsigned long long *)arr[a] + s, (unsigned long long)v);
+    }
+    __device__ __forceinline__ void addf(int a, uint64_t s, double v) { unsafeAtomicAdd((double *)arr[a] + s, v); }
+    __device__ __forceinline__ void min32(int a, uint64_t s, int32_t v) { atomicMin((int *)arr[a] + s, v); }
+    __device__ __forceinline__ void max32(int a, uint64_t s, int32_t v) { atomicMax((int *)arr[a] + s, v); }
+    __device__ __forceinline__ void min64(int a, uint64_t s, int64_t v) { atomicMin((long long *)arr[a] + s, (long long)v); }
+    __device__ __forceinline__ void max64(int a, uint64_t s, int64_t v) { atomicMax((long long *)arr[a] + s, (long long)v); }
+    __device__ __forceinline__ void or32(int a, uint64_t s, uint32_t v) { atomicOr((uint32_t *)arr[a] + s, v); }
+    // limb k of the exact accumulator whose limb 0 is array a (limbs are
+    // allocated back to back, nslots apart)
+    __device__ __forceinline__ void addlimb(int a, int k, uint64_t s, int64_t v) {
+        atomicAdd((unsigned long long *)arr[a] + (uint64_t)k * nslots + s, (unsigned long long)v);
+    }
+};
+
+// Block-local table in LDS: array a starts at byte offset off[a].
+struct LSink {
+    uint8_t *base;
+    const uint32_t *off;
+    uint32_t nslots;
+    template <class T> __device__ __forceinline__ T *p(int a, uint32_t s) { return (T *)(base + off[a]) + s; }
+    __device__ __forceinline__ void add32(int a, uint32_t s, uint32_t v) { atomicAdd(p<uint32_t>(a, s), v); }
+    __device__ __forceinline__ void add64(int a, uint32_t s, int64_t v) {
+        atomicAdd(p<unsigned long long>(a, s), (unsigned long long)v);
+    }
+    __device__ __forceinline__ void addf(int a, uint32_t s, double v) { unsafeAtomicAdd(p<double>(a, s), v); }
+    __device__ __forceinline__ void min32(int a, uint32_t s, int32_t v) { atomicMin(p<int>(a, s), v); }
+    __device__ __forceinline__ void max32(int a, uint32_t s, int32_t v) { atomicMax(p<int>(a, s), v); }
+    __device__ __forceinline__ void min64(int a, uint32_t s, int64_t v) { atomicMin(p<long long>(a, s), (long long)v); }
+    __device__ __forceinline__ void max64(int a, uint32_t s, int64_t v) { atomicMax(p<long long>(a, s), (long long)v); }
+    __device__ __forceinline__ void or32(int a, uint32_t s, uint32_t v) { atomicOr(p<uint32_t>(a, s), v); }
+    __device__ __forceinline__ void addlimb(int a, int k, uint32_t s, int64_t v) {
+        atomicAdd(p<unsigned long long>(a, s) + (uint64_t)k * nslots, (unsigned long long)v);
+    }
+};
+
+// Exact f64 accumulation.  A finite x = +-m * 2^e whose bits all lie in
+// [2^-104, 2^56) is added as a 160-bit fixed-point integer split into five
+// 32-bit chunks; each chunk goes to its own int64 limb, so limbs never carry
+// and atomic adds commute exactly (2^31 additions of headroom per limb).
+// Values outside that window, and Inf/NaN, go to an f64 residual.
+template <class S, class Slot>
+__device__ __forceinline__ void add_f64_exact(S &s, int a_limb, int a_res, Slot slot, double x) {
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
+    uint64_t m = b & 0xFFFFFFFFFFFFFull;
+    int e;
+    if (E == 0x7FFu) { s.addf(a_res, slot, x); return; }
+    if (E == 0) {
+        if (m == 0) return;
+        e = -1074;
+    } else {
+        m |= 1ull << 52;
+        e = (int)E - 1075;
+    }
+    if (e < -kFixScale) {
+        int tz = __builtin_ctzll(m);
+        if (e + tz < -kFixScale) { s.addf(a_res, slot, x); return; }
+        m >>= (-kFixScale - e);
+        e = -kFixScale;
+    }
+    int p = e + kFixScale;
+    int msb = p + 63 - __builtin_clzll(m);
+    if (msb >= 32 * kLimbs) { s.addf(a_res, slot, x); return; }
+    int li = p >> 5, sh = p & 31;
+    uint64_t lo = m << sh;
+    uint64_t hi = sh ? (m >> (64 - sh)) : 0;
+    int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
+    if (b >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
+    if (c0) s.addlimb(a_limb, li, slot, c0);
+    if (c1) s.addlimb(a_limb, li + 1, slot, c1);
+    if (c2) s.addlimb(a_limb, li + 2, slot, c2);
+}
+
+// Correctly rounded double of sum_j limb[j] * 2^(32 j - 104), plus residual.
+__device__ __forceinline__ double limbs_to_double(const int64_t *L, double res) {
+    // 256-bit two's complement accumulation
+    uint64_t w[4] = {0, 0, 0, 0};
+    for (int j = 0; j < kLimbs; ++j) {
+        int64_t v = L[j];
+        int sh = 32 * j;
+        uint64_t ext = v < 0 ? ~0ull : 0ull;
+        uint64_t t[4];
+        // t = sign-extended v << sh
+        int ws = sh >> 6, bs = sh & 63;
+        uint64_t src[4] = {(uint64_t)v, ext, ext, ext};
+        for (int i = 0; i < 4; ++i) {
+            int si = i - ws;
+            uint64_t cur = si >= 0 ? src[si] : 0;
+            uint64_t prev = si - 1 >= 0 ? src[si - 1] : 0;
+            t[i] = bs ? ((cur << bs) | (prev >> (64 - bs))) : cur;
+        }
+        uint64_t c = 0;
+        for (int i = 0; i < 4; ++i) {
+            uint64_t s1 = w[i] + t[i];
+            uint64_t c1 = s1 < w[i];
+            uint64_t s2 = s1 + c;
+            uint64_t c2 = s2 < s1;
+            w[i] = s2;
+            c = c1 | c2;
+        }
+    }
+    bool neg = (int64_t)w[3] < 0;
+    if (neg) {
+        uint64_t c = 1;
+        for (int i = 0; i < 4; ++i) {
+            uint64_t x = ~w[i] + c;
+            c = (c && x == 0) ? 1 : 0;
+            w[i] = x;
+        }
+    }
+    int top = -1;
+    for (int i = 3; i >= 0; --i)
+        if (w[i]) { top = 64 * i + 63 - __builtin_clzll(w[i]); break; }
+    double q = 0.0;
+    if (top >= 0) {
+        // 64-bit window whose bit 63 is the top bit, sticky-jammed into bit 0
+        int lowpos = top - 63;
+        uint64_t win, sticky = 0;
+        if (lowpos <= 0) {
+            int s = -lowpos;
+            win = w[0] << s;
+        } else {
+            int wi = lowpos >> 6, bi = lowpos & 63;
+            win = w[wi] >> bi;
+            if (bi && wi + 1 < 4) win |= w[wi + 1] << (64 - bi);
+            for (int i = 0; i < wi; ++i) sticky |= w[i];
+            if (bi) sticky |= w[wi] & ((1ull << bi) - 1);
+        }
+        if (sticky) win |= 1;
+        q = ldexp((double)win, lowpos - kFixScale);
+    }
+    if (neg) q = -q;
+    return res != 0.0 ? q + res : q;
+}
+
+}  // namespace pmh

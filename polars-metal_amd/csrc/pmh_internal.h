@@ -1,0 +1,241 @@
+// pmh_internal.h -- types shared by the host orchestration and the gfx950
+// kernels of the filtered group-by backend.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pmh.h"
+
+namespace pmh {
+
+// ---------------------------------------------------------------------------
+// errors: C++ exceptions inside the library, status codes at the C ABI
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+void set_last_error(const std::string &msg);
+
+#define PMH_HIP(expr)                                                                   \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess)                                                           \
+            ::pmh::fail(PMH_E_COMPUTE, std::string("polars_metal: HIP error in ") + #expr + \
+                                           ": " + hipGetErrorString(_e));               \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// limits
+// ---------------------------------------------------------------------------
+constexpr int kMaxPredNodes = 32;
+constexpr int kMaxKeys = 16;
+constexpr int kMaxVals = 8;   // distinct aggregated input columns per query
+constexpr int kMaxCols = 32;  // distinct columns referenced by one query
+constexpr int kMaxArrs = 96;  // per-group state arrays
+constexpr int kLimbs = 5;     // exact f64 sum: 5 x 32-bit chunks at scale 2^-104
+constexpr int kFixScale = 104;
+
+// ---------------------------------------------------------------------------
+// dtypes
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int tag_size(int tag) {
+    switch (tag) {
+    case PMH_I64: case PMH_F64: case PMH_U64: return 8;
+    case PMH_I32: case PMH_F32: case PMH_U32: case PMH_UTF8: return 4;
+    case PMH_I16: case PMH_U16: return 2;
+    case PMH_I8: case PMH_U8: return 1;
+    case PMH_BOOL: return 0;  // bit-packed
+    default: return -1;
+    }
+}
+__host__ __device__ inline int tag_data_bits(int tag) {
+    int s = tag_size(tag);
+    return s == 0 ? 1 : s * 8;
+}
+const char *tag_name(int tag);
+int tag_from_wire(const std::string &s);  // -1 when unknown
+
+// ---------------------------------------------------------------------------
+// device-side query description (uploaded once per call, read with scalar
+// loads by every kernel)
+// ---------------------------------------------------------------------------
+struct DevCol {
+    const void *data;
+    const uint8_t *valid;  // nullptr: all valid
+    int32_t tag;
+    int32_t pad;
+};
+
+enum PredKind : int32_t { PK_COLBOOL = 0, PK_CMP = 1, PK_AND = 2, PK_OR = 3, PK_CONST = 4 };
+enum OpndKind : int32_t { OK_COL = 0, OK_LIT = 1 };
+
+// Predicate in topological order; node k may reference only nodes < k.  The
+// result of node k lives in bit k of two per-row 32-bit masks, so evaluation
+// needs no stack and no dynamically indexed registers.
+struct PredNode {
+    int32_t kind;
+    int32_t op;       // pmh_cmp_op for PK_CMP
+    int32_t is_f64;   // PK_CMP: compare as f64 (TotalOrd) instead of i64
+    int32_t lkind, rkind;
+    int32_t lhs, rhs; // column index (OK_COL / PK_COLBOOL) or child node index
+    int32_t cval;     // PK_CONST: literal bool
+    int64_t li, ri;
+    double lf, rf;
+};
+
+enum KeyMode : int32_t {
+    KM_NONE = 0,    // no key columns: one group
+    KM_LANE64 = 1,  // encoded lane fits in 64 bits
+    KM_RAW64 = 2,   // one 64-bit key column: raw bits, null and all-ones via side slots
+    KM_LANE128 = 3  // encoded lane up to 128 bits
+};
+
+struct KeyField {
+    int32_t col, tag, bits, null_off, data_off, pad;
+};
+
+enum ValKind : int32_t { VK_F = 0, VK_I64 = 1, VK_I32 = 2 };
+enum ValNeed : int32_t { NEED_SUM = 1, NEED_MINMAX = 2 };
+
+struct ValSpec {
+    int32_t col, tag, kind, need;
+    int32_t a_cnt, a_limb, a_res, a_sum, a_min, a_max, a_nan, pad;
+};
+
+enum ArrInit : int32_t { AI_ZERO = 0, AI_I32MAX = 1, AI_I32MIN = 2, AI_I64MAX = 3, AI_I64MIN = 4 };
+
+struct Layout {
+    int32_t n_arrs;
+    int32_t a_len;
+    int32_t esz[kMaxArrs];
+    int32_t init[kMaxArrs];
+    int32_t limbs_contig;  // limb arrays of one value column are consecutive
+};
+
+struct QueryDesc {
+    uint64_t n_rows;
+    int32_t n_cols;
+    int32_t n_pred;  // 0: keep all rows
+    DevCol cols[kMaxCols];
+    PredNode pred[kMaxPredNodes];
+    int32_t key_mode, n_keys, total_bits, allones_possible;
+    KeyField keys[kMaxKeys];
+    int32_t n_vals, pad2;
+    ValSpec vals[kMaxVals];
+    Layout layout;
+};
+
+// ---------------------------------------------------------------------------
+// host-side plan
+// ---------------------------------------------------------------------------
+enum AggOpHost { AGG_SUM, AGG_MEAN, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_LEN };
+
+struct PlanAgg {
+    std::string input_col;
+    int op;
+    std::string alias;
+};
+
+struct PlanPredNode {  // parsed PredicateAst
+    std::string kind;
+    std::string name, dtype, op;
+    int64_t ival = 0;
+    double fval = 0.0;
+    bool bval = false;
+    bool lit_is_int = false;
+    std::unique_ptr<PlanPredNode> lhs, rhs;
+};
+
+struct Options {
+    std::string method = "auto";  // auto | hash | sort
+    int64_t capacity_hint = 0;    // 0: plain-method default
+    int64_t lds_slots = -1;       // -1: auto, 0: no block-local table
+    int64_t block = 0;            // threads per block, 0: auto
+    int64_t grid = 0;             // blocks, 0: auto
+    int64_t tile = 0;             // rows per block per tile, 0: auto
+};
+
+struct GroupByPlan {
+    std::vector<std::pair<std::string, int>> keys;
+    std::vector<PlanAgg> aggs;
+    std::unique_ptr<PlanPredNode> predicate;
+    Options opts;
+};
+
+GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate);
+std::string compute_lifting_plan_json(const std::string &json);
+
+// ---------------------------------------------------------------------------
+// device runtime
+// ---------------------------------------------------------------------------
+struct Runtime {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    std::map<std::string, std::pair<void *, size_t>> slabs;
+    void *pinned = nullptr;
+    size_t pinned_size = 0;
+    bool profiling = false;
+    std::map<std::string, std::pair<int64_t, double>> ktimes;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+Runtime &rt();  // initialises the device on first use (throws PMH_E_COMPUTE)
+void *slab(const std::string &name, size_t bytes);  // grow-only named device buffer
+void *pinned_host(size_t bytes);
+// Launch bracketing for the device-time instrumentation.
+struct KernelTimer {
+    KernelTimer(const char *name);
+    ~KernelTimer();
+    std::string name;
+    hipEvent_t a = nullptr, b = nullptr;
+};
+void collect_kernel_times();
+
+// Column resolved for one call: device pointers plus the host descriptor.
+struct Column {
+    std::string name;
+    int tag;
+    const void *d_data;
+    const uint8_t *d_valid;
+    uint64_t n_rows;
+};
+
+// Upload (or pass through) the columns the plan references.
+std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
+                                  const std::vector<std::string> &needed);
+
+// result helpers
+pmh_result *new_result(size_t n_cols);
+void set_out_name(pmh_out_column &c, const std::string &name);
+
+// group-by entry (pmh_groupby.hip)
+pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols,
+                               uint32_t n_cols, uint32_t flags);
+
+// filter entries (pmh_filter.hip)
+pmh_result *run_filter_compact(const uint8_t *pd, const uint8_t *pv, uint64_t n_rows,
+                               const pmh_column *cols, uint32_t n_cols);
+void run_cmp(int is_f64, int col_col, const void *lhs, const uint8_t *lv, const void *rhs,
+             const uint8_t *rv, int64_t rhs_i, double rhs_f, int op, uint64_t n_rows,
+             uint8_t *out_d, uint8_t *out_v);
+void run_bool(int is_or, const uint8_t *ld, const uint8_t *lv, const uint8_t *rd,
+              const uint8_t *rv, uint64_t n_rows, uint8_t *out_d, uint8_t *out_v);
+
+inline uint64_t ceil_div(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+inline uint64_t bitmap_bytes_padded(uint64_t n) {
+    uint64_t b = ((n + 7) / 8 + 3) & ~uint64_t(3);
+    return b < 4 ? 4 : b;
+}
+
+}  // namespace pmh

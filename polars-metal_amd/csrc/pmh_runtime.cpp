@@ -1,0 +1,169 @@
+// pmh_runtime.cpp -- device runtime of the backend: one HIP device and one
+// stream per process, grow-only named device slabs (the analogue of the
+// reference's persistent BuildScratch / StagingPool,
+// crates/polars-metal-kernels/src/groupby_build_partitioned/scratch.rs:29-145,
+// crates/polars-metal-buffer/src/staging.rs:24-58), column staging and the
+// optional per-kernel HIP-event timers.
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <set>
+
+#include "pmh_internal.h"
+
+namespace pmh {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+static Runtime *g_rt = nullptr;
+
+Runtime &rt() {
+    if (g_rt) return *g_rt;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0)
+        fail(PMH_E_COMPUTE, "polars_metal: no HIP device available (the MI355X backend has no CPU fallback)");
+    auto *r = new Runtime();
+    PMH_HIP(hipGetDevice(&r->device));
+    PMH_HIP(hipGetDeviceProperties(&r->prop, r->device));
+    // Default priority: streams of different priorities are not used.
+    PMH_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    g_rt = r;
+    return *r;
+}
+
+void *slab(const std::string &name, size_t bytes) {
+    Runtime &r = rt();
+    if (bytes == 0) bytes = 256;
+    auto it = r.slabs.find(name);
+    if (it != r.slabs.end() && it->second.second >= bytes) return it->second.first;
+    if (it != r.slabs.end()) {
+        PMH_HIP(hipStreamSynchronize(r.stream));
+        PMH_HIP(hipFree(it->second.first));
+        r.slabs.erase(it);
+    }
+    size_t sz = bytes + bytes / 4;
+    sz = (sz + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+    void *p = nullptr;
+    PMH_HIP(hipMalloc(&p, sz));
+    r.slabs[name] = {p, sz};
+    return p;
+}
+
+void *pinned_host(size_t bytes) {
+    Runtime &r = rt();
+    if (r.pinned_size >= bytes && r.pinned) return r.pinned;
+    if (r.pinned) {
+        PMH_HIP(hipStreamSynchronize(r.stream));
+        PMH_HIP(hipHostFree(r.pinned));
+        r.pinned = nullptr;
+    }
+    size_t sz = bytes + bytes / 4 + 4096;
+    PMH_HIP(hipHostMalloc(&r.pinned, sz, hipHostMallocDefault));
+    r.pinned_size = sz;
+    return r.pinned;
+}
+
+KernelTimer::KernelTimer(const char *n) : name(n) {
+    Runtime &r = rt();
+    if (!r.profiling) return;
+    auto take = [&]() {
+        hipEvent_t ev;
+        if (!r.event_pool.empty()) { ev = r.event_pool.back(); r.event_pool.pop_back(); }
+        else PMH_HIP(hipEventCreate(&ev));
+        return ev;
+    };
+    a = take();
+    b = take();
+    PMH_HIP(hipEventRecord(a, r.stream));
+}
+
+KernelTimer::~KernelTimer() {
+    if (!a) return;
+    Runtime &r = rt();
+    if (hipEventRecord(b, r.stream) == hipSuccess) r.pending.push_back({name, {a, b}});
+}
+
+void collect_kernel_times() {
+    Runtime &r = rt();
+    if (r.pending.empty()) return;
+    for (auto &p : r.pending) {
+        (void)hipEventSynchronize(p.second.second);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+            auto &slot = r.ktimes[p.first];
+            slot.first += 1;
+            slot.second += ms;
+        }
+        r.event_pool.push_back(p.second.first);
+        r.event_pool.push_back(p.second.second);
+    }
+    r.pending.clear();
+}
+
+pmh_result *new_result(size_t n_cols) {
+    auto *res = (pmh_result *)calloc(1, sizeof(pmh_result));
+    res->n_columns = (uint32_t)n_cols;
+    res->columns = (pmh_out_column *)calloc(n_cols ? n_cols : 1, sizeof(pmh_out_column));
+    return res;
+}
+
+void set_out_name(pmh_out_column &c, const std::string &name) {
+    char *s = (char *)malloc(name.size() + 1);
+    memcpy(s, name.c_str(), name.size() + 1);
+    c.name = s;
+}
+
+std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
+                                  const std::vector<std::string> &needed) {
+    Runtime &r = rt();
+    std::vector<Column> out;
+    std::set<std::string> seen;
+    int staged = 0;
+    for (const auto &nm : needed) {
+        if (seen.count(nm)) continue;
+        seen.insert(nm);
+        const pmh_column *c = nullptr;
+        for (uint32_t i = 0; i < n_cols; ++i)
+            if (cols[i].name && nm == cols[i].name) { c = &cols[i]; break; }
+        if (!c) fail(PMH_E_KEY, "polars_metal: column not found: " + nm);
+        if (c->tag < 0 || c->tag > PMH_UTF8)
+            fail(PMH_E_VALUE, "polars_metal: unknown dtype tag for column " + nm);
+        int esz = tag_size(c->tag);
+        uint64_t need = esz == 0 ? (n_rows + 7) / 8 : n_rows * (uint64_t)esz;
+        if (c->data_len < need || (!c->data && need))
+            fail(PMH_E_VALUE, "polars_metal: " + std::string(tag_name(c->tag)) + " value buffer too short for " +
+                                  nm + ": " + std::to_string(c->data_len) + " < " + std::to_string(need));
+        uint64_t vneed = (n_rows + 7) / 8;
+        if (c->valid && c->valid_len < vneed)
+            fail(PMH_E_VALUE, "polars_metal: validity buffer too short for " + nm + ": " +
+                                  std::to_string(c->valid_len) + " < " + std::to_string(vneed));
+        Column col;
+        col.name = nm;
+        col.tag = c->tag;
+        col.n_rows = n_rows;
+        if (c->location == PMH_DEVICE) {
+            col.d_data = c->data;
+            col.d_valid = c->valid;
+        } else {
+            // Device copies are padded to 16 bytes so vector loads never run
+            // past the allocation.
+            void *d = slab("in:" + std::to_string(staged), need + 16);
+            if (need) PMH_HIP(hipMemcpyAsync(d, c->data, need, hipMemcpyHostToDevice, r.stream));
+            col.d_data = d;
+            col.d_valid = nullptr;
+            if (c->valid) {
+                void *dv = slab("inv:" + std::to_string(staged), vneed + 16);
+                if (vneed) PMH_HIP(hipMemcpyAsync(dv, c->valid, vneed, hipMemcpyHostToDevice, r.stream));
+                col.d_valid = (const uint8_t *)dv;
+            }
+            ++staged;
+        }
+        out.push_back(col);
+    }
+    return out;
+}
+
+}  // namespace pmh

@@ -1,0 +1,233 @@
+"""Writes tests/golden/cases.json: small group-by / filter cases whose
+expected outputs are computed here in pure Python (dicts, math.fsum, Python
+integers), independently of the C oracle and of the HIP path.
+
+Each case is named for what it pins:
+  arith_mod7        keys i % 7, v = i: arithmetic-series sums, counts, min,
+                    max, mean (closed form; reference shape
+                    tests/python_integration/test_groupby.py:28-39)
+  null_keys         a null key is its own group (test_groupby.py:114-125)
+  null_values       nulls skipped; all-null group: sum 0 (valid), count 0,
+                    min/max/mean null (groupby.rs:1268-1280)
+  totalord_filter   v > 0.0 with -0.0, NaN, +-inf and a null predicate row
+                    (docs/open-questions.md:55-66; test_filter_edges.py:46-92)
+  kleene_filter     (a > 0) | (b > 0) and (a > 0) & (b > 0) with nulls
+  composite_keys    (a i32, b i64) composite keys, i64 wrapping sum
+  cancel_sums       f64 sums that cancel exactly (1e16 + 1 - 1e16) and sums
+                    of 0.1, checked against math.fsum
+  i32_sum_wrap      i32 sum wraps at 2^32 (groupby.rs i32 dispatch)
+  all_unique        every row its own group (test_groupby.py:152)
+  all_same          one group (test_groupby.py:166)
+  bool_keys         Bool key column
+  empty             zero rows
+  all_filtered      predicate removes every row
+
+Run:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+OUT = Path(__file__).resolve().parent / "cases.json"
+
+
+def fenc(x):
+    if x is None:
+        return None
+    if isinstance(x, float):
+        if math.isnan(x):
+            return "nan"
+        return x.hex()
+    return x
+
+
+def col(tag, values):
+    return {"tag": tag, "values": list(values)}
+
+
+def tot_cmp(a, b):
+    an, bn = math.isnan(a), math.isnan(b)
+    if an and bn:
+        return 0
+    if an:
+        return 1
+    if bn:
+        return -1
+    return (a > b) - (a < b)
+
+
+def eval_pred(p, cols, r):
+    """(value, valid) for row r."""
+    k = p["kind"]
+    if k == "Column":
+        v = cols[p["name"]]["values"][r]
+        return v, v is not None
+    if k.startswith("Literal"):
+        return p["value"], True
+    if k == "Compare":
+        a, av = eval_pred(p["lhs"], cols, r)
+        b, bv = eval_pred(p["rhs"], cols, r)
+        if not (av and bv):
+            return False, False
+        if p.get("dtype") == "F64":
+            c = tot_cmp(float(a), float(b))
+        else:
+            c = (a > b) - (a < b)
+        return {"Eq": c == 0, "Ne": c != 0, "Lt": c < 0, "Le": c <= 0, "Gt": c > 0, "Ge": c >= 0}[p["op"]], True
+    if k in ("And", "Or"):
+        a, av = eval_pred(p["lhs"], cols, r)
+        b, bv = eval_pred(p["rhs"], cols, r)
+        dom = k == "Or"
+        if (av and a == dom) or (bv and b == dom):
+            return dom, True
+        if av and bv:
+            return (not dom), True
+        return False, False
+    raise ValueError(k)
+
+
+def wrap(x, bits):
+    x &= (1 << bits) - 1
+    return x - (1 << bits) if x >= 1 << (bits - 1) else x
+
+
+def expected(plan, cols, n):
+    keys = [k[0] for k in plan["keys"]]
+    groups = {}
+    order = []
+    for r in range(n):
+        if plan.get("predicate"):
+            v, ok = eval_pred(plan["predicate"], cols, r)
+            if not (ok and v):
+                continue
+        kt = tuple(cols[k]["values"][r] for k in keys)
+        ktn = tuple((x is None, 0 if x is None else (("f", float(x).hex()) if isinstance(x, float) else x)) for x in kt)
+        if ktn not in groups:
+            groups[ktn] = {"key": kt, "rows": []}
+            order.append(ktn)
+        groups[ktn]["rows"].append(r)
+    out = {k: [] for k in keys}
+    for a in plan["aggs"]:
+        out[a["output_alias"]] = []
+    for ktn in order:
+        g = groups[ktn]
+        for k, x in zip(keys, g["key"]):
+            out[k].append(x)
+        for a in plan["aggs"]:
+            rows = g["rows"]
+            if a["op"] == "Len":
+                out[a["output_alias"]].append(len(rows))
+                continue
+            c = cols[a["input_col"]]
+            vals = [c["values"][r] for r in rows if c["values"][r] is not None]
+            tag = c["tag"]
+            op = a["op"]
+            if op == "Count":
+                out[a["output_alias"]].append(len(vals))
+            elif op == "Sum":
+                if tag == "F64":
+                    out[a["output_alias"]].append(math.fsum(vals))
+                else:
+                    out[a["output_alias"]].append(wrap(sum(vals), 64 if tag == "I64" else 32))
+            elif op == "Mean":
+                if not vals:
+                    out[a["output_alias"]].append(None)
+                elif tag == "F64":
+                    out[a["output_alias"]].append(math.fsum(vals) / len(vals))
+                else:
+                    out[a["output_alias"]].append(float(wrap(sum(vals), 64)) / len(vals))
+            elif op in ("Min", "Max"):
+                if not vals:
+                    out[a["output_alias"]].append(None)
+                elif tag == "F64" and any(math.isnan(x) for x in vals):
+                    out[a["output_alias"]].append(float("nan"))
+                else:
+                    out[a["output_alias"]].append(min(vals) if op == "Min" else max(vals))
+    return {k: [fenc(x) for x in v] for k, v in out.items()}, len(order)
+
+
+def agg(col_, op, alias=None):
+    return {"kind": "Simple", "input_col": col_, "op": op, "output_alias": alias or f"{col_}_{op.lower()}"}
+
+
+FIVE = [agg("v", "Sum"), agg("v", "Count"), agg("w", "Min"), agg("w", "Max"), agg("v", "Mean"),
+        {"kind": "Length", "op": "Len", "output_alias": "len"}]
+
+
+def gt0(name, dtype="F64"):
+    lit = {"kind": "LiteralF64", "value": 0.0} if dtype == "F64" else {"kind": "LiteralI64", "value": 0}
+    return {"kind": "Compare", "op": "Gt", "dtype": dtype, "lhs": {"kind": "Column", "name": name, "dtype": dtype},
+            "rhs": lit}
+
+
+def cases():
+    cs = []
+    n = 70
+    cs.append(("arith_mod7", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", [i % 7 for i in range(n)]), "v": col("F64", [float(i) for i in range(n)]),
+                "w": col("I32", list(range(n)))}, n))
+    keys = [1, None, 2, None, 1, 3, None, 2]
+    cs.append(("null_keys", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", keys), "v": col("F64", [0.5, 1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5]),
+                "w": col("I32", [10, 20, 30, 40, 50, 60, 70, 80])}, len(keys)))
+    cs.append(("null_values", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", [1, 1, 2, 2, 3, 3]), "v": col("F64", [None, None, 1.25, None, -2.0, 4.0]),
+                "w": col("I32", [None, None, 7, None, -3, 9])}, 6))
+    vals = [-0.0, 0.0, float("nan"), 1.0, -1.0, float("inf"), None, float("-inf"), 2.5, float("nan")]
+    cs.append(("totalord_filter", {"keys": [["key", "I64"]], "aggs": [agg("v", "Count"), agg("w", "Sum")],
+                                   "predicate": gt0("v")},
+               {"key": col("I64", [0, 1, 2, 3, 0, 1, 2, 3, 0, 1]), "v": col("F64", vals),
+                "w": col("I32", list(range(10)))}, 10))
+    a = [1, -1, None, 1, None, -1, 1, None, -1]
+    b = [1, 1, 1, -1, -1, -1, None, None, None]
+    for kind in ("And", "Or"):
+        pred = {"kind": kind, "lhs": gt0("a", "I64"), "rhs": gt0("b", "I64")}
+        cs.append((f"kleene_{kind.lower()}", {"keys": [["k", "I64"]], "aggs": [agg("a", "Count"),
+                                                                                {"kind": "Length", "op": "Len", "output_alias": "len"}],
+                                              "predicate": pred},
+                   {"k": col("I64", list(range(9))), "a": col("I64", a), "b": col("I64", b)}, 9))
+    ca = [1, 2, 1, 2, 1, None, 1, 2, (1 << 20) - 1, 0]
+    cb = [5, 5, 5, 6, 1 << 40, 7, 5, 6, (1 << 40) - 1, -9]
+    cv = [1 << 62, 1 << 62, 3, -4, 5, 6, None, 8, -(1 << 63), 1]
+    cs.append(("composite_keys", {"keys": [["a", "I32"], ["b", "I64"]],
+                                  "aggs": [agg("v", "Sum"), agg("v", "Count"), agg("v", "Min"), agg("v", "Max"),
+                                           agg("v", "Mean")]},
+               {"a": col("I32", ca), "b": col("I64", cb), "v": col("I64", cv)}, 10))
+    cv2 = [1e16, 1.0, -1e16, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 1e-300, -1e-300, 2.0 ** -1074, 3.0]
+    ck = [0, 0, 0] + [1] * 10 + [2, 2, 2, 3]
+    cs.append(("cancel_sums", {"keys": [["key", "I64"]], "aggs": [agg("v", "Sum"), agg("v", "Mean")]},
+               {"key": col("I64", ck), "v": col("F64", cv2)}, len(ck)))
+    big = (1 << 31) - 5
+    cs.append(("i32_sum_wrap", {"keys": [["key", "I32"]], "aggs": [agg("v", "Sum"), agg("v", "Mean"), agg("v", "Min"),
+                                                                    agg("v", "Max")]},
+               {"key": col("I32", [7, 7, 7, -7]), "v": col("I32", [big, big, 100, -(1 << 31)])}, 4))
+    cs.append(("all_unique", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", [10 * i - 300 for i in range(64)]), "v": col("F64", [i * 0.25 for i in range(64)]),
+                "w": col("I32", [64 - i for i in range(64)])}, 64))
+    cs.append(("all_same", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", [-1] * 50), "v": col("F64", [(-1) ** i * i / 3.0 for i in range(50)]),
+                "w": col("I32", [i * 37 % 101 for i in range(50)])}, 50))
+    cs.append(("bool_keys", {"keys": [["flag", "Bool"], ["key", "I32"]], "aggs": [agg("v", "Sum"), agg("v", "Count")]},
+               {"flag": col("Bool", [True, False, None, True, False, True]), "key": col("I32", [1, 1, 1, 2, 1, 1]),
+                "v": col("F64", [1.0, 2.0, 3.0, 4.0, 5.0, 6.0])}, 6))
+    cs.append(("empty", {"keys": [["key", "I64"]], "aggs": FIVE},
+               {"key": col("I64", []), "v": col("F64", []), "w": col("I32", [])}, 0))
+    cs.append(("all_filtered", {"keys": [["key", "I64"]], "aggs": FIVE, "predicate": gt0("v")},
+               {"key": col("I64", [1, 2, 3]), "v": col("F64", [-1.0, -2.0, None]), "w": col("I32", [1, 2, 3])}, 3))
+    return cs
+
+
+def main():
+    out = []
+    for name, plan, cols, n in cases():
+        exp, ng = expected(plan, cols, n)
+        enc_cols = {k: {"tag": c["tag"], "values": [fenc(v) for v in c["values"]]} for k, c in cols.items()}
+        out.append({"name": name, "plan": plan, "n_rows": n, "columns": enc_cols, "expected": exp, "n_groups": ng})
+    OUT.write_text(json.dumps(out, indent=1))
+    print(f"wrote {len(out)} cases to {OUT}")
+
+
+if __name__ == "__main__":
+    main()

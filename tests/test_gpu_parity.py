@@ -1,0 +1,249 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Integer, key, count, min/max and null results must be bit-exact.  f64 sums
+and means are compared by bits too: both sides compute the correctly rounded
+exact per-group sum (the oracle with Shewchuk partials, the kernels with
+fixed-point limbs), so any difference is a bug, not rounding noise."""
+import numpy as np
+import pytest
+
+import oracle as orc
+from golden_cases import CASES, expected_of, frame_of
+from helpers import assert_results_equal, run_gpu
+from polars_metal import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _keys(plan):
+    return [k[0] for k in plan["keys"]]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_golden_cases(native, case):
+    frame = frame_of(case)
+    print(f"golden case {case['name']}", flush=True)
+    got = run_gpu(native, case["plan"], frame)
+    assert_results_equal(got, expected_of(case, frame), _keys(case["plan"]), label=case["name"])
+    if not case["plan"].get("predicate"):
+        got2 = run_gpu(native, case["plan"], frame, entry="groupby")
+        assert_results_equal(got2, expected_of(case, frame), _keys(case["plan"]), label=case["name"] + "/groupby")
+
+
+def test_small_config_full_size(native):
+    frame, plan = synth.small(1_000_000, seed=11)
+    got = run_gpu(native, plan, frame)
+    want = orc.groupby(plan, frame)
+    assert want["__n_groups__"] == 1000
+    assert_results_equal(got, want, ["key"], label="small")
+
+
+@pytest.mark.parametrize("rows", [1, 63, 64, 65, 4097, 2_000_000])
+def test_zipf_scaled(native, rows):
+    frame, plan = synth.zipf(rows, seed=rows)
+    print(f"zipf rows={rows}", flush=True)
+    got = run_gpu(native, plan, frame)
+    assert_results_equal(got, orc.groupby(plan, frame), ["key"], label=f"zipf{rows}")
+
+
+def test_highcard_scaled(native):
+    frame, plan = synth.highcard(1_000_000, seed=5, pool=250_000)
+    got = run_gpu(native, plan, frame)
+    want = orc.groupby(plan, frame)
+    assert want["__n_groups__"] > 200_000
+    assert_results_equal(got, want, ["a", "b"], label="highcard")
+
+
+def test_resize_on_overflow(native):
+    frame, plan = synth.zipf(300_000, seed=9)
+    plan = dict(plan, options={"capacity_hint": 8})
+    got = run_gpu(native, plan, frame)
+    st = native.last_stats()
+    assert st["n_passes"] > 1 and st["final_capacity"] > st["initial_capacity"]
+    assert_results_equal(got, orc.groupby(plan, frame), ["key"], label="resize")
+
+
+@pytest.mark.parametrize("opts", [{"lds_slots": 0}, {"lds_slots": 64}, {"block": 256, "tile": 1000},
+                                  {"grid": 3}, {"block": 64, "grid": 1}])
+def test_launch_shapes_agree(native, opts):
+    frame, plan = synth.zipf(200_000, seed=4)
+    plan = dict(plan, options=opts)
+    assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["key"], label=str(opts))
+
+
+KEY_TAGS = ["I64", "I32", "I16", "I8", "U8", "U16", "U32", "U64", "F64", "F32", "Bool"]
+
+
+@pytest.mark.parametrize("tag", KEY_TAGS)
+def test_key_dtypes_with_nulls(native, tag):
+    rng = np.random.default_rng(hash(tag) % 1000)
+    n = 50_000
+    if tag == "Bool":
+        k = rng.random(n) < 0.5
+    elif tag in ("F64", "F32"):
+        k = rng.integers(-20, 20, n).astype(orc.NP_DTYPE[tag]) / 4
+        k[::97] = -0.0
+    else:
+        info = np.iinfo(orc.NP_DTYPE[tag])
+        k = rng.integers(max(info.min, -300), min(info.max, 300), n).astype(orc.NP_DTYPE[tag])
+        k[::101] = info.max  # all-ones patterns for unsigned types
+        k[::103] = info.min
+    frame = {"k": (tag, k, rng.random(n) < 0.95), "v": ("F64", rng.standard_normal(n), rng.random(n) < 0.9),
+             "w": ("I32", rng.integers(-50, 50, n).astype(np.int32), None)}
+    plan = {"keys": [["k", tag]], "aggs": synth.FIVE_AGGS + [{"kind": "Length", "op": "Len", "output_alias": "n"}]}
+    assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["k"], label=tag)
+
+
+def test_i64_all_ones_and_null_side_slots(native):
+    k = np.array([-1, -1, 5, 0, -1, 5, 7], np.int64)
+    valid = np.array([1, 1, 1, 0, 1, 0, 1], bool)
+    frame = {"k": ("I64", k, valid), "v": ("F64", np.arange(7, dtype=np.float64), None)}
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "s"}]}
+    for opts in ({}, {"lds_slots": 0}):
+        p = dict(plan, options=opts)
+        assert_results_equal(run_gpu(native, p, frame), orc.groupby(p, frame), ["k"], label=f"side{opts}")
+
+
+@pytest.mark.parametrize("keys", [["a", "b"], ["b", "c", "a"], ["c", "d"], ["a", "c", "d"]])
+def test_composite_keys(native, keys):
+    rng = np.random.default_rng(len(keys))
+    n = 100_000
+    frame = {"a": ("I32", rng.integers(-5, 5, n).astype(np.int32), rng.random(n) < 0.97),
+             "b": ("I64", rng.integers(-3, 3, n) * (1 << 50), rng.random(n) < 0.97),
+             "c": ("Bool", rng.random(n) < 0.5, rng.random(n) < 0.9),
+             "d": ("U16", rng.integers(0, 4, n).astype(np.uint16), None),
+             "v": ("I64", rng.integers(-1 << 62, 1 << 62, n), rng.random(n) < 0.8)}
+    tags = {k: frame[k][0] for k in keys}
+    plan = {"keys": [[k, tags[k]] for k in keys],
+            "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in
+                     ("Sum", "Count", "Min", "Max", "Mean")]}
+    assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), keys, label=str(keys))
+
+
+@pytest.mark.parametrize("vtag", ["F64", "F32", "I64", "I32"])
+def test_value_dtypes_nan_and_nulls(native, vtag):
+    rng = np.random.default_rng(3)
+    n = 80_000
+    if vtag in ("F64", "F32"):
+        v = (rng.standard_normal(n) * 1e3).astype(orc.NP_DTYPE[vtag])
+        v[rng.random(n) < 0.001] = np.nan
+        v[rng.random(n) < 0.001] = np.inf
+        v[::1000] = np.ldexp(1.0, -1070)  # subnormal-range values take the residual path
+    else:
+        info = np.iinfo(orc.NP_DTYPE[vtag])
+        v = rng.integers(info.min, info.max, n, endpoint=True).astype(orc.NP_DTYPE[vtag])
+    frame = {"k": ("I64", rng.integers(0, 300, n), None), "x": (vtag, v, rng.random(n) < 0.9)}
+    plan = {"keys": [["k", "I64"]],
+            "aggs": [{"kind": "Simple", "input_col": "x", "op": op, "output_alias": op} for op in
+                     ("Sum", "Count", "Min", "Max", "Mean")]}
+    want = orc.groupby(plan, frame)
+    got = run_gpu(native, plan, frame)
+    if vtag in ("F64", "F32"):
+        # groups holding an out-of-window value (here 2^-1070) sum their
+        # residual in f64 and may round twice; compare those within 1 ulp
+        assert_results_equal(got, want, ["k"], float_exact=False, rtol=4e-16, label=vtag)
+    else:
+        assert_results_equal(got, want, ["k"], label=vtag)
+
+
+def test_predicate_shapes(native):
+    rng = np.random.default_rng(8)
+    n = 60_000
+    frame = {"k": ("I32", rng.integers(0, 50, n).astype(np.int32), None),
+             "x": ("F64", rng.standard_normal(n), rng.random(n) < 0.9),
+             "y": ("I16", rng.integers(-100, 100, n).astype(np.int16), rng.random(n) < 0.9),
+             "f": ("Bool", rng.random(n) < 0.5, rng.random(n) < 0.9),
+             "z": ("F32", rng.standard_normal(n).astype(np.float32), None)}
+    frame["x"][1][::50] = np.nan
+    col = lambda n_, d: {"kind": "Column", "name": n_, "dtype": d}  # noqa: E731
+    preds = [
+        {"kind": "Compare", "op": "Le", "dtype": "F64", "lhs": col("x", "F64"), "rhs": {"kind": "LiteralF64", "value": 0.25}},
+        {"kind": "Compare", "op": "Ne", "dtype": "I64", "lhs": col("y", "I64"), "rhs": {"kind": "LiteralI64", "value": 3}},
+        {"kind": "Or", "lhs": col("f", "Bool"),
+         "rhs": {"kind": "Compare", "op": "Gt", "dtype": "F64", "lhs": col("x", "F64"), "rhs": col("z", "F64")}},
+        {"kind": "And", "lhs": {"kind": "Or", "lhs": col("f", "Bool"), "rhs": {"kind": "LiteralBool", "value": False}},
+         "rhs": {"kind": "Compare", "op": "Eq", "dtype": "F64", "lhs": col("x", "F64"), "rhs": col("x", "F64")}},
+        {"kind": "Compare", "op": "Lt", "dtype": "I64", "lhs": {"kind": "LiteralI64", "value": -20}, "rhs": col("y", "I64")},
+    ]
+    for i, p in enumerate(preds):
+        plan = {"keys": [["k", "I32"]], "predicate": p,
+                "aggs": [{"kind": "Simple", "input_col": "x", "op": "Sum", "output_alias": "s"},
+                         {"kind": "Length", "op": "Len", "output_alias": "n"}]}
+        assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["k"], label=f"pred{i}")
+
+
+def test_deterministic_bits(native):
+    frame, plan = synth.zipf(500_000, seed=21)
+    wire = orc.to_wire(frame)
+    a = native.execute_filter_groupby(plan, 500_000, wire)
+    b = native.execute_filter_groupby(plan, 500_000, wire)
+    ca = orc.canonical(orc.from_wire(a), ["key"])
+    cb = orc.canonical(orc.from_wire(b), ["key"])
+    for name in ca:
+        if name != "__n_groups__":
+            assert ca[name][1].tobytes() == cb[name][1].tobytes(), name
+
+
+def test_filter_compact(native):
+    rng = np.random.default_rng(12)
+    for n in [0, 1, 9, 4096, 4097, 300_001]:
+        pd = rng.random(n) < 0.4
+        pv = rng.random(n) < 0.9
+        cols = {"a": ("I64", rng.integers(-1 << 60, 1 << 60, n), rng.random(n) < 0.8),
+                "b": ("F64", rng.standard_normal(n), None),
+                "c": ("Bool", rng.random(n) < 0.5, rng.random(n) < 0.7),
+                "d": ("I32", rng.integers(-5, 5, n).astype(np.int32), None),
+                "e": ("U8", rng.integers(0, 255, n).astype(np.uint8), rng.random(n) < 0.5)}
+        out = native.execute_filter_compact(orc.pack_bits(pd).tobytes(), orc.pack_bits(pv).tobytes(), n,
+                                            orc.to_wire(cols))
+        keep = pd & pv
+        m = int(keep.sum())
+        for (name, (tag, vals, valid)), (data, vb, n_out) in zip(cols.items(), out):
+            assert n_out == m
+            if m == 0:
+                assert data == b"" and vb == b""
+                continue
+            ev, evv = orc.compact(keep, tag, vals, valid)
+            if tag == "Bool":
+                np.testing.assert_array_equal(orc.unpack_bits(data, m), ev)
+            else:
+                np.testing.assert_array_equal(np.frombuffer(data, orc.NP_DTYPE[tag]), ev)
+            np.testing.assert_array_equal(orc.unpack_bits(vb, m), evv)
+            assert len(vb) == max(4, ((m + 7) // 8 + 3) & ~3)
+
+
+@pytest.mark.parametrize("op", ["Eq", "Ne", "Lt", "Le", "Gt", "Ge"])
+def test_cmp_and_bool(native, op):
+    rng = np.random.default_rng(ord(op[0]))
+    n = 10_007
+    a = rng.integers(-3, 3, n)
+    b = rng.integers(-3, 3, n)
+    av = rng.random(n) < 0.9
+    bv = rng.random(n) < 0.9
+    x = rng.choice(np.array([np.nan, -0.0, 0.0, 1.0, -np.inf, 2.0]), n)
+    y = rng.choice(np.array([np.nan, 0.0, 1.0, np.inf]), n)
+    frame = {"a": ("I64", a, av), "b": ("I64", b, bv), "x": ("F64", x, av), "y": ("F64", y, bv)}
+    pk = lambda m: orc.pack_bits(m).tobytes()  # noqa: E731
+    col = lambda n_, d: {"kind": "Column", "name": n_, "dtype": d}  # noqa: E731
+    checks = [
+        (native.cmp_i64_col_scalar(a.tobytes(), pk(av), 1, op, n), {"kind": "Compare", "op": op, "dtype": "I64",
+                                                                    "lhs": col("a", "I64"), "rhs": {"kind": "LiteralI64", "value": 1}}),
+        (native.cmp_i64_col_col(a.tobytes(), pk(av), b.tobytes(), pk(bv), op, n),
+         {"kind": "Compare", "op": op, "dtype": "I64", "lhs": col("a", "I64"), "rhs": col("b", "I64")}),
+        (native.cmp_f64_col_scalar(x.tobytes(), pk(av), 0.0, op, n), {"kind": "Compare", "op": op, "dtype": "F64",
+                                                                      "lhs": col("x", "F64"), "rhs": {"kind": "LiteralF64", "value": 0.0}}),
+        (native.cmp_f64_col_col(x.tobytes(), pk(av), y.tobytes(), pk(bv), op, n),
+         {"kind": "Compare", "op": op, "dtype": "F64", "lhs": col("x", "F64"), "rhs": col("y", "F64")}),
+    ]
+    for (gd, gv), pred in checks:
+        d, v = orc.eval_predicate(pred, frame)
+        np.testing.assert_array_equal(orc.unpack_bits(gv, n), v)
+        np.testing.assert_array_equal(orc.unpack_bits(gd, n), d & v)
+    for kind, fn in (("And", native.bool_and_dispatch), ("Or", native.bool_or_dispatch)):
+        p = a > 0
+        q = b > 0
+        gd, gv = fn(pk(p), pk(av), pk(q), pk(bv), n)
+        fb = {"p": ("Bool", p, av), "q": ("Bool", q, bv)}
+        d, v = orc.eval_predicate({"kind": kind, "lhs": col("p", "Bool"), "rhs": col("q", "Bool")}, fb)
+        np.testing.assert_array_equal(orc.unpack_bits(gv, n), v)
+        np.testing.assert_array_equal(orc.unpack_bits(gd, n), d & v)

@@ -1,0 +1,159 @@
+"""Pins the CPU oracle before it is trusted as the checker (no GPU needed).
+
+Each check is chosen so a plausible oracle mistake fails it: golden cases
+computed independently in pure Python; exhaustive Kleene tables; the
+reference's TotalOrd examples; math.fsum for exact sums; numpy.unique for
+group identity; brute-force compaction; key-encoding width KATs."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from golden_cases import CASES, expected_of, frame_of
+from helpers import assert_results_equal
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    orc.build()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_oracle_matches_golden(case):
+    frame = frame_of(case)
+    got = orc.groupby(case["plan"], frame)
+    want = expected_of(case, frame)
+    keys = [k[0] for k in case["plan"]["keys"]]
+    assert_results_equal(got, want, keys, label=case["name"])
+
+
+def _tri(x):
+    return {None: (False, False), True: (True, True), False: (False, True)}[x]
+
+
+@pytest.mark.parametrize("kind", ["And", "Or"])
+def test_kleene_truth_table_exhaustive(kind):
+    # shaders/logical_bool.metal:1-31 and test_logical_bool.rs:131-270:
+    # AND: false dominates, OR: true dominates, otherwise null is null.
+    table = {
+        "And": {(True, True): True, (True, False): False, (True, None): None, (False, None): False,
+                (False, False): False, (None, None): None},
+        "Or": {(True, True): True, (True, False): True, (True, None): True, (False, None): None,
+               (False, False): False, (None, None): None},
+    }[kind]
+    combos = list(itertools.product([True, False, None], repeat=2))
+    a = [c[0] for c in combos]
+    b = [c[1] for c in combos]
+    frame = {"a": ("Bool", np.array([bool(x) for x in a]), np.array([x is not None for x in a])),
+             "b": ("Bool", np.array([bool(x) for x in b]), np.array([x is not None for x in b]))}
+    pred = {"kind": kind, "lhs": {"kind": "Column", "name": "a", "dtype": "Bool"},
+            "rhs": {"kind": "Column", "name": "b", "dtype": "Bool"}}
+    d, v = orc.eval_predicate(pred, frame)
+    for i, (x, y) in enumerate(combos):
+        want = table.get((x, y), table.get((y, x)))
+        assert (bool(d[i]) if v[i] else None) == want, (kind, x, y)
+
+
+def test_totalord_examples():
+    # docs/open-questions.md:57-64: pl.Series([1.0, NaN, 3.0]) > 0 -> all true;
+    # NaN == NaN -> true; NaN > NaN -> false; -0.0 == 0.0.
+    x = np.array([1.0, np.nan, 3.0, -0.0, np.nan])
+    y = np.array([0.0, 0.0, 0.0, 0.0, np.nan])
+    frame = {"x": ("F64", x, None), "y": ("F64", y, None)}
+    col = lambda n: {"kind": "Column", "name": n, "dtype": "F64"}  # noqa: E731
+    gt = orc.eval_predicate({"kind": "Compare", "op": "Gt", "dtype": "F64", "lhs": col("x"), "rhs": col("y")}, frame)[0]
+    eq = orc.eval_predicate({"kind": "Compare", "op": "Eq", "dtype": "F64", "lhs": col("x"), "rhs": col("y")}, frame)[0]
+    assert list(gt) == [True, True, True, False, False]
+    assert list(eq) == [False, False, False, True, True]
+
+
+def test_compaction_brute_force():
+    rng = np.random.default_rng(0)
+    for n in [0, 1, 7, 8, 9, 63, 64, 65, 1000, 4099]:
+        keep = rng.random(n) < 0.37
+        vals = rng.integers(-1 << 40, 1 << 40, n)
+        valid = rng.random(n) < 0.8
+        got, gv = orc.compact(keep, "I64", vals, valid)
+        np.testing.assert_array_equal(got, vals[keep])
+        np.testing.assert_array_equal(gv, valid[keep])
+        bits = rng.random(n) < 0.5
+        gb, _ = orc.compact(keep, "Bool", bits, None)
+        np.testing.assert_array_equal(gb, bits[keep])
+
+
+def test_scan_kat():
+    # crates/polars-metal-mlx-sys/tests/test_cumsum.rs:12-53: [1,0,1,1,0,1] -> [1,1,2,3,3,4];
+    # compaction places kept row i at prefix[i] - 1.
+    keep = np.array([1, 0, 1, 1, 0, 1], bool)
+    got, _ = orc.compact(keep, "I64", np.arange(6), None)
+    assert list(got) == [0, 2, 3, 5]
+    assert list(np.cumsum(keep)) == [1, 1, 2, 3, 3, 4]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_exact_sum_vs_fsum(seed):
+    rng = np.random.default_rng(seed)
+    parts = [rng.standard_normal(1000), rng.standard_normal(50) * 1e16, -rng.standard_normal(50) * 1e16,
+             np.ldexp(rng.random(20), rng.integers(-1070, -1000, 20)), rng.random(30) * 1e-30]
+    x = np.concatenate(parts)
+    rng.shuffle(x)
+    x = np.concatenate([x, -x[:400]])
+    assert orc.exact_sum(x) == math.fsum(x.tolist())
+    assert orc.exact_sum(np.array([1e16, 1.0, -1e16])) == 1.0
+    assert orc.exact_sum(np.array([0.1] * 10)) == 1.0
+    assert math.isnan(orc.exact_sum(np.array([np.inf, -np.inf])))
+    assert orc.exact_sum(np.array([np.inf, 1.0])) == np.inf
+
+
+def test_group_identity_vs_numpy_unique():
+    rng = np.random.default_rng(7)
+    n = 20000
+    k = rng.integers(0, 500, n)
+    kv = rng.random(n) < 0.9
+    v = rng.standard_normal(n)
+    frame = {"k": ("I64", k, kv), "v": ("F64", v, None)}
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "s"},
+                                             {"kind": "Length", "op": "Len", "output_alias": "n"}]}
+    got = orc.canonical(orc.groupby(plan, frame), ["k"])
+    keys_valid = np.unique(k[kv])
+    assert got["__n_groups__"] == len(keys_valid) + 1  # + the null key
+    tag, gk, gok = got["k"]
+    assert not gok[0] and gok[1:].all()
+    np.testing.assert_array_equal(gk[1:], keys_valid)
+    lens = got["n"][1]
+    assert lens[0] == (~kv).sum()
+    np.testing.assert_array_equal(lens[1:], [np.sum(k[kv] == x) for x in keys_valid])
+    sums = got["s"][1]
+    for i, x in enumerate(keys_valid[:50]):
+        assert sums[i + 1] == math.fsum(v[kv & (k == x)].tolist())
+
+
+def test_key_encoding_widths():
+    # crates/polars-metal-kernels/tests/test_key_encoding.rs: i64 = 65 bits,
+    # i64 + i32 = 98 bits, two i64 + bool... exceed 128 -> TooWide.
+    n = 4
+    f = {"a": ("I64", np.arange(n), None), "b": ("I32", np.arange(n, dtype=np.int32), None),
+         "c": ("Bool", np.array([1, 0, 1, 0], bool), None), "d": ("I64", np.arange(n), None)}
+    assert orc.encode_keys(f, ["a"])[2] == 65
+    assert orc.encode_keys(f, ["a", "b"])[2] == 98
+    assert orc.encode_keys(f, ["c", "b"])[2] == 2 + 33
+    with pytest.raises(ValueError):
+        orc.encode_keys(f, ["a", "d"])
+    # null sets only the null bit: all nulls collapse into one lane
+    g = {"a": ("I64", np.array([5, 6]), np.array([False, False]))}
+    lo, hi, _ = orc.encode_keys(g, ["a"])
+    assert lo[0] == lo[1] == 1 and hi[0] == hi[1] == 0
+    # data bits start right after the null bit, LSB first
+    h = {"a": ("I64", np.array([-1]), None)}
+    lo, hi, _ = orc.encode_keys(h, ["a"])
+    assert int(lo[0]) == (2 ** 64 - 1) << 1 & (2 ** 64 - 1) and int(hi[0]) == 1
+
+
+def test_sequential_sum_mode_differs_only_by_rounding():
+    frame = {"k": ("I64", np.zeros(1000, np.int64), None), "v": ("F64", np.full(1000, 0.1), None)}
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "s"}]}
+    exact = orc.groupby(plan, frame, sum_mode=0)["s"][1][0]
+    seq = orc.groupby(plan, frame, sum_mode=1)["s"][1][0]
+    assert exact == 100.0 and seq != exact and abs(seq - exact) < 1e-10

@@ -68,6 +68,63 @@ __device__ __forceinline__ uint64_t dload_bits(const DevCol &c, uint64_t r) {
 }
 
 // ---------------------------------------------------------------------------
+// row accessors: the predicate, key and aggregate code reads column values
+// through one of these, either straight from memory (MemRow) or from a
+// register cache filled by one batch of loads (CacheRow)
+// ---------------------------------------------------------------------------
+struct MemRow {
+    const QueryDesc *__restrict__ q;
+    uint64_t r;
+    __device__ __forceinline__ bool valid(int c) const { return dvalid(q->cols[c].valid, r); }
+    __device__ __forceinline__ int64_t i64(int c) const { return dload_i64(q->cols[c], r); }
+    __device__ __forceinline__ double f64(int c) const { return dload_f64(q->cols[c], r); }
+    __device__ __forceinline__ uint64_t bits(int c) const { return dload_bits(q->cols[c], r); }
+};
+
+// Raw bits of one row of a column, zero-extended to 64 bits.
+__device__ __forceinline__ uint64_t load_raw(const DevCol &c, uint64_t r) { return dload_bits(c, r); }
+
+__device__ __forceinline__ int64_t raw_to_i64(int tag, uint64_t x) {
+    switch (tag) {
+    case PMH_I32: return (int32_t)(uint32_t)x;
+    case PMH_I16: return (int16_t)(uint16_t)x;
+    case PMH_I8: return (int8_t)(uint8_t)x;
+    case PMH_F64: return (int64_t)__longlong_as_double((long long)x);
+    case PMH_F32: return (int64_t)__uint_as_float((uint32_t)x);
+    default: return (int64_t)x;  // I64, U64, U32, U16, U8, Utf8 codes, Bool
+    }
+}
+
+__device__ __forceinline__ double raw_to_f64(int tag, uint64_t x) {
+    if (tag == PMH_F64) return __longlong_as_double((long long)x);
+    if (tag == PMH_F32) return (double)__uint_as_float((uint32_t)x);
+    return (double)raw_to_i64(tag, x);
+}
+
+// Register cache of up to four columns for one row.  Column indices are
+// wave-uniform; the values live in four scalars (not an array) so the
+// compiler cannot turn the selection back into an indexed private-memory
+// access.
+struct CacheRow {
+    const QueryDesc *__restrict__ q;
+    uint64_t v0, v1, v2, v3;
+    uint32_t vm;  // bit c: column c valid
+    __device__ __forceinline__ uint64_t pick(int c) const {
+        return c == 0 ? v0 : (c == 1 ? v1 : (c == 2 ? v2 : v3));
+    }
+    __device__ __forceinline__ void set(int c, uint64_t x) {
+        if (c == 0) v0 = x;
+        else if (c == 1) v1 = x;
+        else if (c == 2) v2 = x;
+        else v3 = x;
+    }
+    __device__ __forceinline__ bool valid(int c) const { return (vm >> c) & 1; }
+    __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(q->cols[c].tag, pick(c)); }
+    __device__ __forceinline__ double f64(int c) const { return raw_to_f64(q->cols[c].tag, pick(c)); }
+    __device__ __forceinline__ uint64_t bits(int c) const { return pick(c); }
+};
+
+// ---------------------------------------------------------------------------
 // predicate: node k's (data, valid) pair lives in bit k of two masks
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int tot_cmp(double a, double b) {
@@ -88,7 +145,8 @@ __device__ __forceinline__ bool apply_cmp(int op, int c) {
     }
 }
 
-__device__ __forceinline__ bool eval_pred(const QueryDesc *__restrict__ q, uint64_t r) {
+template <class Row>
+__device__ __forceinline__ bool eval_pred(const QueryDesc *__restrict__ q, const Row &row) {
     const int n = q->n_pred;
     if (n == 0) return true;
     uint32_t D = 0, V = 0;
@@ -96,25 +154,23 @@ __device__ __forceinline__ bool eval_pred(const QueryDesc *__restrict__ q, uint6
         const PredNode &nd = q->pred[k];
         bool d = false, v = true;
         switch (nd.kind) {
-        case PK_COLBOOL: {
-            const DevCol &c = q->cols[nd.lhs];
-            v = dvalid(c.valid, r);
-            d = (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
+        case PK_COLBOOL:
+            v = row.valid(nd.lhs);
+            d = row.bits(nd.lhs) & 1;
             break;
-        }
         case PK_CONST: d = nd.cval != 0; break;
         case PK_CMP: {
             bool lv = true, rv = true;
             int c;
             if (nd.is_f64) {
                 double a = nd.lf, b = nd.rf;
-                if (nd.lkind == OK_COL) { const DevCol &cc = q->cols[nd.lhs]; lv = dvalid(cc.valid, r); a = dload_f64(cc, r); }
-                if (nd.rkind == OK_COL) { const DevCol &cc = q->cols[nd.rhs]; rv = dvalid(cc.valid, r); b = dload_f64(cc, r); }
+                if (nd.lkind == OK_COL) { lv = row.valid(nd.lhs); a = row.f64(nd.lhs); }
+                if (nd.rkind == OK_COL) { rv = row.valid(nd.rhs); b = row.f64(nd.rhs); }
                 c = tot_cmp(a, b);
             } else {
                 int64_t a = nd.li, b = nd.ri;
-                if (nd.lkind == OK_COL) { const DevCol &cc = q->cols[nd.lhs]; lv = dvalid(cc.valid, r); a = dload_i64(cc, r); }
-                if (nd.rkind == OK_COL) { const DevCol &cc = q->cols[nd.rhs]; rv = dvalid(cc.valid, r); b = dload_i64(cc, r); }
+                if (nd.lkind == OK_COL) { lv = row.valid(nd.lhs); a = row.i64(nd.lhs); }
+                if (nd.rkind == OK_COL) { rv = row.valid(nd.rhs); b = row.i64(nd.rhs); }
                 c = (a < b) ? -1 : ((a > b) ? 1 : 0);
             }
             v = lv && rv;
@@ -155,22 +211,22 @@ __device__ __forceinline__ void or_shift128(uint64_t &lo, uint64_t &hi, uint64_t
     }
 }
 
-__device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, uint64_t r) {
+template <class Row>
+__device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, const Row &row) {
     Key k{0, 0, 0};
     const int mode = q->key_mode;
     if (mode == KM_NONE) return k;
     if (mode == KM_RAW64) {
-        const DevCol &c = q->cols[q->keys[0].col];
-        if (!dvalid(c.valid, r)) { k.special = 2; return k; }
-        k.lo = dload_bits(c, r);
+        const int c = q->keys[0].col;
+        if (!row.valid(c)) { k.special = 2; return k; }
+        k.lo = row.bits(c);
         if (k.lo == kEmpty) k.special = 1;
         return k;
     }
     for (int i = 0; i < q->n_keys; ++i) {
         const KeyField &f = q->keys[i];
-        const DevCol &c = q->cols[f.col];
-        if (!dvalid(c.valid, r)) or_shift128(k.lo, k.hi, 1, f.null_off);
-        else or_shift128(k.lo, k.hi, dload_bits(c, r), f.data_off);
+        if (!row.valid(f.col)) or_shift128(k.lo, k.hi, 1, f.null_off);
+        else or_shift128(k.lo, k.hi, row.bits(f.col), f.data_off);
     }
     if (q->allones_possible && k.lo == kEmpty && (mode == KM_LANE64 || k.hi == kEmpty)) k.special = 1;
     return k;

@@ -145,17 +145,16 @@ __device__ __forceinline__ int64_t g_find128(const GTable &t, const Key &k, uint
 // ---------------------------------------------------------------------------
 // per-row contribution
 // ---------------------------------------------------------------------------
-template <class S, class Slot>
-__device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, Slot slot, uint64_t r) {
+template <class S, class Slot, class Row>
+__device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, Slot slot, const Row &row) {
     s.add32(q->layout.a_len, slot, 1u);
     const int nv = q->n_vals;
     for (int j = 0; j < nv; ++j) {
         const ValSpec &vs = q->vals[j];
-        const DevCol &c = q->cols[vs.col];
-        if (!dvalid(c.valid, r)) continue;
+        if (!row.valid(vs.col)) continue;
         s.add32(vs.a_cnt, slot, 1u);
         if (vs.kind == VK_F) {
-            double x = dload_f64(c, r);
+            double x = row.f64(vs.col);
             if (vs.need & NEED_SUM) add_f64_exact(s, vs.a_limb, vs.a_res, slot, x);
             if (vs.need & NEED_MINMAX) {
                 if (x != x) {
@@ -167,11 +166,11 @@ __device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, S
                 }
             }
         } else if (vs.kind == VK_I64) {
-            int64_t x = dload_i64(c, r);
+            int64_t x = row.i64(vs.col);
             if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, x);
             if (vs.need & NEED_MINMAX) { s.min64(vs.a_min, slot, x); s.max64(vs.a_max, slot, x); }
         } else {
-            int32_t x = (int32_t)dload_i64(c, r);
+            int32_t x = (int32_t)row.i64(vs.col);
             if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, (int64_t)x);
             if (vs.need & NEED_MINMAX) { s.min32(vs.a_min, slot, x); s.max32(vs.a_max, slot, x); }
         }
@@ -220,7 +219,51 @@ __global__ void fgb_init_table(const Launch *__restrict__ L, uint64_t nslots, in
     if (blockIdx.x == 0 && threadIdx.x < 8) t.ctrl[threadIdx.x] = 0;
 }
 
+// One selected-or-not row of the narrow-key path: predicate, key, then the
+// LDS table (probe / claim) or, when the row finds no LDS slot, the global
+// table.
+template <class Row>
+__device__ __forceinline__ void narrow_row(const Launch *__restrict__ L, const QueryDesc *__restrict__ q,
+                                           const Row &row, uint64_t *lkeys, uint32_t CL, LSink &ls, GSink &gs,
+                                           uint32_t &my_sel) {
+    if (!eval_pred(q, row)) return;
+    ++my_sel;
+    Key k = load_key(q, row);
+    int32_t lslot = -1;
+    uint64_t h = 0;
+    if (k.special) {
+        if (CL) lslot = (int32_t)(CL + k.special - 1);
+    } else {
+        h = hash_key(k.lo, 0);
+        if (CL) {
+            uint32_t s = (uint32_t)(h >> 40) & (CL - 1);
+            for (int p = 0; p < kLdsProbe; ++p) {
+                uint64_t cur = __hip_atomic_load(&lkeys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cur == k.lo) { lslot = (int32_t)s; break; }
+                if (cur == kEmpty) {
+                    unsigned long long old = atomicCAS((unsigned long long *)&lkeys[s], (unsigned long long)kEmpty,
+                                                       (unsigned long long)k.lo);
+                    if (old == kEmpty || old == k.lo) { lslot = (int32_t)s; break; }
+                }
+                s = (s + 1) & (CL - 1);
+            }
+        }
+    }
+    if (lslot >= 0) {
+        add_row(ls, q, (uint32_t)lslot, row);
+    } else {
+        int64_t g = g_find64(L->t, k, h);
+        if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+    }
+}
+
 // Narrow keys (KM_NONE / KM_LANE64 / KM_RAW64): LDS table + global table.
+// CACHED: the query touches at most kCacheCols columns; each thread loads U
+// rows of all of them into registers before evaluating any, so every wave
+// keeps U x (columns) independent loads in flight.
+constexpr int kCacheCols = 4;
+
+template <int U, bool CACHED>
 __global__ __launch_bounds__(1024) void fgb_main_narrow(const Launch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ int s_stop;
@@ -252,6 +295,7 @@ __global__ __launch_bounds__(1024) void fgb_main_narrow(const Launch *__restrict
 
     const uint64_t n = q->n_rows;
     const uint64_t tile = L->tile;
+    const int ncols = q->n_cols;
     uint32_t my_sel = 0;
     for (uint64_t t0 = (uint64_t)blockIdx.x * tile; t0 < n; t0 += (uint64_t)gridDim.x * tile) {
         if (tid == 0) s_stop = (int)__hip_atomic_load(&L->t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -260,36 +304,36 @@ __global__ __launch_bounds__(1024) void fgb_main_narrow(const Launch *__restrict
         __syncthreads();
         if (stop) break;
         const uint64_t tend = min(t0 + tile, n);
-        for (uint64_t r = t0 + tid; r < tend; r += bs) {
-            if (!eval_pred(q, r)) continue;
-            ++my_sel;
-            Key k = load_key(q, r);
-            int32_t lslot = -1;
-            uint64_t h = 0;
-            if (k.special) {
-                if (CL) lslot = (int32_t)(CL + k.special - 1);
-            } else {
-                h = hash_key(k.lo, 0);
-                if (CL) {
-                    uint32_t s = (uint32_t)(h >> 40) & (CL - 1);
-                    for (int p = 0; p < kLdsProbe; ++p) {
-                        uint64_t cur = __hip_atomic_load(&lkeys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (cur == k.lo) { lslot = (int32_t)s; break; }
-                        if (cur == kEmpty) {
-                            unsigned long long old = atomicCAS((unsigned long long *)&lkeys[s],
-                                                               (unsigned long long)kEmpty, (unsigned long long)k.lo);
-                            if (old == kEmpty || old == k.lo) { lslot = (int32_t)s; break; }
+        if constexpr (CACHED) {
+            static_assert(U == 4, "the cached path loads four rows per thread");
+            auto fill = [&](CacheRow &row, uint64_t r) __attribute__((always_inline)) {
+                row.q = q;
+                row.vm = 0;
+                row.v0 = row.v1 = row.v2 = row.v3 = 0;
+                if (r < tend) {
+#pragma unroll
+                    for (int c = 0; c < kCacheCols; ++c) {
+                        if (c < ncols) {
+                            const DevCol &col = q->cols[c];
+                            row.set(c, load_raw(col, r));
+                            row.vm |= (uint32_t)dvalid(col.valid, r) << c;
                         }
-                        s = (s + 1) & (CL - 1);
                     }
                 }
+            };
+            for (uint64_t r0 = t0 + tid; r0 < tend; r0 += 4ull * bs) {
+                CacheRow a0, a1, a2, a3;
+                fill(a0, r0);
+                fill(a1, r0 + bs);
+                fill(a2, r0 + 2ull * bs);
+                fill(a3, r0 + 3ull * bs);
+                narrow_row(L, q, a0, lkeys, CL, ls, gs, my_sel);
+                if (r0 + bs < tend) narrow_row(L, q, a1, lkeys, CL, ls, gs, my_sel);
+                if (r0 + 2ull * bs < tend) narrow_row(L, q, a2, lkeys, CL, ls, gs, my_sel);
+                if (r0 + 3ull * bs < tend) narrow_row(L, q, a3, lkeys, CL, ls, gs, my_sel);
             }
-            if (lslot >= 0) {
-                add_row(ls, q, (uint32_t)lslot, r);
-            } else {
-                int64_t g = g_find64(L->t, k, h);
-                if (g >= 0) add_row(gs, q, (uint64_t)g, r);
-            }
+        } else {
+            for (uint64_t r = t0 + tid; r < tend; r += bs) narrow_row(L, q, MemRow{q, r}, lkeys, CL, ls, gs, my_sel);
         }
     }
     atomicAdd(&s_sel, my_sel);
@@ -328,12 +372,13 @@ __global__ __launch_bounds__(256) void fgb_main_wide(const Launch *__restrict__ 
         if (stop) break;
         const uint64_t tend = min(t0 + tile, n);
         for (uint64_t r = t0 + tid; r < tend; r += bs) {
-            if (!eval_pred(q, r)) continue;
+            MemRow row{q, r};
+            if (!eval_pred(q, row)) continue;
             ++my_sel;
-            Key k = load_key(q, r);
+            Key k = load_key(q, row);
             uint64_t h = k.special ? 0 : hash_key(k.lo, k.hi);
             int64_t g = g_find128(L->t, k, h);
-            if (g >= 0) add_row(gs, q, (uint64_t)g, r);
+            if (g >= 0) add_row(gs, q, (uint64_t)g, row);
         }
     }
     atomicAdd(&s_sel, my_sel);
@@ -812,9 +857,10 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
         uint64_t tiles = ceil_div(std::max<uint64_t>(n_rows, 1), tile);
         if ((uint64_t)grid > tiles) grid = (int)tiles;
     }
+    const bool cached = q.n_cols <= kCacheCols;
+    const void *narrow_fn = cached ? (const void *)fgb_main_narrow<4, true> : (const void *)fgb_main_narrow<1, false>;
     if (lds_bytes > 65536)
-        PMH_HIP(hipFuncSetAttribute((const void *)fgb_main_narrow, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds_bytes));
+        PMH_HIP(hipFuncSetAttribute(narrow_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes));
 
     // ---- passes (plain method: resize x4 on overflow)
     Launch *dL = (Launch *)slab("gb:launch", sizeof(Launch));
@@ -872,8 +918,10 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
             KernelTimer kt(wide ? "fgb_main_wide" : "fgb_main_narrow");
             if (wide)
                 hipLaunchKernelGGL(fgb_main_wide, dim3(grid), dim3(block), 0, R.stream, dL);
+            else if (cached)
+                hipLaunchKernelGGL((fgb_main_narrow<4, true>), dim3(grid), dim3(block), lds_bytes, R.stream, dL);
             else
-                hipLaunchKernelGGL(fgb_main_narrow, dim3(grid), dim3(block), lds_bytes, R.stream, dL);
+                hipLaunchKernelGGL((fgb_main_narrow<1, false>), dim3(grid), dim3(block), lds_bytes, R.stream, dL);
             PMH_HIP(hipGetLastError());
         }
         PMH_HIP(hipMemcpyAsync(ctrl_h, t.ctrl, sizeof(ctrl_h), hipMemcpyDeviceToHost, R.stream));

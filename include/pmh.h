@@ -86,7 +86,8 @@ typedef struct pmh_out_column {
 } pmh_out_column;
 
 /* Group-by path selection recorded for each call (no reference
- * counterpart: the reference logs nothing, SURVEY of _callback.py:22-125). */
+ * counterpart: the reference only logs debug lines,
+ * python/polars_metal/_callback.py:22-125). */
 typedef enum { PMH_METHOD_HASH = 0, PMH_METHOD_SORT = 1 } pmh_method;
 
 typedef struct pmh_exec_stats {
@@ -182,6 +183,16 @@ int pmh_compute_lifting_plan(const char *plan_json, char **out_json);
  * (crates/polars-metal-core/src/udf/groupby/mod.rs:187-201): initialises the
  * device, stream and scratch arena; returns the number of kernels warmed. */
 int pmh_warmup_common_fused_signatures(void);
+
+/* hipMalloc-backed Arrow buffers on the library's device: the rework of
+ * polars-metal-buffer's MetalDevice (crates/polars-metal-buffer/src/device.rs:
+ * new_buffer_uninit :144, new_buffer_zeroed :55, new_buffer_from_bytes :104).
+ * Pointers returned here may be passed as PMH_DEVICE columns. */
+int pmh_buffer_alloc(uint64_t bytes, void **out_device_ptr);
+int pmh_buffer_zeroed(uint64_t bytes, void **out_device_ptr);
+int pmh_buffer_from_host(const void *src, uint64_t bytes, void **out_device_ptr);
+int pmh_buffer_to_host(const void *device_ptr, void *dst, uint64_t bytes);
+int pmh_buffer_free(void *device_ptr);
 
 void pmh_result_free(pmh_result *r);
 void pmh_string_free(char *s);

@@ -144,6 +144,57 @@ int pmh_warmup_common_fused_signatures(void) {
     return st == PMH_OK ? n : -st;
 }
 
+int pmh_buffer_alloc(uint64_t bytes, void **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        rt();
+        *out = nullptr;
+        PMH_HIP(hipMalloc(out, bytes ? bytes : 1));
+    });
+}
+
+int pmh_buffer_zeroed(uint64_t bytes, void **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        Runtime &r = rt();
+        *out = nullptr;
+        PMH_HIP(hipMalloc(out, bytes ? bytes : 1));
+        if (bytes) PMH_HIP(hipMemsetAsync(*out, 0, bytes, r.stream));
+        PMH_HIP(hipStreamSynchronize(r.stream));
+    });
+}
+
+int pmh_buffer_from_host(const void *src, uint64_t bytes, void **out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        if (bytes && !src) fail(PMH_E_VALUE, "polars_metal: source is NULL");
+        Runtime &r = rt();
+        *out = nullptr;
+        PMH_HIP(hipMalloc(out, bytes ? bytes : 1));
+        if (bytes) PMH_HIP(hipMemcpyAsync(*out, src, bytes, hipMemcpyHostToDevice, r.stream));
+        PMH_HIP(hipStreamSynchronize(r.stream));
+    });
+}
+
+int pmh_buffer_to_host(const void *dev, void *dst, uint64_t bytes) {
+    return guarded([&] {
+        if (bytes && (!dev || !dst)) fail(PMH_E_VALUE, "polars_metal: NULL buffer");
+        Runtime &r = rt();
+        if (bytes) PMH_HIP(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, r.stream));
+        PMH_HIP(hipStreamSynchronize(r.stream));
+    });
+}
+
+int pmh_buffer_free(void *dev) {
+    return guarded([&] {
+        Runtime &r = rt();
+        if (dev) {
+            PMH_HIP(hipStreamSynchronize(r.stream));
+            PMH_HIP(hipFree(dev));
+        }
+    });
+}
+
 void pmh_result_free(pmh_result *r) {
     if (!r) return;
     for (uint32_t i = 0; i < r->n_columns; ++i) {

@@ -95,6 +95,11 @@ SIGNATURES = {
     "pmh_bool_or": (ctypes.c_int, [_U8P, _U8P, _U8P, _U8P, ctypes.c_uint64, _U8P, _U8P]),
     "pmh_compute_lifting_plan": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
     "pmh_warmup_common_fused_signatures": (ctypes.c_int, []),
+    "pmh_buffer_alloc": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "pmh_buffer_zeroed": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "pmh_buffer_from_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "pmh_buffer_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
+    "pmh_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
     "pmh_result_free": (None, [ctypes.POINTER(PmhResult)]),
     "pmh_string_free": (None, [ctypes.c_void_p]),
     "pmh_set_profiling": (ctypes.c_int, [ctypes.c_int]),

@@ -257,23 +257,12 @@ class DeviceResult:
             pass
 
 
-_HIP = None
-
-
 def _device_to_host(ptr: int, nbytes: int) -> bytes:
-    """hipMemcpy of device memory into a new bytes object (HIP runtime)."""
-    global _HIP
+    """Copy device memory into a new bytes object."""
     if nbytes == 0:
         return b""
-    if _HIP is None:
-        _HIP = ctypes.CDLL("libamdhip64.so")
-        _HIP.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        _HIP.hipMemcpy.restype = ctypes.c_int
-        _HIP.hipDeviceSynchronize.restype = ctypes.c_int
     buf = ctypes.create_string_buffer(nbytes)
-    rc = _HIP.hipMemcpy(buf, ctypes.c_void_p(ptr), nbytes, 2)  # hipMemcpyDeviceToHost
-    if rc != 0:
-        raise _lib._compute_error_cls()(f"polars_metal: hipMemcpy failed ({rc})")
+    check(_lib.load().pmh_buffer_to_host(ctypes.c_void_p(ptr), buf, nbytes))
     return buf.raw
 
 

@@ -98,6 +98,14 @@ typedef struct pmh_exec_stats {
     uint64_t n_groups;
     uint64_t n_selected;      /* rows that passed the predicate */
     double device_ms;         /* device time of the call (HIP events) */
+    /* row-sample sketch of the partitioned hash engine (zeros otherwise) */
+    uint32_t sample_rows;     /* sampled rows that passed the predicate */
+    uint32_t sample_distinct; /* distinct keys among them */
+    uint32_t sample_f1;       /* keys seen once */
+    uint32_t sample_f2;       /* keys seen twice */
+    uint32_t hot_keys;        /* keys folded in LDS by every scan block */
+    uint32_t partitions;      /* tail partitions */
+    uint64_t tail_rows;       /* selected rows not covered by the hot keys */
 } pmh_exec_stats;
 
 typedef struct pmh_result {

@@ -30,34 +30,13 @@
 #include <cstring>
 #include <functional>
 
-#include "pmh_device.h"
+#include "pmh_table.h"
 
 namespace pmh {
 
 // ---------------------------------------------------------------------------
 // launch descriptor (device memory, read with scalar loads)
 // ---------------------------------------------------------------------------
-struct GTable {
-    uint64_t *klo;
-    uint64_t *khi;      // KM_LANE128 only
-    uint32_t *kstate;   // KM_LANE128 only: 0 empty, 1 being written, 2 published
-    uint32_t *list;     // claimed slots in claim order
-    uint32_t *ctrl;     // [0] groups claimed [1] overflow [2..3] side-slot claims [4] rows selected
-    uint64_t mask;      // capacity - 1 (capacity is a power of two)
-    uint64_t limit;     // distinct keys allowed before the pass overflows
-    uint64_t list_cap;
-    void *arr[kMaxArrs];
-};
-
-constexpr int kOutMax = 64;
-enum OutSrc : int32_t { OS_KEY = 0, OS_AGG = 1 };
-
-struct OutCol {
-    int32_t src, idx, op, tag;  // OS_KEY: idx = key field; OS_AGG: idx = value spec (or -1 for len)
-    void *data;
-    uint64_t *valid;            // bitmap words, nullptr: not produced on device
-};
-
 struct Launch {
     QueryDesc q;
     GTable t;
@@ -68,137 +47,6 @@ struct Launch {
     OutCol out[kOutMax];
 };
 
-constexpr int kLdsProbe = 16;
-
-// ---------------------------------------------------------------------------
-// global table
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void g_register(const GTable &t, uint64_t slot) {
-    uint32_t idx = atomicAdd(&t.ctrl[0], 1u);
-    if (idx < t.list_cap) t.list[idx] = (uint32_t)slot;
-    if (idx >= t.limit) atomicOr(&t.ctrl[1], 1u);
-}
-
-__device__ __forceinline__ int64_t g_side_slot(const GTable &t, int special) {
-    uint32_t s = (uint32_t)special - 1;
-    uint64_t slot = t.mask + 1 + s;
-    if (__hip_atomic_load(&t.ctrl[2 + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-        atomicCAS(&t.ctrl[2 + s], 0u, 1u) == 0u)
-        g_register(t, slot);
-    return (int64_t)slot;
-}
-
-// 64-bit keys: the key word itself is claimed with one CAS, so no other
-// field needs publishing (the state arrays are initialised before the pass).
-__device__ __forceinline__ int64_t g_find64(const GTable &t, const Key &k, uint64_t h) {
-    if (k.special) return g_side_slot(t, k.special);
-    uint64_t s = h & t.mask;
-    for (uint64_t p = 0; p <= t.mask; ++p) {
-        uint64_t cur = __hip_atomic_load(&t.klo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cur == k.lo) return (int64_t)s;
-        if (cur == kEmpty) {
-            unsigned long long old = atomicCAS((unsigned long long *)&t.klo[s], (unsigned long long)kEmpty,
-                                               (unsigned long long)k.lo);
-            if (old == kEmpty) { g_register(t, s); return (int64_t)s; }
-            if (old == k.lo) return (int64_t)s;
-        }
-        s = (s + 1) & t.mask;
-        if ((p & 63) == 63 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            return -1;
-    }
-    atomicOr(&t.ctrl[1], 1u);
-    return -1;
-}
-
-// 128-bit lanes: claim word 0 -> 1 by CAS, write both key words, publish 2
-// with release; readers acquire before comparing.  The claimer finishes in
-// the same loop iteration, so a wave never waits on one of its own lanes.
-__device__ __forceinline__ int64_t g_find128(const GTable &t, const Key &k, uint64_t h) {
-    if (k.special) return g_side_slot(t, k.special);
-    uint64_t s = h & t.mask;
-    uint64_t p = 0;
-    while (p <= t.mask) {
-        uint32_t st = __hip_atomic_load(&t.kstate[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        if (st == 2u) {
-            if (t.klo[s] == k.lo && t.khi[s] == k.hi) return (int64_t)s;
-        } else if (st == 0u) {
-            if (atomicCAS(&t.kstate[s], 0u, 1u) == 0u) {
-                t.klo[s] = k.lo;
-                t.khi[s] = k.hi;
-                __hip_atomic_store(&t.kstate[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                g_register(t, s);
-                return (int64_t)s;
-            }
-            continue;  // lost the race: re-read this slot
-        } else {
-            continue;  // being written by another wave: re-read
-        }
-        s = (s + 1) & t.mask;
-        ++p;
-        if ((p & 63) == 0 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            return -1;
-    }
-    atomicOr(&t.ctrl[1], 1u);
-    return -1;
-}
-
-// ---------------------------------------------------------------------------
-// per-row contribution
-// ---------------------------------------------------------------------------
-template <class S, class Slot, class Row>
-__device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, Slot slot, const Row &row) {
-    s.add32(q->layout.a_len, slot, 1u);
-    const int nv = q->n_vals;
-    for (int j = 0; j < nv; ++j) {
-        const ValSpec &vs = q->vals[j];
-        if (!row.valid(vs.col)) continue;
-        s.add32(vs.a_cnt, slot, 1u);
-        if (vs.kind == VK_F) {
-            double x = row.f64(vs.col);
-            if (vs.need & NEED_SUM) add_f64_exact(s, vs.a_limb, vs.a_res, slot, x);
-            if (vs.need & NEED_MINMAX) {
-                if (x != x) {
-                    s.or32(vs.a_nan, slot, 1u);
-                } else {
-                    int64_t kk = f64_order_key(x);
-                    s.min64(vs.a_min, slot, kk);
-                    s.max64(vs.a_max, slot, kk);
-                }
-            }
-        } else if (vs.kind == VK_I64) {
-            int64_t x = row.i64(vs.col);
-            if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, x);
-            if (vs.need & NEED_MINMAX) { s.min64(vs.a_min, slot, x); s.max64(vs.a_max, slot, x); }
-        } else {
-            int32_t x = (int32_t)row.i64(vs.col);
-            if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, (int64_t)x);
-            if (vs.need & NEED_MINMAX) { s.min32(vs.a_min, slot, x); s.max32(vs.a_max, slot, x); }
-        }
-    }
-}
-
-// Merge LDS slot ls into global slot gs: one atomic per touched array.
-__device__ __forceinline__ void merge_lds_entry(GSink &g, uint64_t gs, LSink &l, uint32_t ls,
-                                                const Layout &lay, const int32_t *ops) {
-    for (int a = 0; a < lay.n_arrs; ++a) {
-        const int op = ops[a];
-        if (mo_size(op) == 4) {
-            uint32_t v = *l.p<uint32_t>(a, ls);
-            if (v == (uint32_t)mo_identity(op)) continue;
-            if (op == MO_ADD32) g.add32(a, gs, v);
-            else if (op == MO_MIN32) g.min32(a, gs, (int32_t)v);
-            else if (op == MO_MAX32) g.max32(a, gs, (int32_t)v);
-            else g.or32(a, gs, v);
-        } else {
-            uint64_t v = *l.p<uint64_t>(a, ls);
-            if (v == mo_identity(op)) continue;
-            if (op == MO_ADD64) g.add64(a, gs, (int64_t)v);
-            else if (op == MO_ADDF64) g.addf(a, gs, __longlong_as_double((long long)v));
-            else if (op == MO_MIN64) g.min64(a, gs, (int64_t)v);
-            else g.max64(a, gs, (int64_t)v);
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // kernels
@@ -386,39 +234,18 @@ __global__ __launch_bounds__(256) void fgb_main_wide(const Launch *__restrict__ 
     if (tid == 0) atomicAdd(&L->t.ctrl[4], s_sel);
 }
 
-// Key field value of one slot, as raw bits and a null flag.
-__device__ __forceinline__ void decode_key_field(const QueryDesc *__restrict__ q, const GTable &t, uint64_t slot,
-                                                 int fi, uint64_t &bits, bool &valid) {
+// Key of global-table slot `slot` as (lo, hi, special): the two side slots
+// past the capacity hold the all-ones key and the null key.
+__device__ __forceinline__ void slot_key(const QueryDesc *__restrict__ q, const GTable &t, uint64_t slot,
+                                         uint64_t &lo, uint64_t &hi, int &special) {
     const uint64_t cap = t.mask + 1;
-    const int mode = q->key_mode;
-    if (mode == KM_RAW64) {
-        if (slot == cap + 1) { bits = 0; valid = false; }
-        else { bits = slot == cap ? kEmpty : t.klo[slot]; valid = true; }
-        return;
-    }
-    uint64_t lo, hi = 0;
-    if (slot == cap) { lo = kEmpty; hi = mode == KM_LANE128 ? kEmpty : 0; }
-    else { lo = t.klo[slot]; if (mode == KM_LANE128) hi = t.khi[slot]; }
-    const KeyField &f = q->keys[fi];
-    auto bit_at = [&](int pos) -> uint64_t { return pos < 64 ? (lo >> pos) & 1 : (hi >> (pos - 64)) & 1; };
-    valid = bit_at(f.null_off) == 0;
-    int off = f.data_off;
-    uint64_t v;
-    if (off == 0) v = lo;
-    else if (off < 64) v = (lo >> off) | (hi << (64 - off));
-    else v = hi >> (off - 64);
-    if (f.bits < 64) v &= (1ull << f.bits) - 1;
-    bits = valid ? v : 0;
-}
-
-__device__ __forceinline__ void store_tag(void *data, uint64_t i, int tag, uint64_t bits) {
-    switch (tag_size(tag)) {
-    case 8: ((uint64_t *)data)[i] = bits; break;
-    case 4: ((uint32_t *)data)[i] = (uint32_t)bits; break;
-    case 2: ((uint16_t *)data)[i] = (uint16_t)bits; break;
-    case 1: ((uint8_t *)data)[i] = (uint8_t)bits; break;
-    default: break;  // Bool: written through the ballot
-    }
+    lo = 0;
+    hi = 0;
+    special = 0;
+    if (slot == cap) { special = 1; return; }
+    if (slot == cap + 1) { special = 2; return; }
+    lo = t.klo[slot];
+    if (q->key_mode == KM_LANE128) hi = t.khi[slot];
 }
 
 // One thread per output group; waves of 64 consecutive groups pack validity
@@ -427,6 +254,7 @@ __global__ __launch_bounds__(256) void fgb_finalize(const Launch *__restrict__ L
     const QueryDesc *__restrict__ q = &L->q;
     const GTable &t = L->t;
     const Layout &lay = q->layout;
+    GReader rd{t.arr};
     uint64_t ng = *(volatile uint32_t *)&t.ctrl[0];
     if (ng > t.list_cap) ng = t.list_cap;
     const uint64_t nr = (ng + 63) & ~63ull;
@@ -435,68 +263,16 @@ __global__ __launch_bounds__(256) void fgb_finalize(const Launch *__restrict__ L
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nr; i += stride) {
         const bool live = i < ng;
         const uint64_t slot = live ? t.list[i] : 0;
+        uint64_t klo = 0, khi = 0;
+        int special = 0;
+        if (live) slot_key(q, t, slot, klo, khi, special);
         for (int o = 0; o < L->n_out; ++o) {
             const OutCol &oc = L->out[o];
             uint64_t bits = 0;
             bool valid = false;
             if (live) {
-                if (oc.src == OS_KEY) {
-                    decode_key_field(q, t, slot, oc.idx, bits, valid);
-                } else if (oc.idx < 0) {
-                    bits = ((const uint32_t *)t.arr[lay.a_len])[slot];
-                    valid = true;
-                } else {
-                    const ValSpec &vs = q->vals[oc.idx];
-                    uint32_t cnt = ((const uint32_t *)t.arr[vs.a_cnt])[slot];
-                    switch (oc.op) {
-                    case AGG_COUNT: bits = cnt; valid = true; break;
-                    case AGG_SUM:
-                    case AGG_MEAN: {
-                        double d = 0.0;
-                        int64_t si = 0;
-                        if (vs.kind == VK_F) {
-                            int64_t limb[kLimbs];
-                            for (int j = 0; j < kLimbs; ++j) limb[j] = ((const int64_t *)t.arr[vs.a_limb + j])[slot];
-                            d = limbs_to_double(limb, ((const double *)t.arr[vs.a_res])[slot]);
-                        } else {
-                            si = ((const int64_t *)t.arr[vs.a_sum])[slot];
-                        }
-                        if (oc.op == AGG_SUM) {
-                            valid = true;
-                            if (vs.kind == VK_F) {
-                                if (oc.tag == PMH_F32) { float f = (float)d; bits = (uint32_t)__float_as_uint(f); }
-                                else bits = (uint64_t)__double_as_longlong(d);
-                            } else {
-                                bits = (uint64_t)si;  // I32 output keeps the low 32 bits: wrapping sum
-                            }
-                        } else {
-                            valid = cnt > 0;
-                            double m = 0.0;
-                            if (valid) m = (vs.kind == VK_F ? d : (double)si) / (double)cnt;
-                            if (oc.tag == PMH_F32) { float f = (float)m; bits = (uint32_t)__float_as_uint(f); }
-                            else bits = (uint64_t)__double_as_longlong(m);
-                        }
-                        break;
-                    }
-                    default: {  // AGG_MIN / AGG_MAX
-                        valid = cnt > 0;
-                        if (!valid) break;
-                        const int a = oc.op == AGG_MIN ? vs.a_min : vs.a_max;
-                        if (vs.kind == VK_F) {
-                            double d = ((const uint32_t *)t.arr[vs.a_nan])[slot]
-                                           ? __longlong_as_double(0x7FF8000000000000ll)
-                                           : f64_from_order_key(((const int64_t *)t.arr[a])[slot]);
-                            if (oc.tag == PMH_F32) { float f = (float)d; bits = (uint32_t)__float_as_uint(f); }
-                            else bits = (uint64_t)__double_as_longlong(d);
-                        } else if (vs.kind == VK_I64) {
-                            bits = (uint64_t)((const int64_t *)t.arr[a])[slot];
-                        } else {
-                            bits = (uint32_t)((const int32_t *)t.arr[a])[slot];
-                        }
-                        break;
-                    }
-                    }
-                }
+                if (oc.src == OS_KEY) key_field(q, oc.idx, klo, khi, special, bits, valid);
+                else agg_value(q, oc, rd, slot, bits, valid);
                 if (oc.tag != PMH_BOOL || oc.src != OS_KEY) store_tag(oc.data, i, oc.tag, bits);
             }
             if (oc.valid) {
@@ -647,8 +423,7 @@ void collect_pred_cols(const PlanPredNode &n, std::vector<std::string> &out) {
 
 }  // namespace
 
-pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols,
-                               uint32_t n_cols, uint32_t flags) {
+void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols) {
     if (n_rows >= (1ull << 31)) fail(PMH_E_VALUE, "polars_metal: row count must be below 2^31");
     if (plan.aggs.empty() && plan.keys.empty()) fail(PMH_E_VALUE, "polars_metal: plan parse error: empty plan");
     // ---- columns referenced by the plan
@@ -678,7 +453,7 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
             fail(PMH_E_VALUE, std::string("polars_metal: unsupported (agg_op, dtype) combination: ") + op_name(a.op) +
                                   " / " + tag_name(c->tag));
     }
-    Runtime &R = rt();
+    rt();
     std::vector<Column> dcols = stage_columns(cols, n_cols, n_rows, needed);
     auto col_index = [&](const std::string &nm) -> int {
         for (size_t i = 0; i < dcols.size(); ++i)
@@ -687,11 +462,9 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     };
     if ((int)dcols.size() > kMaxCols) fail(PMH_E_NOTIMPL, "polars_metal: too many columns in one query");
 
-    // ---- launch descriptor
-    auto Lh = std::make_unique<Launch>();
-    memset(Lh.get(), 0, sizeof(Launch));
-    Launch &L = *Lh;
-    QueryDesc &q = L.q;
+    QueryDesc &q = P.q;
+    memset(&q, 0, sizeof(q));
+    P.n_rows = n_rows;
     q.n_rows = n_rows;
     q.n_cols = (int)dcols.size();
     for (size_t i = 0; i < dcols.size(); ++i) {
@@ -729,6 +502,7 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     else if (off <= 64) { q.key_mode = KM_LANE64; q.allones_possible = off == 64; }
     else { q.key_mode = KM_LANE128; q.allones_possible = off == 128; }
     const bool wide = q.key_mode == KM_LANE128;
+    P.wide = wide;
 
     // values and state layout
     Layout &lay = q.layout;
@@ -781,7 +555,9 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     std::string sig = std::to_string(q.key_mode) + ":";
     for (int a = 0; a < lay.n_arrs; ++a) sig += std::to_string(lay.init[a]) + ",";
     // limb k > 0 of an exact accumulator: addressed from limb 0
-    std::vector<int> limb_k(lay.n_arrs, 0), limb_base(lay.n_arrs, -1);
+    std::vector<int> &limb_k = P.limb_k, &limb_base = P.limb_base;
+    limb_k.assign(lay.n_arrs, 0);
+    limb_base.assign(lay.n_arrs, -1);
     for (int j = 0; j < q.n_vals; ++j)
         if (q.vals[j].kind == VK_F && (q.vals[j].need & NEED_SUM))
             for (int k = 0; k < kLimbs; ++k) { limb_k[q.vals[j].a_limb + k] = k; limb_base[q.vals[j].a_limb + k] = q.vals[j].a_limb; }
@@ -789,10 +565,12 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     // ---- outputs: keys then aggs
     int n_out = q.n_keys + (int)plan.aggs.size();
     if (n_out > kOutMax) fail(PMH_E_NOTIMPL, "polars_metal: too many output columns");
-    L.n_out = n_out;
-    std::vector<std::string> out_names;
+    P.n_out = n_out;
+    std::vector<std::string> &out_names = P.names;
+    out_names.clear();
+    memset(P.outs, 0, sizeof(P.outs));
     for (int i = 0; i < q.n_keys; ++i) {
-        OutCol &oc = L.out[i];
+        OutCol &oc = P.outs[i];
         oc.src = OS_KEY;
         oc.idx = i;
         oc.tag = dcols[q.keys[i].col].tag;
@@ -800,7 +578,7 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     }
     for (size_t ai = 0; ai < plan.aggs.size(); ++ai) {
         const PlanAgg &a = plan.aggs[ai];
-        OutCol &oc = L.out[q.n_keys + ai];
+        OutCol &oc = P.outs[q.n_keys + ai];
         oc.src = OS_AGG;
         oc.op = a.op;
         oc.idx = agg_val[ai];
@@ -812,7 +590,25 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
         }
         out_names.push_back(a.alias);
     }
+    P.sig = sig;
+}
 
+pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
+    Runtime &R = rt();
+    const uint64_t n_rows = P.n_rows;
+    const bool wide = P.wide;
+    const int n_out = P.n_out;
+    const std::vector<std::string> &out_names = P.names;
+    const std::vector<int> &limb_k = P.limb_k, &limb_base = P.limb_base;
+    const std::string &sig = P.sig;
+    auto Lh = std::make_unique<Launch>();
+    memset(Lh.get(), 0, sizeof(Launch));
+    Launch &L = *Lh;
+    L.q = P.q;
+    QueryDesc &q = L.q;
+    Layout &lay = q.layout;
+    L.n_out = n_out;
+    for (int i = 0; i < n_out; ++i) L.out[i] = P.outs[i];
     // ---- sizing
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
     uint64_t hint = plan.opts.capacity_hint;
@@ -1000,6 +796,21 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     }
     collect_kernel_times();
     return res;
+}
+
+pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols,
+                               uint32_t n_cols, uint32_t flags) {
+    Prep P;
+    prepare_query(P, plan, n_rows, cols, n_cols);
+    const std::string &m = plan.opts.method;
+    if (m == "hash_global") return run_hash_global(P, plan, flags);
+    if (m == "auto" || m == "hash") {
+        pmh_result *r = run_hash_partitioned(P, plan, flags);
+        if (r) return r;
+        if (m == "hash") fail(PMH_E_VALUE, "polars_metal: the partitioned hash path cannot run this query shape");
+    }
+    if (m == "sort") fail(PMH_E_NOTIMPL, "polars_metal: the sort path is not built yet");
+    return run_hash_global(P, plan, flags);
 }
 
 }  // namespace pmh

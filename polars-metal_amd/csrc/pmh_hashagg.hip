@@ -1,0 +1,1320 @@
+// pmh_hashagg.hip -- partitioned hash aggregation, the default hash path.
+//
+// Global atomics to scattered addresses run at a few tens of Gops/s on
+// MI355X while LDS atomics run at Tops/s, so every per-row update happens in
+// LDS and global memory only sees coalesced streams:
+//
+//   hs_sample_gather  keys of a strided row sample that pass the predicate
+//   hs_sample_count   one block counts the sample in LDS and lays the most
+//                     frequent keys (count >= 2) out in a hot table shared by
+//                     every block; it also records the sample sketch
+//   hs_scan           one block per CU streams a contiguous chunk of rows with
+//                     the register row cache: predicate, key, hot-table probe
+//                     (read only, no CAS).  Hot rows fold into LDS state; the
+//                     other rows ("tail") are appended, wave-contiguously, to
+//                     the block's spill region.  At the end the block adds its
+//                     hot partials into the global hot state with coalesced
+//                     atomics (slot s of every block maps to global slot s).
+//   rp_*              stable LSD radix passes on the key hash (5 bits each)
+//                     group the tail by partition: histogram per tile, one
+//                     scan per digit, then a scatter that ranks rows with wave
+//                     multisplit ballots, stages the tile in LDS in digit order
+//                     and writes each digit run contiguously.
+//   hs_part_agg       one block per partition folds the partition's tail rows
+//                     into an LDS table and writes its groups straight to the
+//                     outputs; a row that finds no LDS slot goes to the global
+//                     fallback table (finalised by fgb_finalize_into).
+//   hs_hot_emit       writes the hot groups.
+//   hs_pack           turns the per-row validity bytes into Arrow bitmaps.
+#include <algorithm>
+#include <cstring>
+
+#include "pmh_table.h"
+
+namespace pmh {
+
+constexpr int kRadixBits = 5;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kPartThreads = 1024;
+constexpr int kPartRows = 4;               // rows per thread per partition tile
+constexpr int kPartTile = kPartThreads * kPartRows;
+constexpr uint16_t kFlagAllOnes = 1u << 8, kFlagNull = 1u << 9;
+constexpr int kProbe = 16;
+
+struct Spill {
+    uint64_t *klo, *khi;  // khi: wide keys only
+    void *val[kMaxVals];  // value j as raw bits of width vw[j]
+    uint16_t *flags;      // bit j: value j valid; kFlagAllOnes / kFlagNull: side keys
+};
+
+enum StatIdx : int32_t {
+    ST_SELECTED = 0, ST_TAIL = 1, ST_SAMPLE_SEL = 2, ST_SAMPLE_DISTINCT = 3, ST_SAMPLE_F1 = 4,
+    ST_SAMPLE_F2 = 5, ST_HOT = 6, ST_HOT_SAMPLE_ROWS = 7, ST_TILES0 = 8, ST_SAMPLE_NULLKEY = 9,
+    ST_SAMPLE_DROPPED = 10, ST_ERROR = 11, ST_N = 16
+};
+
+struct HLaunch {
+    QueryDesc q;
+    int32_t wide, n_out;
+    int32_t vw[kMaxVals];
+    int32_t col_val[kMaxCols];
+    // sample and hot set
+    uint32_t n_sample, hot_slots, hot_max, pad0;
+    uint64_t *samp_lo, *samp_hi;
+    uint8_t *samp_sp;               // side-key marker of each sampled key
+    uint32_t sample_slots;          // LDS count-table size of hs_sample_count
+    uint32_t pad2;
+    uint64_t *hot_lo, *hot_hi;
+    void *hot_arr[kMaxArrs];
+    uint32_t lds1_off[kMaxArrs];
+    // scan
+    uint64_t chunk;
+    uint32_t nb;
+    uint32_t *tail_n;      // [nb]
+    uint32_t *region_off;  // [nb + 1]
+    uint32_t *tile_off;    // [nb + 1] first-pass tiles per region, prefix
+    Spill spill;
+    // partition passes
+    Spill ping, pong;
+    uint32_t *tile_hist;   // [kRadix][max_tiles]
+    uint32_t *digit_base;  // [kRadix]
+    uint32_t max_tiles, pbits, sortbits, passes;
+    // per-partition aggregation
+    uint32_t tail_slots, pad1;
+    uint32_t lds3_off[kMaxArrs];
+    GTable t;
+    // outputs
+    OutCol out[kOutMax];
+    uint32_t *out_cursor;  // [0] groups written by hs_hot_emit and hs_part_agg
+    uint32_t *stats;       // StatIdx
+};
+
+// Digit of a spilled key for partitioning: side keys go to partition 0.
+__device__ __forceinline__ uint64_t spill_hash(uint64_t lo, uint64_t hi, uint16_t flags) {
+    return (flags & (kFlagAllOnes | kFlagNull)) ? 0 : hash_key(lo, hi);
+}
+
+__device__ __forceinline__ uint64_t load_w(const void *p, int w, uint64_t i) {
+    switch (w) {
+    case 8: return ((const uint64_t *)p)[i];
+    case 4: return ((const uint32_t *)p)[i];
+    case 2: return ((const uint16_t *)p)[i];
+    default: return ((const uint8_t *)p)[i];
+    }
+}
+__device__ __forceinline__ void store_w(void *p, int w, uint64_t i, uint64_t x) {
+    switch (w) {
+    case 8: ((uint64_t *)p)[i] = x; break;
+    case 4: ((uint32_t *)p)[i] = (uint32_t)x; break;
+    case 2: ((uint16_t *)p)[i] = (uint16_t)x; break;
+    default: ((uint8_t *)p)[i] = (uint8_t)x; break;
+    }
+}
+
+// Row accessor over a spilled record: value columns only.
+struct SpillRow {
+    const HLaunch *__restrict__ L;
+    const Spill *S;
+    uint64_t i;
+    uint16_t flags;
+    __device__ __forceinline__ bool valid(int c) const { return (flags >> L->col_val[c]) & 1; }
+    __device__ __forceinline__ uint64_t bits(int c) const {
+        const int j = L->col_val[c];
+        return load_w(S->val[j], L->vw[j], i);
+    }
+    __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(L->q.cols[c].tag, bits(c)); }
+    __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
+};
+
+// ---------------------------------------------------------------------------
+// sample
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void hs_sample_gather(const HLaunch *__restrict__ L) {
+    const QueryDesc *__restrict__ q = &L->q;
+    const uint64_t n = q->n_rows;
+    const uint32_t ns = L->n_sample;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool take = false;
+    Key k{0, 0, 0};
+    if (i < ns && n > 0) {
+        // row = floor(i * n / ns) without 128-bit division (i, n % ns < 2^32)
+        const uint64_t r = (uint64_t)i * (n / ns) + ((uint64_t)i * (n % ns)) / ns;
+        MemRow row{q, r};
+        if (eval_pred(q, row)) {
+            k = load_key(q, row);
+            take = true;
+        }
+    }
+    // wave-aggregated append: one atomic per wave
+    const unsigned long long m = __ballot(take);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    const int leader = m ? __ffsll((long long)m) - 1 : 0;
+    if (lane == leader && m) base = atomicAdd(&L->stats[ST_SAMPLE_SEL], (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (take) {
+        const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+        L->samp_lo[pos] = k.lo;
+        L->samp_hi[pos] = k.hi;
+        L->samp_sp[pos] = (uint8_t)k.special;
+    }
+}
+
+// One block: count sampled keys, pick the hot set, lay it out, reset the
+// global hot state and the output cursor.
+__global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restrict__ L) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    const QueryDesc *__restrict__ q = &L->q;
+    const bool wide = L->wide;
+    const uint32_t kSampleSlots = L->sample_slots;  // power of two
+    uint64_t *s_lo = (uint64_t *)smem;
+    uint64_t *s_hi = s_lo + kSampleSlots;                         // wide only (aliased otherwise)
+    uint32_t *s_cnt = (uint32_t *)(wide ? (s_hi + kSampleSlots) : (s_lo + kSampleSlots));
+    uint32_t *hist = s_cnt + kSampleSlots;                        // 1024 count buckets
+    uint64_t *h_lo = (uint64_t *)(hist + 1024);                   // hot layout
+    uint64_t *h_hi = h_lo + L->hot_slots;
+    __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t CL = L->hot_slots;
+    for (uint32_t i = tid; i < kSampleSlots; i += bs) { s_lo[i] = kEmpty; if (wide) s_hi[i] = kEmpty; s_cnt[i] = 0; }
+    for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
+    for (uint32_t i = tid; i < CL; i += bs) { h_lo[i] = kEmpty; if (wide) h_hi[i] = kEmpty; }
+    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; }
+    __syncthreads();
+    const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
+    uint32_t drops = 0, nullkeys = 0;
+    for (uint32_t i = tid; i < nsel; i += bs) {
+        const uint64_t lo = L->samp_lo[i], hi = wide ? L->samp_hi[i] : 0;
+        if (L->samp_sp[i]) { ++nullkeys; continue; }  // side keys are never hot
+        uint32_t s = (uint32_t)hash_key(lo, wide ? hi : 0) & (kSampleSlots - 1);
+        bool done = false;
+        for (int p = 0; p < 64 && !done; ++p) {
+            // claim by the low word; for wide keys the high word is compared
+            // after the claim (two keys sharing a low word take two slots)
+            uint64_t cur = __hip_atomic_load(&s_lo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (cur == kEmpty) {
+                unsigned long long old = atomicCAS((unsigned long long *)&s_lo[s], (unsigned long long)kEmpty,
+                                                   (unsigned long long)lo);
+                if (old == kEmpty) {
+                    if (wide) __hip_atomic_store(&s_hi[s], hi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    atomicAdd(&s_cnt[s], 1u);
+                    done = true;
+                    break;
+                }
+                cur = old;
+            }
+            if (cur == lo) {
+                if (!wide) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
+                // the claimer stores the high word right after its CAS; the
+                // wait is bounded (a sample key that gives up is only dropped)
+                uint64_t ch = kEmpty;
+                for (int spin = 0; spin < 4096; ++spin) {
+                    ch = __hip_atomic_load(&s_hi[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    if (ch != kEmpty || hi == kEmpty) break;
+                }
+                if (ch == hi) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
+            }
+            s = (s + 1) & (kSampleSlots - 1);
+        }
+        if (!done) ++drops;
+    }
+    if (drops) atomicAdd(&s_drop, drops);
+    if (nullkeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], nullkeys);
+    __syncthreads();
+    // sketch + count histogram
+    uint32_t f1 = 0, f2 = 0, dist = 0;
+    for (uint32_t i = tid; i < kSampleSlots; i += bs) {
+        uint32_t c = s_cnt[i];
+        if (!c) continue;
+        ++dist;
+        if (c == 1) ++f1;
+        if (c == 2) ++f2;
+        atomicAdd(&hist[min(c, 1023u)], 1u);
+    }
+    atomicAdd(&s_f1, f1);
+    atomicAdd(&s_f2, f2);
+    atomicAdd(&s_distinct, dist);
+    __syncthreads();
+    if (tid == 0) {
+        // smallest count threshold T >= 2 such that #keys with count > T fits
+        uint32_t acc = 0, T = 1024;
+        const uint32_t kmax = L->hot_max;
+        for (int c = 1023; c >= 2; --c) {
+            if (acc + hist[c] > kmax) { T = (uint32_t)c; break; }
+            acc += hist[c];
+            T = (uint32_t)c - 1;
+        }
+        if (T < 2) T = 1;  // every key with count >= 2 fits
+        s_thresh = T;
+        s_take_eq = kmax > acc ? kmax - acc : 0;  // keys with count == T still allowed
+        L->stats[ST_SAMPLE_DISTINCT] = s_distinct;
+        L->stats[ST_SAMPLE_F1] = s_f1;
+        L->stats[ST_SAMPLE_F2] = s_f2;
+        L->stats[ST_SAMPLE_DROPPED] = s_drop;
+    }
+    __syncthreads();
+    const uint32_t T = s_thresh;
+    uint32_t hot_rows = 0;
+    for (uint32_t i = tid; i < kSampleSlots && CL; i += bs) {
+        const uint32_t c = min(s_cnt[i], 1023u);
+        bool pick = c > T && c >= 2;
+        if (!pick && c == T && c >= 2) pick = atomicAdd(&s_take_eq, (uint32_t)-1) - 1 < 0x80000000u;
+        if (!pick) continue;
+        const uint64_t lo = s_lo[i], hi = wide ? s_hi[i] : 0;
+        uint32_t s = (uint32_t)(hash_key(lo, hi) >> 40) & (CL - 1);
+        for (uint32_t p = 0; p < CL; ++p) {
+            unsigned long long old = atomicCAS((unsigned long long *)&h_lo[s], (unsigned long long)kEmpty,
+                                               (unsigned long long)lo);
+            if (old == kEmpty) {
+                if (wide) h_hi[s] = hi;
+                atomicAdd(&s_hot, 1u);
+                hot_rows += s_cnt[i];
+                break;
+            }
+            s = (s + 1) & (CL - 1);
+        }
+    }
+    if (hot_rows) atomicAdd(&L->stats[ST_HOT_SAMPLE_ROWS], hot_rows);
+    __syncthreads();
+    for (uint32_t i = tid; i < CL; i += bs) {
+        L->hot_lo[i] = h_lo[i];
+        if (wide) L->hot_hi[i] = h_hi[i];
+        const Layout &lay = q->layout;
+        for (int a = 0; a < lay.n_arrs; ++a) {
+            const int op = lay.init[a];
+            if (mo_size(op) == 4) ((uint32_t *)L->hot_arr[a])[i] = (uint32_t)mo_identity(op);
+            else ((uint64_t *)L->hot_arr[a])[i] = mo_identity(op);
+        }
+    }
+    if (tid == 0) {
+        L->stats[ST_HOT] = s_hot;
+        L->out_cursor[0] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scan
+// ---------------------------------------------------------------------------
+struct ScanCtx {
+    uint64_t *lkeys, *lkeys_hi;
+    uint32_t CL;
+    LSink ls;
+    uint32_t *s_cursor;
+    uint64_t region;  // spill index of this block's region
+};
+
+// One row of the scan: returns true when the row is a tail row (key in k).
+template <class Row>
+__device__ __forceinline__ bool scan_row(const HLaunch *__restrict__ L, const QueryDesc *__restrict__ q,
+                                         const Row &row, bool live, ScanCtx &c, uint32_t &sel, Key &k) {
+    if (!live || !eval_pred(q, row)) return false;
+    ++sel;
+    k = load_key(q, row);
+    if (k.special || c.CL == 0) return true;
+    const uint64_t h = hash_key(k.lo, k.hi);
+    uint32_t s = (uint32_t)(h >> 40) & (c.CL - 1);
+    for (int p = 0; p < kProbe; ++p) {
+        const uint64_t cur = c.lkeys[s];
+        if (cur == kEmpty) return true;
+        if (cur == k.lo && (!L->wide || c.lkeys_hi[s] == k.hi)) {
+            add_row(c.ls, q, s, row);
+            return false;
+        }
+        s = (s + 1) & (c.CL - 1);
+    }
+    return true;
+}
+
+// Wave-contiguous append of the tail rows of one row step.
+template <class Row>
+__device__ __forceinline__ void append_tail(const HLaunch *__restrict__ L, const QueryDesc *__restrict__ q,
+                                            const Row &row, bool is_tail, const Key &k, ScanCtx &c) {
+    const unsigned long long m = __ballot(is_tail);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(c.s_cursor, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    if (!is_tail) return;
+    const uint64_t pos = c.region + base + (uint32_t)__popcll(m & ((1ull << lane) - 1));
+    const Spill &S = L->spill;
+    uint16_t fl = k.special == 1 ? kFlagAllOnes : (k.special == 2 ? kFlagNull : 0);
+    for (int j = 0; j < q->n_vals; ++j) {
+        const int col = q->vals[j].col;
+        if (row.valid(col)) fl |= (uint16_t)(1u << j);
+        store_w(S.val[j], L->vw[j], pos, row.bits(col));
+    }
+    S.klo[pos] = k.lo;
+    if (L->wide) S.khi[pos] = k.hi;
+    S.flags[pos] = fl;
+}
+
+template <bool CACHED>
+__global__ __launch_bounds__(1024) void hs_scan(const HLaunch *__restrict__ L) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t s_cursor, s_sel;
+    const QueryDesc *__restrict__ q = &L->q;
+    const uint32_t CL = L->hot_slots;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const bool wide = L->wide;
+    uint64_t *lkeys = (uint64_t *)smem;
+    uint64_t *lkeys_hi = lkeys + CL;
+    uint8_t *state = (uint8_t *)(wide ? lkeys_hi + CL : lkeys_hi);
+    ScanCtx c{lkeys, lkeys_hi, CL, LSink{state, L->lds1_off, CL}, &s_cursor,
+              (uint64_t)blockIdx.x * L->chunk};
+    if (tid == 0) { s_cursor = 0; s_sel = 0; }
+    for (uint32_t i = tid; i < CL; i += bs) {
+        lkeys[i] = L->hot_lo[i];
+        if (wide) lkeys_hi[i] = L->hot_hi[i];
+    }
+    for (int a = 0; a < q->layout.n_arrs; ++a) {
+        const int op = q->layout.init[a];
+        if (mo_size(op) == 4) {
+            const uint32_t id = (uint32_t)mo_identity(op);
+            for (uint32_t i = tid; i < CL; i += bs) *c.ls.p<uint32_t>(a, i) = id;
+        } else {
+            const uint64_t id = mo_identity(op);
+            for (uint32_t i = tid; i < CL; i += bs) *c.ls.p<uint64_t>(a, i) = id;
+        }
+    }
+    __syncthreads();
+    const uint64_t n = q->n_rows;
+    const uint64_t r_begin = (uint64_t)blockIdx.x * L->chunk;
+    const uint64_t r_end = min(n, r_begin + L->chunk);
+    const int ncols = q->n_cols;
+    uint32_t sel = 0;
+    if constexpr (CACHED) {
+        auto fill = [&](CacheRow &row, uint64_t r) __attribute__((always_inline)) {
+            row.q = q;
+            row.vm = 0;
+            row.v0 = row.v1 = row.v2 = row.v3 = 0;
+            if (r < r_end) {
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    if (cc < ncols) {
+                        const DevCol &col = q->cols[cc];
+                        row.set(cc, load_raw(col, r));
+                        row.vm |= (uint32_t)dvalid(col.valid, r) << cc;
+                    }
+                }
+            }
+        };
+        // every lane runs the same number of steps so the ballots in
+        // append_tail always see whole waves
+        const uint64_t span = r_end > r_begin ? r_end - r_begin : 0;
+        const uint64_t steps = (span + 4ull * bs - 1) / (4ull * bs);
+        for (uint64_t it = 0; it < steps; ++it) {
+            const uint64_t r0 = r_begin + it * 4ull * bs + tid;
+            CacheRow a0, a1, a2, a3;
+            fill(a0, r0);
+            fill(a1, r0 + bs);
+            fill(a2, r0 + 2ull * bs);
+            fill(a3, r0 + 3ull * bs);
+            Key k;
+            bool t;
+            t = scan_row(L, q, a0, r0 < r_end, c, sel, k);
+            append_tail(L, q, a0, t, k, c);
+            t = scan_row(L, q, a1, r0 + bs < r_end, c, sel, k);
+            append_tail(L, q, a1, t, k, c);
+            t = scan_row(L, q, a2, r0 + 2ull * bs < r_end, c, sel, k);
+            append_tail(L, q, a2, t, k, c);
+            t = scan_row(L, q, a3, r0 + 3ull * bs < r_end, c, sel, k);
+            append_tail(L, q, a3, t, k, c);
+        }
+    } else {
+        // Every lane of a wave iterates the same number of times, so the
+        // ballots in append_tail see whole waves.
+        const uint64_t span = r_end > r_begin ? r_end - r_begin : 0;
+        const uint64_t iters = (span + bs - 1) / bs;
+        for (uint64_t it = 0; it < iters; ++it) {
+            const uint64_t r = r_begin + it * bs + tid;
+            const bool live = r < r_end;
+            MemRow row{q, live ? r : r_begin};
+            Key k;
+            const bool t = scan_row(L, q, row, live, c, sel, k);
+            append_tail(L, q, row, t, k, c);
+        }
+    }
+    atomicAdd(&s_sel, sel);
+    __syncthreads();
+    if (tid == 0) {
+        L->tail_n[blockIdx.x] = s_cursor;
+        atomicAdd(&L->stats[ST_SELECTED], s_sel);
+        atomicAdd(&L->stats[ST_TAIL], s_cursor);
+    }
+    // hot partials: block slot s -> global slot s (coalesced atomics)
+    GSink g{L->hot_arr, CL};
+    const int a_len = q->layout.a_len;
+    for (uint32_t i = tid; i < CL; i += bs) {
+        if (*c.ls.p<uint32_t>(a_len, i) == 0) continue;
+        merge_lds_entry(g, i, c.ls, i, q->layout, q->layout.init);
+    }
+}
+
+// Prefix of tail counts per region and of first-pass tiles per region.
+__global__ __launch_bounds__(1024) void hs_regions(const HLaunch *__restrict__ L) {
+    __shared__ uint32_t s_a[1024], s_b[1024];
+    __shared__ uint32_t carry_a, carry_b;
+    if (threadIdx.x == 0) { carry_a = 0; carry_b = 0; }
+    __syncthreads();
+    for (uint32_t b0 = 0; b0 < L->nb; b0 += 1024) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t x = b < L->nb ? L->tail_n[b] : 0;
+        const uint32_t y = (x + kPartTile - 1) / kPartTile;
+        s_a[threadIdx.x] = x;
+        s_b[threadIdx.x] = y;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t xa = threadIdx.x >= (uint32_t)off ? s_a[threadIdx.x - off] : 0;
+            uint32_t xb = threadIdx.x >= (uint32_t)off ? s_b[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_a[threadIdx.x] += xa;
+            s_b[threadIdx.x] += xb;
+            __syncthreads();
+        }
+        if (b < L->nb) {
+            L->region_off[b] = carry_a + s_a[threadIdx.x] - x;
+            L->tile_off[b] = carry_b + s_b[threadIdx.x] - y;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) { carry_a += s_a[1023]; carry_b += s_b[1023]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        L->region_off[L->nb] = carry_a;
+        L->tile_off[L->nb] = carry_b;
+        L->stats[ST_TILES0] = carry_b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// radix partition passes
+// ---------------------------------------------------------------------------
+// Tile t of a pass.  first: tiles live inside the scan regions of `spill`
+// (tile_off maps tiles to regions); otherwise tiles are contiguous in src.
+struct TileSpan {
+    uint64_t src_begin;   // index of the tile's first row in the source arrays
+    uint32_t rows;        // rows in the tile
+    uint64_t order_begin; // rank of the first row in the pass's input order
+};
+
+__device__ __forceinline__ bool tile_span(const HLaunch *__restrict__ L, bool first, uint32_t t, uint32_t ntiles,
+                                          uint64_t total, TileSpan &ts) {
+    if (t >= ntiles) return false;
+    if (!first) {
+        ts.src_begin = (uint64_t)t * kPartTile;
+        ts.order_begin = ts.src_begin;
+        ts.rows = (uint32_t)min<uint64_t>(kPartTile, total - ts.src_begin);
+        return true;
+    }
+    // binary search: last region b with tile_off[b] <= t
+    uint32_t lo = 0, hi = L->nb;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (L->tile_off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t j = t - L->tile_off[lo];
+    const uint32_t n_b = L->tail_n[lo];
+    ts.src_begin = (uint64_t)lo * L->chunk + (uint64_t)j * kPartTile;
+    ts.order_begin = (uint64_t)L->region_off[lo] + (uint64_t)j * kPartTile;
+    ts.rows = min<uint32_t>(kPartTile, n_b - j * kPartTile);
+    return true;
+}
+
+__device__ __forceinline__ uint32_t pass_digit(uint64_t lo, uint64_t hi, uint16_t fl, int shift) {
+    return (uint32_t)(spill_hash(lo, hi, fl) >> shift) & (kRadix - 1);
+}
+
+__global__ __launch_bounds__(kPartThreads) void rp_hist(const HLaunch *__restrict__ L, int first, const Spill *src_p,
+                                                       int shift) {
+    __shared__ uint32_t hist[kRadix];
+    const Spill &src = *src_p;
+    const uint32_t ntiles = first ? L->tile_off[L->nb] : (uint32_t)((L->region_off[L->nb] + kPartTile - 1) / kPartTile);
+    const uint64_t total = L->region_off[L->nb];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (threadIdx.x < kRadix) hist[threadIdx.x] = 0;
+        __syncthreads();
+        TileSpan ts;
+        tile_span(L, first, t, ntiles, total, ts);
+        for (uint32_t k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + threadIdx.x;
+            if (i < ts.rows) {
+                const uint64_t s = ts.src_begin + i;
+                const uint64_t hi = L->wide ? src.khi[s] : 0;
+                atomicAdd(&hist[pass_digit(src.klo[s], hi, src.flags[s], shift)], 1u);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kRadix) L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t] = hist[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// One block per digit: exclusive scan of the digit's tile counts in place;
+// digit totals go to digit_base (then exclusive-scanned by block 0).
+__global__ __launch_bounds__(1024) void rp_scan(const HLaunch *__restrict__ L, int first) {
+    __shared__ uint32_t s[1024];
+    __shared__ uint32_t carry;
+    __shared__ uint32_t totals[kRadix];
+    const uint32_t ntiles = first ? L->tile_off[L->nb] : (uint32_t)((L->region_off[L->nb] + kPartTile - 1) / kPartTile);
+    const uint32_t d = blockIdx.x;
+    uint32_t *row = L->tile_hist + (uint64_t)d * L->max_tiles;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t t0 = 0; t0 < ntiles; t0 += 1024) {
+        const uint32_t t = t0 + threadIdx.x;
+        const uint32_t x = t < ntiles ? row[t] : 0;
+        s[threadIdx.x] = x;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t y = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
+            __syncthreads();
+            s[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (t < ntiles) row[t] = carry + s[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) L->digit_base[d] = carry;  // digit total for now
+    (void)totals;
+}
+
+// Stable scatter of one pass.  Rows are ranked in input order: within a
+// wave by multisplit ballots, across waves and the tile's four row steps by
+// LDS counters; the tile is staged in LDS in digit order and written out so
+// consecutive threads write consecutive positions of each digit run.
+__global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__restrict__ L, int first,
+                                                          const Spill *src_p, const Spill *dst_p, int shift) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t wcnt[kPartThreads / 64][kRadix];
+    __shared__ uint32_t run[kRadix];       // rows of each digit staged so far
+    __shared__ uint32_t dstart[kRadix];    // LDS start of each digit run
+    __shared__ uint32_t gbase[kRadix];     // global start of each digit run for this tile
+    __shared__ uint32_t dtot[kRadix];
+    const Spill &src = *src_p;
+    const Spill &dst = *dst_p;
+    const QueryDesc *__restrict__ q = &L->q;
+    const int nv = q->n_vals;
+    const bool wide = L->wide;
+    const uint32_t ntiles = first ? L->tile_off[L->nb] : (uint32_t)((L->region_off[L->nb] + kPartTile - 1) / kPartTile);
+    const uint64_t total = L->region_off[L->nb];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1;
+    // LDS staging: keys, [hi], flags, digits, then value columns
+    uint64_t *st_lo = (uint64_t *)smem;
+    uint64_t *st_hi = st_lo + kPartTile;
+    uint8_t *p = (uint8_t *)(wide ? st_hi + kPartTile : st_hi);
+    uint16_t *st_fl = (uint16_t *)p;
+    uint8_t *st_d = (uint8_t *)(st_fl + kPartTile);
+    uint8_t *st_v = st_d + kPartTile;
+    // digit totals over all tiles -> exclusive digit bases
+    if (threadIdx.x < kRadix) dtot[threadIdx.x] = L->digit_base[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int d = 0; d < kRadix; ++d) { uint32_t x = dtot[d]; dtot[d] = acc; acc += x; }
+    }
+    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        TileSpan ts;
+        tile_span(L, first, t, ntiles, total, ts);
+        if (threadIdx.x < kRadix) run[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t my_d[kPartRows], my_pos[kPartRows];
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + threadIdx.x;
+            const bool live = i < ts.rows;
+            uint32_t d = 0;
+            if (live) {
+                const uint64_t s = ts.src_begin + i;
+                d = pass_digit(src.klo[s], wide ? src.khi[s] : 0, src.flags[s], shift);
+            }
+            // multisplit: lanes with the same digit
+            unsigned long long peers = __ballot(live);
+#pragma unroll
+            for (int b = 0; b < kRadixBits; ++b) {
+                const unsigned long long m = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? m : ~m;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            if (threadIdx.x < (kPartThreads / 64) * kRadix) wcnt[threadIdx.x / kRadix][threadIdx.x % kRadix] = 0;
+            __syncthreads();
+            if (live && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            // per digit: exclusive scan over waves, then advance the run
+            if (threadIdx.x < kRadix) {
+                const uint32_t dd = threadIdx.x;
+                uint32_t acc = run[dd];
+                for (int w = 0; w < kPartThreads / 64; ++w) {
+                    const uint32_t x = wcnt[w][dd];
+                    wcnt[w][dd] = acc;
+                    acc += x;
+                }
+                run[dd] = acc;
+            }
+            __syncthreads();
+            my_d[k] = d;
+            my_pos[k] = live ? wcnt[wave][d] + rank : 0xFFFFFFFFu;
+            __syncthreads();
+        }
+        // LDS start of each digit run within the tile
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int d = 0; d < kRadix; ++d) { dstart[d] = acc; acc += run[d]; }
+        }
+        if (threadIdx.x < kRadix)
+            gbase[threadIdx.x] = dtot[threadIdx.x] + L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t];
+        __syncthreads();
+        // stage in digit order
+        for (int k = 0; k < kPartRows; ++k) {
+            if (my_pos[k] == 0xFFFFFFFFu) continue;
+            const uint32_t i = k * kPartThreads + threadIdx.x;
+            const uint64_t s = ts.src_begin + i;
+            const uint32_t sp = dstart[my_d[k]] + my_pos[k];
+            st_lo[sp] = src.klo[s];
+            if (wide) st_hi[sp] = src.khi[s];
+            st_fl[sp] = src.flags[s];
+            st_d[sp] = (uint8_t)my_d[k];
+            uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                store_w(vp, w, sp, load_w(src.val[j], w, s));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+        // write runs: consecutive staged rows -> consecutive positions
+        for (uint32_t sp = threadIdx.x; sp < ts.rows; sp += kPartThreads) {
+            const uint32_t d = st_d[sp];
+            const uint64_t g = (uint64_t)gbase[d] + (sp - dstart[d]);
+            dst.klo[g] = st_lo[sp];
+            if (wide) dst.khi[g] = st_hi[sp];
+            dst.flags[g] = st_fl[sp];
+            const uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                store_w(dst.val[j], w, g, load_w(vp, w, sp));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-partition aggregation
+// ---------------------------------------------------------------------------
+// The passes order rows by the low `sortbits` bits of the hash; a partition
+// is the top pbits of that field, so partitions are contiguous runs.
+__device__ __forceinline__ uint32_t part_of(const HLaunch *__restrict__ L, const Spill &S, bool wide, uint64_t i) {
+    const uint64_t v = spill_hash(S.klo[i], wide ? S.khi[i] : 0, S.flags[i]) & ((1ull << L->sortbits) - 1);
+    return (uint32_t)(v >> (L->sortbits - L->pbits));
+}
+
+// First index in [0, n) whose partition is >= p (the array is ordered by
+// partition after the passes).
+__device__ __forceinline__ uint64_t part_lower_bound(const HLaunch *__restrict__ L, const Spill &S, bool wide,
+                                                     uint64_t n, uint32_t p) {
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (part_of(L, S, wide, mid) < p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ L, const Spill *src_p) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint64_t s_range[2];
+    __shared__ uint32_t s_count, s_base;
+    const Spill &S = *src_p;
+    const QueryDesc *__restrict__ q = &L->q;
+    const bool wide = L->wide;
+    const uint32_t CL = L->tail_slots;
+    const uint32_t nls = CL + 2;
+    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const uint32_t pmask = (1u << L->pbits) - 1;
+    const uint64_t n = L->region_off[L->nb];
+    uint64_t *lk = (uint64_t *)smem;
+    uint64_t *lh = lk + nls;
+    uint32_t *lst = (uint32_t *)(lh + nls);  // wide: 0 free, 1 writing, 2 published
+    uint8_t *state = wide ? (uint8_t *)(lst + nls) : (uint8_t *)lh;
+    LSink ls{state, L->lds3_off, nls};
+    GSink gs{L->t.arr, L->t.mask + 3};
+    for (uint32_t p = blockIdx.x; p <= pmask; p += gridDim.x) {
+        if (tid < 2) s_range[tid] = part_lower_bound(L, S, wide, n, p + tid);
+        if (tid == 0) s_count = 0;
+        for (uint32_t i = tid; i < nls; i += bs) {
+            lk[i] = kEmpty;
+            if (wide) { lh[i] = kEmpty; lst[i] = 0; }
+        }
+        for (int a = 0; a < q->layout.n_arrs; ++a) {
+            const int op = q->layout.init[a];
+            if (mo_size(op) == 4) {
+                const uint32_t id = (uint32_t)mo_identity(op);
+                for (uint32_t i = tid; i < nls; i += bs) *ls.p<uint32_t>(a, i) = id;
+            } else {
+                const uint64_t id = mo_identity(op);
+                for (uint32_t i = tid; i < nls; i += bs) *ls.p<uint64_t>(a, i) = id;
+            }
+        }
+        __syncthreads();
+        const uint64_t b = s_range[0], e = p == pmask ? n : s_range[1];
+        for (uint64_t i = b + tid; i < e; i += bs) {
+            const uint16_t fl = S.flags[i];
+            SpillRow row{L, &S, i, fl};
+            Key k{S.klo[i], wide ? S.khi[i] : 0, (fl & kFlagAllOnes) ? 1 : ((fl & kFlagNull) ? 2 : 0)};
+            int32_t slot = -1;
+            uint64_t h = 0;
+            if (k.special) {
+                slot = (int32_t)(CL + k.special - 1);
+            } else {
+                h = hash_key(k.lo, k.hi);
+                uint32_t s = (uint32_t)(h >> 20) & (CL - 1);
+                for (int pr = 0; pr < kProbe && slot < 0; ++pr) {
+                    if (!wide) {
+                        uint64_t cur = __hip_atomic_load(&lk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (cur == kEmpty) {
+                            unsigned long long old = atomicCAS((unsigned long long *)&lk[s],
+                                                               (unsigned long long)kEmpty, (unsigned long long)k.lo);
+                            if (old == kEmpty || old == k.lo) { slot = (int32_t)s; break; }
+                        } else if (cur == k.lo) {
+                            slot = (int32_t)s;
+                            break;
+                        }
+                    } else {
+                        // A claimer publishes within its own loop iteration,
+                        // so the wait ends; the bound turns a broken
+                        // invariant into a reported error instead of a hang.
+                        for (uint32_t spin = 0;; ++spin) {
+                            if (spin > (1u << 22)) { atomicOr(&L->stats[ST_ERROR], 1u); break; }
+                            uint32_t st = __hip_atomic_load(&lst[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (st == 2u) {
+                                if (lk[s] == k.lo && lh[s] == k.hi) slot = (int32_t)s;
+                                break;
+                            }
+                            if (st == 0u) {
+                                if (atomicCAS(&lst[s], 0u, 1u) == 0u) {
+                                    lk[s] = k.lo;
+                                    lh[s] = k.hi;
+                                    __hip_atomic_store(&lst[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                    slot = (int32_t)s;
+                                    break;
+                                }
+                            }
+                            // claimed by another lane: re-read
+                        }
+                        if (slot >= 0) break;
+                    }
+                    s = (s + 1) & (CL - 1);
+                }
+            }
+            if (slot >= 0) {
+                add_row(ls, q, (uint32_t)slot, row);
+            } else {
+                const int64_t g = wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
+                if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+            }
+        }
+        __syncthreads();
+        // emit: compact the used slots of this partition into the outputs
+        LReader rd{state, L->lds3_off};
+        const int a_len = q->layout.a_len;
+        const uint32_t iters = (nls + bs - 1) / bs;
+        uint32_t my = 0;
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t i = it * bs + tid;
+            if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) ++my;
+        }
+        const uint32_t my_off = atomicAdd(&s_count, my);
+        __syncthreads();
+        if (tid == 0) s_base = s_count ? atomicAdd(L->out_cursor, s_count) : 0;
+        __syncthreads();
+        uint32_t o = s_base + my_off;
+        for (uint32_t it = 0; it < iters; ++it) {
+            const uint32_t i = it * bs + tid;
+            if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) {
+                const int special = i >= CL ? (int)(i - CL) + 1 : 0;
+                emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], (special || !wide) ? 0 : lh[i],
+                           special, o);
+                ++o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Hot groups: slots of the global hot state that received rows.
+__global__ __launch_bounds__(256) void hs_hot_emit(const HLaunch *__restrict__ L) {
+    __shared__ uint32_t s_count, s_base;
+    const QueryDesc *__restrict__ q = &L->q;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t CL = L->hot_slots;
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    GReader rd{L->hot_arr};
+    const bool used = i < CL && rd.get<uint32_t>(q->layout.a_len, i) != 0;
+    uint32_t off = 0;
+    if (used) off = atomicAdd(&s_count, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_count ? atomicAdd(L->out_cursor, s_count) : 0;
+    __syncthreads();
+    if (used) emit_group(q, L->out, L->n_out, rd, i, L->hot_lo[i], L->wide ? L->hot_hi[i] : 0, 0, s_base + off);
+}
+
+// Groups of the fallback table, written after the cursor's groups; resets
+// the slots it reads.
+__global__ __launch_bounds__(256) void hs_fallback_emit(const HLaunch *__restrict__ L) {
+    const QueryDesc *__restrict__ q = &L->q;
+    const GTable &t = L->t;
+    const Layout &lay = q->layout;
+    GReader rd{t.arr};
+    uint64_t ng = *(volatile uint32_t *)&t.ctrl[0];
+    if (ng > t.list_cap) ng = t.list_cap;
+    const uint64_t base = *(volatile uint32_t *)L->out_cursor;
+    const uint64_t cap = t.mask + 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t slot = t.list[i];
+        int special = slot == cap ? 1 : (slot == cap + 1 ? 2 : 0);
+        const uint64_t lo = special ? 0 : t.klo[slot];
+        const uint64_t hi = (special || !L->wide) ? 0 : t.khi[slot];
+        emit_group(q, L->out, L->n_out, rd, slot, lo, hi, special, base + i);
+        t.klo[slot] = kEmpty;
+        if (L->wide) { t.khi[slot] = kEmpty; t.kstate[slot] = 0; }
+        for (int a = 0; a < lay.n_arrs; ++a) {
+            const int op = lay.init[a];
+            if (mo_size(op) == 4) ((uint32_t *)t.arr[a])[slot] = (uint32_t)mo_identity(op);
+            else ((uint64_t *)t.arr[a])[slot] = mo_identity(op);
+        }
+    }
+}
+
+// Validity bytes -> bitmaps, Bool key bytes -> bit-packed data.  One thread
+// per output byte; the group count is the cursor plus the fallback groups.
+__global__ void hs_pack(const HLaunch *__restrict__ L) {
+    uint64_t fb = *(volatile uint32_t *)&L->t.ctrl[0];
+    if (fb > L->t.list_cap) fb = L->t.list_cap;
+    const uint64_t ng = (uint64_t)*(volatile uint32_t *)L->out_cursor + fb;
+    const uint64_t nbytes = (ng + 7) / 8;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes;
+         i += (uint64_t)gridDim.x * blockDim.x)
+    for (int o = 0; o < L->n_out; ++o) {
+        const OutCol &oc = L->out[o];
+        if (oc.vbytes && oc.valid) {
+            uint8_t b = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint64_t r = i * 8 + k;
+                if (r < ng && oc.vbytes[r]) b |= (uint8_t)(1u << k);
+            }
+            ((uint8_t *)oc.valid)[i] = b;
+        }
+        if (oc.src == OS_KEY && oc.tag == PMH_BOOL && oc.data_bits) {
+            uint8_t b = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint64_t r = i * 8 + k;
+                if (r < ng && ((const uint8_t *)oc.data)[r]) b |= (uint8_t)(1u << k);
+            }
+            oc.data_bits[i] = b;
+        }
+    }
+}
+
+__global__ void hs_clear_ctrl(const HLaunch *__restrict__ L) {
+    if (threadIdx.x < 8) L->t.ctrl[threadIdx.x] = 0;
+}
+
+
+__global__ void hs_reset(const HLaunch *__restrict__ L) {
+    if (threadIdx.x < ST_N) L->stats[threadIdx.x] = 0;
+    if (threadIdx.x == 0) L->out_cursor[0] = 0;
+}
+
+__global__ void hs_init_table(const HLaunch *__restrict__ L, uint64_t nslots) {
+    const GTable &t = L->t;
+    const Layout &lay = L->q.layout;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += stride) {
+        t.klo[i] = kEmpty;
+        if (L->wide) { t.khi[i] = kEmpty; t.kstate[i] = 0; }
+        for (int a = 0; a < lay.n_arrs; ++a) {
+            const int op = lay.init[a];
+            if (mo_size(op) == 4) ((uint32_t *)t.arr[a])[i] = (uint32_t)mo_identity(op);
+            else ((uint64_t *)t.arr[a])[i] = mo_identity(op);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 8) t.ctrl[threadIdx.x] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// host
+// ---------------------------------------------------------------------------
+namespace {
+
+struct FallbackState {
+    uint64_t cap = 0;
+    std::string sig;
+    bool clean = false;
+};
+FallbackState g_fb;
+
+uint64_t pow2_floor(uint64_t x) {
+    uint64_t p = 1;
+    while (p * 2 <= x) p *= 2;
+    return p;
+}
+uint64_t pow2_ceil(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p *= 2;
+    return p;
+}
+int log2u(uint64_t x) {
+    int b = 0;
+    while ((1ull << b) < x) ++b;
+    return b;
+}
+
+// Carves nslots-entry state arrays from one slab; limb k > 0 of an exact
+// accumulator sits nslots entries after limb k-1.
+void carve_state(const Prep &P, uint8_t *base, size_t &o, uint64_t nslots, void **arr) {
+    const Layout &lay = P.q.layout;
+    for (int a = 0; a < lay.n_arrs; ++a) {
+        if (P.limb_base[a] >= 0 && P.limb_k[a] > 0) {
+            arr[a] = (uint8_t *)arr[P.limb_base[a]] + (size_t)P.limb_k[a] * nslots * 8;
+        } else {
+            const size_t b = P.limb_base[a] >= 0 ? nslots * 8 * kLimbs : nslots * lay.esz[a];
+            arr[a] = base + o;
+            o += (b + 255) & ~size_t(255);
+        }
+    }
+}
+size_t state_bytes(const Prep &P, uint64_t nslots) {
+    const Layout &lay = P.q.layout;
+    size_t o = 0;
+    for (int a = 0; a < lay.n_arrs; ++a) {
+        if (P.limb_base[a] >= 0 && P.limb_k[a] > 0) continue;
+        const size_t b = P.limb_base[a] >= 0 ? nslots * 8 * kLimbs : nslots * lay.esz[a];
+        o += (b + 255) & ~size_t(255);
+    }
+    return o;
+}
+// LDS offsets for nslots entries per array, arrays back to back.
+size_t lds_layout(const Prep &P, uint32_t nslots, uint32_t *off) {
+    const Layout &lay = P.q.layout;
+    size_t o = 0;
+    for (int a = 0; a < lay.n_arrs; ++a) {
+        off[a] = (uint32_t)o;
+        o += (size_t)nslots * lay.esz[a];
+    }
+    return o;
+}
+
+void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, const int32_t *vw, int nv) {
+    S.klo = (uint64_t *)slab(tag + ":klo", cap * 8 + 64);
+    S.khi = wide ? (uint64_t *)slab(tag + ":khi", cap * 8 + 64) : nullptr;
+    for (int j = 0; j < nv; ++j) S.val[j] = slab(tag + ":v" + std::to_string(j), cap * vw[j] + 64);
+    S.flags = (uint16_t *)slab(tag + ":fl", cap * 2 + 64);
+}
+
+}  // namespace
+
+pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flags) {
+    Runtime &R = rt();
+    const QueryDesc &q0 = P.q;
+    const uint64_t n = P.n_rows;
+    const bool wide = P.wide;
+    const int nv = q0.n_vals;
+    const Options &opt = plan.opts;
+    const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
+    const size_t lds_max = std::min<size_t>(R.prop.sharedMemPerBlock ? R.prop.sharedMemPerBlock : 65536, 160 * 1024);
+    const size_t lds_budget = lds_max - 1024;
+
+    auto Lh = std::make_unique<HLaunch>();
+    memset(Lh.get(), 0, sizeof(HLaunch));
+    HLaunch &L = *Lh;
+    L.q = q0;
+    L.wide = wide;
+    L.n_out = P.n_out;
+    size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
+    for (int j = 0; j < nv; ++j) {
+        L.vw[j] = tag_size(q0.vals[j].tag);
+        L.col_val[q0.vals[j].col] = j;
+        rec_bytes += L.vw[j];
+    }
+    // shapes that do not fit the LDS plans go to the global-table engine
+    if ((size_t)kPartTile * rec_bytes > lds_budget) return nullptr;
+    size_t state_per_slot = 0;
+    for (int a = 0; a < q0.layout.n_arrs; ++a) state_per_slot += q0.layout.esz[a];
+    const size_t key_bytes = wide ? 16 : 8;
+
+    // ---- parameters
+    uint32_t nb = (uint32_t)std::min<uint64_t>(cus, std::max<uint64_t>(1, ceil_div(n, 16384)));
+    const uint64_t chunk = std::max<uint64_t>(1, ceil_div(n, nb));
+    nb = (uint32_t)std::max<uint64_t>(1, ceil_div(n, chunk));
+    uint32_t CL = (uint32_t)pow2_floor(lds_budget / (key_bytes + state_per_slot));
+    CL = std::min<uint32_t>(CL, 2048);
+    if (opt.lds_slots >= 0) CL = opt.lds_slots == 0 ? 0 : (uint32_t)std::min<uint64_t>(CL, pow2_floor(opt.lds_slots));
+    if (CL < 16) CL = 0;
+    const uint32_t n_sample = (uint32_t)(opt.sample >= 0 ? std::min<int64_t>(opt.sample, 1 << 20)
+                                                         : (int64_t)std::min<uint64_t>(n, 262144));
+    const uint32_t sample_slots = wide ? 4096 : 8192;
+    uint64_t P_parts = opt.partitions > 0 ? pow2_floor((uint64_t)opt.partitions)
+                                          : pow2_ceil(std::min<uint64_t>(std::max<uint64_t>(n / 100000, 1), 4096));
+    P_parts = std::min<uint64_t>(P_parts, 1u << 15);
+    const int pbits = log2u(P_parts);
+    const int passes = std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
+    const int sortbits = passes * kRadixBits;
+    const size_t tail_slot_bytes = key_bytes + (wide ? 4 : 0) + state_per_slot;
+    uint32_t TS = (uint32_t)std::min<uint64_t>(4096, pow2_floor(lds_budget / tail_slot_bytes));
+    while (TS > 16 && (size_t)(TS + 2) * tail_slot_bytes > lds_budget) TS /= 2;
+    if (opt.tail_slots > 0) TS = (uint32_t)std::min<uint64_t>(TS, pow2_floor(opt.tail_slots));
+    if (TS < 16) return nullptr;
+    const uint32_t max_tiles = (uint32_t)(ceil_div(n, kPartTile) + nb + 1);
+
+    L.n_sample = n_sample;
+    L.hot_slots = CL;
+    L.hot_max = CL / 2;
+    L.sample_slots = sample_slots;
+    L.chunk = chunk;
+    L.nb = nb;
+    L.max_tiles = max_tiles;
+    L.pbits = (uint32_t)pbits;
+    L.sortbits = (uint32_t)sortbits;
+    L.passes = (uint32_t)passes;
+    L.tail_slots = TS;
+
+    // ---- buffers
+    L.samp_lo = (uint64_t *)slab("hs:samp_lo", (uint64_t)n_sample * 8 + 64);
+    L.samp_hi = (uint64_t *)slab("hs:samp_hi", (uint64_t)n_sample * 8 + 64);
+    L.samp_sp = (uint8_t *)slab("hs:samp_sp", (uint64_t)n_sample + 64);
+    L.hot_lo = (uint64_t *)slab("hs:hot_lo", (uint64_t)CL * 8 + 64);
+    L.hot_hi = (uint64_t *)slab("hs:hot_hi", (uint64_t)CL * 8 + 64);
+    {
+        uint8_t *hb = (uint8_t *)slab("hs:hot_state", state_bytes(P, std::max<uint32_t>(CL, 1)) + 4096);
+        size_t o = 0;
+        carve_state(P, hb, o, CL, L.hot_arr);
+    }
+    const size_t lds1 = (size_t)CL * key_bytes + lds_layout(P, CL, L.lds1_off);
+    const size_t lds3 = (size_t)(TS + 2) * (key_bytes + (wide ? 4 : 0)) + lds_layout(P, TS + 2, L.lds3_off);
+    if (lds1 > lds_budget || lds3 > lds_budget) return nullptr;
+    L.tail_n = (uint32_t *)slab("hs:tail_n", (nb + 1) * 4 + 64);
+    L.region_off = (uint32_t *)slab("hs:region_off", (nb + 1) * 4 + 64);
+    L.tile_off = (uint32_t *)slab("hs:tile_off", (nb + 1) * 4 + 64);
+    const uint64_t cap = (uint64_t)nb * chunk;
+    alloc_spill("hs:spill", L.spill, cap, wide, L.vw, nv);
+    alloc_spill("hs:ping", L.ping, cap, wide, L.vw, nv);
+    alloc_spill("hs:pong", L.pong, cap, wide, L.vw, nv);
+    L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)kRadix * max_tiles * 4 + 64);
+    L.digit_base = (uint32_t *)slab("hs:digit_base", kRadix * 4 + 64);
+    L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
+    L.stats = (uint32_t *)slab("hs:stats", ST_N * 4 + 64);
+    // fallback table
+    {
+        GTable &t = L.t;
+        const uint64_t fcap = pow2_ceil(std::min<uint64_t>(std::max<uint64_t>(n / 64, 4096), 1ull << 21));
+        const uint64_t nslots = fcap + 2;
+        t.mask = fcap - 1;
+        t.limit = (uint64_t)(fcap * 0.7);
+        t.list_cap = nslots;
+        size_t bytes = nslots * 8 + (wide ? nslots * 12 : 0) + nslots * 4 + 64 + state_bytes(P, nslots) + 8192;
+        uint8_t *base = (uint8_t *)slab("hs:fb_table", bytes);
+        size_t o = 0;
+        auto carve = [&](size_t b) { uint8_t *p = base + o; o += (b + 255) & ~size_t(255); return p; };
+        t.klo = (uint64_t *)carve(nslots * 8);
+        if (wide) { t.khi = (uint64_t *)carve(nslots * 8); t.kstate = (uint32_t *)carve(nslots * 4); }
+        t.list = (uint32_t *)carve(nslots * 4);
+        t.ctrl = (uint32_t *)carve(64);
+        carve_state(P, base, o, nslots, t.arr);
+        const std::string fsig = P.sig + (wide ? "w" : "n");
+        if (g_fb.cap != fcap || g_fb.sig != fsig) g_fb.clean = false;
+        g_fb.cap = fcap;
+        g_fb.sig = fsig;
+    }
+    // outputs: at most n groups
+    const uint64_t maxg = std::max<uint64_t>(n, 1);
+    for (int oi = 0; oi < P.n_out; ++oi) {
+        OutCol &oc = L.out[oi];
+        oc = P.outs[oi];
+        const size_t esz = std::max(1, tag_size(oc.tag));
+        oc.data = slab("hs:out" + std::to_string(oi), maxg * esz + 64);
+        const bool need_valid = !(oc.src == OS_AGG && (oc.op == AGG_LEN || oc.op == AGG_COUNT));
+        oc.vbytes = need_valid ? (uint8_t *)slab("hs:outvb" + std::to_string(oi), maxg + 64) : nullptr;
+        oc.valid = need_valid ? (uint64_t *)slab("hs:outv" + std::to_string(oi), maxg / 8 + 64) : nullptr;
+        oc.data_bits = (oc.src == OS_KEY && oc.tag == PMH_BOOL)
+                           ? (uint8_t *)slab("hs:outbits" + std::to_string(oi), maxg / 8 + 64)
+                           : nullptr;
+    }
+
+    // ---- launches
+    HLaunch *dL = (HLaunch *)slab("hs:launch", sizeof(HLaunch));
+    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
+    hipEvent_t ev0, ev1;
+    PMH_HIP(hipEventCreate(&ev0));
+    PMH_HIP(hipEventCreate(&ev1));
+    PMH_HIP(hipEventRecord(ev0, R.stream));
+    auto launch_check = [](void) { PMH_HIP(hipGetLastError()); };
+    hipLaunchKernelGGL(hs_reset, dim3(1), dim3(64), 0, R.stream, dL);
+    launch_check();
+    if (!g_fb.clean) {
+        KernelTimer kt("hs_init_table");
+        const uint64_t nslots = L.t.mask + 3;
+        hipLaunchKernelGGL(hs_init_table, dim3((uint32_t)std::min<uint64_t>(ceil_div(nslots, 256), (uint64_t)cus * 16)),
+                           dim3(256), 0, R.stream, dL, nslots);
+        launch_check();
+        g_fb.clean = true;
+    }
+    if (n_sample > 0 && CL > 0) {
+        KernelTimer kt("hs_sample_gather");
+        hipLaunchKernelGGL(hs_sample_gather, dim3((uint32_t)ceil_div(n_sample, 256)), dim3(256), 0, R.stream, dL);
+        launch_check();
+    }
+    {
+        KernelTimer kt("hs_sample_count");
+        const size_t lds = (size_t)sample_slots * (wide ? 20 : 12) + 4096 + (size_t)CL * key_bytes;
+        PMH_HIP(hipFuncSetAttribute((const void *)hs_sample_count, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds));
+        hipLaunchKernelGGL(hs_sample_count, dim3(1), dim3(1024), lds, R.stream, dL);
+        launch_check();
+    }
+    {
+        const bool cached = q0.n_cols <= 4;
+        KernelTimer kt("hs_scan");
+        const size_t l1 = std::max<size_t>(lds1, 16);
+        const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;
+        PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+        if (cached) hipLaunchKernelGGL(hs_scan<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        else hipLaunchKernelGGL(hs_scan<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        launch_check();
+    }
+    hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
+    launch_check();
+    const int pgrid = cus * 2;
+    const size_t stage_lds = (size_t)kPartTile * rec_bytes;
+    PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_lds));
+    const Spill *bufs[3] = {&dL->spill, &dL->ping, &dL->pong};
+    int src = 0, dst = 1;
+    for (int ps = 0; ps < passes; ++ps) {
+        const int shift = ps * kRadixBits;
+        {
+            KernelTimer kt("rp_hist");
+            hipLaunchKernelGGL(rp_hist, dim3(pgrid), dim3(kPartThreads), 0, R.stream, dL, ps == 0 ? 1 : 0, bufs[src],
+                               shift);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scan");
+            hipLaunchKernelGGL(rp_scan, dim3(kRadix), dim3(1024), 0, R.stream, dL, ps == 0 ? 1 : 0);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scatter");
+            hipLaunchKernelGGL(rp_scatter, dim3(pgrid), dim3(kPartThreads), stage_lds, R.stream, dL, ps == 0 ? 1 : 0,
+                               bufs[src], bufs[dst], shift);
+            launch_check();
+        }
+        src = dst;
+        dst = dst == 1 ? 2 : 1;
+    }
+    {
+        KernelTimer kt("hs_part_agg");
+        PMH_HIP(hipFuncSetAttribute((const void *)hs_part_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3));
+        const uint32_t g = (uint32_t)std::min<uint64_t>(P_parts, (uint64_t)cus * 2);
+        hipLaunchKernelGGL(hs_part_agg, dim3(g), dim3(1024), lds3, R.stream, dL, bufs[src]);
+        launch_check();
+    }
+    if (CL) {
+        KernelTimer kt("hs_hot_emit");
+        hipLaunchKernelGGL(hs_hot_emit, dim3((uint32_t)ceil_div(CL, 256)), dim3(256), 0, R.stream, dL);
+        launch_check();
+    }
+    {
+        KernelTimer kt("hs_fallback_emit");
+        hipLaunchKernelGGL(hs_fallback_emit, dim3(cus), dim3(256), 0, R.stream, dL);
+        launch_check();
+    }
+    {
+        KernelTimer kt("hs_pack");
+        hipLaunchKernelGGL(hs_pack, dim3(cus * 4), dim3(256), 0, R.stream, dL);
+        launch_check();
+    }
+    uint32_t host_ctrl[8], host_stats[ST_N], cursor = 0;
+    PMH_HIP(hipMemcpyAsync(host_ctrl, L.t.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(host_stats, L.stats, sizeof(host_stats), hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(&cursor, L.out_cursor, 4, hipMemcpyDeviceToHost, R.stream));
+    hipLaunchKernelGGL(hs_clear_ctrl, dim3(1), dim3(64), 0, R.stream, dL);
+    launch_check();
+    PMH_HIP(hipEventRecord(ev1, R.stream));
+    PMH_HIP(hipEventSynchronize(ev1));
+    float dev_ms = 0.f;
+    (void)hipEventElapsedTime(&dev_ms, ev0, ev1);
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (host_stats[ST_ERROR]) {
+        g_fb.clean = false;
+        fail(PMH_E_COMPUTE, "polars_metal: partitioned group-by detected an inconsistent table state");
+    }
+    if (host_ctrl[1]) {
+        // the fallback table overflowed and is dirty: the caller re-runs the
+        // query on the global-table engine, which resizes
+        g_fb.clean = false;
+        collect_kernel_times();
+        return nullptr;
+    }
+    const uint64_t ng = (uint64_t)cursor + std::min<uint64_t>(host_ctrl[0], L.t.list_cap);
+
+    pmh_result *res = new_result(P.n_out);
+    res->n_rows = ng;
+    res->stats.method = PMH_METHOD_HASH;
+    res->stats.n_passes = 1;
+    res->stats.initial_capacity = L.t.mask + 1;
+    res->stats.final_capacity = L.t.mask + 1;
+    res->stats.n_groups = ng;
+    res->stats.n_selected = host_stats[ST_SELECTED];
+    res->stats.device_ms = dev_ms;
+    res->stats.sample_rows = host_stats[ST_SAMPLE_SEL];
+    res->stats.sample_distinct = host_stats[ST_SAMPLE_DISTINCT];
+    res->stats.sample_f1 = host_stats[ST_SAMPLE_F1];
+    res->stats.sample_f2 = host_stats[ST_SAMPLE_F2];
+    res->stats.hot_keys = host_stats[ST_HOT];
+    res->stats.partitions = (uint32_t)P_parts;
+    res->stats.tail_rows = host_stats[ST_TAIL];
+    const uint64_t vbytes = bitmap_bytes_padded(ng);
+    const uint64_t packed = (ng + 7) / 8;
+    for (int oi = 0; oi < P.n_out; ++oi) {
+        const OutCol &oc = L.out[oi];
+        pmh_out_column &c = res->columns[oi];
+        set_out_name(c, P.names[oi]);
+        c.tag = oc.tag;
+        const bool is_bool = oc.src == OS_KEY && oc.tag == PMH_BOOL;
+        c.data_len = is_bool ? vbytes : ng * tag_size(oc.tag);
+        c.valid_len = vbytes;
+        const void *dsrc = is_bool ? (const void *)oc.data_bits : oc.data;
+        if (flags & PMH_OUT_DEVICE) {
+            c.location = PMH_DEVICE;
+            c.data = (void *)dsrc;
+            c.valid = (uint8_t *)oc.valid;
+        } else {
+            c.location = PMH_HOST;
+            c.data = malloc(c.data_len ? c.data_len : 1);
+            c.valid = (uint8_t *)malloc(vbytes);
+            memset(c.valid, 0, vbytes);
+            if (is_bool) memset(c.data, 0, c.data_len);
+            const uint64_t nb_copy = is_bool ? packed : c.data_len;
+            if (nb_copy) PMH_HIP(hipMemcpyAsync(c.data, dsrc, nb_copy, hipMemcpyDeviceToHost, R.stream));
+            if (oc.valid) {
+                if (packed) PMH_HIP(hipMemcpyAsync(c.valid, oc.valid, packed, hipMemcpyDeviceToHost, R.stream));
+            } else {
+                memset(c.valid, 0xFF, vbytes);  // counts / lens are never null
+            }
+        }
+    }
+    PMH_HIP(hipStreamSynchronize(R.stream));
+    collect_kernel_times();
+    return res;
+}
+
+}  // namespace pmh

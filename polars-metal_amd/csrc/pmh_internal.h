@@ -156,12 +156,15 @@ struct PlanPredNode {  // parsed PredicateAst
 };
 
 struct Options {
-    std::string method = "auto";  // auto | hash | sort
+    std::string method = "auto";  // auto | hash | hash_global | sort
     int64_t capacity_hint = 0;    // 0: plain-method default
     int64_t lds_slots = -1;       // -1: auto, 0: no block-local table
     int64_t block = 0;            // threads per block, 0: auto
     int64_t grid = 0;             // blocks, 0: auto
     int64_t tile = 0;             // rows per block per tile, 0: auto
+    int64_t partitions = 0;       // partitioned engine: tail partitions (power of two), 0: auto
+    int64_t tail_slots = 0;       // partitioned engine: per-partition LDS table slots, 0: auto
+    int64_t sample = -1;          // partitioned engine: sampled rows for the hot set, -1: auto
 };
 
 struct GroupByPlan {

@@ -325,8 +325,8 @@ GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate) {
         for (auto &kv : op->o) {
             const JVal &v = kv.second;
             if (kv.first == "method") {
-                if (v.t != JVal::STR || (v.s != "auto" && v.s != "hash" && v.s != "sort"))
-                    parse_err("options.method must be auto|hash|sort");
+                if (v.t != JVal::STR || (v.s != "auto" && v.s != "hash" && v.s != "hash_global" && v.s != "sort"))
+                    parse_err("options.method must be auto|hash|hash_global|sort");
                 plan.opts.method = v.s;
             } else {
                 if (v.t != JVal::INT) parse_err("options." + kv.first + " must be an integer");
@@ -335,6 +335,9 @@ GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate) {
                 else if (kv.first == "block") plan.opts.block = v.i;
                 else if (kv.first == "grid") plan.opts.grid = v.i;
                 else if (kv.first == "tile") plan.opts.tile = v.i;
+                else if (kv.first == "partitions") plan.opts.partitions = v.i;
+                else if (kv.first == "tail_slots") plan.opts.tail_slots = v.i;
+                else if (kv.first == "sample") plan.opts.sample = v.i;
                 else parse_err("unknown option " + kv.first);
             }
         }

@@ -1,0 +1,311 @@
+// pmh_table.h -- the global open-addressing table and the per-row /
+// per-entry aggregation helpers shared by the group-by kernels.
+#pragma once
+
+#include "pmh_device.h"
+
+namespace pmh {
+
+// ---------------------------------------------------------------------------
+// global hash table
+// ---------------------------------------------------------------------------
+struct GTable {
+    uint64_t *klo;
+    uint64_t *khi;      // KM_LANE128 only
+    uint32_t *kstate;   // KM_LANE128 only: 0 empty, 1 being written, 2 published
+    uint32_t *list;     // claimed slots in claim order
+    uint32_t *ctrl;     // [0] groups claimed [1] overflow [2..3] side-slot claims [4] rows selected
+    uint64_t mask;      // capacity - 1 (capacity is a power of two)
+    uint64_t limit;     // distinct keys allowed before the pass overflows
+    uint64_t list_cap;
+    void *arr[kMaxArrs];
+};
+
+constexpr int kLdsProbe = 16;
+
+// ---------------------------------------------------------------------------
+// global table
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void g_register(const GTable &t, uint64_t slot) {
+    uint32_t idx = atomicAdd(&t.ctrl[0], 1u);
+    if (idx < t.list_cap) t.list[idx] = (uint32_t)slot;
+    if (idx >= t.limit) atomicOr(&t.ctrl[1], 1u);
+}
+
+__device__ __forceinline__ int64_t g_side_slot(const GTable &t, int special) {
+    uint32_t s = (uint32_t)special - 1;
+    uint64_t slot = t.mask + 1 + s;
+    if (__hip_atomic_load(&t.ctrl[2 + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+        atomicCAS(&t.ctrl[2 + s], 0u, 1u) == 0u)
+        g_register(t, slot);
+    return (int64_t)slot;
+}
+
+// 64-bit keys: the key word itself is claimed with one CAS, so no other
+// field needs publishing (the state arrays are initialised before the pass).
+__device__ __forceinline__ int64_t g_find64(const GTable &t, const Key &k, uint64_t h) {
+    if (k.special) return g_side_slot(t, k.special);
+    uint64_t s = h & t.mask;
+    for (uint64_t p = 0; p <= t.mask; ++p) {
+        uint64_t cur = __hip_atomic_load(&t.klo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == k.lo) return (int64_t)s;
+        if (cur == kEmpty) {
+            unsigned long long old = atomicCAS((unsigned long long *)&t.klo[s], (unsigned long long)kEmpty,
+                                               (unsigned long long)k.lo);
+            if (old == kEmpty) { g_register(t, s); return (int64_t)s; }
+            if (old == k.lo) return (int64_t)s;
+        }
+        s = (s + 1) & t.mask;
+        if ((p & 63) == 63 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return -1;
+    }
+    atomicOr(&t.ctrl[1], 1u);
+    return -1;
+}
+
+// 128-bit lanes: claim word 0 -> 1 by CAS, write both key words, publish 2
+// with release; readers acquire before comparing.  The claimer finishes in
+// the same loop iteration, so a wave never waits on one of its own lanes.
+__device__ __forceinline__ int64_t g_find128(const GTable &t, const Key &k, uint64_t h) {
+    if (k.special) return g_side_slot(t, k.special);
+    uint64_t s = h & t.mask;
+    uint64_t p = 0;
+    while (p <= t.mask) {
+        uint32_t st = __hip_atomic_load(&t.kstate[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (st == 2u) {
+            if (t.klo[s] == k.lo && t.khi[s] == k.hi) return (int64_t)s;
+        } else if (st == 0u) {
+            if (atomicCAS(&t.kstate[s], 0u, 1u) == 0u) {
+                t.klo[s] = k.lo;
+                t.khi[s] = k.hi;
+                __hip_atomic_store(&t.kstate[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                g_register(t, s);
+                return (int64_t)s;
+            }
+            continue;  // lost the race: re-read this slot
+        } else {
+            continue;  // being written by another wave: re-read
+        }
+        s = (s + 1) & t.mask;
+        ++p;
+        if ((p & 63) == 0 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            return -1;
+    }
+    atomicOr(&t.ctrl[1], 1u);
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// per-row contribution
+// ---------------------------------------------------------------------------
+template <class S, class Slot, class Row>
+__device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, Slot slot, const Row &row) {
+    s.add32(q->layout.a_len, slot, 1u);
+    const int nv = q->n_vals;
+    for (int j = 0; j < nv; ++j) {
+        const ValSpec &vs = q->vals[j];
+        if (!row.valid(vs.col)) continue;
+        s.add32(vs.a_cnt, slot, 1u);
+        if (vs.kind == VK_F) {
+            double x = row.f64(vs.col);
+            if (vs.need & NEED_SUM) add_f64_exact(s, vs.a_limb, vs.a_res, slot, x);
+            if (vs.need & NEED_MINMAX) {
+                if (x != x) {
+                    s.or32(vs.a_nan, slot, 1u);
+                } else {
+                    int64_t kk = f64_order_key(x);
+                    s.min64(vs.a_min, slot, kk);
+                    s.max64(vs.a_max, slot, kk);
+                }
+            }
+        } else if (vs.kind == VK_I64) {
+            int64_t x = row.i64(vs.col);
+            if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, x);
+            if (vs.need & NEED_MINMAX) { s.min64(vs.a_min, slot, x); s.max64(vs.a_max, slot, x); }
+        } else {
+            int32_t x = (int32_t)row.i64(vs.col);
+            if (vs.need & NEED_SUM) s.add64(vs.a_sum, slot, (int64_t)x);
+            if (vs.need & NEED_MINMAX) { s.min32(vs.a_min, slot, x); s.max32(vs.a_max, slot, x); }
+        }
+    }
+}
+
+// Merge LDS slot ls into global slot gs: one atomic per touched array.
+__device__ __forceinline__ void merge_lds_entry(GSink &g, uint64_t gs, LSink &l, uint32_t ls,
+                                                const Layout &lay, const int32_t *ops) {
+    for (int a = 0; a < lay.n_arrs; ++a) {
+        const int op = ops[a];
+        if (mo_size(op) == 4) {
+            uint32_t v = *l.p<uint32_t>(a, ls);
+            if (v == (uint32_t)mo_identity(op)) continue;
+            if (op == MO_ADD32) g.add32(a, gs, v);
+            else if (op == MO_MIN32) g.min32(a, gs, (int32_t)v);
+            else if (op == MO_MAX32) g.max32(a, gs, (int32_t)v);
+            else g.or32(a, gs, v);
+        } else {
+            uint64_t v = *l.p<uint64_t>(a, ls);
+            if (v == mo_identity(op)) continue;
+            if (op == MO_ADD64) g.add64(a, gs, (int64_t)v);
+            else if (op == MO_ADDF64) g.addf(a, gs, __longlong_as_double((long long)v));
+            else if (op == MO_MIN64) g.min64(a, gs, (int64_t)v);
+            else g.max64(a, gs, (int64_t)v);
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// output columns and group emission (shared by every finalising kernel)
+// ---------------------------------------------------------------------------
+constexpr int kOutMax = 64;
+enum OutSrc : int32_t { OS_KEY = 0, OS_AGG = 1 };
+
+struct OutCol {
+    int32_t src, idx, op, tag;  // OS_KEY: idx = key field; OS_AGG: idx = value spec (-1: len)
+    void *data;                 // element width of tag (Bool keys: one byte per row, packed later)
+    uint64_t *valid;            // ballot-written bitmap words (hash path v1), or nullptr
+    uint8_t *vbytes;            // one validity byte per row (partitioned path), or nullptr
+    uint8_t *data_bits;         // Bool keys on the partitioned path: packed data
+};
+
+// Reads of per-group state arrays, from global memory or from LDS.
+struct GReader {
+    void *const *arr;
+    template <class T> __device__ __forceinline__ T get(int a, uint64_t s) const { return ((const T *)arr[a])[s]; }
+};
+struct LReader {
+    uint8_t *base;
+    const uint32_t *off;
+    template <class T> __device__ __forceinline__ T get(int a, uint32_t s) const {
+        return ((const T *)(base + off[a]))[s];
+    }
+};
+
+// Key field fi of a group whose key is (lo, hi, special) in the lane layout.
+__device__ __forceinline__ void key_field(const QueryDesc *__restrict__ q, int fi, uint64_t lo, uint64_t hi,
+                                          int special, uint64_t &bits, bool &valid) {
+    const int mode = q->key_mode;
+    if (mode == KM_RAW64) {
+        if (special == 2) { bits = 0; valid = false; }
+        else { bits = special == 1 ? kEmpty : lo; valid = true; }
+        return;
+    }
+    if (special == 1) { lo = kEmpty; hi = mode == KM_LANE128 ? kEmpty : 0; }
+    const KeyField &f = q->keys[fi];
+    const int no = f.null_off;
+    valid = ((no < 64 ? (lo >> no) : (hi >> (no - 64))) & 1) == 0;
+    const int off = f.data_off;
+    uint64_t v;
+    if (off == 0) v = lo;
+    else if (off < 64) v = (lo >> off) | (hi << (64 - off));
+    else v = hi >> (off - 64);
+    if (f.bits < 64) v &= (1ull << f.bits) - 1;
+    bits = valid ? v : 0;
+}
+
+// Value of aggregate output oc for the group in slot s of reader rd.
+template <class Rd, class Slot>
+__device__ __forceinline__ void agg_value(const QueryDesc *__restrict__ q, const OutCol &oc, const Rd &rd, Slot s,
+                                          uint64_t &bits, bool &valid) {
+    const Layout &lay = q->layout;
+    if (oc.idx < 0) { bits = rd.template get<uint32_t>(lay.a_len, s); valid = true; return; }
+    const ValSpec &vs = q->vals[oc.idx];
+    const uint32_t cnt = rd.template get<uint32_t>(vs.a_cnt, s);
+    bits = 0;
+    valid = false;
+    switch (oc.op) {
+    case AGG_COUNT: bits = cnt; valid = true; return;
+    case AGG_SUM:
+    case AGG_MEAN: {
+        double d = 0.0;
+        int64_t si = 0;
+        if (vs.kind == VK_F) {
+            int64_t limb[kLimbs];
+#pragma unroll
+            for (int j = 0; j < kLimbs; ++j) limb[j] = rd.template get<int64_t>(vs.a_limb + j, s);
+            d = limbs_to_double(limb, rd.template get<double>(vs.a_res, s));
+        } else {
+            si = rd.template get<int64_t>(vs.a_sum, s);
+        }
+        if (oc.op == AGG_SUM) {
+            valid = true;
+            if (vs.kind == VK_F) {
+                if (oc.tag == PMH_F32) bits = (uint32_t)__float_as_uint((float)d);
+                else bits = (uint64_t)__double_as_longlong(d);
+            } else {
+                bits = (uint64_t)si;  // an I32 output keeps the low 32 bits: wrapping sum
+            }
+        } else {
+            valid = cnt > 0;
+            double m = 0.0;
+            if (valid) m = (vs.kind == VK_F ? d : (double)si) / (double)cnt;
+            if (oc.tag == PMH_F32) bits = (uint32_t)__float_as_uint((float)m);
+            else bits = (uint64_t)__double_as_longlong(m);
+        }
+        return;
+    }
+    default: {  // AGG_MIN / AGG_MAX
+        valid = cnt > 0;
+        if (!valid) return;
+        const int a = oc.op == AGG_MIN ? vs.a_min : vs.a_max;
+        if (vs.kind == VK_F) {
+            double d = rd.template get<uint32_t>(vs.a_nan, s) ? __longlong_as_double(0x7FF8000000000000ll)
+                                                               : f64_from_order_key(rd.template get<int64_t>(a, s));
+            if (oc.tag == PMH_F32) bits = (uint32_t)__float_as_uint((float)d);
+            else bits = (uint64_t)__double_as_longlong(d);
+        } else if (vs.kind == VK_I64) {
+            bits = (uint64_t)rd.template get<int64_t>(a, s);
+        } else {
+            bits = (uint32_t)rd.template get<int32_t>(a, s);
+        }
+        return;
+    }
+    }
+}
+
+__device__ __forceinline__ void store_tag(void *data, uint64_t i, int tag, uint64_t bits) {
+    switch (tag_size(tag)) {
+    case 8: ((uint64_t *)data)[i] = bits; break;
+    case 4: ((uint32_t *)data)[i] = (uint32_t)bits; break;
+    case 2: ((uint16_t *)data)[i] = (uint16_t)bits; break;
+    case 1: ((uint8_t *)data)[i] = (uint8_t)bits; break;
+    default: ((uint8_t *)data)[i] = (uint8_t)(bits & 1); break;  // Bool: a byte, packed later
+    }
+}
+
+// Writes every output column of one group at output row o (byte validity).
+template <class Rd, class Slot>
+__device__ __forceinline__ void emit_group(const QueryDesc *__restrict__ q, const OutCol *outs, int n_out,
+                                           const Rd &rd, Slot s, uint64_t lo, uint64_t hi, int special,
+                                           uint64_t o) {
+    for (int i = 0; i < n_out; ++i) {
+        const OutCol &oc = outs[i];
+        uint64_t bits;
+        bool valid;
+        if (oc.src == OS_KEY) key_field(q, oc.idx, lo, hi, special, bits, valid);
+        else agg_value(q, oc, rd, s, bits, valid);
+        store_tag(oc.data, o, oc.tag, bits);
+        if (oc.vbytes) oc.vbytes[o] = valid ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host: a query prepared once, run by one of the group-by engines
+// ---------------------------------------------------------------------------
+struct Prep {
+    QueryDesc q;
+    OutCol outs[kOutMax];  // src / idx / op / tag; engines fill the pointers
+    int n_out = 0;
+    std::vector<std::string> names;
+    std::vector<int> limb_k, limb_base;
+    std::string sig;
+    bool wide = false;
+    uint64_t n_rows = 0;
+};
+
+void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
+pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);
+// Returns nullptr when the query shape does not fit the partitioned engine.
+pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flags);
+
+}  // namespace pmh

@@ -61,6 +61,13 @@ class PmhExecStats(ctypes.Structure):
         ("n_groups", ctypes.c_uint64),
         ("n_selected", ctypes.c_uint64),
         ("device_ms", ctypes.c_double),
+        ("sample_rows", ctypes.c_uint32),
+        ("sample_distinct", ctypes.c_uint32),
+        ("sample_f1", ctypes.c_uint32),
+        ("sample_f2", ctypes.c_uint32),
+        ("hot_keys", ctypes.c_uint32),
+        ("partitions", ctypes.c_uint32),
+        ("tail_rows", ctypes.c_uint64),
     ]
 
 

@@ -154,6 +154,13 @@ def _stats_dict(st: _lib.PmhExecStats) -> dict:
         "n_groups": st.n_groups,
         "n_selected": st.n_selected,
         "device_ms": st.device_ms,
+        "sample_rows": st.sample_rows,
+        "sample_distinct": st.sample_distinct,
+        "sample_f1": st.sample_f1,
+        "sample_f2": st.sample_f2,
+        "hot_keys": st.hot_keys,
+        "partitions": st.partitions,
+        "tail_rows": st.tail_rows,
     }
 
 

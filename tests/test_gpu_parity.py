@@ -46,25 +46,49 @@ def test_zipf_scaled(native, rows):
     assert_results_equal(got, orc.groupby(plan, frame), ["key"], label=f"zipf{rows}")
 
 
-def test_highcard_scaled(native):
+@pytest.mark.parametrize("method", ["hash", "hash_global"])
+def test_highcard_scaled(native, method):
     frame, plan = synth.highcard(1_000_000, seed=5, pool=250_000)
+    plan = dict(plan, options={"method": method})
     got = run_gpu(native, plan, frame)
     want = orc.groupby(plan, frame)
     assert want["__n_groups__"] > 200_000
     assert_results_equal(got, want, ["a", "b"], label="highcard")
 
 
+def test_partitioned_stats_and_fallback(native):
+    frame, plan = synth.zipf(1_000_000, seed=31)
+    got = run_gpu(native, dict(plan, options={"method": "hash"}), frame)
+    st = native.last_stats()
+    assert st["hot_keys"] > 100 and st["sample_rows"] > 1000 and 0 < st["tail_rows"] < st["n_selected"]
+    want = orc.groupby(plan, frame)
+    assert_results_equal(got, want, ["key"], label="partitioned")
+    # tiny per-partition tables push most tail keys to the fallback table
+    got = run_gpu(native, dict(plan, options={"method": "hash", "partitions": 1, "tail_slots": 16}), frame)
+    assert_results_equal(got, want, ["key"], label="fallback")
+
+
 def test_resize_on_overflow(native):
     frame, plan = synth.zipf(300_000, seed=9)
-    plan = dict(plan, options={"capacity_hint": 8})
+    plan = dict(plan, options={"capacity_hint": 8, "method": "hash_global"})
     got = run_gpu(native, plan, frame)
     st = native.last_stats()
     assert st["n_passes"] > 1 and st["final_capacity"] > st["initial_capacity"]
     assert_results_equal(got, orc.groupby(plan, frame), ["key"], label="resize")
 
 
-@pytest.mark.parametrize("opts", [{"lds_slots": 0}, {"lds_slots": 64}, {"block": 256, "tile": 1000},
-                                  {"grid": 3}, {"block": 64, "grid": 1}])
+ENGINE_OPTS = [
+    {"method": "hash_global", "lds_slots": 0}, {"method": "hash_global", "lds_slots": 64},
+    {"method": "hash_global", "block": 256, "tile": 1000}, {"method": "hash_global", "grid": 3},
+    {"method": "hash_global", "block": 64, "grid": 1},
+    {"method": "hash"}, {"method": "hash", "lds_slots": 0}, {"method": "hash", "sample": 0},
+    {"method": "hash", "partitions": 1}, {"method": "hash", "partitions": 32768},
+    {"method": "hash", "partitions": 2, "tail_slots": 16},
+    {"method": "hash", "lds_slots": 16, "sample": 1000},
+]
+
+
+@pytest.mark.parametrize("opts", ENGINE_OPTS, ids=[str(o) for o in ENGINE_OPTS])
 def test_launch_shapes_agree(native, opts):
     frame, plan = synth.zipf(200_000, seed=4)
     plan = dict(plan, options=opts)
@@ -74,9 +98,10 @@ def test_launch_shapes_agree(native, opts):
 KEY_TAGS = ["I64", "I32", "I16", "I8", "U8", "U16", "U32", "U64", "F64", "F32", "Bool"]
 
 
+@pytest.mark.parametrize("method", ["hash", "hash_global"])
 @pytest.mark.parametrize("tag", KEY_TAGS)
-def test_key_dtypes_with_nulls(native, tag):
-    rng = np.random.default_rng(hash(tag) % 1000)
+def test_key_dtypes_with_nulls(native, tag, method):
+    rng = np.random.default_rng(KEY_TAGS.index(tag))
     n = 50_000
     if tag == "Bool":
         k = rng.random(n) < 0.5
@@ -90,7 +115,8 @@ def test_key_dtypes_with_nulls(native, tag):
         k[::103] = info.min
     frame = {"k": (tag, k, rng.random(n) < 0.95), "v": ("F64", rng.standard_normal(n), rng.random(n) < 0.9),
              "w": ("I32", rng.integers(-50, 50, n).astype(np.int32), None)}
-    plan = {"keys": [["k", tag]], "aggs": synth.FIVE_AGGS + [{"kind": "Length", "op": "Len", "output_alias": "n"}]}
+    plan = {"keys": [["k", tag]], "aggs": synth.FIVE_AGGS + [{"kind": "Length", "op": "Len", "output_alias": "n"}],
+            "options": {"method": method}}
     assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["k"], label=tag)
 
 
@@ -99,13 +125,15 @@ def test_i64_all_ones_and_null_side_slots(native):
     valid = np.array([1, 1, 1, 0, 1, 0, 1], bool)
     frame = {"k": ("I64", k, valid), "v": ("F64", np.arange(7, dtype=np.float64), None)}
     plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "s"}]}
-    for opts in ({}, {"lds_slots": 0}):
+    for opts in ({}, {"lds_slots": 0}, {"method": "hash_global"}, {"method": "hash_global", "lds_slots": 0},
+                 {"method": "hash", "partitions": 4, "tail_slots": 16}):
         p = dict(plan, options=opts)
         assert_results_equal(run_gpu(native, p, frame), orc.groupby(p, frame), ["k"], label=f"side{opts}")
 
 
+@pytest.mark.parametrize("method", ["hash", "hash_global"])
 @pytest.mark.parametrize("keys", [["a", "b"], ["b", "c", "a"], ["c", "d"], ["a", "c", "d"]])
-def test_composite_keys(native, keys):
+def test_composite_keys(native, keys, method):
     rng = np.random.default_rng(len(keys))
     n = 100_000
     frame = {"a": ("I32", rng.integers(-5, 5, n).astype(np.int32), rng.random(n) < 0.97),
@@ -116,7 +144,7 @@ def test_composite_keys(native, keys):
     tags = {k: frame[k][0] for k in keys}
     plan = {"keys": [[k, tags[k]] for k in keys],
             "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in
-                     ("Sum", "Count", "Min", "Max", "Mean")]}
+                     ("Sum", "Count", "Min", "Max", "Mean")], "options": {"method": method}}
     assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), keys, label=str(keys))
 
 

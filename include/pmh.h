@@ -106,6 +106,7 @@ typedef struct pmh_exec_stats {
     uint32_t hot_keys;        /* keys folded in LDS by every scan block */
     uint32_t partitions;      /* tail partitions */
     uint64_t tail_rows;       /* selected rows not covered by the hot keys */
+    double distinct_estimate; /* distinct keys the engine sized its partitions for */
 } pmh_exec_stats;
 
 typedef struct pmh_result {

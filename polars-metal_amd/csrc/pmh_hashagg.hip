@@ -27,6 +27,7 @@
 //   hs_hot_emit       writes the hot groups.
 //   hs_pack           turns the per-row validity bytes into Arrow bitmaps.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "pmh_table.h"
@@ -62,7 +63,7 @@ struct HLaunch {
     uint32_t n_sample, hot_slots, hot_max, pad0;
     uint64_t *samp_lo, *samp_hi;
     uint8_t *samp_sp;               // side-key marker of each sampled key
-    uint32_t sample_slots;          // LDS count-table size of hs_sample_count
+    uint32_t sample_slots;          // LDS fingerprint-table size of hs_sample_count
     uint32_t pad2;
     uint64_t *hot_lo, *hot_hi;
     void *hot_arr[kMaxArrs];
@@ -87,6 +88,7 @@ struct HLaunch {
     OutCol out[kOutMax];
     uint32_t *out_cursor;  // [0] groups written by hs_hot_emit and hs_part_agg
     uint32_t *stats;       // StatIdx
+    uint32_t *sketch_hist; // [1024]: sampled keys seen c times (c clipped at 1023)
 };
 
 // Digit of a spilled key for partitioning: side keys go to partition 0.
@@ -160,71 +162,53 @@ __global__ __launch_bounds__(256) void hs_sample_gather(const HLaunch *__restric
     }
 }
 
-// One block: count sampled keys, pick the hot set, lay it out, reset the
-// global hot state and the output cursor.
+// One block: count the sampled keys by 32-bit fingerprint in LDS, pick the
+// most frequent fingerprints (up to hot_max), recover their keys from the
+// sample, lay them out in the hot table, and record the sketch (sampled rows,
+// distinct fingerprints, f1, f2).  It also resets the global hot state.
 __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     const QueryDesc *__restrict__ q = &L->q;
     const bool wide = L->wide;
-    const uint32_t kSampleSlots = L->sample_slots;  // power of two
-    uint64_t *s_lo = (uint64_t *)smem;
-    uint64_t *s_hi = s_lo + kSampleSlots;                         // wide only (aliased otherwise)
-    uint32_t *s_cnt = (uint32_t *)(wide ? (s_hi + kSampleSlots) : (s_lo + kSampleSlots));
-    uint32_t *hist = s_cnt + kSampleSlots;                        // 1024 count buckets
-    uint64_t *h_lo = (uint64_t *)(hist + 1024);                   // hot layout
+    const uint32_t SS = L->sample_slots;  // power of two
+    uint32_t *s_fp = (uint32_t *)smem;    // fingerprint | 1, 0 = free
+    uint32_t *s_cnt = s_fp + SS;          // count; bit 31: picked; bit 30: key placed
+    uint32_t *hist = s_cnt + SS;          // 1024 count buckets
+    uint64_t *h_lo = (uint64_t *)(hist + 1024);
     uint64_t *h_hi = h_lo + L->hot_slots;
-    __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop;
+    __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop, s_hotrows;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t CL = L->hot_slots;
-    for (uint32_t i = tid; i < kSampleSlots; i += bs) { s_lo[i] = kEmpty; if (wide) s_hi[i] = kEmpty; s_cnt[i] = 0; }
+    for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
     for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
     for (uint32_t i = tid; i < CL; i += bs) { h_lo[i] = kEmpty; if (wide) h_hi[i] = kEmpty; }
-    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; }
+    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; }
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
-    uint32_t drops = 0, nullkeys = 0;
+    uint32_t drops = 0, sidekeys = 0;
     for (uint32_t i = tid; i < nsel; i += bs) {
-        const uint64_t lo = L->samp_lo[i], hi = wide ? L->samp_hi[i] : 0;
-        if (L->samp_sp[i]) { ++nullkeys; continue; }  // side keys are never hot
-        uint32_t s = (uint32_t)hash_key(lo, wide ? hi : 0) & (kSampleSlots - 1);
+        if (L->samp_sp[i]) { ++sidekeys; continue; }  // side keys are never hot
+        const uint64_t h = hash_key(L->samp_lo[i], wide ? L->samp_hi[i] : 0);
+        const uint32_t fp = (uint32_t)(h >> 32) | 1u;
+        uint32_t s = (uint32_t)h & (SS - 1);
         bool done = false;
-        for (int p = 0; p < 64 && !done; ++p) {
-            // claim by the low word; for wide keys the high word is compared
-            // after the claim (two keys sharing a low word take two slots)
-            uint64_t cur = __hip_atomic_load(&s_lo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (cur == kEmpty) {
-                unsigned long long old = atomicCAS((unsigned long long *)&s_lo[s], (unsigned long long)kEmpty,
-                                                   (unsigned long long)lo);
-                if (old == kEmpty) {
-                    if (wide) __hip_atomic_store(&s_hi[s], hi, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    atomicAdd(&s_cnt[s], 1u);
-                    done = true;
-                    break;
-                }
-                cur = old;
+        for (int p = 0; p < 64; ++p) {
+            uint32_t cur = s_fp[s];
+            if (cur == 0) {
+                const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
+                cur = old == 0u ? fp : old;
             }
-            if (cur == lo) {
-                if (!wide) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
-                // the claimer stores the high word right after its CAS; the
-                // wait is bounded (a sample key that gives up is only dropped)
-                uint64_t ch = kEmpty;
-                for (int spin = 0; spin < 4096; ++spin) {
-                    ch = __hip_atomic_load(&s_hi[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (ch != kEmpty || hi == kEmpty) break;
-                }
-                if (ch == hi) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
-            }
-            s = (s + 1) & (kSampleSlots - 1);
+            if (cur == fp) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
+            s = (s + 1) & (SS - 1);
         }
         if (!done) ++drops;
     }
     if (drops) atomicAdd(&s_drop, drops);
-    if (nullkeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], nullkeys);
+    if (sidekeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], sidekeys);
     __syncthreads();
-    // sketch + count histogram
     uint32_t f1 = 0, f2 = 0, dist = 0;
-    for (uint32_t i = tid; i < kSampleSlots; i += bs) {
-        uint32_t c = s_cnt[i];
+    for (uint32_t i = tid; i < SS; i += bs) {
+        const uint32_t c = s_cnt[i];
         if (!c) continue;
         ++dist;
         if (c == 1) ++f1;
@@ -235,18 +219,17 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     atomicAdd(&s_f2, f2);
     atomicAdd(&s_distinct, dist);
     __syncthreads();
+    for (uint32_t i = tid; i < 1024; i += bs) L->sketch_hist[i] = hist[i];
     if (tid == 0) {
-        // smallest count threshold T >= 2 such that #keys with count > T fits
-        uint32_t acc = 0, T = 1024;
+        // largest count threshold T such that the keys with count > T fit
+        uint32_t acc = 0, T = 0;
         const uint32_t kmax = L->hot_max;
-        for (int c = 1023; c >= 2; --c) {
+        for (int c = 1023; c >= 1; --c) {
             if (acc + hist[c] > kmax) { T = (uint32_t)c; break; }
             acc += hist[c];
-            T = (uint32_t)c - 1;
         }
-        if (T < 2) T = 1;  // every key with count >= 2 fits
-        s_thresh = T;
-        s_take_eq = kmax > acc ? kmax - acc : 0;  // keys with count == T still allowed
+        s_thresh = T;                             // T == 0: every sampled key fits
+        s_take_eq = T ? kmax - acc : 0;           // keys with count == T still allowed
         L->stats[ST_SAMPLE_DISTINCT] = s_distinct;
         L->stats[ST_SAMPLE_F1] = s_f1;
         L->stats[ST_SAMPLE_F2] = s_f2;
@@ -254,27 +237,47 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     }
     __syncthreads();
     const uint32_t T = s_thresh;
-    uint32_t hot_rows = 0;
-    for (uint32_t i = tid; i < kSampleSlots && CL; i += bs) {
+    for (uint32_t i = tid; i < SS && CL; i += bs) {
         const uint32_t c = min(s_cnt[i], 1023u);
-        bool pick = c > T && c >= 2;
-        if (!pick && c == T && c >= 2) pick = atomicAdd(&s_take_eq, (uint32_t)-1) - 1 < 0x80000000u;
-        if (!pick) continue;
-        const uint64_t lo = s_lo[i], hi = wide ? s_hi[i] : 0;
-        uint32_t s = (uint32_t)(hash_key(lo, hi) >> 40) & (CL - 1);
-        for (uint32_t p = 0; p < CL; ++p) {
-            unsigned long long old = atomicCAS((unsigned long long *)&h_lo[s], (unsigned long long)kEmpty,
-                                               (unsigned long long)lo);
-            if (old == kEmpty) {
-                if (wide) h_hi[s] = hi;
-                atomicAdd(&s_hot, 1u);
-                hot_rows += s_cnt[i];
+        if (!c) continue;
+        bool pick = c > T;
+        if (!pick && c == T) pick = (int32_t)atomicSub(&s_take_eq, 1u) > 0;
+        if (pick) s_cnt[i] |= 1u << 31;
+    }
+    __syncthreads();
+    // recover the keys of picked fingerprints: the first sampled row of each
+    // picked fingerprint places its key in the hot table
+    uint32_t hot_rows = 0;
+    for (uint32_t i = tid; i < nsel && CL; i += bs) {
+        if (L->samp_sp[i]) continue;
+        const uint64_t lo = L->samp_lo[i], hi = wide ? L->samp_hi[i] : 0;
+        const uint64_t h = hash_key(lo, hi);
+        const uint32_t fp = (uint32_t)(h >> 32) | 1u;
+        uint32_t s = (uint32_t)h & (SS - 1);
+        for (int p = 0; p < 64; ++p) {
+            const uint32_t cur = s_fp[s];
+            if (cur == 0) break;
+            if (cur == fp) {
+                if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
+                    hot_rows += s_cnt[s] & 0x3FFFFFFFu;
+                    uint32_t t = (uint32_t)(h >> 40) & (CL - 1);
+                    for (uint32_t pp = 0; pp < CL; ++pp) {
+                        unsigned long long old = atomicCAS((unsigned long long *)&h_lo[t], (unsigned long long)kEmpty,
+                                                           (unsigned long long)lo);
+                        if (old == kEmpty) {
+                            if (wide) h_hi[t] = hi;
+                            atomicAdd(&s_hot, 1u);
+                            break;
+                        }
+                        t = (t + 1) & (CL - 1);
+                    }
+                }
                 break;
             }
-            s = (s + 1) & (CL - 1);
+            s = (s + 1) & (SS - 1);
         }
     }
-    if (hot_rows) atomicAdd(&L->stats[ST_HOT_SAMPLE_ROWS], hot_rows);
+    if (hot_rows) atomicAdd(&s_hotrows, hot_rows);
     __syncthreads();
     for (uint32_t i = tid; i < CL; i += bs) {
         L->hot_lo[i] = h_lo[i];
@@ -288,6 +291,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     }
     if (tid == 0) {
         L->stats[ST_HOT] = s_hot;
+        L->stats[ST_HOT_SAMPLE_ROWS] = s_hotrows;
         L->out_cursor[0] = 0;
     }
 }
@@ -1013,6 +1017,50 @@ size_t lds_layout(const Prep &P, uint32_t nslots, uint32_t *off) {
     return o;
 }
 
+}  // namespace
+
+Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
+                   uint32_t dropped, const std::vector<uint32_t> &hist) {
+    Sketch k;
+    k.sample_rows = s_sel;
+    k.sample_size = n_sample;
+    k.n_rows = n;
+    k.distinct = d;
+    k.f1 = f1;
+    k.f2 = f2;
+    k.dropped = dropped;
+    k.n_sel = n_sample ? (double)n * s_sel / n_sample : 0.0;
+    // F2 moment of the sample frequencies, and the Shlosser sums
+    const double qf = k.n_sel > 0 ? std::min(1.0, s_sel / k.n_sel) : 1.0;
+    double F2 = 0, num = 0, den = 0;
+    for (int c = 1; c < 1024; ++c) {
+        if (!hist[c]) continue;
+        F2 += (double)c * c * hist[c];
+        num += std::pow(1.0 - qf, c) * hist[c];
+        den += c * qf * std::pow(1.0 - qf, c - 1) * hist[c];
+    }
+    k.F2 = F2;
+    k.shl_num = num;
+    k.shl_den = den;
+    return k;
+}
+
+// Distinct keys among the selected rows: the geometric mean of GEE
+// (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
+// estimator (Haas et al., 1995; it overshoots), floored at the sample's own
+// distinct count.  A full fingerprint table extrapolates linearly.
+double plain_distinct_estimate(const Sketch &k) {
+    if (k.sample_rows == 0) return 0.0;
+    const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
+    if (k.dropped) return std::max(d, k.n_sel * (d + k.dropped) / s);
+    if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
+    const double gee = std::sqrt(k.n_sel / s) * f1 + (d - f1);
+    const double shl = k.shl_den > 0 ? d + f1 * k.shl_num / k.shl_den : gee;
+    return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));
+}
+
+namespace {
+
 void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, const int32_t *vw, int nv) {
     S.klo = (uint64_t *)slab(tag + ":klo", cap * 8 + 64);
     S.khi = wide ? (uint64_t *)slab(tag + ":khi", cap * 8 + 64) : nullptr;
@@ -1059,15 +1107,11 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     CL = std::min<uint32_t>(CL, 2048);
     if (opt.lds_slots >= 0) CL = opt.lds_slots == 0 ? 0 : (uint32_t)std::min<uint64_t>(CL, pow2_floor(opt.lds_slots));
     if (CL < 16) CL = 0;
-    const uint32_t n_sample = (uint32_t)(opt.sample >= 0 ? std::min<int64_t>(opt.sample, 1 << 20)
-                                                         : (int64_t)std::min<uint64_t>(n, 262144));
-    const uint32_t sample_slots = wide ? 4096 : 8192;
-    uint64_t P_parts = opt.partitions > 0 ? pow2_floor((uint64_t)opt.partitions)
-                                          : pow2_ceil(std::min<uint64_t>(std::max<uint64_t>(n / 100000, 1), 4096));
-    P_parts = std::min<uint64_t>(P_parts, 1u << 15);
-    const int pbits = log2u(P_parts);
-    const int passes = std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
-    const int sortbits = passes * kRadixBits;
+    // a 64K-row sample sketch (the sample is capped so its distinct keys fit
+    // the fingerprint table)
+    const uint32_t n_sample = (uint32_t)(opt.sample >= 0 ? std::min<int64_t>(opt.sample, 1 << 16)
+                                                         : (int64_t)std::min<uint64_t>(n, 1 << 16));
+    const uint32_t sample_slots = wide ? 8192 : 16384;
     const size_t tail_slot_bytes = key_bytes + (wide ? 4 : 0) + state_per_slot;
     uint32_t TS = (uint32_t)std::min<uint64_t>(4096, pow2_floor(lds_budget / tail_slot_bytes));
     while (TS > 16 && (size_t)(TS + 2) * tail_slot_bytes > lds_budget) TS /= 2;
@@ -1082,9 +1126,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.chunk = chunk;
     L.nb = nb;
     L.max_tiles = max_tiles;
-    L.pbits = (uint32_t)pbits;
-    L.sortbits = (uint32_t)sortbits;
-    L.passes = (uint32_t)passes;
     L.tail_slots = TS;
 
     // ---- buffers
@@ -1112,6 +1153,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.digit_base = (uint32_t *)slab("hs:digit_base", kRadix * 4 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
     L.stats = (uint32_t *)slab("hs:stats", ST_N * 4 + 64);
+    L.sketch_hist = (uint32_t *)slab("hs:sketch_hist", 1024 * 4 + 64);
     // fallback table
     {
         GTable &t = L.t;
@@ -1174,12 +1216,54 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     {
         KernelTimer kt("hs_sample_count");
-        const size_t lds = (size_t)sample_slots * (wide ? 20 : 12) + 4096 + (size_t)CL * key_bytes;
+        const size_t lds = (size_t)sample_slots * 8 + 4096 + (size_t)CL * key_bytes;
         PMH_HIP(hipFuncSetAttribute((const void *)hs_sample_count, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)lds));
         hipLaunchKernelGGL(hs_sample_count, dim3(1), dim3(1024), lds, R.stream, dL);
         launch_check();
     }
+    // ---- partition count from the sketch (one small read-back)
+    uint32_t sk[ST_N];
+    std::vector<uint32_t> hist(1024);
+    PMH_HIP(hipMemcpyAsync(sk, L.stats, sizeof(sk), hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(hist.data(), L.sketch_hist, 1024 * 4, hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipStreamSynchronize(R.stream));
+    const double tail_cap = 0.5 * TS;  // target keys per partition
+    Sketch sketch = sketch_from(sk[ST_SAMPLE_SEL], n_sample, n, sk[ST_SAMPLE_DISTINCT], sk[ST_SAMPLE_F1],
+                                sk[ST_SAMPLE_F2], sk[ST_SAMPLE_DROPPED], hist);
+    double d_est = plain_distinct_estimate(sketch);
+    if (opt.distinct_hint > 0) d_est = (double)opt.distinct_hint;
+    const double d_tail = std::max(0.0, d_est - sk[ST_HOT]);
+    uint64_t P_parts;
+    if (opt.partitions > 0) {
+        P_parts = pow2_floor((uint64_t)opt.partitions);
+    } else {
+        P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_tail / tail_cap)));
+        if (d_tail > (double)(1u << 15) * tail_cap * 1.5 && opt.method == "auto") {
+            // beyond what partitions of one LDS table can hold
+            collect_kernel_times();
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+            return nullptr;
+        }
+    }
+    P_parts = std::min<uint64_t>(P_parts, 1u << 15);
+    const int pbits = log2u(P_parts);
+    const int passes = std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
+    const int sortbits = passes * kRadixBits;
+    L.pbits = (uint32_t)pbits;
+    L.sortbits = (uint32_t)sortbits;
+    L.passes = (uint32_t)passes;
+    // per-partition table: about twice the expected keys per partition, so
+    // the per-partition LDS reset stays proportional to the work
+    if (opt.tail_slots <= 0) {
+        const double per = d_tail / (double)P_parts;
+        uint32_t ts = (uint32_t)std::min<uint64_t>(TS, std::max<uint64_t>(256, pow2_ceil((uint64_t)(2.5 * per) + 1)));
+        L.tail_slots = ts;
+    }
+    const size_t lds3b = (size_t)(L.tail_slots + 2) * (key_bytes + (wide ? 4 : 0)) +
+                         lds_layout(P, L.tail_slots + 2, L.lds3_off);
+    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     {
         const bool cached = q0.n_cols <= 4;
         KernelTimer kt("hs_scan");
@@ -1221,9 +1305,9 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     {
         KernelTimer kt("hs_part_agg");
-        PMH_HIP(hipFuncSetAttribute((const void *)hs_part_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3));
+        PMH_HIP(hipFuncSetAttribute((const void *)hs_part_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3b));
         const uint32_t g = (uint32_t)std::min<uint64_t>(P_parts, (uint64_t)cus * 2);
-        hipLaunchKernelGGL(hs_part_agg, dim3(g), dim3(1024), lds3, R.stream, dL, bufs[src]);
+        hipLaunchKernelGGL(hs_part_agg, dim3(g), dim3(1024), lds3b, R.stream, dL, bufs[src]);
         launch_check();
     }
     if (CL) {
@@ -1282,6 +1366,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     res->stats.hot_keys = host_stats[ST_HOT];
     res->stats.partitions = (uint32_t)P_parts;
     res->stats.tail_rows = host_stats[ST_TAIL];
+    res->stats.distinct_estimate = d_est;
     const uint64_t vbytes = bitmap_bytes_padded(ng);
     const uint64_t packed = (ng + 7) / 8;
     for (int oi = 0; oi < P.n_out; ++oi) {

@@ -165,6 +165,7 @@ struct Options {
     int64_t partitions = 0;       // partitioned engine: tail partitions (power of two), 0: auto
     int64_t tail_slots = 0;       // partitioned engine: per-partition LDS table slots, 0: auto
     int64_t sample = -1;          // partitioned engine: sampled rows for the hot set, -1: auto
+    int64_t distinct_hint = 0;    // partitioned engine: expected distinct keys (a sizer's prediction), 0: sketch
 };
 
 struct GroupByPlan {

@@ -303,6 +303,16 @@ struct Prep {
     uint64_t n_rows = 0;
 };
 
+// Sample sketch of the partitioned engine (host side).
+struct Sketch {
+    uint32_t sample_rows = 0, sample_size = 0, distinct = 0, f1 = 0, f2 = 0, dropped = 0;
+    uint64_t n_rows = 0;
+    double n_sel = 0, F2 = 0, shl_num = 0, shl_den = 0;
+};
+Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
+                   uint32_t dropped, const std::vector<uint32_t> &hist);
+double plain_distinct_estimate(const Sketch &k);
+
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);
 // Returns nullptr when the query shape does not fit the partitioned engine.

@@ -68,6 +68,7 @@ class PmhExecStats(ctypes.Structure):
         ("hot_keys", ctypes.c_uint32),
         ("partitions", ctypes.c_uint32),
         ("tail_rows", ctypes.c_uint64),
+        ("distinct_estimate", ctypes.c_double),
     ]
 
 

@@ -161,6 +161,7 @@ def _stats_dict(st: _lib.PmhExecStats) -> dict:
         "hot_keys": st.hot_keys,
         "partitions": st.partitions,
         "tail_rows": st.tail_rows,
+        "distinct_estimate": st.distinct_estimate,
     }
 
 

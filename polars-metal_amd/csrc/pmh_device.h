@@ -9,6 +9,17 @@ namespace pmh {
 
 constexpr uint64_t kEmpty = ~0ull;
 
+// Global-memory accessors.  Column and table pointers reach the kernels
+// through descriptors in device memory, so the compiler cannot prove their
+// address space and would emit flat instructions, which also count against
+// the LDS wait counter and serialise HBM loads behind LDS atomics.  These
+// casts pin them to the global address space.
+#define PMH_G __attribute__((address_space(1)))
+template <class T> __device__ __forceinline__ T gld(const void *p, uint64_t i) { return ((const PMH_G T *)p)[i]; }
+template <class T> __device__ __forceinline__ void gst(void *p, uint64_t i, T v) { ((PMH_G T *)p)[i] = v; }
+template <class T> __device__ __forceinline__ PMH_G T *gp(void *p) { return (PMH_G T *)p; }
+template <class T> __device__ __forceinline__ const PMH_G T *gp(const void *p) { return (const PMH_G T *)p; }
+
 // Every per-group state array merges with one of these operators.
 enum MergeOp : int32_t { MO_ADD32 = 0, MO_ADD64, MO_ADDF64, MO_MIN32, MO_MAX32, MO_MIN64, MO_MAX64, MO_OR32 };
 
@@ -29,28 +40,28 @@ __host__ __device__ inline uint64_t mo_identity(int op) {
 // column loads (the tag is wave-uniform, so the switch does not diverge)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool dvalid(const uint8_t *v, uint64_t r) {
-    return v ? ((v[r >> 3] >> (r & 7)) & 1) : true;
+    return v ? ((gld<uint8_t>(v, r >> 3) >> (r & 7)) & 1) : true;
 }
 
 __device__ __forceinline__ int64_t dload_i64(const DevCol &c, uint64_t r) {
     switch (c.tag) {
-    case PMH_I64: case PMH_U64: return ((const int64_t *)c.data)[r];
-    case PMH_I32: return ((const int32_t *)c.data)[r];
-    case PMH_U32: case PMH_UTF8: return ((const uint32_t *)c.data)[r];
-    case PMH_I16: return ((const int16_t *)c.data)[r];
-    case PMH_U16: return ((const uint16_t *)c.data)[r];
-    case PMH_I8: return ((const int8_t *)c.data)[r];
-    case PMH_U8: return ((const uint8_t *)c.data)[r];
-    case PMH_BOOL: return (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
-    case PMH_F64: return (int64_t)((const double *)c.data)[r];
-    case PMH_F32: return (int64_t)((const float *)c.data)[r];
+    case PMH_I64: case PMH_U64: return gld<int64_t>(c.data, r);
+    case PMH_I32: return gld<int32_t>(c.data, r);
+    case PMH_U32: case PMH_UTF8: return gld<uint32_t>(c.data, r);
+    case PMH_I16: return gld<int16_t>(c.data, r);
+    case PMH_U16: return gld<uint16_t>(c.data, r);
+    case PMH_I8: return gld<int8_t>(c.data, r);
+    case PMH_U8: return gld<uint8_t>(c.data, r);
+    case PMH_BOOL: return (gld<uint8_t>(c.data, r >> 3) >> (r & 7)) & 1;
+    case PMH_F64: return (int64_t)gld<double>(c.data, r);
+    case PMH_F32: return (int64_t)gld<float>(c.data, r);
     default: return 0;
     }
 }
 
 __device__ __forceinline__ double dload_f64(const DevCol &c, uint64_t r) {
-    if (c.tag == PMH_F64) return ((const double *)c.data)[r];
-    if (c.tag == PMH_F32) return (double)((const float *)c.data)[r];
+    if (c.tag == PMH_F64) return gld<double>(c.data, r);
+    if (c.tag == PMH_F32) return (double)gld<float>(c.data, r);
     return (double)dload_i64(c, r);
 }
 
@@ -58,11 +69,11 @@ __device__ __forceinline__ double dload_f64(const DevCol &c, uint64_t r) {
 // (crates/polars-metal-kernels/src/groupby.rs:226-292).
 __device__ __forceinline__ uint64_t dload_bits(const DevCol &c, uint64_t r) {
     switch (c.tag) {
-    case PMH_I64: case PMH_U64: case PMH_F64: return ((const uint64_t *)c.data)[r];
-    case PMH_I32: case PMH_U32: case PMH_F32: case PMH_UTF8: return ((const uint32_t *)c.data)[r];
-    case PMH_I16: case PMH_U16: return ((const uint16_t *)c.data)[r];
-    case PMH_I8: case PMH_U8: return ((const uint8_t *)c.data)[r];
-    case PMH_BOOL: return (((const uint8_t *)c.data)[r >> 3] >> (r & 7)) & 1;
+    case PMH_I64: case PMH_U64: case PMH_F64: return gld<uint64_t>(c.data, r);
+    case PMH_I32: case PMH_U32: case PMH_F32: case PMH_UTF8: return gld<uint32_t>(c.data, r);
+    case PMH_I16: case PMH_U16: return gld<uint16_t>(c.data, r);
+    case PMH_I8: case PMH_U8: return gld<uint8_t>(c.data, r);
+    case PMH_BOOL: return (gld<uint8_t>(c.data, r >> 3) >> (r & 7)) & 1;
     default: return 0;
     }
 }
@@ -260,20 +271,36 @@ __device__ __forceinline__ double f64_from_order_key(int64_t k) {
 struct GSink {
     void *const *arr;
     uint64_t nslots;
-    __device__ __forceinline__ void add32(int a, uint64_t s, uint32_t v) { atomicAdd((uint32_t *)arr[a] + s, v); }
-    __device__ __forceinline__ void add64(int a, uint64_t s, int64_t v) {
-        atomicAdd((unsigned long long *)arr[a] + s, (unsigned long long)v);
+    template <class T> __device__ __forceinline__ PMH_G T *at(int a, uint64_t s) { return gp<T>(arr[a]) + s; }
+    __device__ __forceinline__ void add32(int a, uint64_t s, uint32_t v) {
+        __hip_atomic_fetch_add(at<uint32_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __device__ __forceinline__ void addf(int a, uint64_t s, double v) { unsafeAtomicAdd((double *)arr[a] + s, v); }
-    __device__ __forceinline__ void min32(int a, uint64_t s, int32_t v) { atomicMin((int *)arr[a] + s, v); }
-    __device__ __forceinline__ void max32(int a, uint64_t s, int32_t v) { atomicMax((int *)arr[a] + s, v); }
-    __device__ __forceinline__ void min64(int a, uint64_t s, int64_t v) { atomicMin((long long *)arr[a] + s, (long long)v); }
-    __device__ __forceinline__ void max64(int a, uint64_t s, int64_t v) { atomicMax((long long *)arr[a] + s, (long long)v); }
-    __device__ __forceinline__ void or32(int a, uint64_t s, uint32_t v) { atomicOr((uint32_t *)arr[a] + s, v); }
+    __device__ __forceinline__ void add64(int a, uint64_t s, int64_t v) {
+        __hip_atomic_fetch_add(at<uint64_t>(a, s), (uint64_t)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void addf(int a, uint64_t s, double v) {
+        __hip_atomic_fetch_add(at<double>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void min32(int a, uint64_t s, int32_t v) {
+        __hip_atomic_fetch_min(at<int32_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void max32(int a, uint64_t s, int32_t v) {
+        __hip_atomic_fetch_max(at<int32_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void min64(int a, uint64_t s, int64_t v) {
+        __hip_atomic_fetch_min(at<int64_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void max64(int a, uint64_t s, int64_t v) {
+        __hip_atomic_fetch_max(at<int64_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void or32(int a, uint64_t s, uint32_t v) {
+        __hip_atomic_fetch_or(at<uint32_t>(a, s), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     // limb k of the exact accumulator whose limb 0 is array a (limbs are
     // allocated back to back, nslots apart)
     __device__ __forceinline__ void addlimb(int a, int k, uint64_t s, int64_t v) {
-        atomicAdd((unsigned long long *)arr[a] + (uint64_t)k * nslots + s, (unsigned long long)v);
+        __hip_atomic_fetch_add(gp<uint64_t>(arr[a]) + (uint64_t)k * nslots + s, (uint64_t)v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 };
 

@@ -805,9 +805,11 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
     const std::string &m = plan.opts.method;
     if (m == "hash_global") return run_hash_global(P, plan, flags);
     if (m == "auto" || m == "hash") {
+        // nullptr: the shape does not fit the partitioned engine's LDS plans,
+        // or its fallback table overflowed; the global-table engine (which
+        // resizes) then runs the query
         pmh_result *r = run_hash_partitioned(P, plan, flags);
         if (r) return r;
-        if (m == "hash") fail(PMH_E_VALUE, "polars_metal: the partitioned hash path cannot run this query shape");
     }
     if (m == "sort") fail(PMH_E_NOTIMPL, "polars_metal: the sort path is not built yet");
     return run_hash_global(P, plan, flags);

@@ -113,6 +113,24 @@ __device__ __forceinline__ void store_w(void *p, int w, uint64_t i, uint64_t x) 
     }
 }
 
+// Spill arrays live in global memory.
+__device__ __forceinline__ uint64_t gload_w(const void *p, int w, uint64_t i) {
+    switch (w) {
+    case 8: return gld<uint64_t>(p, i);
+    case 4: return gld<uint32_t>(p, i);
+    case 2: return gld<uint16_t>(p, i);
+    default: return gld<uint8_t>(p, i);
+    }
+}
+__device__ __forceinline__ void gstore_w(void *p, int w, uint64_t i, uint64_t x) {
+    switch (w) {
+    case 8: gst<uint64_t>(p, i, x); break;
+    case 4: gst<uint32_t>(p, i, (uint32_t)x); break;
+    case 2: gst<uint16_t>(p, i, (uint16_t)x); break;
+    default: gst<uint8_t>(p, i, (uint8_t)x); break;
+    }
+}
+
 // Row accessor over a spilled record: value columns only.
 struct SpillRow {
     const HLaunch *__restrict__ L;
@@ -122,7 +140,7 @@ struct SpillRow {
     __device__ __forceinline__ bool valid(int c) const { return (flags >> L->col_val[c]) & 1; }
     __device__ __forceinline__ uint64_t bits(int c) const {
         const int j = L->col_val[c];
-        return load_w(S->val[j], L->vw[j], i);
+        return gload_w(S->val[j], L->vw[j], i);
     }
     __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(L->q.cols[c].tag, bits(c)); }
     __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
@@ -347,11 +365,11 @@ __device__ __forceinline__ void append_tail(const HLaunch *__restrict__ L, const
     for (int j = 0; j < q->n_vals; ++j) {
         const int col = q->vals[j].col;
         if (row.valid(col)) fl |= (uint16_t)(1u << j);
-        store_w(S.val[j], L->vw[j], pos, row.bits(col));
+        gstore_w(S.val[j], L->vw[j], pos, row.bits(col));
     }
-    S.klo[pos] = k.lo;
-    if (L->wide) S.khi[pos] = k.hi;
-    S.flags[pos] = fl;
+    gst<uint64_t>(S.klo, pos, k.lo);
+    if (L->wide) gst<uint64_t>(S.khi, pos, k.hi);
+    gst<uint16_t>(S.flags, pos, fl);
 }
 
 template <bool CACHED>
@@ -546,8 +564,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_hist(const HLaunch *__restric
             const uint32_t i = k * kPartThreads + threadIdx.x;
             if (i < ts.rows) {
                 const uint64_t s = ts.src_begin + i;
-                const uint64_t hi = L->wide ? src.khi[s] : 0;
-                atomicAdd(&hist[pass_digit(src.klo[s], hi, src.flags[s], shift)], 1u);
+                const uint64_t hi = L->wide ? gld<uint64_t>(src.khi, s) : 0;
+                atomicAdd(&hist[pass_digit(gld<uint64_t>(src.klo, s), hi, gld<uint16_t>(src.flags, s), shift)], 1u);
             }
         }
         __syncthreads();
@@ -635,7 +653,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
             uint32_t d = 0;
             if (live) {
                 const uint64_t s = ts.src_begin + i;
-                d = pass_digit(src.klo[s], wide ? src.khi[s] : 0, src.flags[s], shift);
+                d = pass_digit(gld<uint64_t>(src.klo, s), wide ? gld<uint64_t>(src.khi, s) : 0,
+                               gld<uint16_t>(src.flags, s), shift);
             }
             // multisplit: lanes with the same digit
             unsigned long long peers = __ballot(live);
@@ -679,14 +698,14 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
             const uint32_t i = k * kPartThreads + threadIdx.x;
             const uint64_t s = ts.src_begin + i;
             const uint32_t sp = dstart[my_d[k]] + my_pos[k];
-            st_lo[sp] = src.klo[s];
-            if (wide) st_hi[sp] = src.khi[s];
-            st_fl[sp] = src.flags[s];
+            st_lo[sp] = gld<uint64_t>(src.klo, s);
+            if (wide) st_hi[sp] = gld<uint64_t>(src.khi, s);
+            st_fl[sp] = gld<uint16_t>(src.flags, s);
             st_d[sp] = (uint8_t)my_d[k];
             uint8_t *vp = st_v;
             for (int j = 0; j < nv; ++j) {
                 const int w = L->vw[j];
-                store_w(vp, w, sp, load_w(src.val[j], w, s));
+                store_w(vp, w, sp, gload_w(src.val[j], w, s));
                 vp += (size_t)w * kPartTile;
             }
         }
@@ -695,13 +714,13 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
         for (uint32_t sp = threadIdx.x; sp < ts.rows; sp += kPartThreads) {
             const uint32_t d = st_d[sp];
             const uint64_t g = (uint64_t)gbase[d] + (sp - dstart[d]);
-            dst.klo[g] = st_lo[sp];
-            if (wide) dst.khi[g] = st_hi[sp];
-            dst.flags[g] = st_fl[sp];
+            gst<uint64_t>(dst.klo, g, st_lo[sp]);
+            if (wide) gst<uint64_t>(dst.khi, g, st_hi[sp]);
+            gst<uint16_t>(dst.flags, g, st_fl[sp]);
             const uint8_t *vp = st_v;
             for (int j = 0; j < nv; ++j) {
                 const int w = L->vw[j];
-                store_w(dst.val[j], w, g, load_w(vp, w, sp));
+                gstore_w(dst.val[j], w, g, load_w(vp, w, sp));
                 vp += (size_t)w * kPartTile;
             }
         }
@@ -715,7 +734,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
 // The passes order rows by the low `sortbits` bits of the hash; a partition
 // is the top pbits of that field, so partitions are contiguous runs.
 __device__ __forceinline__ uint32_t part_of(const HLaunch *__restrict__ L, const Spill &S, bool wide, uint64_t i) {
-    const uint64_t v = spill_hash(S.klo[i], wide ? S.khi[i] : 0, S.flags[i]) & ((1ull << L->sortbits) - 1);
+    const uint64_t v = spill_hash(gld<uint64_t>(S.klo, i), wide ? gld<uint64_t>(S.khi, i) : 0, gld<uint16_t>(S.flags, i)) &
+                       ((1ull << L->sortbits) - 1);
     return (uint32_t)(v >> (L->sortbits - L->pbits));
 }
 
@@ -770,9 +790,10 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
         __syncthreads();
         const uint64_t b = s_range[0], e = p == pmask ? n : s_range[1];
         for (uint64_t i = b + tid; i < e; i += bs) {
-            const uint16_t fl = S.flags[i];
+            const uint16_t fl = gld<uint16_t>(S.flags, i);
             SpillRow row{L, &S, i, fl};
-            Key k{S.klo[i], wide ? S.khi[i] : 0, (fl & kFlagAllOnes) ? 1 : ((fl & kFlagNull) ? 2 : 0)};
+            Key k{gld<uint64_t>(S.klo, i), wide ? gld<uint64_t>(S.khi, i) : 0,
+                  (fl & kFlagAllOnes) ? 1 : ((fl & kFlagNull) ? 2 : 0)};
             int32_t slot = -1;
             uint64_t h = 0;
             if (k.special) {
@@ -1209,7 +1230,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         launch_check();
         g_fb.clean = true;
     }
-    if (n_sample > 0 && CL > 0) {
+    if (n_sample > 0) {
         KernelTimer kt("hs_sample_gather");
         hipLaunchKernelGGL(hs_sample_gather, dim3((uint32_t)ceil_div(n_sample, 256)), dim3(256), 0, R.stream, dL);
         launch_check();
@@ -1239,7 +1260,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         P_parts = pow2_floor((uint64_t)opt.partitions);
     } else {
         P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_tail / tail_cap)));
-        if (d_tail > (double)(1u << 15) * tail_cap * 1.5 && opt.method == "auto") {
+        if (d_tail > (double)(1u << 15) * tail_cap * 1.5) {
             // beyond what partitions of one LDS table can hold
             collect_kernel_times();
             (void)hipEventDestroy(ev0);

@@ -27,17 +27,21 @@ constexpr int kLdsProbe = 16;
 // global table
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void g_register(const GTable &t, uint64_t slot) {
-    uint32_t idx = atomicAdd(&t.ctrl[0], 1u);
-    if (idx < t.list_cap) t.list[idx] = (uint32_t)slot;
-    if (idx >= t.limit) atomicOr(&t.ctrl[1], 1u);
+    uint32_t idx = __hip_atomic_fetch_add(gp<uint32_t>(t.ctrl), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (idx < t.list_cap) gst<uint32_t>(t.list, idx, (uint32_t)slot);
+    if (idx >= t.limit) __hip_atomic_fetch_or(gp<uint32_t>(t.ctrl) + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ int64_t g_side_slot(const GTable &t, int special) {
     uint32_t s = (uint32_t)special - 1;
     uint64_t slot = t.mask + 1 + s;
-    if (__hip_atomic_load(&t.ctrl[2 + s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
-        atomicCAS(&t.ctrl[2 + s], 0u, 1u) == 0u)
-        g_register(t, slot);
+    PMH_G uint32_t *claim = gp<uint32_t>(t.ctrl) + 2 + s;
+    if (__hip_atomic_load(claim, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        uint32_t old = 0;
+        __hip_atomic_compare_exchange_strong(claim, &old, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 0) g_register(t, slot);
+    }
     return (int64_t)slot;
 }
 
@@ -47,19 +51,20 @@ __device__ __forceinline__ int64_t g_find64(const GTable &t, const Key &k, uint6
     if (k.special) return g_side_slot(t, k.special);
     uint64_t s = h & t.mask;
     for (uint64_t p = 0; p <= t.mask; ++p) {
-        uint64_t cur = __hip_atomic_load(&t.klo[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint64_t cur = __hip_atomic_load(gp<uint64_t>(t.klo) + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (cur == k.lo) return (int64_t)s;
         if (cur == kEmpty) {
-            unsigned long long old = atomicCAS((unsigned long long *)&t.klo[s], (unsigned long long)kEmpty,
-                                               (unsigned long long)k.lo);
+            uint64_t old = kEmpty;
+            __hip_atomic_compare_exchange_strong(gp<uint64_t>(t.klo) + s, &old, k.lo, __ATOMIC_RELAXED,
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (old == kEmpty) { g_register(t, s); return (int64_t)s; }
             if (old == k.lo) return (int64_t)s;
         }
         s = (s + 1) & t.mask;
-        if ((p & 63) == 63 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        if ((p & 63) == 63 && __hip_atomic_load(gp<uint32_t>(t.ctrl) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return -1;
     }
-    atomicOr(&t.ctrl[1], 1u);
+    __hip_atomic_fetch_or(gp<uint32_t>(t.ctrl) + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return -1;
 }
 
@@ -71,14 +76,18 @@ __device__ __forceinline__ int64_t g_find128(const GTable &t, const Key &k, uint
     uint64_t s = h & t.mask;
     uint64_t p = 0;
     while (p <= t.mask) {
-        uint32_t st = __hip_atomic_load(&t.kstate[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        PMH_G uint32_t *state = gp<uint32_t>(t.kstate) + s;
+        uint32_t st = __hip_atomic_load(state, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
         if (st == 2u) {
-            if (t.klo[s] == k.lo && t.khi[s] == k.hi) return (int64_t)s;
+            if (gld<uint64_t>(t.klo, s) == k.lo && gld<uint64_t>(t.khi, s) == k.hi) return (int64_t)s;
         } else if (st == 0u) {
-            if (atomicCAS(&t.kstate[s], 0u, 1u) == 0u) {
-                t.klo[s] = k.lo;
-                t.khi[s] = k.hi;
-                __hip_atomic_store(&t.kstate[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t old = 0;
+            __hip_atomic_compare_exchange_strong(state, &old, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) {
+                gst<uint64_t>(t.klo, s, k.lo);
+                gst<uint64_t>(t.khi, s, k.hi);
+                __hip_atomic_store(state, 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 g_register(t, s);
                 return (int64_t)s;
             }
@@ -88,10 +97,10 @@ __device__ __forceinline__ int64_t g_find128(const GTable &t, const Key &k, uint
         }
         s = (s + 1) & t.mask;
         ++p;
-        if ((p & 63) == 0 && __hip_atomic_load(&t.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        if ((p & 63) == 0 && __hip_atomic_load(gp<uint32_t>(t.ctrl) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
             return -1;
     }
-    atomicOr(&t.ctrl[1], 1u);
+    __hip_atomic_fetch_or(gp<uint32_t>(t.ctrl) + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return -1;
 }
 
@@ -171,7 +180,7 @@ struct OutCol {
 // Reads of per-group state arrays, from global memory or from LDS.
 struct GReader {
     void *const *arr;
-    template <class T> __device__ __forceinline__ T get(int a, uint64_t s) const { return ((const T *)arr[a])[s]; }
+    template <class T> __device__ __forceinline__ T get(int a, uint64_t s) const { return gld<T>(arr[a], s); }
 };
 struct LReader {
     uint8_t *base;
@@ -265,11 +274,11 @@ __device__ __forceinline__ void agg_value(const QueryDesc *__restrict__ q, const
 
 __device__ __forceinline__ void store_tag(void *data, uint64_t i, int tag, uint64_t bits) {
     switch (tag_size(tag)) {
-    case 8: ((uint64_t *)data)[i] = bits; break;
-    case 4: ((uint32_t *)data)[i] = (uint32_t)bits; break;
-    case 2: ((uint16_t *)data)[i] = (uint16_t)bits; break;
-    case 1: ((uint8_t *)data)[i] = (uint8_t)bits; break;
-    default: ((uint8_t *)data)[i] = (uint8_t)(bits & 1); break;  // Bool: a byte, packed later
+    case 8: gst<uint64_t>(data, i, bits); break;
+    case 4: gst<uint32_t>(data, i, (uint32_t)bits); break;
+    case 2: gst<uint16_t>(data, i, (uint16_t)bits); break;
+    case 1: gst<uint8_t>(data, i, (uint8_t)bits); break;
+    default: gst<uint8_t>(data, i, (uint8_t)(bits & 1)); break;  // Bool: a byte, packed later
     }
 }
 
@@ -285,7 +294,7 @@ __device__ __forceinline__ void emit_group(const QueryDesc *__restrict__ q, cons
         if (oc.src == OS_KEY) key_field(q, oc.idx, lo, hi, special, bits, valid);
         else agg_value(q, oc, rd, s, bits, valid);
         store_tag(oc.data, o, oc.tag, bits);
-        if (oc.vbytes) oc.vbytes[o] = valid ? 1 : 0;
+        if (oc.vbytes) gst<uint8_t>(oc.vbytes, o, valid ? 1 : 0);
     }
 }
 

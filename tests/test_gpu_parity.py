@@ -63,9 +63,23 @@ def test_partitioned_stats_and_fallback(native):
     assert st["hot_keys"] > 100 and st["sample_rows"] > 1000 and 0 < st["tail_rows"] < st["n_selected"]
     want = orc.groupby(plan, frame)
     assert_results_equal(got, want, ["key"], label="partitioned")
-    # tiny per-partition tables push most tail keys to the fallback table
-    got = run_gpu(native, dict(plan, options={"method": "hash", "partitions": 1, "tail_slots": 16}), frame)
-    assert_results_equal(got, want, ["key"], label="fallback")
+    # tables too small for the tail: under "auto" the fallback table
+    # overflows and the query re-runs on the global-table engine
+    got = run_gpu(native, dict(plan, options={"method": "auto", "partitions": 1, "tail_slots": 16}), frame)
+    assert_results_equal(got, want, ["key"], label="auto-rerun")
+
+
+def test_partitioned_fallback_table(native):
+    # ~2000 keys, 8 hot slots, one 16-slot partition table: most keys take
+    # the fallback global table inside the partitioned engine
+    rng = np.random.default_rng(77)
+    n = 100_000
+    frame = {"key": ("I64", rng.integers(0, 2000, n), rng.random(n) < 0.99),
+             "v": ("F64", rng.standard_normal(n), rng.random(n) < 0.9),
+             "w": ("I32", rng.integers(0, 1000, n).astype(np.int32), None)}
+    plan = {"keys": [["key", "I64"]], "aggs": synth.FIVE_AGGS,
+            "options": {"method": "hash", "lds_slots": 16, "partitions": 1, "tail_slots": 16}}
+    assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["key"], label="fallback")
 
 
 def test_resize_on_overflow(native):

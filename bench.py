@@ -43,10 +43,15 @@ def cpu_baseline(rows: int) -> dict:
                       f"sums, {res['__n_groups__']} groups, {dt:.2f} s"}
 
 
-def traffic_from_profiles() -> float | None:
-    """HBM bytes per launch of the main kernel from the committed rocprofv3
+# Row-scan kernels of the engines, in order of preference for the roofline:
+# the scan reads every input byte once and dominates the step.
+SCAN_KERNELS = ("hs_scan_fast", "hs_scan", "fgb_main_narrow")
+
+
+def traffic_from_profiles(kernel: str) -> float | None:
+    """HBM bytes per launch of the scan kernel from the committed rocprofv3
     PMC summary, when one exists."""
-    f = ROOT / "profiles" / "fgb_main_narrow_zipf100m_traffic.json"
+    f = ROOT / "profiles" / f"{kernel}_zipf100m_traffic.json"
     if not f.exists():
         return None
     try:
@@ -98,7 +103,7 @@ def main() -> int:
     kt = _native.kernel_times()
     _native.set_profiling(False)
 
-    main_name = "fgb_main_narrow"
+    main_name = next((k for k in SCAN_KERNELS if kt.get(k, (0, 0.0))[0]), SCAN_KERNELS[0])
     launches, ms = kt.get(main_name, (0, 0.0))
     avg_ms = ms / launches if launches else float("nan")
     alg_bytes = BYTES_PER_ROW * n
@@ -126,7 +131,7 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": traffic_from_profiles(),
+                     "traffic": traffic_from_profiles(main_name),
                      "kernel": main_name, "kernel_avg_ms": avg_ms,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "kernel_ms": {k: v[1] / v[0] for k, v in kt.items() if v[0]},

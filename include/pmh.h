@@ -107,6 +107,8 @@ typedef struct pmh_exec_stats {
     uint32_t partitions;      /* tail partitions */
     uint64_t tail_rows;       /* selected rows not covered by the hot keys */
     double distinct_estimate; /* distinct keys the engine sized its partitions for */
+    int32_t scan_kernel;      /* partitioned engine's row scan: 0 generic, 1 specialised */
+    int32_t reserved;
 } pmh_exec_stats;
 
 typedef struct pmh_result {

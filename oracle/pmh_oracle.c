@@ -29,7 +29,10 @@
  * Geometric Predicates", 1997) and the half-even final correction of
  * R. Hettinger's msum recipe -- the method behind Python's math.fsum.  A
  * second mode (sum_mode = 1) sums sequentially in row order, which is what
- * a one-thread run of the reference's fold computes.
+ * a one-thread run of the reference's fold computes.  Either way a zero
+ * sum is +0.0, as the fold's +0.0 identity makes it.  f64 min/max fold in
+ * row order with a strict comparison, so a zero extreme keeps the sign of
+ * the group's first zero.
  *
  * Parity pins (tests/test_oracle.py): predicate truth tables vs the
  * reference's exhaustive Kleene table and TotalOrd examples; compaction vs a
@@ -428,7 +431,14 @@ orc_gb *orc_groupby(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_
             }
         }
         if (xs) {
-            for (uint64_t i = 0; i < ng; i++) { g->sum_f[j][i] = xsum_result(&xs[i]); xsum_free(&xs[i]); }
+            /* The reference's fold starts every group at +0.0 (groupby.rs:1304,
+             * 1318), so a group whose values are all -0.0 sums to +0.0, not to
+             * the -0.0 an fsum-style sum keeps. */
+            for (uint64_t i = 0; i < ng; i++) {
+                const double r = xsum_result(&xs[i]);
+                g->sum_f[j][i] = r == 0.0 ? 0.0 : r;
+                xsum_free(&xs[i]);
+            }
             free(xs);
         }
     }

@@ -48,6 +48,16 @@ struct Spill {
     uint16_t *flags;      // bit j: value j valid; kFlagAllOnes / kFlagNull: side keys
 };
 
+// Host-lowered form of a conjunctive predicate for the specialised scan: a
+// row is selected iff, for every r < nr, column rcol[r] is valid and its
+// int64 image (the value itself, or its TotalOrd image when rflt[r]) lies in
+// [rlo[r], rhi[r]].
+struct FastScan {
+    int32_t enabled, nr;
+    int32_t rcol[2], rflt[2];
+    int64_t rlo[2], rhi[2];
+};
+
 enum StatIdx : int32_t {
     ST_SELECTED = 0, ST_TAIL = 1, ST_SAMPLE_SEL = 2, ST_SAMPLE_DISTINCT = 3, ST_SAMPLE_F1 = 4,
     ST_SAMPLE_F2 = 5, ST_HOT = 6, ST_HOT_SAMPLE_ROWS = 7, ST_TILES0 = 8, ST_SAMPLE_NULLKEY = 9,
@@ -89,6 +99,7 @@ struct HLaunch {
     uint32_t *out_cursor;  // [0] groups written by hs_hot_emit and hs_part_agg
     uint32_t *stats;       // StatIdx
     uint32_t *sketch_hist; // [1024]: sampled keys seen c times (c clipped at 1023)
+    FastScan fs;
 };
 
 // Digit of a spilled key for partitioning: side keys go to partition 0.
@@ -473,6 +484,10 @@ __global__ __launch_bounds__(1024) void hs_scan(const HLaunch *__restrict__ L) {
         merge_lds_entry(g, i, c.ls, i, q->layout, q->layout.init);
     }
 }
+
+}  // namespace pmh
+#include "pmh_scanfast.h"
+namespace pmh {
 
 // Prefix of tail counts per region and of first-pass tiles per region.
 __global__ __launch_bounds__(1024) void hs_regions(const HLaunch *__restrict__ L) {
@@ -1082,6 +1097,78 @@ double plain_distinct_estimate(const Sketch &k) {
 
 namespace {
 
+// Lower the predicate for hs_scan_fast: an AND tree whose leaves compare one
+// column with a literal (any operator but Ne), test a Boolean column, or are
+// the literal true.  Leaves on the same column and compare type intersect.
+// Returns false when the predicate has another shape.
+bool lower_fast_scan(const QueryDesc &q, FastScan &fs) {
+    fs = FastScan{};
+    if (q.n_pred == 0) return true;
+    const int64_t MN = INT64_MIN, MX = INT64_MAX;
+    auto add = [&](int col, int flt, int64_t lo, int64_t hi) {
+        for (int r = 0; r < fs.nr; ++r)
+            if (fs.rcol[r] == col && fs.rflt[r] == flt) {
+                fs.rlo[r] = std::max(fs.rlo[r], lo);
+                fs.rhi[r] = std::min(fs.rhi[r], hi);
+                return true;
+            }
+        if (fs.nr == 2) return false;
+        fs.rcol[fs.nr] = col;
+        fs.rflt[fs.nr] = flt;
+        fs.rlo[fs.nr] = lo;
+        fs.rhi[fs.nr] = hi;
+        ++fs.nr;
+        return true;
+    };
+    std::vector<int> stack{q.n_pred - 1};
+    while (!stack.empty()) {
+        const PredNode &nd = q.pred[stack.back()];
+        stack.pop_back();
+        switch (nd.kind) {
+        case PK_AND: stack.push_back(nd.lhs); stack.push_back(nd.rhs); break;
+        case PK_CONST: if (!nd.cval) return false; break;
+        case PK_COLBOOL:
+            if (q.cols[nd.lhs].tag != PMH_BOOL || !add(nd.lhs, 0, 1, 1)) return false;
+            break;
+        case PK_CMP: {
+            const bool lcol = nd.lkind == OK_COL, rcol = nd.rkind == OK_COL;
+            if (lcol == rcol) return false;
+            int op = nd.op;
+            if (!lcol) op = op == PMH_LT ? PMH_GT : op == PMH_GT ? PMH_LT : op == PMH_LE ? PMH_GE : op == PMH_GE ? PMH_LE : op;
+            const int col = lcol ? nd.lhs : nd.rhs;
+            int64_t k;
+            if (nd.is_f64) {
+                const double x = lcol ? nd.rf : nd.lf;
+                int64_t b;
+                memcpy(&b, &x, 8);
+                k = tot_key_bits(b, std::isnan(x));
+            } else {
+                k = lcol ? nd.ri : nd.li;
+            }
+            int64_t lo, hi;
+            switch (op) {
+            case PMH_EQ: lo = hi = k; break;
+            case PMH_LT:
+                if (k == MN) { lo = MX; hi = MN; }  // empty
+                else { lo = MN; hi = k - 1; }
+                break;
+            case PMH_LE: lo = MN; hi = k; break;
+            case PMH_GT:
+                if (k == MX) { lo = MX; hi = MN; }  // empty
+                else { lo = k + 1; hi = MX; }
+                break;
+            case PMH_GE: lo = k; hi = MX; break;
+            default: return false;
+            }
+            if (!add(col, nd.is_f64 ? 1 : 0, lo, hi)) return false;
+            break;
+        }
+        default: return false;
+        }
+    }
+    return true;
+}
+
 void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, const int32_t *vw, int nv) {
     S.klo = (uint64_t *)slab(tag + ":klo", cap * 8 + 64);
     S.khi = wide ? (uint64_t *)slab(tag + ":khi", cap * 8 + 64) : nullptr;
@@ -1108,6 +1195,11 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.q = q0;
     L.wide = wide;
     L.n_out = P.n_out;
+    if (opt.fast_scan != 0 && !wide && q0.n_cols <= 4 && nv <= 2 && q0.n_keys <= 4 &&
+        (q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64 || q0.key_mode == KM_NONE) && lower_fast_scan(q0, L.fs))
+        L.fs.enabled = 1;
+    else
+        L.fs = FastScan{};
     size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
     for (int j = 0; j < nv; ++j) {
         L.vw[j] = tag_size(q0.vals[j].tag);
@@ -1287,12 +1379,21 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     {
         const bool cached = q0.n_cols <= 4;
-        KernelTimer kt("hs_scan");
         const size_t l1 = std::max<size_t>(lds1, 16);
-        const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;
-        PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
-        if (cached) hipLaunchKernelGGL(hs_scan<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
-        else hipLaunchKernelGGL(hs_scan<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        if (L.fs.enabled) {
+            KernelTimer kt("hs_scan_fast");
+            const bool raw = q0.key_mode == KM_RAW64;
+            const void *fn = raw ? (const void *)hs_scan_fast<true> : (const void *)hs_scan_fast<false>;
+            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+            if (raw) hipLaunchKernelGGL(hs_scan_fast<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
+            else hipLaunchKernelGGL(hs_scan_fast<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        } else {
+            KernelTimer kt("hs_scan");
+            const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;
+            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+            if (cached) hipLaunchKernelGGL(hs_scan<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
+            else hipLaunchKernelGGL(hs_scan<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        }
         launch_check();
     }
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
@@ -1388,6 +1489,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     res->stats.partitions = (uint32_t)P_parts;
     res->stats.tail_rows = host_stats[ST_TAIL];
     res->stats.distinct_estimate = d_est;
+    res->stats.scan_kernel = L.fs.enabled;
     const uint64_t vbytes = bitmap_bytes_padded(ng);
     const uint64_t packed = (ng + 7) / 8;
     for (int oi = 0; oi < P.n_out; ++oi) {

@@ -166,6 +166,7 @@ struct Options {
     int64_t tail_slots = 0;       // partitioned engine: per-partition LDS table slots, 0: auto
     int64_t sample = -1;          // partitioned engine: sampled rows for the hot set, -1: auto
     int64_t distinct_hint = 0;    // partitioned engine: expected distinct keys (a sizer's prediction), 0: sketch
+    int64_t fast_scan = 1;        // partitioned engine: 1 use the specialised scan where the query fits it, 0 never
 };
 
 struct GroupByPlan {

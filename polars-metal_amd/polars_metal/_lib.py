@@ -69,6 +69,8 @@ class PmhExecStats(ctypes.Structure):
         ("partitions", ctypes.c_uint32),
         ("tail_rows", ctypes.c_uint64),
         ("distinct_estimate", ctypes.c_double),
+        ("scan_kernel", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

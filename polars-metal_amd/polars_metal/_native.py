@@ -162,6 +162,7 @@ def _stats_dict(st: _lib.PmhExecStats) -> dict:
         "partitions": st.partitions,
         "tail_rows": st.tail_rows,
         "distinct_estimate": st.distinct_estimate,
+        "scan_kernel": "specialised" if st.scan_kernel == 1 else "generic",
     }
 
 

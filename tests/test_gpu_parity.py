@@ -214,6 +214,108 @@ def test_predicate_shapes(native):
         assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["k"], label=f"pred{i}")
 
 
+I64_MIN, I64_MAX = -(1 << 63), (1 << 63) - 1
+
+
+def _fast_scan_frame():
+    rng = np.random.default_rng(9)
+    n = 70_000
+    x = rng.standard_normal(n)
+    x[::37] = np.nan
+    x[::41] = 0.0
+    x[::43] = -0.0
+    x[::47] = np.inf
+    x[::53] = -np.inf
+    key = rng.integers(0, 400, n)
+    key[::101] = -1  # the all-ones key lane
+    return {"k": ("I64", key, rng.random(n) < 0.98),
+            "x": ("F64", x, rng.random(n) < 0.9),
+            "y": ("I16", rng.integers(-100, 100, n).astype(np.int16), rng.random(n) < 0.9),
+            "f": ("Bool", rng.random(n) < 0.5, rng.random(n) < 0.9)}
+
+
+def _cmp(op, lhs, rhs, dtype):
+    return {"kind": "Compare", "op": op, "dtype": dtype, "lhs": lhs, "rhs": rhs}
+
+
+def _c(name, dtype):
+    return {"kind": "Column", "name": name, "dtype": dtype}
+
+
+def _lf(v):
+    return {"kind": "LiteralF64", "value": v}
+
+
+def _li(v):
+    return {"kind": "LiteralI64", "value": v}
+
+
+def _and(a, b):
+    return {"kind": "And", "lhs": a, "rhs": b}
+
+
+# (predicate, lowered to intervals for the specialised scan)
+FAST_SCAN_PREDS = [
+    (_cmp("Gt", _c("x", "F64"), _lf(float("nan")), "F64"), True),    # nothing sorts above NaN
+    (_cmp("Lt", _c("x", "F64"), _lf(float("nan")), "F64"), True),    # every non-NaN value
+    (_cmp("Ge", _c("x", "F64"), _lf(float("nan")), "F64"), True),    # NaN only
+    (_cmp("Ge", _c("x", "F64"), _lf(-0.0), "F64"), True),            # both zeros and up
+    (_cmp("Eq", _c("x", "F64"), _lf(0.0), "F64"), True),             # -0.0 == +0.0
+    (_cmp("Lt", _c("x", "F64"), _lf(float("-inf")), "F64"), True),   # empty
+    (_cmp("Le", _c("x", "F64"), _lf(float("inf")), "F64"), True),    # all but NaN
+    (_cmp("Lt", _c("y", "I64"), _li(I64_MIN), "I64"), True),        # empty
+    (_cmp("Gt", _c("y", "I64"), _li(I64_MAX), "I64"), True),        # empty
+    (_cmp("Ge", _li(5), _c("y", "I64"), "I64"), True),              # literal on the left
+    (_cmp("Gt", _c("y", "F64"), _lf(2.5), "F64"), True),            # integer column, float compare
+    (_and(_and(_cmp("Gt", _c("x", "F64"), _lf(-1.0), "F64"), _cmp("Le", _c("x", "F64"), _lf(1.0), "F64")),
+          _and(_cmp("Ge", _c("y", "I64"), _li(-50), "I64"), _cmp("Lt", _c("y", "I64"), _li(50), "I64"))), True),
+    (_and(_c("f", "Bool"), _cmp("Gt", _c("x", "F64"), _lf(0.0), "F64")), True),
+    (_and(_cmp("Ge", _c("y", "I64"), _li(10), "I64"), _cmp("Ge", _c("y", "I64"), _li(20), "I64")), True),
+    ({"kind": "LiteralBool", "value": True}, True),
+    (_and(_and(_c("f", "Bool"), _cmp("Gt", _c("x", "F64"), _lf(0.0), "F64")),
+          _cmp("Lt", _c("y", "I64"), _li(0), "I64")), False),            # three columns
+    (_cmp("Ne", _c("y", "I64"), _li(3), "I64"), False),
+]
+
+
+@pytest.mark.parametrize("i", range(len(FAST_SCAN_PREDS)))
+def test_fast_scan_predicates(native, i):
+    pred, lowered = FAST_SCAN_PREDS[i]
+    frame = _fast_scan_frame()
+    plan = {"keys": [["k", "I64"]], "predicate": pred,
+            "aggs": [{"kind": "Simple", "input_col": "x", "op": op, "output_alias": op} for op in
+                     ("Sum", "Count", "Min", "Max", "Mean")] + [{"kind": "Length", "op": "Len", "output_alias": "n"}]}
+    want = orc.groupby(plan, frame)
+    print(f"fast-scan predicate {i}", flush=True)
+    for fast in (1, 0):
+        p = dict(plan, options={"method": "hash", "fast_scan": fast, "lds_slots": 64})
+        got = run_gpu(native, p, frame)
+        assert native.last_stats()["scan_kernel"] == ("specialised" if fast and lowered else "generic")
+        assert_results_equal(got, want, ["k"], label=f"pred{i}/fast{fast}", zero_sign_free=("Min", "Max"))
+
+
+@pytest.mark.parametrize("keys", [["a"], ["a", "b"], ["b", "c"]])
+def test_fast_scan_lane_keys(native, keys):
+    # narrow keys build a lane of at most 64 bits: the specialised scan's
+    # lane path, against the generic scan and the oracle
+    rng = np.random.default_rng(10)
+    n = 90_000
+    frame = {"a": ("I32", rng.integers(-30, 30, n).astype(np.int32), rng.random(n) < 0.95),
+             "b": ("U8", rng.integers(0, 256, n).astype(np.uint8), rng.random(n) < 0.95),
+             "c": ("I16", np.full(n, -1, np.int16), None),
+             "v": ("I64", rng.integers(-1 << 40, 1 << 40, n), rng.random(n) < 0.9),
+             "w": ("I32", rng.integers(0, 1000, n).astype(np.int32), None)}
+    plan = {"keys": [[k, frame[k][0]] for k in keys],
+            "predicate": _cmp("Lt", _c("w", "I64"), _li(700), "I64"),
+            "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in
+                     ("Sum", "Count", "Min", "Max", "Mean")]}
+    want = orc.groupby(plan, frame)
+    for fast in (1, 0):
+        got = run_gpu(native, dict(plan, options={"method": "hash", "fast_scan": fast}), frame)
+        assert native.last_stats()["scan_kernel"] == ("specialised" if fast else "generic")
+        assert_results_equal(got, want, keys, label=f"{keys}/fast{fast}")
+
+
 def test_deterministic_bits(native):
     frame, plan = synth.zipf(500_000, seed=21)
     wire = orc.to_wire(frame)

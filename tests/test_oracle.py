@@ -130,6 +130,25 @@ def test_group_identity_vs_numpy_unique():
         assert sums[i + 1] == math.fsum(v[kv & (k == x)].tolist())
 
 
+def test_signed_zero_sum_and_min_max():
+    # The reference folds f64 sums from +0.0 (groupby.rs:1303-1321): an
+    # all -0.0 group sums to +0.0 in any fold order.  Min/max fold in row
+    # order with a strict comparison (groupby.rs:1471-1520): a zero extreme
+    # keeps the sign of the group's first zero.
+    v = np.array([-0.0, -0.0, 0.0, -0.0, -0.0, 1.0, -1.0])
+    k = np.array([0, 0, 1, 1, 2, 2, 3])
+    frame = {"k": ("I64", k, None), "v": ("F64", v, None)}
+    plan = {"keys": [["k", "I64"]],
+            "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in ("Sum", "Min", "Max")]}
+    for mode in (0, 1):
+        got = orc.canonical(orc.groupby(plan, frame, sum_mode=mode), ["k"])
+        s, mn, mx = got["Sum"][1], got["Min"][1], got["Max"][1]
+        assert [math.copysign(1.0, x) for x in s[:2]] == [1.0, 1.0], mode   # +0.0 sums
+        assert math.copysign(1.0, mn[0]) == -1.0 and math.copysign(1.0, mx[0]) == -1.0
+        assert math.copysign(1.0, mn[1]) == 1.0 and math.copysign(1.0, mx[1]) == 1.0  # first zero is +0.0
+        assert mn[2] == -0.0 and mx[2] == 1.0 and mn[3] == -1.0
+
+
 def test_key_encoding_widths():
     # crates/polars-metal-kernels/tests/test_key_encoding.rs: i64 = 65 bits,
     # i64 + i32 = 98 bits, two i64 + bool... exceed 128 -> TooWide.

@@ -56,6 +56,7 @@ struct FastScan {
     int32_t enabled, nr;
     int32_t rcol[2], rflt[2];
     int64_t rlo[2], rhi[2];
+    uint32_t ring_off, pad;  // byte offset of the per-wave row rings in LDS
 };
 
 enum StatIdx : int32_t {
@@ -1195,7 +1196,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.q = q0;
     L.wide = wide;
     L.n_out = P.n_out;
-    if (opt.fast_scan != 0 && !wide && q0.n_cols <= 4 && nv <= 2 && q0.n_keys <= 4 &&
+    if (opt.fast_scan != 0 && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 32) &&
         (q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64 || q0.key_mode == KM_NONE) && lower_fast_scan(q0, L.fs))
         L.fs.enabled = 1;
     else
@@ -1216,7 +1217,11 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     uint32_t nb = (uint32_t)std::min<uint64_t>(cus, std::max<uint64_t>(1, ceil_div(n, 16384)));
     const uint64_t chunk = std::max<uint64_t>(1, ceil_div(n, nb));
     nb = (uint32_t)std::max<uint64_t>(1, ceil_div(n, chunk));
-    uint32_t CL = (uint32_t)pow2_floor(lds_budget / (key_bytes + state_per_slot));
+    // the specialised scan keeps a row ring per wave next to the hot table
+    // and has only 8 bytes of static LDS
+    const size_t ring_bytes = (size_t)16 * kRing * 4;
+    const size_t hot_budget = L.fs.enabled ? lds_max - 64 - ring_bytes : lds_budget;
+    uint32_t CL = (uint32_t)pow2_floor(hot_budget / (key_bytes + state_per_slot));
     CL = std::min<uint32_t>(CL, 2048);
     if (opt.lds_slots >= 0) CL = opt.lds_slots == 0 ? 0 : (uint32_t)std::min<uint64_t>(CL, pow2_floor(opt.lds_slots));
     if (CL < 16) CL = 0;
@@ -1252,9 +1257,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         size_t o = 0;
         carve_state(P, hb, o, CL, L.hot_arr);
     }
-    const size_t lds1 = (size_t)CL * key_bytes + lds_layout(P, CL, L.lds1_off);
+    size_t lds1 = (size_t)CL * key_bytes + lds_layout(P, CL, L.lds1_off);
     const size_t lds3 = (size_t)(TS + 2) * (key_bytes + (wide ? 4 : 0)) + lds_layout(P, TS + 2, L.lds3_off);
-    if (lds1 > lds_budget || lds3 > lds_budget) return nullptr;
+    if (L.fs.enabled) {
+        L.fs.ring_off = (uint32_t)((lds1 + 15) & ~size_t(15));
+        lds1 = L.fs.ring_off + ring_bytes;
+        if (lds1 > hot_budget + ring_bytes) return nullptr;
+    } else if (lds1 > lds_budget) {
+        return nullptr;
+    }
+    if (lds3 > lds_budget) return nullptr;
     L.tail_n = (uint32_t *)slab("hs:tail_n", (nb + 1) * 4 + 64);
     L.region_off = (uint32_t *)slab("hs:region_off", (nb + 1) * 4 + 64);
     L.tile_off = (uint32_t *)slab("hs:tile_off", (nb + 1) * 4 + 64);

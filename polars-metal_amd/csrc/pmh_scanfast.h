@@ -1,19 +1,31 @@
 // pmh_scanfast.h -- specialised phase-1 scan of the partitioned hash engine.
 //
-// Included by pmh_hashagg.hip after HLaunch, ScanCtx and the spill helpers.
-// It serves the common query shape: the predicate is a conjunction of
-// column-vs-literal comparisons (lowered on the host to at most two closed
-// intervals on an order-preserving int64 image of the column), the key is
-// one 64-bit column or a lane of at most 64 bits over at most four columns,
-// and at most two columns are aggregated.  The generic scan re-reads the
-// query descriptor with scalar loads for every row; each of those waits
-// (s_waitcnt lgkmcnt) also drains the block's outstanding LDS atomics.  Here
-// every descriptor field, LDS array address and spill pointer is read once
-// into registers before the row loop, rows are handled four per lane per
-// step, and one LDS cursor reservation covers the tail rows of all four.
+// Included by pmh_hashagg.hip after HLaunch, the spill helpers and the
+// generic scan.  It serves the common query shape: the predicate is a
+// conjunction of column-vs-literal comparisons (lowered on the host to at
+// most two closed intervals on an order-preserving int64 image of a column),
+// the key is one 64-bit column or a lane of at most 64 bits over at most four
+// columns, and at most two columns are aggregated.
+//
+// Two ideas make it cheaper than the generic scan:
+//  * every descriptor field, LDS array address and spill pointer is read
+//    into registers once per block, and tag dispatch happens once per column
+//    per step, not per row;
+//  * rows are compacted before the per-row work.  Each wave evaluates the
+//    predicate on coalesced loads of the predicate columns only, queues the
+//    offsets of the selected rows in a per-wave LDS ring, and whenever 64 are
+//    queued runs one dense round over them: gather the key and value columns
+//    for those rows, hash, probe the hot table, fold hot rows into LDS and
+//    append the rest to the block's spill region.  At 30% selectivity that
+//    removes three quarters of the idle lanes from the expensive part, and
+//    key/value cache lines holding no selected row are never fetched.
 #pragma once
 
 namespace pmh {
+
+// Per-wave ring of selected-row offsets: at most 63 carried rows plus one
+// step of 64 new ones are queued at any time.
+constexpr uint32_t kRing = 127;
 
 // Order-preserving int64 image of a double under Polars' TotalOrd (the order
 // of tot_cmp): -0.0 and +0.0 map together, every NaN maps above +inf.
@@ -24,17 +36,20 @@ __host__ __device__ inline int64_t tot_key_bits(int64_t b, bool is_nan) {
 }
 __device__ __forceinline__ int64_t tot_key(double x) { return tot_key_bits(__double_as_longlong(x), x != x); }
 
-// Exact-sum sink over two LDS arrays addressed directly (limb k of slot s is
-// limb[k * CL + s]).
-struct FSink {
-    unsigned long long *limb;
-    double *res;
-    uint32_t CL;
-    __device__ __forceinline__ void addf(int, uint32_t s, double v) { unsafeAtomicAdd(res + s, v); }
-    __device__ __forceinline__ void addlimb(int, int k, uint32_t s, int64_t v) {
-        atomicAdd(limb + (uint32_t)k * CL + s, (unsigned long long)v);
-    }
+// A column as one role of the query sees it.
+struct RoleCol {
+    const void *data;
+    const uint8_t *valid;
+    int32_t tag, w;  // w: bytes per value, 0 for bit-packed Boolean
 };
+__device__ __forceinline__ RoleCol role_col(const QueryDesc *__restrict__ q, int c) {
+    RoleCol r;
+    r.data = q->cols[c].data;
+    r.valid = q->cols[c].valid;
+    r.tag = q->cols[c].tag;
+    r.w = r.tag == PMH_BOOL ? 0 : tag_size(r.tag);
+    return r;
+}
 
 __device__ __forceinline__ uint64_t load_bits_w(const void *p, int w, uint64_t r) {
     switch (w) {
@@ -46,6 +61,84 @@ __device__ __forceinline__ uint64_t load_bits_w(const void *p, int w, uint64_t r
     }
 }
 
+// Predicate images of U values of one column (raw bits, zero-extended): the
+// column's TotalOrd image when flt, else its int64 value.  The tag switch is
+// taken once for all U values.
+template <int U>
+__device__ __forceinline__ void pred_images(int tag, bool flt, const uint64_t (&raw)[U], int64_t (&img)[U]) {
+    if (flt) {
+        if (tag == PMH_F64) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) img[u] = tot_key(__longlong_as_double((long long)raw[u]));
+        } else if (tag == PMH_F32) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) img[u] = tot_key((double)__uint_as_float((uint32_t)raw[u]));
+        } else {
+#pragma unroll
+            for (int u = 0; u < U; ++u) img[u] = tot_key((double)raw_to_i64(tag, raw[u]));
+        }
+        return;
+    }
+    switch (tag) {
+    case PMH_I32:
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = (int32_t)(uint32_t)raw[u];
+        break;
+    case PMH_I16:
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = (int16_t)(uint16_t)raw[u];
+        break;
+    case PMH_I8:
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = (int8_t)(uint8_t)raw[u];
+        break;
+    case PMH_F64:
+    case PMH_F32:
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = raw_to_i64(tag, raw[u]);
+        break;
+    default:  // I64, U64, U32, U16, U8, Utf8 codes, Bool: the raw bits
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = (int64_t)raw[u];
+        break;
+    }
+}
+
+// Exact f64 accumulation into LDS limbs (limb k of slot s at limb[k * SL + s])
+// with one divergent region only for the rare values outside the fixed-point
+// window (and Inf/NaN), which go to the residual.
+__device__ __forceinline__ void lds_add_f64(unsigned long long *limb, double *res, uint32_t SL, uint32_t s, double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
+    uint64_t m = b & 0xFFFFFFFFFFFFFull;
+    if (E) m |= 1ull << 52;
+    const int e = E ? (int)E - 1075 : -1074;
+    // in the window: every set bit of m * 2^e in [2^-kFixScale, 2^(32*kLimbs-kFixScale))
+    const int sh0 = -kFixScale - e;  // > 0: low bits that must be zero
+    const bool low_ok = m == 0 || sh0 <= 0 || (sh0 < 64 && (m & ((1ull << sh0) - 1)) == 0);
+    const uint64_t mm = sh0 > 0 ? (sh0 < 64 ? m >> sh0 : 0) : m;
+    const int p = sh0 > 0 ? 0 : e + kFixScale;
+    const int msb = mm ? p + 63 - __builtin_clzll(mm) : 0;
+    const bool in_window = E != 0x7FFu && low_ok && msb < 32 * kLimbs;
+    if (in_window) {
+        if (mm) {
+            const int li = p >> 5, sh = p & 31;
+            const uint64_t lo = mm << sh;
+            const uint64_t hi = sh ? (mm >> (64 - sh)) : 0;
+            int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
+            if (b >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
+            // a chunk above the top limb is zero (msb < 32 * kLimbs): clamp
+            // its limb index and add the zero
+            const int l1 = min(li + 1, kLimbs - 1), l2 = min(li + 2, kLimbs - 1);
+            atomicAdd(limb + (uint32_t)li * SL + s, (unsigned long long)c0);
+            atomicAdd(limb + (uint32_t)l1 * SL + s, (unsigned long long)c1);
+            atomicAdd(limb + (uint32_t)l2 * SL + s, (unsigned long long)c2);
+        }
+    } else {
+        unsafeAtomicAdd(res + s, x);
+    }
+}
+
 template <bool RAW64>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -53,49 +146,39 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     const QueryDesc *__restrict__ q = &L->q;
     const uint32_t CL = L->hot_slots;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
-    const int lane = (int)(tid & 63);
+    const uint32_t lane = tid & 63;
+    const unsigned long long below = (1ull << lane) - 1;
     uint64_t *lkeys = (uint64_t *)smem;
     uint8_t *state = (uint8_t *)(lkeys + CL);
+    uint32_t *ring = (uint32_t *)(smem + L->fs.ring_off) + (tid >> 6) * kRing;
 
     // ---- descriptor, once per block
-    const int ncols = q->n_cols;
-    const void *cdat[4];
-    const uint8_t *cval[4];
-    int ctag[4], cw[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const bool in = k < ncols;
-        cdat[k] = in ? q->cols[k].data : nullptr;
-        cval[k] = in ? q->cols[k].valid : nullptr;
-        ctag[k] = in ? q->cols[k].tag : PMH_I64;
-        cw[k] = ctag[k] == PMH_BOOL ? 0 : tag_size(ctag[k]);
-    }
     const int nr = L->fs.nr;
-    int rcol[2], rflt[2], rtag[2];
-    int64_t rlo[2], rhi[2];
+    RoleCol pc[2];
+    int pflt[2];
+    int64_t plo[2], phi[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        rcol[r] = L->fs.rcol[r];
-        rflt[r] = L->fs.rflt[r];
-        rlo[r] = L->fs.rlo[r];
-        rhi[r] = L->fs.rhi[r];
-        rtag[r] = rcol[r] == 0 ? ctag[0] : rcol[r] == 1 ? ctag[1] : rcol[r] == 2 ? ctag[2] : ctag[3];
+        pc[r] = role_col(q, r < nr ? L->fs.rcol[r] : 0);
+        pflt[r] = L->fs.rflt[r];
+        plo[r] = L->fs.rlo[r];
+        phi[r] = L->fs.rhi[r];
     }
-    // key: one raw column, or up to four lane fields
     const int nk = RAW64 ? 1 : (q->key_mode == KM_NONE ? 0 : q->n_keys);
     const bool allones = q->allones_possible != 0;
-    int kcol[4], knull[4], kdata[4];
+    RoleCol kc[4];
+    int knull[4], kdata[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        kcol[i] = i < nk ? q->keys[i].col : 0;
+        kc[i] = role_col(q, i < nk ? q->keys[i].col : 0);
         knull[i] = i < nk ? q->keys[i].null_off : 0;
         kdata[i] = i < nk ? q->keys[i].data_off : 0;
     }
-    // aggregated columns and their LDS arrays
     const int nv = q->n_vals;
     auto lds_at = [&](int a) -> uint8_t * { return a >= 0 ? state + L->lds1_off[a] : state; };
     uint32_t *len_p = (uint32_t *)lds_at(q->layout.a_len);
-    int vcol[2], vtag[2], vkind[2], vneed[2], vwid[2];
+    RoleCol vc[2];
+    int vkind[2], vneed[2], vwid[2];
     uint32_t *cnt_p[2], *nan_p[2];
     unsigned long long *limb_p[2], *sum_p[2];
     double *res_p[2];
@@ -105,8 +188,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     for (int j = 0; j < 2; ++j) {
         const bool in = j < nv;
         const ValSpec &S = q->vals[in ? j : 0];
-        vcol[j] = in ? S.col : 0;
-        vtag[j] = in ? S.tag : PMH_I64;
+        vc[j] = role_col(q, in ? S.col : 0);
         vkind[j] = in ? S.kind : VK_I64;
         vneed[j] = in ? S.need : 0;
         vwid[j] = in ? L->vw[j] : 8;
@@ -129,17 +211,15 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     // ---- LDS: hot keys and state identities
     if (tid == 0) { s_cursor = 0; s_sel = 0; }
     for (uint32_t i = tid; i < CL; i += bs) lkeys[i] = L->hot_lo[i];
-    {
-        LSink ls{state, L->lds1_off, CL};
-        for (int a = 0; a < q->layout.n_arrs; ++a) {
-            const int op = q->layout.init[a];
-            if (mo_size(op) == 4) {
-                const uint32_t id = (uint32_t)mo_identity(op);
-                for (uint32_t i = tid; i < CL; i += bs) *ls.p<uint32_t>(a, i) = id;
-            } else {
-                const uint64_t id = mo_identity(op);
-                for (uint32_t i = tid; i < CL; i += bs) *ls.p<uint64_t>(a, i) = id;
-            }
+    LSink ls{state, L->lds1_off, CL};
+    for (int a = 0; a < q->layout.n_arrs; ++a) {
+        const int op = q->layout.init[a];
+        if (mo_size(op) == 4) {
+            const uint32_t id = (uint32_t)mo_identity(op);
+            for (uint32_t i = tid; i < CL; i += bs) *ls.p<uint32_t>(a, i) = id;
+        } else {
+            const uint64_t id = mo_identity(op);
+            for (uint32_t i = tid; i < CL; i += bs) *ls.p<uint64_t>(a, i) = id;
         }
     }
     __syncthreads();
@@ -148,176 +228,177 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     const uint64_t r_begin = region;
     const uint64_t r_end = min(n, r_begin + L->chunk);
     const uint64_t span = r_end > r_begin ? r_end - r_begin : 0;
-    // every lane runs the same number of steps: the ballots below always see
+    constexpr int U = 4;
+    // every lane of a wave runs the same steps and rounds, so ballots see
     // whole waves
-    const uint64_t steps = (span + 4ull * bs - 1) / (4ull * bs);
-    uint32_t sel_count = 0;
+    const uint64_t steps = (span + (uint64_t)U * bs - 1) / ((uint64_t)U * bs);
+    uint32_t qhead = 0, qn = 0;  // wave-uniform ring state
+    uint32_t wave_sel = 0;
 
-    for (uint64_t it = 0; it < steps; ++it) {
-        const uint64_t r0 = r_begin + it * 4ull * bs + tid;
-        bool live[4];
+    for (uint64_t it = 0; it <= steps; ++it) {
+        const bool drain = it == steps;
+        uint32_t selbits = 0;  // bit u: row u of this lane passes
+        const uint64_t r0 = r_begin + it * (uint64_t)U * bs + tid;
+        if (!drain) {
+            bool live[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) live[u] = r0 + (uint64_t)u * bs < r_end;
-        // ---- loads, column by column so each width switch is paid once
-        uint64_t v[4][4];
-        uint32_t vm[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u][k] = 0;
-            if (k >= ncols) continue;
-            const int w = cw[k];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (live[u]) v[u][k] = load_bits_w(cdat[k], w, r0 + (uint64_t)u * bs);
-            if (cval[k]) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    if (live[u]) vm[u] |= (uint32_t)dvalid(cval[k], r0 + (uint64_t)u * bs) << k;
-            } else {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) vm[u] |= 1u << k;
+            for (int u = 0; u < U; ++u) {
+                live[u] = r0 + (uint64_t)u * bs < r_end;
+                selbits |= (uint32_t)live[u] << u;
             }
-        }
-        auto pick = [&](int u, int k) -> uint64_t {
-            return k == 0 ? v[u][0] : (k == 1 ? v[u][1] : (k == 2 ? v[u][2] : v[u][3]));
-        };
-        // ---- predicate: interval tests
-        bool sel[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            bool ok = live[u];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 if (r >= nr) continue;
-                const uint64_t raw = pick(u, rcol[r]);
-                const int64_t key = rflt[r] ? tot_key(raw_to_f64(rtag[r], raw)) : raw_to_i64(rtag[r], raw);
-                ok = ok && ((vm[u] >> rcol[r]) & 1) && key >= rlo[r] && key <= rhi[r];
-            }
-            sel[u] = ok;
-            sel_count += ok ? 1u : 0u;
-        }
-        // ---- keys and hot-table probes
-        uint64_t key[4];
-        int special[4];
-        int32_t slot[4];
+                uint64_t raw[U];
+                uint32_t vb = 0;
+                const int w = pc[r].w;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            key[u] = 0;
-            special[u] = 0;
-            if (RAW64) {
-                if (!((vm[u] >> kcol[0]) & 1)) special[u] = 2;
-                else {
-                    key[u] = pick(u, kcol[0]);
-                    special[u] = key[u] == kEmpty ? 1 : 0;
-                }
-            } else {
+                for (int u = 0; u < U; ++u) raw[u] = live[u] ? load_bits_w(pc[r].data, w, r0 + (uint64_t)u * bs) : 0;
+                if (pc[r].valid) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    if (i >= nk) continue;
-                    if ((vm[u] >> kcol[i]) & 1) key[u] |= pick(u, kcol[i]) << kdata[i];
-                    else key[u] |= 1ull << knull[i];
+                    for (int u = 0; u < U; ++u)
+                        vb |= (uint32_t)(live[u] && dvalid(pc[r].valid, r0 + (uint64_t)u * bs)) << u;
+                } else {
+                    vb = 0xFu;
                 }
-                special[u] = (allones && key[u] == kEmpty) ? 1 : 0;
-            }
-            slot[u] = -1;
-            if (sel[u] && special[u] == 0 && CL) {
-                uint32_t s = (uint32_t)(hash_key(key[u], 0) >> 40) & (CL - 1);
-                for (int p = 0; p < kProbe; ++p) {
-                    const uint64_t cur = lkeys[s];
-                    if (cur == key[u]) { slot[u] = (int32_t)s; break; }
-                    if (cur == kEmpty) break;
-                    s = (s + 1) & (CL - 1);
-                }
+                int64_t img[U];
+                pred_images<U>(pc[r].tag, pflt[r] != 0, raw, img);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (img[u] < plo[r] || img[u] > phi[r]) vb &= ~(1u << u);
+                selbits &= vb;
             }
         }
-        // ---- hot rows: LDS state updates
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (slot[u] >= 0) atomicAdd(len_p + slot[u], 1u);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            if (j >= nv) continue;
-            const int vc = vcol[j], tg = vtag[j], need = vneed[j];
-            if (vkind[j] == VK_F) {
-                FSink fs{limb_p[j], res_p[j], CL};
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (slot[u] < 0 || !((vm[u] >> vc) & 1)) continue;
-                    const uint32_t sl = (uint32_t)slot[u];
-                    const double x = raw_to_f64(tg, pick(u, vc));
-                    atomicAdd(cnt_p[j] + sl, 1u);
-                    if (need & NEED_SUM) add_f64_exact(fs, 0, 0, sl, x);
-                    if (need & NEED_MINMAX) {
-                        if (x != x) atomicOr(nan_p[j] + sl, 1u);
+#pragma unroll 1
+        for (int u = 0; u < (drain ? 1 : U); ++u) {
+            if (!drain) {
+                const bool s = (selbits >> u) & 1;
+                const unsigned long long M = __ballot(s);
+                if (s) {
+                    uint32_t pos = qhead + qn + (uint32_t)__popcll(M & below);
+                    pos = pos >= kRing ? pos - kRing : pos;
+                    ring[pos] = (uint32_t)(r0 + (uint64_t)u * bs - r_begin);
+                }
+                const uint32_t c = (uint32_t)__popcll(M);
+                qn += c;
+                wave_sel += c;
+            }
+#pragma unroll 1
+            while (qn >= 64 || (drain && qn > 0)) {
+                // ---- one dense round over the oldest min(qn, 64) queued rows
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const uint32_t cnt = qn < 64 ? qn : 64;
+                const bool act = lane < cnt;
+                uint32_t rp = qhead + lane;
+                rp = rp >= kRing ? rp - kRing : rp;
+                const uint64_t row = r_begin + (act ? ring[rp] : 0);
+                qhead = qhead + cnt >= kRing ? qhead + cnt - kRing : qhead + cnt;
+                qn -= cnt;
+                // key
+                uint64_t key = 0;
+                int special = 0;
+                if (RAW64) {
+                    if (act) {
+                        if (!dvalid(kc[0].valid, row)) special = 2;
                         else {
-                            const long long kk = (long long)f64_order_key(x);
-                            atomicMin((long long *)min_p[j] + sl, kk);
-                            atomicMax((long long *)max_p[j] + sl, kk);
+                            key = gld<uint64_t>(kc[0].data, row);
+                            special = key == kEmpty ? 1 : 0;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        if (i >= nk) continue;
+                        if (!act) continue;
+                        if (dvalid(kc[i].valid, row)) key |= load_bits_w(kc[i].data, kc[i].w, row) << kdata[i];
+                        else key |= 1ull << knull[i];
+                    }
+                    special = (allones && key == kEmpty) ? 1 : 0;
+                }
+                // values
+                uint64_t vr[2] = {0, 0};
+                uint32_t vv = 0;  // bit j: value j valid
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (j >= nv) continue;
+                    if (act) {
+                        vr[j] = load_bits_w(vc[j].data, vc[j].w, row);
+                        vv |= (uint32_t)dvalid(vc[j].valid, row) << j;
+                    }
+                }
+                // hot-table probe
+                int32_t slot = -1;
+                if (act && special == 0 && CL) {
+                    uint32_t s = (uint32_t)(hash_key(key, 0) >> 40) & (CL - 1);
+                    for (int p = 0; p < kProbe; ++p) {
+                        const uint64_t cur = lkeys[s];
+                        if (cur == key) { slot = (int32_t)s; break; }
+                        if (cur == kEmpty) break;
+                        s = (s + 1) & (CL - 1);
+                    }
+                }
+                // hot rows: fold into LDS
+                if (slot >= 0) {
+                    const uint32_t sl = (uint32_t)slot;
+                    atomicAdd(len_p + sl, 1u);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        if (j >= nv || !((vv >> j) & 1)) continue;
+                        const int need = vneed[j];
+                        atomicAdd(cnt_p[j] + sl, 1u);
+                        if (vkind[j] == VK_F) {
+                            const double x = vc[j].tag == PMH_F64 ? __longlong_as_double((long long)vr[j])
+                                                                  : (double)__uint_as_float((uint32_t)vr[j]);
+                            if (need & NEED_SUM) lds_add_f64(limb_p[j], res_p[j], CL, sl, x);
+                            if (need & NEED_MINMAX) {
+                                const bool nan = x != x;
+                                const long long kk = (long long)f64_order_key(x);
+                                atomicOr(nan_p[j] + sl, nan ? 1u : 0u);
+                                atomicMin((long long *)min_p[j] + sl, nan ? 0x7FFFFFFFFFFFFFFFll : kk);
+                                atomicMax((long long *)max_p[j] + sl, nan ? (long long)0x8000000000000000ull : kk);
+                            }
+                        } else if (vkind[j] == VK_I64) {
+                            const int64_t x = (int64_t)vr[j];
+                            if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)x);
+                            if (need & NEED_MINMAX) {
+                                atomicMin((long long *)min_p[j] + sl, (long long)x);
+                                atomicMax((long long *)max_p[j] + sl, (long long)x);
+                            }
+                        } else {
+                            const int32_t x = (int32_t)(uint32_t)vr[j];
+                            if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)(int64_t)x);
+                            if (need & NEED_MINMAX) {
+                                atomicMin((int *)min_p[j] + sl, x);
+                                atomicMax((int *)max_p[j] + sl, x);
+                            }
                         }
                     }
                 }
-            } else if (vkind[j] == VK_I64) {
+                // tail rows: wave-contiguous append
+                const bool tail = act && slot < 0;
+                const unsigned long long m = __ballot(tail);
+                if (m) {
+                    const int leader = __ffsll((long long)__ballot(1)) - 1;
+                    uint32_t base = 0;
+                    if ((int)lane == leader) base = atomicAdd(&s_cursor, (uint32_t)__popcll(m));
+                    base = __shfl(base, leader);
+                    if (tail) {
+                        const uint64_t pos = region + base + (uint32_t)__popcll(m & below);
+                        uint16_t fl = special == 1 ? kFlagAllOnes : (special == 2 ? kFlagNull : 0);
+                        fl |= (uint16_t)vv;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (slot[u] < 0 || !((vm[u] >> vc) & 1)) continue;
-                    const uint32_t sl = (uint32_t)slot[u];
-                    const int64_t x = raw_to_i64(tg, pick(u, vc));
-                    atomicAdd(cnt_p[j] + sl, 1u);
-                    if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)x);
-                    if (need & NEED_MINMAX) {
-                        atomicMin((long long *)min_p[j] + sl, (long long)x);
-                        atomicMax((long long *)max_p[j] + sl, (long long)x);
+                        for (int j = 0; j < 2; ++j)
+                            if (j < nv) gstore_w(sval[j], vwid[j], pos, vr[j]);
+                        gst<uint64_t>(sklo, pos, key);
+                        gst<uint16_t>(sflags, pos, fl);
                     }
                 }
-            } else {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    if (slot[u] < 0 || !((vm[u] >> vc) & 1)) continue;
-                    const uint32_t sl = (uint32_t)slot[u];
-                    const int32_t x = (int32_t)raw_to_i64(tg, pick(u, vc));
-                    atomicAdd(cnt_p[j] + sl, 1u);
-                    if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)(int64_t)x);
-                    if (need & NEED_MINMAX) {
-                        atomicMin((int *)min_p[j] + sl, x);
-                        atomicMax((int *)max_p[j] + sl, x);
-                    }
-                }
-            }
-        }
-        // ---- tail rows: one cursor reservation per wave for all four rows
-        unsigned long long m[4];
-        uint32_t tot = 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            m[u] = __ballot(sel[u] && slot[u] < 0);
-            tot += (uint32_t)__popcll(m[u]);
-        }
-        if (tot) {
-            const int leader = __ffsll((long long)__ballot(1)) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(&s_cursor, tot);
-            base = __shfl(base, leader);
-            const unsigned long long below = (1ull << lane) - 1;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if ((m[u] >> lane) & 1) {
-                    const uint64_t pos = region + base + (uint32_t)__popcll(m[u] & below);
-                    uint16_t fl = special[u] == 1 ? kFlagAllOnes : (special[u] == 2 ? kFlagNull : 0);
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        if (j >= nv) continue;
-                        fl |= (uint16_t)(((vm[u] >> vcol[j]) & 1) << j);
-                        gstore_w(sval[j], vwid[j], pos, pick(u, vcol[j]));
-                    }
-                    gst<uint64_t>(sklo, pos, key[u]);
-                    gst<uint16_t>(sflags, pos, fl);
-                }
-                base += (uint32_t)__popcll(m[u]);
             }
         }
     }
-    atomicAdd(&s_sel, sel_count);
+    if (lane == 0) atomicAdd(&s_sel, wave_sel);
     __syncthreads();
     if (tid == 0) {
         L->tail_n[blockIdx.x] = s_cursor;
@@ -326,7 +407,6 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     }
     // hot partials: block slot s -> global slot s (coalesced atomics)
     GSink g{L->hot_arr, CL};
-    LSink ls{state, L->lds1_off, CL};
     const int a_len = q->layout.a_len;
     for (uint32_t i = tid; i < CL; i += bs) {
         if (*ls.p<uint32_t>(a_len, i) == 0) continue;

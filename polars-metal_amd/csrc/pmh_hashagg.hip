@@ -56,7 +56,8 @@ struct FastScan {
     int32_t enabled, nr;
     int32_t rcol[2], rflt[2];
     int64_t rlo[2], rhi[2];
-    uint32_t ring_off, pad;  // byte offset of the per-wave row rings in LDS
+    uint32_t ring_off;       // byte offset of the per-wave row rings in LDS
+    int32_t rows;            // rows per lane per step (4 or 8)
 };
 
 enum StatIdx : int32_t {
@@ -1196,11 +1197,19 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.q = q0;
     L.wide = wide;
     L.n_out = P.n_out;
-    if (opt.fast_scan != 0 && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 32) &&
+    // the specialised scan reads values as aligned 8-byte words, which needs
+    // every column naturally aligned (Arrow buffers and numpy arrays are)
+    bool aligned = true;
+    for (int c = 0; c < q0.n_cols; ++c) {
+        const int w = tag_size(q0.cols[c].tag);
+        if (w > 1 && ((uintptr_t)q0.cols[c].data % (uintptr_t)w) != 0) aligned = false;
+    }
+    if (opt.fast_scan != 0 && aligned && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 32) &&
         (q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64 || q0.key_mode == KM_NONE) && lower_fast_scan(q0, L.fs))
         L.fs.enabled = 1;
     else
         L.fs = FastScan{};
+    L.fs.rows = opt.scan_rows == 8 ? 8 : 4;
     size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
     for (int j = 0; j < nv; ++j) {
         L.vw[j] = tag_size(q0.vals[j].tag);
@@ -1395,10 +1404,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         if (L.fs.enabled) {
             KernelTimer kt("hs_scan_fast");
             const bool raw = q0.key_mode == KM_RAW64;
-            const void *fn = raw ? (const void *)hs_scan_fast<true> : (const void *)hs_scan_fast<false>;
+            const bool r8 = L.fs.rows == 8;
+            const void *fn = raw ? (r8 ? (const void *)hs_scan_fast<true, 8> : (const void *)hs_scan_fast<true, 4>)
+                                 : (r8 ? (const void *)hs_scan_fast<false, 8> : (const void *)hs_scan_fast<false, 4>);
             PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
-            if (raw) hipLaunchKernelGGL(hs_scan_fast<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
-            else hipLaunchKernelGGL(hs_scan_fast<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+            void *args[] = {(void *)&dL};
+            PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));
         } else {
             KernelTimer kt("hs_scan");
             const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;

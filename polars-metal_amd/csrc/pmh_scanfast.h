@@ -7,25 +7,36 @@
 // the key is one 64-bit column or a lane of at most 64 bits over at most four
 // columns, and at most two columns are aggregated.
 //
-// Two ideas make it cheaper than the generic scan:
-//  * every descriptor field, LDS array address and spill pointer is read
-//    into registers once per block, and tag dispatch happens once per column
-//    per step, not per row;
-//  * rows are compacted before the per-row work.  Each wave evaluates the
-//    predicate on coalesced loads of the predicate columns only, queues the
-//    offsets of the selected rows in a per-wave LDS ring, and whenever 64 are
-//    queued runs one dense round over them: gather the key and value columns
-//    for those rows, hash, probe the hot table, fold hot rows into LDS and
-//    append the rest to the block's spill region.  At 30% selectivity that
-//    removes three quarters of the idle lanes from the expensive part, and
-//    key/value cache lines holding no selected row are never fetched.
+// What makes it cheaper than the generic scan:
+//  * Every descriptor field, LDS array address and spill pointer is read
+//    into registers once per block.
+//  * Column loads have one form for every width: the aligned 8-byte word
+//    holding the value (or validity bit) is loaded, and the value is cut out
+//    with a shift and mask whose parameters are wave-uniform.  A naturally
+//    aligned value never straddles words, and an aligned word never crosses
+//    a page, so the load is always mapped.  With no per-width switch and no
+//    lane predicate around a load, nothing forces the compiler to wait for
+//    one load before issuing the next: a step's loads are all in flight
+//    together and one wait covers them.
+//  * Rows are compacted before the per-row work.  Each wave evaluates the
+//    predicate on coalesced loads of the predicate columns only (issued one
+//    step ahead), queues the offsets of the selected rows in a per-wave LDS
+//    ring, and whenever 64 are queued runs one dense round over them: gather
+//    the key and value words for those rows, hash, probe the hot table, fold
+//    hot rows into LDS and append the rest to the block's spill region.  At
+//    30% selectivity that removes three quarters of the idle lanes from the
+//    expensive part, and key/value cache lines holding no selected row are
+//    never fetched.
 #pragma once
 
 namespace pmh {
 
 // Per-wave ring of selected-row offsets: at most 63 carried rows plus one
-// step of 64 new ones are queued at any time.
+// row step of 64 new ones are queued at any time.
 constexpr uint32_t kRing = 127;
+
+// Stand-in validity bitmap for columns without one: every bit set.
+__device__ const uint64_t pmh_all_valid[1] = {~0ull};
 
 // Order-preserving int64 image of a double under Polars' TotalOrd (the order
 // of tot_cmp): -0.0 and +0.0 map together, every NaN maps above +inf.
@@ -36,30 +47,34 @@ __host__ __device__ inline int64_t tot_key_bits(int64_t b, bool is_nan) {
 }
 __device__ __forceinline__ int64_t tot_key(double x) { return tot_key_bits(__double_as_longlong(x), x != x); }
 
-// A column as one role of the query sees it.
+// A column as one role of the query sees it, addressed by bit position:
+// value r occupies bits [bit0 + (r << bshift), ... + width) of the word
+// array at base; its validity bit is bit vbit0 + r * vstride of vbase.
 struct RoleCol {
-    const void *data;
-    const uint8_t *valid;
-    int32_t tag, w;  // w: bytes per value, 0 for bit-packed Boolean
+    const uint64_t *base, *vbase;
+    uint64_t mask;
+    uint32_t bit0, vbit0, vstride;
+    int32_t bshift, tag, w;  // w: bytes per value, 0 for bit-packed Boolean
 };
 __device__ __forceinline__ RoleCol role_col(const QueryDesc *__restrict__ q, int c) {
     RoleCol r;
-    r.data = q->cols[c].data;
-    r.valid = q->cols[c].valid;
+    const uintptr_t d = (uintptr_t)q->cols[c].data;
+    const uintptr_t v = (uintptr_t)q->cols[c].valid;
     r.tag = q->cols[c].tag;
     r.w = r.tag == PMH_BOOL ? 0 : tag_size(r.tag);
+    r.base = (const uint64_t *)(d & ~uintptr_t(7));
+    r.bit0 = (uint32_t)(d & 7) * 8;
+    r.bshift = r.w == 8 ? 6 : r.w == 4 ? 5 : r.w == 2 ? 4 : r.w == 1 ? 3 : 0;
+    r.mask = r.w == 8 ? ~0ull : r.w == 0 ? 1ull : (1ull << (8 * r.w)) - 1;
+    r.vbase = v ? (const uint64_t *)(v & ~uintptr_t(7)) : pmh_all_valid;
+    r.vbit0 = v ? (uint32_t)(v & 7) * 8 : 0;
+    r.vstride = v ? 1 : 0;
     return r;
 }
-
-__device__ __forceinline__ uint64_t load_bits_w(const void *p, int w, uint64_t r) {
-    switch (w) {
-    case 8: return gld<uint64_t>(p, r);
-    case 4: return gld<uint32_t>(p, r);
-    case 2: return gld<uint16_t>(p, r);
-    case 1: return gld<uint8_t>(p, r);
-    default: return (gld<uint8_t>(p, r >> 3) >> (r & 7)) & 1;  // bit-packed Boolean
-    }
-}
+__device__ __forceinline__ uint64_t vpos(const RoleCol &c, uint64_t r) { return c.bit0 + (r << c.bshift); }
+__device__ __forceinline__ uint64_t bpos(const RoleCol &c, uint64_t r) { return c.vbit0 + r * c.vstride; }
+__device__ __forceinline__ uint64_t word_at(const uint64_t *base, uint64_t pos) { return gld<uint64_t>(base, pos >> 6); }
+__device__ __forceinline__ uint64_t cut(uint64_t word, uint64_t pos, uint64_t mask) { return (word >> (pos & 63)) & mask; }
 
 // Predicate images of U values of one column (raw bits, zero-extended): the
 // column's TotalOrd image when flt, else its int64 value.  The tag switch is
@@ -139,7 +154,7 @@ __device__ __forceinline__ void lds_add_f64(unsigned long long *limb, double *re
     }
 }
 
-template <bool RAW64>
+template <bool RAW64, int U>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_cursor, s_sel;
@@ -228,38 +243,48 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     const uint64_t r_begin = region;
     const uint64_t r_end = min(n, r_begin + L->chunk);
     const uint64_t span = r_end > r_begin ? r_end - r_begin : 0;
-    constexpr int U = 4;
     // every lane of a wave runs the same steps and rounds, so ballots see
     // whole waves
     const uint64_t steps = (span + (uint64_t)U * bs - 1) / ((uint64_t)U * bs);
+    const uint64_t r_last = span ? r_end - 1 : r_begin;  // in-range stand-in for rows past the end
     uint32_t qhead = 0, qn = 0;  // wave-uniform ring state
     uint32_t wave_sel = 0;
+
+    // Predicate words run one step ahead: the loads for step it + 1 are in
+    // flight while step it's rounds run.
+    uint64_t pw[2][U], pv[2][U];
+    auto load_pred = [&](uint64_t step) __attribute__((always_inline)) {
+        const uint64_t rs = r_begin + step * (uint64_t)U * bs + tid;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) continue;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
+                pw[r][u] = word_at(pc[r].base, vpos(pc[r], row));
+                pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row));
+            }
+        }
+    };
+    if (steps) load_pred(0);
 
     for (uint64_t it = 0; it <= steps; ++it) {
         const bool drain = it == steps;
         uint32_t selbits = 0;  // bit u: row u of this lane passes
         const uint64_t r0 = r_begin + it * (uint64_t)U * bs + tid;
         if (!drain) {
-            bool live[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                live[u] = r0 + (uint64_t)u * bs < r_end;
-                selbits |= (uint32_t)live[u] << u;
-            }
+            for (int u = 0; u < U; ++u) selbits |= (uint32_t)(r0 + (uint64_t)u * bs < r_end) << u;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 if (r >= nr) continue;
                 uint64_t raw[U];
                 uint32_t vb = 0;
-                const int w = pc[r].w;
 #pragma unroll
-                for (int u = 0; u < U; ++u) raw[u] = live[u] ? load_bits_w(pc[r].data, w, r0 + (uint64_t)u * bs) : 0;
-                if (pc[r].valid) {
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-                        vb |= (uint32_t)(live[u] && dvalid(pc[r].valid, r0 + (uint64_t)u * bs)) << u;
-                } else {
-                    vb = 0xFu;
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t row = min(r0 + (uint64_t)u * bs, r_last);
+                    raw[u] = cut(pw[r][u], vpos(pc[r], row), pc[r].mask);
+                    vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], row), 1) << u;
                 }
                 int64_t img[U];
                 pred_images<U>(pc[r].tag, pflt[r] != 0, raw, img);
@@ -268,6 +293,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     if (img[u] < plo[r] || img[u] > phi[r]) vb &= ~(1u << u);
                 selbits &= vb;
             }
+            if (it + 1 < steps) load_pred(it + 1);
         }
 #pragma unroll 1
         for (int u = 0; u < (drain ? 1 : U); ++u) {
@@ -293,50 +319,60 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 const bool act = lane < cnt;
                 uint32_t rp = qhead + lane;
                 rp = rp >= kRing ? rp - kRing : rp;
-                const uint64_t row = r_begin + (act ? ring[rp] : 0);
+                const uint32_t off = ring[rp];
+                const uint64_t row = r_begin + (act ? off : 0);
                 qhead = qhead + cnt >= kRing ? qhead + cnt - kRing : qhead + cnt;
                 qn -= cnt;
-                // key
+                // gathers: every word first, then the cuts
+                uint64_t kw[4], kv[4], vw_[2], vv_[2];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    kw[i] = kv[i] = 0;
+                    if (i >= nk) continue;
+                    kw[i] = word_at(kc[i].base, vpos(kc[i], row));
+                    kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
+                }
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    vw_[j] = vv_[j] = 0;
+                    if (j >= nv) continue;
+                    vw_[j] = word_at(vc[j].base, vpos(vc[j], row));
+                    vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+                }
                 uint64_t key = 0;
                 int special = 0;
                 if (RAW64) {
-                    if (act) {
-                        if (!dvalid(kc[0].valid, row)) special = 2;
-                        else {
-                            key = gld<uint64_t>(kc[0].data, row);
-                            special = key == kEmpty ? 1 : 0;
-                        }
+                    if (!cut(kv[0], bpos(kc[0], row), 1)) special = 2;
+                    else {
+                        key = kw[0];
+                        special = key == kEmpty ? 1 : 0;
                     }
                 } else {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         if (i >= nk) continue;
-                        if (!act) continue;
-                        if (dvalid(kc[i].valid, row)) key |= load_bits_w(kc[i].data, kc[i].w, row) << kdata[i];
-                        else key |= 1ull << knull[i];
+                        key |= cut(kv[i], bpos(kc[i], row), 1) ? cut(kw[i], vpos(kc[i], row), kc[i].mask) << kdata[i]
+                                                                : 1ull << knull[i];
                     }
                     special = (allones && key == kEmpty) ? 1 : 0;
                 }
-                // values
                 uint64_t vr[2] = {0, 0};
                 uint32_t vv = 0;  // bit j: value j valid
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     if (j >= nv) continue;
-                    if (act) {
-                        vr[j] = load_bits_w(vc[j].data, vc[j].w, row);
-                        vv |= (uint32_t)dvalid(vc[j].valid, row) << j;
-                    }
+                    vr[j] = cut(vw_[j], vpos(vc[j], row), vc[j].mask);
+                    vv |= (uint32_t)cut(vv_[j], bpos(vc[j], row), 1) << j;
                 }
                 // hot-table probe
                 int32_t slot = -1;
                 if (act && special == 0 && CL) {
-                    uint32_t s = (uint32_t)(hash_key(key, 0) >> 40) & (CL - 1);
-                    for (int p = 0; p < kProbe; ++p) {
-                        const uint64_t cur = lkeys[s];
-                        if (cur == key) { slot = (int32_t)s; break; }
+                    uint32_t sp = (uint32_t)(hash_key(key, 0) >> 40) & (CL - 1);
+                    for (int pr = 0; pr < kProbe; ++pr) {
+                        const uint64_t cur = lkeys[sp];
+                        if (cur == key) { slot = (int32_t)sp; break; }
                         if (cur == kEmpty) break;
-                        s = (s + 1) & (CL - 1);
+                        sp = (sp + 1) & (CL - 1);
                     }
                 }
                 // hot rows: fold into LDS

@@ -58,6 +58,8 @@ struct FastScan {
     int64_t rlo[2], rhi[2];
     uint32_t ring_off;       // byte offset of the per-wave row rings in LDS
     int32_t rows;            // rows per lane per step (4 or 8)
+    int32_t touch;           // touch key/value rows one step ahead of their gathers
+    int32_t pad;
 };
 
 enum StatIdx : int32_t {
@@ -1210,6 +1212,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     else
         L.fs = FastScan{};
     L.fs.rows = opt.scan_rows == 8 ? 8 : 4;
+    L.fs.touch = opt.scan_touch != 0 ? 1 : 0;
     size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
     for (int j = 0; j < nv; ++j) {
         L.vw[j] = tag_size(q0.vals[j].tag);

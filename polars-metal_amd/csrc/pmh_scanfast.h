@@ -251,8 +251,12 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     uint32_t wave_sel = 0;
 
     // Predicate words run one step ahead: the loads for step it + 1 are in
-    // flight while step it's rounds run.
+    // flight while step it's rounds run.  Beside them go touches: one dword
+    // of the step's key and value rows, so that when the rounds gather those
+    // rows a step later their lines are already in cache.
+    const bool touch = L->fs.touch != 0;
     uint64_t pw[2][U], pv[2][U];
+    uint32_t tw[3][U];
     auto load_pred = [&](uint64_t step) __attribute__((always_inline)) {
         const uint64_t rs = r_begin + step * (uint64_t)U * bs + tid;
 #pragma unroll
@@ -263,6 +267,15 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
                 pw[r][u] = word_at(pc[r].base, vpos(pc[r], row));
                 pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row));
+            }
+        }
+        if (touch) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
+                tw[0][u] = gld<uint32_t>(kc[0].base, vpos(kc[0], row) >> 5);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) tw[1 + j][u] = j < nv ? gld<uint32_t>(vc[j].base, vpos(vc[j], row) >> 5) : 0;
             }
         }
     };
@@ -292,6 +305,11 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 for (int u = 0; u < U; ++u)
                     if (img[u] < plo[r] || img[u] > phi[r]) vb &= ~(1u << u);
                 selbits &= vb;
+            }
+            if (touch) {
+                // the touches of this step are not needed, only their lines
+#pragma unroll
+                for (int u = 0; u < U; ++u) asm volatile("" ::"v"(tw[0][u]), "v"(tw[1][u]), "v"(tw[2][u]));
             }
             if (it + 1 < steps) load_pred(it + 1);
         }

@@ -811,7 +811,7 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
         pmh_result *r = run_hash_partitioned(P, plan, flags);
         if (r) return r;
     }
-    if (m == "sort") fail(PMH_E_NOTIMPL, "polars_metal: the sort path is not built yet");
+    if (m == "sort") return run_sort(P, plan, flags);
     return run_hash_global(P, plan, flags);
 }
 

@@ -1022,6 +1022,19 @@ int log2u(uint64_t x) {
     return b;
 }
 
+// LDS offsets for nslots entries per array, arrays back to back.
+size_t lds_layout(const Prep &P, uint32_t nslots, uint32_t *off) {
+    const Layout &lay = P.q.layout;
+    size_t o = 0;
+    for (int a = 0; a < lay.n_arrs; ++a) {
+        off[a] = (uint32_t)o;
+        o += (size_t)nslots * lay.esz[a];
+    }
+    return o;
+}
+
+}  // namespace
+
 // Carves nslots-entry state arrays from one slab; limb k > 0 of an exact
 // accumulator sits nslots entries after limb k-1.
 void carve_state(const Prep &P, uint8_t *base, size_t &o, uint64_t nslots, void **arr) {
@@ -1046,18 +1059,6 @@ size_t state_bytes(const Prep &P, uint64_t nslots) {
     }
     return o;
 }
-// LDS offsets for nslots entries per array, arrays back to back.
-size_t lds_layout(const Prep &P, uint32_t nslots, uint32_t *off) {
-    const Layout &lay = P.q.layout;
-    size_t o = 0;
-    for (int a = 0; a < lay.n_arrs; ++a) {
-        off[a] = (uint32_t)o;
-        o += (size_t)nslots * lay.esz[a];
-    }
-    return o;
-}
-
-}  // namespace
 
 Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
                    uint32_t dropped, const std::vector<uint32_t> &hist) {

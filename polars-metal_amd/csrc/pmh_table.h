@@ -324,6 +324,12 @@ double plain_distinct_estimate(const Sketch &k);
 
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);
+// Sort engine: radix sort of the selected rows' key lanes, then runs fold.
+pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags);
+// Per-group state arrays of nslots entries carved from one slab (limb k > 0
+// of an exact accumulator sits nslots entries after limb k-1).
+void carve_state(const Prep &P, uint8_t *base, size_t &o, uint64_t nslots, void **arr);
+size_t state_bytes(const Prep &P, uint64_t nslots);
 // Returns nullptr when the query shape does not fit the partitioned engine.
 pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flags);
 

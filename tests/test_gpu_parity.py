@@ -30,6 +30,62 @@ def test_golden_cases(native, case):
         assert_results_equal(got2, expected_of(case, frame), _keys(case["plan"]), label=case["name"] + "/groupby")
 
 
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_golden_cases_sort(native, case):
+    frame = frame_of(case)
+    plan = dict(case["plan"], options={"method": "sort"})
+    got = run_gpu(native, plan, frame)
+    assert_results_equal(got, expected_of(case, frame), _keys(case["plan"]), label=case["name"] + "/sort")
+    assert native.last_stats()["method"] == "sort"
+
+
+def _sorted_bits(res, keys):
+    c = orc.canonical(orc.from_wire(res), keys)
+    return {k: (v[1].tobytes(), v[2].tobytes()) for k, v in c.items() if k != "__n_groups__"}
+
+
+@pytest.mark.parametrize("rows", [1_000, 2_000_000])
+def test_sort_agrees_with_hash_bit_exact(native, rows):
+    # hash-vs-sort agreement is the reference's own oracle for its sort path
+    # (both must produce the same groups and the same bits)
+    frame, plan = synth.zipf(rows, seed=rows + 7)
+    wire = orc.to_wire(frame)
+    a = native.execute_filter_groupby(dict(plan, options={"method": "hash"}), rows, wire)
+    b = native.execute_filter_groupby(dict(plan, options={"method": "sort"}), rows, wire)
+    st = native.last_stats()
+    assert st["method"] == "sort" and 1 <= st["n_passes"] <= 3  # keys 1..1e6: at most 20 varying bits
+    assert _sorted_bits(a, ["key"]) == _sorted_bits(b, ["key"])
+    assert_results_equal(orc.from_wire(b), orc.groupby(plan, frame), ["key"], label=f"sort{rows}")
+
+
+def test_sort_edge_shapes(native):
+    rng = np.random.default_rng(5)
+    n = 50_000
+    aggs = [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in ("Sum", "Count", "Min", "Max", "Mean")]
+    v = ("F64", rng.standard_normal(n), rng.random(n) < 0.9)
+    cases = [
+        # one group (no varying key bit: zero radix passes)
+        ({"k": ("I64", np.full(n, 7, np.int64), None), "v": v}, [["k", "I64"]]),
+        # compressed key wider than 64 bits: two random i64 halves
+        ({"a": ("I64", rng.integers(-2**62, 2**62, n), rng.random(n) < 0.95),
+          "b": ("I32", rng.integers(-2**31, 2**31 - 1, n).astype(np.int32), None), "v": v}, [["a", "I64"], ["b", "I32"]]),
+        # all-ones and null i64 keys beside ordinary ones
+        ({"k": ("I64", np.where(rng.random(n) < 0.1, -1, rng.integers(0, 50, n)), rng.random(n) < 0.9), "v": v},
+         [["k", "I64"]]),
+    ]
+    for i, (frame, keys) in enumerate(cases):
+        plan = {"keys": keys, "aggs": aggs, "options": {"method": "sort"}}
+        assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), [k[0] for k in keys],
+                             label=f"edge{i}")
+    # a predicate that keeps nothing
+    frame = {"k": ("I64", rng.integers(0, 9, n), None), "v": v}
+    plan = {"keys": [["k", "I64"]], "aggs": aggs, "options": {"method": "sort"},
+            "predicate": {"kind": "Compare", "op": "Gt", "dtype": "F64", "lhs": {"kind": "Column", "name": "v", "dtype": "F64"},
+                          "rhs": {"kind": "LiteralF64", "value": 1e300}}}
+    got = run_gpu(native, plan, frame)
+    assert got["__n_groups__"] == 0
+
+
 def test_small_config_full_size(native):
     frame, plan = synth.small(1_000_000, seed=11)
     got = run_gpu(native, plan, frame)
@@ -46,7 +102,7 @@ def test_zipf_scaled(native, rows):
     assert_results_equal(got, orc.groupby(plan, frame), ["key"], label=f"zipf{rows}")
 
 
-@pytest.mark.parametrize("method", ["hash", "hash_global"])
+@pytest.mark.parametrize("method", ["hash", "hash_global", "sort"])
 def test_highcard_scaled(native, method):
     frame, plan = synth.highcard(1_000_000, seed=5, pool=250_000)
     plan = dict(plan, options={"method": method})
@@ -115,7 +171,7 @@ def test_launch_shapes_agree(native, opts):
 KEY_TAGS = ["I64", "I32", "I16", "I8", "U8", "U16", "U32", "U64", "F64", "F32", "Bool"]
 
 
-@pytest.mark.parametrize("method", ["hash", "hash_global"])
+@pytest.mark.parametrize("method", ["hash", "hash_global", "sort"])
 @pytest.mark.parametrize("tag", KEY_TAGS)
 def test_key_dtypes_with_nulls(native, tag, method):
     rng = np.random.default_rng(KEY_TAGS.index(tag))
@@ -148,7 +204,7 @@ def test_i64_all_ones_and_null_side_slots(native):
         assert_results_equal(run_gpu(native, p, frame), orc.groupby(p, frame), ["k"], label=f"side{opts}")
 
 
-@pytest.mark.parametrize("method", ["hash", "hash_global"])
+@pytest.mark.parametrize("method", ["hash", "hash_global", "sort"])
 @pytest.mark.parametrize("keys", [["a", "b"], ["b", "c", "a"], ["c", "d"], ["a", "c", "d"]])
 def test_composite_keys(native, keys, method):
     rng = np.random.default_rng(len(keys))

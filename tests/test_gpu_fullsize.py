@@ -54,3 +54,73 @@ def test_zipf_100m_properties(native):
         rows = vsel & (key == ks[i])
         assert out["v_sum"][0][i] == orc.exact_sum(v[rows]), ks[i]
         assert out["v_count"][0][i] == rows.sum()
+
+
+def _splitmix(x):
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _fetch_all(res):
+    ng = res.n_rows
+    out = {}
+    for c in res.columns:
+        tag, data, valid = res.fetch(c["name"])
+        vals = np.frombuffer(data, orc.NP_DTYPE[tag])[:ng]
+        ok = orc.unpack_bits(valid, ng) if valid is not None else np.ones(ng, bool)
+        out[c["name"]] = (vals, ok)
+    return out
+
+
+def _group_checksum(out):
+    """Order-independent checksum over groups: the wrapping sum of a mix of
+    each group's key and aggregates (equal multisets give equal sums)."""
+    with np.errstate(over="ignore"):
+        a = out["a"][0].astype(np.int64).view(np.uint64)
+        b = out["b"][0].view(np.uint64)
+        s = out["v_sum"][0].view(np.uint64)
+        c = out["v_count"][0].astype(np.uint64)
+        x = _splitmix(a * np.uint64(0xD6E8FEB86659FD93) ^ _splitmix(b) ^ (s * np.uint64(0x9E3779B97F4A7C15))
+                      ^ (c << np.uint64(40)))
+        return int(x.sum(dtype=np.uint64))
+
+
+@pytest.mark.timeout(600)
+def test_highcard_200m_hash_and_sort_agree(native):
+    """The 200M-row composite-key workload (~49M groups) through both the
+    partitioned hash engine and the sort engine: same group count, same
+    multiset of (key, sum, count) rows, totals conserved, and the sort
+    engine's output in lane order (b, then a)."""
+    from polars_metal.buffer import device_columns
+    n = 200_000_000
+    print("generating the 200M-row high-cardinality workload (numpy)", flush=True)
+    frame, plan = synth.highcard(n, seed=3)
+    cols, bufs = device_columns(frame)
+    v, v_ok = frame["v"][1], frame["v"][2]
+    with np.errstate(over="ignore"):
+        total = int(v[v_ok].sum(dtype=np.int64))
+    n_valid = int(v_ok.sum())
+    del frame
+    outs = {}
+    for method in ("hash", "sort"):
+        print(f"running {method}", flush=True)
+        res = native.execute_filter_groupby_device(dict(plan, options={"method": method}), n, cols)
+        print(f"{method}: {res.n_rows} groups, {res.stats['device_ms']:.2f} ms", flush=True)
+        assert res.stats["method"] == method
+        outs[method] = _fetch_all(res)
+        res.free()
+    h, s = outs["hash"], outs["sort"]
+    assert len(h["a"][0]) == len(s["a"][0]) > 45_000_000
+    for o in (h, s):
+        assert int(o["v_count"][0].sum(dtype=np.int64)) == n_valid
+        with np.errstate(over="ignore"):
+            assert int(o["v_sum"][0].sum(dtype=np.int64)) == total
+    assert _group_checksum(h) == _group_checksum(s)
+    # sort output: strictly increasing in (b, a) as unsigned bit patterns
+    bs = s["b"][0].view(np.uint64)
+    as_ = s["a"][0].view(np.uint32)
+    step_b = bs[1:] > bs[:-1]
+    same_b = bs[1:] == bs[:-1]
+    assert (step_b | (same_b & (as_[1:] > as_[:-1]))).all()

@@ -108,7 +108,7 @@ typedef struct pmh_exec_stats {
     uint64_t tail_rows;       /* selected rows not covered by the hot keys */
     double distinct_estimate; /* distinct keys the engine sized its partitions for */
     int32_t scan_kernel;      /* partitioned engine's row scan: 0 generic, 1 specialised */
-    int32_t reserved;
+    int32_t sizer_choice;     /* learned sizer under method "auto": 0 not used, 1 chose hash, 2 chose sort */
 } pmh_exec_stats;
 
 typedef struct pmh_result {
@@ -215,6 +215,46 @@ void pmh_string_free(char *s);
  * and the summed milliseconds as JSON {"name": [launches, ms], ...}. */
 int pmh_set_profiling(int enabled);
 int pmh_kernel_times(char **out_json);
+
+/* ------------------------------------------------------------------------
+ * Learned sizer.  No counterpart in the reference, which sizes nothing from
+ * data (its nearest precedents are fixed thresholds, cost.rs:38 and
+ * groupby.rs:493); these entries exist for the cardinality / path model
+ * that method "auto" may consult.  The model is a 3-layer fp32 MLP
+ * (PMH_MLP_IN -> 64 -> 64 -> 2) trained and run on the device.  Outputs per
+ * sample: [log2(groups + 1) / 32, logit(sort engine faster than hash)].
+ * ---------------------------------------------------------------------- */
+#define PMH_MLP_IN 16
+#define PMH_MLP_HIDDEN 64
+#define PMH_MLP_PARAMS (PMH_MLP_IN * 64 + 64 + 64 * 64 + 64 + 64 * 2 + 2)
+
+typedef struct pmh_mlp pmh_mlp;
+
+/* Runs only the partitioned engine's 64K-row sample sketch of a group-by
+ * plan (predicate included) and returns its PMH_MLP_IN features and the
+ * sketch fields of *stats.  No counterpart in the reference. */
+int pmh_sketch_features(const char *plan_json, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols,
+                        float *features, pmh_exec_stats *stats);
+
+/* A device-resident model initialised from PMH_MLP_PARAMS host floats. */
+int pmh_mlp_new(const float *init_params, pmh_mlp **out);
+void pmh_mlp_free(pmh_mlp *m);
+int pmh_mlp_params(pmh_mlp *m, float *out_params);
+/* y[2n] = model(x[n][PMH_MLP_IN]) (host arrays). */
+int pmh_mlp_forward(pmh_mlp *m, const float *x, uint32_t n, float *y);
+/* Gradient of the mean loss over (x, y) at the current parameters into
+ * grad[PMH_MLP_PARAMS]; *loss gets the mean loss.  Targets y[2n]:
+ * [log2(groups + 1) / 32, 1 sort faster | 0 hash faster | -1 unknown]. */
+int pmh_mlp_grad(pmh_mlp *m, const float *x, const float *y, uint32_t n, float lambda, float *grad, float *loss);
+/* Uploads a training set, then runs minibatch Adam over it on the device. */
+int pmh_mlp_set_data(pmh_mlp *m, const float *x, const float *y, uint32_t n);
+int pmh_mlp_train(pmh_mlp *m, uint32_t epochs, uint32_t batch, float lr, float lambda, uint32_t seed,
+                  float *last_epoch_loss);
+/* Installs PMH_MLP_PARAMS floats as the model method "auto" consults after
+ * its sample sketch (NULL uninstalls).  The model only picks the engine and
+ * sizes the partitioned tables; the overflow fallback keeps every result
+ * exact whatever it predicts. */
+int pmh_sizer_install(const float *params);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,7 @@
 #include <sstream>
 
 #include "pmh_internal.h"
+#include "pmh_mlp.h"
 
 namespace pmh {
 const char *last_error();
@@ -235,6 +236,76 @@ int pmh_kernel_times(char **out_json) {
         os << "}";
         *out_json = dup_string(os.str());
     });
+}
+
+int pmh_sketch_features(const char *plan_json, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols,
+                        float *features, pmh_exec_stats *stats) {
+    return guarded([&] {
+        if (!features) fail(PMH_E_VALUE, "polars_metal: features is NULL");
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        GroupByPlan plan = parse_groupby_plan(plan_json, true);
+        run_sketch(plan, n_rows, cols, n_cols, features, stats);
+    });
+}
+
+struct pmh_mlp {
+    pmh::Mlp *m;
+};
+
+int pmh_mlp_new(const float *init_params, pmh_mlp **out) {
+    return guarded([&] {
+        if (!out || !init_params) fail(PMH_E_VALUE, "polars_metal: mlp_new needs parameters and an out pointer");
+        *out = nullptr;
+        *out = new pmh_mlp{mlp_new(init_params)};
+    });
+}
+
+void pmh_mlp_free(pmh_mlp *m) {
+    if (!m) return;
+    guarded([&] { mlp_free(m->m); });
+    delete m;
+}
+
+int pmh_mlp_params(pmh_mlp *m, float *out_params) {
+    return guarded([&] {
+        if (!m || !out_params) fail(PMH_E_VALUE, "polars_metal: mlp_params needs a model and a buffer");
+        mlp_get_params(m->m, out_params);
+    });
+}
+
+int pmh_mlp_forward(pmh_mlp *m, const float *x, uint32_t n, float *y) {
+    return guarded([&] {
+        if (!m || (n && (!x || !y))) fail(PMH_E_VALUE, "polars_metal: mlp_forward needs a model and buffers");
+        mlp_forward_host(m->m, x, n, y);
+    });
+}
+
+int pmh_mlp_grad(pmh_mlp *m, const float *x, const float *y, uint32_t n, float lambda, float *grad, float *loss) {
+    return guarded([&] {
+        if (!m || !x || !y || !grad) fail(PMH_E_VALUE, "polars_metal: mlp_grad needs a model and buffers");
+        const float l = mlp_grad_host(m->m, x, y, n, lambda, grad);
+        if (loss) *loss = l;
+    });
+}
+
+int pmh_mlp_set_data(pmh_mlp *m, const float *x, const float *y, uint32_t n) {
+    return guarded([&] {
+        if (!m || (n && (!x || !y))) fail(PMH_E_VALUE, "polars_metal: mlp_set_data needs a model and buffers");
+        mlp_set_data(m->m, x, y, n);
+    });
+}
+
+int pmh_mlp_train(pmh_mlp *m, uint32_t epochs, uint32_t batch, float lr, float lambda, uint32_t seed,
+                  float *last_epoch_loss) {
+    return guarded([&] {
+        if (!m) fail(PMH_E_VALUE, "polars_metal: mlp_train needs a model");
+        const float l = mlp_train(m->m, epochs, batch, lr, lambda, seed);
+        if (last_epoch_loss) *last_epoch_loss = l;
+    });
+}
+
+int pmh_sizer_install(const float *params) {
+    return guarded([&] { sizer_install(params); });
 }
 
 }  // extern "C"

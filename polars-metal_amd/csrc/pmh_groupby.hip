@@ -810,9 +810,29 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
         // resizes) then runs the query
         pmh_result *r = run_hash_partitioned(P, plan, flags);
         if (r) return r;
+        if (P.route == 1) {  // the learned sizer chose the sort engine
+            r = run_sort(P, plan, flags);
+            r->stats.sizer_choice = 2;
+            return r;
+        }
     }
     if (m == "sort") return run_sort(P, plan, flags);
     return run_hash_global(P, plan, flags);
+}
+
+// The partitioned engine's sample sketch alone: the learned sizer's
+// features and the sketch fields of the stats.
+void run_sketch(GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols, float *feat,
+                pmh_exec_stats *stats) {
+    plan.opts.method = "hash";
+    plan.opts.sketch_only = true;
+    Prep P;
+    prepare_query(P, plan, n_rows, cols, n_cols);
+    pmh_result *r = run_hash_partitioned(P, plan, 0);
+    if (!r) fail(PMH_E_NOTIMPL, "polars_metal: no sample sketch for this query shape");
+    if (stats) *stats = r->stats;
+    pmh_result_free(r);
+    memcpy(feat, P.feat, sizeof(P.feat));
 }
 
 }  // namespace pmh

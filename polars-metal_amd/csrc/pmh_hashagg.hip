@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "pmh_mlp.h"
 #include "pmh_table.h"
 
 namespace pmh {
@@ -46,20 +47,6 @@ struct Spill {
     uint64_t *klo, *khi;  // khi: wide keys only
     void *val[kMaxVals];  // value j as raw bits of width vw[j]
     uint16_t *flags;      // bit j: value j valid; kFlagAllOnes / kFlagNull: side keys
-};
-
-// Host-lowered form of a conjunctive predicate for the specialised scan: a
-// row is selected iff, for every r < nr, column rcol[r] is valid and its
-// int64 image (the value itself, or its TotalOrd image when rflt[r]) lies in
-// [rlo[r], rhi[r]].
-struct FastScan {
-    int32_t enabled, nr;
-    int32_t rcol[2], rflt[2];
-    int64_t rlo[2], rhi[2];
-    uint32_t ring_off;       // byte offset of the per-wave row rings in LDS
-    int32_t rows;            // rows per lane per step (4 or 8)
-    int32_t touch;           // touch key/value rows one step ahead of their gathers
-    int32_t pad;
 };
 
 enum StatIdx : int32_t {
@@ -199,6 +186,23 @@ __global__ __launch_bounds__(256) void hs_sample_gather(const HLaunch *__restric
 // most frequent fingerprints (up to hot_max), recover their keys from the
 // sample, lay them out in the hot table, and record the sketch (sampled rows,
 // distinct fingerprints, f1, f2).  It also resets the global hot state.
+// The hot table (CL slots, a power of two, CL >= 16): slots [0, CL - 2)
+// hold the hashed hot keys; slot CL - 2 is the group of the all-ones key
+// and slot CL - 1 the group of the null key of a single 64-bit key column,
+// the two keys the empty marker cannot stand for.  Probes never visit them.
+__device__ __forceinline__ uint32_t hot_home(uint64_t h, uint32_t CL) {
+    const uint32_t s = (uint32_t)(h >> 40) & (CL - 1);
+    return s >= CL - 2 ? s - 2 : s;
+}
+__device__ __forceinline__ uint32_t hot_next(uint32_t s, uint32_t CL) { return s + 1 == CL - 2 ? 0 : s + 1; }
+__device__ __forceinline__ uint32_t hot_side_slot(int special, uint32_t CL) { return CL - 3 + (uint32_t)special; }
+
+// Probe limit of the sample fingerprint table: a key that finds no slot in
+// this many probes is dropped (and counted), which keeps a sample with more
+// distinct keys than slots from probing the whole table for every row.  A
+// key placed within kSampleProbe probes is found again within as many.
+constexpr int kSampleProbe = 8;
+
 __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     const QueryDesc *__restrict__ q = &L->q;
@@ -225,7 +229,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         const uint32_t fp = (uint32_t)(h >> 32) | 1u;
         uint32_t s = (uint32_t)h & (SS - 1);
         bool done = false;
-        for (int p = 0; p < 64; ++p) {
+        for (int p = 0; p < kSampleProbe; ++p) {
             uint32_t cur = s_fp[s];
             if (cur == 0) {
                 const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
@@ -287,14 +291,14 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         const uint64_t h = hash_key(lo, hi);
         const uint32_t fp = (uint32_t)(h >> 32) | 1u;
         uint32_t s = (uint32_t)h & (SS - 1);
-        for (int p = 0; p < 64; ++p) {
+        for (int p = 0; p < kSampleProbe; ++p) {
             const uint32_t cur = s_fp[s];
             if (cur == 0) break;
             if (cur == fp) {
                 if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
                     hot_rows += s_cnt[s] & 0x3FFFFFFFu;
-                    uint32_t t = (uint32_t)(h >> 40) & (CL - 1);
-                    for (uint32_t pp = 0; pp < CL; ++pp) {
+                    uint32_t t = hot_home(h, CL);
+                    for (uint32_t pp = 0; pp < CL - 2; ++pp) {
                         unsigned long long old = atomicCAS((unsigned long long *)&h_lo[t], (unsigned long long)kEmpty,
                                                            (unsigned long long)lo);
                         if (old == kEmpty) {
@@ -302,7 +306,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
                             atomicAdd(&s_hot, 1u);
                             break;
                         }
-                        t = (t + 1) & (CL - 1);
+                        t = hot_next(t, CL);
                     }
                 }
                 break;
@@ -349,7 +353,7 @@ __device__ __forceinline__ bool scan_row(const HLaunch *__restrict__ L, const Qu
     k = load_key(q, row);
     if (k.special || c.CL == 0) return true;
     const uint64_t h = hash_key(k.lo, k.hi);
-    uint32_t s = (uint32_t)(h >> 40) & (c.CL - 1);
+    uint32_t s = hot_home(h, c.CL);
     for (int p = 0; p < kProbe; ++p) {
         const uint64_t cur = c.lkeys[s];
         if (cur == kEmpty) return true;
@@ -357,7 +361,7 @@ __device__ __forceinline__ bool scan_row(const HLaunch *__restrict__ L, const Qu
             add_row(c.ls, q, s, row);
             return false;
         }
-        s = (s + 1) & (c.CL - 1);
+        s = hot_next(s, c.CL);
     }
     return true;
 }
@@ -908,7 +912,14 @@ __global__ __launch_bounds__(256) void hs_hot_emit(const HLaunch *__restrict__ L
     __syncthreads();
     if (threadIdx.x == 0) s_base = s_count ? atomicAdd(L->out_cursor, s_count) : 0;
     __syncthreads();
-    if (used) emit_group(q, L->out, L->n_out, rd, i, L->hot_lo[i], L->wide ? L->hot_hi[i] : 0, 0, s_base + off);
+    if (!used) return;
+    if (i >= CL - 2) {  // side slots: the all-ones key, the null key
+        const int special = (int)(i - (CL - 3));
+        emit_group(q, L->out, L->n_out, rd, i, special == 1 ? kEmpty : 0, special == 1 && L->wide ? kEmpty : 0,
+                   special, s_base + off);
+    } else {
+        emit_group(q, L->out, L->n_out, rd, i, L->hot_lo[i], L->wide ? L->hot_hi[i] : 0, 0, s_base + off);
+    }
 }
 
 // Groups of the fallback table, written after the cursor's groups; resets
@@ -1090,17 +1101,51 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
 // (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
 // estimator (Haas et al., 1995; it overshoots), floored at the sample's own
 // distinct count.  A full fingerprint table extrapolates linearly.
-double plain_distinct_estimate(const Sketch &k) {
+double gee_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
+    return std::sqrt(std::max(k.n_sel, s) / s) * f1 + (d - f1);
+}
+double shlosser_estimate(const Sketch &k) {
+    if (k.sample_rows == 0) return 0.0;
+    return k.shl_den > 0 ? k.distinct + k.f1 * k.shl_num / k.shl_den : gee_estimate(k);
+}
+double plain_distinct_estimate(const Sketch &k) {
+    if (k.sample_rows == 0) return 0.0;
+    const double s = k.sample_rows, d = k.distinct;
     if (k.dropped) return std::max(d, k.n_sel * (d + k.dropped) / s);
     if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
-    const double gee = std::sqrt(k.n_sel / s) * f1 + (d - f1);
-    const double shl = k.shl_den > 0 ? d + f1 * k.shl_num / k.shl_den : gee;
+    const double gee = gee_estimate(k), shl = shlosser_estimate(k);
     return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));
 }
 
+// Features of the learned sizer: scale-free quantities of the sketch, each
+// roughly in [0, 1].
+void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f) {
+    for (int i = 0; i < PMH_MLP_IN; ++i) f[i] = 0.f;
+    const double s = std::max<double>(1.0, k.sample_rows);
+    auto lg = [](double x) { return (float)(std::log2(std::max(0.0, x) + 1.0) / 32.0); };
+    f[0] = lg((double)k.n_rows);
+    f[1] = k.sample_size ? (float)((double)k.sample_rows / k.sample_size) : 0.f;
+    f[2] = lg(k.distinct);
+    f[3] = (float)(k.f1 / s);
+    f[4] = (float)(k.f2 / s);
+    f[5] = (float)(k.distinct / s);
+    f[6] = lg(gee_estimate(k));
+    f[7] = lg(shlosser_estimate(k));
+    f[8] = lg(plain_distinct_estimate(k));
+    f[9] = (float)(k.F2 / (s * s));
+    f[10] = (float)(null_keys / s);
+    f[11] = k.dropped ? 1.f : 0.f;
+    f[12] = lane_bits <= 33 ? 1.f : 0.f;
+    f[13] = lane_bits > 33 && lane_bits <= 65 ? 1.f : 0.f;
+    f[14] = lane_bits > 65 ? 1.f : 0.f;
+    f[15] = lg(k.n_sel);
+}
+
 namespace {
+
+}  // namespace
 
 // Lower the predicate for hs_scan_fast: an AND tree whose leaves compare one
 // column with a literal (any operator but Ne), test a Boolean column, or are
@@ -1173,6 +1218,8 @@ bool lower_fast_scan(const QueryDesc &q, FastScan &fs) {
     }
     return true;
 }
+
+namespace {
 
 void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, const int32_t *vw, int nv) {
     S.klo = (uint64_t *)slab(tag + ":klo", cap * 8 + 64);
@@ -1370,6 +1417,38 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     Sketch sketch = sketch_from(sk[ST_SAMPLE_SEL], n_sample, n, sk[ST_SAMPLE_DISTINCT], sk[ST_SAMPLE_F1],
                                 sk[ST_SAMPLE_F2], sk[ST_SAMPLE_DROPPED], hist);
     double d_est = plain_distinct_estimate(sketch);
+    sizer_features(sketch, sk[ST_SAMPLE_NULLKEY], q0.total_bits, P.feat);
+    if (opt.sketch_only) {
+        collect_kernel_times();
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+        pmh_result *res = new_result(0);
+        res->stats.method = PMH_METHOD_HASH;
+        res->stats.sample_rows = sk[ST_SAMPLE_SEL];
+        res->stats.sample_distinct = sk[ST_SAMPLE_DISTINCT];
+        res->stats.sample_f1 = sk[ST_SAMPLE_F1];
+        res->stats.sample_f2 = sk[ST_SAMPLE_F2];
+        res->stats.hot_keys = sk[ST_HOT];
+        res->stats.distinct_estimate = d_est;
+        return res;
+    }
+    if (opt.method == "auto" && opt.distinct_hint <= 0 && sizer_installed()) {
+        // the learned sizer picks the engine and, for hash, the cardinality
+        // the partitions are sized for; overflow still falls back exactly
+        float pred[2];
+        sizer_predict(P.feat, pred);
+        if (pred[1] > 0.f) {
+            P.route = 1;
+            P.sizer_choice = 2;
+            collect_kernel_times();
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+            return nullptr;
+        }
+        P.sizer_choice = 1;
+        const double g = std::exp2(std::min(40.0, (double)pred[0] * 32.0)) - 1.0;
+        d_est = std::max<double>(g, sk[ST_SAMPLE_DISTINCT]);
+    }
     if (opt.distinct_hint > 0) d_est = (double)opt.distinct_hint;
     const double d_tail = std::max(0.0, d_est - sk[ST_HOT]);
     uint64_t P_parts;
@@ -1517,6 +1596,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     res->stats.tail_rows = host_stats[ST_TAIL];
     res->stats.distinct_estimate = d_est;
     res->stats.scan_kernel = L.fs.enabled;
+    res->stats.sizer_choice = P.sizer_choice;
     const uint64_t vbytes = bitmap_bytes_padded(ng);
     const uint64_t packed = (ng + 7) / 8;
     for (int oi = 0; oi < P.n_out; ++oi) {

@@ -169,6 +169,7 @@ struct Options {
     int64_t fast_scan = 1;        // partitioned engine: 1 use the specialised scan where the query fits it, 0 never
     int64_t scan_rows = 4;        // specialised scan: rows per lane per step (4 or 8)
     int64_t scan_touch = 0;       // specialised scan: 1 touches key/value rows one step ahead of their gathers
+    bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 
 struct GroupByPlan {
@@ -229,6 +230,10 @@ void set_out_name(pmh_out_column &c, const std::string &name);
 // group-by entry (pmh_groupby.hip)
 pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols,
                                uint32_t n_cols, uint32_t flags);
+
+// sample sketch only: the learned sizer's features (pmh_groupby.hip)
+void run_sketch(GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols, float *feat,
+                pmh_exec_stats *stats);
 
 // filter entries (pmh_filter.hip)
 pmh_result *run_filter_compact(const uint8_t *pd, const uint8_t *pv, uint64_t n_rows,

@@ -29,95 +29,13 @@
 //    never fetched.
 #pragma once
 
+#include "pmh_words.h"
+
 namespace pmh {
 
 // Per-wave ring of selected-row offsets: at most 63 carried rows plus one
 // row step of 64 new ones are queued at any time.
 constexpr uint32_t kRing = 127;
-
-// Stand-in validity bitmap for columns without one: every bit set.
-__device__ const uint64_t pmh_all_valid[1] = {~0ull};
-
-// Order-preserving int64 image of a double under Polars' TotalOrd (the order
-// of tot_cmp): -0.0 and +0.0 map together, every NaN maps above +inf.
-__host__ __device__ inline int64_t tot_key_bits(int64_t b, bool is_nan) {
-    if (is_nan) return 0x7FFFFFFFFFFFFFFFll;
-    if ((b & 0x7FFFFFFFFFFFFFFFll) == 0) return 0;
-    return b >= 0 ? b : (b ^ 0x7FFFFFFFFFFFFFFFll);
-}
-__device__ __forceinline__ int64_t tot_key(double x) { return tot_key_bits(__double_as_longlong(x), x != x); }
-
-// A column as one role of the query sees it, addressed by bit position:
-// value r occupies bits [bit0 + (r << bshift), ... + width) of the word
-// array at base; its validity bit is bit vbit0 + r * vstride of vbase.
-struct RoleCol {
-    const uint64_t *base, *vbase;
-    uint64_t mask;
-    uint32_t bit0, vbit0, vstride;
-    int32_t bshift, tag, w;  // w: bytes per value, 0 for bit-packed Boolean
-};
-__device__ __forceinline__ RoleCol role_col(const QueryDesc *__restrict__ q, int c) {
-    RoleCol r;
-    const uintptr_t d = (uintptr_t)q->cols[c].data;
-    const uintptr_t v = (uintptr_t)q->cols[c].valid;
-    r.tag = q->cols[c].tag;
-    r.w = r.tag == PMH_BOOL ? 0 : tag_size(r.tag);
-    r.base = (const uint64_t *)(d & ~uintptr_t(7));
-    r.bit0 = (uint32_t)(d & 7) * 8;
-    r.bshift = r.w == 8 ? 6 : r.w == 4 ? 5 : r.w == 2 ? 4 : r.w == 1 ? 3 : 0;
-    r.mask = r.w == 8 ? ~0ull : r.w == 0 ? 1ull : (1ull << (8 * r.w)) - 1;
-    r.vbase = v ? (const uint64_t *)(v & ~uintptr_t(7)) : pmh_all_valid;
-    r.vbit0 = v ? (uint32_t)(v & 7) * 8 : 0;
-    r.vstride = v ? 1 : 0;
-    return r;
-}
-__device__ __forceinline__ uint64_t vpos(const RoleCol &c, uint64_t r) { return c.bit0 + (r << c.bshift); }
-__device__ __forceinline__ uint64_t bpos(const RoleCol &c, uint64_t r) { return c.vbit0 + r * c.vstride; }
-__device__ __forceinline__ uint64_t word_at(const uint64_t *base, uint64_t pos) { return gld<uint64_t>(base, pos >> 6); }
-__device__ __forceinline__ uint64_t cut(uint64_t word, uint64_t pos, uint64_t mask) { return (word >> (pos & 63)) & mask; }
-
-// Predicate images of U values of one column (raw bits, zero-extended): the
-// column's TotalOrd image when flt, else its int64 value.  The tag switch is
-// taken once for all U values.
-template <int U>
-__device__ __forceinline__ void pred_images(int tag, bool flt, const uint64_t (&raw)[U], int64_t (&img)[U]) {
-    if (flt) {
-        if (tag == PMH_F64) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) img[u] = tot_key(__longlong_as_double((long long)raw[u]));
-        } else if (tag == PMH_F32) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) img[u] = tot_key((double)__uint_as_float((uint32_t)raw[u]));
-        } else {
-#pragma unroll
-            for (int u = 0; u < U; ++u) img[u] = tot_key((double)raw_to_i64(tag, raw[u]));
-        }
-        return;
-    }
-    switch (tag) {
-    case PMH_I32:
-#pragma unroll
-        for (int u = 0; u < U; ++u) img[u] = (int32_t)(uint32_t)raw[u];
-        break;
-    case PMH_I16:
-#pragma unroll
-        for (int u = 0; u < U; ++u) img[u] = (int16_t)(uint16_t)raw[u];
-        break;
-    case PMH_I8:
-#pragma unroll
-        for (int u = 0; u < U; ++u) img[u] = (int8_t)(uint8_t)raw[u];
-        break;
-    case PMH_F64:
-    case PMH_F32:
-#pragma unroll
-        for (int u = 0; u < U; ++u) img[u] = raw_to_i64(tag, raw[u]);
-        break;
-    default:  // I64, U64, U32, U16, U8, Utf8 codes, Bool: the raw bits
-#pragma unroll
-        for (int u = 0; u < U; ++u) img[u] = (int64_t)raw[u];
-        break;
-    }
-}
 
 // Exact f64 accumulation into LDS limbs (limb k of slot s at limb[k * SL + s])
 // with one divergent region only for the rare values outside the fixed-point
@@ -383,14 +301,18 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     vv |= (uint32_t)cut(vv_[j], bpos(vc[j], row), 1) << j;
                 }
                 // hot-table probe
+                // the all-ones and null keys fold into the hot table's side
+                // slots instead of spilling (they would otherwise pile up on
+                // one partition slot of the tail)
                 int32_t slot = -1;
+                if (act && special && CL) slot = (int32_t)hot_side_slot(special, CL);
                 if (act && special == 0 && CL) {
-                    uint32_t sp = (uint32_t)(hash_key(key, 0) >> 40) & (CL - 1);
+                    uint32_t sp = hot_home(hash_key(key, 0), CL);
                     for (int pr = 0; pr < kProbe; ++pr) {
                         const uint64_t cur = lkeys[sp];
                         if (cur == key) { slot = (int32_t)sp; break; }
                         if (cur == kEmpty) break;
-                        sp = (sp + 1) & (CL - 1);
+                        sp = hot_next(sp, CL);
                     }
                 }
                 // hot rows: fold into LDS

@@ -34,6 +34,7 @@
 #include <cstring>
 
 #include "pmh_table.h"
+#include "pmh_words.h"
 
 namespace pmh {
 
@@ -71,7 +72,40 @@ struct SLaunch {
     uint64_t *gk_lo, *gk_hi;
     void *arr[kMaxArrs];
     OutCol out[kOutMax];
+    FastScan fs;  // the predicate as intervals, when fast is set
+    int32_t fast, pad3;
+    // per-block partials of the bits pass (or_lo, or_hi, and_lo, and_hi,
+    // selected rows), then each block's first record for the encode pass:
+    // both passes map rows to blocks the same way, so no pass needs a
+    // global cursor (one contended address would serialise every wave)
+    uint64_t *bpart;  // [nblk][5]
+    uint64_t *boff;   // [nblk]
+    uint32_t nblk, pad4;
 };
+
+// Wave partials of the bits pass into the block's entry of bpart.
+__device__ __forceinline__ void block_bits_out(const SLaunch *__restrict__ L, uint64_t olo, uint64_t ohi,
+                                               uint64_t alo, uint64_t ahi, uint32_t cnt) {
+    __shared__ unsigned long long s[5];
+    if (threadIdx.x == 0) { s[0] = s[1] = 0; s[2] = s[3] = ~0ull; s[4] = 0; }
+    __syncthreads();
+    for (int o = 32; o; o >>= 1) {
+        olo |= __shfl_xor(olo, o);
+        ohi |= __shfl_xor(ohi, o);
+        alo &= __shfl_xor(alo, o);
+        ahi &= __shfl_xor(ahi, o);
+        cnt += __shfl_xor(cnt, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr(&s[0], olo);
+        atomicOr(&s[1], ohi);
+        atomicAnd(&s[2], alo);
+        atomicAnd(&s[3], ahi);
+        atomicAdd(&s[4], (unsigned long long)cnt);
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) gst<uint64_t>(L->bpart, (uint64_t)blockIdx.x * 5 + threadIdx.x, s[threadIdx.x]);
+}
 
 // The lane the sort orders by.  Lane modes: the key encoding itself.  One
 // 64-bit key column: the reference's encoding of that key (null bit 0, data
@@ -153,18 +187,50 @@ __global__ __launch_bounds__(256) void so_bits(const SLaunch *__restrict__ L) {
         ahi &= hi;
         ++sel;
     }
-    olo = wave_or(olo);
-    ohi = wave_or(ohi);
-    alo = wave_and(alo);
-    ahi = wave_and(ahi);
-    uint32_t s = sel;
-    for (int o = 32; o; o >>= 1) s += __shfl_xor(s, o);
-    if ((threadIdx.x & 63) == 0) {
-        atomicOr((unsigned long long *)&L->ctr[SC_OR_LO], olo);
-        atomicOr((unsigned long long *)&L->ctr[SC_OR_HI], ohi);
-        atomicAnd((unsigned long long *)&L->ctr[SC_AND_LO], alo);
-        atomicAnd((unsigned long long *)&L->ctr[SC_AND_HI], ahi);
-        atomicAdd((unsigned long long *)&L->ctr[SC_SEL], (unsigned long long)s);
+    block_bits_out(L, olo, ohi, alo, ahi, sel);
+}
+
+// Reduces the blocks' partials into ctr and turns the blocks' selected
+// counts into their first record (exclusive prefix).
+__global__ __launch_bounds__(1024) void so_bits_final(const SLaunch *__restrict__ L) {
+    __shared__ unsigned long long s[4];
+    __shared__ uint64_t sc[1024];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) { s[0] = s[1] = 0; s[2] = s[3] = ~0ull; carry = 0; }
+    __syncthreads();
+    uint64_t olo = 0, ohi = 0, alo = ~0ull, ahi = ~0ull;
+    for (uint32_t b = threadIdx.x; b < L->nblk; b += blockDim.x) {
+        olo |= L->bpart[b * 5 + 0];
+        ohi |= L->bpart[b * 5 + 1];
+        alo &= L->bpart[b * 5 + 2];
+        ahi &= L->bpart[b * 5 + 3];
+    }
+    atomicOr(&s[0], olo);
+    atomicOr(&s[1], ohi);
+    atomicAnd(&s[2], alo);
+    atomicAnd(&s[3], ahi);
+    for (uint32_t b0 = 0; b0 < L->nblk; b0 += 1024) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint64_t x = b < L->nblk ? L->bpart[b * 5 + 4] : 0;
+        sc[threadIdx.x] = x;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const uint64_t y = threadIdx.x >= (uint32_t)off ? sc[threadIdx.x - off] : 0;
+            __syncthreads();
+            sc[threadIdx.x] += y;
+            __syncthreads();
+        }
+        if (b < L->nblk) L->boff[b] = carry + sc[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += sc[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        L->ctr[SC_OR_LO] = s[0];
+        L->ctr[SC_OR_HI] = s[1];
+        L->ctr[SC_AND_LO] = s[2];
+        L->ctr[SC_AND_HI] = s[3];
+        L->ctr[SC_SEL] = carry;
     }
 }
 
@@ -173,9 +239,156 @@ __global__ void so_reset(const SLaunch *__restrict__ L) {
 }
 
 // ---------------------------------------------------------------------------
+// 1+2, the common shape: an interval predicate (see FastScan) and at most
+// four key fields, read as aligned words (pmh_words.h), four rows per lane,
+// every load issued before any value is used.  ENCODE false: varying bits
+// (as so_bits); true: records (as so_encode).
+// ---------------------------------------------------------------------------
+template <bool ENCODE>
+__global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ L) {
+    constexpr int U = 4;
+    __shared__ Run s_runs[kMaxRuns];
+    __shared__ uint32_t s_cur;
+    const QueryDesc *__restrict__ q = &L->q;
+    if (threadIdx.x < kMaxRuns) s_runs[threadIdx.x] = L->runs[threadIdx.x];
+    if (threadIdx.x == 0) s_cur = 0;
+    __syncthreads();
+    const uint64_t bbase = ENCODE ? L->boff[blockIdx.x] : 0;
+    const int nruns = L->n_runs;
+    const int nr = L->fs.nr;
+    RoleCol pc[2];
+    int pflt[2];
+    int64_t plo[2], phi[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        pc[r] = role_col(q, r < nr ? L->fs.rcol[r] : 0);
+        pflt[r] = L->fs.rflt[r];
+        plo[r] = L->fs.rlo[r];
+        phi[r] = L->fs.rhi[r];
+    }
+    const bool raw = q->key_mode == KM_RAW64;
+    const int nk = q->key_mode == KM_NONE ? 0 : q->n_keys;
+    RoleCol kc[4];
+    int knull[4], kdata[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        kc[i] = role_col(q, i < nk ? q->keys[i].col : 0);
+        knull[i] = i < nk ? q->keys[i].null_off : 0;
+        kdata[i] = i < nk ? q->keys[i].data_off : 0;
+    }
+    const bool two = L->kw == 2;
+    const int lane = threadIdx.x & 63;
+    const uint64_t n = q->n_rows;
+    const uint64_t per = (uint64_t)gridDim.x * blockDim.x * U;
+    const uint64_t iters = (n + per - 1) / per;  // uniform: ballots see whole waves
+    uint64_t olo = 0, ohi = 0, alo = ~0ull, ahi = ~0ull;
+    uint32_t cnt = 0;
+    for (uint64_t it = 0; it < iters; ++it) {
+        const uint64_t base = it * per + (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x;
+        uint64_t row[U];
+        uint32_t sel = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            row[u] = base + (uint64_t)u * blockDim.x;
+            sel |= (uint32_t)(row[u] < n) << u;
+            row[u] = row[u] < n ? row[u] : n - 1;
+        }
+        uint64_t pw[2][U], pv[2][U], kw[4][U], kv[4][U];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) continue;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                pw[r][u] = word_at(pc[r].base, vpos(pc[r], row[u]));
+                pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row[u]));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) kw[i][u] = kv[i][u] = 0;
+            if (i >= nk) continue;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                kw[i][u] = word_at(kc[i].base, vpos(kc[i], row[u]));
+                kv[i][u] = word_at(kc[i].vbase, bpos(kc[i], row[u]));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            if (r >= nr) continue;
+            uint64_t v[U];
+            uint32_t vb = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                v[u] = cut(pw[r][u], vpos(pc[r], row[u]), pc[r].mask);
+                vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], row[u]), 1) << u;
+            }
+            int64_t img[U];
+            pred_images<U>(pc[r].tag, pflt[r] != 0, v, img);
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (img[u] < plo[r] || img[u] > phi[r]) vb &= ~(1u << u);
+            sel &= vb;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint64_t lo = 0, hi = 0;
+            if (raw) {
+                const bool ok = cut(kv[0][u], bpos(kc[0], row[u]), 1);
+                lo = ok ? kw[0][u] << 1 : 1;
+                hi = ok ? kw[0][u] >> 63 : 0;
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (i >= nk) continue;
+                    if (cut(kv[i][u], bpos(kc[i], row[u]), 1))
+                        put_bits(lo, hi, cut(kw[i][u], vpos(kc[i], row[u]), kc[i].mask), kdata[i]);
+                    else
+                        put_bits(lo, hi, 1, knull[i]);
+                }
+            }
+            const bool s = (sel >> u) & 1;
+            if (!ENCODE) {
+                if (s) {
+                    olo |= lo;
+                    ohi |= hi;
+                    alo &= lo;
+                    ahi &= hi;
+                    ++cnt;
+                }
+                continue;
+            }
+            uint64_t clo = 0, chi = 0;
+            for (int k = 0; k < nruns; ++k) {
+                const Run rr = s_runs[k];
+                put_bits(clo, chi, bits_at(lo, hi, rr.pos, rr.len), rr.at);
+            }
+            const unsigned long long m = __ballot(s);
+            if (!m) continue;
+            const int leader = __ffsll((long long)m) - 1;
+            uint64_t b = 0;
+            if (lane == leader) b = bbase + atomicAdd(&s_cur, (uint32_t)__popcll(m));
+            b = __shfl(b, leader);
+            if (s) {
+                const uint64_t pos = b + (uint64_t)__popcll(m & ((1ull << lane) - 1));
+                gst<uint64_t>(L->ka_lo, pos, clo);
+                if (two) gst<uint64_t>(L->ka_hi, pos, chi);
+                gst<uint32_t>(L->ra, pos, (uint32_t)row[u]);
+            }
+        }
+    }
+    if (!ENCODE) block_bits_out(L, olo, ohi, alo, ahi, cnt);
+}
+
+// ---------------------------------------------------------------------------
 // 2. records
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void so_encode(const SLaunch *__restrict__ L) {
+    __shared__ uint32_t s_cur;
+    if (threadIdx.x == 0) s_cur = 0;
+    __syncthreads();
+    const uint64_t bbase = L->boff[blockIdx.x];
     const QueryDesc *__restrict__ q = &L->q;
     const uint64_t n = q->n_rows;
     const int lane = threadIdx.x & 63;
@@ -200,8 +413,7 @@ __global__ __launch_bounds__(256) void so_encode(const SLaunch *__restrict__ L) 
         if (!m) continue;
         const int leader = __ffsll((long long)m) - 1;
         uint64_t base = 0;
-        if (lane == leader)
-            base = atomicAdd((unsigned long long *)&L->ctr[SC_CURSOR], (unsigned long long)__popcll(m));
+        if (lane == leader) base = bbase + atomicAdd(&s_cur, (uint32_t)__popcll(m));
         base = __shfl(base, leader);
         if (!s) continue;
         const uint64_t pos = base + (uint64_t)__popcll(m & ((1ull << lane) - 1));
@@ -614,6 +826,214 @@ __global__ __launch_bounds__(kSortThreads) void so_reduce(const SLaunch *__restr
     }
 }
 
+// The common shape (at most two aggregated columns): one wave per tile of
+// kSortTile records, 64 records per iteration.  Runs inside an iteration
+// fold with segmented scans as in so_reduce; the iteration's last run stays
+// open in registers (carry) and joins the next iteration's first run, so a
+// group spanning many records costs one atomic per tile and state array
+// instead of one per wave.
+struct Carry {
+    uint32_t len, cnt[2], nan[2];
+    int64_t limb[2][kLimbs], sum[2], mn[2], mx[2];
+};
+
+__global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                     const uint64_t *khi, const uint32_t *krow) {
+    const QueryDesc *__restrict__ q = &L->q;
+    const Layout &lay = q->layout;
+    const bool two = L->kw == 2;
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (t >= L->ntiles) return;  // whole waves leave together
+    const int nv = q->n_vals;
+    RoleCol vc[2];
+    ValSpec vs[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        vs[j] = q->vals[j < nv ? j : 0];
+        vc[j] = role_col(q, vs[j].col);
+    }
+    GSink g{L->arr, L->gcap};
+    auto addu = [](uint32_t a, uint32_t b) { return a + b; };
+    auto add64 = [](int64_t a, int64_t b) { return a + b; };
+    auto mn64 = [](int64_t a, int64_t b) { return a < b ? a : b; };
+    auto mx64 = [](int64_t a, int64_t b) { return a > b ? a : b; };
+    Carry c{};
+    bool have = false;
+    uint64_t cgid = 0;
+    auto reset = [&]() {
+        c.len = 0;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            c.cnt[j] = c.nan[j] = 0;
+            c.sum[j] = 0;
+            c.mn[j] = INT64_MAX;
+            c.mx[j] = INT64_MIN;
+#pragma unroll
+            for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = 0;
+        }
+    };
+    // one lane writes the totals of one run of group gid
+    auto flush = [&](uint64_t gid, const Carry &x) {
+        if (x.len) g.add32(lay.a_len, gid, x.len);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= nv || !x.cnt[j]) continue;
+            const ValSpec &S = vs[j];
+            g.add32(S.a_cnt, gid, x.cnt[j]);
+            if (S.need & NEED_SUM) {
+                if (S.kind == VK_F) {
+#pragma unroll
+                    for (int z = 0; z < kLimbs; ++z)
+                        if (x.limb[j][z]) g.addlimb(S.a_limb, z, gid, x.limb[j][z]);
+                } else {
+                    g.add64(S.a_sum, gid, x.sum[j]);
+                }
+            }
+            if (S.need & NEED_MINMAX) {
+                if (S.kind == VK_I32) {
+                    if (x.mn[j] != INT64_MAX) g.min32(S.a_min, gid, (int32_t)x.mn[j]);
+                    if (x.mx[j] != INT64_MIN) g.max32(S.a_max, gid, (int32_t)x.mx[j]);
+                } else {
+                    if (x.mn[j] != INT64_MAX) g.min64(S.a_min, gid, x.mn[j]);
+                    if (x.mx[j] != INT64_MIN) g.max64(S.a_max, gid, x.mx[j]);
+                }
+                if (S.kind == VK_F && x.nan[j]) g.or32(S.a_nan, gid, 1u);
+            }
+        }
+    };
+    reset();
+    const uint64_t tb = (uint64_t)t * kSortTile;
+    uint64_t before = 0;  // group starts seen in this tile before the current iteration
+    for (uint32_t k = 0; k < kSortTile; k += 64) {
+        const uint64_t i = tb + k + lane;
+        const bool live = i < L->n_sel;
+        const unsigned long long lv = __ballot(live);
+        if (!lv) break;
+        const bool start = live && is_start(klo, khi, two, i);
+        const unsigned long long sm = __ballot(start);
+        const uint64_t gid = (uint64_t)L->tile_cnt[t] + before + (uint32_t)__popcll(sm & ((2ull << lane) - 1)) - 1;
+        before += (uint32_t)__popcll(sm);
+        if (start) {
+            gst<uint64_t>(L->gk_lo, gid, gld<uint64_t>(klo, i));
+            if (two) gst<uint64_t>(L->gk_hi, gid, gld<uint64_t>(khi, i));
+        }
+        // the open run of the previous iteration ends where this one starts
+        if (have && (sm & 1ull)) {
+            if (lane == 0) flush(cgid, c);
+            have = false;
+            reset();
+        }
+        const unsigned long long heads = sm | 1ull;
+        const int seg_start = 63 - __builtin_clzll(heads & ((2ull << lane) - 1));
+        const int last_live = 63 - __builtin_clzll(lv);
+        const bool seg_last = live && (lane == last_live || ((heads >> (lane + 1)) & 1));
+        // the row's contributions: every word first
+        const uint64_t row = live ? gld<uint32_t>(krow, i) : 0;
+        uint64_t w[2], vb[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            w[j] = vb[j] = 0;
+            if (j >= nv) continue;
+            w[j] = word_at(vc[j].base, vpos(vc[j], row));
+            vb[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+        }
+        Carry x;
+        x.len = seg_scan<uint32_t>(live ? 1u : 0u, seg_start, lane, addu);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            x.cnt[j] = x.nan[j] = 0;
+            x.sum[j] = 0;
+            x.mn[j] = INT64_MAX;
+            x.mx[j] = INT64_MIN;
+#pragma unroll
+            for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = 0;
+            if (j >= nv) continue;
+            const ValSpec &S = vs[j];
+            const uint64_t raw = cut(w[j], vpos(vc[j], row), vc[j].mask);
+            const bool ok = live && cut(vb[j], bpos(vc[j], row), 1);
+            x.cnt[j] = seg_scan<uint32_t>(ok ? 1u : 0u, seg_start, lane, addu);
+            if (S.kind == VK_F) {
+                const double v = vc[j].tag == PMH_F64 ? __longlong_as_double((long long)raw)
+                                                      : (double)__uint_as_float((uint32_t)raw);
+                if (S.need & NEED_SUM) {
+                    int64_t lb[kLimbs] = {0, 0, 0, 0, 0};
+                    if (ok) {
+                        const uint64_t bb = (uint64_t)__double_as_longlong(v);
+                        const uint32_t E = (uint32_t)(bb >> 52) & 0x7FFu;
+                        uint64_t m = bb & 0xFFFFFFFFFFFFFull;
+                        if (E) m |= 1ull << 52;
+                        const int e = E ? (int)E - 1075 : -1074;
+                        const int sh0 = -kFixScale - e;
+                        const bool low_ok = m == 0 || sh0 <= 0 || (sh0 < 64 && (m & ((1ull << sh0) - 1)) == 0);
+                        const uint64_t mm = sh0 > 0 ? (sh0 < 64 ? m >> sh0 : 0) : m;
+                        const int p = sh0 > 0 ? 0 : e + kFixScale;
+                        const int msb = mm ? p + 63 - __builtin_clzll(mm) : 0;
+                        if (E != 0x7FFu && low_ok && msb < 32 * kLimbs) {
+                            const int li = p >> 5, sh = p & 31;
+                            const uint64_t lo = mm << sh;
+                            const uint64_t hi = sh ? (mm >> (64 - sh)) : 0;
+                            int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
+                            if (bb >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
+#pragma unroll
+                            for (int z = 0; z < kLimbs; ++z)
+                                lb[z] = (z == li ? c0 : 0) + (z == li + 1 ? c1 : 0) + (z == li + 2 ? c2 : 0);
+                        } else {
+                            g.addf(S.a_res, gid, v);  // outside the window (rare): straight to the residual
+                        }
+                    }
+#pragma unroll
+                    for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = seg_scan<int64_t>(lb[z], seg_start, lane, add64);
+                }
+                if (S.need & NEED_MINMAX) {
+                    const bool nan = ok && v != v;
+                    const int64_t kk = f64_order_key(v);
+                    x.mn[j] = seg_scan<int64_t>(ok && !nan ? kk : INT64_MAX, seg_start, lane, mn64);
+                    x.mx[j] = seg_scan<int64_t>(ok && !nan ? kk : INT64_MIN, seg_start, lane, mx64);
+                    x.nan[j] = seg_scan<uint32_t>(nan ? 1u : 0u, seg_start, lane, addu);
+                }
+            } else {
+                const int64_t v = S.kind == VK_I64 ? (int64_t)raw : (int64_t)(int32_t)(uint32_t)raw;
+                if (S.need & NEED_SUM) x.sum[j] = seg_scan<int64_t>(ok ? v : 0, seg_start, lane, add64);
+                if (S.need & NEED_MINMAX) {
+                    x.mn[j] = seg_scan<int64_t>(ok ? v : INT64_MAX, seg_start, lane, mn64);
+                    x.mx[j] = seg_scan<int64_t>(ok ? v : INT64_MIN, seg_start, lane, mx64);
+                }
+            }
+        }
+        // a continuing run: the carry joins this iteration's first run
+        if (have && seg_start == 0 && seg_last) {
+            x.len += c.len;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                x.cnt[j] += c.cnt[j];
+                x.nan[j] += c.nan[j];
+                x.sum[j] += c.sum[j];
+                x.mn[j] = x.mn[j] < c.mn[j] ? x.mn[j] : c.mn[j];
+                x.mx[j] = x.mx[j] > c.mx[j] ? x.mx[j] : c.mx[j];
+#pragma unroll
+                for (int z = 0; z < kLimbs; ++z) x.limb[j][z] += c.limb[j][z];
+            }
+        }
+        // closed runs go out; the last run of the iteration becomes the carry
+        if (seg_last && lane != last_live) flush(gid, x);
+        cgid = __shfl(gid, last_live);
+        c.len = __shfl(x.len, last_live);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            c.cnt[j] = __shfl(x.cnt[j], last_live);
+            c.nan[j] = __shfl(x.nan[j], last_live);
+            c.sum[j] = __shfl(x.sum[j], last_live);
+            c.mn[j] = __shfl(x.mn[j], last_live);
+            c.mx[j] = __shfl(x.mx[j], last_live);
+#pragma unroll
+            for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = __shfl(x.limb[j][z], last_live);
+        }
+        have = true;
+    }
+    if (have && lane == 0) flush(cgid, c);
+}
+
 // ---------------------------------------------------------------------------
 // 6. outputs
 // ---------------------------------------------------------------------------
@@ -701,7 +1121,6 @@ void plan_runs(SLaunch &L, uint64_t vlo, uint64_t vhi) {
 }  // namespace
 
 pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
-    (void)plan;
     Runtime &R = rt();
     const uint64_t n = P.n_rows;
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
@@ -711,6 +1130,17 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     L.q = P.q;
     L.n_out = P.n_out;
     L.ctr = (uint64_t *)slab("so:ctr", SC_N * 8);
+    L.nblk = (uint32_t)(cus * 8);
+    L.bpart = (uint64_t *)slab("so:bpart", (uint64_t)L.nblk * 5 * 8 + 64);
+    L.boff = (uint64_t *)slab("so:boff", (uint64_t)L.nblk * 8 + 64);
+    {
+        bool aligned = true;
+        for (int c = 0; c < L.q.n_cols; ++c) {
+            const int w = tag_size(L.q.cols[c].tag);
+            if (w > 1 && ((uintptr_t)L.q.cols[c].data % (uintptr_t)w) != 0) aligned = false;
+        }
+        L.fast = aligned && L.q.n_keys <= 4 && plan.opts.fast_scan != 0 && lower_fast_scan(L.q, L.fs) ? 1 : 0;
+    }
     SLaunch *dL = (SLaunch *)slab("so:launch", sizeof(SLaunch));
     hipEvent_t ev0, ev1;
     PMH_HIP(hipEventCreate(&ev0));
@@ -720,11 +1150,17 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     // ---- 1. varying lane bits and the selected-row count
     PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
     hipLaunchKernelGGL(so_reset, dim3(1), dim3(64), 0, R.stream, dL);
-    {
+    if (L.fast) {
+        KernelTimer kt("so_bits_fast");
+        hipLaunchKernelGGL(so_scan_fast<false>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+        launch_check();
+    } else {
         KernelTimer kt("so_bits");
         hipLaunchKernelGGL(so_bits, dim3(cus * 8), dim3(256), 0, R.stream, dL);
         launch_check();
     }
+    hipLaunchKernelGGL(so_bits_final, dim3(1), dim3(1024), 0, R.stream, dL);
+    launch_check();
     uint64_t c[SC_N];
     PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
     PMH_HIP(hipStreamSynchronize(R.stream));
@@ -762,7 +1198,11 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     uint64_t *slo = L.ka_lo, *shi = L.ka_hi, *dlo = L.kb_lo, *dhi = L.kb_hi;
     uint32_t *srow = L.ra, *drow = L.rb;
     if (n_sel) {
-        {
+        if (L.fast) {
+            KernelTimer kt("so_encode_fast");
+            hipLaunchKernelGGL(so_scan_fast<true>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+            launch_check();
+        } else {
             KernelTimer kt("so_encode");
             hipLaunchKernelGGL(so_encode, dim3(cus * 8), dim3(256), 0, R.stream, dL);
             launch_check();
@@ -835,7 +1275,11 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                                dim3(256), 0, R.stream, dL, ng);
             launch_check();
         }
-        {
+        if (L.fast && P.q.n_vals <= 2) {
+            KernelTimer kt("so_reduce_fast");
+            hipLaunchKernelGGL(so_reduce_fast, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo, shi, srow);
+            launch_check();
+        } else {
             KernelTimer kt("so_reduce");
             hipLaunchKernelGGL(so_reduce, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, srow);
             launch_check();

@@ -298,6 +298,23 @@ __device__ __forceinline__ void emit_group(const QueryDesc *__restrict__ q, cons
     }
 }
 
+// Host-lowered form of a conjunctive predicate for the specialised scan: a
+// row is selected iff, for every r < nr, column rcol[r] is valid and its
+// int64 image (the value itself, or its TotalOrd image when rflt[r]) lies in
+// [rlo[r], rhi[r]].
+struct FastScan {
+    int32_t enabled, nr;
+    int32_t rcol[2], rflt[2];
+    int64_t rlo[2], rhi[2];
+    uint32_t ring_off;       // byte offset of the per-wave row rings in LDS
+    int32_t rows;            // rows per lane per step (4 or 8)
+    int32_t touch;           // touch key/value rows one step ahead of their gathers
+    int32_t pad;
+};
+
+// Lowers q's predicate into fs (false: another shape).
+bool lower_fast_scan(const QueryDesc &q, FastScan &fs);
+
 // ---------------------------------------------------------------------------
 // host: a query prepared once, run by one of the group-by engines
 // ---------------------------------------------------------------------------
@@ -310,6 +327,9 @@ struct Prep {
     std::string sig;
     bool wide = false;
     uint64_t n_rows = 0;
+    float feat[PMH_MLP_IN] = {};  // the learned sizer's features, once the sketch ran
+    int route = 0;                // 1: the learned sizer sent the query to the sort engine
+    int sizer_choice = 0;         // pmh_exec_stats::sizer_choice
 };
 
 // Sample sketch of the partitioned engine (host side).
@@ -321,6 +341,10 @@ struct Sketch {
 Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
                    uint32_t dropped, const std::vector<uint32_t> &hist);
 double plain_distinct_estimate(const Sketch &k);
+double gee_estimate(const Sketch &k);
+double shlosser_estimate(const Sketch &k);
+// The learned sizer's PMH_MLP_IN features of a sketch.
+void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f);
 
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);

@@ -70,7 +70,7 @@ class PmhExecStats(ctypes.Structure):
         ("tail_rows", ctypes.c_uint64),
         ("distinct_estimate", ctypes.c_double),
         ("scan_kernel", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sizer_choice", ctypes.c_int32),
     ]
 
 
@@ -86,6 +86,7 @@ class PmhResult(ctypes.Structure):
 # Every symbol include/pmh.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 _U8P = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "pmh_last_error": (ctypes.c_char_p, []),
     "pmh_version_string": (ctypes.c_char_p, []),
@@ -114,6 +115,17 @@ SIGNATURES = {
     "pmh_string_free": (None, [ctypes.c_void_p]),
     "pmh_set_profiling": (ctypes.c_int, [ctypes.c_int]),
     "pmh_kernel_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "pmh_sketch_features": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
+                                           ctypes.c_uint32, _FP, ctypes.POINTER(PmhExecStats)]),
+    "pmh_mlp_new": (ctypes.c_int, [_FP, ctypes.POINTER(_P)]),
+    "pmh_mlp_free": (None, [_P]),
+    "pmh_mlp_params": (ctypes.c_int, [_P, _FP]),
+    "pmh_mlp_forward": (ctypes.c_int, [_P, _FP, ctypes.c_uint32, _FP]),
+    "pmh_mlp_grad": (ctypes.c_int, [_P, _FP, _FP, ctypes.c_uint32, ctypes.c_float, _FP, _FP]),
+    "pmh_mlp_set_data": (ctypes.c_int, [_P, _FP, _FP, ctypes.c_uint32]),
+    "pmh_mlp_train": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_uint32, _FP]),
+    "pmh_sizer_install": (ctypes.c_int, [_FP]),
 }
 
 _lib = None

@@ -163,6 +163,7 @@ def _stats_dict(st: _lib.PmhExecStats) -> dict:
         "tail_rows": st.tail_rows,
         "distinct_estimate": st.distinct_estimate,
         "scan_kernel": "specialised" if st.scan_kernel == 1 else "generic",
+        "sizer_choice": {0: None, 1: "hash", 2: "sort"}.get(st.sizer_choice),
     }
 
 
@@ -275,11 +276,9 @@ def _device_to_host(ptr: int, nbytes: int) -> bytes:
     return buf.raw
 
 
-def execute_filter_groupby_device(plan_dict: dict, n_rows: int, columns: Iterable[tuple]) -> DeviceResult:
-    """``columns``: ``(name, tag, data_ptr, data_len, valid_ptr, valid_len)``
-    with device pointers (``valid_ptr`` 0 for no validity)."""
-    lib = _lib.load()
-    columns = list(columns)
+def _device_column_array(columns: list):
+    """PmhColumn array for ``(name, tag, data_ptr, data_len, valid_ptr,
+    valid_len)`` device columns, and the objects that must outlive it."""
     arr = (_lib.PmhColumn * max(1, len(columns)))()
     keep = []
     for i, (name, tag, dp, dl, vp, vl) in enumerate(columns):
@@ -292,6 +291,15 @@ def execute_filter_groupby_device(plan_dict: dict, n_rows: int, columns: Iterabl
         arr[i].data_len = dl
         arr[i].valid = vp or None
         arr[i].valid_len = vl if vp else 0
+    return arr, keep
+
+
+def execute_filter_groupby_device(plan_dict: dict, n_rows: int, columns: Iterable[tuple]) -> DeviceResult:
+    """``columns``: ``(name, tag, data_ptr, data_len, valid_ptr, valid_len)``
+    with device pointers (``valid_ptr`` 0 for no validity)."""
+    lib = _lib.load()
+    columns = list(columns)
+    arr, keep = _device_column_array(columns)
     res_p = ctypes.POINTER(_lib.PmhResult)()
     check(lib.pmh_execute_filter_groupby(_plan_json(plan_dict), n_rows, arr, len(columns), PMH_OUT_DEVICE,
                                          ctypes.byref(res_p)))

@@ -154,3 +154,59 @@ def pack_bits_device(b):
     bb = torch.nn.functional.pad(b.to(torch.uint8), (0, pad)).view(-1, 8)
     weights = torch.tensor([1, 2, 4, 8, 16, 32, 64, 128], dtype=torch.uint8, device=b.device)
     return (bb * weights).sum(dim=1, dtype=torch.uint8)
+
+
+# ---------------------------------------------------------------------------
+# the learned sizer's job stream: rows and distinct keys log-uniform, key
+# Zipf exponent in [0, 1.5], key dtype i32 / i64 / u128 (an (i64, i32)
+# composite: a 98-bit lane), null fraction in [0, 0.2]
+# ---------------------------------------------------------------------------
+def sizer_job_specs(n_jobs: int, seed: int, rows_lo: float = 1e3, rows_hi: float = 1e7) -> list[dict]:
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n_jobs):
+        rows = int(np.exp(rng.uniform(np.log(rows_lo), np.log(rows_hi))))
+        distinct = max(1, int(np.exp(rng.uniform(0.0, np.log(rows)))))
+        specs.append({"rows": rows, "distinct": distinct, "s": float(rng.uniform(0.0, 1.5)),
+                      "dtype": ["I32", "I64", "U128"][int(rng.integers(0, 3))],
+                      "null_frac": float(rng.uniform(0.0, 0.2)), "seed": int(seed * 1_000_003 + i)})
+    return specs
+
+
+def _power_law_ranks(rng, n: int, d: int, s: float) -> np.ndarray:
+    """n ranks in [0, d) with P(rank k) ~ (k+1)^-s: the inverse CDF of the
+    continuous bounded power law on [1, d + 1), floored."""
+    u = rng.random(n)
+    if abs(s - 1.0) < 1e-9:
+        x = np.power(float(d + 1), u)
+    else:
+        a = 1.0 - s
+        x = np.power(1.0 + u * (np.power(float(d + 1), a) - 1.0), 1.0 / a)
+    return np.minimum(np.floor(x).astype(np.int64) - 1, d - 1)
+
+
+def sizer_job(spec: dict) -> tuple[dict, dict]:
+    rng = np.random.default_rng(spec["seed"])
+    n, d = spec["rows"], spec["distinct"]
+    r = _power_law_ranks(rng, n, d, spec["s"]).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        # odd multipliers: bijections of the 64- and 32-bit rings, so d ranks
+        # give d distinct keys
+        k64 = (r * np.uint64(0x9E3779B97F4A7C15) + np.uint64(0x632BE59BD9B4E019)).view(np.int64)
+        k32 = ((r * np.uint64(2654435761) + np.uint64(12345)) & np.uint64(0xFFFFFFFF)).astype(np.uint32).view(np.int32)
+    kv = rng.random(n) >= spec["null_frac"]
+    frame = {"v": ("F64", rng.random(n), None)}
+    if spec["dtype"] == "I32":
+        frame["k"] = ("I32", k32, kv)
+        keys = [["k", "I32"]]
+    elif spec["dtype"] == "I64":
+        frame["k"] = ("I64", k64, kv)
+        keys = [["k", "I64"]]
+    else:
+        frame["k"] = ("I64", k64, kv)
+        frame["k2"] = ("I32", k32, None)
+        keys = [["k", "I64"], ["k2", "I32"]]
+    aggs = [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "v_sum"},
+            {"kind": "Simple", "input_col": "v", "op": "Count", "output_alias": "v_count"},
+            {"kind": "Length", "op": "Len", "output_alias": "n"}]
+    return frame, {"keys": keys, "aggs": aggs}

@@ -346,11 +346,13 @@ def test_fast_scan_predicates(native, i):
                      ("Sum", "Count", "Min", "Max", "Mean")] + [{"kind": "Length", "op": "Len", "output_alias": "n"}]}
     want = orc.groupby(plan, frame)
     print(f"fast-scan predicate {i}", flush=True)
-    for fast in (1, 0):
-        p = dict(plan, options={"method": "hash", "fast_scan": fast, "lds_slots": 64})
-        got = run_gpu(native, p, frame)
-        assert native.last_stats()["scan_kernel"] == ("specialised" if fast and lowered else "generic")
-        assert_results_equal(got, want, ["k"], label=f"pred{i}/fast{fast}", zero_sign_free=("Min", "Max"))
+    for method in ("hash", "sort"):
+        for fast in (1, 0):
+            p = dict(plan, options={"method": method, "fast_scan": fast, "lds_slots": 64})
+            got = run_gpu(native, p, frame)
+            if method == "hash":
+                assert native.last_stats()["scan_kernel"] == ("specialised" if fast and lowered else "generic")
+            assert_results_equal(got, want, ["k"], label=f"pred{i}/{method}/fast{fast}", zero_sign_free=("Min", "Max"))
 
 
 @pytest.mark.parametrize("keys", [["a"], ["a", "b"], ["b", "c"]])

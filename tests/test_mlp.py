@@ -1,0 +1,150 @@
+"""The learned sizer's network: the CPU restatement (oracle/mlp.py) pinned by
+finite differences and Adam's closed form, then the device kernels
+(pmh_mlp_*) checked against it, trained, and used for routing."""
+import numpy as np
+import pytest
+
+import mlp as mo
+import oracle as orc
+from helpers import assert_results_equal, run_gpu
+
+
+def _batch(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, mo.F)) * 2 - 0.5
+    y = np.stack([rng.random(n), rng.choice([-1.0, 0.0, 1.0], n)], axis=1)
+    return x, y
+
+
+# ---------------------------------------------------------------------------
+# CPU: pin the oracle
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("lam", [0.0, 1.0, 2.5])
+def test_oracle_grad_matches_finite_differences(lam):
+    p = mo.init_params(3)
+    x, y = _batch(4, 17)
+    g = mo.grad(p, x, y, lam)
+    rng = np.random.default_rng(5)
+    # every bias and a sample of weights in each layer
+    idx = np.concatenate([rng.choice(mo.N_PARAMS, 300, replace=False), np.arange(mo.N_PARAMS - 2, mo.N_PARAMS)])
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        fd = (mo.loss(p + e, x, y, lam) - mo.loss(p - e, x, y, lam)) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_oracle_adam_first_step_is_lr_sign():
+    p = np.zeros(5)
+    g = np.array([3.0, -2.0, 1e-3, -1e-3, 0.0])
+    p1, m, v = mo.adam_step(p, g, np.zeros(5), np.zeros(5), 1, 0.01)
+    np.testing.assert_allclose(p1[:4], -0.01 * np.sign(g[:4]), rtol=1e-4)
+    assert p1[4] == 0.0
+
+
+def test_oracle_layout_matches_library():
+    from polars_metal import learned
+    assert mo.N_PARAMS == learned.MLP_PARAMS
+    assert mo.F == learned.MLP_IN
+
+
+# ---------------------------------------------------------------------------
+# GPU: the kernels against the oracle
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 5, 64, 333])
+def test_device_forward_matches_oracle(native, n):
+    from polars_metal import learned
+    p = learned.init_params(7)
+    x, _ = _batch(8, n)
+    m = learned.DeviceMLP(p)
+    got = m.forward(x.astype(np.float32))
+    want, _ = mo.forward(p.astype(np.float64), x.astype(np.float32).astype(np.float64))
+    np.testing.assert_allclose(got, want, rtol=1e-4, atol=1e-5)
+    np.testing.assert_array_equal(m.params(), p)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,lam", [(1, 1.0), (40, 1.0), (513, 0.5), (96, 0.0)])
+def test_device_gradient_matches_oracle(native, n, lam):
+    from polars_metal import learned
+    p = learned.init_params(9)
+    x, y = _batch(10, n)
+    x32, y32 = x.astype(np.float32), y.astype(np.float32)
+    m = learned.DeviceMLP(p)
+    g, loss = m.grad(x32, y32, lam)
+    gw = mo.grad(p.astype(np.float64), x32.astype(np.float64), y32.astype(np.float64), lam)
+    lw = mo.loss(p.astype(np.float64), x32.astype(np.float64), y32.astype(np.float64), lam)
+    assert abs(loss - lw) <= 1e-4 * max(1.0, abs(lw))
+    # fp32 accumulation against float64: scale the tolerance by the largest
+    # gradient component
+    scale = float(np.abs(gw).max())
+    np.testing.assert_allclose(g, gw, rtol=0, atol=2e-4 * scale)
+    # and the direction is the same
+    cos = float(g @ gw / (np.linalg.norm(g) * np.linalg.norm(gw)))
+    assert cos > 0.99999
+
+
+@pytest.mark.gpu
+def test_device_training_fits_a_known_function(native):
+    """Adam on the device fits y0 = mean(x) and y1 = [x0 > x1]; its first
+    step equals the oracle's Adam step from the same gradient."""
+    from polars_metal import learned
+    rng = np.random.default_rng(12)
+    x = rng.random((2048, mo.F)).astype(np.float32)
+    y = np.stack([x.mean(1), (x[:, 0] > x[:, 1]).astype(np.float32)], axis=1).astype(np.float32)
+    p0 = learned.init_params(13)
+    m = learned.DeviceMLP(p0)
+    g, l0 = m.grad(x[:64], y[:64])
+    # one step on one batch of 64 (the whole "data set")
+    m1 = learned.DeviceMLP(p0)
+    m1.fit(x[:64], y[:64], epochs=1, batch=64, lr=1e-3)
+    p1, _, _ = mo.adam_step(p0.astype(np.float64), g.astype(np.float64), 0.0, 0.0, 1, 1e-3)
+    np.testing.assert_allclose(m1.params(), p1, rtol=0, atol=1e-5)
+    loss = m.fit(x, y, epochs=60, batch=64, lr=3e-3)
+    assert loss < 0.05 * l0
+    pred = m.forward(x)
+    assert np.mean((pred[:, 1] > 0) == (y[:, 1] > 0.5)) > 0.95
+    assert np.median(np.abs(pred[:, 0] - y[:, 0])) < 0.02
+
+
+# ---------------------------------------------------------------------------
+# GPU: features and routing
+# ---------------------------------------------------------------------------
+@pytest.mark.gpu
+def test_sketch_features(native):
+    from polars_metal import learned, synth
+    from polars_metal.buffer import device_columns
+    frame, plan = synth.zipf(500_000, seed=3)
+    cols, bufs = device_columns(frame)
+    f, st = learned.sketch_features(plan, 500_000, cols)
+    assert f.shape == (16,) and np.isfinite(f).all()
+    assert st["sample_rows"] > 15_000 and 0 < st["sample_distinct"] <= st["sample_rows"]
+    assert abs(f[1] - 0.3) < 0.02                   # sample selectivity of (w >= 200) & (w < 500)
+    assert f[13] == 1.0 and f[12] == 0.0 and f[14] == 0.0  # one i64 key: the 65-bit lane class
+    assert abs(f[0] * 32 - np.log2(500_001)) < 1e-4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bias", [-20.0, 20.0])
+def test_installed_sizer_routes_without_changing_results(native, bias):
+    """A model whose path logit is forced negative (hash) or positive (sort):
+    the route follows it, the results stay bit-identical to the oracle."""
+    from polars_metal import learned, synth
+    p = learned.init_params(1)
+    p[-1] = bias  # b3[1]: the path logit's bias
+    frame, plan = synth.zipf(300_000, seed=9)
+    want = orc.groupby(plan, frame)
+    learned.install(p)
+    try:
+        got = run_gpu(native, dict(plan, options={"method": "auto"}), frame)
+        st = native.last_stats()
+    finally:
+        learned.install(None)
+    assert st["sizer_choice"] == ("sort" if bias > 0 else "hash")
+    assert st["method"] == ("sort" if bias > 0 else "hash")
+    assert_results_equal(got, want, ["key"], label=f"sizer{bias}")
+    # uninstalled: auto is the plain method again
+    run_gpu(native, dict(plan, options={"method": "auto"}), frame)
+    assert native.last_stats()["sizer_choice"] is None

@@ -1455,14 +1455,11 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     if (opt.partitions > 0) {
         P_parts = pow2_floor((uint64_t)opt.partitions);
     } else {
+        // an estimate beyond what the largest partition count holds still
+        // runs here: estimates of near-unique samples overshoot, and keys
+        // that miss a full partition table go to the fallback table; only
+        // its overflow sends the query to the global-table engine
         P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_tail / tail_cap)));
-        if (d_tail > (double)(1u << 15) * tail_cap * 1.5) {
-            // beyond what partitions of one LDS table can hold
-            collect_kernel_times();
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
-            return nullptr;
-        }
     }
     P_parts = std::min<uint64_t>(P_parts, 1u << 15);
     const int pbits = log2u(P_parts);

@@ -143,22 +143,29 @@ def targets(n_groups: int, t_hash_ms: float | None, t_sort_ms: float | None) -> 
     return np.array([np.log2(n_groups + 1.0) / 32.0, y1], np.float32)
 
 
+def timed(run_query, plan: dict, n: int, cols: list) -> tuple[dict, float]:
+    """Stats and wall milliseconds of one synchronous query."""
+    t = time.perf_counter()
+    st = run_query(plan, n, cols)
+    return st, (time.perf_counter() - t) * 1e3
+
+
 def collect_logs(jobs, run_query, log=print) -> tuple[np.ndarray, np.ndarray, list]:
     """Logs of the plain method: for each job ``(plan, n_rows, columns)``,
-    the sketch features, the true group count and both engines' device
-    times.  ``run_query(plan, n_rows, columns)`` runs one query and returns
-    its stats."""
+    the sketch features, the true group count and both engines' wall times
+    (each call returns only after its device work).  ``run_query(plan,
+    n_rows, columns)`` runs one query and returns its stats."""
     xs, ys, meta = [], [], []
     t0 = time.perf_counter()
     for i, (plan, n, cols) in enumerate(jobs):
         feat, _ = sketch_features(plan, n, cols)
-        sh = run_query(dict(plan, options={"method": "hash"}), n, cols)
-        ss = run_query(dict(plan, options={"method": "sort"}), n, cols)
+        sh, th = timed(run_query, dict(plan, options={"method": "hash"}), n, cols)
+        ss, ts = timed(run_query, dict(plan, options={"method": "sort"}), n, cols)
         if sh["n_groups"] != ss["n_groups"]:
             raise AssertionError(f"job {i}: hash and sort disagree on the group count")
         xs.append(feat)
-        ys.append(targets(sh["n_groups"], sh["device_ms"], ss["device_ms"]))
-        meta.append({"rows": n, "groups": sh["n_groups"], "hash_ms": sh["device_ms"], "sort_ms": ss["device_ms"]})
+        ys.append(targets(sh["n_groups"], th, ts))
+        meta.append({"rows": n, "groups": sh["n_groups"], "hash_ms": th, "sort_ms": ts})
         if log and (i + 1) % 100 == 0:
             log(f"logged {i + 1} jobs in {time.perf_counter() - t0:.1f} s")
     return np.stack(xs), np.stack(ys), meta

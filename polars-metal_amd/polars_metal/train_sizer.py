@@ -46,21 +46,41 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
     specs = synth.sizer_job_specs(args.jobs, args.seed, rows_hi=args.rows_hi)
+    # grow every device buffer once, at the largest job, so no logged call
+    # pays an allocation
+    big = max(specs, key=lambda s: s["rows"])
+    for plan, n, cols in job_stream([dict(big, dtype=d) for d in ("I32", "I64", "U128")]):
+        for m in ("hash", "sort"):
+            run_query(dict(plan, options={"method": m}), n, cols)
     x, y, meta = learned.collect_logs(job_stream(specs), run_query, log=lambda s: print(s, flush=True))
     print(f"collected {len(meta)} jobs in {time.perf_counter() - t0:.1f} s", flush=True)
     if args.logs:
         np.savez(args.logs, x=x, y=y, rows=[m["rows"] for m in meta], groups=[m["groups"] for m in meta],
                  hash_ms=[m["hash_ms"] for m in meta], sort_ms=[m["sort_ms"] for m in meta])
+    # split by job: every fifth job is held out
+    test = np.arange(len(meta)) % 5 == 0
+    train = ~test
     model = learned.DeviceMLP(learned.init_params(args.seed))
-    loss = model.fit(x, y, epochs=args.epochs, batch=64, lr=3e-3, lam=1.0, seed=args.seed)
-    pred = model.forward(x)
+    loss = model.fit(x[train], y[train], epochs=args.epochs, batch=64, lr=3e-3, lam=1.0, seed=args.seed)
     params = model.params()
-    acc = float(np.mean((pred[:, 1] > 0) == (y[:, 1] > 0.5)))
-    err = float(np.median(np.abs(pred[:, 0] - y[:, 0]) * 32))
-    print(f"trained: loss {loss:.4f}, path accuracy {acc:.3f}, median |log2 error| {err:.3f}", flush=True)
+    th = np.array([m["hash_ms"] for m in meta])
+    ts = np.array([m["sort_ms"] for m in meta])
+    report = {}
+    for name, sel in (("train", train), ("held_out", test)):
+        pred = model.forward(x[sel])
+        pick_sort = pred[:, 1] > 0
+        chosen = np.where(pick_sort, ts[sel], th[sel])
+        report[name] = {
+            "jobs": int(sel.sum()),
+            "path_accuracy": float(np.mean(pick_sort == (y[sel, 1] > 0.5))),
+            "median_log2_error": float(np.median(np.abs(pred[:, 0] - y[sel, 0]) * 32)),
+            "plain_ms": float(th[sel].sum()), "learned_ms": float(chosen.sum()),
+            "best_ms": float(np.minimum(th[sel], ts[sel]).sum()),
+            "gain": float(th[sel].sum() / chosen.sum()),
+        }
+        print(f"{name}: {report[name]}", flush=True)
     learned.save_model(params, args.out, meta={"jobs": args.jobs, "seed": args.seed, "rows_hi": args.rows_hi,
-                                                "epochs": args.epochs, "final_loss": loss, "train_path_accuracy": acc,
-                                                "train_median_log2_error": err})
+                                                "epochs": args.epochs, "final_loss": loss, "report": report})
     return 0
 
 

@@ -256,6 +256,42 @@ int pmh_mlp_train(pmh_mlp *m, uint32_t epochs, uint32_t batch, float lr, float l
  * exact whatever it predicts. */
 int pmh_sizer_install(const float *params);
 
+/* ------------------------------------------------------------------------
+ * Batched group-by.  No counterpart in the reference, which runs one query
+ * per call with its own dispatches and blocking waits (command.rs:137-254);
+ * this entry is for callers with many queries of one shape in flight.
+ * ---------------------------------------------------------------------- */
+typedef struct pmh_batch_query {
+    uint64_t n_rows;
+    const pmh_column *columns;  /* this query's columns (device memory), matched by name */
+    uint32_t n_columns;
+    const char *predicate_json; /* PredicateAst dict as JSON, or NULL: keep every row */
+} pmh_batch_query;
+
+typedef struct pmh_batch_stats {
+    double device_ms;        /* sketch to last output, HIP events */
+    uint32_t n_queries, n_tiles;
+    uint64_t n_partitions;
+    uint32_t n_rerun;        /* queries re-run alone (output capacity or fallback overflow) */
+    uint32_t tile_rows, part_slots;
+    int32_t graph_reused;    /* 1: the captured graph of an earlier batch was replayed */
+    int32_t learned;         /* 1: the installed sizer sized the partitions */
+    int32_t pad;
+} pmh_batch_stats;
+
+/* Runs n_queries filtered group-bys that share template_plan_json's keys
+ * and aggregates (its predicate is ignored; each query brings its own):
+ * one sketch launch, then one captured graph for every query.  results
+ * receives n_queries pmh_result pointers (free each with pmh_result_free).
+ * With PMH_OUT_DEVICE, a query the batch could not finish (its groups
+ * outgrew the rows sized from its sketch) is re-run alone and comes back
+ * with host columns: each column's location says where it is.
+ * Keys: one lane of at most 64 bits; at most two aggregated columns;
+ * predicates: conjunctions of column-vs-literal comparisons
+ * (PMH_E_NOTIMPL otherwise).  flags as pmh_execute_filter_groupby. */
+int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *queries, uint32_t n_queries,
+                      uint32_t flags, pmh_result **results, pmh_batch_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

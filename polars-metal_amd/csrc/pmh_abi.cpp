@@ -9,6 +9,7 @@
 
 #include "pmh_internal.h"
 #include "pmh_mlp.h"
+#include "pmh_table.h"
 
 namespace pmh {
 const char *last_error();
@@ -306,6 +307,29 @@ int pmh_mlp_train(pmh_mlp *m, uint32_t epochs, uint32_t batch, float lr, float l
 
 int pmh_sizer_install(const float *params) {
     return guarded([&] { sizer_install(params); });
+}
+
+int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *queries, uint32_t n_queries,
+                      uint32_t flags, pmh_result **results, pmh_batch_stats *stats) {
+    return guarded([&] {
+        if (!results || (n_queries && !queries)) fail(PMH_E_VALUE, "polars_metal: batch needs queries and results");
+        if (!template_plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        if (flags & ~PMH_OUT_DEVICE) fail(PMH_E_VALUE, "polars_metal: unknown flags");
+        for (uint32_t i = 0; i < n_queries; ++i) results[i] = nullptr;
+        GroupByPlan tmpl = parse_groupby_plan(template_plan_json, true);
+        tmpl.predicate.reset();
+        std::vector<BatchQueryIn> qs(n_queries);
+        for (uint32_t i = 0; i < n_queries; ++i)
+            qs[i] = BatchQueryIn{queries[i].n_rows, queries[i].columns, queries[i].n_columns, queries[i].predicate_json};
+        std::vector<pmh_result *> out;
+        try {
+            run_batch(tmpl, qs, flags, out, stats);
+        } catch (...) {
+            for (auto *r : out) if (r) pmh_result_free(r);
+            throw;
+        }
+        for (uint32_t i = 0; i < n_queries; ++i) results[i] = out[i];
+    });
 }
 
 }  // extern "C"

@@ -461,6 +461,7 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
         fail(PMH_E_KEY, "polars_metal: column not found: " + nm);
     };
     if ((int)dcols.size() > kMaxCols) fail(PMH_E_NOTIMPL, "polars_metal: too many columns in one query");
+    P.dcols = dcols;
 
     QueryDesc &q = P.q;
     memset(&q, 0, sizeof(q));
@@ -833,6 +834,17 @@ void run_sketch(GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint
     if (stats) *stats = r->stats;
     pmh_result_free(r);
     memcpy(feat, P.feat, sizeof(P.feat));
+}
+
+void lower_predicate_into(QueryDesc &q, const GroupByPlan &plan, const Prep &P) {
+    q.n_pred = 0;
+    if (!plan.predicate) return;
+    auto col_index = [&](const std::string &nm) -> int {
+        for (size_t i = 0; i < P.dcols.size(); ++i)
+            if (P.dcols[i].name == nm) return (int)i;
+        fail(PMH_E_KEY, "polars_metal: column not found: " + nm);
+    };
+    lower_pred(*plan.predicate, q, P.dcols, col_index, true);
 }
 
 }  // namespace pmh

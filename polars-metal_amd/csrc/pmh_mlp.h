@@ -23,5 +23,6 @@ float mlp_train(Mlp *M, uint32_t epochs, uint32_t batch, float lr, float lambda,
 void sizer_install(const float *params);
 bool sizer_installed();
 void sizer_predict(const float *feat, float *out);  // out[0]: log2(groups+1)/32, out[1]: logit(sort faster)
+void sizer_predict_many(const float *feat, uint32_t n, float *out);  // n rows of features, 2n outputs
 
 }  // namespace pmh

@@ -344,4 +344,17 @@ void sizer_predict(const float *feat, float *out) {
     PMH_HIP(hipStreamSynchronize(R.stream));
 }
 
+void sizer_predict_many(const float *feat, uint32_t n, float *out) {
+    if (!n) return;
+    Runtime &R = rt();
+    static DBuf buf;
+    buf.ensure((size_t)n * (F + 2));
+    PMH_HIP(hipMemcpyAsync(buf.p, feat, (size_t)n * F * 4, hipMemcpyHostToDevice, R.stream));
+    float *dy = buf.p + (size_t)n * F;
+    hipLaunchKernelGGL(mlp_forward, dim3((n + 3) / 4), dim3(256), 0, R.stream, g_sizer.dev, buf.p, n, dy);
+    launch_check();
+    PMH_HIP(hipMemcpyAsync(out, dy, (size_t)n * 2 * 4, hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipStreamSynchronize(R.stream));
+}
+
 }  // namespace pmh

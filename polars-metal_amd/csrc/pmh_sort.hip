@@ -1122,7 +1122,6 @@ void plan_runs(SLaunch &L, uint64_t vlo, uint64_t vhi) {
 
 pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     Runtime &R = rt();
-    const uint64_t n = P.n_rows;
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
     auto Lh = std::make_unique<SLaunch>();
     memset(Lh.get(), 0, sizeof(SLaunch));

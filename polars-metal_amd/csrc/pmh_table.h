@@ -327,6 +327,7 @@ struct Prep {
     std::string sig;
     bool wide = false;
     uint64_t n_rows = 0;
+    std::vector<Column> dcols;    // the query's columns, in QueryDesc.cols order
     float feat[PMH_MLP_IN] = {};  // the learned sizer's features, once the sketch ran
     int route = 0;                // 1: the learned sizer sent the query to the sort engine
     int sizer_choice = 0;         // pmh_exec_stats::sizer_choice
@@ -348,6 +349,19 @@ void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f
 
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);
+// Lowers plan's predicate over P's columns into q (replacing q's predicate).
+void lower_predicate_into(QueryDesc &q, const GroupByPlan &plan, const Prep &P);
+
+// One query of a batch (pmh_batch.hip).
+struct BatchQueryIn {
+    uint64_t n_rows;
+    const pmh_column *cols;
+    uint32_t n_cols;
+    const char *predicate;  // PredicateAst JSON, or nullptr
+};
+void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uint32_t flags,
+               std::vector<pmh_result *> &results, pmh_batch_stats *stats);
+
 // Sort engine: radix sort of the selected rows' key lanes, then runs fold.
 pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags);
 // Per-group state arrays of nslots entries carved from one slab (limb k > 0

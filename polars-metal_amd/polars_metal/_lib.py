@@ -74,6 +74,30 @@ class PmhExecStats(ctypes.Structure):
     ]
 
 
+class PmhBatchQuery(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64),
+        ("columns", ctypes.POINTER(PmhColumn)),
+        ("n_columns", ctypes.c_uint32),
+        ("predicate_json", ctypes.c_char_p),
+    ]
+
+
+class PmhBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("device_ms", ctypes.c_double),
+        ("n_queries", ctypes.c_uint32),
+        ("n_tiles", ctypes.c_uint32),
+        ("n_partitions", ctypes.c_uint64),
+        ("n_rerun", ctypes.c_uint32),
+        ("tile_rows", ctypes.c_uint32),
+        ("part_slots", ctypes.c_uint32),
+        ("graph_reused", ctypes.c_int32),
+        ("learned", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 class PmhResult(ctypes.Structure):
     _fields_ = [
         ("n_rows", ctypes.c_uint64),
@@ -126,6 +150,9 @@ SIGNATURES = {
     "pmh_mlp_train": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_uint32, _FP]),
     "pmh_sizer_install": (ctypes.c_int, [_FP]),
+    "pmh_execute_batch": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(PmhBatchQuery), ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.POINTER(PmhResult)),
+                                         ctypes.POINTER(PmhBatchStats)]),
 }
 
 _lib = None

@@ -243,6 +243,7 @@ class DeviceResult:
             self.columns.append({
                 "name": c.name.decode(), "tag": TAG_NAMES[c.tag], "data": c.data or 0,
                 "data_len": c.data_len, "valid": c.valid or 0, "valid_len": c.valid_len,
+                "location": c.location,
             })
 
     def fetch(self, name: str):
@@ -250,6 +251,10 @@ class DeviceResult:
         or None)``."""
         for c in self.columns:
             if c["name"] == name:
+                if c["location"] == PMH_HOST:  # a batch query re-run alone
+                    data = ctypes.string_at(c["data"], c["data_len"]) if c["data_len"] else b""
+                    valid = ctypes.string_at(c["valid"], c["valid_len"]) if c["valid"] else None
+                    return c["tag"], data, valid
                 data = _device_to_host(c["data"], c["data_len"])
                 valid = _device_to_host(c["valid"], c["valid_len"]) if c["valid"] else None
                 return c["tag"], data, valid

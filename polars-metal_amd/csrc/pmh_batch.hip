@@ -41,13 +41,14 @@ constexpr uint32_t kBMaxParts = 4096;  // partitions per query
 constexpr int kBSample = 2048;      // sampled rows per query for the sketch
 constexpr int kBSampleSlots = 4096; // its LDS table
 constexpr int kBProbe = 16;         // LDS table probes before a key goes to the fallback table
+constexpr int kBKeys = 4;           // key columns of a batch whose key lane is packed (lane mode 0)
 
 struct BQuery {
     uint64_t n_rows;
     const void *data[kBCols];
     const uint8_t *valid[kBCols];
     FastScan fs;         // predicate intervals (nr = 0: none)
-    uint32_t P, pbits;   // partitions (a power of two)
+    uint32_t P, pad;     // partitions
     uint64_t part_base;  // first partition in the batch arrays
     uint64_t out_base, out_cap;
     uint64_t vb_base;    // first byte of this query's validity bitmaps
@@ -83,70 +84,152 @@ struct BLaunch {
 
 constexpr uint16_t kBAllOnes = 1u << 14, kBNull = 1u << 15;
 
-__device__ __forceinline__ RoleCol bq_col(const BLaunch *__restrict__ L, const BQuery &Q, int c) {
-    RoleCol r;
-    const uintptr_t d = (uintptr_t)Q.data[c];
-    const uintptr_t v = (uintptr_t)Q.valid[c];
-    r.tag = L->q.cols[c].tag;
-    r.w = r.tag == PMH_BOOL ? 0 : tag_size(r.tag);
-    r.base = (const uint64_t *)(d & ~uintptr_t(7));
-    r.bit0 = (uint32_t)(d & 7) * 8;
-    r.bshift = r.w == 8 ? 6 : r.w == 4 ? 5 : r.w == 2 ? 4 : r.w == 1 ? 3 : 0;
-    r.mask = r.w == 8 ? ~0ull : r.w == 0 ? 1ull : (1ull << (8 * r.w)) - 1;
-    r.vbase = v ? (const uint64_t *)(v & ~uintptr_t(7)) : pmh_all_valid;
-    r.vbit0 = v ? (uint32_t)(v & 7) * 8 : 0;
-    r.vstride = v ? 1 : 0;
-    return r;
-}
-
-// One row of one query: predicate, key lane (special: 1 all-ones, 2 null).
-struct BRowKey {
-    bool sel;
-    uint64_t lane;
-    int special;
+// One column of one query: its data and validity pointers (validity null:
+// no nulls).  Widths and tags come from the template.
+struct BCol {
+    const uint8_t *d, *v;
 };
 
-__device__ __forceinline__ BRowKey bq_row(const BLaunch *__restrict__ L, const BQuery &Q, uint64_t r) {
-    const QueryDesc *__restrict__ q = &L->q;
-    BRowKey o{true, 0, 0};
-    for (int i = 0; i < Q.fs.nr; ++i) {
-        const RoleCol c = bq_col(L, Q, Q.fs.rcol[i]);
-        const uint64_t vp = vpos(c, r), bp = bpos(c, r);
-        const uint64_t w = word_at(c.base, vp), vb = word_at(c.vbase, bp);
-        uint64_t raw[1] = {cut(w, vp, c.mask)};
-        int64_t img[1];
-        pred_images<1>(c.tag, Q.fs.rflt[i] != 0, raw, img);
-        o.sel = o.sel && cut(vb, bp, 1) && img[0] >= Q.fs.rlo[i] && img[0] <= Q.fs.rhi[i];
-    }
-    if (L->lane_mode == 1) {
-        const RoleCol c = bq_col(L, Q, q->keys[0].col);
-        const uint64_t vp = vpos(c, r), bp = bpos(c, r);
-        const uint64_t w = word_at(c.base, vp), vb = word_at(c.vbase, bp);
-        if (!cut(vb, bp, 1)) o.special = 2;
-        else {
-            o.lane = w;
-            o.special = w == kEmpty ? 1 : 0;
+// The columns that one tile of a query reads, resolved once per tile into
+// statically indexed locals.  (A copy of the whole BQuery would be indexed
+// by column number at run time, which puts it in scratch memory.)
+struct TileCols {
+    uint64_t n_rows, part_base;
+    uint32_t P, tile_base;
+    int32_t nr, nk;
+    int32_t rflt[2], rtag[2];
+    int64_t rlo[2], rhi[2];
+    BCol pc[2], kc[kBKeys], vc[2];
+};
+
+__device__ __forceinline__ BCol bq_col(const BQuery *__restrict__ Qp, int c) {
+    return BCol{(const uint8_t *)Qp->data[c], Qp->valid[c]};
+}
+
+__device__ __forceinline__ void tile_cols(const BLaunch *__restrict__ L, uint32_t qi, TileCols &T) {
+    const BQuery *__restrict__ Qp = L->qs + qi;
+    T.n_rows = Qp->n_rows;
+    T.part_base = Qp->part_base;
+    T.P = Qp->P;
+    T.tile_base = Qp->tile_base;
+    T.nr = Qp->fs.nr;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (i < T.nr) {
+            const int c = Qp->fs.rcol[i];
+            T.pc[i] = bq_col(Qp, c);
+            T.rtag[i] = L->q.cols[c].tag;
+            T.rflt[i] = Qp->fs.rflt[i];
+            T.rlo[i] = Qp->fs.rlo[i];
+            T.rhi[i] = Qp->fs.rhi[i];
         }
-        return o;
-    }
-    for (int i = 0; i < q->n_keys; ++i) {
-        const RoleCol c = bq_col(L, Q, q->keys[i].col);
-        const uint64_t vp = vpos(c, r), bp = bpos(c, r);
-        const uint64_t w = word_at(c.base, vp), vb = word_at(c.vbase, bp);
-        if (cut(vb, bp, 1)) o.lane |= cut(w, vp, c.mask) << q->keys[i].data_off;
-        else o.lane |= 1ull << q->keys[i].null_off;
-    }
-    o.special = (q->allones_possible && o.lane == kEmpty) ? 1 : 0;
-    return o;
+    T.nk = L->lane_mode == 1 ? 1 : L->q.n_keys;
+#pragma unroll
+    for (int i = 0; i < kBKeys; ++i)
+        if (i < T.nk) T.kc[i] = bq_col(Qp, L->q.keys[i].col);
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (j < L->q.n_vals) T.vc[j] = bq_col(Qp, L->q.vals[j].col);
 }
 
-__device__ __forceinline__ uint32_t bq_part(const BQuery &Q, const BRowKey &k) {
-    if (k.special || Q.pbits == 0) return 0;
-    return (uint32_t)(hash_key(k.lane, 0) >> (64 - Q.pbits));
+// Raw bits of U values of a naturally aligned column (zero-extended; Bool
+// bit-packed).  The width switch is uniform and taken once for all U loads.
+template <int U>
+__device__ __forceinline__ void col_bits(int tag, const uint8_t *d, const uint64_t (&r)[U], uint64_t (&x)[U]) {
+    if (tag == PMH_BOOL) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = (gld<uint8_t>(d, r[u] >> 3) >> (r[u] & 7)) & 1u;
+        return;
+    }
+    switch (tag_size(tag)) {
+    case 8:
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = gld<uint64_t>(d, r[u]);
+        break;
+    case 4:
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = gld<uint32_t>(d, r[u]);
+        break;
+    case 2:
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = gld<uint16_t>(d, r[u]);
+        break;
+    default:
+#pragma unroll
+        for (int u = 0; u < U; ++u) x[u] = gld<uint8_t>(d, r[u]);
+        break;
+    }
 }
 
-// binary search: the query of tile t
-__device__ __forceinline__ uint32_t tile_query(const BLaunch *__restrict__ L, uint32_t t) { return L->tile_q[t]; }
+// Validity bits of U rows (a column without a bitmap: all 1, no loads).
+template <int U>
+__device__ __forceinline__ void col_valid(const uint8_t *v, const uint64_t (&r)[U], uint32_t (&vb)[U]) {
+    if (!v) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) vb[u] = 1u;
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) vb[u] = (gld<uint8_t>(v, r[u] >> 3) >> (r[u] & 7)) & 1u;
+}
+
+// Predicate of U rows (rows must be in range; sel[u] starts as the caller's
+// in-range flag).
+template <int U>
+__device__ __forceinline__ void eval_pred(const TileCols &T, const uint64_t (&r)[U], bool (&sel)[U]) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (i >= T.nr) break;
+        uint64_t raw[U];
+        uint32_t vb[U];
+        col_bits<U>(T.rtag[i], T.pc[i].d, r, raw);
+        col_valid<U>(T.pc[i].v, r, vb);
+        int64_t img[U];
+        pred_images<U>(T.rtag[i], T.rflt[i] != 0, raw, img);
+#pragma unroll
+        for (int u = 0; u < U; ++u) sel[u] = sel[u] && vb[u] && img[u] >= T.rlo[i] && img[u] <= T.rhi[i];
+    }
+}
+
+// Key lanes of U rows; special: 1 the all-ones lane, 2 the null key of a
+// one-column 64-bit key.
+template <int U>
+__device__ __forceinline__ void eval_keys(const BLaunch *__restrict__ L, const TileCols &T, const uint64_t (&r)[U],
+                                          uint64_t (&lane)[U], uint32_t (&special)[U]) {
+    if (L->lane_mode == 1) {
+        uint32_t vb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) lane[u] = gld<uint64_t>(T.kc[0].d, r[u]);
+        col_valid<U>(T.kc[0].v, r, vb);
+#pragma unroll
+        for (int u = 0; u < U; ++u) special[u] = !vb[u] ? 2u : (lane[u] == kEmpty ? 1u : 0u);
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) lane[u] = 0;
+#pragma unroll
+    for (int i = 0; i < kBKeys; ++i) {
+        if (i >= T.nk) break;
+        const KeyField &kf = L->q.keys[i];
+        uint64_t x[U];
+        uint32_t vb[U];
+        col_bits<U>(L->q.cols[kf.col].tag, T.kc[i].d, r, x);
+        col_valid<U>(T.kc[i].v, r, vb);
+        const int doff = kf.data_off, noff = kf.null_off;
+#pragma unroll
+        for (int u = 0; u < U; ++u) lane[u] |= vb[u] ? x[u] << doff : 1ull << noff;
+    }
+    const bool ao = L->q.allones_possible;
+#pragma unroll
+    for (int u = 0; u < U; ++u) special[u] = (ao && lane[u] == kEmpty) ? 1u : 0u;
+}
+
+// Partition of a key: multiply-shift of the hash's high word onto [0, P)
+// (P need not be a power of two); side keys live in partition 0.
+__device__ __forceinline__ uint32_t bq_part(uint32_t P, uint64_t lane, uint32_t special) {
+    if (special || P <= 1) return 0;
+    return (uint32_t)(((hash_key(lane, 0) >> 32) * (uint64_t)P) >> 32);
+}
 
 // ---------------------------------------------------------------------------
 // sketch: one block per query
@@ -157,23 +240,35 @@ __global__ __launch_bounds__(256) void bt_sketch(const BLaunch *__restrict__ L) 
     __shared__ uint32_t s_sel, s_d, s_f1, s_f2, s_null;
     const uint32_t qi = blockIdx.x;
     if (qi >= L->n_queries) return;
-    const BQuery Q = L->qs[qi];
+    TileCols T;
+    tile_cols(L, qi, T);
     for (uint32_t i = threadIdx.x; i < kBSampleSlots; i += blockDim.x) { keys[i] = kEmpty; cnts[i] = 0; }
     if (threadIdx.x == 0) { s_sel = s_d = s_f1 = s_f2 = s_null = 0; }
     __syncthreads();
-    const uint64_t n = Q.n_rows;
+    const uint64_t n = T.n_rows;
     const uint32_t ns = (uint32_t)min<uint64_t>(n, kBSample);
-    for (uint32_t i = threadIdx.x; i < ns; i += blockDim.x) {
-        const uint64_t r = (uint64_t)i * (n / ns) + (uint64_t)i * (n % ns) / ns;
-        const BRowKey k = bq_row(L, Q, r);
-        if (!k.sel) continue;
+    constexpr int U = kBSample / 256;
+    uint64_t r[U], lane[U];
+    bool sel[U];
+    uint32_t sp[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t i = threadIdx.x + u * 256;
+        sel[u] = i < ns;
+        r[u] = sel[u] ? (uint64_t)i * (n / ns) + (uint64_t)i * (n % ns) / ns : 0;
+    }
+    eval_pred<U>(T, r, sel);
+    eval_keys<U>(L, T, r, lane, sp);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!sel[u]) continue;
         atomicAdd(&s_sel, 1u);
-        if (k.special) { atomicAdd(&s_null, 1u); continue; }
-        uint32_t s = (uint32_t)(hash_key(k.lane, 0) >> 32) & (kBSampleSlots - 1);
+        if (sp[u]) { atomicAdd(&s_null, 1u); continue; }
+        uint32_t s = (uint32_t)(hash_key(lane[u], 0) >> 32) & (kBSampleSlots - 1);
         for (int p = 0; p < 16; ++p) {
             const unsigned long long old = atomicCAS((unsigned long long *)&keys[s], (unsigned long long)kEmpty,
-                                                     (unsigned long long)k.lane);
-            if (old == kEmpty || old == k.lane) { atomicAdd(&cnts[s], 1u); break; }
+                                                     (unsigned long long)lane[u]);
+            if (old == kEmpty || old == lane[u]) { atomicAdd(&cnts[s], 1u); break; }
             s = (s + 1) & (kBSampleSlots - 1);
         }
     }
@@ -221,62 +316,102 @@ __global__ void bt_fill(const BLaunch *__restrict__ L) {
     if (i0 < 16) t.ctrl[i0] = 0;
 }
 
+// Tiles of every query: the histogram pass counts selected rows per (query,
+// partition); the scatter pass writes them partition-contiguous as records
+// (key lane, value bits, flags).  A thread takes 16 rows of its tile (at
+// most 8192 rows, 512 threads) in two passes of 8, each pass issuing all its
+// loads before using any.
 template <bool SCATTER>
-__global__ __launch_bounds__(1024) void bt_tiles(const BLaunch *__restrict__ L) {
+__global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
     __shared__ uint32_t hist[kBMaxParts];
-    __shared__ uint64_t base[kBMaxParts];
+    __shared__ uint64_t base[SCATTER ? kBMaxParts : 1];
     __shared__ uint32_t s_sel;
+    constexpr int U = 8, kPass = 2;
     const int nv = L->q.n_vals;
+    const int vtag[2] = {L->q.cols[L->q.vals[0].col].tag, nv > 1 ? L->q.cols[L->q.vals[1].col].tag : 0};
+    const uint32_t bs = blockDim.x, tid = threadIdx.x;
+    const uint32_t tile_rows = L->tile_rows;
     for (uint32_t t = blockIdx.x; t < L->n_tiles; t += gridDim.x) {
-        const uint32_t qi = tile_query(L, t);
-        const BQuery Q = L->qs[qi];
-        const uint64_t r0 = (uint64_t)(t - Q.tile_base) * L->tile_rows;
-        const uint64_t r1 = min<uint64_t>(Q.n_rows, r0 + L->tile_rows);
-        for (uint32_t p = threadIdx.x; p < Q.P; p += blockDim.x) hist[p] = 0;
-        if (threadIdx.x == 0) s_sel = 0;
+        const uint32_t qi = L->tile_q[t];
+        TileCols T;
+        tile_cols(L, qi, T);
+        const uint64_t r0 = (uint64_t)(t - T.tile_base) * tile_rows;
+        const uint64_t r1 = min<uint64_t>(T.n_rows, r0 + tile_rows);
+        for (uint32_t p = tid; p < T.P; p += bs) hist[p] = 0;
+        if (tid == 0) s_sel = 0;
         __syncthreads();
-        // rows of this tile handled by this thread (at most 16: tile_rows <= 16384)
-        constexpr int kMaxRows = 16;
-        uint32_t part[kMaxRows], rank[kMaxRows];
+        // per row: special << 28 | partition << 16 | rank, or ~0u when the
+        // row is not selected (partition < 4096, rank < 8192)
+        uint32_t pr[kPass][U];
         uint32_t mysel = 0;
 #pragma unroll
-        for (int k = 0; k < kMaxRows; ++k) {
-            const uint64_t r = r0 + (uint64_t)k * blockDim.x + threadIdx.x;
-            part[k] = 0xFFFFFFFFu;
-            if (r >= r1) continue;
-            const BRowKey key = bq_row(L, Q, r);
-            if (!key.sel) continue;
-            ++mysel;
-            part[k] = bq_part(Q, key);
-            rank[k] = atomicAdd(&hist[part[k]], 1u);
+        for (int ps = 0; ps < kPass; ++ps) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) pr[ps][u] = ~0u;
+            if ((uint32_t)ps * U * bs >= tile_rows) continue;
+            uint64_t r[U], lane[U];
+            bool sel[U];
+            uint32_t sp[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t rr = r0 + (uint64_t)(ps * U + u) * bs + tid;
+                sel[u] = rr < r1;
+                r[u] = sel[u] ? rr : r0;
+            }
+            eval_pred<U>(T, r, sel);
+            eval_keys<U>(L, T, r, lane, sp);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!sel[u]) continue;
+                ++mysel;
+                const uint32_t p = bq_part(T.P, lane[u], sp[u]);
+                pr[ps][u] = (sp[u] << 28) | (p << 16) | atomicAdd(&hist[p], 1u);
+            }
         }
         if (mysel) atomicAdd(&s_sel, mysel);
         __syncthreads();
         if (!SCATTER) {
-            for (uint32_t p = threadIdx.x; p < Q.P; p += blockDim.x)
-                if (hist[p]) atomicAdd(&L->part_cnt[Q.part_base + p], hist[p]);
-            if (threadIdx.x == 0 && s_sel) atomicAdd(&L->q_sel[qi], s_sel);
+            for (uint32_t p = tid; p < T.P; p += bs)
+                if (hist[p]) atomicAdd(&L->part_cnt[T.part_base + p], hist[p]);
+            if (tid == 0 && s_sel) atomicAdd(&L->q_sel[qi], s_sel);
         } else {
-            for (uint32_t p = threadIdx.x; p < Q.P; p += blockDim.x)
+            for (uint32_t p = tid; p < T.P; p += bs)
                 if (hist[p])
-                    base[p] = atomicAdd((unsigned long long *)&L->part_cur[Q.part_base + p], (unsigned long long)hist[p]);
+                    base[p] = atomicAdd((unsigned long long *)&L->part_cur[T.part_base + p], (unsigned long long)hist[p]);
             __syncthreads();
 #pragma unroll
-            for (int k = 0; k < kMaxRows; ++k) {
-                if (part[k] == 0xFFFFFFFFu) continue;
-                const uint64_t r = r0 + (uint64_t)k * blockDim.x + threadIdx.x;
-                const BRowKey key = bq_row(L, Q, r);
-                const uint64_t pos = base[part[k]] + rank[k];
-                uint16_t fl = key.special == 1 ? kBAllOnes : (key.special == 2 ? kBNull : 0);
-                for (int j = 0; j < nv; ++j) {
-                    const RoleCol c = bq_col(L, Q, L->q.vals[j].col);
-                    const uint64_t vp = vpos(c, r), bp = bpos(c, r);
-                    const uint64_t w = word_at(c.base, vp), vb = word_at(c.vbase, bp);
-                    gst<uint64_t>(L->rv[j], pos, cut(w, vp, c.mask));
-                    fl |= (uint16_t)(cut(vb, bp, 1) << j);
+            for (int ps = 0; ps < kPass; ++ps) {
+                if ((uint32_t)ps * U * bs >= tile_rows) continue;
+                uint64_t r[U], lane[U], vw[2][U];
+                uint32_t sp[U], vb[2][U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t rr = r0 + (uint64_t)(ps * U + u) * bs + tid;
+                    r[u] = pr[ps][u] != ~0u ? rr : r0;
                 }
-                gst<uint64_t>(L->rk, pos, key.lane);
-                gst<uint16_t>(L->rf, pos, fl);
+                eval_keys<U>(L, T, r, lane, sp);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (j >= nv) break;
+                    col_bits<U>(vtag[j], T.vc[j].d, r, vw[j]);
+                    col_valid<U>(T.vc[j].v, r, vb[j]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t x = pr[ps][u];
+                    if (x == ~0u) continue;
+                    const uint64_t pos = base[(x >> 16) & 0xFFFu] + (x & 0xFFFFu);
+                    const uint32_t special = x >> 28;
+                    uint16_t fl = special == 1 ? kBAllOnes : (special == 2 ? kBNull : 0);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        if (j >= nv) break;
+                        gst<uint64_t>(L->rv[j], pos, vw[j][u]);
+                        fl |= (uint16_t)(vb[j][u] << j);
+                    }
+                    gst<uint64_t>(L->rk, pos, lane[u]);
+                    gst<uint16_t>(L->rf, pos, fl);
+                }
             }
         }
         __syncthreads();
@@ -350,32 +485,41 @@ __global__ void bt_scan_add(const BLaunch *__restrict__ L) {
     }
 }
 
-// Records of batch partition i, as the aggregation code reads rows.
+// One record of a batch partition, with its value bits already loaded.
 struct BRec {
     const BLaunch *__restrict__ L;
-    uint64_t i;
     uint16_t fl;
+    uint64_t v0, v1;
     __device__ __forceinline__ bool valid(int c) const { return (fl >> L->col_val[c]) & 1; }
-    __device__ __forceinline__ uint64_t bits(int c) const { return gld<uint64_t>(L->rv[L->col_val[c]], i); }
+    __device__ __forceinline__ uint64_t bits(int c) const { return L->col_val[c] ? v1 : v0; }
     __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(L->q.cols[c].tag, bits(c)); }
     __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
 };
 
+// (query, partition) work items: fold the partition's records into an LDS
+// table, then emit its groups into the query's output rows.  The table has
+// cl live slots, the smallest power of two holding twice the partition's
+// records (at most part_slots), then the all-ones and null side slots.
 __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, const uint32_t *lds_off) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_count, s_base;
     const QueryDesc *__restrict__ q = &L->q;
     const uint32_t CL = L->part_slots;
-    const uint32_t nls = CL + 2;
+    const uint32_t nls_max = CL + 2;
     uint64_t *lk = (uint64_t *)smem;
-    uint8_t *state = (uint8_t *)(lk + nls);
-    LSink ls{state, lds_off, nls};
+    uint8_t *state = (uint8_t *)(lk + nls_max);
+    LSink ls{state, lds_off, nls_max};
     GSink gs{L->t.arr, L->fb_slots};
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    const int nv = q->n_vals;
+    constexpr int U = 4;
     for (uint64_t pi = blockIdx.x; pi < L->n_parts; pi += gridDim.x) {
         const uint64_t b = L->part_off[pi], e = L->part_off[pi + 1];
         if (b == e) continue;  // uniform: the whole block skips
         const uint32_t qi = L->part_q[pi];
+        uint32_t cl = 64;
+        while (cl < CL && (uint64_t)cl < 2 * (e - b)) cl *= 2;
+        const uint32_t nls = cl + 2;
         for (uint32_t i = tid; i < nls; i += bs) lk[i] = kEmpty;
         for (int a = 0; a < q->layout.n_arrs; ++a) {
             const int op = q->layout.init[a];
@@ -389,41 +533,58 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
         }
         if (tid == 0) s_count = 0;
         __syncthreads();
-        for (uint64_t i = b + tid; i < e; i += bs) {
-            const uint16_t fl = gld<uint16_t>(L->rf, i);
-            const BRec row{L, i, fl};
-            const uint64_t lane = gld<uint64_t>(L->rk, i);
-            const int special = (fl & kBAllOnes) ? 1 : ((fl & kBNull) ? 2 : 0);
-            int32_t slot = -1;
-            uint64_t h = 0;
-            if (special) {
-                slot = (int32_t)(CL + special - 1);
-            } else {
-                h = hash_key(lane, 0);
-                uint32_t s = (uint32_t)(h >> 20) & (CL - 1);
-                for (int pr = 0; pr < kBProbe; ++pr) {
-                    const uint64_t cur = __hip_atomic_load(&lk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    if (cur == lane) { slot = (int32_t)s; break; }
-                    if (cur == kEmpty) {
-                        const unsigned long long old = atomicCAS((unsigned long long *)&lk[s],
-                                                                 (unsigned long long)kEmpty, (unsigned long long)lane);
-                        if (old == kEmpty || old == lane) { slot = (int32_t)s; break; }
-                    }
-                    s = (s + 1) & (CL - 1);
-                }
+        for (uint64_t i0 = b + tid; i0 < e; i0 += (uint64_t)U * bs) {
+            uint64_t ln[U], v0[U], v1[U];
+            uint16_t fl[U];
+            bool in[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + (uint64_t)u * bs;
+                in[u] = i < e;
+                const uint64_t ii = in[u] ? i : b;
+                fl[u] = gld<uint16_t>(L->rf, ii);
+                ln[u] = gld<uint64_t>(L->rk, ii);
+                v0[u] = nv > 0 ? gld<uint64_t>(L->rv[0], ii) : 0;
+                v1[u] = nv > 1 ? gld<uint64_t>(L->rv[1], ii) : 0;
             }
-            if (slot >= 0) {
-                add_row(ls, q, (uint32_t)slot, row);
-            } else {
-                const Key k{lane, (uint64_t)qi, 0};
-                if (!L->q_fb[qi]) L->q_fb[qi] = 1;
-                const int64_t g = g_find128(L->t, k, h ^ ((uint64_t)qi * 0x9E3779B97F4A7C15ull));
-                if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!in[u]) continue;
+                const BRec row{L, fl[u], v0[u], v1[u]};
+                const uint64_t lane = ln[u];
+                const int special = (fl[u] & kBAllOnes) ? 1 : ((fl[u] & kBNull) ? 2 : 0);
+                int32_t slot = -1;
+                uint64_t h = 0;
+                if (special) {
+                    slot = (int32_t)(cl + special - 1);
+                } else {
+                    h = hash_key(lane, 0);
+                    uint32_t s = (uint32_t)(h >> 20) & (cl - 1);
+                    for (int pr = 0; pr < kBProbe; ++pr) {
+                        const uint64_t cur = __hip_atomic_load(&lk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if (cur == lane) { slot = (int32_t)s; break; }
+                        if (cur == kEmpty) {
+                            const unsigned long long old = atomicCAS((unsigned long long *)&lk[s],
+                                                                     (unsigned long long)kEmpty, (unsigned long long)lane);
+                            if (old == kEmpty || old == lane) { slot = (int32_t)s; break; }
+                        }
+                        s = (s + 1) & (cl - 1);
+                    }
+                }
+                if (slot >= 0) {
+                    add_row(ls, q, (uint32_t)slot, row);
+                } else {
+                    const Key k{lane, (uint64_t)qi, 0};
+                    if (!L->q_fb[qi]) L->q_fb[qi] = 1;
+                    const int64_t g = g_find128(L->t, k, h ^ ((uint64_t)qi * 0x9E3779B97F4A7C15ull));
+                    if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+                }
             }
         }
         __syncthreads();
         // emit this partition's groups into the query's output rows
-        const BQuery &Q = L->qs[qi];
+        const BQuery *__restrict__ Qp = L->qs + qi;
+        const uint64_t out_cap = Qp->out_cap, out_base = Qp->out_base;
         const int a_len = q->layout.a_len;
         uint32_t my = 0;
         for (uint32_t i = tid; i < nls; i += bs) my += *ls.p<uint32_t>(a_len, i) != 0;
@@ -435,9 +596,9 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
         LReader rd{state, lds_off};
         for (uint32_t i = tid; i < nls; i += bs) {
             if (*ls.p<uint32_t>(a_len, i) == 0) continue;
-            if (o < Q.out_cap) {
-                const int special = i >= CL ? (int)(i - CL) + 1 : 0;
-                emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], 0, special, Q.out_base + o);
+            if (o < out_cap) {
+                const int special = i >= cl ? (int)(i - cl) + 1 : 0;
+                emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], 0, special, out_base + o);
             }
             ++o;
         }
@@ -494,11 +655,6 @@ uint64_t pow2_ceil(uint64_t x) {
     while (p < x) p *= 2;
     return p;
 }
-int log2u(uint64_t x) {
-    int b = 0;
-    while ((1ull << b) < x) ++b;
-    return b;
-}
 
 // The captured chain and the buffers it was captured against.
 struct BatchGraph {
@@ -521,9 +677,10 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     Prep P;
     prepare_query(P, tmpl, qs[0].n_rows, qs[0].cols, qs[0].n_cols);
     const QueryDesc &q0 = P.q;
-    if (q0.n_cols > kBCols || q0.n_vals > 2 || !(q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64))
-        fail(PMH_E_NOTIMPL, "polars_metal: batch needs one key lane of at most 64 bits, at most two aggregated "
-                            "columns and at most 8 referenced columns");
+    if (q0.n_cols > kBCols || q0.n_vals > 2 || q0.n_keys > kBKeys ||
+        !(q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64))
+        fail(PMH_E_NOTIMPL, "polars_metal: batch needs one key lane of at most 64 bits from at most 4 key columns, "
+                            "at most two aggregated columns and at most 8 referenced columns");
     auto Lh = std::make_unique<BLaunch>();
     memset(Lh.get(), 0, sizeof(BLaunch));
     BLaunch &L = *Lh;
@@ -533,8 +690,8 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     L.lane_mode = q0.key_mode == KM_RAW64 ? 1 : 0;
     for (int j = 0; j < q0.n_vals; ++j) L.col_val[q0.vals[j].col] = j;
     const Options &opt = tmpl.opts;
-    L.tile_rows = (uint32_t)std::min<int64_t>(16384, std::max<int64_t>(1024, opt.tile > 0 ? opt.tile : 8192));
-    L.tile_rows = (L.tile_rows + 1023) / 1024 * 1024;
+    L.tile_rows = (uint32_t)std::min<int64_t>(8192, std::max<int64_t>(1024, opt.tile > 0 ? opt.tile : 8192));
+    L.tile_rows = pow2_ceil(L.tile_rows);  // 16 rows per thread of a 64..512-thread block
     // LDS table of a (query, partition) work item: the largest power of two
     // up to 4096 that fits (or the option's size); partitions are sized for
     // half of it
@@ -547,7 +704,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         while (c * 2 <= (uint64_t)opt.tail_slots && c * 2 <= CL) c *= 2;
         CL = c;
     }
-    const uint64_t part_keys = CL / 2;
+    const double part_keys = CL * 0.5;
     // ---- per query: columns by name, predicate intervals (cached by text)
     std::vector<BQuery> bq(nq);
     std::map<std::string, FastScan> pred_cache;
@@ -638,8 +795,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
                                ? (double)sk[i * 5 + 0] / (double)std::min<uint64_t>(Q.n_rows, kBSample)
                                : 0.0;
         const uint64_t n_sel_est = (uint64_t)std::ceil(sel * Q.n_rows) + 64;
-        Q.P = (uint32_t)std::min<uint64_t>(kBMaxParts, pow2_ceil((uint64_t)std::max(1.0, std::ceil(est[i] / part_keys))));
-        Q.pbits = (uint32_t)log2u(Q.P);
+        Q.P = (uint32_t)std::min<uint64_t>(kBMaxParts, (uint64_t)std::max(1.0, std::ceil(est[i] / part_keys)));
         Q.part_base = n_parts;
         n_parts += Q.P;
         Q.n_tiles = (uint32_t)ceil_div(Q.n_rows, L.tile_rows);
@@ -727,10 +883,13 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     PMH_HIP(hipFuncSetAttribute((const void *)bt_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_agg));
     // ---- the chain: captured once per buffer layout, replayed after
     const uint32_t grid = (uint32_t)cus * 4;
+    const uint32_t tile_block = L.tile_rows / 16;
+    // a table of at most 512 slots leaves room for several blocks per CU
+    const uint32_t agg_block = CL >= 1024 ? 1024 : 512;
     char keybuf[512];
-    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u", (void *)dL, (void *)L.rk, (void *)L.part_cnt,
-             (void *)L.part_off, (void *)L.t.klo, (void *)L.out[0].data, (void *)d_lds_off, lds_agg, grid,
-             (unsigned)P.n_out);
+    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u", (void *)dL, (void *)L.rk,
+             (void *)L.part_cnt, (void *)L.part_off, (void *)L.t.klo, (void *)L.out[0].data, (void *)d_lds_off, lds_agg,
+             grid, (unsigned)P.n_out, tile_block, agg_block);
     bool reused = g_graph.exec && g_graph.key == keybuf;
     if (!reused) {
         if (g_graph.exec) {
@@ -740,12 +899,12 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipGraph_t graph;
         PMH_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(bt_fill, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_tiles<false>, dim3(grid), dim3(1024), 0, R.stream, dL);
+        hipLaunchKernelGGL(bt_tiles<false>, dim3(grid), dim3(tile_block), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_local, dim3(grid), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_blocks, dim3(1), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_add, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(1024), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_agg, dim3(grid), dim3(1024), lds_agg, R.stream, dL, (const uint32_t *)d_lds_off);
+        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), 0, R.stream, dL);
+        hipLaunchKernelGGL(bt_agg, dim3(grid), dim3(agg_block), lds_agg, R.stream, dL, (const uint32_t *)d_lds_off);
         hipLaunchKernelGGL(bt_fb_emit, dim3(grid), dim3(256), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_pack, dim3(grid), dim3(256), 0, R.stream, dL);
         PMH_HIP(hipStreamEndCapture(R.stream, &graph));

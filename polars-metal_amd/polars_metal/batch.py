@@ -22,9 +22,9 @@ from typing import Iterable, Sequence
 from . import _lib
 from ._native import DeviceResult, _device_column_array, _plan_json, execute_filter_groupby_device
 
-CANDIDATES = [{"tile": 4096, "tail_slots": 1024}, {"tile": 8192, "tail_slots": 1024},
-              {"tile": 8192, "tail_slots": 2048}, {"tile": 16384, "tail_slots": 2048},
-              {"tile": 16384, "tail_slots": 4096}]
+CANDIDATES = [{"tile": 4096, "tail_slots": 512}, {"tile": 8192, "tail_slots": 512},
+              {"tile": 4096, "tail_slots": 1024}, {"tile": 8192, "tail_slots": 1024},
+              {"tile": 8192, "tail_slots": 256}]
 _tuned: dict = {}
 
 

@@ -8,8 +8,17 @@ already resident in HBM, outputs left in HBM):
 
 Prints one JSON line with value = input rows per second, the roofline of the
 dominant kernel (HBM-bound; algorithmic bytes = 20.125 B per input row:
-8 key + 8 v + 4 w + 1/8 validity) and the CPU baseline (the C oracle, one
-core, on a bounded sample of the same distribution).
+8 key + 8 v + 4 w + 1/8 validity), the CPU baseline (the C oracle, one
+core, on a bounded sample of the same distribution) and the learned sizer's
+live speedup over the plain method on a bounded stream of fresh jobs.
+
+Other workloads (one JSON line each, not the default):
+  --workload highcard   200M rows, composite key (a i32, b i64) over ~49M
+                        groups, sum/count of v i64 with 10% nulls: the hash and
+                        the u128 radix-sort engines, checked to agree.
+  --workload batch      4096 filtered group-by queries (rows log-uniform in
+                        [1e3, 1e6]) as one batch in a captured graph, against
+                        the same queries run one call at a time.
 """
 from __future__ import annotations
 
@@ -60,20 +69,210 @@ def traffic_from_profiles(kernel: str) -> float | None:
         return None
 
 
+def log(msg: str) -> None:
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def sizer_ab(n_jobs: int, seed: int = 777) -> dict:
+    """Live A/B of the learned sizer: each fresh job (the training stream's
+    distribution, another seed) runs under method "auto" without and with
+    the shipped model; both are warmed once per job, then timed once each,
+    alternating which goes first.  Results are identical by construction
+    (the engines' fallbacks keep every output exact); group counts are
+    checked."""
+    from polars_metal import _native, learned, synth
+    from polars_metal.buffer import device_columns
+    specs = synth.sizer_job_specs(n_jobs, seed, rows_hi=1e7)
+    meta = learned.install_shipped()
+    learned.install(None)
+    t_plain = t_learned = 0.0
+    n_rows = 0
+    picks = {"hash": 0, "sort": 0}
+    for i, spec in enumerate(specs):
+        frame, plan = synth.sizer_job(spec)
+        cols, bufs = device_columns(frame)
+        del frame
+        n = spec["rows"]
+        n_rows += n
+        p = dict(plan, options={"method": "auto"})
+        times, groups = {}, {}
+        for mode in (("plain", "learned") if i % 2 == 0 else ("learned", "plain")):
+            if mode == "learned":
+                learned.install_shipped()
+            for rep in range(2):
+                t0 = time.perf_counter()
+                r = _native.execute_filter_groupby_device(p, n, cols)
+                dt = time.perf_counter() - t0
+                groups[mode] = r.stats["n_groups"]
+                if mode == "learned":
+                    picks[r.stats["method"]] = picks.get(r.stats["method"], 0) + (rep == 1)
+                r.free()
+            times[mode] = dt
+            learned.install(None)
+        if groups["plain"] != groups["learned"]:
+            raise AssertionError(f"sizer A/B job {i}: group counts differ")
+        t_plain += times["plain"]
+        t_learned += times["learned"]
+        del bufs
+    return {"speedup": t_plain / t_learned, "jobs": n_jobs, "rows": n_rows, "plain_ms": t_plain * 1e3,
+            "learned_ms": t_learned * 1e3, "learned_picks": picks,
+            "sample": f"{n_jobs} fresh jobs (seed {seed}): rows log-uniform [1e3, 1e7], distinct log-uniform "
+                      f"[1, rows], key power law s~U[0,1.5], i32/i64/u128 keys, nulls U[0,0.2]; wall time per "
+                      f"synchronous call after one warm call, method auto without vs with the shipped model",
+            "model_held_out_gain": meta.get("report", {}).get("held_out", {}).get("gain")}
+
+
+def group_checksum(res, names) -> tuple:
+    """Order-independent checksum of a device result's groups: the wrapping
+    sum over groups of a mix of the named columns' bits and validity."""
+    import numpy as np
+    ng = res.n_rows
+    acc = np.zeros(ng, np.uint64)
+    with np.errstate(over="ignore"):
+        for k, name in enumerate(names):
+            tag, data, valid = res.fetch(name)
+            w = {"I64": 8, "F64": 8, "U64": 8, "I32": 4, "U32": 4}[tag]
+            x = np.frombuffer(data, {8: np.uint64, 4: np.uint32}[w], ng).astype(np.uint64)
+            if valid is not None:
+                ok = np.unpackbits(np.frombuffer(valid, np.uint8), bitorder="little")[:ng].astype(np.uint64)
+                x = x * ok + (ok ^ np.uint64(1)) * np.uint64(0x5BD1E995)
+            acc = (acc ^ x) * np.uint64(0x9E3779B97F4A7C15) + np.uint64(k + 1)
+            acc ^= acc >> np.uint64(29)
+    return ng, int(acc.sum(dtype=np.uint64))
+
+
+def bench_highcard(args) -> dict:
+    """Both engines on the 200M-row composite-key workload."""
+    from polars_metal import _native, synth
+    from polars_metal.buffer import device_columns
+    n = 200_000_000
+    log(f"generating {n} rows (numpy, seed 3; 50M-pair pool) and uploading")
+    frame, plan = synth.highcard(n, seed=3)
+    cols, bufs = device_columns(frame)
+    del frame
+    names = [k[0] for k in plan["keys"]] + [a["output_alias"] for a in plan["aggs"]]
+    engines = {}
+    for method in ("hash", "sort"):
+        p = dict(plan, options={"method": method})
+        for _ in range(args.warmup):
+            _native.execute_filter_groupby_device(p, n, cols).free()
+        _native.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = _native.execute_filter_groupby_device(p, n, cols)
+            st = r.stats
+            r.free()
+        dt = (time.perf_counter() - t0) / args.steps
+        kt = _native.kernel_times()
+        _native.set_profiling(False)
+        kernel_ms = sum(v[1] for v in kt.values()) / max(1, args.steps)
+        r = _native.execute_filter_groupby_device(p, n, cols)
+        chk = group_checksum(r, names)
+        r.free()
+        engines[method] = {"ms_per_query": dt * 1e3, "kernel_ms": kernel_ms, "groups": chk[0],
+                           "checksum": chk[1], "method_run": st["method"]}
+        log(f"{method}: {engines[method]}")
+    agree = engines["hash"]["groups"] == engines["sort"]["groups"] and \
+        engines["hash"]["checksum"] == engines["sort"]["checksum"]
+    best = min(engines, key=lambda m: engines[m]["ms_per_query"])
+    alg = (4 + 8 + 8 + 0.125) * n + engines[best]["groups"] * (4 + 8 + 8 + 4)
+    achieved = alg / (engines[best]["kernel_ms"] * 1e-3) / 1e9
+    return {
+        "metric": "high-cardinality group-by rows/s (hash and u128 radix sort, bit-exact agreement)",
+        "value": n / (engines[best]["ms_per_query"] * 1e-3), "unit": "rows/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": engines[best]["ms_per_query"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "i64",
+        "data": "synthetic (numpy default_rng seed 3, uploaded before timing)",
+        "config": {"workload": "highcard-200M: group_by([a i32 U[0,2^20), b i64 U[0,2^40)]) from a 50M-pair pool "
+                               "(~49M groups).agg(sum(v), count(v)); v i64 U[-1e9,1e9] 10% nulls",
+                   "rows": n, "engines": engines, "agree": agree, "faster": best},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": f"all kernels of the {best} engine",
+                     "kernel_avg_ms": engines[best]["kernel_ms"], "algorithmic_bytes_per_launch": alg},
+    }
+
+
+def bench_batch(args) -> dict:
+    """4096 queries batched into one captured graph vs one call each."""
+    from polars_metal import _native, batch, synth
+    from polars_metal.buffer import device_columns
+    specs = synth.batch_specs(4096)
+    log("generating and uploading 4096 queries (numpy, seed 5)")
+    queries, bufs, plan = [], [], None
+    for s_ in specs:
+        f, p = synth.batch_query(s_)
+        cols, b = device_columns(f)
+        bufs.append(b)
+        queries.append((s_["rows"], cols, p.get("predicate")))
+        plan = {"keys": p["keys"], "aggs": p["aggs"]}
+    rows = sum(s_["rows"] for s_ in specs)
+    batch.sequential(plan, queries[:64])
+    log("sequential pass over all queries")
+    t0 = time.perf_counter()
+    seq = batch.sequential(plan, queries)
+    t_seq = time.perf_counter() - t0
+    opts = batch.autotune(plan, queries)
+    log(f"autotuned launch shape {opts}")
+    for _ in range(args.warmup):
+        res, st = batch.execute_batch(plan, queries, opts)
+        for r in res:
+            r.free()
+    _native.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res, st = batch.execute_batch(plan, queries, opts)
+        groups = [r.n_rows for r in res]
+        for r in res:
+            r.free()
+    dt = (time.perf_counter() - t0) / args.steps
+    kt = _native.kernel_times()
+    _native.set_profiling(False)
+    if groups != [q["n_groups"] for q in seq]:
+        raise AssertionError("batch and sequential group counts differ")
+    graph_ms = kt.get("bt_graph", (1, float("nan")))[1] / max(1, kt.get("bt_graph", (1, 0))[0])
+    n_groups = sum(groups)
+    alg = 16.0 * rows + 44.0 * n_groups  # key + v in; key, sum, count, min, max, mean out
+    achieved = alg / (graph_ms * 1e-3) / 1e9
+    return {
+        "metric": "batched filtered group-by: queries/s of 4096 queries in one captured graph vs sequential calls",
+        "value": 4096 / dt, "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+i64", "data": "synthetic (numpy default_rng, seeds 5*100003+i, uploaded before timing)",
+        "config": {"workload": "batch-4096: rows log-uniform [1e3,1e6] per query, key i64 U{1..rows/10}, v f64 "
+                               "U[0,1), 20% of queries filter v>0.5; sum/count/min/max/mean of v",
+                   "rows": rows, "groups": n_groups, "sequential_ms": t_seq * 1e3,
+                   "speedup_vs_sequential": t_seq / dt, "rows_per_s": rows / dt, "launch_shape": opts,
+                   "device_ms": st["device_ms"], "n_rerun": st["n_rerun"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "bt_graph (the captured chain)",
+                     "kernel_avg_ms": graph_ms, "algorithmic_bytes_per_launch": alg},
+        "kernel_ms": {k: v[1] / v[0] for k, v in kt.items() if v[0]},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("zipf100m", "highcard", "batch"), default="zipf100m")
     ap.add_argument("--rows", type=int, default=100_000_000)
     ap.add_argument("--cpu-rows", type=int, default=20_000_000)
+    ap.add_argument("--sizer-jobs", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sizer", action="store_true")
     ap.add_argument("--options", type=str, default="", help="JSON plan options (launch shape overrides)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the multi-GPU sharding is a design only "
               "(see the sharding section of the design notes)", file=sys.stderr)
         return 2
+    if args.workload == "highcard":
+        print(json.dumps(bench_highcard(args)), flush=True)
+        return 0
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return 0
 
     from polars_metal import _native, synth
     from polars_metal.buffer import device_columns
@@ -136,8 +335,12 @@ def main() -> int:
                      "algorithmic_bytes_per_launch": alg_bytes},
         "kernel_ms": {k: v[1] / v[0] for k, v in kt.items() if v[0]},
     }
+    del bufs, cols
+    if not args.no_sizer:
+        log(f"learned sizer A/B on {args.sizer_jobs} fresh jobs")
+        out["learned_vs_plain"] = sizer_ab(args.sizer_jobs)
     if not args.no_cpu_baseline:
-        print("bench: CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
+        log("CPU baseline (oracle, 1 core)")
         out["cpu_baseline"] = cpu_baseline(args.cpu_rows)
     print(json.dumps(out), flush=True)
     return 0

@@ -816,6 +816,18 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
             r->stats.sizer_choice = 2;
             return r;
         }
+        if (P.overflowed) {
+            // sized for too few keys: once more with four times the
+            // estimate (more partitions and a larger fallback table share)
+            // before the global-table engine
+            P.d_retry = std::min(4.0 * std::max(P.d_used, 1024.0), (double)n_rows + 1.0);
+            P.overflowed = false;
+            r = run_hash_partitioned(P, plan, flags);
+            if (r) {
+                r->stats.n_passes = 2;
+                return r;
+            }
+        }
     }
     if (m == "sort") return run_sort(P, plan, flags);
     return run_hash_global(P, plan, flags);

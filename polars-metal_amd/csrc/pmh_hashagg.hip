@@ -1077,8 +1077,11 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
     k.sample_rows = s_sel;
     k.sample_size = n_sample;
     k.n_rows = n;
-    k.distinct = d;
-    k.f1 = f1;
+    // A dropped row's key found no slot within the probe cap, so none of
+    // that key's rows were counted.  Keys arriving late meet a fuller table,
+    // so dropped rows are mostly rare keys: each counts as a singleton.
+    k.distinct = d + dropped;
+    k.f1 = f1 + dropped;
     k.f2 = f2;
     k.dropped = dropped;
     k.n_sel = n_sample ? (double)n * s_sel / n_sample : 0.0;
@@ -1086,10 +1089,11 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
     const double qf = k.n_sel > 0 ? std::min(1.0, s_sel / k.n_sel) : 1.0;
     double F2 = 0, num = 0, den = 0;
     for (int c = 1; c < 1024; ++c) {
-        if (!hist[c]) continue;
-        F2 += (double)c * c * hist[c];
-        num += std::pow(1.0 - qf, c) * hist[c];
-        den += c * qf * std::pow(1.0 - qf, c - 1) * hist[c];
+        const double h = (double)hist[c] + (c == 1 ? (double)dropped : 0.0);
+        if (h == 0) continue;
+        F2 += (double)c * c * h;
+        num += std::pow(1.0 - qf, c) * h;
+        den += c * qf * std::pow(1.0 - qf, c - 1) * h;
     }
     k.F2 = F2;
     k.shl_num = num;
@@ -1100,7 +1104,9 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
 // Distinct keys among the selected rows: the geometric mean of GEE
 // (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
 // estimator (Haas et al., 1995; it overshoots), floored at the sample's own
-// distinct count.  A full fingerprint table extrapolates linearly.
+// distinct count.  A fingerprint table that dropped most of the sample (far
+// more distinct keys than slots: a near-unique sample) extrapolates
+// linearly.
 double gee_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
@@ -1113,7 +1119,7 @@ double shlosser_estimate(const Sketch &k) {
 double plain_distinct_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct;
-    if (k.dropped) return std::max(d, k.n_sel * (d + k.dropped) / s);
+    if (2.0 * k.dropped > s) return std::max(d, k.n_sel * d / s);
     if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
     const double gee = gee_estimate(k), shl = shlosser_estimate(k);
     return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));
@@ -1136,7 +1142,7 @@ void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f
     f[8] = lg(plain_distinct_estimate(k));
     f[9] = (float)(k.F2 / (s * s));
     f[10] = (float)(null_keys / s);
-    f[11] = k.dropped ? 1.f : 0.f;
+    f[11] = (float)(k.dropped / s);
     f[12] = lane_bits <= 33 ? 1.f : 0.f;
     f[13] = lane_bits > 33 && lane_bits <= 65 ? 1.f : 0.f;
     f[14] = lane_bits > 65 ? 1.f : 0.f;
@@ -1432,7 +1438,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         res->stats.distinct_estimate = d_est;
         return res;
     }
-    if (opt.method == "auto" && opt.distinct_hint <= 0 && sizer_installed()) {
+    if (opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && sizer_installed()) {
         // the learned sizer picks the engine and, for hash, the cardinality
         // the partitions are sized for; overflow still falls back exactly
         float pred[2];
@@ -1450,6 +1456,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         d_est = std::max<double>(g, sk[ST_SAMPLE_DISTINCT]);
     }
     if (opt.distinct_hint > 0) d_est = (double)opt.distinct_hint;
+    if (P.d_retry > 0) d_est = P.d_retry;
     const double d_tail = std::max(0.0, d_est - sk[ST_HOT]);
     uint64_t P_parts;
     if (opt.partitions > 0) {
@@ -1460,6 +1467,13 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         // that miss a full partition table go to the fallback table; only
         // its overflow sends the query to the global-table engine
         P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_tail / tail_cap)));
+        // and enough partitions for the aggregation to spread over the
+        // device: about 8K tail rows (selected rows outside the hot keys,
+        // projected from the sample) per partition, up to two radix passes
+        const double hot_frac =
+            sk[ST_SAMPLE_SEL] ? std::min(1.0, (double)sk[ST_HOT_SAMPLE_ROWS] / sk[ST_SAMPLE_SEL]) : 0.0;
+        const double tail_rows = sketch.n_sel * (1.0 - hot_frac);
+        P_parts = std::max<uint64_t>(P_parts, std::min<uint64_t>(1024, pow2_floor((uint64_t)(tail_rows / 8192) + 1)));
     }
     P_parts = std::min<uint64_t>(P_parts, 1u << 15);
     const int pbits = log2u(P_parts);
@@ -1568,8 +1582,10 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     if (host_ctrl[1]) {
         // the fallback table overflowed and is dirty: the caller re-runs the
-        // query on the global-table engine, which resizes
+        // query with a larger estimate, or on the global-table engine
         g_fb.clean = false;
+        P.overflowed = true;
+        P.d_used = d_est;
         collect_kernel_times();
         return nullptr;
     }

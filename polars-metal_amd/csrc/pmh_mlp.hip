@@ -334,14 +334,30 @@ void sizer_install(const float *params) {
 
 bool sizer_installed() { return g_sizer.on; }
 
+// One query's prediction, evaluated on the host: a single 16-64-64-2
+// forward is about 5K multiply-adds, far less than a device round trip
+// (copy in, launch, copy out, synchronise) on the query's critical path.
+// Same function as mlp_forward, up to the order of float additions.
 void sizer_predict(const float *feat, float *out) {
-    Runtime &R = rt();
-    float *dx = g_sizer.dev + kMlpParams, *dy = dx + F;
-    PMH_HIP(hipMemcpyAsync(dx, feat, F * 4, hipMemcpyHostToDevice, R.stream));
-    hipLaunchKernelGGL(mlp_forward, dim3(1), dim3(64), 0, R.stream, g_sizer.dev, dx, 1u, dy);
-    launch_check();
-    PMH_HIP(hipMemcpyAsync(out, dy, 2 * 4, hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipStreamSynchronize(R.stream));
+    const float *p = g_sizer.host;
+    float h1[H], h2[H];
+    for (int j = 0; j < H; ++j) {
+        float z = p[O_B1 + j];
+        for (int i = 0; i < F; ++i) z += p[O_W1 + i * H + j] * feat[i];
+        h1[j] = z > 0.f ? z : 0.f;
+    }
+    for (int j = 0; j < H; ++j) {
+        float z = p[O_B2 + j];
+        for (int k = 0; k < H; ++k) z += p[O_W2 + k * H + j] * h1[k];
+        h2[j] = z > 0.f ? z : 0.f;
+    }
+    float o0 = p[O_B3 + 0], o1 = p[O_B3 + 1];
+    for (int j = 0; j < H; ++j) {
+        o0 += p[O_W3 + j * 2 + 0] * h2[j];
+        o1 += p[O_W3 + j * 2 + 1] * h2[j];
+    }
+    out[0] = o0;
+    out[1] = o1;
 }
 
 void sizer_predict_many(const float *feat, uint32_t n, float *out) {

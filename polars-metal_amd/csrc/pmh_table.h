@@ -331,6 +331,9 @@ struct Prep {
     float feat[PMH_MLP_IN] = {};  // the learned sizer's features, once the sketch ran
     int route = 0;                // 1: the learned sizer sent the query to the sort engine
     int sizer_choice = 0;         // pmh_exec_stats::sizer_choice
+    bool overflowed = false;      // the partitioned engine's fallback table overflowed
+    double d_used = 0;            // the distinct estimate that run was sized for
+    double d_retry = 0;           // > 0: size the partitioned engine for this many keys
 };
 
 // Sample sketch of the partitioned engine (host side).

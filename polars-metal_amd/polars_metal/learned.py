@@ -13,8 +13,8 @@ engine and the cardinality the hash partitions are sized for, and the
 engines' overflow fallbacks keep every output exact whatever it predicts.
 
 Targets per logged job: ``y0 = log2(groups + 1) / 32`` and ``y1 = 1`` if the
-sort engine was faster than the plain hash path, ``0`` otherwise (``-1``:
-not timed)."""
+sort engine, reached through the sketch the router runs first, was faster
+than the plain hash path, ``0`` otherwise (``-1``: not timed)."""
 from __future__ import annotations
 
 import ctypes
@@ -116,6 +116,14 @@ def install(params: np.ndarray | None) -> None:
     _lib.check(lib.pmh_sizer_install(_fp(p)))
 
 
+def install_shipped() -> dict:
+    """Install the model shipped with the package (trained on the device by
+    ``python -m polars_metal.train_sizer``); returns its training report."""
+    d = json.loads(MODEL_PATH.read_text())
+    install(np.asarray(d["params"], np.float32))
+    return d.get("meta", {})
+
+
 def load_model(path: Path = MODEL_PATH) -> np.ndarray:
     d = json.loads(Path(path).read_text())
     return np.asarray(d["params"], np.float32)
@@ -153,19 +161,23 @@ def timed(run_query, plan: dict, n: int, cols: list) -> tuple[dict, float]:
 def collect_logs(jobs, run_query, log=print) -> tuple[np.ndarray, np.ndarray, list]:
     """Logs of the plain method: for each job ``(plan, n_rows, columns)``,
     the sketch features, the true group count and both engines' wall times
-    (each call returns only after its device work).  ``run_query(plan,
-    n_rows, columns)`` runs one query and returns its stats."""
+    (each call returns only after its device work).  A query the sizer
+    routes to sort has run the sketch first, so the sort time logged is the
+    sketch's plus the sort engine's.  ``run_query(plan, n_rows, columns)``
+    runs one query and returns its stats."""
     xs, ys, meta = [], [], []
     t0 = time.perf_counter()
     for i, (plan, n, cols) in enumerate(jobs):
+        t = time.perf_counter()
         feat, _ = sketch_features(plan, n, cols)
+        tk = (time.perf_counter() - t) * 1e3
         sh, th = timed(run_query, dict(plan, options={"method": "hash"}), n, cols)
         ss, ts = timed(run_query, dict(plan, options={"method": "sort"}), n, cols)
         if sh["n_groups"] != ss["n_groups"]:
             raise AssertionError(f"job {i}: hash and sort disagree on the group count")
         xs.append(feat)
-        ys.append(targets(sh["n_groups"], th, ts))
-        meta.append({"rows": n, "groups": sh["n_groups"], "hash_ms": th, "sort_ms": ts})
+        ys.append(targets(sh["n_groups"], th, ts + tk))
+        meta.append({"rows": n, "groups": sh["n_groups"], "hash_ms": th, "sort_ms": ts + tk, "sketch_ms": tk})
         if log and (i + 1) % 100 == 0:
             log(f"logged {i + 1} jobs in {time.perf_counter() - t0:.1f} s")
     return np.stack(xs), np.stack(ys), meta

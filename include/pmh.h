@@ -204,6 +204,11 @@ int pmh_buffer_zeroed(uint64_t bytes, void **out_device_ptr);
 int pmh_buffer_from_host(const void *src, uint64_t bytes, void **out_device_ptr);
 int pmh_buffer_to_host(const void *device_ptr, void *dst, uint64_t bytes);
 int pmh_buffer_free(void *device_ptr);
+/* Device-to-device copy of `bytes` (synchronous).  Replaces the device
+ * buffer copies of the reference's buffer crate
+ * (crates/polars-metal-buffer/src/device.rs:55-153); also the known-size
+ * copy that calibrates HBM traffic counters. */
+int pmh_buffer_copy(void *dst_device, const void *src_device, uint64_t bytes);
 
 void pmh_result_free(pmh_result *r);
 void pmh_string_free(char *s);

@@ -187,6 +187,15 @@ int pmh_buffer_to_host(const void *dev, void *dst, uint64_t bytes) {
     });
 }
 
+int pmh_buffer_copy(void *dst, const void *src, uint64_t bytes) {
+    return guarded([&] {
+        if (bytes && (!dst || !src)) fail(PMH_E_VALUE, "polars_metal: NULL buffer");
+        Runtime &r = rt();
+        if (bytes) PMH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, r.stream));
+        PMH_HIP(hipStreamSynchronize(r.stream));
+    });
+}
+
 int pmh_buffer_free(void *dev) {
     return guarded([&] {
         Runtime &r = rt();

@@ -40,6 +40,9 @@ constexpr int kRadix = 1 << kRadixBits;
 constexpr int kPartThreads = 1024;
 constexpr int kPartRows = 4;               // rows per thread per partition tile
 constexpr int kPartTile = kPartThreads * kPartRows;
+constexpr int kWideRadixBits = 10;  // one-pass partitioning: up to 1024 partitions
+constexpr int kWideRadix = 1 << kWideRadixBits;
+static_assert(kWideRadix == kPartThreads, "one thread per digit in the block scans");
 constexpr uint16_t kFlagAllOnes = 1u << 8, kFlagNull = 1u << 9;
 constexpr int kProbe = 16;
 
@@ -80,6 +83,7 @@ struct HLaunch {
     Spill ping, pong;
     uint32_t *tile_hist;   // [kRadix][max_tiles]
     uint32_t *digit_base;  // [kRadix]
+    uint64_t *part_start;  // [partitions + 1]: first spill row of each partition after the passes
     uint32_t max_tiles, pbits, sortbits, passes;
     // per-partition aggregation
     uint32_t tail_slots, pad1;
@@ -134,6 +138,22 @@ __device__ __forceinline__ void gstore_w(void *p, int w, uint64_t i, uint64_t x)
 }
 
 // Row accessor over a spilled record: value columns only.
+// A spill row whose first two value columns are already loaded.
+struct SpillRowV {
+    const HLaunch *__restrict__ L;
+    const Spill *S;
+    uint64_t i;
+    uint16_t flags;
+    uint64_t v0, v1;
+    __device__ __forceinline__ bool valid(int c) const { return (flags >> L->col_val[c]) & 1; }
+    __device__ __forceinline__ uint64_t bits(int c) const {
+        const int j = L->col_val[c];
+        return j == 0 ? v0 : j == 1 ? v1 : gload_w(S->val[j], L->vw[j], i);
+    }
+    __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(L->q.cols[c].tag, bits(c)); }
+    __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
+};
+
 struct SpillRow {
     const HLaunch *__restrict__ L;
     const Spill *S;
@@ -147,6 +167,34 @@ struct SpillRow {
     __device__ __forceinline__ int64_t i64(int c) const { return raw_to_i64(L->q.cols[c].tag, bits(c)); }
     __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
 };
+
+// Exclusive scan of one value per thread across a 1024-thread block;
+// s_wave holds 16 words.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *s_wave) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) s_wave[wave] = inc;
+    __syncthreads();
+    if (wave == 0) {
+        const uint32_t w = lane < 16 ? s_wave[lane] : 0;
+        uint32_t wi = w;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const uint32_t y = __shfl_up(wi, o);
+            if (lane >= o) wi += y;
+        }
+        if (lane < 16) s_wave[lane] = wi - w;
+    }
+    __syncthreads();
+    const uint32_t r = s_wave[wave] + inc - x;
+    __syncthreads();  // s_wave may be reused right after
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // sample
@@ -214,6 +262,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     uint64_t *h_lo = (uint64_t *)(hist + 1024);
     uint64_t *h_hi = h_lo + L->hot_slots;
     __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop, s_hotrows;
+    __shared__ uint32_t s_wave[16];
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t CL = L->hot_slots;
     for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
@@ -223,22 +272,39 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
     uint32_t drops = 0, sidekeys = 0;
-    for (uint32_t i = tid; i < nsel; i += bs) {
-        if (L->samp_sp[i]) { ++sidekeys; continue; }  // side keys are never hot
-        const uint64_t h = hash_key(L->samp_lo[i], wide ? L->samp_hi[i] : 0);
-        const uint32_t fp = (uint32_t)(h >> 32) | 1u;
-        uint32_t s = (uint32_t)h & (SS - 1);
-        bool done = false;
-        for (int p = 0; p < kSampleProbe; ++p) {
-            uint32_t cur = s_fp[s];
-            if (cur == 0) {
-                const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
-                cur = old == 0u ? fp : old;
-            }
-            if (cur == fp) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
-            s = (s + 1) & (SS - 1);
+    // sampled keys in batches of kSampleU rows per thread, every load of a
+    // batch issued before the first probe
+    constexpr int kSampleU = 8;
+    for (uint32_t i0 = tid; i0 < nsel; i0 += bs * kSampleU) {
+        uint64_t lo[kSampleU], hi[kSampleU];
+        uint8_t sp[kSampleU];
+#pragma unroll
+        for (int u = 0; u < kSampleU; ++u) {
+            const uint32_t i = i0 + u * bs;
+            const uint32_t ii = i < nsel ? i : 0;
+            lo[u] = gld<uint64_t>(L->samp_lo, ii);
+            hi[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
+            sp[u] = gld<uint8_t>(L->samp_sp, ii);
         }
-        if (!done) ++drops;
+#pragma unroll
+        for (int u = 0; u < kSampleU; ++u) {
+            if (i0 + u * bs >= nsel) break;
+            if (sp[u]) { ++sidekeys; continue; }  // side keys are never hot
+            const uint64_t h = hash_key(lo[u], hi[u]);
+            const uint32_t fp = (uint32_t)(h >> 32) | 1u;
+            uint32_t s = (uint32_t)h & (SS - 1);
+            bool done = false;
+            for (int p = 0; p < kSampleProbe; ++p) {
+                uint32_t cur = s_fp[s];
+                if (cur == 0) {
+                    const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
+                    cur = old == 0u ? fp : old;
+                }
+                if (cur == fp) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
+                s = (s + 1) & (SS - 1);
+            }
+            if (!done) ++drops;
+        }
     }
     if (drops) atomicAdd(&s_drop, drops);
     if (sidekeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], sidekeys);
@@ -257,16 +323,23 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     atomicAdd(&s_distinct, dist);
     __syncthreads();
     for (uint32_t i = tid; i < 1024; i += bs) L->sketch_hist[i] = hist[i];
-    if (tid == 0) {
-        // largest count threshold T such that the keys with count > T fit
-        uint32_t acc = 0, T = 0;
+    {
+        // the largest count threshold T such that the keys with count > T
+        // fit in hot_max: with S(c) the keys of count >= c, T is the c >= 1
+        // where S(c) > hot_max >= S(c + 1) (none: T = 0, every key fits).
+        // Thread t holds bucket c = 1023 - t, so a block scan gives S(c + 1).
+        const uint32_t c = 1023 - tid;
+        const uint32_t x = c >= 1 ? hist[c] : 0;
+        const uint32_t above = block_excl_scan(x, s_wave);  // S(c + 1)
         const uint32_t kmax = L->hot_max;
-        for (int c = 1023; c >= 1; --c) {
-            if (acc + hist[c] > kmax) { T = (uint32_t)c; break; }
-            acc += hist[c];
+        if (tid == 0) { s_thresh = 0; s_take_eq = 0; }
+        __syncthreads();
+        if (c >= 1 && above + x > kmax && above <= kmax) {
+            s_thresh = c;
+            s_take_eq = kmax - above;  // keys with count == T still allowed
         }
-        s_thresh = T;                             // T == 0: every sampled key fits
-        s_take_eq = T ? kmax - acc : 0;           // keys with count == T still allowed
+    }
+    if (tid == 0) {
         L->stats[ST_SAMPLE_DISTINCT] = s_distinct;
         L->stats[ST_SAMPLE_F1] = s_f1;
         L->stats[ST_SAMPLE_F2] = s_f2;
@@ -285,33 +358,47 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     // recover the keys of picked fingerprints: the first sampled row of each
     // picked fingerprint places its key in the hot table
     uint32_t hot_rows = 0;
-    for (uint32_t i = tid; i < nsel && CL; i += bs) {
-        if (L->samp_sp[i]) continue;
-        const uint64_t lo = L->samp_lo[i], hi = wide ? L->samp_hi[i] : 0;
-        const uint64_t h = hash_key(lo, hi);
-        const uint32_t fp = (uint32_t)(h >> 32) | 1u;
-        uint32_t s = (uint32_t)h & (SS - 1);
-        for (int p = 0; p < kSampleProbe; ++p) {
-            const uint32_t cur = s_fp[s];
-            if (cur == 0) break;
-            if (cur == fp) {
-                if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
-                    hot_rows += s_cnt[s] & 0x3FFFFFFFu;
-                    uint32_t t = hot_home(h, CL);
-                    for (uint32_t pp = 0; pp < CL - 2; ++pp) {
-                        unsigned long long old = atomicCAS((unsigned long long *)&h_lo[t], (unsigned long long)kEmpty,
-                                                           (unsigned long long)lo);
-                        if (old == kEmpty) {
-                            if (wide) h_hi[t] = hi;
-                            atomicAdd(&s_hot, 1u);
-                            break;
+    for (uint32_t i0 = tid; i0 < nsel && CL; i0 += bs * kSampleU) {
+        uint64_t lob[kSampleU], hib[kSampleU];
+        uint8_t spb[kSampleU];
+#pragma unroll
+        for (int u = 0; u < kSampleU; ++u) {
+            const uint32_t i = i0 + u * bs;
+            const uint32_t ii = i < nsel ? i : 0;
+            lob[u] = gld<uint64_t>(L->samp_lo, ii);
+            hib[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
+            spb[u] = gld<uint8_t>(L->samp_sp, ii);
+        }
+#pragma unroll
+        for (int u = 0; u < kSampleU; ++u) {
+            if (i0 + u * bs >= nsel) break;
+            if (spb[u]) continue;
+            const uint64_t lo = lob[u], hi = hib[u];
+            const uint64_t h = hash_key(lo, hi);
+            const uint32_t fp = (uint32_t)(h >> 32) | 1u;
+            uint32_t s = (uint32_t)h & (SS - 1);
+            for (int p = 0; p < kSampleProbe; ++p) {
+                const uint32_t cur = s_fp[s];
+                if (cur == 0) break;
+                if (cur == fp) {
+                    if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
+                        hot_rows += s_cnt[s] & 0x3FFFFFFFu;
+                        uint32_t t = hot_home(h, CL);
+                        for (uint32_t pp = 0; pp < CL - 2; ++pp) {
+                            unsigned long long old = atomicCAS((unsigned long long *)&h_lo[t], (unsigned long long)kEmpty,
+                                                               (unsigned long long)lo);
+                            if (old == kEmpty) {
+                                if (wide) h_hi[t] = hi;
+                                atomicAdd(&s_hot, 1u);
+                                break;
+                            }
+                            t = hot_next(t, CL);
                         }
-                        t = hot_next(t, CL);
                     }
+                    break;
                 }
-                break;
+                s = (s + 1) & (SS - 1);
             }
-            s = (s + 1) & (SS - 1);
         }
     }
     if (hot_rows) atomicAdd(&s_hotrows, hot_rows);
@@ -330,6 +417,75 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         L->stats[ST_HOT] = s_hot;
         L->stats[ST_HOT_SAMPLE_ROWS] = s_hotrows;
         L->out_cursor[0] = 0;
+    }
+    // A table that dropped more than an eighth of the sample (far more
+    // distinct sampled keys than slots) leaves a frequency profile biased to
+    // the keys that arrived first.  The sketch then comes from a hash
+    // subsample instead (distinct sampling): only keys whose hash has its top
+    // three bits clear, an eighth of the key space, with all of their sampled
+    // rows, counted again in the cleared table.  Their counts times eight
+    // estimate the whole sample's, and the estimators are linear in them.
+    if (s_drop * 8 > nsel) {
+        __syncthreads();
+        for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
+        for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
+        if (tid == 0) { s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; }
+        __syncthreads();
+        uint32_t drops2 = 0;
+        for (uint32_t i0 = tid; i0 < nsel; i0 += bs * kSampleU) {
+            uint64_t lo[kSampleU], hi[kSampleU];
+            uint8_t sp[kSampleU];
+#pragma unroll
+            for (int u = 0; u < kSampleU; ++u) {
+                const uint32_t i = i0 + u * bs;
+                const uint32_t ii = i < nsel ? i : 0;
+                lo[u] = gld<uint64_t>(L->samp_lo, ii);
+                hi[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
+                sp[u] = gld<uint8_t>(L->samp_sp, ii);
+            }
+#pragma unroll
+            for (int u = 0; u < kSampleU; ++u) {
+                if (i0 + u * bs >= nsel) break;
+                if (sp[u]) continue;
+                const uint64_t h = hash_key(lo[u], hi[u]);
+                if (h >> 61) continue;
+                const uint32_t fp = (uint32_t)(h >> 32) | 1u;
+                uint32_t s = (uint32_t)h & (SS - 1);
+                bool done = false;
+                for (int p = 0; p < kSampleProbe; ++p) {
+                    uint32_t cur = s_fp[s];
+                    if (cur == 0) {
+                        const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
+                        cur = old == 0u ? fp : old;
+                    }
+                    if (cur == fp) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
+                    s = (s + 1) & (SS - 1);
+                }
+                if (!done) ++drops2;
+            }
+        }
+        if (drops2) atomicAdd(&s_drop, drops2);
+        __syncthreads();
+        uint32_t g1 = 0, g2 = 0, gd = 0;
+        for (uint32_t i = tid; i < SS; i += bs) {
+            const uint32_t c = s_cnt[i];
+            if (!c) continue;
+            ++gd;
+            if (c == 1) ++g1;
+            if (c == 2) ++g2;
+            atomicAdd(&hist[min(c, 1023u)], 1u);
+        }
+        atomicAdd(&s_f1, g1);
+        atomicAdd(&s_f2, g2);
+        atomicAdd(&s_distinct, gd);
+        __syncthreads();
+        for (uint32_t i = tid; i < 1024; i += bs) L->sketch_hist[i] = hist[i] * 8;
+        if (tid == 0) {
+            L->stats[ST_SAMPLE_DISTINCT] = s_distinct * 8;
+            L->stats[ST_SAMPLE_F1] = s_f1 * 8;
+            L->stats[ST_SAMPLE_F2] = s_f2 * 8;
+            L->stats[ST_SAMPLE_DROPPED] = s_drop * 8;
+        }
     }
 }
 
@@ -751,6 +907,138 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     }
 }
 
+// One-pass partitioning, for at most 1024 partitions.  Rows only need to be
+// grouped by partition, not kept in a stable order, so a tile ranks its rows
+// with LDS counters and a single pass over the scan's tail replaces the
+// stable five-bit passes.  The digit is the partition: the low pbits of the
+// spill hash.
+
+__global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restrict__ L, const Spill *src_p,
+                                                        uint32_t nd) {
+    __shared__ uint32_t hist[kWideRadix];
+    const Spill &src = *src_p;
+    const uint32_t ntiles = L->tile_off[L->nb];
+    const uint64_t total = L->region_off[L->nb];
+    const bool wide = L->wide;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        if (threadIdx.x < nd) hist[threadIdx.x] = 0;
+        __syncthreads();
+        TileSpan ts;
+        tile_span(L, true, t, ntiles, total, ts);
+        uint32_t d[kPartRows];
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + threadIdx.x;
+            const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
+            d[k] = (uint32_t)spill_hash(gld<uint64_t>(src.klo, s), wide ? gld<uint64_t>(src.khi, s) : 0,
+                                        gld<uint16_t>(src.flags, s)) &
+                   (nd - 1);
+        }
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k)
+            if (k * kPartThreads + threadIdx.x < ts.rows) atomicAdd(&hist[d[k]], 1u);
+        __syncthreads();
+        if (threadIdx.x < nd) L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t] = hist[threadIdx.x];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__restrict__ L, const Spill *src_p,
+                                                           const Spill *dst_p, uint32_t nd) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t run[kWideRadix];     // rows of each digit in this tile
+    __shared__ uint32_t dstart[kWideRadix];  // LDS start of each digit run
+    __shared__ uint32_t gbase[kWideRadix];   // global start of each digit run for this tile
+    __shared__ uint32_t dtot[kWideRadix];    // global start of each digit
+    __shared__ uint32_t s_wave[16];
+    const Spill &src = *src_p;
+    const Spill &dst = *dst_p;
+    const QueryDesc *__restrict__ q = &L->q;
+    const int nv = q->n_vals;
+    const bool wide = L->wide;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ntiles = L->tile_off[L->nb];
+    const uint64_t total = L->region_off[L->nb];
+    // LDS staging: keys, [hi], flags, digits (16-bit), then value columns
+    uint64_t *st_lo = (uint64_t *)smem;
+    uint64_t *st_hi = st_lo + kPartTile;
+    uint8_t *p = (uint8_t *)(wide ? st_hi + kPartTile : st_hi);
+    uint16_t *st_fl = (uint16_t *)p;
+    uint16_t *st_d = st_fl + kPartTile;
+    uint8_t *st_v = (uint8_t *)(st_d + kPartTile);
+    {
+        const uint32_t x = tid < nd ? L->digit_base[tid] : 0;
+        const uint32_t ex = block_excl_scan(x, s_wave);
+        dtot[tid] = ex;
+        // the digit is the partition: its start after this pass
+        if (blockIdx.x == 0 && tid < nd) L->part_start[tid] = ex;
+        if (blockIdx.x == 0 && tid == 0) L->part_start[nd] = total;
+    }
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        TileSpan ts;
+        tile_span(L, true, t, ntiles, total, ts);
+        run[tid] = 0;
+        __syncthreads();
+        uint64_t klo[kPartRows], khi[kPartRows];
+        uint16_t fl[kPartRows];
+        uint32_t dg[kPartRows], rk[kPartRows];
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + tid;
+            const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
+            klo[k] = gld<uint64_t>(src.klo, s);
+            khi[k] = wide ? gld<uint64_t>(src.khi, s) : 0;
+            fl[k] = gld<uint16_t>(src.flags, s);
+        }
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            dg[k] = (uint32_t)spill_hash(klo[k], khi[k], fl[k]) & (nd - 1);
+            rk[k] = k * kPartThreads + tid < ts.rows ? atomicAdd(&run[dg[k]], 1u) : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        {
+            const uint32_t c = run[tid];
+            const uint32_t st = block_excl_scan(c, s_wave);
+            dstart[tid] = st;
+            gbase[tid] = tid < nd ? dtot[tid] + L->tile_hist[(uint64_t)tid * L->max_tiles + t] : 0;
+        }
+        __syncthreads();
+        // stage in digit order
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            if (rk[k] == 0xFFFFFFFFu) continue;
+            const uint64_t s = ts.src_begin + k * kPartThreads + tid;
+            const uint32_t sp = dstart[dg[k]] + rk[k];
+            st_lo[sp] = klo[k];
+            if (wide) st_hi[sp] = khi[k];
+            st_fl[sp] = fl[k];
+            st_d[sp] = (uint16_t)dg[k];
+            uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                store_w(vp, w, sp, gload_w(src.val[j], w, s));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+        // write runs: consecutive staged rows -> consecutive positions
+        for (uint32_t sp = tid; sp < ts.rows; sp += kPartThreads) {
+            const uint32_t d = st_d[sp];
+            const uint64_t g = (uint64_t)gbase[d] + (sp - dstart[d]);
+            gst<uint64_t>(dst.klo, g, st_lo[sp]);
+            if (wide) gst<uint64_t>(dst.khi, g, st_hi[sp]);
+            gst<uint16_t>(dst.flags, g, st_fl[sp]);
+            const uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                gstore_w(dst.val[j], w, g, load_w(vp, w, sp));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // per-partition aggregation
 // ---------------------------------------------------------------------------
@@ -775,6 +1063,16 @@ __device__ __forceinline__ uint64_t part_lower_bound(const HLaunch *__restrict__
     return lo;
 }
 
+// Partition starts after stable multi-digit passes: one binary search per
+// partition, all in parallel.
+__global__ __launch_bounds__(256) void hs_part_bounds(const HLaunch *__restrict__ L, const Spill *src_p) {
+    const Spill &S = *src_p;
+    const uint32_t P = 1u << L->pbits;
+    const uint64_t n = L->region_off[L->nb];
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p <= P; p += gridDim.x * blockDim.x)
+        L->part_start[p] = p == 0 ? 0 : p == P ? n : part_lower_bound(L, S, L->wide, n, p);
+}
+
 __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ L, const Spill *src_p) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint64_t s_range[2];
@@ -786,7 +1084,6 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     const uint32_t nls = CL + 2;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t pmask = (1u << L->pbits) - 1;
-    const uint64_t n = L->region_off[L->nb];
     uint64_t *lk = (uint64_t *)smem;
     uint64_t *lh = lk + nls;
     uint32_t *lst = (uint32_t *)(lh + nls);  // wide: 0 free, 1 writing, 2 published
@@ -794,7 +1091,7 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     LSink ls{state, L->lds3_off, nls};
     GSink gs{L->t.arr, L->t.mask + 3};
     for (uint32_t p = blockIdx.x; p <= pmask; p += gridDim.x) {
-        if (tid < 2) s_range[tid] = part_lower_bound(L, S, wide, n, p + tid);
+        if (tid < 2) s_range[tid] = L->part_start[p + tid];
         if (tid == 0) s_count = 0;
         for (uint32_t i = tid; i < nls; i += bs) {
             lk[i] = kEmpty;
@@ -811,62 +1108,82 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
             }
         }
         __syncthreads();
-        const uint64_t b = s_range[0], e = p == pmask ? n : s_range[1];
-        for (uint64_t i = b + tid; i < e; i += bs) {
-            const uint16_t fl = gld<uint16_t>(S.flags, i);
-            SpillRow row{L, &S, i, fl};
-            Key k{gld<uint64_t>(S.klo, i), wide ? gld<uint64_t>(S.khi, i) : 0,
-                  (fl & kFlagAllOnes) ? 1 : ((fl & kFlagNull) ? 2 : 0)};
-            int32_t slot = -1;
-            uint64_t h = 0;
-            if (k.special) {
-                slot = (int32_t)(CL + k.special - 1);
-            } else {
-                h = hash_key(k.lo, k.hi);
-                uint32_t s = (uint32_t)(h >> 20) & (CL - 1);
-                for (int pr = 0; pr < kProbe && slot < 0; ++pr) {
-                    if (!wide) {
-                        uint64_t cur = __hip_atomic_load(&lk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if (cur == kEmpty) {
-                            unsigned long long old = atomicCAS((unsigned long long *)&lk[s],
-                                                               (unsigned long long)kEmpty, (unsigned long long)k.lo);
-                            if (old == kEmpty || old == k.lo) { slot = (int32_t)s; break; }
-                        } else if (cur == k.lo) {
-                            slot = (int32_t)s;
-                            break;
-                        }
-                    } else {
-                        // A claimer publishes within its own loop iteration,
-                        // so the wait ends; the bound turns a broken
-                        // invariant into a reported error instead of a hang.
-                        for (uint32_t spin = 0;; ++spin) {
-                            if (spin > (1u << 22)) { atomicOr(&L->stats[ST_ERROR], 1u); break; }
-                            uint32_t st = __hip_atomic_load(&lst[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                            if (st == 2u) {
-                                if (lk[s] == k.lo && lh[s] == k.hi) slot = (int32_t)s;
+        const uint64_t b = s_range[0], e = s_range[1];
+        // four rows per thread per step, every load issued before the first
+        // probe
+        constexpr int U = 4;
+        const int nv = q->n_vals;
+        for (uint64_t i0 = b + tid; i0 < e; i0 += (uint64_t)U * bs) {
+            uint16_t flb[U];
+            uint64_t klb[U], khb[U], v0b[U], v1b[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + (uint64_t)u * bs;
+                const uint64_t ii = i < e ? i : b;
+                flb[u] = gld<uint16_t>(S.flags, ii);
+                klb[u] = gld<uint64_t>(S.klo, ii);
+                khb[u] = wide ? gld<uint64_t>(S.khi, ii) : 0;
+                v0b[u] = nv > 0 ? gload_w(S.val[0], L->vw[0], ii) : 0;
+                v1b[u] = nv > 1 ? gload_w(S.val[1], L->vw[1], ii) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + (uint64_t)u * bs;
+                if (i >= e) break;
+                const uint16_t fl = flb[u];
+                SpillRowV row{L, &S, i, fl, v0b[u], v1b[u]};
+                Key k{klb[u], khb[u], (fl & kFlagAllOnes) ? 1 : ((fl & kFlagNull) ? 2 : 0)};
+                int32_t slot = -1;
+                uint64_t h = 0;
+                if (k.special) {
+                    slot = (int32_t)(CL + k.special - 1);
+                } else {
+                    h = hash_key(k.lo, k.hi);
+                    uint32_t s = (uint32_t)(h >> 20) & (CL - 1);
+                    for (int pr = 0; pr < kProbe && slot < 0; ++pr) {
+                        if (!wide) {
+                            uint64_t cur = __hip_atomic_load(&lk[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                            if (cur == kEmpty) {
+                                unsigned long long old = atomicCAS((unsigned long long *)&lk[s],
+                                                                   (unsigned long long)kEmpty, (unsigned long long)k.lo);
+                                if (old == kEmpty || old == k.lo) { slot = (int32_t)s; break; }
+                            } else if (cur == k.lo) {
+                                slot = (int32_t)s;
                                 break;
                             }
-                            if (st == 0u) {
-                                if (atomicCAS(&lst[s], 0u, 1u) == 0u) {
-                                    lk[s] = k.lo;
-                                    lh[s] = k.hi;
-                                    __hip_atomic_store(&lst[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                    slot = (int32_t)s;
+                        } else {
+                            // A claimer publishes within its own loop iteration,
+                            // so the wait ends; the bound turns a broken
+                            // invariant into a reported error instead of a hang.
+                            for (uint32_t spin = 0;; ++spin) {
+                                if (spin > (1u << 22)) { atomicOr(&L->stats[ST_ERROR], 1u); break; }
+                                uint32_t st = __hip_atomic_load(&lst[s], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                if (st == 2u) {
+                                    if (lk[s] == k.lo && lh[s] == k.hi) slot = (int32_t)s;
                                     break;
                                 }
+                                if (st == 0u) {
+                                    if (atomicCAS(&lst[s], 0u, 1u) == 0u) {
+                                        lk[s] = k.lo;
+                                        lh[s] = k.hi;
+                                        __hip_atomic_store(&lst[s], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                                        slot = (int32_t)s;
+                                        break;
+                                    }
+                                }
+                                // claimed by another lane: re-read
                             }
-                            // claimed by another lane: re-read
+                            if (slot >= 0) break;
                         }
-                        if (slot >= 0) break;
+                        s = (s + 1) & (CL - 1);
                     }
-                    s = (s + 1) & (CL - 1);
                 }
-            }
-            if (slot >= 0) {
-                add_row(ls, q, (uint32_t)slot, row);
-            } else {
-                const int64_t g = wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
-                if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+                if (slot >= 0) {
+                    add_row(ls, q, (uint32_t)slot, row);
+                } else {
+                    const int64_t g = wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
+                    if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+                }
             }
         }
         __syncthreads();
@@ -1104,9 +1421,9 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
 // Distinct keys among the selected rows: the geometric mean of GEE
 // (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
 // estimator (Haas et al., 1995; it overshoots), floored at the sample's own
-// distinct count.  A fingerprint table that dropped most of the sample (far
-// more distinct keys than slots: a near-unique sample) extrapolates
-// linearly.
+// distinct count.  A sample of almost only singletons (a near-unique
+// selection, or a fingerprint table that still dropped most of it)
+// extrapolates linearly.
 double gee_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
@@ -1119,7 +1436,7 @@ double shlosser_estimate(const Sketch &k) {
 double plain_distinct_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct;
-    if (2.0 * k.dropped > s) return std::max(d, k.n_sel * d / s);
+    if (2.0 * k.dropped > s || k.f1 >= 0.95 * s) return std::max(d, k.n_sel * d / s);
     if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
     const double gee = gee_estimate(k), shl = shlosser_estimate(k);
     return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));
@@ -1340,8 +1657,9 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     alloc_spill("hs:spill", L.spill, cap, wide, L.vw, nv);
     alloc_spill("hs:ping", L.ping, cap, wide, L.vw, nv);
     alloc_spill("hs:pong", L.pong, cap, wide, L.vw, nv);
-    L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)kRadix * max_tiles * 4 + 64);
-    L.digit_base = (uint32_t *)slab("hs:digit_base", kRadix * 4 + 64);
+    L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)std::max(kRadix, kWideRadix) * max_tiles * 4 + 64);
+    L.digit_base = (uint32_t *)slab("hs:digit_base", std::max(kRadix, kWideRadix) * 4 + 64);
+    L.part_start = (uint64_t *)slab("hs:part_start", ((1ull << 15) + 2) * 8 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
     L.stats = (uint32_t *)slab("hs:stats", ST_N * 4 + 64);
     L.sketch_hist = (uint32_t *)slab("hs:sketch_hist", 1024 * 4 + 64);
@@ -1477,8 +1795,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     P_parts = std::min<uint64_t>(P_parts, 1u << 15);
     const int pbits = log2u(P_parts);
-    const int passes = std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
-    const int sortbits = passes * kRadixBits;
+    // up to 1024 partitions: one unstable pass on the partition digit, when
+    // its LDS staging (16-bit digits) fits
+    const size_t stage_lds1 = (size_t)kPartTile * (rec_bytes + 1);
+    const bool one_pass = pbits <= kWideRadixBits && stage_lds1 + 5 * kWideRadix * 4 <= lds_budget;
+    const int passes = one_pass ? 1 : std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
+    const int sortbits = one_pass ? pbits : passes * kRadixBits;
     L.pbits = (uint32_t)pbits;
     L.sortbits = (uint32_t)sortbits;
     L.passes = (uint32_t)passes;
@@ -1520,7 +1842,29 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_lds));
     const Spill *bufs[3] = {&dL->spill, &dL->ping, &dL->pong};
     int src = 0, dst = 1;
-    for (int ps = 0; ps < passes; ++ps) {
+    if (one_pass) {
+        const uint32_t nd = 1u << pbits;
+        PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)stage_lds1));
+        {
+            KernelTimer kt("rp_hist");
+            hipLaunchKernelGGL(rp_hist1, dim3(pgrid), dim3(kPartThreads), 0, R.stream, dL, bufs[0], nd);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scan");
+            hipLaunchKernelGGL(rp_scan, dim3(nd), dim3(1024), 0, R.stream, dL, 1);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scatter");
+            hipLaunchKernelGGL(rp_scatter1, dim3(pgrid), dim3(kPartThreads), stage_lds1, R.stream, dL, bufs[0], bufs[1],
+                               nd);
+            launch_check();
+        }
+        src = 1;
+    }
+    for (int ps = 0; ps < passes && !one_pass; ++ps) {
         const int shift = ps * kRadixBits;
         {
             KernelTimer kt("rp_hist");
@@ -1541,6 +1885,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         }
         src = dst;
         dst = dst == 1 ? 2 : 1;
+    }
+    if (!one_pass) {
+        KernelTimer kt("hs_part_bounds");
+        hipLaunchKernelGGL(hs_part_bounds, dim3((uint32_t)ceil_div(P_parts + 1, 256)), dim3(256), 0, R.stream, dL,
+                           bufs[src]);
+        launch_check();
     }
     {
         KernelTimer kt("hs_part_agg");

@@ -135,6 +135,7 @@ SIGNATURES = {
     "pmh_buffer_from_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "pmh_buffer_to_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
     "pmh_buffer_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "pmh_buffer_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64]),
     "pmh_result_free": (None, [ctypes.POINTER(PmhResult)]),
     "pmh_string_free": (None, [ctypes.c_void_p]),
     "pmh_set_profiling": (ctypes.c_int, [ctypes.c_int]),

@@ -41,6 +41,13 @@ class DeviceBuffer:
             check(_lib.load().pmh_buffer_to_host(ctypes.c_void_p(self.ptr), out.ctypes.data, out.nbytes))
         return out
 
+    def copy_from(self, src: "DeviceBuffer", nbytes: int | None = None) -> None:
+        """Device-to-device copy of ``nbytes`` (default: all of src)."""
+        n = src.nbytes if nbytes is None else nbytes
+        if n > min(self.nbytes, src.nbytes):
+            raise ValueError("copy larger than a buffer")
+        check(_lib.load().pmh_buffer_copy(ctypes.c_void_p(self.ptr), ctypes.c_void_p(src.ptr), n))
+
     def free(self) -> None:
         if self.ptr:
             check(_lib.load().pmh_buffer_free(ctypes.c_void_p(self.ptr)))

@@ -1,0 +1,135 @@
+"""rocprofv3 summaries: kernel-trace tables and the scan kernel's HBM traffic.
+
+Traffic: run a workload under ``rocprofv3 --pmc FETCH_SIZE`` and again under
+``--pmc WRITE_SIZE`` (one counter group per run), do the same for a plain
+device copy of known size, then ``traffic`` reads the four result databases
+and writes the bytes per launch, each counter scaled by what it reads for
+the copy (which moves exactly its size each way):
+
+    rocprofv3 --pmc FETCH_SIZE -d OUT/f -o run -- python3 rocprof.py zipf100m
+    rocprofv3 --pmc WRITE_SIZE -d OUT/w -o run -- python3 rocprof.py zipf100m
+    rocprofv3 --pmc FETCH_SIZE -d OUT/cf -o run -- python3 rocprof.py copy
+    rocprofv3 --pmc WRITE_SIZE -d OUT/cw -o run -- python3 rocprof.py copy
+    python3 rocprof.py traffic OUT hs_scan_fast > traffic.json
+
+The counters are KiB per dispatch (summed over the device's channels).
+
+Kernel traces: ``python3 rocprof.py trace DB_DIR "title" "command"`` prints
+a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database."""
+from __future__ import annotations
+
+import argparse
+import collections
+import glob
+import json
+import sqlite3
+import sys
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+if str(_PKG.parent) not in sys.path:
+    sys.path.insert(0, str(_PKG.parent))
+
+COPY_BYTES = 1 << 30
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def run_zipf100m(steps: int, rows: int) -> None:
+    from polars_metal import _native, synth
+    from polars_metal.buffer import device_columns
+    frame, plan = synth.zipf(rows, seed=2)
+    cols, bufs = device_columns(frame)
+    del frame
+    for i in range(steps):
+        r = _native.execute_filter_groupby_device(plan, rows, cols)
+        print(f"step {i}: {r.stats['n_groups']} groups", flush=True)
+        r.free()
+
+
+def run_copy(steps: int) -> None:
+    from polars_metal.buffer import DeviceBuffer
+    a = DeviceBuffer.alloc(COPY_BYTES, zeroed=True)
+    b = DeviceBuffer.alloc(COPY_BYTES)
+    for i in range(steps):
+        b.copy_from(a)
+        print(f"copy {i}: {COPY_BYTES} bytes", flush=True)
+
+
+def counter_kib(db_dir: Path, counter: str) -> dict:
+    """Mean value of one counter per dispatch, per kernel name."""
+    db = glob.glob(str(db_dir / "*.db"))[0]
+    c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(counters_collection)")]
+    ki, ni, vi = cols.index("kernel_name"), cols.index("counter_name"), cols.index("value")
+    di = cols.index("dispatch_id")
+    tot = collections.defaultdict(float)
+    disp = collections.defaultdict(set)
+    for r in c.execute("select * from counters_collection"):
+        if r[ni] != counter:
+            continue
+        name = r[ki].split("(")[0].split("::")[-1].split("<")[0].strip()
+        tot[name] += r[vi]
+        disp[name].add(r[di])
+    return {k: tot[k] / len(disp[k]) for k in tot}
+
+
+def summarize(out: Path, kernel: str) -> dict:
+    f = counter_kib(out / "f", "FETCH_SIZE")
+    w = counter_kib(out / "w", "WRITE_SIZE")
+    cf = counter_kib(out / "cf", "FETCH_SIZE")
+    cw = counter_kib(out / "cw", "WRITE_SIZE")
+    fetch_factor = COPY_BYTES / (cf[COPY_KERNEL] * 1024.0)
+    write_factor = COPY_BYTES / (cw[COPY_KERNEL] * 1024.0)
+    fetch = f[kernel] * 1024.0 * fetch_factor
+    write = w[kernel] * 1024.0 * write_factor
+    return {"kernel": kernel, "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+            "hbm_bytes_per_launch": fetch + write, "fetch_kib_raw": f[kernel], "write_kib_raw": w[kernel],
+            "calibration": {"copy_bytes": COPY_BYTES, "copy_fetch_kib": cf[COPY_KERNEL],
+                            "copy_write_kib": cw[COPY_KERNEL], "fetch_factor": fetch_factor,
+                            "write_factor": write_factor}}
+
+
+def short_name(name: str) -> str:
+    n = name.replace("pmh::(anonymous namespace)::", "").replace("pmh::", "").replace("void ", "")
+    return n.split("(")[0].strip()
+
+
+def trace_markdown(db_dir: Path, title: str, command: str) -> str:
+    db = glob.glob(str(db_dir / "*.db"))[0]
+    c = sqlite3.connect(db)
+    shape = {}
+    for r in c.execute("select name, grid_x, workgroup_x, lds_size, vgpr_count, sgpr_count, scratch_size "
+                       "from kernels"):
+        shape.setdefault(r[0], r[1:])
+    lines = [f"# {title}", "", "Command (from the repository root, on one MI355X):", "", "```", command, "```", "",
+             "rocprofv3 kernel-trace statistics (durations in microseconds; grid in work-items):", "",
+             "| kernel | calls | total us | avg us | % | grid | block | LDS B | VGPR | SGPR | scratch |",
+             "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for name, calls, tot, avg, pct in c.execute(
+            "select name, total_calls, total_duration, average, percentage from top_kernels"):
+        g = shape.get(name, (None,) * 6)
+        lines.append(f"| `{short_name(name)}` | {calls} | {tot:.1f} | {avg:.1f} | {pct:.1f} | "
+                     f"{g[0]} | {g[1]} | {g[2]} | {g[3]} | {g[4]} | {g[5]} |")
+    return "\n".join(lines) + "\n"
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("mode", choices=("zipf100m", "copy", "traffic", "trace"))
+    ap.add_argument("args", nargs="*")
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=100_000_000)
+    a = ap.parse_args(argv)
+    if a.mode == "zipf100m":
+        run_zipf100m(a.steps, a.rows)
+    elif a.mode == "copy":
+        run_copy(a.steps)
+    elif a.mode == "traffic":
+        print(json.dumps(summarize(Path(a.args[0]), a.args[1] if len(a.args) > 1 else "hs_scan_fast"), indent=1))
+    else:
+        print(trace_markdown(Path(a.args[0]), a.args[1], a.args[2]), end="")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

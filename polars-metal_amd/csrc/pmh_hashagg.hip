@@ -84,6 +84,8 @@ struct HLaunch {
     uint32_t *tile_hist;   // [kRadix][max_tiles]
     uint32_t *digit_base;  // [kRadix]
     uint64_t *part_start;  // [partitions + 1]: first spill row of each partition after the passes
+    uint32_t *xtot;        // one-pass partitioning: [8][kWideRadix] rows per (XCD slot, digit)
+    uint64_t *xcur;        // [8][kWideRadix]: next free row of each (XCD slot, digit) sub-region
     uint32_t max_tiles, pbits, sortbits, passes;
     // per-partition aggregation
     uint32_t tail_slots, pad1;
@@ -657,6 +659,7 @@ namespace pmh {
 __global__ __launch_bounds__(1024) void hs_regions(const HLaunch *__restrict__ L) {
     __shared__ uint32_t s_a[1024], s_b[1024];
     __shared__ uint32_t carry_a, carry_b;
+    for (uint32_t i = threadIdx.x; i < 8 * kWideRadix; i += blockDim.x) L->xtot[i] = 0;
     if (threadIdx.x == 0) { carry_a = 0; carry_b = 0; }
     __syncthreads();
     for (uint32_t b0 = 0; b0 < L->nb; b0 += 1024) {
@@ -912,7 +915,15 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
 // with LDS counters and a single pass over the scan's tail replaces the
 // stable five-bit passes.  The digit is the partition: the low pbits of the
 // spill hash.
-
+//
+// The grid is a multiple of 8 and block b takes tiles b, b + grid, ..., so
+// all tiles of one "XCD slot" x = b mod 8 run on one XCD (workgroups are
+// dispatched round-robin over the eight XCDs).  Each partition is laid out
+// as eight sub-regions, one per slot, and a tile claims its runs in its
+// slot's sub-region with one atomic per digit: the runs that meet in one
+// sub-region are written through one XCD's L2, which merges their partial
+// lines before they reach HBM.  Placement decides only speed: both passes
+// use the same tile -> block -> slot mapping.
 __global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restrict__ L, const Spill *src_p,
                                                         uint32_t nd) {
     __shared__ uint32_t hist[kWideRadix];
@@ -920,15 +931,16 @@ __global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restri
     const uint32_t ntiles = L->tile_off[L->nb];
     const uint64_t total = L->region_off[L->nb];
     const bool wide = L->wide;
+    const uint32_t tid = threadIdx.x;
+    hist[tid] = 0;
+    __syncthreads();
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        if (threadIdx.x < nd) hist[threadIdx.x] = 0;
-        __syncthreads();
         TileSpan ts;
         tile_span(L, true, t, ntiles, total, ts);
         uint32_t d[kPartRows];
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
-            const uint32_t i = k * kPartThreads + threadIdx.x;
+            const uint32_t i = k * kPartThreads + tid;
             const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
             d[k] = (uint32_t)spill_hash(gld<uint64_t>(src.klo, s), wide ? gld<uint64_t>(src.khi, s) : 0,
                                         gld<uint16_t>(src.flags, s)) &
@@ -936,11 +948,35 @@ __global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restri
         }
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k)
-            if (k * kPartThreads + threadIdx.x < ts.rows) atomicAdd(&hist[d[k]], 1u);
-        __syncthreads();
-        if (threadIdx.x < nd) L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t] = hist[threadIdx.x];
-        __syncthreads();
+            if (k * kPartThreads + tid < ts.rows) atomicAdd(&hist[d[k]], 1u);
     }
+    __syncthreads();
+    const uint32_t x = blockIdx.x & 7;
+    if (tid < nd && hist[tid]) atomicAdd(&L->xtot[x * kWideRadix + tid], hist[tid]);
+}
+
+// One block: the start of every (digit, slot) sub-region, digit-major, as
+// the slots' cursors; the partition starts.
+__global__ __launch_bounds__(kPartThreads) void rp_scan1(const HLaunch *__restrict__ L, uint32_t nd) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t d = threadIdx.x;
+    uint32_t c[8], sum = 0;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        c[x] = d < nd ? L->xtot[x * kWideRadix + d] : 0;
+        sum += c[x];
+    }
+    const uint32_t base = block_excl_scan(sum, s_wave);
+    if (d < nd) {
+        uint64_t o = base;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+            L->xcur[x * kWideRadix + d] = o;
+            o += c[x];
+        }
+        L->part_start[d] = base;
+    }
+    if (d == 0) L->part_start[nd] = L->region_off[L->nb];
 }
 
 __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__restrict__ L, const Spill *src_p,
@@ -948,8 +984,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t run[kWideRadix];     // rows of each digit in this tile
     __shared__ uint32_t dstart[kWideRadix];  // LDS start of each digit run
-    __shared__ uint32_t gbase[kWideRadix];   // global start of each digit run for this tile
-    __shared__ uint32_t dtot[kWideRadix];    // global start of each digit
+    __shared__ uint64_t gbase[kWideRadix];   // global start of each digit run for this tile
     __shared__ uint32_t s_wave[16];
     const Spill &src = *src_p;
     const Spill &dst = *dst_p;
@@ -959,6 +994,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
     const uint32_t tid = threadIdx.x;
     const uint32_t ntiles = L->tile_off[L->nb];
     const uint64_t total = L->region_off[L->nb];
+    uint64_t *cur = L->xcur + (blockIdx.x & 7) * kWideRadix;
     // LDS staging: keys, [hi], flags, digits (16-bit), then value columns
     uint64_t *st_lo = (uint64_t *)smem;
     uint64_t *st_hi = st_lo + kPartTile;
@@ -966,14 +1002,6 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
     uint16_t *st_fl = (uint16_t *)p;
     uint16_t *st_d = st_fl + kPartTile;
     uint8_t *st_v = (uint8_t *)(st_d + kPartTile);
-    {
-        const uint32_t x = tid < nd ? L->digit_base[tid] : 0;
-        const uint32_t ex = block_excl_scan(x, s_wave);
-        dtot[tid] = ex;
-        // the digit is the partition: its start after this pass
-        if (blockIdx.x == 0 && tid < nd) L->part_start[tid] = ex;
-        if (blockIdx.x == 0 && tid == 0) L->part_start[nd] = total;
-    }
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         TileSpan ts;
         tile_span(L, true, t, ntiles, total, ts);
@@ -998,9 +1026,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
         __syncthreads();
         {
             const uint32_t c = run[tid];
-            const uint32_t st = block_excl_scan(c, s_wave);
-            dstart[tid] = st;
-            gbase[tid] = tid < nd ? dtot[tid] + L->tile_hist[(uint64_t)tid * L->max_tiles + t] : 0;
+            dstart[tid] = block_excl_scan(c, s_wave);
+            gbase[tid] = c ? atomicAdd((unsigned long long *)&cur[tid], (unsigned long long)c) : 0;
         }
         __syncthreads();
         // stage in digit order
@@ -1024,7 +1051,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
         // write runs: consecutive staged rows -> consecutive positions
         for (uint32_t sp = tid; sp < ts.rows; sp += kPartThreads) {
             const uint32_t d = st_d[sp];
-            const uint64_t g = (uint64_t)gbase[d] + (sp - dstart[d]);
+            const uint64_t g = gbase[d] + (sp - dstart[d]);
             gst<uint64_t>(dst.klo, g, st_lo[sp]);
             if (wide) gst<uint64_t>(dst.khi, g, st_hi[sp]);
             gst<uint16_t>(dst.flags, g, st_fl[sp]);
@@ -1660,6 +1687,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)std::max(kRadix, kWideRadix) * max_tiles * 4 + 64);
     L.digit_base = (uint32_t *)slab("hs:digit_base", std::max(kRadix, kWideRadix) * 4 + 64);
     L.part_start = (uint64_t *)slab("hs:part_start", ((1ull << 15) + 2) * 8 + 64);
+    L.xtot = (uint32_t *)slab("hs:xtot", 8 * kWideRadix * 4 + 64);
+    L.xcur = (uint64_t *)slab("hs:xcur", 8 * kWideRadix * 8 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
     L.stats = (uint32_t *)slab("hs:stats", ST_N * 4 + 64);
     L.sketch_hist = (uint32_t *)slab("hs:sketch_hist", 1024 * 4 + 64);
@@ -1837,7 +1866,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
     launch_check();
-    const int pgrid = cus * 2;
+    const int pgrid = (cus * 2 + 7) / 8 * 8;  // a multiple of 8: the one-pass XCD slots
     const size_t stage_lds = (size_t)kPartTile * rec_bytes;
     PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_lds));
     const Spill *bufs[3] = {&dL->spill, &dL->ping, &dL->pong};
@@ -1853,7 +1882,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         }
         {
             KernelTimer kt("rp_scan");
-            hipLaunchKernelGGL(rp_scan, dim3(nd), dim3(1024), 0, R.stream, dL, 1);
+            hipLaunchKernelGGL(rp_scan1, dim3(1), dim3(kPartThreads), 0, R.stream, dL, nd);
             launch_check();
         }
         {

@@ -510,6 +510,7 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
     uint8_t *state = (uint8_t *)(lk + nls_max);
     LSink ls{state, lds_off, nls_max};
     GSink gs{L->t.arr, L->fb_slots};
+    const LdsAgg2 A = lds_agg2(q, state, lds_off, nls_max);
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const int nv = q->n_vals;
     constexpr int U = 4;
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
                     }
                 }
                 if (slot >= 0) {
-                    add_row(ls, q, (uint32_t)slot, row);
+                    lds_agg2_add(A, (uint32_t)slot, fl[u], v0[u], v1[u]);
                 } else {
                     const Key k{lane, (uint64_t)qi, 0};
                     if (!L->q_fb[qi]) L->q_fb[qi] = 1;

@@ -1105,7 +1105,7 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     __shared__ uint64_t s_range[2];
     __shared__ uint32_t s_count, s_base;
     const Spill &S = *src_p;
-    const QueryDesc *__restrict__ q = &L->q;
+    const QueryDesc *__restrict__ q = as_const(&L->q);
     const bool wide = L->wide;
     const uint32_t CL = L->tail_slots;
     const uint32_t nls = CL + 2;
@@ -1117,6 +1117,8 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     uint8_t *state = wide ? (uint8_t *)(lst + nls) : (uint8_t *)lh;
     LSink ls{state, L->lds3_off, nls};
     GSink gs{L->t.arr, L->t.mask + 3};
+    const LdsAgg2 A = lds_agg2(q, state, L->lds3_off, nls);
+    const bool agg2 = q->n_vals <= 2;
     for (uint32_t p = blockIdx.x; p <= pmask; p += gridDim.x) {
         if (tid < 2) s_range[tid] = L->part_start[p + tid];
         if (tid == 0) s_count = 0;
@@ -1206,7 +1208,8 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
                     }
                 }
                 if (slot >= 0) {
-                    add_row(ls, q, (uint32_t)slot, row);
+                    if (agg2) lds_agg2_add(A, (uint32_t)slot, fl, v0b[u], v1b[u]);
+                    else add_row(ls, q, (uint32_t)slot, row);
                 } else {
                     const int64_t g = wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
                     if (g >= 0) add_row(gs, q, (uint64_t)g, row);
@@ -1448,9 +1451,9 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
 // Distinct keys among the selected rows: the geometric mean of GEE
 // (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
 // estimator (Haas et al., 1995; it overshoots), floored at the sample's own
-// distinct count.  A sample of almost only singletons (a near-unique
-// selection, or a fingerprint table that still dropped most of it)
-// extrapolates linearly.
+// distinct count.  A sample of singletons with no key seen twice (a
+// near-unique selection), or a fingerprint table that still dropped most of
+// it, extrapolates linearly; a few doubletons already say that keys repeat.
 double gee_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
@@ -1463,7 +1466,7 @@ double shlosser_estimate(const Sketch &k) {
 double plain_distinct_estimate(const Sketch &k) {
     if (k.sample_rows == 0) return 0.0;
     const double s = k.sample_rows, d = k.distinct;
-    if (2.0 * k.dropped > s || k.f1 >= 0.95 * s) return std::max(d, k.n_sel * d / s);
+    if (2.0 * k.dropped > s || (k.f2 == 0 && k.f1 >= 0.95 * s)) return std::max(d, k.n_sel * d / s);
     if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
     const double gee = gee_estimate(k), shl = shlosser_estimate(k);
     return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));

@@ -37,40 +37,6 @@ namespace pmh {
 // row step of 64 new ones are queued at any time.
 constexpr uint32_t kRing = 127;
 
-// Exact f64 accumulation into LDS limbs (limb k of slot s at limb[k * SL + s])
-// with one divergent region only for the rare values outside the fixed-point
-// window (and Inf/NaN), which go to the residual.
-__device__ __forceinline__ void lds_add_f64(unsigned long long *limb, double *res, uint32_t SL, uint32_t s, double x) {
-    const uint64_t b = (uint64_t)__double_as_longlong(x);
-    const uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
-    uint64_t m = b & 0xFFFFFFFFFFFFFull;
-    if (E) m |= 1ull << 52;
-    const int e = E ? (int)E - 1075 : -1074;
-    // in the window: every set bit of m * 2^e in [2^-kFixScale, 2^(32*kLimbs-kFixScale))
-    const int sh0 = -kFixScale - e;  // > 0: low bits that must be zero
-    const bool low_ok = m == 0 || sh0 <= 0 || (sh0 < 64 && (m & ((1ull << sh0) - 1)) == 0);
-    const uint64_t mm = sh0 > 0 ? (sh0 < 64 ? m >> sh0 : 0) : m;
-    const int p = sh0 > 0 ? 0 : e + kFixScale;
-    const int msb = mm ? p + 63 - __builtin_clzll(mm) : 0;
-    const bool in_window = E != 0x7FFu && low_ok && msb < 32 * kLimbs;
-    if (in_window) {
-        if (mm) {
-            const int li = p >> 5, sh = p & 31;
-            const uint64_t lo = mm << sh;
-            const uint64_t hi = sh ? (mm >> (64 - sh)) : 0;
-            int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
-            if (b >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
-            // a chunk above the top limb is zero (msb < 32 * kLimbs): clamp
-            // its limb index and add the zero
-            const int l1 = min(li + 1, kLimbs - 1), l2 = min(li + 2, kLimbs - 1);
-            atomicAdd(limb + (uint32_t)li * SL + s, (unsigned long long)c0);
-            atomicAdd(limb + (uint32_t)l1 * SL + s, (unsigned long long)c1);
-            atomicAdd(limb + (uint32_t)l2 * SL + s, (unsigned long long)c2);
-        }
-    } else {
-        unsafeAtomicAdd(res + s, x);
-    }
-}
 
 template <bool RAW64, int U>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {

@@ -139,6 +139,87 @@ __device__ __forceinline__ void add_row(S &s, const QueryDesc *__restrict__ q, S
     }
 }
 
+// add_row into an LDS table, for queries with at most two value columns,
+// with the value specs resolved once into LDS pointers: per row, no field
+// of the query descriptor is read from memory.  A record's validity bit of
+// value j is bit j of its flags; its value bits are raw (width of the tag).
+struct LdsAgg2 {
+    int nv;
+    uint32_t nslots;  // slots per state array (the limb stride)
+    uint32_t *len_p;
+    int kind[2], need[2], tag[2];
+    uint32_t *cnt_p[2], *nan_p[2];
+    unsigned long long *limb_p[2], *sum_p[2];
+    double *res_p[2];
+    void *min_p[2], *max_p[2];
+};
+
+__device__ __forceinline__ LdsAgg2 lds_agg2(const QueryDesc *__restrict__ q, uint8_t *state, const uint32_t *lds_off,
+                                            uint32_t nslots) {
+    LdsAgg2 A;
+    A.nv = q->n_vals;
+    A.nslots = nslots;
+    auto at = [&](int a) -> uint8_t * { return a >= 0 ? state + lds_off[a] : state; };
+    A.len_p = (uint32_t *)at(q->layout.a_len);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const bool in = j < A.nv;
+        const ValSpec &S = q->vals[in ? j : 0];
+        A.kind[j] = in ? S.kind : VK_I64;
+        A.need[j] = in ? S.need : 0;
+        A.tag[j] = in ? S.tag : PMH_I64;
+        const bool sum_f = (A.need[j] & NEED_SUM) && A.kind[j] == VK_F;
+        const bool sum_i = (A.need[j] & NEED_SUM) && A.kind[j] != VK_F;
+        const bool mm = (A.need[j] & NEED_MINMAX) != 0;
+        A.cnt_p[j] = (uint32_t *)at(in ? S.a_cnt : -1);
+        A.limb_p[j] = (unsigned long long *)at(sum_f ? S.a_limb : -1);
+        A.res_p[j] = (double *)at(sum_f ? S.a_res : -1);
+        A.sum_p[j] = (unsigned long long *)at(sum_i ? S.a_sum : -1);
+        A.min_p[j] = at(mm ? S.a_min : -1);
+        A.max_p[j] = at(mm ? S.a_max : -1);
+        A.nan_p[j] = (uint32_t *)at(mm && A.kind[j] == VK_F ? S.a_nan : -1);
+    }
+    return A;
+}
+
+__device__ __forceinline__ void lds_agg2_add(const LdsAgg2 &A, uint32_t sl, uint32_t flags, uint64_t v0, uint64_t v1) {
+    atomicAdd(A.len_p + sl, 1u);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= A.nv) break;
+        if (!((flags >> j) & 1u)) continue;
+        const uint64_t raw = j ? v1 : v0;
+        atomicAdd(A.cnt_p[j] + sl, 1u);
+        if (A.kind[j] == VK_F) {
+            const double x = raw_to_f64(A.tag[j], raw);
+            if (A.need[j] & NEED_SUM) lds_add_f64(A.limb_p[j], A.res_p[j], A.nslots, sl, x);
+            if (A.need[j] & NEED_MINMAX) {
+                if (x != x) {
+                    atomicOr(A.nan_p[j] + sl, 1u);
+                } else {
+                    const long long kk = (long long)f64_order_key(x);
+                    atomicMin((long long *)A.min_p[j] + sl, kk);
+                    atomicMax((long long *)A.max_p[j] + sl, kk);
+                }
+            }
+        } else if (A.kind[j] == VK_I64) {
+            const int64_t x = raw_to_i64(A.tag[j], raw);
+            if (A.need[j] & NEED_SUM) atomicAdd(A.sum_p[j] + sl, (unsigned long long)x);
+            if (A.need[j] & NEED_MINMAX) {
+                atomicMin((long long *)A.min_p[j] + sl, (long long)x);
+                atomicMax((long long *)A.max_p[j] + sl, (long long)x);
+            }
+        } else {
+            const int32_t x = (int32_t)raw_to_i64(A.tag[j], raw);
+            if (A.need[j] & NEED_SUM) atomicAdd(A.sum_p[j] + sl, (unsigned long long)(int64_t)x);
+            if (A.need[j] & NEED_MINMAX) {
+                atomicMin((int *)A.min_p[j] + sl, x);
+                atomicMax((int *)A.max_p[j] + sl, x);
+            }
+        }
+    }
+}
+
 // Merge LDS slot ls into global slot gs: one atomic per touched array.
 __device__ __forceinline__ void merge_lds_entry(GSink &g, uint64_t gs, LSink &l, uint32_t ls,
                                                 const Layout &lay, const int32_t *ops) {

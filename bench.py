@@ -122,6 +122,27 @@ def sizer_ab(n_jobs: int, seed: int = 777) -> dict:
             "model_held_out_gain": meta.get("report", {}).get("held_out", {}).get("gain")}
 
 
+def host_path(frame: dict, plan: dict, n: int, reps: int = 2) -> dict:
+    """The drop-in path through host buffers: execute_filter_groupby with
+    Arrow bytes in host memory, so the input crosses PCIe inside the call and
+    the outputs come back as bytes."""
+    import numpy as np
+    from polars_metal import _native
+    cols = []
+    for name, (tag, values, valid) in frame.items():
+        vb = np.packbits(np.asarray(valid, bool), bitorder="little").tobytes() if valid is not None else None
+        cols.append((name, tag, np.ascontiguousarray(values).tobytes(), vb))
+    _native.execute_filter_groupby(plan, n, cols)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = _native.execute_filter_groupby(plan, n, cols)
+    dt = (time.perf_counter() - t0) / reps
+    in_bytes = sum(len(c[2]) + (len(c[3]) if c[3] else 0) for c in cols)
+    return {"ms_per_query": dt * 1e3, "rows_per_s": n / dt, "input_bytes": in_bytes,
+            "groups": len(out[0][2]) // 8,
+            "note": "host (pageable) Arrow buffers in, host bytes out; PCIe copies inside the timed call"}
+
+
 def group_checksum(res, names) -> tuple:
     """Order-independent checksum of a device result's groups: the wrapping
     sum over groups of a mix of the named columns' bits and validity."""
@@ -261,6 +282,7 @@ def main() -> int:
     ap.add_argument("--sizer-jobs", type=int, default=120)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sizer", action="store_true")
+    ap.add_argument("--no-host-path", action="store_true")
     ap.add_argument("--options", type=str, default="", help="JSON plan options (launch shape overrides)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -281,7 +303,6 @@ def main() -> int:
     print(f"bench: generating {n} rows (numpy, seed 2) and uploading", file=sys.stderr, flush=True)
     frame, plan = synth.zipf(n, seed=2)
     cols, bufs = device_columns(frame)
-    del frame
     if args.options:
         plan["options"] = json.loads(args.options)
 
@@ -336,6 +357,10 @@ def main() -> int:
         "kernel_ms": {k: v[1] / v[0] for k, v in kt.items() if v[0]},
     }
     del bufs, cols
+    if not args.no_host_path:
+        log("host-buffer (PCIe-inclusive) path")
+        out["pcie_inclusive"] = host_path(frame, plan, n)
+    del frame
     if not args.no_sizer:
         log(f"learned sizer A/B on {args.sizer_jobs} fresh jobs")
         out["learned_vs_plain"] = sizer_ab(args.sizer_jobs)

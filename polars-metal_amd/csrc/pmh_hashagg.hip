@@ -8,22 +8,35 @@
 //   hs_sample_count   one block counts the sample in LDS and lays the most
 //                     frequent keys (count >= 2) out in a hot table shared by
 //                     every block; it also records the sample sketch
-//   hs_scan           one block per CU streams a contiguous chunk of rows with
-//                     the register row cache: predicate, key, hot-table probe
-//                     (read only, no CAS).  Hot rows fold into LDS state; the
-//                     other rows ("tail") are appended, wave-contiguously, to
-//                     the block's spill region.  At the end the block adds its
-//                     hot partials into the global hot state with coalesced
-//                     atomics (slot s of every block maps to global slot s).
-//   rp_*              stable LSD radix passes on the key hash (5 bits each)
-//                     group the tail by partition: histogram per tile, one
-//                     scan per digit, then a scatter that ranks rows with wave
-//                     multisplit ballots, stages the tile in LDS in digit order
-//                     and writes each digit run contiguously.
+//   hs_scan_fast      one block per CU streams a contiguous chunk of rows:
+//                     predicates lowered to at most two intervals on an
+//                     order-preserving int64 image, columns read as aligned
+//                     8-byte words, selected rows compacted through a per-wave
+//                     LDS ring so probes and folds run on dense waves,
+//                     hot-table probe (read only, no CAS).  Hot rows fold into
+//                     LDS state; the other rows ("tail") are appended,
+//                     wave-contiguously, to the block's spill region.  At the
+//                     end the block adds its hot partials into the global hot
+//                     state with coalesced atomics (slot s of every block maps
+//                     to global slot s).  hs_scan is the general form (any
+//                     predicate tree, a register row cache).
+//   rp_hist1/rp_scan1/rp_scatter1
+//                     up to 1024 partitions: one pass groups the tail by
+//                     partition (the low bits of the key hash); tiles rank
+//                     rows with LDS counters, and each partition is laid out
+//                     as eight per-XCD sub-regions so the runs that meet in a
+//                     sub-region merge in one XCD's L2.
+//   rp_hist/rp_scan/rp_scatter
+//                     more partitions: stable LSD radix passes on the key hash
+//                     (5 bits each): histogram per tile, one scan per digit,
+//                     then a scatter that ranks rows with wave multisplit
+//                     ballots, stages the tile in LDS in digit order and
+//                     writes each digit run contiguously; hs_part_bounds then
+//                     finds the partition starts.
 //   hs_part_agg       one block per partition folds the partition's tail rows
 //                     into an LDS table and writes its groups straight to the
 //                     outputs; a row that finds no LDS slot goes to the global
-//                     fallback table (finalised by fgb_finalize_into).
+//                     fallback table (written by hs_fallback_emit).
 //   hs_hot_emit       writes the hot groups.
 //   hs_pack           turns the per-row validity bytes into Arrow bitmaps.
 #include <algorithm>

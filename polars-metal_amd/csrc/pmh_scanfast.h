@@ -150,7 +150,14 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
             for (int u = 0; u < U; ++u) {
                 const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
                 pw[r][u] = word_at(pc[r].base, vpos(pc[r], row));
-                pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row));
+            }
+            // columns without a validity bitmap load no validity words
+            if (pc[r].vstride) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
+                    pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row));
+                }
             }
         }
         if (touch) {
@@ -176,12 +183,19 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
             for (int r = 0; r < 2; ++r) {
                 if (r >= nr) continue;
                 uint64_t raw[U];
-                uint32_t vb = 0;
+                uint32_t vb = (1u << U) - 1;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint64_t row = min(r0 + (uint64_t)u * bs, r_last);
                     raw[u] = cut(pw[r][u], vpos(pc[r], row), pc[r].mask);
-                    vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], row), 1) << u;
+                }
+                if (pc[r].vstride) {
+                    vb = 0;
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint64_t row = min(r0 + (uint64_t)u * bs, r_last);
+                        vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], row), 1) << u;
+                    }
                 }
                 int64_t img[U];
                 pred_images<U>(pc[r].tag, pflt[r] != 0, raw, img);
@@ -226,25 +240,28 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 qhead = qhead + cnt >= kRing ? qhead + cnt - kRing : qhead + cnt;
                 qn -= cnt;
                 // gathers: every word first, then the cuts
+                // (columns without a validity bitmap load no validity word)
                 uint64_t kw[4], kv[4], vw_[2], vv_[2];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    kw[i] = kv[i] = 0;
+                    kw[i] = 0;
+                    kv[i] = ~0ull;
                     if (i >= nk) continue;
                     kw[i] = word_at(kc[i].base, vpos(kc[i], row));
-                    kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
+                    if (kc[i].vstride) kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
                 }
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
-                    vw_[j] = vv_[j] = 0;
+                    vw_[j] = 0;
+                    vv_[j] = ~0ull;
                     if (j >= nv) continue;
                     vw_[j] = word_at(vc[j].base, vpos(vc[j], row));
-                    vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+                    if (vc[j].vstride) vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
                 }
                 uint64_t key = 0;
                 int special = 0;
                 if (RAW64) {
-                    if (!cut(kv[0], bpos(kc[0], row), 1)) special = 2;
+                    if (kc[0].vstride && !cut(kv[0], bpos(kc[0], row), 1)) special = 2;
                     else {
                         key = kw[0];
                         special = key == kEmpty ? 1 : 0;
@@ -253,8 +270,8 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         if (i >= nk) continue;
-                        key |= cut(kv[i], bpos(kc[i], row), 1) ? cut(kw[i], vpos(kc[i], row), kc[i].mask) << kdata[i]
-                                                                : 1ull << knull[i];
+                        const bool ok = !kc[i].vstride || cut(kv[i], bpos(kc[i], row), 1);
+                        key |= ok ? cut(kw[i], vpos(kc[i], row), kc[i].mask) << kdata[i] : 1ull << knull[i];
                     }
                     special = (allones && key == kEmpty) ? 1 : 0;
                 }
@@ -264,7 +281,8 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 for (int j = 0; j < 2; ++j) {
                     if (j >= nv) continue;
                     vr[j] = cut(vw_[j], vpos(vc[j], row), vc[j].mask);
-                    vv |= (uint32_t)cut(vv_[j], bpos(vc[j], row), 1) << j;
+                    const bool ok = !vc[j].vstride || cut(vv_[j], bpos(vc[j], row), 1);
+                    vv |= (uint32_t)ok << j;
                 }
                 // hot-table probe
                 // the all-ones and null keys fold into the hot table's side

@@ -1706,8 +1706,9 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.xtot = (uint32_t *)slab("hs:xtot", 8 * kWideRadix * 4 + 64);
     L.xcur = (uint64_t *)slab("hs:xcur", 8 * kWideRadix * 8 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
-    L.stats = (uint32_t *)slab("hs:stats", ST_N * 4 + 64);
-    L.sketch_hist = (uint32_t *)slab("hs:sketch_hist", 1024 * 4 + 64);
+    // stats and the sketch histogram are adjacent: one read-back
+    L.stats = (uint32_t *)slab("hs:stats", (ST_N + 1024) * 4 + 64);
+    L.sketch_hist = L.stats + ST_N;
     // fallback table
     {
         GTable &t = L.t;
@@ -1747,7 +1748,17 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
 
     // ---- launches
     HLaunch *dL = (HLaunch *)slab("hs:launch", sizeof(HLaunch));
-    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
+    // Control traffic goes through pinned host memory: the descriptor
+    // uploads from a staged copy, and read-backs land in a mailbox, so each
+    // copy is a plain DMA on the stream instead of a staged, waited-for copy
+    // of pageable memory.  The query ends synchronised, so a later query
+    // never overwrites a staged copy still being read.
+    const size_t up_bytes = (sizeof(HLaunch) + 255) & ~size_t(255);
+    uint8_t *pin = (uint8_t *)pinned_host(up_bytes + (ST_N + 1024 + 64) * 4);
+    HLaunch *up = (HLaunch *)pin;
+    uint32_t *mbox = (uint32_t *)(pin + up_bytes);
+    memcpy(up, &L, sizeof(HLaunch));
+    PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     hipEvent_t ev0, ev1;
     PMH_HIP(hipEventCreate(&ev0));
     PMH_HIP(hipEventCreate(&ev1));
@@ -1779,9 +1790,10 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // ---- partition count from the sketch (one small read-back)
     uint32_t sk[ST_N];
     std::vector<uint32_t> hist(1024);
-    PMH_HIP(hipMemcpyAsync(sk, L.stats, sizeof(sk), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipMemcpyAsync(hist.data(), L.sketch_hist, 1024 * 4, hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(mbox, L.stats, (ST_N + 1024) * 4, hipMemcpyDeviceToHost, R.stream));
     PMH_HIP(hipStreamSynchronize(R.stream));
+    memcpy(sk, mbox, sizeof(sk));
+    memcpy(hist.data(), mbox + ST_N, 1024 * 4);
     const double tail_cap = 0.5 * TS;  // target keys per partition
     Sketch sketch = sketch_from(sk[ST_SAMPLE_SEL], n_sample, n, sk[ST_SAMPLE_DISTINCT], sk[ST_SAMPLE_F1],
                                 sk[ST_SAMPLE_F2], sk[ST_SAMPLE_DROPPED], hist);
@@ -1858,7 +1870,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     const size_t lds3b = (size_t)(L.tail_slots + 2) * (key_bytes + (wide ? 4 : 0)) +
                          lds_layout(P, L.tail_slots + 2, L.lds3_off);
-    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
+    memcpy(up, &L, sizeof(HLaunch));
+    PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     {
         const bool cached = q0.n_cols <= 4;
         const size_t l1 = std::max<size_t>(lds1, 16);
@@ -1960,13 +1973,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         launch_check();
     }
     uint32_t host_ctrl[8], host_stats[ST_N], cursor = 0;
-    PMH_HIP(hipMemcpyAsync(host_ctrl, L.t.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipMemcpyAsync(host_stats, L.stats, sizeof(host_stats), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipMemcpyAsync(&cursor, L.out_cursor, 4, hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(mbox, L.t.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(mbox + 8, L.stats, sizeof(host_stats), hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipMemcpyAsync(mbox + 8 + ST_N, L.out_cursor, 4, hipMemcpyDeviceToHost, R.stream));
     hipLaunchKernelGGL(hs_clear_ctrl, dim3(1), dim3(64), 0, R.stream, dL);
     launch_check();
     PMH_HIP(hipEventRecord(ev1, R.stream));
     PMH_HIP(hipEventSynchronize(ev1));
+    memcpy(host_ctrl, mbox, sizeof(host_ctrl));
+    memcpy(host_stats, mbox + 8, sizeof(host_stats));
+    cursor = mbox[8 + ST_N];
     float dev_ms = 0.f;
     (void)hipEventElapsedTime(&dev_ms, ev0, ev1);
     (void)hipEventDestroy(ev0);

@@ -153,6 +153,7 @@ ENGINE_OPTS = [
     {"method": "hash_global", "block": 64, "grid": 1},
     {"method": "hash"}, {"method": "hash", "lds_slots": 0}, {"method": "hash", "sample": 0},
     {"method": "hash", "partitions": 1}, {"method": "hash", "partitions": 32768},
+    {"method": "hash", "partitions": 1024}, {"method": "hash", "partitions": 2048},
     {"method": "hash", "partitions": 2, "tail_slots": 16},
     {"method": "hash", "lds_slots": 16, "sample": 1000},
     {"method": "hash", "scan_rows": 8}, {"method": "hash", "scan_rows": 8, "lds_slots": 0},
@@ -166,6 +167,19 @@ def test_launch_shapes_agree(native, opts):
     frame, plan = synth.zipf(200_000, seed=4)
     plan = dict(plan, options=opts)
     assert_results_equal(run_gpu(native, plan, frame), orc.groupby(plan, frame), ["key"], label=str(opts))
+
+
+@pytest.mark.parametrize("parts", [1024, 256, 2048])
+def test_partition_layouts_with_many_tail_tiles(native, parts):
+    """Enough tail rows for hundreds of partition tiles, so every XCD slot of
+    the one-pass layout (up to 1024 partitions) and the stable multi-pass
+    layout (more) carry many runs per partition."""
+    frame, plan = synth.zipf(3_000_000, seed=8)
+    plan = dict(plan, options={"method": "hash", "partitions": parts})
+    got = run_gpu(native, plan, frame)
+    st = native.last_stats()
+    assert st["partitions"] == parts and st["tail_rows"] > 200_000
+    assert_results_equal(got, orc.groupby(plan, frame), ["key"], label=f"parts{parts}")
 
 
 KEY_TAGS = ["I64", "I32", "I16", "I8", "U8", "U16", "U32", "U64", "F64", "F32", "Bool"]

@@ -27,12 +27,13 @@
 //                     as eight per-XCD sub-regions so the runs that meet in a
 //                     sub-region merge in one XCD's L2.
 //   rp_hist/rp_scan/rp_scatter
-//                     more partitions: stable LSD radix passes on the key hash
-//                     (5 bits each): histogram per tile, one scan per digit,
-//                     then a scatter that ranks rows with wave multisplit
-//                     ballots, stages the tile in LDS in digit order and
-//                     writes each digit run contiguously; hs_part_bounds then
-//                     finds the partition starts.
+//                     more partitions: the unstable pass takes the low bits
+//                     and stable LSD passes (7 bits each) the rest: histogram
+//                     per tile, one scan per digit, then a scatter that ranks
+//                     rows with wave multisplit ballots, stages the tile in
+//                     LDS in digit order and writes each digit run
+//                     contiguously; hs_part_bounds then finds the partition
+//                     starts.
 //   hs_part_agg       one block per partition folds the partition's tail rows
 //                     into an LDS table and writes its groups straight to the
 //                     outputs; a row that finds no LDS slot goes to the global
@@ -48,7 +49,7 @@
 
 namespace pmh {
 
-constexpr int kRadixBits = 5;
+constexpr int kRadixBits = 7;  // stable passes
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kPartThreads = 1024;
 constexpr int kPartRows = 4;               // rows per thread per partition tile
@@ -812,6 +813,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     __shared__ uint32_t dstart[kRadix];    // LDS start of each digit run
     __shared__ uint32_t gbase[kRadix];     // global start of each digit run for this tile
     __shared__ uint32_t dtot[kRadix];
+    __shared__ uint32_t s_wave[16];
     const Spill &src = *src_p;
     const Spill &dst = *dst_p;
     const QueryDesc *__restrict__ q = &L->q;
@@ -829,11 +831,10 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     uint8_t *st_d = (uint8_t *)(st_fl + kPartTile);
     uint8_t *st_v = st_d + kPartTile;
     // digit totals over all tiles -> exclusive digit bases
-    if (threadIdx.x < kRadix) dtot[threadIdx.x] = L->digit_base[threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int d = 0; d < kRadix; ++d) { uint32_t x = dtot[d]; dtot[d] = acc; acc += x; }
+    {
+        const uint32_t x = threadIdx.x < kRadix ? L->digit_base[threadIdx.x] : 0;
+        const uint32_t ex = block_excl_scan(x, s_wave);
+        if (threadIdx.x < kRadix) dtot[threadIdx.x] = ex;
     }
     __syncthreads();
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -859,7 +860,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
                 peers &= ((d >> b) & 1) ? m : ~m;
             }
             const uint32_t rank = (uint32_t)__popcll(peers & lt);
-            if (threadIdx.x < (kPartThreads / 64) * kRadix) wcnt[threadIdx.x / kRadix][threadIdx.x % kRadix] = 0;
+            for (uint32_t w = threadIdx.x; w < (kPartThreads / 64) * kRadix; w += kPartThreads)
+                wcnt[w / kRadix][w % kRadix] = 0;
             __syncthreads();
             if (live && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
             __syncthreads();
@@ -880,12 +882,14 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
             __syncthreads();
         }
         // LDS start of each digit run within the tile
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int d = 0; d < kRadix; ++d) { dstart[d] = acc; acc += run[d]; }
+        {
+            const uint32_t c = threadIdx.x < kRadix ? run[threadIdx.x] : 0;
+            const uint32_t st = block_excl_scan(c, s_wave);
+            if (threadIdx.x < kRadix) {
+                dstart[threadIdx.x] = st;
+                gbase[threadIdx.x] = dtot[threadIdx.x] + L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t];
+            }
         }
-        if (threadIdx.x < kRadix)
-            gbase[threadIdx.x] = dtot[threadIdx.x] + L->tile_hist[(uint64_t)threadIdx.x * L->max_tiles + t];
         __syncthreads();
         // stage in digit order
         for (int k = 0; k < kPartRows; ++k) {
@@ -1464,7 +1468,7 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
 // Distinct keys among the selected rows: the geometric mean of GEE
 // (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
 // estimator (Haas et al., 1995; it overshoots), floored at the sample's own
-// distinct count.  A sample of singletons with no key seen twice (a
+// distinct count and at Chao's lower bound (Chao, 1984).  A sample of singletons with no key seen twice (a
 // near-unique selection), or a fingerprint table that still dropped most of
 // it, extrapolates linearly; a few doubletons already say that keys repeat.
 double gee_estimate(const Sketch &k) {
@@ -1482,7 +1486,11 @@ double plain_distinct_estimate(const Sketch &k) {
     if (2.0 * k.dropped > s || (k.f2 == 0 && k.f1 >= 0.95 * s)) return std::max(d, k.n_sel * d / s);
     if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
     const double gee = gee_estimate(k), shl = shlosser_estimate(k);
-    return std::max(d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)));
+    // Chao's lower bound (bias-corrected Chao1) lifts the mean where keys
+    // repeat a few times each across a selection the sample barely covers
+    // (there GEE undershoots far more than Shlosser overshoots)
+    const double chao = std::min(k.n_sel, d + (double)k.f1 * (k.f1 > 0 ? k.f1 - 1.0 : 0.0) / (2.0 * (k.f2 + 1.0)));
+    return std::max({d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)), chao});
 }
 
 // Features of the learned sizer: scale-free quantities of the sketch, each
@@ -1634,7 +1642,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         rec_bytes += L.vw[j];
     }
     // shapes that do not fit the LDS plans go to the global-table engine
-    if ((size_t)kPartTile * rec_bytes > lds_budget) return nullptr;
+    // the stable scatter stages a tile and keeps per-wave digit counts
+    if ((size_t)kPartTile * rec_bytes + (size_t)kRadix * 4 * (kPartThreads / 64 + 4) + 64 > lds_budget) return nullptr;
     size_t state_per_slot = 0;
     for (int a = 0; a < q0.layout.n_arrs; ++a) state_per_slot += q0.layout.esz[a];
     const size_t key_bytes = wide ? 16 : 8;
@@ -1854,10 +1863,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     const int pbits = log2u(P_parts);
     // up to 1024 partitions: one unstable pass on the partition digit, when
     // its LDS staging (16-bit digits) fits
+    // up to 1024 partitions: one unstable pass on the partition digit; more:
+    // an unstable pass on the low pbits - 7 bits, then one stable 7-bit pass
+    // (LSD order needs stability from the second pass on), when the unstable
+    // pass's LDS staging (16-bit digits) fits
     const size_t stage_lds1 = (size_t)kPartTile * (rec_bytes + 1);
-    const bool one_pass = pbits <= kWideRadixBits && stage_lds1 + 5 * kWideRadix * 4 <= lds_budget;
-    const int passes = one_pass ? 1 : std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
-    const int sortbits = one_pass ? pbits : passes * kRadixBits;
+    const bool wide_fits = stage_lds1 + 5 * kWideRadix * 4 <= lds_budget;
+    const bool one_pass = pbits <= kWideRadixBits && wide_fits;
+    const bool two_pass = !one_pass && wide_fits && pbits - kRadixBits <= kWideRadixBits;
+    const int passes = one_pass ? 1 : two_pass ? 2 : std::max(1, (pbits + kRadixBits - 1) / kRadixBits);
+    const int sortbits = (one_pass || two_pass) ? pbits : passes * kRadixBits;
     L.pbits = (uint32_t)pbits;
     L.sortbits = (uint32_t)sortbits;
     L.passes = (uint32_t)passes;
@@ -1900,8 +1915,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_lds));
     const Spill *bufs[3] = {&dL->spill, &dL->ping, &dL->pong};
     int src = 0, dst = 1;
-    if (one_pass) {
-        const uint32_t nd = 1u << pbits;
+    if (one_pass || two_pass) {
+        const uint32_t nd = 1u << (one_pass ? pbits : pbits - kRadixBits);
         PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter1, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)stage_lds1));
         {
@@ -1921,8 +1936,29 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             launch_check();
         }
         src = 1;
+        dst = 2;
     }
-    for (int ps = 0; ps < passes && !one_pass; ++ps) {
+    if (two_pass) {
+        const int shift = pbits - kRadixBits;
+        {
+            KernelTimer kt("rp_hist");
+            hipLaunchKernelGGL(rp_hist, dim3(pgrid), dim3(kPartThreads), 0, R.stream, dL, 0, bufs[src], shift);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scan");
+            hipLaunchKernelGGL(rp_scan, dim3(kRadix), dim3(1024), 0, R.stream, dL, 0);
+            launch_check();
+        }
+        {
+            KernelTimer kt("rp_scatter");
+            hipLaunchKernelGGL(rp_scatter, dim3(pgrid), dim3(kPartThreads), stage_lds, R.stream, dL, 0, bufs[src],
+                               bufs[dst], shift);
+            launch_check();
+        }
+        src = dst;
+    }
+    for (int ps = 0; ps < passes && !one_pass && !two_pass; ++ps) {
         const int shift = ps * kRadixBits;
         {
             KernelTimer kt("rp_hist");

@@ -229,22 +229,41 @@ def execute_filter_groupby(plan_dict: dict, n_rows: int, columns: Sequence[tuple
 
 
 class DeviceResult:
-    """Outputs of execute_filter_groupby_device.  Columns are device
-    pointers owned by the library: valid until the next call or free()."""
+    """Outputs of execute_filter_groupby_device (or of one batch query).
+    Columns are pointers owned by the library, valid until the next call or
+    free(); ``columns`` and ``stats`` are read from the result on first use."""
+
+    __slots__ = ("_p", "_columns", "_stats")
 
     def __init__(self, res_p):
         self._p = res_p
-        r = res_p.contents
-        self.n_rows = r.n_rows
-        self.stats = _stats_dict(r.stats)
-        self.columns = []
-        for i in range(r.n_columns):
-            c = r.columns[i]
-            self.columns.append({
-                "name": c.name.decode(), "tag": TAG_NAMES[c.tag], "data": c.data or 0,
-                "data_len": c.data_len, "valid": c.valid or 0, "valid_len": c.valid_len,
-                "location": c.location,
-            })
+        self._columns = None
+        self._stats = None
+
+    @property
+    def n_rows(self) -> int:
+        return self._p.contents.n_rows
+
+    @property
+    def stats(self) -> dict:
+        if self._stats is None:
+            self._stats = _stats_dict(self._p.contents.stats)
+        return self._stats
+
+    @property
+    def columns(self) -> list:
+        if self._columns is None:
+            r = self._p.contents
+            cols = []
+            for i in range(r.n_columns):
+                c = r.columns[i]
+                cols.append({
+                    "name": c.name.decode(), "tag": TAG_NAMES[c.tag], "data": c.data or 0,
+                    "data_len": c.data_len, "valid": c.valid or 0, "valid_len": c.valid_len,
+                    "location": c.location,
+                })
+            self._columns = cols
+        return self._columns
 
     def fetch(self, name: str):
         """Copy one output column to host: ``(tag, data bytes, valid bytes

@@ -19,8 +19,18 @@ import math
 import time
 from typing import Iterable, Sequence
 
+import numpy as np
+
 from . import _lib
-from ._native import DeviceResult, _device_column_array, _plan_json, execute_filter_groupby_device
+from ._native import PMH_DEVICE, DeviceResult, _plan_json, _tag, execute_filter_groupby_device
+
+# numpy images of PmhColumn and PmhBatchQuery, so a batch's descriptor
+# tables are filled column-wise instead of field by field through ctypes
+_COL_DT = np.dtype([("name", "u8"), ("tag", "i4"), ("location", "i4"), ("data", "u8"), ("data_len", "u8"),
+                    ("valid", "u8"), ("valid_len", "u8")])
+_QRY_DT = np.dtype([("n_rows", "u8"), ("columns", "u8"), ("n_columns", "u4"), ("pad", "u4"), ("predicate", "u8")])
+assert _COL_DT.itemsize == ctypes.sizeof(_lib.PmhColumn)
+assert _QRY_DT.itemsize == ctypes.sizeof(_lib.PmhBatchQuery)
 
 CANDIDATES = [{"tile": 4096, "tail_slots": 512}, {"tile": 8192, "tail_slots": 512},
               {"tile": 4096, "tail_slots": 1024}, {"tile": 8192, "tail_slots": 1024},
@@ -43,24 +53,57 @@ def execute_batch(template_plan: dict, queries: Sequence[tuple], options: dict |
     if options:
         plan["options"] = dict(plan.get("options", {}), **options)
     n = len(queries)
-    arr = (_lib.PmhBatchQuery * max(1, n))()
     keep = []
-    for i, (n_rows, cols, pred) in enumerate(queries):
-        carr, k = _device_column_array(list(cols))
-        keep.append((carr, k))
-        arr[i].n_rows = n_rows
-        arr[i].columns = carr
-        arr[i].n_columns = len(cols)
-        if pred is not None:
-            pj = json.dumps(pred).encode()
-            keep.append(pj)
-            arr[i].predicate_json = pj
+    strings: dict = {}  # text -> address of a NUL-terminated copy kept alive in `keep`
+
+    def cstr(b: bytes) -> int:
+        a = strings.get(b)
+        if a is None:
+            buf = ctypes.create_string_buffer(b)
+            keep.append(buf)
+            a = strings[b] = ctypes.addressof(buf)
+        return a
+
+    preds: dict = {}  # id(predicate dict) -> address of its JSON text
+    tags: dict = {}
+    rows, counts, pred_addr = [], [], []
+    names, ctag, data, dlen, valid, vlen = [], [], [], [], [], []
+    for n_rows, cols, pred in queries:
+        rows.append(n_rows)
+        counts.append(len(cols))
+        if pred is None:
+            pred_addr.append(0)
         else:
-            arr[i].predicate_json = None
+            a = preds.get(id(pred))
+            if a is None:
+                keep.append(pred)  # keeps id(pred) unique while cached
+                a = preds[id(pred)] = cstr(json.dumps(pred).encode())
+            pred_addr.append(a)
+        for name, tag, dp, dl, vp, vl in cols:
+            names.append(cstr(name.encode()))
+            t = tags.get(tag)
+            if t is None:
+                t = tags[tag] = _tag(tag)
+            ctag.append(t)
+            data.append(dp)
+            dlen.append(dl)
+            valid.append(vp or 0)
+            vlen.append(vl if vp else 0)
+    ca = np.zeros(max(1, len(names)), _COL_DT)
+    if names:
+        ca["name"], ca["tag"], ca["location"] = names, ctag, PMH_DEVICE
+        ca["data"], ca["data_len"], ca["valid"], ca["valid_len"] = data, dlen, valid, vlen
+    qa = np.zeros(max(1, n), _QRY_DT)
+    if n:
+        offs = np.zeros(n, np.uint64)
+        np.cumsum(counts[:-1], out=offs[1:])
+        qa["n_rows"], qa["n_columns"], qa["predicate"] = rows, counts, pred_addr
+        qa["columns"] = np.uint64(ca.ctypes.data) + offs * np.uint64(_COL_DT.itemsize)
     res = (ctypes.POINTER(_lib.PmhResult) * max(1, n))()
     st = _lib.PmhBatchStats()
-    _lib.check(lib.pmh_execute_batch(_plan_json(plan), arr, n, _lib.PMH_OUT_DEVICE, res, ctypes.byref(st)))
-    del keep
+    _lib.check(lib.pmh_execute_batch(_plan_json(plan), qa.ctypes.data_as(ctypes.POINTER(_lib.PmhBatchQuery)), n,
+                                     _lib.PMH_OUT_DEVICE, res, ctypes.byref(st)))
+    del keep, ca, qa
     return [DeviceResult(res[i]) for i in range(n)], _stats_dict(st)
 
 

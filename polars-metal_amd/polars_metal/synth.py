@@ -114,16 +114,19 @@ def batch_specs(n_queries: int = 4096, seed: int = 5, lo: float = 1e3, hi: float
             for i, (r, f) in enumerate(zip(rows, filt))]
 
 
+_BATCH_AGGS = [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": f"v_{op.lower()}"}
+               for op in ("Sum", "Count", "Min", "Max", "Mean")]
+_BATCH_PRED = pred_gt_f64("v", 0.5)  # one object shared by the filtered queries
+
+
 def batch_query(spec: dict) -> tuple[dict, dict]:
     rng = np.random.default_rng(spec["seed"])
     r = spec["rows"]
     frame = {"key": ("I64", rng.integers(1, spec["distinct"] + 1, r, dtype=np.int64), None),
              "v": ("F64", rng.random(r), None)}
-    aggs = [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": f"v_{op.lower()}"}
-            for op in ("Sum", "Count", "Min", "Max", "Mean")]
-    plan = {"keys": [["key", "I64"]], "aggs": aggs}
+    plan = {"keys": [["key", "I64"]], "aggs": _BATCH_AGGS}
     if spec["filter"]:
-        plan["predicate"] = pred_gt_f64("v", 0.5)
+        plan["predicate"] = _BATCH_PRED
     return frame, plan
 
 

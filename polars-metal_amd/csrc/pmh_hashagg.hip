@@ -250,15 +250,38 @@ __global__ __launch_bounds__(256) void hs_sample_gather(const HLaunch *__restric
 // most frequent fingerprints (up to hot_max), recover their keys from the
 // sample, lay them out in the hot table, and record the sketch (sampled rows,
 // distinct fingerprints, f1, f2).  It also resets the global hot state.
-// The hot table (CL slots, a power of two, CL >= 16): slots [0, CL - 2)
-// hold the hashed hot keys; slot CL - 2 is the group of the all-ones key
-// and slot CL - 1 the group of the null key of a single 64-bit key column,
-// the two keys the empty marker cannot stand for.  Probes never visit them.
-__device__ __forceinline__ uint32_t hot_home(uint64_t h, uint32_t CL) {
-    const uint32_t s = (uint32_t)(h >> 40) & (CL - 1);
-    return s >= CL - 2 ? s - 2 : s;
+// The hot table (CL slots, a power of two, CL >= 16) is read only during the
+// scan, so it is laid out once for cheap, bounded lookups: buckets of four
+// slots, the first CL/4 - 1 of them for hashed hot keys.  A key goes to the
+// first free slot of its first bucket, or of its second when the first was
+// full; a key that finds both full stays out of the hot set (its rows take
+// the tail path).  Buckets only fill, so a lookup that sees a free slot in
+// the key's first bucket stops there.  Slot CL - 2 is the group of the
+// all-ones key and slot CL - 1 the group of the null key of a single 64-bit
+// key column, the two keys the empty marker cannot stand for.
+__device__ __forceinline__ uint32_t hot_buckets(uint32_t CL) { return CL / 4 - 1; }
+__device__ __forceinline__ uint32_t hot_b1(uint64_t h, uint32_t nb) { return (uint32_t)(((h >> 32) * nb) >> 32); }
+__device__ __forceinline__ uint32_t hot_b2(uint64_t h, uint32_t nb) {
+    return (uint32_t)(((h & 0xFFFFFFFFull) * nb) >> 32);
 }
-__device__ __forceinline__ uint32_t hot_next(uint32_t s, uint32_t CL) { return s + 1 == CL - 2 ? 0 : s + 1; }
+// Slot of a (non-special) 64-bit key in an LDS copy of the hot keys, or -1.
+__device__ __forceinline__ int32_t hot_find(const uint64_t *lk, uint64_t key, uint64_t h, uint32_t CL) {
+    const uint32_t nb = hot_buckets(CL);
+    const uint32_t b1 = hot_b1(h, nb) * 4;
+    bool open = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t c = lk[b1 + j];
+        if (c == key) return (int32_t)(b1 + j);
+        open |= c == kEmpty;
+    }
+    if (open) return -1;
+    const uint32_t b2 = hot_b2(h, nb) * 4;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        if (lk[b2 + j] == key) return (int32_t)(b2 + j);
+    return -1;
+}
 __device__ __forceinline__ uint32_t hot_side_slot(int special, uint32_t CL) { return CL - 3 + (uint32_t)special; }
 
 // Probe limit of the sample fingerprint table: a key that finds no slot in
@@ -398,17 +421,21 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
                 if (cur == 0) break;
                 if (cur == fp) {
                     if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
-                        hot_rows += s_cnt[s] & 0x3FFFFFFFu;
-                        uint32_t t = hot_home(h, CL);
-                        for (uint32_t pp = 0; pp < CL - 2; ++pp) {
-                            unsigned long long old = atomicCAS((unsigned long long *)&h_lo[t], (unsigned long long)kEmpty,
-                                                               (unsigned long long)lo);
-                            if (old == kEmpty) {
-                                if (wide) h_hi[t] = hi;
-                                atomicAdd(&s_hot, 1u);
-                                break;
+                        const uint32_t nbk = hot_buckets(CL);
+                        bool placed = false;
+                        for (int pass = 0; pass < 2 && !placed; ++pass) {
+                            const uint32_t b = (pass ? hot_b2(h, nbk) : hot_b1(h, nbk)) * 4;
+                            for (int jj = 0; jj < 4; ++jj) {
+                                const unsigned long long old = atomicCAS(
+                                    (unsigned long long *)&h_lo[b + jj], (unsigned long long)kEmpty, (unsigned long long)lo);
+                                if (old == kEmpty) {
+                                    if (wide) h_hi[b + jj] = hi;
+                                    atomicAdd(&s_hot, 1u);
+                                    hot_rows += s_cnt[s] & 0x3FFFFFFFu;
+                                    placed = true;
+                                    break;
+                                }
                             }
-                            t = hot_next(t, CL);
                         }
                     }
                     break;
@@ -525,15 +552,18 @@ __device__ __forceinline__ bool scan_row(const HLaunch *__restrict__ L, const Qu
     k = load_key(q, row);
     if (k.special || c.CL == 0) return true;
     const uint64_t h = hash_key(k.lo, k.hi);
-    uint32_t s = hot_home(h, c.CL);
-    for (int p = 0; p < kProbe; ++p) {
+    const uint32_t nb = hot_buckets(c.CL);
+    const uint32_t b1 = hot_b1(h, nb) * 4, b2 = hot_b2(h, nb) * 4;
+    bool open = false;
+    for (int p = 0; p < 8; ++p) {
+        if (p == 4 && open) return true;
+        const uint32_t s = p < 4 ? b1 + p : b2 + (p - 4);
         const uint64_t cur = c.lkeys[s];
-        if (cur == kEmpty) return true;
+        open |= p < 4 && cur == kEmpty;
         if (cur == k.lo && (!L->wide || c.lkeys_hi[s] == k.hi)) {
             add_row(c.ls, q, s, row);
             return false;
         }
-        s = hot_next(s, c.CL);
     }
     return true;
 }
@@ -1674,7 +1704,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
 
     L.n_sample = n_sample;
     L.hot_slots = CL;
-    L.hot_max = CL / 2;
+    L.hot_max = CL ? (CL / 4 - 1) * 4 * 85 / 100 : 0;  // hot keys placed in the buckets' slots
     L.sample_slots = sample_slots;
     L.chunk = chunk;
     L.nb = nb;

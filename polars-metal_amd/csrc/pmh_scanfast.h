@@ -290,15 +290,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 // one partition slot of the tail)
                 int32_t slot = -1;
                 if (act && special && CL) slot = (int32_t)hot_side_slot(special, CL);
-                if (act && special == 0 && CL) {
-                    uint32_t sp = hot_home(hash_key(key, 0), CL);
-                    for (int pr = 0; pr < kProbe; ++pr) {
-                        const uint64_t cur = lkeys[sp];
-                        if (cur == key) { slot = (int32_t)sp; break; }
-                        if (cur == kEmpty) break;
-                        sp = hot_next(sp, CL);
-                    }
-                }
+                if (act && special == 0 && CL) slot = hot_find(lkeys, key, hash_key(key, 0), CL);
                 // hot rows: fold into LDS
                 if (slot >= 0) {
                     const uint32_t sl = (uint32_t)slot;

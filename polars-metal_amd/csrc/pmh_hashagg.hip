@@ -302,12 +302,13 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     uint64_t *h_hi = h_lo + L->hot_slots;
     __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop, s_hotrows;
     __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_eq_used;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t CL = L->hot_slots;
     for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
     for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
     for (uint32_t i = tid; i < CL; i += bs) { h_lo[i] = kEmpty; if (wide) h_hi[i] = kEmpty; }
-    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; }
+    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; s_eq_used = 0; }
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
     uint32_t drops = 0, sidekeys = 0;
@@ -386,11 +387,23 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     }
     __syncthreads();
     const uint32_t T = s_thresh;
-    for (uint32_t i = tid; i < SS && CL; i += bs) {
+    // keys with count == T take the remaining places in wave order: one
+    // atomic per wave ranks its lanes (SS is a multiple of the block size, so
+    // every lane runs every iteration)
+    const uint32_t lane = tid & 63;
+    for (uint32_t i0 = 0; i0 < SS && CL; i0 += bs) {
+        const uint32_t i = i0 + tid;
         const uint32_t c = min(s_cnt[i], 1023u);
-        if (!c) continue;
-        bool pick = c > T;
-        if (!pick && c == T) pick = (int32_t)atomicSub(&s_take_eq, 1u) > 0;
+        bool pick = c && c > T;
+        const bool eq = c && c == T;
+        const unsigned long long m = __ballot(eq);
+        if (m) {
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_eq_used, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (eq) pick = base + (uint32_t)__popcll(m & ((1ull << lane) - 1)) < s_take_eq;
+        }
         if (pick) s_cnt[i] |= 1u << 31;
     }
     __syncthreads();

@@ -466,3 +466,21 @@ def test_cmp_and_bool(native, op):
         d, v = orc.eval_predicate({"kind": kind, "lhs": col("p", "Bool"), "rhs": col("q", "Bool")}, fb)
         np.testing.assert_array_equal(orc.unpack_bits(gv, n), v)
         np.testing.assert_array_equal(orc.unpack_bits(gd, n), d & v)
+
+
+def test_device_buffers_round_trip_and_copy(native):
+    """pmh_buffer_*: host -> device -> device copy -> host, zeroed
+    allocation, and a copy larger than its buffer refused."""
+    from polars_metal.buffer import DeviceBuffer
+    rng = np.random.default_rng(31)
+    a = rng.integers(0, 256, 1_000_003, dtype=np.uint8)
+    src = DeviceBuffer.from_numpy(a)
+    dst = DeviceBuffer.alloc(a.nbytes)
+    dst.copy_from(src)
+    assert np.array_equal(dst.to_numpy(np.uint8), a)
+    z = DeviceBuffer.alloc(4096, zeroed=True)
+    assert not z.to_numpy(np.uint8).any()
+    with pytest.raises(ValueError):
+        z.copy_from(src)
+    for b in (src, dst, z):
+        b.free()

@@ -1160,13 +1160,24 @@ __global__ __launch_bounds__(256) void hs_part_bounds(const HLaunch *__restrict_
         L->part_start[p] = p == 0 ? 0 : p == P ? n : part_lower_bound(L, S, L->wide, n, p);
 }
 
+// A tail row whose key found no slot in its partition's LDS table folds
+// into the global fallback table: rare, so kept out of line.
+__device__ __noinline__ void part_fallback(const HLaunch *__restrict__ L, Key k, uint64_t h, SpillRowV row) {
+    GSink gs{L->t.arr, L->t.mask + 3};
+    const int64_t g = L->wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
+    if (g >= 0) add_row(gs, &L->q, (uint64_t)g, row);
+}
+
+// WIDE: keys of more than 64 bits; SIG: the aggregate signature (-1: the
+// general fold, for more than two value columns).
+template <bool WIDE, int SIG>
 __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ L, const Spill *src_p) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint64_t s_range[2];
     __shared__ uint32_t s_count, s_base;
     const Spill &S = *src_p;
     const QueryDesc *__restrict__ q = as_const(&L->q);
-    const bool wide = L->wide;
+    constexpr bool wide = WIDE;
     const uint32_t CL = L->tail_slots;
     const uint32_t nls = CL + 2;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
@@ -1176,9 +1187,7 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     uint32_t *lst = (uint32_t *)(lh + nls);  // wide: 0 free, 1 writing, 2 published
     uint8_t *state = wide ? (uint8_t *)(lst + nls) : (uint8_t *)lh;
     LSink ls{state, L->lds3_off, nls};
-    GSink gs{L->t.arr, L->t.mask + 3};
     const LdsAgg2 A = lds_agg2(q, state, L->lds3_off, nls);
-    const bool agg2 = q->n_vals <= 2;
     for (uint32_t p = blockIdx.x; p <= pmask; p += gridDim.x) {
         if (tid < 2) s_range[tid] = L->part_start[p + tid];
         if (tid == 0) s_count = 0;
@@ -1268,11 +1277,10 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
                     }
                 }
                 if (slot >= 0) {
-                    if (agg2) lds_agg2_add(A, (uint32_t)slot, fl, v0b[u], v1b[u]);
+                    if constexpr (SIG >= 0) lds_agg2_add_sig<SIG>(A, (uint32_t)slot, fl, v0b[u], v1b[u]);
                     else add_row(ls, q, (uint32_t)slot, row);
                 } else {
-                    const int64_t g = wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
-                    if (g >= 0) add_row(gs, q, (uint64_t)g, row);
+                    part_fallback(L, k, h, row);
                 }
             }
         }
@@ -2031,9 +2039,20 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     {
         KernelTimer kt("hs_part_agg");
-        PMH_HIP(hipFuncSetAttribute((const void *)hs_part_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3b));
+        const int sig = q0.n_vals <= 2 ? agg_signature(q0) : -1;
+        static const void *const kPartAgg[2][kNumAggSigs + 1] = {
+            {(const void *)hs_part_agg<false, -1>, (const void *)hs_part_agg<false, 0>,
+             (const void *)hs_part_agg<false, 1>, (const void *)hs_part_agg<false, 2>,
+             (const void *)hs_part_agg<false, 3>, (const void *)hs_part_agg<false, 4>},
+            {(const void *)hs_part_agg<true, -1>, (const void *)hs_part_agg<true, 0>, (const void *)hs_part_agg<true, 1>,
+             (const void *)hs_part_agg<true, 2>, (const void *)hs_part_agg<true, 3>, (const void *)hs_part_agg<true, 4>}};
+        static_assert(kNumAggSigs == 5, "one instantiation per signature");
+        const void *fn = kPartAgg[wide ? 1 : 0][sig + 1];
+        PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds3b));
         const uint32_t g = (uint32_t)std::min<uint64_t>(P_parts, (uint64_t)cus * 2);
-        hipLaunchKernelGGL(hs_part_agg, dim3(g), dim3(1024), lds3b, R.stream, dL, bufs[src]);
+        const Spill *sp = bufs[src];
+        void *args[] = {(void *)&dL, (void *)&sp};
+        PMH_HIP(hipLaunchKernel(fn, dim3(g), dim3(1024), args, lds3b, R.stream));
         launch_check();
     }
     if (CL) {

@@ -220,6 +220,89 @@ __device__ __forceinline__ void lds_agg2_add(const LdsAgg2 &A, uint32_t sl, uint
     }
 }
 
+// Compile-time aggregate signatures of the common query shapes, so the
+// per-row fold of the hottest kernels carries no branch on the plan:
+//   1: sum/mean of an f64 column and min/max of an i32 column
+//   2: sum of an i64 column
+//   3: sum, min and max of one f64 column
+//   4: sum of one f64 column
+// (counts and lens ride along with every signature).  0: any shape with at
+// most two value columns, resolved at run time.
+struct AggSig {
+    int nv, kind0, need0, tag0, kind1, need1, tag1;
+};
+constexpr AggSig kAggSigs[] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {2, VK_F, NEED_SUM, PMH_F64, VK_I32, NEED_MINMAX, PMH_I32},
+    {1, VK_I64, NEED_SUM, PMH_I64, 0, 0, 0},
+    {1, VK_F, NEED_SUM | NEED_MINMAX, PMH_F64, 0, 0, 0},
+    {1, VK_F, NEED_SUM, PMH_F64, 0, 0, 0},
+};
+constexpr int kNumAggSigs = sizeof(kAggSigs) / sizeof(kAggSigs[0]);
+
+// The signature of q (0 when none matches).
+inline int agg_signature(const QueryDesc &q) {
+    for (int s = 1; s < kNumAggSigs; ++s) {
+        const AggSig &g = kAggSigs[s];
+        if (q.n_vals != g.nv) continue;
+        bool ok = true;
+        for (int j = 0; j < g.nv; ++j) {
+            const ValSpec &v = q.vals[j];
+            const int kind = j ? g.kind1 : g.kind0, need = j ? g.need1 : g.need0, tag = j ? g.tag1 : g.tag0;
+            ok = ok && v.kind == kind && v.need == need && v.tag == tag;
+        }
+        if (ok) return s;
+    }
+    return 0;
+}
+
+template <int KIND, int NEED, int TAG>
+__device__ __forceinline__ void lds_fold_value(const LdsAgg2 &A, int j, uint32_t sl, bool valid, uint64_t raw) {
+    if (!valid) return;
+    atomicAdd(A.cnt_p[j] + sl, 1u);
+    if constexpr (KIND == VK_F) {
+        const double x = TAG == PMH_F64 ? __longlong_as_double((long long)raw) : raw_to_f64(TAG, raw);
+        if constexpr ((NEED & NEED_SUM) != 0) lds_add_f64(A.limb_p[j], A.res_p[j], A.nslots, sl, x);
+        if constexpr ((NEED & NEED_MINMAX) != 0) {
+            if (x != x) {
+                atomicOr(A.nan_p[j] + sl, 1u);
+            } else {
+                const long long kk = (long long)f64_order_key(x);
+                atomicMin((long long *)A.min_p[j] + sl, kk);
+                atomicMax((long long *)A.max_p[j] + sl, kk);
+            }
+        }
+    } else if constexpr (KIND == VK_I64) {
+        const int64_t x = raw_to_i64(TAG, raw);
+        if constexpr ((NEED & NEED_SUM) != 0) atomicAdd(A.sum_p[j] + sl, (unsigned long long)x);
+        if constexpr ((NEED & NEED_MINMAX) != 0) {
+            atomicMin((long long *)A.min_p[j] + sl, (long long)x);
+            atomicMax((long long *)A.max_p[j] + sl, (long long)x);
+        }
+    } else {
+        const int32_t x = (int32_t)raw_to_i64(TAG, raw);
+        if constexpr ((NEED & NEED_SUM) != 0) atomicAdd(A.sum_p[j] + sl, (unsigned long long)(int64_t)x);
+        if constexpr ((NEED & NEED_MINMAX) != 0) {
+            atomicMin((int *)A.min_p[j] + sl, x);
+            atomicMax((int *)A.max_p[j] + sl, x);
+        }
+    }
+}
+
+// lds_agg2_add for signature SIG (0: resolved at run time).
+template <int SIG>
+__device__ __forceinline__ void lds_agg2_add_sig(const LdsAgg2 &A, uint32_t sl, uint32_t flags, uint64_t v0,
+                                                 uint64_t v1) {
+    if constexpr (SIG == 0) {
+        lds_agg2_add(A, sl, flags, v0, v1);
+    } else {
+        constexpr AggSig G = kAggSigs[SIG];
+        atomicAdd(A.len_p + sl, 1u);
+        lds_fold_value<G.kind0, G.need0, G.tag0>(A, 0, sl, flags & 1u, v0);
+        if constexpr (G.nv > 1) lds_fold_value<G.kind1, G.need1, G.tag1>(A, 1, sl, (flags >> 1) & 1u, v1);
+    }
+}
+
 // Merge LDS slot ls into global slot gs: one atomic per touched array.
 __device__ __forceinline__ void merge_lds_entry(GSink &g, uint64_t gs, LSink &l, uint32_t ls,
                                                 const Layout &lay, const int32_t *ops) {

@@ -1700,13 +1700,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     const size_t key_bytes = wide ? 16 : 8;
 
     // ---- parameters
-    uint32_t nb = (uint32_t)std::min<uint64_t>(cus, std::max<uint64_t>(1, ceil_div(n, 16384)));
+    // scan blocks: scan_blocks resident per CU, each with its share of LDS
+    const uint32_t per_cu = (uint32_t)std::min<int64_t>(4, std::max<int64_t>(1, opt.scan_blocks));
+    uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, ceil_div(n, 16384)));
     const uint64_t chunk = std::max<uint64_t>(1, ceil_div(n, nb));
     nb = (uint32_t)std::max<uint64_t>(1, ceil_div(n, chunk));
     // the specialised scan keeps a row ring per wave next to the hot table
     // and has only 8 bytes of static LDS
     const size_t ring_bytes = (size_t)16 * kRing * 4;
-    const size_t hot_budget = L.fs.enabled ? lds_max - 64 - ring_bytes : lds_budget;
+    const size_t share = lds_max / per_cu;
+    const size_t hot_budget = L.fs.enabled ? share - 64 - ring_bytes : share - 1024;
     uint32_t CL = (uint32_t)pow2_floor(hot_budget / (key_bytes + state_per_slot));
     CL = std::min<uint32_t>(CL, 2048);
     if (opt.lds_slots >= 0) CL = opt.lds_slots == 0 ? 0 : (uint32_t)std::min<uint64_t>(CL, pow2_floor(opt.lds_slots));

@@ -705,7 +705,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         while (c * 2 <= (uint64_t)opt.tail_slots && c * 2 <= CL) c *= 2;
         CL = c;
     }
-    const double part_keys = CL * 0.5;
+    const double part_keys = CL * 0.8;
     // ---- per query: columns by name, predicate intervals (cached by text)
     std::vector<BQuery> bq(nq);
     std::map<std::string, FastScan> pred_cache;

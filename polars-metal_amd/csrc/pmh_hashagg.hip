@@ -265,22 +265,28 @@ __device__ __forceinline__ uint32_t hot_b2(uint64_t h, uint32_t nb) {
     return (uint32_t)(((h & 0xFFFFFFFFull) * nb) >> 32);
 }
 // Slot of a (non-special) 64-bit key in an LDS copy of the hot keys, or -1.
+// Both buckets are read up front (four 16-byte LDS loads behind one wait)
+// and the slot is chosen with selects: the first match in the first bucket,
+// else -1 when that bucket has a free slot, else the first match in the
+// second.  lk must be 16-byte aligned (buckets start at multiples of four).
+typedef unsigned long long hot_pair __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ int32_t hot_find(const uint64_t *lk, uint64_t key, uint64_t h, uint32_t CL) {
     const uint32_t nb = hot_buckets(CL);
-    const uint32_t b1 = hot_b1(h, nb) * 4;
-    bool open = false;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint64_t c = lk[b1 + j];
-        if (c == key) return (int32_t)(b1 + j);
-        open |= c == kEmpty;
-    }
-    if (open) return -1;
-    const uint32_t b2 = hot_b2(h, nb) * 4;
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-        if (lk[b2 + j] == key) return (int32_t)(b2 + j);
-    return -1;
+    const uint32_t b1 = hot_b1(h, nb) * 4, b2 = hot_b2(h, nb) * 4;
+    const hot_pair *p1 = (const hot_pair *)(lk + b1), *p2 = (const hot_pair *)(lk + b2);
+    const hot_pair x0 = p1[0], x1 = p1[1], y0 = p2[0], y1 = p2[1];
+    int32_t s = -1;
+    s = y1.y == key ? (int32_t)b2 + 3 : s;
+    s = y1.x == key ? (int32_t)b2 + 2 : s;
+    s = y0.y == key ? (int32_t)b2 + 1 : s;
+    s = y0.x == key ? (int32_t)b2 : s;
+    const bool open = (x0.x == kEmpty) | (x0.y == kEmpty) | (x1.x == kEmpty) | (x1.y == kEmpty);
+    s = open ? -1 : s;
+    s = x1.y == key ? (int32_t)b1 + 3 : s;
+    s = x1.x == key ? (int32_t)b1 + 2 : s;
+    s = x0.y == key ? (int32_t)b1 + 1 : s;
+    s = x0.x == key ? (int32_t)b1 : s;
+    return s;
 }
 __device__ __forceinline__ uint32_t hot_side_slot(int special, uint32_t CL) { return CL - 3 + (uint32_t)special; }
 
@@ -1948,8 +1954,18 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             KernelTimer kt("hs_scan_fast");
             const bool raw = q0.key_mode == KM_RAW64;
             const bool r8 = L.fs.rows == 8;
-            const void *fn = raw ? (r8 ? (const void *)hs_scan_fast<true, 8> : (const void *)hs_scan_fast<true, 4>)
-                                 : (r8 ? (const void *)hs_scan_fast<false, 8> : (const void *)hs_scan_fast<false, 4>);
+            // four rows per lane: one instantiation per aggregate signature;
+            // eight: the fold resolved at run time
+            static const void *const kScan4[2][kNumAggSigs] = {
+                {(const void *)hs_scan_fast<false, 4, 0>, (const void *)hs_scan_fast<false, 4, 1>,
+                 (const void *)hs_scan_fast<false, 4, 2>, (const void *)hs_scan_fast<false, 4, 3>,
+                 (const void *)hs_scan_fast<false, 4, 4>},
+                {(const void *)hs_scan_fast<true, 4, 0>, (const void *)hs_scan_fast<true, 4, 1>,
+                 (const void *)hs_scan_fast<true, 4, 2>, (const void *)hs_scan_fast<true, 4, 3>,
+                 (const void *)hs_scan_fast<true, 4, 4>}};
+            const int sig = agg_signature(q0);  // the fast scan takes at most two value columns
+            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0> : (const void *)hs_scan_fast<false, 8, 0>)
+                                : kScan4[raw ? 1 : 0][sig];
             PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
             void *args[] = {(void *)&dL};
             PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));

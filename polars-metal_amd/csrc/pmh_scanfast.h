@@ -38,7 +38,9 @@ namespace pmh {
 constexpr uint32_t kRing = 127;
 
 
-template <bool RAW64, int U>
+// SIG: the query's compile-time aggregate signature (0: resolved at run
+// time), so the fold of hot rows carries no branch on the plan.
+template <bool RAW64, int U, int SIG>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_cursor, s_sel;
@@ -73,34 +75,16 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
         knull[i] = i < nk ? q->keys[i].null_off : 0;
         kdata[i] = i < nk ? q->keys[i].data_off : 0;
     }
-    const int nv = q->n_vals;
-    auto lds_at = [&](int a) -> uint8_t * { return a >= 0 ? state + L->lds1_off[a] : state; };
-    uint32_t *len_p = (uint32_t *)lds_at(q->layout.a_len);
+    const int nv = SIG > 0 ? kAggSigs[SIG].nv : q->n_vals;
+    const LdsAgg2 A = lds_agg2(q, state, L->lds1_off, CL);
     RoleCol vc[2];
-    int vkind[2], vneed[2], vwid[2];
-    uint32_t *cnt_p[2], *nan_p[2];
-    unsigned long long *limb_p[2], *sum_p[2];
-    double *res_p[2];
-    void *min_p[2], *max_p[2];
+    int vwid[2];
     void *sval[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const bool in = j < nv;
-        const ValSpec &S = q->vals[in ? j : 0];
-        vc[j] = role_col(q, in ? S.col : 0);
-        vkind[j] = in ? S.kind : VK_I64;
-        vneed[j] = in ? S.need : 0;
+        vc[j] = role_col(q, in ? q->vals[j].col : 0);
         vwid[j] = in ? L->vw[j] : 8;
-        const bool sum_f = (vneed[j] & NEED_SUM) && vkind[j] == VK_F;
-        const bool sum_i = (vneed[j] & NEED_SUM) && vkind[j] != VK_F;
-        const bool mm = (vneed[j] & NEED_MINMAX) != 0;
-        cnt_p[j] = (uint32_t *)lds_at(in ? S.a_cnt : -1);
-        limb_p[j] = (unsigned long long *)lds_at(sum_f ? S.a_limb : -1);
-        res_p[j] = (double *)lds_at(sum_f ? S.a_res : -1);
-        sum_p[j] = (unsigned long long *)lds_at(sum_i ? S.a_sum : -1);
-        min_p[j] = lds_at(mm ? S.a_min : -1);
-        max_p[j] = lds_at(mm ? S.a_max : -1);
-        nan_p[j] = (uint32_t *)lds_at(mm && vkind[j] == VK_F ? S.a_nan : -1);
         sval[j] = in ? L->spill.val[j] : nullptr;
     }
     uint64_t *sklo = L->spill.klo;
@@ -292,42 +276,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 if (act && special && CL) slot = (int32_t)hot_side_slot(special, CL);
                 if (act && special == 0 && CL) slot = hot_find(lkeys, key, hash_key(key, 0), CL);
                 // hot rows: fold into LDS
-                if (slot >= 0) {
-                    const uint32_t sl = (uint32_t)slot;
-                    atomicAdd(len_p + sl, 1u);
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        if (j >= nv || !((vv >> j) & 1)) continue;
-                        const int need = vneed[j];
-                        atomicAdd(cnt_p[j] + sl, 1u);
-                        if (vkind[j] == VK_F) {
-                            const double x = vc[j].tag == PMH_F64 ? __longlong_as_double((long long)vr[j])
-                                                                  : (double)__uint_as_float((uint32_t)vr[j]);
-                            if (need & NEED_SUM) lds_add_f64(limb_p[j], res_p[j], CL, sl, x);
-                            if (need & NEED_MINMAX) {
-                                const bool nan = x != x;
-                                const long long kk = (long long)f64_order_key(x);
-                                atomicOr(nan_p[j] + sl, nan ? 1u : 0u);
-                                atomicMin((long long *)min_p[j] + sl, nan ? 0x7FFFFFFFFFFFFFFFll : kk);
-                                atomicMax((long long *)max_p[j] + sl, nan ? (long long)0x8000000000000000ull : kk);
-                            }
-                        } else if (vkind[j] == VK_I64) {
-                            const int64_t x = (int64_t)vr[j];
-                            if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)x);
-                            if (need & NEED_MINMAX) {
-                                atomicMin((long long *)min_p[j] + sl, (long long)x);
-                                atomicMax((long long *)max_p[j] + sl, (long long)x);
-                            }
-                        } else {
-                            const int32_t x = (int32_t)(uint32_t)vr[j];
-                            if (need & NEED_SUM) atomicAdd(sum_p[j] + sl, (unsigned long long)(int64_t)x);
-                            if (need & NEED_MINMAX) {
-                                atomicMin((int *)min_p[j] + sl, x);
-                                atomicMax((int *)max_p[j] + sl, x);
-                            }
-                        }
-                    }
-                }
+                if (slot >= 0) lds_agg2_add_sig<SIG>(A, (uint32_t)slot, vv, vr[0], vr[1]);
                 // tail rows: wave-contiguous append
                 const bool tail = act && slot < 0;
                 const unsigned long long m = __ballot(tail);

@@ -1685,13 +1685,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         const int w = tag_size(q0.cols[c].tag);
         if (w > 1 && ((uintptr_t)q0.cols[c].data % (uintptr_t)w) != 0) aligned = false;
     }
-    if (opt.fast_scan != 0 && aligned && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 32) &&
+    if (opt.fast_scan != 0 && aligned && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 25) &&
         (q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64 || q0.key_mode == KM_NONE) && lower_fast_scan(q0, L.fs))
         L.fs.enabled = 1;
     else
         L.fs = FastScan{};
     L.fs.rows = opt.scan_rows == 8 ? 8 : 4;
-    L.fs.touch = opt.scan_touch != 0 ? 1 : 0;
     size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
     for (int j = 0; j < nv; ++j) {
         L.vw[j] = tag_size(q0.vals[j].tag);

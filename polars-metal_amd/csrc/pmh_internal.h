@@ -169,7 +169,6 @@ struct Options {
     int64_t fast_scan = 1;        // partitioned engine: 1 use the specialised scan where the query fits it, 0 never
     int64_t scan_rows = 4;        // specialised scan: rows per lane per step (4 or 8)
     int64_t scan_blocks = 1;      // partitioned scan: resident blocks per CU (the hot table shares LDS)
-    int64_t scan_touch = 0;       // specialised scan: 1 touches key/value rows one step ahead of their gathers
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 

@@ -46,50 +46,60 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     __shared__ uint32_t s_cursor, s_sel;
     const QueryDesc *__restrict__ q = &L->q;
     const uint32_t CL = L->hot_slots;
-    const uint32_t tid = threadIdx.x, bs = blockDim.x;
+    // launched with 1024 threads
+    constexpr uint32_t bs = 1024;
+    const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const unsigned long long below = (1ull << lane) - 1;
     uint64_t *lkeys = (uint64_t *)smem;
     uint8_t *state = (uint8_t *)(lkeys + CL);
     uint32_t *ring = (uint32_t *)(smem + L->fs.ring_off) + (tid >> 6) * kRing;
 
+    // The block's rows [r_begin, r_end), addressed from r_begin in 32 bits
+    // (a block spans fewer than 2^25 rows); its spill region starts at
+    // r_begin too.
+    const uint64_t n = q->n_rows;
+    const uint64_t r_begin = (uint64_t)blockIdx.x * L->chunk;
+    const uint64_t r_end = min(n, r_begin + L->chunk);
+    const uint32_t span = r_end > r_begin ? (uint32_t)(r_end - r_begin) : 0;
+    const uint32_t r_last = span ? span - 1 : 0;  // in-range stand-in for rows past the end
+
     // ---- descriptor, once per block
     const int nr = L->fs.nr;
-    RoleCol pc[2];
+    BlockCol pc[2];
     int pflt[2];
     int64_t plo[2], phi[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-        pc[r] = role_col(q, r < nr ? L->fs.rcol[r] : 0);
+        pc[r] = block_col(role_col(q, r < nr ? L->fs.rcol[r] : 0), r_begin);
         pflt[r] = L->fs.rflt[r];
         plo[r] = L->fs.rlo[r];
         phi[r] = L->fs.rhi[r];
     }
     const int nk = RAW64 ? 1 : (q->key_mode == KM_NONE ? 0 : q->n_keys);
     const bool allones = q->allones_possible != 0;
-    RoleCol kc[4];
+    BlockCol kc[4];
     int knull[4], kdata[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        kc[i] = role_col(q, i < nk ? q->keys[i].col : 0);
+        kc[i] = block_col(role_col(q, i < nk ? q->keys[i].col : 0), r_begin);
         knull[i] = i < nk ? q->keys[i].null_off : 0;
         kdata[i] = i < nk ? q->keys[i].data_off : 0;
     }
     const int nv = SIG > 0 ? kAggSigs[SIG].nv : q->n_vals;
     const LdsAgg2 A = lds_agg2(q, state, L->lds1_off, CL);
-    RoleCol vc[2];
+    BlockCol vc[2];
     int vwid[2];
     void *sval[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         const bool in = j < nv;
-        vc[j] = role_col(q, in ? q->vals[j].col : 0);
+        vc[j] = block_col(role_col(q, in ? q->vals[j].col : 0), r_begin);
         vwid[j] = in ? L->vw[j] : 8;
-        sval[j] = in ? L->spill.val[j] : nullptr;
+        sval[j] = in ? (void *)((uint8_t *)L->spill.val[j] + r_begin * (uint64_t)L->vw[j]) : nullptr;
     }
-    uint64_t *sklo = L->spill.klo;
-    uint16_t *sflags = L->spill.flags;
-    const uint64_t region = (uint64_t)blockIdx.x * L->chunk;
+    uint64_t *sklo = L->spill.klo + r_begin;
+    uint16_t *sflags = L->spill.flags + r_begin;
 
     // ---- LDS: hot keys and state identities
     if (tid == 0) { s_cursor = 0; s_sel = 0; }
@@ -107,79 +117,50 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     }
     __syncthreads();
 
-    const uint64_t n = q->n_rows;
-    const uint64_t r_begin = region;
-    const uint64_t r_end = min(n, r_begin + L->chunk);
-    const uint64_t span = r_end > r_begin ? r_end - r_begin : 0;
     // every lane of a wave runs the same steps and rounds, so ballots see
     // whole waves
-    const uint64_t steps = (span + (uint64_t)U * bs - 1) / ((uint64_t)U * bs);
-    const uint64_t r_last = span ? r_end - 1 : r_begin;  // in-range stand-in for rows past the end
+    const uint32_t steps = (span + U * bs - 1) / (U * bs);
     uint32_t qhead = 0, qn = 0;  // wave-uniform ring state
     uint32_t wave_sel = 0;
 
     // Predicate words run one step ahead: the loads for step it + 1 are in
-    // flight while step it's rounds run.  Beside them go touches: one dword
-    // of the step's key and value rows, so that when the rounds gather those
-    // rows a step later their lines are already in cache.
-    const bool touch = L->fs.touch != 0;
+    // flight while step it's rounds run.
     uint64_t pw[2][U], pv[2][U];
-    uint32_t tw[3][U];
-    auto load_pred = [&](uint64_t step) __attribute__((always_inline)) {
-        const uint64_t rs = r_begin + step * (uint64_t)U * bs + tid;
+    auto load_pred = [&](uint32_t step) __attribute__((always_inline)) {
+        const uint32_t rs = step * (U * bs) + tid;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             if (r >= nr) continue;
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
-                pw[r][u] = word_at(pc[r].base, vpos(pc[r], row));
-            }
+            for (int u = 0; u < U; ++u) pw[r][u] = word_at(pc[r].base, vpos(pc[r], min(rs + u * bs, r_last)));
             // columns without a validity bitmap load no validity words
             if (pc[r].vstride) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
-                    pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], row));
-                }
-            }
-        }
-        if (touch) {
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint64_t row = min(rs + (uint64_t)u * bs, r_last);
-                tw[0][u] = gld<uint32_t>(kc[0].base, vpos(kc[0], row) >> 5);
-#pragma unroll
-                for (int j = 0; j < 2; ++j) tw[1 + j][u] = j < nv ? gld<uint32_t>(vc[j].base, vpos(vc[j], row) >> 5) : 0;
+                for (int u = 0; u < U; ++u) pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], min(rs + u * bs, r_last)));
             }
         }
     };
     if (steps) load_pred(0);
 
-    for (uint64_t it = 0; it <= steps; ++it) {
+    for (uint32_t it = 0; it <= steps; ++it) {
         const bool drain = it == steps;
         uint32_t selbits = 0;  // bit u: row u of this lane passes
-        const uint64_t r0 = r_begin + it * (uint64_t)U * bs + tid;
+        const uint32_t r0 = it * (U * bs) + tid;
         if (!drain) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) selbits |= (uint32_t)(r0 + (uint64_t)u * bs < r_end) << u;
+            for (int u = 0; u < U; ++u) selbits |= (uint32_t)(r0 + u * bs < span) << u;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 if (r >= nr) continue;
                 uint64_t raw[U];
                 uint32_t vb = (1u << U) - 1;
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t row = min(r0 + (uint64_t)u * bs, r_last);
-                    raw[u] = cut(pw[r][u], vpos(pc[r], row), pc[r].mask);
-                }
+                for (int u = 0; u < U; ++u) raw[u] = cut(pw[r][u], vpos(pc[r], min(r0 + u * bs, r_last)), pc[r].mask);
                 if (pc[r].vstride) {
                     vb = 0;
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const uint64_t row = min(r0 + (uint64_t)u * bs, r_last);
-                        vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], row), 1) << u;
-                    }
+                    for (int u = 0; u < U; ++u)
+                        vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], min(r0 + u * bs, r_last)), 1) << u;
                 }
                 int64_t img[U];
                 pred_images<U>(pc[r].tag, pflt[r] != 0, raw, img);
@@ -187,11 +168,6 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 for (int u = 0; u < U; ++u)
                     if (img[u] < plo[r] || img[u] > phi[r]) vb &= ~(1u << u);
                 selbits &= vb;
-            }
-            if (touch) {
-                // the touches of this step are not needed, only their lines
-#pragma unroll
-                for (int u = 0; u < U; ++u) asm volatile("" ::"v"(tw[0][u]), "v"(tw[1][u]), "v"(tw[2][u]));
             }
             if (it + 1 < steps) load_pred(it + 1);
         }
@@ -203,7 +179,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 if (s) {
                     uint32_t pos = qhead + qn + (uint32_t)__popcll(M & below);
                     pos = pos >= kRing ? pos - kRing : pos;
-                    ring[pos] = (uint32_t)(r0 + (uint64_t)u * bs - r_begin);
+                    ring[pos] = r0 + u * bs;
                 }
                 const uint32_t c = (uint32_t)__popcll(M);
                 qn += c;
@@ -220,7 +196,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 uint32_t rp = qhead + lane;
                 rp = rp >= kRing ? rp - kRing : rp;
                 const uint32_t off = ring[rp];
-                const uint64_t row = r_begin + (act ? off : 0);
+                const uint32_t row = act ? off : 0;
                 qhead = qhead + cnt >= kRing ? qhead + cnt - kRing : qhead + cnt;
                 qn -= cnt;
                 // gathers: every word first, then the cuts
@@ -286,7 +262,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     if ((int)lane == leader) base = atomicAdd(&s_cursor, (uint32_t)__popcll(m));
                     base = __shfl(base, leader);
                     if (tail) {
-                        const uint64_t pos = region + base + (uint32_t)__popcll(m & below);
+                        const uint32_t pos = base + (uint32_t)__popcll(m & below);
                         uint16_t fl = special == 1 ? kFlagAllOnes : (special == 2 ? kFlagNull : 0);
                         fl |= (uint16_t)vv;
 #pragma unroll

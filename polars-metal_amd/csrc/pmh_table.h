@@ -472,7 +472,6 @@ struct FastScan {
     int64_t rlo[2], rhi[2];
     uint32_t ring_off;       // byte offset of the per-wave row rings in LDS
     int32_t rows;            // rows per lane per step (4 or 8)
-    int32_t touch;           // touch key/value rows one step ahead of their gathers
     int32_t pad;
 };
 

@@ -54,11 +54,45 @@ __device__ __forceinline__ uint64_t bpos(const RoleCol &c, uint64_t r) { return 
 __device__ __forceinline__ uint64_t word_at(const uint64_t *base, uint64_t pos) { return gld<uint64_t>(base, pos >> 6); }
 __device__ __forceinline__ uint64_t cut(uint64_t word, uint64_t pos, uint64_t mask) { return (word >> (pos & 63)) & mask; }
 
+// A column as one block of a scan sees it: the word bases advanced to the
+// block's first row, so the bit positions of the block's rows (fewer than
+// 2^25 of them) are 32-bit and a load is a uniform base plus a 32-bit offset.
+struct BlockCol {
+    const uint64_t *base, *vbase;
+    uint64_t mask;
+    uint32_t bit0, vbit0, vstride;
+    int32_t bshift, tag;
+};
+__device__ __forceinline__ BlockCol block_col(const RoleCol &c, uint64_t r0) {
+    BlockCol b;
+    const uint64_t p = c.bit0 + (r0 << c.bshift), vp = c.vbit0 + r0 * c.vstride;
+    b.base = c.base + (p >> 6);
+    b.vbase = c.vbase + (vp >> 6);  // the all-valid stand-in stays put (vstride 0)
+    b.bit0 = (uint32_t)(p & 63);
+    b.vbit0 = (uint32_t)(vp & 63);
+    b.vstride = c.vstride;
+    b.mask = c.mask;
+    b.bshift = c.bshift;
+    b.tag = c.tag;
+    return b;
+}
+__device__ __forceinline__ uint32_t vpos(const BlockCol &c, uint32_t r) { return c.bit0 + (r << c.bshift); }
+__device__ __forceinline__ uint32_t bpos(const BlockCol &c, uint32_t r) { return c.vbit0 + r * c.vstride; }
+__device__ __forceinline__ uint64_t word_at(const uint64_t *base, uint32_t pos) { return gld<uint64_t>(base, pos >> 6); }
+
 // Predicate images of U values of one column (raw bits, zero-extended): the
 // column's TotalOrd image when flt, else its int64 value.  The tag switch is
 // taken once for all U values.
 template <int U>
 __device__ __forceinline__ void pred_images(int tag, bool flt, const uint64_t (&raw)[U], int64_t (&img)[U]) {
+    if (!flt && tag != PMH_F64 && tag != PMH_F32) {
+        // integers: the raw bits are zero-extended, so one shift pair with a
+        // wave-uniform count sign-extends the signed widths (0 for the rest)
+        const int sh = tag == PMH_I32 ? 32 : tag == PMH_I16 ? 48 : tag == PMH_I8 ? 56 : 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) img[u] = (int64_t)(raw[u] << sh) >> sh;
+        return;
+    }
     if (flt) {
         if (tag == PMH_F64) {
 #pragma unroll

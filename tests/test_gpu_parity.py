@@ -157,7 +157,6 @@ ENGINE_OPTS = [
     {"method": "hash", "partitions": 2, "tail_slots": 16},
     {"method": "hash", "lds_slots": 16, "sample": 1000},
     {"method": "hash", "scan_rows": 8}, {"method": "hash", "scan_rows": 8, "lds_slots": 0},
-    {"method": "hash", "scan_touch": 1},
     {"method": "hash", "fast_scan": 0},
 ]
 

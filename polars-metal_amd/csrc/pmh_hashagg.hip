@@ -1708,7 +1708,9 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // scan blocks: scan_blocks resident per CU, each with its share of LDS
     const uint32_t per_cu = (uint32_t)std::min<int64_t>(4, std::max<int64_t>(1, opt.scan_blocks));
     uint32_t nb = (uint32_t)std::min<uint64_t>((uint64_t)cus * per_cu, std::max<uint64_t>(1, ceil_div(n, 16384)));
-    const uint64_t chunk = std::max<uint64_t>(1, ceil_div(n, nb));
+    // blocks start on 64-row boundaries: no cache line or validity word of
+    // a column is split between two blocks
+    const uint64_t chunk = std::max<uint64_t>(64, ceil_div(ceil_div(n, nb), 64) * 64);
     nb = (uint32_t)std::max<uint64_t>(1, ceil_div(n, chunk));
     // the specialised scan keeps a row ring per wave next to the hot table
     // and has only 8 bytes of static LDS

@@ -123,20 +123,23 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     uint32_t qhead = 0, qn = 0;  // wave-uniform ring state
     uint32_t wave_sel = 0;
 
+    // A step covers U * 1024 rows; wave w takes the U * 64 contiguous rows
+    // from w * U * 64, 64 per sub-step, so a wave's cache lines are its own.
+    const uint32_t wslice = (tid >> 6) * (U * 64);
     // Predicate words run one step ahead: the loads for step it + 1 are in
     // flight while step it's rounds run.
     uint64_t pw[2][U], pv[2][U];
     auto load_pred = [&](uint32_t step) __attribute__((always_inline)) {
-        const uint32_t rs = step * (U * bs) + tid;
+        const uint32_t rs = step * (U * bs) + wslice + lane;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             if (r >= nr) continue;
 #pragma unroll
-            for (int u = 0; u < U; ++u) pw[r][u] = word_at(pc[r].base, vpos(pc[r], min(rs + u * bs, r_last)));
+            for (int u = 0; u < U; ++u) pw[r][u] = word_at(pc[r].base, vpos(pc[r], min(rs + u * 64, r_last)));
             // columns without a validity bitmap load no validity words
             if (pc[r].vstride) {
 #pragma unroll
-                for (int u = 0; u < U; ++u) pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], min(rs + u * bs, r_last)));
+                for (int u = 0; u < U; ++u) pv[r][u] = word_at(pc[r].vbase, bpos(pc[r], min(rs + u * 64, r_last)));
             }
         }
     };
@@ -145,22 +148,22 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     for (uint32_t it = 0; it <= steps; ++it) {
         const bool drain = it == steps;
         uint32_t selbits = 0;  // bit u: row u of this lane passes
-        const uint32_t r0 = it * (U * bs) + tid;
+        const uint32_t r0 = it * (U * bs) + wslice + lane;
         if (!drain) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) selbits |= (uint32_t)(r0 + u * bs < span) << u;
+            for (int u = 0; u < U; ++u) selbits |= (uint32_t)(r0 + u * 64 < span) << u;
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 if (r >= nr) continue;
                 uint64_t raw[U];
                 uint32_t vb = (1u << U) - 1;
 #pragma unroll
-                for (int u = 0; u < U; ++u) raw[u] = cut(pw[r][u], vpos(pc[r], min(r0 + u * bs, r_last)), pc[r].mask);
+                for (int u = 0; u < U; ++u) raw[u] = cut(pw[r][u], vpos(pc[r], min(r0 + u * 64, r_last)), pc[r].mask);
                 if (pc[r].vstride) {
                     vb = 0;
 #pragma unroll
                     for (int u = 0; u < U; ++u)
-                        vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], min(r0 + u * bs, r_last)), 1) << u;
+                        vb |= (uint32_t)cut(pv[r][u], bpos(pc[r], min(r0 + u * 64, r_last)), 1) << u;
                 }
                 int64_t img[U];
                 pred_images<U>(pc[r].tag, pflt[r] != 0, raw, img);
@@ -179,7 +182,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 if (s) {
                     uint32_t pos = qhead + qn + (uint32_t)__popcll(M & below);
                     pos = pos >= kRing ? pos - kRing : pos;
-                    ring[pos] = r0 + u * bs;
+                    ring[pos] = r0 + u * 64;
                 }
                 const uint32_t c = (uint32_t)__popcll(M);
                 qn += c;

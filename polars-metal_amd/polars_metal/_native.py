@@ -57,7 +57,12 @@ def _jsonable(obj: Any) -> Any:
 def _plan_json(plan: dict) -> bytes:
     if not isinstance(plan, dict):
         raise ValueError("polars_metal: plan parse error: plan must be a dict")
-    return json.dumps(_jsonable(plan), allow_nan=True).encode("utf-8")
+    try:
+        # json writes tuples as lists and stringifies int/float/bool keys
+        # itself; the normalising copy is only needed for other key types
+        return json.dumps(plan, allow_nan=True).encode("utf-8")
+    except TypeError:
+        return json.dumps(_jsonable(plan), allow_nan=True).encode("utf-8")
 
 
 # ---------------------------------------------------------------------------

@@ -1955,18 +1955,20 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             KernelTimer kt("hs_scan_fast");
             const bool raw = q0.key_mode == KM_RAW64;
             const bool r8 = L.fs.rows == 8;
-            // four rows per lane: one instantiation per aggregate signature;
-            // eight: the fold resolved at run time
-            static const void *const kScan4[2][kNumAggSigs] = {
-                {(const void *)hs_scan_fast<false, 4, 0>, (const void *)hs_scan_fast<false, 4, 1>,
-                 (const void *)hs_scan_fast<false, 4, 2>, (const void *)hs_scan_fast<false, 4, 3>,
-                 (const void *)hs_scan_fast<false, 4, 4>},
-                {(const void *)hs_scan_fast<true, 4, 0>, (const void *)hs_scan_fast<true, 4, 1>,
-                 (const void *)hs_scan_fast<true, 4, 2>, (const void *)hs_scan_fast<true, 4, 3>,
-                 (const void *)hs_scan_fast<true, 4, 4>}};
+            // four rows per lane: one instantiation per aggregate signature
+            // and predicate width; eight: the fold resolved at run time
+#define PMH_SCAN4(R, N)                                                                                 \
+    {(const void *)hs_scan_fast<R, 4, 0, N>, (const void *)hs_scan_fast<R, 4, 1, N>,                     \
+     (const void *)hs_scan_fast<R, 4, 2, N>, (const void *)hs_scan_fast<R, 4, 3, N>,                     \
+     (const void *)hs_scan_fast<R, 4, 4, N>}
+            static const void *const kScan4[2][2][kNumAggSigs] = {{PMH_SCAN4(false, false), PMH_SCAN4(false, true)},
+                                                                    {PMH_SCAN4(true, false), PMH_SCAN4(true, true)}};
+#undef PMH_SCAN4
+            static_assert(kNumAggSigs == 5, "one instantiation per signature");
             const int sig = agg_signature(q0);  // the fast scan takes at most two value columns
-            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0> : (const void *)hs_scan_fast<false, 8, 0>)
-                                : kScan4[raw ? 1 : 0][sig];
+            const bool nr2 = L.fs.nr > 1;
+            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0, true> : (const void *)hs_scan_fast<false, 8, 0, true>)
+                                : kScan4[raw ? 1 : 0][nr2 ? 1 : 0][sig];
             PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
             void *args[] = {(void *)&dL};
             PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));

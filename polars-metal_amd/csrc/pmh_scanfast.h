@@ -39,8 +39,10 @@ constexpr uint32_t kRing = 127;
 
 
 // SIG: the query's compile-time aggregate signature (0: resolved at run
-// time), so the fold of hot rows carries no branch on the plan.
-template <bool RAW64, int U, int SIG>
+// time), so the fold of hot rows carries no branch on the plan.  NR2: the
+// predicate may test two columns (else at most one, and the second
+// column's descriptor takes no registers).
+template <bool RAW64, int U, int SIG, bool NR2>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_cursor, s_sel;
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     const uint32_t r_last = span ? span - 1 : 0;  // in-range stand-in for rows past the end
 
     // ---- descriptor, once per block
-    const int nr = L->fs.nr;
+    const int nr = NR2 ? L->fs.nr : min(L->fs.nr, 1);
     BlockCol pc[2];
     int pflt[2];
     int64_t plo[2], phi[2];
@@ -97,6 +99,24 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
         vc[j] = block_col(role_col(q, in ? q->vals[j].col : 0), r_begin);
         vwid[j] = in ? L->vw[j] : 8;
         sval[j] = in ? (void *)((uint8_t *)L->spill.val[j] + r_begin * (uint64_t)L->vw[j]) : nullptr;
+    }
+    // a signature fixes the value columns' tags and a raw key is 8 bytes, so
+    // their widths, shifts and masks are constants rather than live registers
+    if constexpr (SIG > 0) {
+        constexpr AggSig G = kAggSigs[SIG];
+#pragma unroll
+        for (int j = 0; j < G.nv; ++j) {
+            const int t = j ? G.tag1 : G.tag0;
+            const int w = tag_size(t);
+            vc[j].tag = t;
+            vc[j].bshift = w == 8 ? 6 : w == 4 ? 5 : w == 2 ? 4 : 3;
+            vc[j].mask = w == 8 ? ~0ull : (1ull << (8 * w)) - 1;
+            vwid[j] = w;
+        }
+    }
+    if constexpr (RAW64) {
+        kc[0].bshift = 6;
+        kc[0].mask = ~0ull;
     }
     uint64_t *sklo = L->spill.klo + r_begin;
     uint16_t *sflags = L->spill.flags + r_begin;

@@ -892,15 +892,25 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
         if (threadIdx.x < kRadix) run[threadIdx.x] = 0;
         __syncthreads();
         uint32_t my_d[kPartRows], my_pos[kPartRows];
+        // every row of the tile (keys, flags, the first two value columns)
+        // loads before the first rank: one wait on memory per tile
+        uint64_t klo[kPartRows], khi[kPartRows], v0[kPartRows], v1[kPartRows];
+        uint16_t fl[kPartRows];
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + threadIdx.x;
+            const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
+            klo[k] = gld<uint64_t>(src.klo, s);
+            khi[k] = wide ? gld<uint64_t>(src.khi, s) : 0;
+            fl[k] = gld<uint16_t>(src.flags, s);
+            v0[k] = nv > 0 ? gload_w(src.val[0], L->vw[0], s) : 0;
+            v1[k] = nv > 1 ? gload_w(src.val[1], L->vw[1], s) : 0;
+        }
+#pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + threadIdx.x;
             const bool live = i < ts.rows;
-            uint32_t d = 0;
-            if (live) {
-                const uint64_t s = ts.src_begin + i;
-                d = pass_digit(gld<uint64_t>(src.klo, s), wide ? gld<uint64_t>(src.khi, s) : 0,
-                               gld<uint16_t>(src.flags, s), shift);
-            }
+            const uint32_t d = live ? pass_digit(klo[k], khi[k], fl[k], shift) : 0;
             // multisplit: lanes with the same digit
             unsigned long long peers = __ballot(live);
 #pragma unroll
@@ -941,19 +951,20 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
         }
         __syncthreads();
         // stage in digit order
+#pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             if (my_pos[k] == 0xFFFFFFFFu) continue;
             const uint32_t i = k * kPartThreads + threadIdx.x;
             const uint64_t s = ts.src_begin + i;
             const uint32_t sp = dstart[my_d[k]] + my_pos[k];
-            st_lo[sp] = gld<uint64_t>(src.klo, s);
-            if (wide) st_hi[sp] = gld<uint64_t>(src.khi, s);
-            st_fl[sp] = gld<uint16_t>(src.flags, s);
+            st_lo[sp] = klo[k];
+            if (wide) st_hi[sp] = khi[k];
+            st_fl[sp] = fl[k];
             st_d[sp] = (uint8_t)my_d[k];
             uint8_t *vp = st_v;
             for (int j = 0; j < nv; ++j) {
                 const int w = L->vw[j];
-                store_w(vp, w, sp, gload_w(src.val[j], w, s));
+                store_w(vp, w, sp, j == 0 ? v0[k] : j == 1 ? v1[k] : gload_w(src.val[j], w, s));
                 vp += (size_t)w * kPartTile;
             }
         }
@@ -1073,7 +1084,9 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
         tile_span(L, true, t, ntiles, total, ts);
         run[tid] = 0;
         __syncthreads();
-        uint64_t klo[kPartRows], khi[kPartRows];
+        // the first two value columns load with the keys, so the tile waits
+        // on memory once before ranking instead of again while staging
+        uint64_t klo[kPartRows], khi[kPartRows], v0[kPartRows], v1[kPartRows];
         uint16_t fl[kPartRows];
         uint32_t dg[kPartRows], rk[kPartRows];
 #pragma unroll
@@ -1083,6 +1096,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
             klo[k] = gld<uint64_t>(src.klo, s);
             khi[k] = wide ? gld<uint64_t>(src.khi, s) : 0;
             fl[k] = gld<uint16_t>(src.flags, s);
+            v0[k] = nv > 0 ? gload_w(src.val[0], L->vw[0], s) : 0;
+            v1[k] = nv > 1 ? gload_w(src.val[1], L->vw[1], s) : 0;
         }
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
@@ -1109,7 +1124,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
             uint8_t *vp = st_v;
             for (int j = 0; j < nv; ++j) {
                 const int w = L->vw[j];
-                store_w(vp, w, sp, gload_w(src.val[j], w, s));
+                store_w(vp, w, sp, j == 0 ? v0[k] : j == 1 ? v1[k] : gload_w(src.val[j], w, s));
                 vp += (size_t)w * kPartTile;
             }
         }

@@ -81,6 +81,7 @@ struct HLaunch {
     uint32_t n_sample, hot_slots, hot_max, pad0;
     uint64_t *samp_lo, *samp_hi;
     uint8_t *samp_sp;               // side-key marker of each sampled key
+    uint64_t *skey_lo, *skey_hi;    // key of each fingerprint slot (written by the row that claims it)
     uint32_t sample_slots;          // LDS fingerprint-table size of hs_sample_count
     uint32_t pad2;
     uint64_t *hot_lo, *hot_hi;
@@ -345,6 +346,11 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
                 if (cur == 0) {
                     const uint32_t old = atomicCAS(&s_fp[s], 0u, fp);
                     cur = old == 0u ? fp : old;
+                    if (old == 0u) {
+                        // the claimer records the slot's key for the placement
+                        gst<uint64_t>(L->skey_lo, s, lo[u]);
+                        if (wide) gst<uint64_t>(L->skey_hi, s, hi[u]);
+                    }
                 }
                 if (cur == fp) { atomicAdd(&s_cnt[s], 1u); done = true; break; }
                 s = (s + 1) & (SS - 1);
@@ -392,14 +398,40 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         L->stats[ST_SAMPLE_DROPPED] = s_drop;
     }
     __syncthreads();
+    // Picks and placement, one slot per thread: keys with count > T, and
+    // keys with count == T in wave order up to the remaining places (one
+    // atomic per wave ranks its lanes; SS is a multiple of the block size, so
+    // every lane runs every iteration).  A picked slot's key, recorded by the
+    // row that claimed the slot, goes to the first free slot of its first
+    // bucket, else of its second; a key that finds both full stays out of
+    // the hot set, and its rows all go to one tail partition, so the hottest
+    // keys are placed first: keys sampled at least 8 times in the picking
+    // pass, then 2..7 times, then once, each tier after a barrier.
     const uint32_t T = s_thresh;
-    // keys with count == T take the remaining places in wave order: one
-    // atomic per wave ranks its lanes (SS is a multiple of the block size, so
-    // every lane runs every iteration)
     const uint32_t lane = tid & 63;
+    const uint32_t nbk = CL ? hot_buckets(CL) : 0;
+    uint32_t hot_rows = 0;
+    auto place = [&](uint32_t i, uint32_t cnt) {
+        const uint64_t lo = gld<uint64_t>(L->skey_lo, i), hi = wide ? gld<uint64_t>(L->skey_hi, i) : 0;
+        const uint64_t h = hash_key(lo, hi);
+        for (int pass = 0; pass < 2; ++pass) {
+            const uint32_t b = (pass ? hot_b2(h, nbk) : hot_b1(h, nbk)) * 4;
+            for (int jj = 0; jj < 4; ++jj) {
+                const unsigned long long old =
+                    atomicCAS((unsigned long long *)&h_lo[b + jj], (unsigned long long)kEmpty, (unsigned long long)lo);
+                if (old == kEmpty) {
+                    if (wide) h_hi[b + jj] = hi;
+                    atomicAdd(&s_hot, 1u);
+                    hot_rows += cnt;
+                    return;
+                }
+            }
+        }
+    };
     for (uint32_t i0 = 0; i0 < SS && CL; i0 += bs) {
         const uint32_t i = i0 + tid;
-        const uint32_t c = min(s_cnt[i], 1023u);
+        const uint32_t cnt = s_cnt[i];
+        const uint32_t c = min(cnt, 1023u);
         bool pick = c && c > T;
         const bool eq = c && c == T;
         const unsigned long long m = __ballot(eq);
@@ -410,59 +442,17 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
             base = __shfl(base, leader);
             if (eq) pick = base + (uint32_t)__popcll(m & ((1ull << lane) - 1)) < s_take_eq;
         }
-        if (pick) s_cnt[i] |= 1u << 31;
+        if (pick && cnt >= 8) place(i, cnt);
+        else if (pick) s_cnt[i] = cnt | (1u << 31);
     }
     __syncthreads();
-    // recover the keys of picked fingerprints: the first sampled row of each
-    // picked fingerprint places its key in the hot table
-    uint32_t hot_rows = 0;
-    for (uint32_t i0 = tid; i0 < nsel && CL; i0 += bs * kSampleU) {
-        uint64_t lob[kSampleU], hib[kSampleU];
-        uint8_t spb[kSampleU];
-#pragma unroll
-        for (int u = 0; u < kSampleU; ++u) {
-            const uint32_t i = i0 + u * bs;
-            const uint32_t ii = i < nsel ? i : 0;
-            lob[u] = gld<uint64_t>(L->samp_lo, ii);
-            hib[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
-            spb[u] = gld<uint8_t>(L->samp_sp, ii);
-        }
-#pragma unroll
-        for (int u = 0; u < kSampleU; ++u) {
-            if (i0 + u * bs >= nsel) break;
-            if (spb[u]) continue;
-            const uint64_t lo = lob[u], hi = hib[u];
-            const uint64_t h = hash_key(lo, hi);
-            const uint32_t fp = (uint32_t)(h >> 32) | 1u;
-            uint32_t s = (uint32_t)h & (SS - 1);
-            for (int p = 0; p < kSampleProbe; ++p) {
-                const uint32_t cur = s_fp[s];
-                if (cur == 0) break;
-                if (cur == fp) {
-                    if ((s_cnt[s] >> 31) && !((atomicOr(&s_cnt[s], 1u << 30) >> 30) & 1)) {
-                        const uint32_t nbk = hot_buckets(CL);
-                        bool placed = false;
-                        for (int pass = 0; pass < 2 && !placed; ++pass) {
-                            const uint32_t b = (pass ? hot_b2(h, nbk) : hot_b1(h, nbk)) * 4;
-                            for (int jj = 0; jj < 4; ++jj) {
-                                const unsigned long long old = atomicCAS(
-                                    (unsigned long long *)&h_lo[b + jj], (unsigned long long)kEmpty, (unsigned long long)lo);
-                                if (old == kEmpty) {
-                                    if (wide) h_hi[b + jj] = hi;
-                                    atomicAdd(&s_hot, 1u);
-                                    hot_rows += s_cnt[s] & 0x3FFFFFFFu;
-                                    placed = true;
-                                    break;
-                                }
-                            }
-                        }
-                    }
-                    break;
-                }
-                s = (s + 1) & (SS - 1);
-            }
-        }
+    for (uint32_t i = tid; i < SS && CL; i += bs) {
+        const uint32_t v = s_cnt[i];
+        if ((v >> 31) && (v & 0x7FFFFFFFu) >= 2) place(i, v & 0x7FFFFFFFu);
     }
+    __syncthreads();
+    for (uint32_t i = tid; i < SS && CL; i += bs)
+        if (s_cnt[i] == ((1u << 31) | 1u)) place(i, 1);
     if (hot_rows) atomicAdd(&s_hotrows, hot_rows);
     __syncthreads();
     for (uint32_t i = tid; i < CL; i += bs) {
@@ -1761,6 +1751,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.samp_lo = (uint64_t *)slab("hs:samp_lo", (uint64_t)n_sample * 8 + 64);
     L.samp_hi = (uint64_t *)slab("hs:samp_hi", (uint64_t)n_sample * 8 + 64);
     L.samp_sp = (uint8_t *)slab("hs:samp_sp", (uint64_t)n_sample + 64);
+    L.skey_lo = (uint64_t *)slab("hs:skey_lo", (uint64_t)sample_slots * 8 + 64);
+    L.skey_hi = (uint64_t *)slab("hs:skey_hi", (uint64_t)sample_slots * 8 + 64);
     L.hot_lo = (uint64_t *)slab("hs:hot_lo", (uint64_t)CL * 8 + 64);
     L.hot_hi = (uint64_t *)slab("hs:hot_hi", (uint64_t)CL * 8 + 64);
     {

@@ -1185,7 +1185,7 @@ template <bool WIDE, int SIG>
 __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ L, const Spill *src_p) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint64_t s_range[2];
-    __shared__ uint32_t s_count, s_base;
+    __shared__ uint32_t s_base, s_pwave[16];
     const Spill &S = *src_p;
     const QueryDesc *__restrict__ q = as_const(&L->q);
     constexpr bool wide = WIDE;
@@ -1201,7 +1201,6 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
     const LdsAgg2 A = lds_agg2(q, state, L->lds3_off, nls);
     for (uint32_t p = blockIdx.x; p <= pmask; p += gridDim.x) {
         if (tid < 2) s_range[tid] = L->part_start[p + tid];
-        if (tid == 0) s_count = 0;
         for (uint32_t i = tid; i < nls; i += bs) {
             lk[i] = kEmpty;
             if (wide) { lh[i] = kEmpty; lst[i] = 0; }
@@ -1305,9 +1304,9 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
             const uint32_t i = it * bs + tid;
             if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) ++my;
         }
-        const uint32_t my_off = atomicAdd(&s_count, my);
-        __syncthreads();
-        if (tid == 0) s_base = s_count ? atomicAdd(L->out_cursor, s_count) : 0;
+        // thread-ordered offsets, so a wave's groups land side by side
+        const uint32_t my_off = block_excl_scan(my, s_pwave);
+        if (tid == bs - 1) s_base = my_off + my ? atomicAdd(L->out_cursor, my_off + my) : 0;
         __syncthreads();
         uint32_t o = s_base + my_off;
         for (uint32_t it = 0; it < iters; ++it) {

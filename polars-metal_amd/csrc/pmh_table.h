@@ -376,10 +376,18 @@ __device__ __forceinline__ void key_field(const QueryDesc *__restrict__ q, int f
     bits = valid ? v : 0;
 }
 
-// Value of aggregate output oc for the group in slot s of reader rd.
+// Exact f64 sums already converted for one group: the Sum and the Mean
+// output of a column share one conversion of its limbs.
+struct SumCache {
+    int idx[2] = {-1, -1};
+    double d[2] = {0.0, 0.0};
+};
+
+// Value of aggregate output oc for the group in slot s of reader rd (sc:
+// the group's converted sums, or null).
 template <class Rd, class Slot>
 __device__ __forceinline__ void agg_value(const QueryDesc *__restrict__ q, const OutCol &oc, const Rd &rd, Slot s,
-                                          uint64_t &bits, bool &valid) {
+                                          uint64_t &bits, bool &valid, SumCache *sc = nullptr) {
     const Layout &lay = q->layout;
     if (oc.idx < 0) { bits = rd.template get<uint32_t>(lay.a_len, s); valid = true; return; }
     const ValSpec &vs = q->vals[oc.idx];
@@ -393,10 +401,21 @@ __device__ __forceinline__ void agg_value(const QueryDesc *__restrict__ q, const
         double d = 0.0;
         int64_t si = 0;
         if (vs.kind == VK_F) {
-            int64_t limb[kLimbs];
+            if (sc && sc->idx[0] == oc.idx) {
+                d = sc->d[0];
+            } else if (sc && sc->idx[1] == oc.idx) {
+                d = sc->d[1];
+            } else {
+                int64_t limb[kLimbs];
 #pragma unroll
-            for (int j = 0; j < kLimbs; ++j) limb[j] = rd.template get<int64_t>(vs.a_limb + j, s);
-            d = limbs_to_double(limb, rd.template get<double>(vs.a_res, s));
+                for (int j = 0; j < kLimbs; ++j) limb[j] = rd.template get<int64_t>(vs.a_limb + j, s);
+                d = limbs_to_double(limb, rd.template get<double>(vs.a_res, s));
+                if (sc) {
+                    const int e = sc->idx[0] < 0 ? 0 : 1;
+                    sc->idx[e] = oc.idx;
+                    sc->d[e] = d;
+                }
+            }
         } else {
             si = rd.template get<int64_t>(vs.a_sum, s);
         }
@@ -451,12 +470,13 @@ template <class Rd, class Slot>
 __device__ __forceinline__ void emit_group(const QueryDesc *__restrict__ q, const OutCol *outs, int n_out,
                                            const Rd &rd, Slot s, uint64_t lo, uint64_t hi, int special,
                                            uint64_t o) {
+    SumCache sc;
     for (int i = 0; i < n_out; ++i) {
         const OutCol &oc = outs[i];
         uint64_t bits;
         bool valid;
         if (oc.src == OS_KEY) key_field(q, oc.idx, lo, hi, special, bits, valid);
-        else agg_value(q, oc, rd, s, bits, valid);
+        else agg_value(q, oc, rd, s, bits, valid, &sc);
         store_tag(oc.data, o, oc.tag, bits);
         if (oc.vbytes) gst<uint8_t>(oc.vbytes, o, valid ? 1 : 0);
     }

@@ -309,13 +309,13 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     uint64_t *h_hi = h_lo + L->hot_slots;
     __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop, s_hotrows;
     __shared__ uint32_t s_wave[16];
-    __shared__ uint32_t s_eq_used;
+    __shared__ uint32_t s_eq_used, s_npick;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t CL = L->hot_slots;
     for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
     for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
     for (uint32_t i = tid; i < CL; i += bs) { h_lo[i] = kEmpty; if (wide) h_hi[i] = kEmpty; }
-    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; s_eq_used = 0; }
+    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; s_eq_used = 0; s_npick = 0; }
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
     uint32_t drops = 0, sidekeys = 0;
@@ -398,36 +398,16 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         L->stats[ST_SAMPLE_DROPPED] = s_drop;
     }
     __syncthreads();
-    // Picks and placement, one slot per thread: keys with count > T, and
-    // keys with count == T in wave order up to the remaining places (one
-    // atomic per wave ranks its lanes; SS is a multiple of the block size, so
-    // every lane runs every iteration).  A picked slot's key, recorded by the
-    // row that claimed the slot, goes to the first free slot of its first
-    // bucket, else of its second; a key that finds both full stays out of
-    // the hot set, and its rows all go to one tail partition, so the hottest
-    // keys are placed first: keys sampled at least 8 times in the picking
-    // pass, then 2..7 times, then once, each tier after a barrier.
+    // Picks, one slot per thread: keys with count > T, and keys with count
+    // == T in wave order up to the remaining places (one atomic per wave
+    // ranks its lanes; SS is a multiple of the block size, so every lane runs
+    // every iteration).  A picked slot joins a compact list (slot, sampled
+    // count) in the fingerprint array, whose counting is done (CL <= 2048
+    // entries of 8 B inside its 4 * SS >= 32 KB).
     const uint32_t T = s_thresh;
     const uint32_t lane = tid & 63;
-    const uint32_t nbk = CL ? hot_buckets(CL) : 0;
-    uint32_t hot_rows = 0;
-    auto place = [&](uint32_t i, uint32_t cnt) {
-        const uint64_t lo = gld<uint64_t>(L->skey_lo, i), hi = wide ? gld<uint64_t>(L->skey_hi, i) : 0;
-        const uint64_t h = hash_key(lo, hi);
-        for (int pass = 0; pass < 2; ++pass) {
-            const uint32_t b = (pass ? hot_b2(h, nbk) : hot_b1(h, nbk)) * 4;
-            for (int jj = 0; jj < 4; ++jj) {
-                const unsigned long long old =
-                    atomicCAS((unsigned long long *)&h_lo[b + jj], (unsigned long long)kEmpty, (unsigned long long)lo);
-                if (old == kEmpty) {
-                    if (wide) h_hi[b + jj] = hi;
-                    atomicAdd(&s_hot, 1u);
-                    hot_rows += cnt;
-                    return;
-                }
-            }
-        }
-    };
+    uint32_t *pk_slot = s_fp;
+    uint32_t *pk_cnt = s_fp + CL;
     for (uint32_t i0 = 0; i0 < SS && CL; i0 += bs) {
         const uint32_t i = i0 + tid;
         const uint32_t cnt = s_cnt[i];
@@ -442,17 +422,64 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
             base = __shfl(base, leader);
             if (eq) pick = base + (uint32_t)__popcll(m & ((1ull << lane) - 1)) < s_take_eq;
         }
-        if (pick && cnt >= 8) place(i, cnt);
-        else if (pick) s_cnt[i] = cnt | (1u << 31);
+        const unsigned long long pm = __ballot(pick);
+        if (pm) {
+            const int leader = __ffsll((long long)pm) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_npick, (uint32_t)__popcll(pm));
+            base = __shfl(base, leader);
+            const uint32_t e = base + (uint32_t)__popcll(pm & ((1ull << lane) - 1));
+            if (pick && e < CL) {
+                pk_slot[e] = i;
+                pk_cnt[e] = cnt;
+            }
+        }
     }
     __syncthreads();
-    for (uint32_t i = tid; i < SS && CL; i += bs) {
-        const uint32_t v = s_cnt[i];
-        if ((v >> 31) && (v & 0x7FFFFFFFu) >= 2) place(i, v & 0x7FFFFFFFu);
+    // the picked keys, recorded by the rows that claimed their slots, move
+    // into the count array, whose counts are now in the list (16 B per
+    // entry inside its 4 * SS bytes)
+    const uint32_t npick = min(s_npick, CL);
+    uint64_t *pk_lo = (uint64_t *)s_cnt;
+    uint64_t *pk_hi = pk_lo + CL;
+    for (uint32_t e = tid; e < npick; e += bs) {
+        const uint32_t i = pk_slot[e];
+        pk_lo[e] = gld<uint64_t>(L->skey_lo, i);
+        pk_hi[e] = wide ? gld<uint64_t>(L->skey_hi, i) : 0;
     }
     __syncthreads();
-    for (uint32_t i = tid; i < SS && CL; i += bs)
-        if (s_cnt[i] == ((1u << 31) | 1u)) place(i, 1);
+    // Placement: a key goes to the first free slot of its first bucket, else
+    // of its second; a key that finds both full stays out of the hot set, and
+    // its rows all go to one tail partition, so the hottest keys are placed
+    // first: sampled at least 8 times, then 2..7 times, then once, each tier
+    // after a barrier.
+    const uint32_t nbk = CL ? hot_buckets(CL) : 0;
+    uint32_t hot_rows = 0;
+    constexpr uint32_t kTierLo[3] = {8, 2, 1}, kTierHi[3] = {0xFFFFFFFFu, 7, 1};
+    for (int tier = 0; tier < 3; ++tier) {
+        for (uint32_t e = tid; e < npick; e += bs) {
+            const uint32_t cnt = pk_cnt[e];
+            if (cnt < kTierLo[tier] || cnt > kTierHi[tier]) continue;
+            const uint64_t lo = pk_lo[e], hi = pk_hi[e];
+            const uint64_t h = hash_key(lo, hi);
+            bool placed = false;
+            for (int pass = 0; pass < 2 && !placed; ++pass) {
+                const uint32_t b = (pass ? hot_b2(h, nbk) : hot_b1(h, nbk)) * 4;
+                for (int jj = 0; jj < 4; ++jj) {
+                    const unsigned long long old = atomicCAS((unsigned long long *)&h_lo[b + jj],
+                                                             (unsigned long long)kEmpty, (unsigned long long)lo);
+                    if (old == kEmpty) {
+                        if (wide) h_hi[b + jj] = hi;
+                        atomicAdd(&s_hot, 1u);
+                        hot_rows += cnt;
+                        placed = true;
+                        break;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
     if (hot_rows) atomicAdd(&s_hotrows, hot_rows);
     __syncthreads();
     for (uint32_t i = tid; i < CL; i += bs) {
@@ -1730,6 +1757,10 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     const uint32_t n_sample = (uint32_t)(opt.sample >= 0 ? std::min<int64_t>(opt.sample, 1 << 16)
                                                          : (int64_t)std::min<uint64_t>(n, 1 << 16));
     const uint32_t sample_slots = wide ? 8192 : 16384;
+    // hs_sample_count lays its pick list (8 B per hot slot) and the picked
+    // keys (16 B per hot slot) into its two 4 B-per-slot sample arrays
+    if ((size_t)CL * 16 > (size_t)sample_slots * 4)
+        fail(PMH_E_RUNTIME, "polars_metal: hot table larger than the sample table holds");
     const size_t tail_slot_bytes = key_bytes + (wide ? 4 : 0) + state_per_slot;
     uint32_t TS = (uint32_t)std::min<uint64_t>(4096, pow2_floor(lds_budget / tail_slot_bytes));
     while (TS > 16 && (size_t)(TS + 2) * tail_slot_bytes > lds_budget) TS /= 2;

@@ -6,8 +6,9 @@
 //
 //   hs_sample_gather  keys of a strided row sample that pass the predicate
 //   hs_sample_count   one block counts the sample in LDS and lays the most
-//                     frequent keys (count >= 2) out in a hot table shared by
-//                     every block; it also records the sample sketch
+//                     frequent keys out in a hot table shared by every block
+//                     (two-choice buckets of four slots, the most sampled
+//                     keys placed first); it also records the sample sketch
 //   hs_scan_fast      one block per CU streams a contiguous chunk of rows:
 //                     predicates lowered to at most two intervals on an
 //                     order-preserving int64 image, columns read as aligned
@@ -1895,10 +1896,56 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         launch_check();
     }
     // ---- partition count from the sketch (one small read-back)
+    // The scan reads nothing the planning below sets (partition bits,
+    // passes, per-partition table), so it is enqueued behind the sketch's
+    // copy and the host plans while it runs; the planned descriptor uploads
+    // behind it in stream order.  A query the learned sizer then routes to
+    // the sort engine has run one scan for nothing (its results are unused).
     uint32_t sk[ST_N];
     std::vector<uint32_t> hist(1024);
     PMH_HIP(hipMemcpyAsync(mbox, L.stats, (ST_N + 1024) * 4, hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipStreamSynchronize(R.stream));
+    hipEvent_t ev_sketch;
+    PMH_HIP(hipEventCreateWithFlags(&ev_sketch, hipEventDisableTiming));
+    PMH_HIP(hipEventRecord(ev_sketch, R.stream));
+    auto launch_scan = [&]() {
+        const bool cached = q0.n_cols <= 4;
+        const size_t l1 = std::max<size_t>(lds1, 16);
+        if (L.fs.enabled) {
+            KernelTimer kt("hs_scan_fast");
+            const bool raw = q0.key_mode == KM_RAW64;
+            const bool r8 = L.fs.rows == 8;
+            // four rows per lane: one instantiation per aggregate signature
+            // and predicate width; eight: the fold resolved at run time
+#define PMH_SCAN4(R, N)                                                                                 \
+    {(const void *)hs_scan_fast<R, 4, 0, N>, (const void *)hs_scan_fast<R, 4, 1, N>,                     \
+     (const void *)hs_scan_fast<R, 4, 2, N>, (const void *)hs_scan_fast<R, 4, 3, N>,                     \
+     (const void *)hs_scan_fast<R, 4, 4, N>}
+            static const void *const kScan4[2][2][kNumAggSigs] = {{PMH_SCAN4(false, false), PMH_SCAN4(false, true)},
+                                                                    {PMH_SCAN4(true, false), PMH_SCAN4(true, true)}};
+#undef PMH_SCAN4
+            static_assert(kNumAggSigs == 5, "one instantiation per signature");
+            const int sig = agg_signature(q0);  // the fast scan takes at most two value columns
+            const bool nr2 = L.fs.nr > 1;
+            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0, true> : (const void *)hs_scan_fast<false, 8, 0, true>)
+                                : kScan4[raw ? 1 : 0][nr2 ? 1 : 0][sig];
+            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+            void *args[] = {(void *)&dL};
+            PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));
+        } else {
+            KernelTimer kt("hs_scan");
+            const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;
+            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+            if (cached) hipLaunchKernelGGL(hs_scan<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
+            else hipLaunchKernelGGL(hs_scan<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
+        }
+        launch_check();
+    };
+    if (!opt.sketch_only) launch_scan();
+    {
+        const hipError_t e = hipEventSynchronize(ev_sketch);
+        (void)hipEventDestroy(ev_sketch);
+        PMH_HIP(e);
+    }
     memcpy(sk, mbox, sizeof(sk));
     memcpy(hist.data(), mbox + ST_N, 1024 * 4);
     const double tail_cap = 0.5 * TS;  // target keys per partition
@@ -1985,39 +2032,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
                          lds_layout(P, L.tail_slots + 2, L.lds3_off);
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
-    {
-        const bool cached = q0.n_cols <= 4;
-        const size_t l1 = std::max<size_t>(lds1, 16);
-        if (L.fs.enabled) {
-            KernelTimer kt("hs_scan_fast");
-            const bool raw = q0.key_mode == KM_RAW64;
-            const bool r8 = L.fs.rows == 8;
-            // four rows per lane: one instantiation per aggregate signature
-            // and predicate width; eight: the fold resolved at run time
-#define PMH_SCAN4(R, N)                                                                                 \
-    {(const void *)hs_scan_fast<R, 4, 0, N>, (const void *)hs_scan_fast<R, 4, 1, N>,                     \
-     (const void *)hs_scan_fast<R, 4, 2, N>, (const void *)hs_scan_fast<R, 4, 3, N>,                     \
-     (const void *)hs_scan_fast<R, 4, 4, N>}
-            static const void *const kScan4[2][2][kNumAggSigs] = {{PMH_SCAN4(false, false), PMH_SCAN4(false, true)},
-                                                                    {PMH_SCAN4(true, false), PMH_SCAN4(true, true)}};
-#undef PMH_SCAN4
-            static_assert(kNumAggSigs == 5, "one instantiation per signature");
-            const int sig = agg_signature(q0);  // the fast scan takes at most two value columns
-            const bool nr2 = L.fs.nr > 1;
-            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0, true> : (const void *)hs_scan_fast<false, 8, 0, true>)
-                                : kScan4[raw ? 1 : 0][nr2 ? 1 : 0][sig];
-            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
-            void *args[] = {(void *)&dL};
-            PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));
-        } else {
-            KernelTimer kt("hs_scan");
-            const void *fn = cached ? (const void *)hs_scan<true> : (const void *)hs_scan<false>;
-            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
-            if (cached) hipLaunchKernelGGL(hs_scan<true>, dim3(nb), dim3(1024), l1, R.stream, dL);
-            else hipLaunchKernelGGL(hs_scan<false>, dim3(nb), dim3(1024), l1, R.stream, dL);
-        }
-        launch_check();
-    }
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
     launch_check();
     const int pgrid = (cus * 2 + 7) / 8 * 8;  // a multiple of 8: the one-pass XCD slots

@@ -781,7 +781,8 @@ pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             }
         }
     }
-    PMH_HIP(hipStreamSynchronize(R.stream));
+    // host outputs wait for their copies; device outputs were complete at ev1
+    if (!(flags & PMH_OUT_DEVICE)) PMH_HIP(hipStreamSynchronize(R.stream));
     // Bool key data and bitmaps come back as whole 64-bit words; clear the
     // bits past the last group so padding is zero like pack_valid_bitmap's.
     if (!(flags & PMH_OUT_DEVICE)) {

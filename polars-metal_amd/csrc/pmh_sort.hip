@@ -1338,7 +1338,8 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             }
         }
     }
-    PMH_HIP(hipStreamSynchronize(R.stream));
+    // host outputs wait for their copies; device outputs were complete at ev1
+    if (!(flags & PMH_OUT_DEVICE)) PMH_HIP(hipStreamSynchronize(R.stream));
     collect_kernel_times();
     return res;
 }

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     kw[i] = 0;
                     kv[i] = ~0ull;
                     if (i >= nk) continue;
-                    kw[i] = word_at(kc[i].base, vpos(kc[i], row));
+                    kw[i] = word_at_nt(kc[i].base, vpos(kc[i], row));
                     if (kc[i].vstride) kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
                 }
 #pragma unroll
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     vw_[j] = 0;
                     vv_[j] = ~0ull;
                     if (j >= nv) continue;
-                    vw_[j] = word_at(vc[j].base, vpos(vc[j], row));
+                    vw_[j] = word_at_nt(vc[j].base, vpos(vc[j], row));
                     if (vc[j].vstride) vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
                 }
                 uint64_t key = 0;

@@ -79,6 +79,11 @@ __device__ __forceinline__ BlockCol block_col(const RoleCol &c, uint64_t r0) {
 __device__ __forceinline__ uint32_t vpos(const BlockCol &c, uint32_t r) { return c.bit0 + (r << c.bshift); }
 __device__ __forceinline__ uint32_t bpos(const BlockCol &c, uint32_t r) { return c.vbit0 + r * c.vstride; }
 __device__ __forceinline__ uint64_t word_at(const uint64_t *base, uint32_t pos) { return gld<uint64_t>(base, pos >> 6); }
+// The same load marked non-temporal: a word read once (a gathered key or
+// value) should not push out lines that are about to be read again.
+__device__ __forceinline__ uint64_t word_at_nt(const uint64_t *base, uint32_t pos) {
+    return __builtin_nontemporal_load((const PMH_G uint64_t *)base + (pos >> 6));
+}
 
 // Predicate images of U values of one column (raw bits, zero-extended): the
 // column's TotalOrd image when flt, else its int64 value.  The tag switch is

@@ -25,6 +25,11 @@ template <class T> __device__ __forceinline__ const T *as_const(const T *p) {
 }
 template <class T> __device__ __forceinline__ T gld(const void *p, uint64_t i) { return ((const PMH_G T *)p)[i]; }
 template <class T> __device__ __forceinline__ void gst(void *p, uint64_t i, T v) { ((PMH_G T *)p)[i] = v; }
+// A load marked non-temporal: for data read once (spill records), so it does
+// not push out lines still in use (a scatter's partially written runs).
+template <class T> __device__ __forceinline__ T gld_nt(const void *p, uint64_t i) {
+    return __builtin_nontemporal_load((const PMH_G T *)p + i);
+}
 template <class T> __device__ __forceinline__ PMH_G T *gp(void *p) { return (PMH_G T *)p; }
 template <class T> __device__ __forceinline__ const PMH_G T *gp(const void *p) { return (const PMH_G T *)p; }
 

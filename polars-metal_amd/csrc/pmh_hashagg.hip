@@ -146,6 +146,14 @@ __device__ __forceinline__ uint64_t gload_w(const void *p, int w, uint64_t i) {
     default: return gld<uint8_t>(p, i);
     }
 }
+__device__ __forceinline__ uint64_t gload_w_nt(const void *p, int w, uint64_t i) {
+    switch (w) {
+    case 8: return gld_nt<uint64_t>(p, i);
+    case 4: return gld_nt<uint32_t>(p, i);
+    case 2: return gld_nt<uint16_t>(p, i);
+    default: return gld_nt<uint8_t>(p, i);
+    }
+}
 __device__ __forceinline__ void gstore_w(void *p, int w, uint64_t i, uint64_t x) {
     switch (w) {
     case 8: gst<uint64_t>(p, i, x); break;
@@ -918,11 +926,11 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + threadIdx.x;
             const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
-            klo[k] = gld<uint64_t>(src.klo, s);
-            khi[k] = wide ? gld<uint64_t>(src.khi, s) : 0;
-            fl[k] = gld<uint16_t>(src.flags, s);
-            v0[k] = nv > 0 ? gload_w(src.val[0], L->vw[0], s) : 0;
-            v1[k] = nv > 1 ? gload_w(src.val[1], L->vw[1], s) : 0;
+            klo[k] = gld_nt<uint64_t>(src.klo, s);
+            khi[k] = wide ? gld_nt<uint64_t>(src.khi, s) : 0;
+            fl[k] = gld_nt<uint16_t>(src.flags, s);
+            v0[k] = nv > 0 ? gload_w_nt(src.val[0], L->vw[0], s) : 0;
+            v1[k] = nv > 1 ? gload_w_nt(src.val[1], L->vw[1], s) : 0;
         }
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
@@ -1037,8 +1045,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restri
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + tid;
             const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
-            d[k] = (uint32_t)spill_hash(gld<uint64_t>(src.klo, s), wide ? gld<uint64_t>(src.khi, s) : 0,
-                                        gld<uint16_t>(src.flags, s)) &
+            d[k] = (uint32_t)spill_hash(gld_nt<uint64_t>(src.klo, s), wide ? gld_nt<uint64_t>(src.khi, s) : 0,
+                                        gld_nt<uint16_t>(src.flags, s)) &
                    (nd - 1);
         }
 #pragma unroll
@@ -1111,11 +1119,11 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + tid;
             const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
-            klo[k] = gld<uint64_t>(src.klo, s);
-            khi[k] = wide ? gld<uint64_t>(src.khi, s) : 0;
-            fl[k] = gld<uint16_t>(src.flags, s);
-            v0[k] = nv > 0 ? gload_w(src.val[0], L->vw[0], s) : 0;
-            v1[k] = nv > 1 ? gload_w(src.val[1], L->vw[1], s) : 0;
+            klo[k] = gld_nt<uint64_t>(src.klo, s);
+            khi[k] = wide ? gld_nt<uint64_t>(src.khi, s) : 0;
+            fl[k] = gld_nt<uint16_t>(src.flags, s);
+            v0[k] = nv > 0 ? gload_w_nt(src.val[0], L->vw[0], s) : 0;
+            v1[k] = nv > 1 ? gload_w_nt(src.val[1], L->vw[1], s) : 0;
         }
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
@@ -1256,11 +1264,11 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
             for (int u = 0; u < U; ++u) {
                 const uint64_t i = i0 + (uint64_t)u * bs;
                 const uint64_t ii = i < e ? i : b;
-                flb[u] = gld<uint16_t>(S.flags, ii);
-                klb[u] = gld<uint64_t>(S.klo, ii);
-                khb[u] = wide ? gld<uint64_t>(S.khi, ii) : 0;
-                v0b[u] = nv > 0 ? gload_w(S.val[0], L->vw[0], ii) : 0;
-                v1b[u] = nv > 1 ? gload_w(S.val[1], L->vw[1], ii) : 0;
+                flb[u] = gld_nt<uint16_t>(S.flags, ii);
+                klb[u] = gld_nt<uint64_t>(S.klo, ii);
+                khb[u] = wide ? gld_nt<uint64_t>(S.khi, ii) : 0;
+                v0b[u] = nv > 0 ? gload_w_nt(S.val[0], L->vw[0], ii) : 0;
+                v1b[u] = nv > 1 ? gload_w_nt(S.val[1], L->vw[1], ii) : 0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {

@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kSortThreads) void so_hist(const SLaunch *__restric
         for (int k = 0; k < kSortRows; ++k) {
             const uint64_t i = b + (uint64_t)k * kSortThreads + threadIdx.x;
             if (i < L->n_sel) {
-                const uint32_t d = digit_of(gld<uint64_t>(klo, i), two ? gld<uint64_t>(khi, i) : 0, shift);
+                const uint32_t d = digit_of(gld_nt<uint64_t>(klo, i), two ? gld_nt<uint64_t>(khi, i) : 0, shift);
                 atomicAdd(&hist[wave][d], 1u);
             }
         }
@@ -533,19 +533,22 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
         uint32_t my_d[kSortRows], my_pos[kSortRows];
         uint64_t my_lo[kSortRows], my_hi[kSortRows];
         uint32_t my_row[kSortRows];
+        // every record of the tile loads before the first rank (one wait on
+        // memory per tile), non-temporally: each is read once, and the
+        // tile's runs are still being merged in L2
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k) {
+            const uint32_t i = k * kSortThreads + threadIdx.x;
+            const uint64_t src = b + (i < rows ? i : 0);  // rows >= 1
+            my_lo[k] = gld_nt<uint64_t>(slo, src);
+            my_hi[k] = two ? gld_nt<uint64_t>(shi, src) : 0;
+            my_row[k] = gld_nt<uint32_t>(srow, src);
+        }
 #pragma unroll
         for (int k = 0; k < kSortRows; ++k) {
             const uint32_t i = k * kSortThreads + threadIdx.x;
             const bool live = i < rows;
-            uint32_t d = 0;
-            my_lo[k] = my_hi[k] = 0;
-            my_row[k] = 0;
-            if (live) {
-                my_lo[k] = gld<uint64_t>(slo, b + i);
-                if (two) my_hi[k] = gld<uint64_t>(shi, b + i);
-                my_row[k] = gld<uint32_t>(srow, b + i);
-                d = digit_of(my_lo[k], my_hi[k], shift);
-            }
+            const uint32_t d = live ? digit_of(my_lo[k], my_hi[k], shift) : 0;
             // multisplit: lanes holding the same digit
             unsigned long long peers = __ballot(live);
 #pragma unroll

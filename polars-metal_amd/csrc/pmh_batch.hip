@@ -770,9 +770,10 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     std::vector<double> est(nq);
     std::vector<float> feats;
     const bool learned = sizer_installed() && opt.method == "auto";
+    // the batch sketch holds f1 and f2 only: a three-entry count histogram
+    std::vector<uint32_t> hist(3, 0);
     for (uint32_t i = 0; i < nq; ++i) {
         const uint32_t ns = (uint32_t)std::min<uint64_t>(bq[i].n_rows, kBSample);
-        std::vector<uint32_t> hist(1024, 0);
         hist[1] = sk[i * 5 + 2];
         hist[2] = sk[i * 5 + 3];
         Sketch k = sketch_from(sk[i * 5 + 0], ns, bq[i].n_rows, sk[i * 5 + 1], sk[i * 5 + 2], sk[i * 5 + 3], 0, hist);

@@ -1549,7 +1549,10 @@ Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, ui
     // F2 moment of the sample frequencies, and the Shlosser sums
     const double qf = k.n_sel > 0 ? std::min(1.0, s_sel / k.n_sel) : 1.0;
     double F2 = 0, num = 0, den = 0;
-    for (int c = 1; c < 1024; ++c) {
+    // hist[c]: sampled keys seen c times (c < 1024; a shorter histogram is
+    // zero past its end)
+    const int nc = (int)std::min<size_t>(hist.size(), 1024);
+    for (int c = 1; c < nc; ++c) {
         const double h = (double)hist[c] + (c == 1 ? (double)dropped : 0.0);
         if (h == 0) continue;
         F2 += (double)c * c * h;

@@ -483,3 +483,44 @@ def test_device_buffers_round_trip_and_copy(native):
         z.copy_from(src)
     for b in (src, dst, z):
         b.free()
+
+
+# Aggregate shapes of the specialised scan's fold, one per compile-time
+# signature (the first four) and one resolved at run time.
+SCAN_SIG_AGGS = [
+    [("v", "Sum"), ("v", "Mean"), ("w", "Min"), ("w", "Max")],  # f64 sum + i32 min/max
+    [("a", "Sum")],                                              # i64 sum
+    [("v", "Sum"), ("v", "Min"), ("v", "Max")],                  # f64 sum + min/max
+    [("v", "Sum")],                                              # f64 sum
+    [("a", "Sum"), ("a", "Min"), ("w", "Max")],                  # no signature
+]
+
+
+@pytest.mark.parametrize("key", ["I64", "I32"])
+@pytest.mark.parametrize("two_pred", [False, True])
+@pytest.mark.parametrize("sig", range(len(SCAN_SIG_AGGS)))
+def test_scan_signatures_hot_and_tail(native, sig, two_pred, key):
+    """Every instantiation of the specialised scan (raw 64-bit key or lane
+    key, one or two predicate columns, each aggregate signature) on skewed
+    keys, so rows fold into the hot table and spill to the partitions."""
+    rng = np.random.default_rng(100 + sig)
+    n = 300_000
+    k = rng.zipf(1.2, n) % 40_000
+    frame = {"k": (key, k.astype(np.int64 if key == "I64" else np.int32), None),
+             "v": ("F64", rng.standard_normal(n), rng.random(n) < 0.95),
+             "w": ("I32", rng.integers(0, 1000, n).astype(np.int32), None),
+             "a": ("I64", rng.integers(-10**12, 10**12, n), rng.random(n) < 0.9)}
+    pred = _cmp("Ge", _c("w", "I32"), _li(100), "I64")
+    if two_pred:
+        pred = _and(pred, _cmp("Gt", _c("v", "F64"), _lf(-1.5), "F64"))
+    plan = {"keys": [["k", key]], "predicate": pred,
+            "aggs": [{"kind": "Simple", "input_col": c, "op": op, "output_alias": f"{c}_{op}"}
+                     for c, op in SCAN_SIG_AGGS[sig]],
+            "options": {"method": "hash"}}
+    want = orc.groupby(plan, frame)
+    print(f"scan signature {sig} two_pred={two_pred} key={key}", flush=True)
+    got = run_gpu(native, plan, frame)
+    st = native.last_stats()
+    assert st["scan_kernel"] == "specialised"
+    assert st["hot_keys"] > 0 and st["tail_rows"] > 0
+    assert_results_equal(got, want, ["k"], label=f"sig{sig}/{two_pred}/{key}", zero_sign_free=("v_Min", "v_Max"))

@@ -411,7 +411,14 @@ def test_filter_compact(native):
                 "b": ("F64", rng.standard_normal(n), None),
                 "c": ("Bool", rng.random(n) < 0.5, rng.random(n) < 0.7),
                 "d": ("I32", rng.integers(-5, 5, n).astype(np.int32), None),
-                "e": ("U8", rng.integers(0, 255, n).astype(np.uint8), rng.random(n) < 0.5)}
+                "e": ("U8", rng.integers(0, 255, n).astype(np.uint8), rng.random(n) < 0.5),
+                # every other fixed width (the reference compacts only i64 / f64 / bool)
+                "f": ("F32", rng.standard_normal(n).astype(np.float32), rng.random(n) < 0.8),
+                "g": ("I8", rng.integers(-128, 127, n).astype(np.int8), None),
+                "h": ("I16", rng.integers(-30000, 30000, n).astype(np.int16), rng.random(n) < 0.6),
+                "i": ("U16", rng.integers(0, 65535, n).astype(np.uint16), None),
+                "j": ("U32", rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32), rng.random(n) < 0.9),
+                "l": ("U64", rng.integers(0, 1 << 63, n, dtype=np.uint64) * 2 + 1, None)}
         out = native.execute_filter_compact(orc.pack_bits(pd).tobytes(), orc.pack_bits(pv).tobytes(), n,
                                             orc.to_wire(cols))
         keep = pd & pv

@@ -500,6 +500,8 @@ struct BRec {
 // table, then emit its groups into the query's output rows.  The table has
 // cl live slots, the smallest power of two holding twice the partition's
 // records (at most part_slots), then the all-ones and null side slots.
+// SIG: the plan's compile-time aggregate signature (0: resolved at run time).
+template <int SIG>
 __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, const uint32_t *lds_off) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_count, s_base;
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
                     }
                 }
                 if (slot >= 0) {
-                    lds_agg2_add(A, (uint32_t)slot, fl[u], v0[u], v1[u]);
+                    lds_agg2_add_sig<SIG>(A, (uint32_t)slot, fl[u], v0[u], v1[u]);
                 } else {
                     const Key k{lane, (uint64_t)qi, 0};
                     if (!L->q_fb[qi]) L->q_fb[qi] = 1;
@@ -882,16 +884,22 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     PMH_HIP(hipMemcpyAsync(L.part_q, part_q.data(), n_parts * 4, hipMemcpyHostToDevice, R.stream));
     PMH_HIP(hipMemcpyAsync(d_lds_off, lds_off_h, sizeof(lds_off_h), hipMemcpyHostToDevice, R.stream));
     PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(BLaunch), hipMemcpyHostToDevice, R.stream));
-    PMH_HIP(hipFuncSetAttribute((const void *)bt_agg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_agg));
+    // the fold is specialised by the plan's aggregate signature
+    static const void *const kBtAgg[] = {(const void *)bt_agg<0>, (const void *)bt_agg<1>, (const void *)bt_agg<2>,
+                                         (const void *)bt_agg<3>, (const void *)bt_agg<4>};
+    static_assert(sizeof(kBtAgg) / sizeof(kBtAgg[0]) == kNumAggSigs, "one instantiation per signature");
+    const int sig = agg_signature(q0);
+    const void *agg_fn = kBtAgg[sig];
+    PMH_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_agg));
     // ---- the chain: captured once per buffer layout, replayed after
     const uint32_t grid = (uint32_t)cus * 4;
     const uint32_t tile_block = L.tile_rows / 16;
     // a table of at most 512 slots leaves room for several blocks per CU
     const uint32_t agg_block = CL >= 1024 ? 1024 : 512;
     char keybuf[512];
-    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u", (void *)dL, (void *)L.rk,
+    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u %d", (void *)dL, (void *)L.rk,
              (void *)L.part_cnt, (void *)L.part_off, (void *)L.t.klo, (void *)L.out[0].data, (void *)d_lds_off, lds_agg,
-             grid, (unsigned)P.n_out, tile_block, agg_block);
+             grid, (unsigned)P.n_out, tile_block, agg_block, sig);
     bool reused = g_graph.exec && g_graph.key == keybuf;
     if (!reused) {
         if (g_graph.exec) {
@@ -906,7 +914,11 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipLaunchKernelGGL(bt_scan_blocks, dim3(1), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_add, dim3(grid), dim3(256), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_agg, dim3(grid), dim3(agg_block), lds_agg, R.stream, dL, (const uint32_t *)d_lds_off);
+        {
+            const uint32_t *lo = d_lds_off;
+            void *args[] = {(void *)&dL, (void *)&lo};
+            PMH_HIP(hipLaunchKernel(agg_fn, dim3(grid), dim3(agg_block), args, lds_agg, R.stream));
+        }
         hipLaunchKernelGGL(bt_fb_emit, dim3(grid), dim3(256), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_pack, dim3(grid), dim3(256), 0, R.stream, dL);
         PMH_HIP(hipStreamEndCapture(R.stream, &graph));

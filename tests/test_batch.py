@@ -124,3 +124,40 @@ def test_batch_autotune_and_sequential_agree(native):
     _check(frames, queries, results, label="tuned/")
     seq = batch.sequential(_plan(), queries)
     assert [s["n_groups"] for s in seq] == [r.n_rows for r in results]
+
+
+# One aggregate shape per compile-time signature of the batch fold (the
+# first four) and one resolved at run time.
+BATCH_SIG_AGGS = [
+    [("v", "Sum"), ("v", "Mean"), ("w", "Min"), ("w", "Max")],
+    [("a", "Sum")],
+    [("v", "Sum"), ("v", "Min"), ("v", "Max")],
+    [("v", "Sum")],
+    [("a", "Sum"), ("a", "Min"), ("w", "Max")],
+]
+
+
+@pytest.mark.parametrize("sig", range(len(BATCH_SIG_AGGS)))
+def test_batch_aggregate_signatures(native, sig):
+    from polars_metal import batch
+    from polars_metal.buffer import device_columns
+    rng = np.random.default_rng(40 + sig)
+    plan = {"keys": [["key", "I64"]],
+            "aggs": [{"kind": "Simple", "input_col": c, "op": op, "output_alias": f"{c}_{op.lower()}"}
+                     for c, op in BATCH_SIG_AGGS[sig]]}
+    frames, queries, bufs = [], [], []
+    for _ in range(24):
+        rows = int(np.exp(rng.uniform(np.log(1e3), np.log(3e4))))
+        f = {"key": ("I64", rng.integers(1, max(2, rows // 10) + 1, rows), None),
+             "v": ("F64", rng.standard_normal(rows), rng.random(rows) >= 0.05),
+             "w": ("I32", rng.integers(-1000, 1000, rows).astype(np.int32), None),
+             "a": ("I64", rng.integers(-10**12, 10**12, rows), rng.random(rows) >= 0.1)}
+        cols, b = device_columns(f)
+        frames.append(f)
+        queries.append((rows, cols, None))
+        bufs.append(b)
+    results, st = batch.execute_batch(plan, queries)
+    assert st["n_queries"] == 24
+    for i, (f, r) in enumerate(zip(frames, results)):
+        assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"sig{sig}/q{i}",
+                             zero_sign_free=("v_min", "v_max"))

@@ -263,7 +263,8 @@ def bench_batch(args) -> dict:
                                "U[0,1), 20% of queries filter v>0.5; sum/count/min/max/mean of v",
                    "rows": rows, "groups": n_groups, "sequential_ms": t_seq * 1e3,
                    "speedup_vs_sequential": t_seq / dt, "rows_per_s": rows / dt, "launch_shape": opts,
-                   "device_ms": st["device_ms"], "n_rerun": st["n_rerun"]},
+                   "device_ms": st["device_ms"], "n_rerun": st["n_rerun"],
+                   "n_fallback_keys": st["n_fallback_keys"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "bt_graph (the captured chain)",
                      "kernel_avg_ms": graph_ms, "algorithmic_bytes_per_launch": alg},

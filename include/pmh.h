@@ -281,7 +281,7 @@ typedef struct pmh_batch_stats {
     uint32_t tile_rows, part_slots;
     int32_t graph_reused;    /* 1: the captured graph of an earlier batch was replayed */
     int32_t learned;         /* 1: the installed sizer sized the partitions */
-    int32_t pad;
+    uint32_t n_fallback_keys; /* keys that missed their partition's LDS table (global fallback table) */
 } pmh_batch_stats;
 
 /* Runs n_queries filtered group-bys that share template_plan_json's keys

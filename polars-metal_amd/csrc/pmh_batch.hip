@@ -40,7 +40,8 @@ constexpr int kBCols = 8;           // columns of the template a batch may refer
 constexpr uint32_t kBMaxParts = 4096;  // partitions per query
 constexpr int kBSample = 2048;      // sampled rows per query for the sketch
 constexpr int kBSampleSlots = 4096; // its LDS table
-constexpr int kBProbe = 16;         // LDS table probes before a key goes to the fallback table
+constexpr uint32_t kBMaxSlots = 4096;  // LDS table slots of a (query, partition) work item, at most
+constexpr int kBProbe = 32;         // LDS table probes (triangular steps) before a key goes to the fallback table
 constexpr int kBKeys = 4;           // key columns of a batch whose key lane is packed (lane mode 0)
 
 struct BQuery {
@@ -504,7 +505,8 @@ struct BRec {
 template <int SIG>
 __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, const uint32_t *lds_off) {
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ uint32_t s_count, s_base;
+    __shared__ uint32_t s_count, s_base, s_pwave[16];
+    __shared__ uint16_t s_live[kBMaxSlots + 2];  // live slots of the partition, dense
     const QueryDesc *__restrict__ q = &L->q;
     const uint32_t CL = L->part_slots;
     const uint32_t nls_max = CL + 2;
@@ -534,7 +536,6 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
                 for (uint32_t i = tid; i < nls; i += bs) *ls.p<uint64_t>(a, i) = id;
             }
         }
-        if (tid == 0) s_count = 0;
         __syncthreads();
         for (uint64_t i0 = b + tid; i0 < e; i0 += (uint64_t)U * bs) {
             uint64_t ln[U], v0[U], v1[U];
@@ -571,7 +572,10 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
                                                                      (unsigned long long)kEmpty, (unsigned long long)lane);
                             if (old == kEmpty || old == lane) { slot = (int32_t)s; break; }
                         }
-                        s = (s + 1) & (cl - 1);
+                        // triangular steps visit every slot of the power-of-two
+                        // table and break up the runs linear steps pile into at
+                        // the 0.8 load partitions are sized for
+                        s = (s + (uint32_t)pr + 1) & (cl - 1);
                     }
                 }
                 if (slot >= 0) {
@@ -589,21 +593,27 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
         const BQuery *__restrict__ Qp = L->qs + qi;
         const uint64_t out_cap = Qp->out_cap, out_base = Qp->out_base;
         const int a_len = q->layout.a_len;
+        // the live slots, compacted in thread order, so the groups are
+        // emitted by as few waves as they fill and in one pass
         uint32_t my = 0;
         for (uint32_t i = tid; i < nls; i += bs) my += *ls.p<uint32_t>(a_len, i) != 0;
-        const uint32_t my_off = atomicAdd(&s_count, my);
+        uint32_t pos = block_excl_scan(my, s_pwave);
+        if (tid == bs - 1) {
+            s_count = pos + my;
+            s_base = pos + my ? atomicAdd(&L->out_cursor[qi], pos + my) : 0;
+        }
+        for (uint32_t i = tid; i < nls; i += bs)
+            if (*ls.p<uint32_t>(a_len, i) != 0) s_live[pos++] = (uint16_t)i;
         __syncthreads();
-        if (tid == 0) s_base = s_count ? atomicAdd(&L->out_cursor[qi], s_count) : 0;
-        __syncthreads();
-        uint64_t o = (uint64_t)s_base + my_off;
+        const uint32_t n_live = s_count;
+        const uint64_t base = s_base;
         LReader rd{state, lds_off};
-        for (uint32_t i = tid; i < nls; i += bs) {
-            if (*ls.p<uint32_t>(a_len, i) == 0) continue;
-            if (o < out_cap) {
+        for (uint32_t j = tid; j < n_live; j += bs) {
+            const uint32_t i = s_live[j];
+            if (base + j < out_cap) {
                 const int special = i >= cl ? (int)(i - cl) + 1 : 0;
-                emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], 0, special, out_base + o);
+                emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], 0, special, out_base + base + j);
             }
-            ++o;
         }
         __syncthreads();
     }
@@ -696,12 +706,13 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     L.tile_rows = (uint32_t)std::min<int64_t>(8192, std::max<int64_t>(1024, opt.tile > 0 ? opt.tile : 8192));
     L.tile_rows = pow2_ceil(L.tile_rows);  // 16 rows per thread of a 64..512-thread block
     // LDS table of a (query, partition) work item: the largest power of two
-    // up to 4096 that fits (or the option's size); partitions are sized for
-    // half of it
+    // up to kBMaxSlots that fits beside bt_agg's static LDS (its live-slot
+    // list, under 9 KB), or the option's size; partitions are sized for 0.8
+    // of it
     size_t state_per_slot = 0;
     for (int a = 0; a < q0.layout.n_arrs; ++a) state_per_slot += q0.layout.esz[a];
-    uint32_t CL = 4096;
-    while (CL > 256 && (size_t)(CL + 2) * (8 + state_per_slot) > lds_max - 1024) CL /= 2;
+    uint32_t CL = kBMaxSlots;
+    while (CL > 256 && (size_t)(CL + 2) * (8 + state_per_slot) > lds_max - 10240) CL /= 2;
     if (opt.tail_slots > 0) {
         uint32_t c = 256;
         while (c * 2 <= (uint64_t)opt.tail_slots && c * 2 <= CL) c *= 2;
@@ -1020,6 +1031,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         stats->part_slots = CL;
         stats->graph_reused = reused ? 1 : 0;
         stats->learned = learned ? 1 : 0;
+        stats->n_fallback_keys = (uint32_t)std::min<uint64_t>(fb_ctrl[0], L.t.list_cap);
     }
 }
 

@@ -194,33 +194,6 @@ struct SpillRow {
     __device__ __forceinline__ double f64(int c) const { return raw_to_f64(L->q.cols[c].tag, bits(c)); }
 };
 
-// Exclusive scan of one value per thread across a 1024-thread block;
-// s_wave holds 16 words.  Every thread of the block must call it.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *s_wave) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o);
-        if (lane >= o) inc += y;
-    }
-    if (lane == 63) s_wave[wave] = inc;
-    __syncthreads();
-    if (wave == 0) {
-        const uint32_t w = lane < 16 ? s_wave[lane] : 0;
-        uint32_t wi = w;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const uint32_t y = __shfl_up(wi, o);
-            if (lane >= o) wi += y;
-        }
-        if (lane < 16) s_wave[lane] = wi - w;
-    }
-    __syncthreads();
-    const uint32_t r = s_wave[wave] + inc - x;
-    __syncthreads();  // s_wave may be reused right after
-    return r;
-}
 
 // ---------------------------------------------------------------------------
 // sample

@@ -94,7 +94,7 @@ class PmhBatchStats(ctypes.Structure):
         ("part_slots", ctypes.c_uint32),
         ("graph_reused", ctypes.c_int32),
         ("learned", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("n_fallback_keys", ctypes.c_uint32),
     ]
 
 

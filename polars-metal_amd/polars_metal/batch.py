@@ -41,7 +41,8 @@ _tuned: dict = {}
 def _stats_dict(st: _lib.PmhBatchStats) -> dict:
     return {"device_ms": st.device_ms, "n_queries": st.n_queries, "n_tiles": st.n_tiles,
             "n_partitions": st.n_partitions, "n_rerun": st.n_rerun, "tile_rows": st.tile_rows,
-            "part_slots": st.part_slots, "graph_reused": bool(st.graph_reused), "learned": bool(st.learned)}
+            "part_slots": st.part_slots, "graph_reused": bool(st.graph_reused), "learned": bool(st.learned),
+            "n_fallback_keys": st.n_fallback_keys}
 
 
 def execute_batch(template_plan: dict, queries: Sequence[tuple], options: dict | None = None):

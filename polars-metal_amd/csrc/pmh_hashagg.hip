@@ -104,7 +104,8 @@ struct HLaunch {
     uint64_t *xcur;        // [8][kWideRadix]: next free row of each (XCD slot, digit) sub-region
     uint32_t max_tiles, pbits, sortbits, passes;
     // per-partition aggregation
-    uint32_t tail_slots, pad1;
+    uint32_t tail_slots;
+    uint32_t live_off;  // byte offset in hs_part_agg's LDS of its dense live-slot list (0: none)
     uint32_t lds3_off[kMaxArrs];
     GTable t;
     // outputs
@@ -1194,7 +1195,7 @@ template <bool WIDE, int SIG>
 __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ L, const Spill *src_p) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint64_t s_range[2];
-    __shared__ uint32_t s_base, s_pwave[16];
+    __shared__ uint32_t s_base, s_ntot, s_pwave[16];
     const Spill &S = *src_p;
     const QueryDesc *__restrict__ q = as_const(&L->q);
     constexpr bool wide = WIDE;
@@ -1315,16 +1316,38 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
         }
         // thread-ordered offsets, so a wave's groups land side by side
         const uint32_t my_off = block_excl_scan(my, s_pwave);
-        if (tid == bs - 1) s_base = my_off + my ? atomicAdd(L->out_cursor, my_off + my) : 0;
-        __syncthreads();
-        uint32_t o = s_base + my_off;
-        for (uint32_t it = 0; it < iters; ++it) {
-            const uint32_t i = it * bs + tid;
-            if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) {
+        if (tid == bs - 1) {
+            s_ntot = my_off + my;
+            s_base = my_off + my ? atomicAdd(L->out_cursor, my_off + my) : 0;
+        }
+        if (L->live_off) {
+            // the live slots listed densely, so the groups are emitted by as
+            // few waves as they fill, in one pass
+            uint16_t *live = (uint16_t *)(smem + L->live_off);
+            uint32_t pos = my_off;
+            for (uint32_t it = 0; it < iters; ++it) {
+                const uint32_t i = it * bs + tid;
+                if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) live[pos++] = (uint16_t)i;
+            }
+            __syncthreads();
+            const uint32_t n_live = s_ntot, base = s_base;
+            for (uint32_t j = tid; j < n_live; j += bs) {
+                const uint32_t i = live[j];
                 const int special = i >= CL ? (int)(i - CL) + 1 : 0;
                 emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], (special || !wide) ? 0 : lh[i],
-                           special, o);
-                ++o;
+                           special, base + j);
+            }
+        } else {
+            __syncthreads();
+            uint32_t o = s_base + my_off;
+            for (uint32_t it = 0; it < iters; ++it) {
+                const uint32_t i = it * bs + tid;
+                if (i < nls && *ls.p<uint32_t>(a_len, i) != 0) {
+                    const int special = i >= CL ? (int)(i - CL) + 1 : 0;
+                    emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], (special || !wide) ? 0 : lh[i],
+                               special, o);
+                    ++o;
+                }
             }
         }
         __syncthreads();
@@ -2012,8 +2035,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         uint32_t ts = (uint32_t)std::min<uint64_t>(TS, std::max<uint64_t>(256, pow2_ceil((uint64_t)(2.5 * per) + 1)));
         L.tail_slots = ts;
     }
-    const size_t lds3b = (size_t)(L.tail_slots + 2) * (key_bytes + (wide ? 4 : 0)) +
-                         lds_layout(P, L.tail_slots + 2, L.lds3_off);
+    size_t lds3b = (size_t)(L.tail_slots + 2) * (key_bytes + (wide ? 4 : 0)) +
+                   lds_layout(P, L.tail_slots + 2, L.lds3_off);
+    {
+        // the emit's dense list of live slots (2 bytes a slot), where it fits
+        // and costs no block per CU
+        const size_t off = (lds3b + 15) & ~size_t(15), with = off + (size_t)(L.tail_slots + 2) * 2;
+        auto per_cu = [&](size_t b) { return std::min<size_t>(2, lds_max / b); };
+        L.live_off = with <= lds_budget && per_cu(with) >= per_cu(lds3b) ? (uint32_t)off : 0;
+        if (L.live_off) lds3b = with;
+    }
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);

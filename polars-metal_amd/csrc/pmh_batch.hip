@@ -633,31 +633,40 @@ __global__ __launch_bounds__(256) void bt_fb_emit(const BLaunch *__restrict__ L)
 }
 
 // per query: validity bytes -> bitmaps at the query's own bit 0
-__global__ __launch_bounds__(256) void bt_pack(const BLaunch *__restrict__ L) {
+// Byte flags of rows [0, ng) at src into the bitmap at dst ((ng + 7) / 8
+// bytes): each wave takes 64-row chunks, one coalesced byte per lane, and
+// eight lanes write the ballot's eight bytes; four chunks' loads are issued
+// before their ballots.
+__device__ __forceinline__ void pack_flags(const uint8_t *src, uint8_t *dst, uint64_t ng) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t nbytes = (ng + 7) / 8, nchunks = (ng + 63) / 64;
+    constexpr int U = 4;
+    for (uint64_t c0 = wave; c0 < nchunks; c0 += (uint64_t)U * nw) {
+        uint8_t f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t r = (c0 + (uint64_t)u * nw) * 64 + lane;
+            f[u] = r < ng ? src[r] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t c = c0 + (uint64_t)u * nw;
+            const uint64_t m = __ballot(f[u] != 0);
+            if (c < nchunks && lane < 8 && c * 8 + lane < nbytes) dst[c * 8 + lane] = (uint8_t)(m >> (8 * lane));
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void bt_pack(const BLaunch *__restrict__ L) {
     for (uint32_t qi = blockIdx.x; qi < L->n_queries; qi += gridDim.x) {
         const BQuery &Q = L->qs[qi];
         const uint64_t ng = min<uint64_t>(L->out_cursor[qi], Q.out_cap);
-        const uint64_t nbytes = (ng + 7) / 8;
-        for (uint64_t i = threadIdx.x; i < nbytes; i += blockDim.x)
-            for (int o = 0; o < L->n_out; ++o) {
-                const OutCol &oc = L->out[o];
-                if (oc.vbytes && oc.valid) {
-                    uint8_t bt = 0;
-                    for (int k = 0; k < 8; ++k) {
-                        const uint64_t r = i * 8 + k;
-                        if (r < ng && oc.vbytes[Q.out_base + r]) bt |= (uint8_t)(1u << k);
-                    }
-                    ((uint8_t *)oc.valid)[Q.vb_base + i] = bt;
-                }
-                if (oc.src == OS_KEY && oc.tag == PMH_BOOL && oc.data_bits) {
-                    uint8_t bt = 0;
-                    for (int k = 0; k < 8; ++k) {
-                        const uint64_t r = i * 8 + k;
-                        if (r < ng && ((const uint8_t *)oc.data)[Q.out_base + r]) bt |= (uint8_t)(1u << k);
-                    }
-                    oc.data_bits[Q.vb_base + i] = bt;
-                }
-            }
+        for (int o = 0; o < L->n_out; ++o) {
+            const OutCol &oc = L->out[o];
+            if (oc.vbytes && oc.valid) pack_flags(oc.vbytes + Q.out_base, (uint8_t *)oc.valid + Q.vb_base, ng);
+            if (oc.src == OS_KEY && oc.tag == PMH_BOOL && oc.data_bits)
+                pack_flags((const uint8_t *)oc.data + Q.out_base, oc.data_bits + Q.vb_base, ng);
+        }
     }
 }
 
@@ -931,7 +940,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
             PMH_HIP(hipLaunchKernel(agg_fn, dim3(grid), dim3(agg_block), args, lds_agg, R.stream));
         }
         hipLaunchKernelGGL(bt_fb_emit, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_pack, dim3(grid), dim3(256), 0, R.stream, dL);
+        hipLaunchKernelGGL(bt_pack, dim3(grid), dim3(1024), 0, R.stream, dL);
         PMH_HIP(hipStreamEndCapture(R.stream, &graph));
         PMH_HIP(hipGraphInstantiate(&g_graph.exec, graph, nullptr, nullptr, 0));
         PMH_HIP(hipGraphDestroy(graph));

@@ -501,12 +501,18 @@ struct BRec {
 // table, then emit its groups into the query's output rows.  The table has
 // cl live slots, the smallest power of two holding twice the partition's
 // records (at most part_slots), then the all-ones and null side slots.
+// The whole table is set to its identities once per block; after that the
+// emit puts back each slot it emits, so every partition starts from a clean
+// table without a pass over all its slots.
 // SIG: the plan's compile-time aggregate signature (0: resolved at run time).
 template <int SIG>
 __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, const uint32_t *lds_off) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_count, s_base, s_pwave[16];
     __shared__ uint16_t s_live[kBMaxSlots + 2];  // live slots of the partition, dense
+    // per state array: LDS byte offset, identity, element bytes
+    __shared__ uint32_t s_aoff[kMaxArrs], s_asz[kMaxArrs];
+    __shared__ uint64_t s_aid[kMaxArrs];
     const QueryDesc *__restrict__ q = &L->q;
     const uint32_t CL = L->part_slots;
     const uint32_t nls_max = CL + 2;
@@ -517,7 +523,28 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
     const LdsAgg2 A = lds_agg2(q, state, lds_off, nls_max);
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const int nv = q->n_vals;
+    const int n_arrs = q->layout.n_arrs;
     constexpr int U = 4;
+    if (tid < (uint32_t)n_arrs) {
+        const int op = q->layout.init[tid];
+        s_aoff[tid] = lds_off[tid];
+        s_asz[tid] = (uint32_t)mo_size(op);
+        s_aid[tid] = mo_size(op) == 4 ? (uint64_t)(uint32_t)mo_identity(op) : mo_identity(op);
+    }
+    __syncthreads();
+    // slot i of every state array back to its identity
+    auto reset_slot = [&](uint32_t i) {
+        for (int a = 0; a < n_arrs; ++a) {
+            uint8_t *p = state + s_aoff[a];
+            if (s_asz[a] == 4) ((uint32_t *)p)[i] = (uint32_t)s_aid[a];
+            else ((uint64_t *)p)[i] = s_aid[a];
+        }
+    };
+    for (uint32_t i = tid; i < nls_max; i += bs) {
+        lk[i] = kEmpty;
+        reset_slot(i);
+    }
+    __syncthreads();
     for (uint64_t pi = blockIdx.x; pi < L->n_parts; pi += gridDim.x) {
         const uint64_t b = L->part_off[pi], e = L->part_off[pi + 1];
         if (b == e) continue;  // uniform: the whole block skips
@@ -525,18 +552,6 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
         uint32_t cl = 64;
         while (cl < CL && (uint64_t)cl < 2 * (e - b)) cl *= 2;
         const uint32_t nls = cl + 2;
-        for (uint32_t i = tid; i < nls; i += bs) lk[i] = kEmpty;
-        for (int a = 0; a < q->layout.n_arrs; ++a) {
-            const int op = q->layout.init[a];
-            if (mo_size(op) == 4) {
-                const uint32_t id = (uint32_t)mo_identity(op);
-                for (uint32_t i = tid; i < nls; i += bs) *ls.p<uint32_t>(a, i) = id;
-            } else {
-                const uint64_t id = mo_identity(op);
-                for (uint32_t i = tid; i < nls; i += bs) *ls.p<uint64_t>(a, i) = id;
-            }
-        }
-        __syncthreads();
         for (uint64_t i0 = b + tid; i0 < e; i0 += (uint64_t)U * bs) {
             uint64_t ln[U], v0[U], v1[U];
             uint16_t fl[U];
@@ -610,10 +625,13 @@ __global__ __launch_bounds__(1024) void bt_agg(const BLaunch *__restrict__ L, co
         LReader rd{state, lds_off};
         for (uint32_t j = tid; j < n_live; j += bs) {
             const uint32_t i = s_live[j];
-            if (base + j < out_cap) {
-                const int special = i >= cl ? (int)(i - cl) + 1 : 0;
+            const int special = i >= cl ? (int)(i - cl) + 1 : 0;
+            if (base + j < out_cap)
                 emit_group(q, L->out, L->n_out, rd, i, special ? 0 : lk[i], 0, special, out_base + base + j);
-            }
+            // the slot back to empty for the next partition (also when its
+            // group did not fit: the query re-runs alone)
+            lk[i] = kEmpty;
+            reset_slot(i);
         }
         __syncthreads();
     }
@@ -716,12 +734,12 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     L.tile_rows = pow2_ceil(L.tile_rows);  // 16 rows per thread of a 64..512-thread block
     // LDS table of a (query, partition) work item: the largest power of two
     // up to kBMaxSlots that fits beside bt_agg's static LDS (its live-slot
-    // list, under 9 KB), or the option's size; partitions are sized for 0.8
-    // of it
+    // list and state-array table, under 10 KB), or the option's size;
+    // partitions are sized for 0.8 of it
     size_t state_per_slot = 0;
     for (int a = 0; a < q0.layout.n_arrs; ++a) state_per_slot += q0.layout.esz[a];
     uint32_t CL = kBMaxSlots;
-    while (CL > 256 && (size_t)(CL + 2) * (8 + state_per_slot) > lds_max - 10240) CL /= 2;
+    while (CL > 256 && (size_t)(CL + 2) * (8 + state_per_slot) > lds_max - 12288) CL /= 2;
     if (opt.tail_slots > 0) {
         uint32_t c = 256;
         while (c * 2 <= (uint64_t)opt.tail_slots && c * 2 <= CL) c *= 2;

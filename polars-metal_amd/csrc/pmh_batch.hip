@@ -321,12 +321,17 @@ __global__ void bt_fill(const BLaunch *__restrict__ L) {
 // partition); the scatter pass writes them partition-contiguous as records
 // (key lane, value bits, flags).  A thread takes 16 rows of its tile (at
 // most 8192 rows, 512 threads) in two passes of 8, each pass issuing all its
-// loads before using any.
+// loads before using any.  The scatter pass ranks the tile's rows within
+// their partitions, lists them in partition order in LDS (dynamic, 4 bytes a
+// row: partition << 13 | row in tile), then walks that list re-reading each
+// row (its lines are in L2 from the ranking), so consecutive lanes write
+// consecutive records of a partition's run.
 template <bool SCATTER>
 __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
-    __shared__ uint32_t hist[kBMaxParts];
-    __shared__ uint64_t base[SCATTER ? kBMaxParts : 1];
-    __shared__ uint32_t s_sel;
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t hist[kBMaxParts];  // counts; then (scatter) each run's start in the list
+    __shared__ uint32_t base[SCATTER ? kBMaxParts : 1];  // each run's start in the query's records
+    __shared__ uint32_t s_sel, s_wave[16];
     constexpr int U = 8, kPass = 2;
     const int nv = L->q.n_vals;
     const int vtag[2] = {L->q.cols[L->q.vals[0].col].tag, nv > 1 ? L->q.cols[L->q.vals[1].col].tag : 0};
@@ -376,19 +381,48 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
                 if (hist[p]) atomicAdd(&L->part_cnt[T.part_base + p], hist[p]);
             if (tid == 0 && s_sel) atomicAdd(&L->q_sel[qi], s_sel);
         } else {
-            for (uint32_t p = tid; p < T.P; p += bs)
-                if (hist[p])
-                    base[p] = atomicAdd((unsigned long long *)&L->part_cur[T.part_base + p], (unsigned long long)hist[p]);
+            uint32_t *list = (uint32_t *)smem;  // tile_rows entries
+            // runs claimed in the query's records; list starts by a block
+            // scan of the counts, each thread over a chunk of partitions
+            const uint64_t qoff = L->part_off[T.part_base];
+            const uint32_t per = (T.P + bs - 1) / bs;
+            const uint32_t p0 = min(T.P, tid * per), p1 = min(T.P, p0 + per);
+            uint32_t sum = 0;
+            for (uint32_t p = p0; p < p1; ++p) {
+                const uint32_t h = hist[p];
+                if (h)
+                    base[p] = (uint32_t)(atomicAdd((unsigned long long *)&L->part_cur[T.part_base + p],
+                                                   (unsigned long long)h) -
+                                         qoff);
+                sum += h;
+            }
+            uint32_t acc = block_excl_scan(sum, s_wave);
+            for (uint32_t p = p0; p < p1; ++p) {
+                const uint32_t h = hist[p];
+                hist[p] = acc;
+                acc += h;
+            }
             __syncthreads();
 #pragma unroll
             for (int ps = 0; ps < kPass; ++ps) {
-                if ((uint32_t)ps * U * bs >= tile_rows) continue;
-                uint64_t r[U], lane[U], vw[2][U];
-                uint32_t sp[U], vb[2][U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint64_t rr = r0 + (uint64_t)(ps * U + u) * bs + tid;
-                    r[u] = pr[ps][u] != ~0u ? rr : r0;
+                    const uint32_t x = pr[ps][u];
+                    if (x == ~0u) continue;
+                    const uint32_t p = (x >> 16) & 0xFFFu;
+                    list[hist[p] + (x & 0xFFFFu)] = (p << 13) | ((uint32_t)(ps * U + u) * bs + tid);
+                }
+            }
+            __syncthreads();
+            const uint32_t nsel = s_sel;
+            for (uint32_t s0 = 0; s0 < nsel; s0 += U * bs) {
+                uint64_t r[U], lane[U], vw[2][U];
+                uint32_t sp[U], vb[2][U], ent[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t s = s0 + (uint32_t)u * bs + tid;
+                    ent[u] = s < nsel ? list[s] : ~0u;
+                    r[u] = ent[u] != ~0u ? r0 + (ent[u] & 8191u) : r0;
                 }
                 eval_keys<U>(L, T, r, lane, sp);
 #pragma unroll
@@ -399,11 +433,10 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint32_t x = pr[ps][u];
-                    if (x == ~0u) continue;
-                    const uint64_t pos = base[(x >> 16) & 0xFFFu] + (x & 0xFFFFu);
-                    const uint32_t special = x >> 28;
-                    uint16_t fl = special == 1 ? kBAllOnes : (special == 2 ? kBNull : 0);
+                    if (ent[u] == ~0u) continue;
+                    const uint32_t p = ent[u] >> 13, s = s0 + (uint32_t)u * bs + tid;
+                    const uint64_t pos = qoff + base[p] + (s - hist[p]);
+                    uint16_t fl = sp[u] == 1 ? kBAllOnes : (sp[u] == 2 ? kBNull : 0);
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         if (j >= nv) break;
@@ -755,6 +788,9 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         BQuery &Q = bq[i];
         memset(&Q, 0, sizeof(Q));
         Q.n_rows = qs[i].n_rows;
+        // the scatter addresses a query's records with 32-bit offsets
+        if (Q.n_rows >= (1ull << 32))
+            fail(PMH_E_VALUE, "polars_metal: batch query " + std::to_string(i) + " holds 2^32 rows or more");
         for (int c = 0; c < q0.n_cols; ++c) {
             const pmh_column *col = nullptr;
             for (uint32_t k = 0; k < qs[i].n_cols; ++k)
@@ -951,7 +987,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipLaunchKernelGGL(bt_scan_local, dim3(grid), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_blocks, dim3(1), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_add, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), 0, R.stream, dL);
+        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), (size_t)L.tile_rows * 4, R.stream, dL);
         {
             const uint32_t *lo = d_lds_off;
             void *args[] = {(void *)&dL, (void *)&lo};

@@ -16,7 +16,6 @@ from __future__ import annotations
 import ctypes
 import json
 import math
-import time
 from typing import Iterable, Sequence
 
 import numpy as np
@@ -132,10 +131,11 @@ def _shape_key(template_plan: dict, queries: Sequence[tuple]) -> tuple:
             int(math.log2(max(1, len(queries)))), int(math.log2(max(1.0, mean))))
 
 
-def autotune(template_plan: dict, queries: Sequence[tuple], sample: int = 512, repeats: int = 2,
+def autotune(template_plan: dict, queries: Sequence[tuple], sample: int = 512, repeats: int = 3,
              candidates: Iterable[dict] = CANDIDATES) -> dict:
     """The fastest candidate launch shape on a spread subset of the batch
-    (cached per plan shape and size class)."""
+    (cached per plan shape and size class), by the engine's own HIP-event
+    time of each run, best of ``repeats``."""
     key = _shape_key(template_plan, queries)
     if key in _tuned:
         return _tuned[key]
@@ -145,9 +145,8 @@ def autotune(template_plan: dict, queries: Sequence[tuple], sample: int = 512, r
     for cand in candidates:
         t = float("inf")
         for _ in range(repeats):
-            t0 = time.perf_counter()
-            results, _ = execute_batch(template_plan, sub, cand)
-            t = min(t, time.perf_counter() - t0)
+            results, st = execute_batch(template_plan, sub, cand)
+            t = min(t, st["device_ms"])
             for r in results:
                 r.free()
         if t < best_t:

@@ -983,7 +983,9 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipGraph_t graph;
         PMH_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(bt_fill, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_tiles<false>, dim3(grid), dim3(tile_block), 0, R.stream, dL);
+        // 16 KB of LDS a block: twice the blocks of the other kernels, for
+        // more loads in flight
+        hipLaunchKernelGGL(bt_tiles<false>, dim3(grid * 2), dim3(tile_block), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_local, dim3(grid), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_blocks, dim3(1), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_add, dim3(grid), dim3(256), 0, R.stream, dL);

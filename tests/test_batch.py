@@ -162,3 +162,29 @@ def test_batch_aggregate_signatures(native, sig):
     for i, (f, r) in enumerate(zip(frames, results)):
         assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"sig{sig}/q{i}",
                              zero_sign_free=("v_min", "v_max"))
+
+
+def test_batch_bool_key(native):
+    """A Bool key, with and without nulls: the key column comes back as bits
+    packed from the batch's byte rows, beside its validity."""
+    from polars_metal import batch
+    from polars_metal.buffer import device_columns
+    rng = np.random.default_rng(77)
+    plan = {"keys": [["flag", "Bool"]],
+            "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "v_sum"},
+                     {"kind": "Length", "op": "Len", "output_alias": "n"}]}
+    frames, queries, bufs = [], [], []
+    for i in range(16):
+        rows = int(rng.integers(100, 20_000))
+        f = {"flag": ("Bool", rng.random(rows) < 0.5, (rng.random(rows) >= 0.1) if i % 2 else None),
+             "v": ("F64", rng.standard_normal(rows), None)}
+        cols, b = device_columns(f)
+        frames.append(f)
+        queries.append((rows, cols, None))
+        bufs.append(b)
+    results, st = batch.execute_batch(plan, queries)
+    assert st["n_queries"] == 16
+    for i, (f, r) in enumerate(zip(frames, results)):
+        got = _fetch(r)
+        assert got["__n_groups__"] == (3 if i % 2 else 2)
+        assert_results_equal(got, orc.groupby(plan, f), ["flag"], label=f"bool/q{i}")

@@ -62,6 +62,7 @@ struct BLaunch {
     QueryDesc q;  // the template: layout, keys, values, column tags
     uint32_t n_queries, n_tiles, tile_rows, part_slots;  // part_slots: LDS table size (power of two)
     uint64_t n_parts;
+    uint32_t scat_parts, scat_pad;  // bt_tiles<scatter>: hist and base entries (the largest P, even)
     BQuery *qs;
     uint32_t *tile_q;      // [n_tiles]
     uint32_t *part_q;      // [n_parts]
@@ -322,16 +323,20 @@ __global__ void bt_fill(const BLaunch *__restrict__ L) {
 // (key lane, value bits, flags).  A thread takes 16 rows of its tile (at
 // most 8192 rows, 512 threads) in two passes of 8, each pass issuing all its
 // loads before using any.  The scatter pass ranks the tile's rows within
-// their partitions, lists them in partition order in LDS (dynamic, 4 bytes a
-// row: partition << 13 | row in tile), then walks that list re-reading each
-// row (its lines are in L2 from the ranking), so consecutive lanes write
-// consecutive records of a partition's run.
+// their partitions, then lists them in partition order in LDS, 4 bytes a row
+// (value validity bits << 25 | partition << 13 | row in tile) beside their
+// value bits, which are loaded in row order; it then walks the list,
+// re-reading only each row's key (its lines are in L2 from the ranking), so
+// consecutive lanes write consecutive records of a partition's run.  Its LDS
+// is dynamic: hist and base (L->scat_parts entries each), the list, then the
+// value bits (n_vals arrays of tile_rows).
 template <bool SCATTER>
 __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ uint32_t hist[kBMaxParts];  // counts; then (scatter) each run's start in the list
-    __shared__ uint32_t base[SCATTER ? kBMaxParts : 1];  // each run's start in the query's records
+    __shared__ uint32_t s_hist[SCATTER ? 1 : kBMaxParts];
     __shared__ uint32_t s_sel, s_wave[16];
+    // counts; then (scatter) each run's start in the list
+    uint32_t *hist = SCATTER ? (uint32_t *)smem : s_hist;
     constexpr int U = 8, kPass = 2;
     const int nv = L->q.n_vals;
     const int vtag[2] = {L->q.cols[L->q.vals[0].col].tag, nv > 1 ? L->q.cols[L->q.vals[1].col].tag : 0};
@@ -381,7 +386,9 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
                 if (hist[p]) atomicAdd(&L->part_cnt[T.part_base + p], hist[p]);
             if (tid == 0 && s_sel) atomicAdd(&L->q_sel[qi], s_sel);
         } else {
-            uint32_t *list = (uint32_t *)smem;  // tile_rows entries
+            uint32_t *base = hist + L->scat_parts;  // each run's start in the query's records
+            uint32_t *list = base + L->scat_parts;
+            uint64_t *vals = (uint64_t *)(list + tile_rows);
             // runs claimed in the query's records; list starts by a block
             // scan of the counts, each thread over a chunk of partitions
             const uint64_t qoff = L->part_off[T.part_base];
@@ -403,28 +410,17 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
                 acc += h;
             }
             __syncthreads();
+            // list entries, and value bits loaded in row order
 #pragma unroll
             for (int ps = 0; ps < kPass; ++ps) {
+                if ((uint32_t)ps * U * bs >= tile_rows) continue;
+                uint64_t r[U], vw[2][U];
+                uint32_t vb[2][U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    const uint32_t x = pr[ps][u];
-                    if (x == ~0u) continue;
-                    const uint32_t p = (x >> 16) & 0xFFFu;
-                    list[hist[p] + (x & 0xFFFFu)] = (p << 13) | ((uint32_t)(ps * U + u) * bs + tid);
+                    const uint64_t rr = r0 + (uint64_t)(ps * U + u) * bs + tid;
+                    r[u] = pr[ps][u] != ~0u ? rr : r0;
                 }
-            }
-            __syncthreads();
-            const uint32_t nsel = s_sel;
-            for (uint32_t s0 = 0; s0 < nsel; s0 += U * bs) {
-                uint64_t r[U], lane[U], vw[2][U];
-                uint32_t sp[U], vb[2][U], ent[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t s = s0 + (uint32_t)u * bs + tid;
-                    ent[u] = s < nsel ? list[s] : ~0u;
-                    r[u] = ent[u] != ~0u ? r0 + (ent[u] & 8191u) : r0;
-                }
-                eval_keys<U>(L, T, r, lane, sp);
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     if (j >= nv) break;
@@ -433,15 +429,42 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
+                    const uint32_t x = pr[ps][u];
+                    if (x == ~0u) continue;
+                    const uint32_t p = (x >> 16) & 0xFFFu, s = hist[p] + (x & 0xFFFFu);
+                    uint32_t ent = (p << 13) | ((uint32_t)(ps * U + u) * bs + tid);
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        if (j >= nv) break;
+                        vals[(uint32_t)j * tile_rows + s] = vw[j][u];
+                        ent |= (vb[j][u] & 1u) << (25 + j);
+                    }
+                    list[s] = ent;
+                }
+            }
+            __syncthreads();
+            const uint32_t nsel = s_sel;
+            for (uint32_t s0 = 0; s0 < nsel; s0 += U * bs) {
+                uint64_t r[U], lane[U];
+                uint32_t sp[U], ent[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t s = s0 + (uint32_t)u * bs + tid;
+                    ent[u] = s < nsel ? list[s] : ~0u;
+                    r[u] = ent[u] != ~0u ? r0 + (ent[u] & 8191u) : r0;
+                }
+                eval_keys<U>(L, T, r, lane, sp);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
                     if (ent[u] == ~0u) continue;
-                    const uint32_t p = ent[u] >> 13, s = s0 + (uint32_t)u * bs + tid;
+                    const uint32_t p = (ent[u] >> 13) & 0xFFFu, s = s0 + (uint32_t)u * bs + tid;
                     const uint64_t pos = qoff + base[p] + (s - hist[p]);
                     uint16_t fl = sp[u] == 1 ? kBAllOnes : (sp[u] == 2 ? kBNull : 0);
 #pragma unroll
                     for (int j = 0; j < 2; ++j) {
                         if (j >= nv) break;
-                        gst<uint64_t>(L->rv[j], pos, vw[j][u]);
-                        fl |= (uint16_t)(vb[j][u] << j);
+                        gst<uint64_t>(L->rv[j], pos, vals[(uint32_t)j * tile_rows + s]);
+                        fl |= (uint16_t)(((ent[u] >> (25 + j)) & 1u) << j);
                     }
                     gst<uint64_t>(L->rk, pos, lane[u]);
                     gst<uint16_t>(L->rf, pos, fl);
@@ -765,6 +788,11 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     const Options &opt = tmpl.opts;
     L.tile_rows = (uint32_t)std::min<int64_t>(8192, std::max<int64_t>(1024, opt.tile > 0 ? opt.tile : 8192));
     L.tile_rows = pow2_ceil(L.tile_rows);  // 16 rows per thread of a 64..512-thread block
+    // the scatter's LDS (hist and base for up to kBMaxParts partitions, the
+    // list, the value bits) must fit
+    while (L.tile_rows > 1024 &&
+           (size_t)8 * kBMaxParts + (size_t)L.tile_rows * (4 + 8 * q0.n_vals) > lds_max - 1024)
+        L.tile_rows /= 2;
     // LDS table of a (query, partition) work item: the largest power of two
     // up to kBMaxSlots that fits beside bt_agg's static LDS (its live-slot
     // list and state-array table, under 10 KB), or the option's size;
@@ -892,6 +920,10 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     if (n_tiles >= (1ull << 32)) fail(PMH_E_VALUE, "polars_metal: batch too large");
     L.n_tiles = (uint32_t)n_tiles;
     L.n_parts = n_parts;
+    uint32_t max_p = 1;
+    for (uint32_t i = 0; i < nq; ++i) max_p = std::max(max_p, bq[i].P);
+    L.scat_parts = (max_p + 1) & ~1u;  // keeps the value bits 8-byte aligned
+    const size_t scat_lds = (size_t)8 * L.scat_parts + (size_t)L.tile_rows * (4 + 8 * q0.n_vals);
     std::vector<uint32_t> tile_q(n_tiles), part_q(n_parts);
     for (uint32_t i = 0; i < nq; ++i) {
         for (uint32_t t = 0; t < bq[i].n_tiles; ++t) tile_q[bq[i].tile_base + t] = i;
@@ -965,15 +997,17 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     const int sig = agg_signature(q0);
     const void *agg_fn = kBtAgg[sig];
     PMH_HIP(hipFuncSetAttribute(agg_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_agg));
+    PMH_HIP(hipFuncSetAttribute((const void *)bt_tiles<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)scat_lds));
     // ---- the chain: captured once per buffer layout, replayed after
     const uint32_t grid = (uint32_t)cus * 4;
     const uint32_t tile_block = L.tile_rows / 16;
     // a table of at most 512 slots leaves room for several blocks per CU
     const uint32_t agg_block = CL >= 1024 ? 1024 : 512;
     char keybuf[512];
-    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u %d", (void *)dL, (void *)L.rk,
+    snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u %d %zu", (void *)dL, (void *)L.rk,
              (void *)L.part_cnt, (void *)L.part_off, (void *)L.t.klo, (void *)L.out[0].data, (void *)d_lds_off, lds_agg,
-             grid, (unsigned)P.n_out, tile_block, agg_block, sig);
+             grid, (unsigned)P.n_out, tile_block, agg_block, sig, scat_lds);
     bool reused = g_graph.exec && g_graph.key == keybuf;
     if (!reused) {
         if (g_graph.exec) {
@@ -989,7 +1023,7 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipLaunchKernelGGL(bt_scan_local, dim3(grid), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_blocks, dim3(1), dim3(1024), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_scan_add, dim3(grid), dim3(256), 0, R.stream, dL);
-        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), (size_t)L.tile_rows * 4, R.stream, dL);
+        hipLaunchKernelGGL(bt_tiles<true>, dim3(grid), dim3(tile_block), scat_lds, R.stream, dL);
         {
             const uint32_t *lo = d_lds_off;
             void *args[] = {(void *)&dL, (void *)&lo};

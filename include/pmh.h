@@ -293,7 +293,8 @@ typedef struct pmh_batch_stats {
  * with host columns: each column's location says where it is.
  * Keys: one lane of at most 64 bits; at most two aggregated columns;
  * predicates: conjunctions of column-vs-literal comparisons
- * (PMH_E_NOTIMPL otherwise).  flags as pmh_execute_filter_groupby. */
+ * (PMH_E_NOTIMPL otherwise).  Each query holds fewer than 2^32 rows
+ * (PMH_E_VALUE otherwise).  flags as pmh_execute_filter_groupby. */
 int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *queries, uint32_t n_queries,
                       uint32_t flags, pmh_result **results, pmh_batch_stats *stats);
 

@@ -188,3 +188,16 @@ def test_batch_bool_key(native):
         got = _fetch(r)
         assert got["__n_groups__"] == (3 if i % 2 else 2)
         assert_results_equal(got, orc.groupby(plan, f), ["flag"], label=f"bool/q{i}")
+
+
+def test_batch_refuses_a_query_of_2_32_rows(native):
+    """A query's records are addressed with 32-bit offsets: a query claiming
+    2^32 rows is refused on the host before any launch, and the next batch
+    runs as usual."""
+    from polars_metal import batch
+    frames, queries, bufs = _queries(11, 3)
+    rows, cols, pred = queries[1]
+    with pytest.raises(ValueError, match=r"2\^32"):
+        batch.execute_batch(_plan(), [queries[0], (1 << 32, cols, pred), queries[2]])
+    results, st = batch.execute_batch(_plan(), queries)
+    _check(frames, queries, results, label="after/")

@@ -236,7 +236,8 @@ def encode_keys(frame: dict, keys: list):
 def groupby(plan: dict, frame: dict, sum_mode: int = 0) -> dict:
     """Run plan (keys, aggs, optional predicate) on frame.  Returns an ordered
     dict ``name -> (tag, values, valid)`` with groups in first-occurrence
-    order, plus the key '__n_groups__'."""
+    order, plus the keys '__n_groups__' and '__ops__' (output alias -> agg
+    op, which the comparison helpers read)."""
     n = frame_rows(frame)
     keep_mask = None
     if plan.get("predicate"):
@@ -327,6 +328,7 @@ def groupby(plan: dict, frame: dict, sum_mode: int = 0) -> dict:
             else:
                 raise ValueError(op)
     out["__n_groups__"] = ng
+    out["__ops__"] = {a["output_alias"]: ("Len" if a.get("kind") == "Length" else a.get("op")) for a in plan["aggs"]}
     return out
 
 
@@ -372,7 +374,7 @@ def canonical(result: dict, keys: list) -> dict:
     order = np.lexsort(sort_cols)
     out = {}
     for nm, val in result.items():
-        if nm == "__n_groups__":
+        if nm.startswith("__"):
             out[nm] = val
             continue
         tag, v, ok = val

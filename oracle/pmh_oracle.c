@@ -30,9 +30,11 @@
  * R. Hettinger's msum recipe -- the method behind Python's math.fsum.  A
  * second mode (sum_mode = 1) sums sequentially in row order, which is what
  * a one-thread run of the reference's fold computes.  Either way a zero
- * sum is +0.0, as the fold's +0.0 identity makes it.  f64 min/max fold in
- * row order with a strict comparison, so a zero extreme keeps the sign of
- * the group's first zero.
+ * sum is +0.0, as the fold's +0.0 identity makes it.  Float min/max order
+ * -0.0 below +0.0, so the result does not depend on row order (the
+ * reference's Rayon fold keeps whichever zero it met first), and a group
+ * holding a NaN yields the canonical quiet NaN.  F32 values widen exactly
+ * to f64; F32 sums and means are the f64 result rounded once more to f32.
  *
  * Parity pins (tests/test_oracle.py): predicate truth tables vs the
  * reference's exhaustive Kleene table and TotalOrd examples; compaction vs a
@@ -342,6 +344,23 @@ struct orc_gb {
 
 static int is_float_tag(int t) { return t == ORC_F64 || t == ORC_F32; }
 
+/* 0x7FF8000000000000: the quiet NaN that Rust's f64::NAN denotes. */
+static double canon_nan(void) {
+    const uint64_t b = 0x7FF8000000000000ULL;
+    double d;
+    memcpy(&d, &b, sizeof d);
+    return d;
+}
+
+/* a < b on non-NaN doubles with -0.0 ordered below +0.0.  The reference
+ * folds min/max with a strict IEEE compare in Rayon's unspecified order
+ * (groupby.rs:1471-1520), so a group holding both zeros gets whichever came
+ * first; this rule picks -0.0 for min and +0.0 for max in any order. */
+static int zero_signed_lt(double a, double b) {
+    if (a < b) return 1;
+    return a == 0.0 && b == 0.0 && signbit(a) && !signbit(b);
+}
+
 orc_gb *orc_groupby(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_col *keys,
                     int n_vals, const orc_col *vals, int sum_mode) {
     if (n_vals > ORC_MAX_VALS || n_vals < 0) return NULL;
@@ -418,11 +437,12 @@ orc_gb *orc_groupby(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_
             if (fl) {
                 double v = load_f64(c->tag, c->data, r);
                 if (xs) xsum_add(&xs[gi], v); else g->sum_f[j][gi] += v;
-                /* NaN poisons min and max (groupby.rs:1467-1469) */
-                if (isnan(v) || isnan(g->min_f[j][gi])) g->min_f[j][gi] = NAN;
-                else if (v < g->min_f[j][gi]) g->min_f[j][gi] = v;
-                if (isnan(v) || isnan(g->max_f[j][gi])) g->max_f[j][gi] = NAN;
-                else if (v > g->max_f[j][gi]) g->max_f[j][gi] = v;
+                /* NaN poisons min and max and the result is the canonical
+                 * quiet NaN, Rust's f64::NAN (groupby.rs:1467-1469, :1494) */
+                if (isnan(v) || isnan(g->min_f[j][gi])) g->min_f[j][gi] = canon_nan();
+                else if (zero_signed_lt(v, g->min_f[j][gi])) g->min_f[j][gi] = v;
+                if (isnan(v) || isnan(g->max_f[j][gi])) g->max_f[j][gi] = canon_nan();
+                else if (zero_signed_lt(g->max_f[j][gi], v)) g->max_f[j][gi] = v;
             } else {
                 int64_t v = load_i64(c->tag, c->data, r);
                 g->sum_i[j][gi] = (int64_t)((uint64_t)g->sum_i[j][gi] + (uint64_t)v); /* wrapping */

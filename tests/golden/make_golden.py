@@ -21,6 +21,23 @@ Each case is named for what it pins:
   bool_keys         Bool key column
   empty             zero rows
   all_filtered      predicate removes every row
+  nan_minmax_f64    NaN poisons f64 min and max; a null NaN does not
+                    (reference KATs crates/polars-metal-kernels/tests/
+                    test_groupby_aggregate.rs:469-497: [1, NaN, 3] -> min
+                    NaN, [10, NaN, 5] -> max NaN)
+  signed_zero_f64   groups holding -0.0 and +0.0 in either order: min is
+                    -0.0 and max is +0.0 whatever the row order
+  f32_aggs          F32 sum / mean are the exact sum rounded to f64, then
+                    to f32 (so [1, 2^-24, 2^-24] sums to 1 + 2^-23, where
+                    an f32 running sum gives 1); min / max with NaN, +-0.0,
+                    +-inf
+  f32_nan_keys_f64  F32 values under a predicate on an F64 column
+
+Float min/max rule: NaN anywhere in the group (among non-null rows) gives
+the canonical quiet NaN; otherwise the extreme value, with -0.0 ordered
+below +0.0 (the reference's Rayon fold keeps whichever zero it met first,
+crates/polars-metal-kernels/src/groupby.rs:1471-1520, so no row order is
+fixed there; this rule is order independent).
 
 Run:  python tests/golden/make_golden.py
 """
@@ -28,6 +45,7 @@ from __future__ import annotations
 
 import json
 import math
+import struct
 from pathlib import Path
 
 OUT = Path(__file__).resolve().parent / "cases.json"
@@ -45,6 +63,40 @@ def fenc(x):
 
 def col(tag, values):
     return {"tag": tag, "values": list(values)}
+
+
+def f32r(x):
+    """Round a Python float to the nearest f32 (ties to even), as a float."""
+    if math.isnan(x) or math.isinf(x):
+        return x
+    return struct.unpack("<f", struct.pack("<f", x))[0]
+
+
+def xsum(vals):
+    """Correctly rounded sum; IEEE special values as a running sum gives
+    them (NaN, or +inf and -inf together -> NaN; one infinity -> it)."""
+    if any(math.isnan(x) for x in vals) or (inf_in(vals, 1) and inf_in(vals, -1)):
+        return float("nan")
+    if inf_in(vals, 1) or inf_in(vals, -1):
+        return float("inf") if inf_in(vals, 1) else float("-inf")
+    return math.fsum(vals) + 0.0
+
+
+def inf_in(vals, sign):
+    return any(math.isinf(x) and math.copysign(1.0, x) == sign for x in vals)
+
+
+def fmin(vals):
+    """Float min: NaN poisons; -0.0 below +0.0."""
+    if any(math.isnan(x) for x in vals):
+        return float("nan")
+    return min(vals, key=lambda x: (x, math.copysign(1.0, x)))
+
+
+def fmax(vals):
+    if any(math.isnan(x) for x in vals):
+        return float("nan")
+    return max(vals, key=lambda x: (x, math.copysign(1.0, x)))
 
 
 def tot_cmp(a, b):
@@ -128,21 +180,25 @@ def expected(plan, cols, n):
                 out[a["output_alias"]].append(len(vals))
             elif op == "Sum":
                 if tag == "F64":
-                    out[a["output_alias"]].append(math.fsum(vals))
+                    out[a["output_alias"]].append(xsum(vals))
+                elif tag == "F32":
+                    out[a["output_alias"]].append(f32r(xsum(vals)))
                 else:
                     out[a["output_alias"]].append(wrap(sum(vals), 64 if tag == "I64" else 32))
             elif op == "Mean":
                 if not vals:
                     out[a["output_alias"]].append(None)
                 elif tag == "F64":
-                    out[a["output_alias"]].append(math.fsum(vals) / len(vals))
+                    out[a["output_alias"]].append(xsum(vals) / len(vals))
+                elif tag == "F32":
+                    out[a["output_alias"]].append(f32r(xsum(vals) / len(vals)))
                 else:
                     out[a["output_alias"]].append(float(wrap(sum(vals), 64)) / len(vals))
             elif op in ("Min", "Max"):
                 if not vals:
                     out[a["output_alias"]].append(None)
-                elif tag == "F64" and any(math.isnan(x) for x in vals):
-                    out[a["output_alias"]].append(float("nan"))
+                elif tag in ("F64", "F32"):
+                    out[a["output_alias"]].append(fmin(vals) if op == "Min" else fmax(vals))
                 else:
                     out[a["output_alias"]].append(min(vals) if op == "Min" else max(vals))
     return {k: [fenc(x) for x in v] for k, v in out.items()}, len(order)
@@ -216,6 +272,35 @@ def cases():
                {"key": col("I64", []), "v": col("F64", []), "w": col("I32", [])}, 0))
     cs.append(("all_filtered", {"keys": [["key", "I64"]], "aggs": FIVE, "predicate": gt0("v")},
                {"key": col("I64", [1, 2, 3]), "v": col("F64", [-1.0, -2.0, None]), "w": col("I32", [1, 2, 3])}, 3))
+    nan = float("nan")
+    inf = float("inf")
+    # groups 0 and 1 are the reference's KATs; 2: NaN only under a null;
+    # 3: all null; 4: NaN alone; 5: no NaN
+    nk = [0, 0, 0, 1, 1, 1, 2, 2, 3, 4, 5, 5, 5]
+    nv = [1.0, nan, 3.0, 10.0, nan, 5.0, None, 7.0, None, nan, -inf, 2.0, inf]
+    cs.append(("nan_minmax_f64", {"keys": [["key", "I64"]],
+                                  "aggs": [agg("v", "Min"), agg("v", "Max"), agg("v", "Count"), agg("v", "Sum")]},
+               {"key": col("I64", nk), "v": col("F64", nv)}, len(nk)))
+    # nulls hide a NaN in group 2: a null NaN row must not poison
+    cs[-1][2]["v"]["values"][6] = None
+    zk = [0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 5, 5, 6, 6]
+    zv = [0.0, -0.0, -0.0, 0.0, 0.0, 0.0, 1.0, -0.0, -1.0, -0.0, 0.0, None, -0.0, -0.0]
+    cs.append(("signed_zero_f64", {"keys": [["key", "I64"]],
+                                   "aggs": [agg("v", "Min"), agg("v", "Max"), agg("v", "Sum"), agg("v", "Mean")]},
+               {"key": col("I64", zk), "v": col("F64", zv)}, len(zk)))
+    t = 2.0 ** -24
+    fk = [0, 0, 0, 1, 1, 1, 2, 2, 3, 3, 3, 4, 4, 5, 5, 6, 6, 6, 7]
+    fv = [1.0, t, t, 1e8, 1.0, -1e8, nan, 2.5, 0.0, -0.0, 0.0, -inf, 3.0, None, None, 0.1, 0.2, 0.3, 16777217.0]
+    fv = [None if x is None else f32r(x) for x in fv]
+    cs.append(("f32_aggs", {"keys": [["key", "I32"]],
+                            "aggs": [agg("v", "Sum"), agg("v", "Mean"), agg("v", "Min"), agg("v", "Max"),
+                                     agg("v", "Count"), {"kind": "Length", "op": "Len", "output_alias": "len"}]},
+               {"key": col("I32", fk), "v": col("F32", fv)}, len(fk)))
+    gv = [f32r(x) for x in [0.5, -1.5, nan, 4.0, -0.0, 0.0, 1e-3, 7.0]]
+    gp = [1.0, 2.0, -1.0, nan, 0.0, 3.0, None, 5.0]
+    cs.append(("f32_under_f64_pred", {"keys": [["key", "I64"]], "predicate": gt0("p"),
+                                      "aggs": [agg("v", "Sum"), agg("v", "Min"), agg("v", "Max"), agg("v", "Mean")]},
+               {"key": col("I64", [1, 1, 1, 2, 2, 2, 1, 2]), "v": col("F32", gv), "p": col("F64", gp)}, 8))
     return cs
 
 

@@ -61,8 +61,7 @@ def _plan(key_tag="I64"):
 def _check(frames, queries, results, key_tag="I64", label=""):
     for i, (f, (rows, cols, pred), r) in enumerate(zip(frames, queries, results)):
         plan = dict(_plan(key_tag), predicate=pred) if pred else _plan(key_tag)
-        assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"{label}q{i}",
-                             zero_sign_free=("v_min", "v_max"))
+        assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"{label}q{i}")
 
 
 @pytest.mark.parametrize("key_tag,nulls", [("I64", False), ("I64", True), ("I32", True)])
@@ -160,8 +159,7 @@ def test_batch_aggregate_signatures(native, sig):
     results, st = batch.execute_batch(plan, queries)
     assert st["n_queries"] == 24
     for i, (f, r) in enumerate(zip(frames, results)):
-        assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"sig{sig}/q{i}",
-                             zero_sign_free=("v_min", "v_max"))
+        assert_results_equal(_fetch(r), orc.groupby(plan, f), ["key"], label=f"sig{sig}/q{i}")
 
 
 def test_batch_bool_key(native):
@@ -190,14 +188,41 @@ def test_batch_bool_key(native):
         assert_results_equal(got, orc.groupby(plan, f), ["flag"], label=f"bool/q{i}")
 
 
-def test_batch_refuses_a_query_of_2_32_rows(native):
-    """A query's records are addressed with 32-bit offsets: a query claiming
-    2^32 rows is refused on the host before any launch, and the next batch
-    runs as usual."""
+def test_batch_refuses_oversized_or_short_queries(native):
+    """Every query is checked on the host before any launch, not only the
+    first: a query claiming 2^31 rows or more (the single-query limit, which
+    a query the batch cannot finish is re-run on) and a query whose buffers
+    are shorter than its row count are refused, and the next batch runs as
+    usual."""
     from polars_metal import batch
     frames, queries, bufs = _queries(11, 3)
     rows, cols, pred = queries[1]
-    with pytest.raises(ValueError, match=r"2\^32"):
-        batch.execute_batch(_plan(), [queries[0], (1 << 32, cols, pred), queries[2]])
+    for n in (1 << 31, (1 << 32) - 1, 1 << 32):
+        with pytest.raises(ValueError, match=r"2\^31"):
+            batch.execute_batch(_plan(), [queries[0], (n, cols, pred), queries[2]])
+    with pytest.raises(ValueError, match="buffer too short"):
+        batch.execute_batch(_plan(), [queries[0], (rows + 64, cols, pred), queries[2]])
     results, st = batch.execute_batch(_plan(), queries)
     _check(frames, queries, results, label="after/")
+
+
+def test_batch_predicate_on_a_column_no_aggregate_reads(native):
+    """filter(w > t).group_by(key).agg(v...): the template names key and v
+    only; w comes from the queries' predicates."""
+    from polars_metal import batch
+    from polars_metal.buffer import device_columns
+    rng = np.random.default_rng(17)
+    frames, queries, bufs = [], [], []
+    for i in range(6):
+        rows = int(rng.integers(2000, 40000))
+        f = _frame(rng, rows)
+        f["w"] = ("I32", rng.integers(0, 100, rows).astype(np.int32), rng.random(rows) < 0.9)
+        pred = {"kind": "Compare", "op": "Ge", "dtype": "I64", "lhs": {"kind": "Column", "name": "w", "dtype": "I64"},
+                "rhs": {"kind": "LiteralI64", "value": int(10 * i)}} if i % 2 == 0 else None
+        cols, b = device_columns(f)
+        frames.append(f)
+        queries.append((rows, cols, pred))
+        bufs.append(b)
+    results, st = batch.execute_batch(_plan(), queries)
+    assert st["n_queries"] == 6
+    _check(frames, queries, results, label="predw/")

@@ -132,9 +132,10 @@ def test_group_identity_vs_numpy_unique():
 
 def test_signed_zero_sum_and_min_max():
     # The reference folds f64 sums from +0.0 (groupby.rs:1303-1321): an
-    # all -0.0 group sums to +0.0 in any fold order.  Min/max fold in row
-    # order with a strict comparison (groupby.rs:1471-1520): a zero extreme
-    # keeps the sign of the group's first zero.
+    # all -0.0 group sums to +0.0 in any fold order.  Min/max order -0.0
+    # below +0.0, so a group holding both zeros gives min -0.0 and max +0.0
+    # in every row order (the reference's Rayon fold, groupby.rs:1471-1520,
+    # fixes no order).
     v = np.array([-0.0, -0.0, 0.0, -0.0, -0.0, 1.0, -1.0])
     k = np.array([0, 0, 1, 1, 2, 2, 3])
     frame = {"k": ("I64", k, None), "v": ("F64", v, None)}
@@ -145,8 +146,76 @@ def test_signed_zero_sum_and_min_max():
         s, mn, mx = got["Sum"][1], got["Min"][1], got["Max"][1]
         assert [math.copysign(1.0, x) for x in s[:2]] == [1.0, 1.0], mode   # +0.0 sums
         assert math.copysign(1.0, mn[0]) == -1.0 and math.copysign(1.0, mx[0]) == -1.0
-        assert math.copysign(1.0, mn[1]) == 1.0 and math.copysign(1.0, mx[1]) == 1.0  # first zero is +0.0
-        assert mn[2] == -0.0 and mx[2] == 1.0 and mn[3] == -1.0
+        assert math.copysign(1.0, mn[1]) == -1.0 and math.copysign(1.0, mx[1]) == 1.0  # both zeros: -0.0 / +0.0
+        assert mn[2] == -0.0 and math.copysign(1.0, mn[2]) == -1.0 and mx[2] == 1.0 and mn[3] == -1.0
+
+
+def _bits64(x):
+    return np.asarray(x, np.float64).view(np.uint64)
+
+
+def test_nan_poisoning_kats():
+    # crates/polars-metal-kernels/tests/test_groupby_aggregate.rs:469-497:
+    # min of [1, NaN, 3] and max of [10, NaN, 5] are NaN.  The NaN written is
+    # Rust's f64::NAN (groupby.rs:1494), the quiet NaN 0x7FF8000000000000,
+    # whatever NaN the input held.
+    odd_nan = np.array([0xFFF8000000000001], np.uint64).view(np.float64)[0]
+    for vals, op in (([1.0, np.nan, 3.0], "Min"), ([10.0, np.nan, 5.0], "Max"), ([odd_nan, 2.0], "Min"),
+                     ([4.0, odd_nan], "Max")):
+        frame = {"k": ("I64", np.zeros(len(vals), np.int64), None), "v": ("F64", np.array(vals), None)}
+        plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": "m"}]}
+        got = orc.groupby(plan, frame)["m"]
+        assert got[2][0] and _bits64(got[1])[0] == 0x7FF8000000000000, (vals, op)
+    # a null NaN does not poison; a NaN under a filter does not either
+    frame = {"k": ("I64", np.zeros(3, np.int64), None), "v": ("F64", np.array([np.nan, 2.0, -3.0]),
+                                                               np.array([False, True, True]))}
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op}
+                                             for op in ("Min", "Max")]}
+    got = orc.groupby(plan, frame)
+    assert got["Min"][1][0] == -3.0 and got["Max"][1][0] == 2.0
+
+
+def test_float_min_max_independent_of_row_order():
+    # every permutation of a group with both zeros, a NaN-free mix and
+    # +-inf gives the same bits
+    import itertools
+    base = [0.0, -0.0, 3.5, -0.0, 0.0]
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op}
+                                             for op in ("Min", "Max")]}
+    seen = set()
+    for perm in itertools.permutations(base):
+        frame = {"k": ("I64", np.zeros(5, np.int64), None), "v": ("F64", np.array(perm), None)}
+        got = orc.groupby(plan, frame)
+        seen.add((int(_bits64(got["Min"][1])[0]), int(_bits64(got["Max"][1])[0])))
+    assert seen == {(0x8000000000000000, int(_bits64(3.5)))}
+    for perm in itertools.permutations([0.0, -0.0, -0.0]):
+        frame = {"k": ("I64", np.zeros(3, np.int64), None), "v": ("F64", np.array(perm), None)}
+        got = orc.groupby(plan, frame)
+        assert (_bits64(got["Min"][1])[0], _bits64(got["Max"][1])[0]) == (0x8000000000000000, 0)
+
+
+def test_f32_sum_rounds_once_from_exact():
+    # F32 values widen exactly to f64; the group sum is the correctly rounded
+    # f64 sum, then rounded to f32.  [1, 2^-24, 2^-24]: exact 1 + 2^-23, which
+    # f32 holds, where an f32 running sum rounds each step back to 1.
+    # [1e8, 1, -1e8]: exact 1, where an f32 running sum gives 0.
+    t = np.float32(2.0 ** -24)
+    cases = [([1.0, t, t], np.float32(1.0 + 2.0 ** -23)), ([1e8, 1.0, -1e8], np.float32(1.0)),
+             ([0.1, 0.2, 0.3], np.float32(float(np.float32(0.1)) + float(np.float32(0.2)) + float(np.float32(0.3))))]
+    for vals, want in cases:
+        frame = {"k": ("I32", np.zeros(len(vals), np.int32), None), "v": ("F32", np.array(vals, np.float32), None)}
+        plan = {"keys": [["k", "I32"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op}
+                                                 for op in ("Sum", "Mean", "Min", "Max")]}
+        got = orc.groupby(plan, frame)
+        assert got["Sum"][0] == "F32" and got["Mean"][0] == "F32"
+        assert got["Sum"][1][0].view(np.uint32) == want.view(np.uint32), vals
+        exact = math.fsum(float(x) for x in np.array(vals, np.float32))
+        assert got["Mean"][1][0] == np.float32(exact / len(vals))
+        assert got["Min"][1][0] == min(np.array(vals, np.float32)) and got["Max"][1][0] == max(np.array(vals, np.float32))
+    # an F32 NaN min is the f32 quiet NaN 0x7FC00000
+    frame = {"k": ("I32", np.zeros(2, np.int32), None), "v": ("F32", np.array([np.nan, 1.0], np.float32), None)}
+    plan = {"keys": [["k", "I32"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Min", "output_alias": "m"}]}
+    assert orc.groupby(plan, frame)["m"][1].view(np.uint32)[0] == 0x7FC00000
 
 
 def test_key_encoding_widths():

@@ -19,7 +19,11 @@
  *     (crates/polars-metal-core/src/udf/common.rs:74-84).
  *   - Group order in results is unspecified; callers sort by key
  *     (crates/polars-metal-core/src/udf/groupby/mod.rs:720-746).
- *   - Row counts must be below 2^31.
+ *   - Row counts must be below 2^31; every entry returns PMH_E_VALUE for
+ *     2^31 rows or more, before any device work.  The reference accepts up
+ *     to u32::MAX (crates/polars-metal-kernels/src/filter.rs:73-77,
+ *     groupby.rs:349-350); this backend narrows that by one bit because its
+ *     kernels address rows with signed 32-bit offsets.
  *   - Calls are serialised per process; no entry point may be called from two
  *     host threads at once on the same device.
  */
@@ -291,10 +295,13 @@ typedef struct pmh_batch_stats {
  * With PMH_OUT_DEVICE, a query the batch could not finish (its groups
  * outgrew the rows sized from its sketch) is re-run alone and comes back
  * with host columns: each column's location says where it is.
- * Keys: one lane of at most 64 bits; at most two aggregated columns;
- * predicates: conjunctions of column-vs-literal comparisons
- * (PMH_E_NOTIMPL otherwise).  Each query holds fewer than 2^32 rows
- * (PMH_E_VALUE otherwise).  flags as pmh_execute_filter_groupby. */
+ * Keys: one lane of at most 64 bits; at most two aggregated columns; at
+ * most 8 columns referenced by the keys, the aggregates and the queries'
+ * predicates together; predicates: conjunctions of column-vs-literal
+ * comparisons (PMH_E_NOTIMPL otherwise).  Each query holds fewer than 2^31
+ * rows, and each of its columns is checked against its own row count
+ * (buffer lengths, NULL data) before any launch (PMH_E_VALUE otherwise).
+ * flags as pmh_execute_filter_groupby. */
 int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *queries, uint32_t n_queries,
                       uint32_t flags, pmh_result **results, pmh_batch_stats *stats);
 

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <sstream>
 
 #include "pmh_internal.h"
@@ -330,6 +331,39 @@ int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *que
         std::vector<BatchQueryIn> qs(n_queries);
         for (uint32_t i = 0; i < n_queries; ++i)
             qs[i] = BatchQueryIn{queries[i].n_rows, queries[i].columns, queries[i].n_columns, queries[i].predicate_json};
+        // columns the queries' predicates read are staged with the template
+        // even when no key or aggregate names them
+        std::set<std::string> texts, have;
+        for (auto &k : tmpl.keys) have.insert(k.first);
+        for (auto &a : tmpl.aggs) if (a.op != AGG_LEN) have.insert(a.input_col);
+        for (uint32_t i = 0; i < n_queries; ++i) {
+            if (!queries[i].predicate_json || !texts.insert(queries[i].predicate_json).second) continue;
+            const std::string js = std::string("{\"keys\": [], \"aggs\": [], \"predicate\": ") +
+                                   queries[i].predicate_json + "}";
+            GroupByPlan pq = parse_groupby_plan(js, true);
+            if (!pq.predicate) continue;
+            std::vector<std::string> names;
+            collect_pred_cols(*pq.predicate, names);
+            for (auto &nm : names)
+                if (have.insert(nm).second) tmpl.extra_cols.push_back(nm);
+        }
+        // every query's row count and buffers, on the host before the device
+        // is touched
+        std::vector<std::string> needed;
+        for (auto &k : tmpl.keys) needed.push_back(k.first);
+        for (auto &a : tmpl.aggs) if (a.op != AGG_LEN) needed.push_back(a.input_col);
+        for (auto &c : tmpl.extra_cols) needed.push_back(c);
+        for (uint32_t i = 0; i < n_queries; ++i) {
+            const std::string qn = "polars_metal: batch query " + std::to_string(i);
+            if (qs[i].n_rows >= (1ull << 31)) fail(PMH_E_VALUE, qn + " holds 2^31 rows or more");
+            for (auto &nm : needed) {
+                const pmh_column *col = nullptr;
+                for (uint32_t k = 0; k < qs[i].n_cols && qs[i].cols; ++k)
+                    if (qs[i].cols[k].name && nm == qs[i].cols[k].name) { col = &qs[i].cols[k]; break; }
+                if (!col) fail(PMH_E_KEY, qn + " has no column " + nm);
+                check_column(*col, qs[i].n_rows, nm);
+            }
+        }
         std::vector<pmh_result *> out;
         try {
             run_batch(tmpl, qs, flags, out, stats);

@@ -816,14 +816,20 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         BQuery &Q = bq[i];
         memset(&Q, 0, sizeof(Q));
         Q.n_rows = qs[i].n_rows;
-        // the scatter addresses a query's records with 32-bit offsets
-        if (Q.n_rows >= (1ull << 32))
-            fail(PMH_E_VALUE, "polars_metal: batch query " + std::to_string(i) + " holds 2^32 rows or more");
+        // the same row limit as a single query, which is also what a query
+        // the batch cannot finish is re-run on
+        if (Q.n_rows >= (1ull << 31))
+            fail(PMH_E_VALUE, "polars_metal: batch query " + std::to_string(i) + " holds 2^31 rows or more");
+        if (Q.n_rows && (!qs[i].cols || !qs[i].n_cols))
+            fail(PMH_E_VALUE, "polars_metal: batch query " + std::to_string(i) + " has no columns");
         for (int c = 0; c < q0.n_cols; ++c) {
             const pmh_column *col = nullptr;
             for (uint32_t k = 0; k < qs[i].n_cols; ++k)
                 if (qs[i].cols[k].name && names[c] == qs[i].cols[k].name) col = &qs[i].cols[k];
             if (!col) fail(PMH_E_KEY, "polars_metal: batch query " + std::to_string(i) + " has no column " + names[c]);
+            // every query's buffers are checked against its own row count
+            // before any launch (query 0's again here, cheaply)
+            check_column(*col, Q.n_rows, names[c]);
             if (col->location != PMH_DEVICE)
                 fail(PMH_E_VALUE, "polars_metal: batch columns must be device-resident");
             if (col->tag != q0.cols[c].tag)

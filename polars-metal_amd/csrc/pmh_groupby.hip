@@ -415,13 +415,13 @@ int lower_pred(const PlanPredNode &n, QueryDesc &q, const std::vector<Column> &c
     fail(PMH_E_VALUE, "polars_metal: predicate node " + n.kind + " does not yield a boolean");
 }
 
+}  // namespace
+
 void collect_pred_cols(const PlanPredNode &n, std::vector<std::string> &out) {
     if (n.kind == "Column") out.push_back(n.name);
     if (n.lhs) collect_pred_cols(*n.lhs, out);
     if (n.rhs) collect_pred_cols(*n.rhs, out);
 }
-
-}  // namespace
 
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols) {
     if (n_rows >= (1ull << 31)) fail(PMH_E_VALUE, "polars_metal: row count must be below 2^31");
@@ -432,6 +432,7 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
     for (auto &a : plan.aggs)
         if (a.op != AGG_LEN) needed.push_back(a.input_col);
     if (plan.predicate) collect_pred_cols(*plan.predicate, needed);
+    for (auto &c : plan.extra_cols) needed.push_back(c);
     // dtype checks before any device work (the same failures the reference
     // reports from its parser / agg_kind_for, legacy.rs:22-48)
     auto find_host = [&](const std::string &nm) -> const pmh_column * {
@@ -453,6 +454,7 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
             fail(PMH_E_VALUE, std::string("polars_metal: unsupported (agg_op, dtype) combination: ") + op_name(a.op) +
                                   " / " + tag_name(c->tag));
     }
+    for (auto &nm : needed) check_column(*find_host(nm), n_rows, nm);
     rt();
     std::vector<Column> dcols = stage_columns(cols, n_cols, n_rows, needed);
     auto col_index = [&](const std::string &nm) -> int {

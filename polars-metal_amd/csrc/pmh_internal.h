@@ -177,7 +177,13 @@ struct GroupByPlan {
     std::vector<PlanAgg> aggs;
     std::unique_ptr<PlanPredNode> predicate;
     Options opts;
+    // columns staged beside the keys, aggregate inputs and predicate (a
+    // batch template lists the columns its queries' own predicates read)
+    std::vector<std::string> extra_cols;
 };
+
+// Appends every column a PredicateAst reads, in tree order.
+void collect_pred_cols(const PlanPredNode &n, std::vector<std::string> &out);
 
 GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate);
 std::string compute_lifting_plan_json(const std::string &json);
@@ -220,6 +226,9 @@ struct Column {
 };
 
 // Upload (or pass through) the columns the plan references.
+// Refuses a column whose value or validity buffer is shorter than n_rows
+// need, or whose tag is unknown (PMH_E_VALUE), before any device work.
+void check_column(const pmh_column &c, uint64_t n_rows, const std::string &nm);
 std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
                                   const std::vector<std::string> &needed);
 

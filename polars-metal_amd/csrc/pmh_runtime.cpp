@@ -116,6 +116,19 @@ void set_out_name(pmh_out_column &c, const std::string &name) {
     c.name = s;
 }
 
+void check_column(const pmh_column &c, uint64_t n_rows, const std::string &nm) {
+    if (c.tag < 0 || c.tag > PMH_UTF8) fail(PMH_E_VALUE, "polars_metal: unknown dtype tag for column " + nm);
+    const int esz = tag_size(c.tag);
+    const uint64_t need = esz == 0 ? (n_rows + 7) / 8 : n_rows * (uint64_t)esz;
+    if (c.data_len < need || (!c.data && need))
+        fail(PMH_E_VALUE, "polars_metal: " + std::string(tag_name(c.tag)) + " value buffer too short for " + nm +
+                              ": " + std::to_string(c.data_len) + " < " + std::to_string(need));
+    const uint64_t vneed = (n_rows + 7) / 8;
+    if (c.valid && c.valid_len < vneed)
+        fail(PMH_E_VALUE, "polars_metal: validity buffer too short for " + nm + ": " + std::to_string(c.valid_len) +
+                              " < " + std::to_string(vneed));
+}
+
 std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
                                   const std::vector<std::string> &needed) {
     Runtime &r = rt();
@@ -129,17 +142,10 @@ std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint6
         for (uint32_t i = 0; i < n_cols; ++i)
             if (cols[i].name && nm == cols[i].name) { c = &cols[i]; break; }
         if (!c) fail(PMH_E_KEY, "polars_metal: column not found: " + nm);
-        if (c->tag < 0 || c->tag > PMH_UTF8)
-            fail(PMH_E_VALUE, "polars_metal: unknown dtype tag for column " + nm);
-        int esz = tag_size(c->tag);
-        uint64_t need = esz == 0 ? (n_rows + 7) / 8 : n_rows * (uint64_t)esz;
-        if (c->data_len < need || (!c->data && need))
-            fail(PMH_E_VALUE, "polars_metal: " + std::string(tag_name(c->tag)) + " value buffer too short for " +
-                                  nm + ": " + std::to_string(c->data_len) + " < " + std::to_string(need));
-        uint64_t vneed = (n_rows + 7) / 8;
-        if (c->valid && c->valid_len < vneed)
-            fail(PMH_E_VALUE, "polars_metal: validity buffer too short for " + nm + ": " +
-                                  std::to_string(c->valid_len) + " < " + std::to_string(vneed));
+        check_column(*c, n_rows, nm);
+        const int esz = tag_size(c->tag);
+        const uint64_t need = esz == 0 ? (n_rows + 7) / 8 : n_rows * (uint64_t)esz;
+        const uint64_t vneed = (n_rows + 7) / 8;
         Column col;
         col.name = nm;
         col.tag = c->tag;

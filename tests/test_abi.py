@@ -128,3 +128,46 @@ def test_no_gpu_means_compute_error_not_fallback():
     with pytest.raises(Exception) as ei:
         _native.execute_groupby(_gb_plan(), 1, COLS)
     assert "no HIP device" in str(ei.value)
+
+
+def test_row_limit_is_2_31_before_any_device_work():
+    # The reference takes up to u32::MAX rows (filter.rs:73-77); this backend
+    # narrows that to 2^31 - 1 and refuses more with ValueError on the host,
+    # before the device is touched (so the same error here without a GPU).
+    from polars_metal import _native
+    for n in (1 << 31, (1 << 32) - 1):
+        with pytest.raises(ValueError, match=r"below 2\^31"):
+            _native.execute_groupby(_gb_plan(), n, COLS)
+        with pytest.raises(ValueError, match=r"below 2\^31"):
+            _native.execute_filter_groupby(_gb_plan(), n, COLS)
+
+
+def _fake_device_cols(rows, short=None):
+    # device pointers that are never dereferenced: the batch checks every
+    # query's buffers on the host first
+    cols = []
+    for name, tag, w in (("key", "I64", 8), ("v", "F64", 8), ("w", "I32", 4)):
+        n = rows * w - (8 if name == short else 0)
+        cols.append((name, tag, 0x10000, n, 0, 0))
+    return cols
+
+
+def test_batch_checks_every_query_before_any_device_work():
+    from polars_metal import batch
+    plan = {"keys": [["key", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "s"}]}
+    pred_w = {"kind": "Compare", "op": "Gt", "dtype": "I64", "lhs": {"kind": "Column", "name": "w", "dtype": "I64"},
+              "rhs": {"kind": "LiteralI64", "value": 0}}
+    good = (1000, _fake_device_cols(1000), None)
+    # a later query whose value buffer is shorter than its row count
+    with pytest.raises(ValueError, match="value buffer too short for v"):
+        batch.execute_batch(plan, [good, (1000, _fake_device_cols(1000, short="v"), None)], options={"tile": 4096})
+    # a column read only by a query's predicate is checked too
+    with pytest.raises(ValueError, match="value buffer too short for w"):
+        batch.execute_batch(plan, [good, (1000, _fake_device_cols(1000, short="w"), pred_w)], options={"tile": 4096})
+    # 2^31 rows in any query
+    with pytest.raises(ValueError, match=r"2\^31"):
+        batch.execute_batch(plan, [good, ((1 << 31), _fake_device_cols(1000), None)], options={"tile": 4096})
+    # a query without a column the template names
+    with pytest.raises(KeyError, match="has no column v"):
+        batch.execute_batch(plan, [good, (10, [c for c in _fake_device_cols(10) if c[0] != "v"], None)],
+                            options={"tile": 4096})

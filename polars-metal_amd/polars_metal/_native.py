@@ -28,7 +28,7 @@ from ._lib import PMH_DEVICE, PMH_HOST, PMH_OUT_DEVICE, PMH_OUT_HOST, TAGS, TAG_
 
 __all__ = [
     "version_string", "device_name", "execute_groupby", "execute_filter_groupby",
-    "execute_filter_groupby_device", "execute_filter_compact", "cmp_i64_col_scalar",
+    "execute_filter_groupby_device", "execute_filter_groupby_ptrs", "execute_filter_compact", "cmp_i64_col_scalar",
     "cmp_i64_col_col", "cmp_f64_col_scalar", "cmp_f64_col_col", "bool_and_dispatch",
     "bool_or_dispatch", "compute_lifting_plan", "warmup_common_fused_signatures",
     "set_profiling", "kernel_times", "last_stats",
@@ -70,30 +70,48 @@ def _plan_json(plan: dict) -> bytes:
 # (crates/polars-metal-core/src/udf/groupby/mod.rs:818-936) and groups on u32
 # codes; the same happens here, in Python, before the device call.
 # ---------------------------------------------------------------------------
-def _parse_utf8(data: bytes, valid: bytes, n_rows: int) -> list:
+def _utf8_codes(data: bytes, valid: bytes | None, n_rows: int):
     """Packed wire form ``[n u32][(n+1) x i32 offsets][bytes]``
-    (python/polars_metal/_udf.py:1497-1548) -> list of bytes / None."""
+    (python/polars_metal/_udf.py:1497-1548) -> (u32 codes, dictionary).
+    Codes are first-seen dictionary indices, null rows code 0 (the
+    reference's build_dict_nullable, crates/polars-metal-buffer/src/
+    dict.rs:40-66); the dictionary is built by Arrow's C++ hash table."""
+    import numpy as np
+    import pyarrow as pa
+    import pyarrow.compute as pc
     (n,) = struct.unpack_from("<I", data, 0)
     if n != n_rows:
         raise ValueError(f"polars_metal: Utf8 column has {n} rows, expected {n_rows}")
-    offs = struct.unpack_from(f"<{n + 1}i", data, 4)
-    base = 4 + 4 * (n + 1)
-    out = []
-    for i in range(n):
-        ok = (valid[i >> 3] >> (i & 7)) & 1 if valid else 1
-        out.append(data[base + offs[i]: base + offs[i + 1]] if ok else None)
-    return out
+    if len(data) < 4 + 4 * (n + 1):
+        raise ValueError("polars_metal: Utf8 column truncated in its offsets")
+    offs = pa.py_buffer(data[4:4 + 4 * (n + 1)])
+    blob = pa.py_buffer(data[4 + 4 * (n + 1):])
+    vb = pa.py_buffer(valid) if valid else None
+    arr = pa.Array.from_buffers(pa.string(), n, [vb, offs, blob], null_count=-1)
+    enc = pc.dictionary_encode(arr)
+    codes = np.ascontiguousarray(enc.indices.fill_null(0).cast(pa.uint32()).to_numpy(zero_copy_only=False))
+    return codes.tobytes(), enc.dictionary
 
 
-def _encode_utf8(values: list, valid_bits: list) -> bytes:
-    n = len(values)
-    offs = [0]
-    blob = bytearray()
-    for v, ok in zip(values, valid_bits):
-        if ok and v is not None:
-            blob += v
-        offs.append(len(blob))
-    return struct.pack("<I", n) + struct.pack(f"<{n + 1}i", *offs) + bytes(blob)
+def _encode_utf8(dictionary, codes: bytes, valid: bytes, n: int) -> bytes:
+    """u32 codes + dictionary -> the packed wire form (nulls as empty)."""
+    import numpy as np
+    import pyarrow as pa
+    if n == 0:
+        return struct.pack("<I", 0) + struct.pack("<i", 0)
+    ok = np.unpackbits(np.frombuffer(valid, np.uint8), bitorder="little")[:n].astype(bool)
+    c = np.frombuffer(codes, np.uint32, count=n)
+    if len(dictionary) == 0:
+        strs = pa.array([""] * n, pa.string())
+    else:
+        strs = dictionary.take(pa.array(np.where(ok, c, 0)))
+    lens = np.where(ok, np.diff(np.frombuffer(strs.buffers()[1], np.int32, count=n + 1, offset=4 * strs.offset)), 0)
+    offs = np.zeros(n + 1, np.int32)
+    np.cumsum(lens, out=offs[1:])
+    src = np.frombuffer(strs.buffers()[2], np.uint8) if strs.buffers()[2] is not None else np.zeros(0, np.uint8)
+    so = np.frombuffer(strs.buffers()[1], np.int32, count=n + 1, offset=4 * strs.offset)
+    blob = b"".join(bytes(src[so[i]:so[i + 1]]) for i in np.flatnonzero(ok)) if ok.any() else b""
+    return struct.pack("<I", n) + offs.tobytes() + blob
 
 
 @dataclass
@@ -114,15 +132,7 @@ def _prepare_columns(columns: Sequence[tuple], n_rows: int) -> _Prepared:
         tag = TAG_ALIASES.get(tag, tag)
         t = _tag(tag)
         if tag == "Utf8":
-            strings = _parse_utf8(bytes(data), bytes(valid) if valid else b"", n_rows)
-            lut: dict = {}
-            codes = bytearray(4 * n_rows)
-            for r, s in enumerate(strings):
-                if s is not None:
-                    code = lut.setdefault(s, len(lut))
-                    struct.pack_into("<I", codes, 4 * r, code)
-            dicts[name] = [k for k, _ in sorted(lut.items(), key=lambda kv: kv[1])]
-            data = bytes(codes)
+            data, dicts[name] = _utf8_codes(bytes(data), bytes(valid) if valid else None, n_rows)
         data = bytes(data) if not isinstance(data, bytes) else data
         valid = bytes(valid) if valid is not None and not isinstance(valid, bytes) else valid
         nb = name.encode("utf-8")
@@ -183,10 +193,7 @@ def _collect_host_result(res_p, dicts: dict) -> list:
         data = ctypes.string_at(c.data, c.data_len) if c.data_len else b""
         valid = ctypes.string_at(c.valid, c.valid_len) if c.valid_len else b""
         if tag == "Utf8":
-            lut = dicts.get(name, [])
-            codes = struct.unpack_from(f"<{n}I", data, 0) if n else ()
-            vbits = [(valid[j >> 3] >> (j & 7)) & 1 for j in range(n)]
-            data = _encode_utf8([lut[cd] if ok else None for cd, ok in zip(codes, vbits)], vbits)
+            data = _encode_utf8(dicts[name], data, valid, n)
         out.append((name, tag, data, valid))
     return out
 
@@ -305,9 +312,10 @@ def _device_to_host(ptr: int, nbytes: int) -> bytes:
     return buf.raw
 
 
-def _device_column_array(columns: list):
+def _device_column_array(columns: list, location: int = PMH_DEVICE):
     """PmhColumn array for ``(name, tag, data_ptr, data_len, valid_ptr,
-    valid_len)`` device columns, and the objects that must outlive it."""
+    valid_len)`` columns (device pointers, or host addresses with
+    ``location=PMH_HOST``), and the objects that must outlive it."""
     arr = (_lib.PmhColumn * max(1, len(columns)))()
     keep = []
     for i, (name, tag, dp, dl, vp, vl) in enumerate(columns):
@@ -315,7 +323,7 @@ def _device_column_array(columns: list):
         keep.append(nb)
         arr[i].name = nb
         arr[i].tag = _tag(tag)
-        arr[i].location = PMH_DEVICE
+        arr[i].location = location
         arr[i].data = dp
         arr[i].data_len = dl
         arr[i].valid = vp or None
@@ -333,6 +341,26 @@ def execute_filter_groupby_device(plan_dict: dict, n_rows: int, columns: Iterabl
     check(lib.pmh_execute_filter_groupby(_plan_json(plan_dict), n_rows, arr, len(columns), PMH_OUT_DEVICE,
                                          ctypes.byref(res_p)))
     return DeviceResult(res_p)
+
+
+def execute_filter_groupby_ptrs(plan_dict: dict, n_rows: int, columns: Iterable[tuple],
+                                location: int = PMH_HOST) -> list:
+    """The fused call over columns given by address, ``(name, tag, data_ptr,
+    data_len, valid_ptr, valid_len)`` (host memory by default, device
+    memory with ``location=PMH_DEVICE``); outputs come back as the
+    reference's host tuples.  Nothing is copied on the Python side."""
+    lib = _lib.load()
+    columns = list(columns)
+    arr, keep = _device_column_array(columns, location)
+    res_p = ctypes.POINTER(_lib.PmhResult)()
+    check(lib.pmh_execute_filter_groupby(_plan_json(plan_dict), n_rows, arr, len(columns), PMH_OUT_HOST,
+                                         ctypes.byref(res_p)))
+    try:
+        _LAST_STATS.clear()
+        _LAST_STATS.update(_stats_dict(res_p.contents.stats))
+        return _collect_host_result(res_p, {})
+    finally:
+        lib.pmh_result_free(res_p)
 
 
 def execute_filter_compact(pred_data: bytes, pred_valid: bytes, n_rows: int, columns: Sequence[tuple]) -> list:

@@ -6,15 +6,15 @@ device copy of known size, then ``traffic`` reads the four result databases
 and writes the bytes per launch, each counter scaled by what it reads for
 the copy (which moves exactly its size each way):
 
-    rocprofv3 --pmc FETCH_SIZE -d OUT/f -o run -- python3 rocprof.py zipf100m
-    rocprofv3 --pmc WRITE_SIZE -d OUT/w -o run -- python3 rocprof.py zipf100m
-    rocprofv3 --pmc FETCH_SIZE -d OUT/cf -o run -- python3 rocprof.py copy
-    rocprofv3 --pmc WRITE_SIZE -d OUT/cw -o run -- python3 rocprof.py copy
-    python3 rocprof.py traffic OUT hs_scan_fast > traffic.json
+    rocprofv3 --pmc FETCH_SIZE -d OUT/f -o run -- python3 rocprof_summary.py zipf100m
+    rocprofv3 --pmc WRITE_SIZE -d OUT/w -o run -- python3 rocprof_summary.py zipf100m
+    rocprofv3 --pmc FETCH_SIZE -d OUT/cf -o run -- python3 rocprof_summary.py copy
+    rocprofv3 --pmc WRITE_SIZE -d OUT/cw -o run -- python3 rocprof_summary.py copy
+    python3 rocprof_summary.py traffic OUT hs_scan_fast > traffic.json
 
 The counters are KiB per dispatch (summed over the device's channels).
 
-Kernel traces: ``python3 rocprof.py trace DB_DIR "title" "command"`` prints
+Kernel traces: ``python3 rocprof_summary.py trace DB_DIR "title" "command"`` prints
 a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database."""
 from __future__ import annotations
 
@@ -26,9 +26,9 @@ import sqlite3
 import sys
 from pathlib import Path
 
-_PKG = Path(__file__).resolve().parent
-if str(_PKG.parent) not in sys.path:
-    sys.path.insert(0, str(_PKG.parent))
+_PKG = Path(__file__).resolve().parent / "polars-metal_amd"
+if str(_PKG) not in sys.path:
+    sys.path.insert(0, str(_PKG))
 
 COPY_BYTES = 1 << 30
 COPY_KERNEL = "__amd_rocclr_copyBuffer"

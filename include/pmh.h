@@ -157,6 +157,33 @@ int pmh_execute_groupby(const char *plan_json, uint64_t n_rows, const pmh_column
 int pmh_execute_filter_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
                                uint32_t n_columns, uint32_t flags, pmh_result **out);
 
+/* Allocation-free form of pmh_execute_filter_groupby.  No counterpart in
+ * the reference; it is for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc, code next to HIP graph
+ * capture).  Every intermediate buffer lives in the library's static
+ * device array of PMH_NOALLOC_SCRATCH_BYTES, one per device, held until
+ * the process ends; the mailbox and events it uses are made at set-up.
+ *   - columns: device-resident (PMH_DEVICE) only; n_rows at most
+ *     PMH_NOALLOC_MAX_ROWS.  Larger inputs, host columns or a plan whose
+ *     buffers exceed the array are refused with PMH_E_VALUE.
+ *   - outputs: n_outputs caller-owned device columns, keys then aggregates
+ *     in plan order.  On entry data/data_len and valid/valid_len (valid may
+ *     be NULL: no bitmap wanted) are the caller's buffers and their sizes in
+ *     bytes; on return tag, location, data_len and valid_len describe what
+ *     was written (Bool data and bitmaps padded to 4 bytes, min 4, as for
+ *     pmh_execute_filter_groupby).  name is not touched.
+ *   - *n_groups receives the group count.  If an output is too small for
+ *     it the call returns PMH_E_VALUE with *n_groups set and no output
+ *     written, so the caller can size its buffers and call again.
+ * Calls on one device share its array and each clears what it needs
+ * first; calls that overlap on one device race.  Synchronous: outputs are
+ * complete on return. */
+#define PMH_NOALLOC_MAX_ROWS 262144u
+#define PMH_NOALLOC_SCRATCH_BYTES (256u << 20)
+int pmh_execute_filter_groupby_noalloc(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
+                                       uint32_t n_columns, pmh_out_column *outputs, uint32_t n_outputs,
+                                       uint64_t *n_groups);
+
 /* Replaces _native.execute_filter_compact
  * (crates/polars-metal-core/src/udf/compact.rs:46-147): keep = pred_data AND
  * pred_valid (bit-packed, host memory); every column (I64/F64/Bool/I32/F32/

@@ -4,6 +4,7 @@
 // (crates/polars-metal-core/src/error.rs:19-33).
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <set>
 #include <sstream>
@@ -84,6 +85,47 @@ int pmh_execute_filter_groupby(const char *plan_json, uint64_t n_rows, const pmh
         if (flags & ~PMH_OUT_DEVICE) fail(PMH_E_VALUE, "polars_metal: unknown flags");
         GroupByPlan plan = parse_groupby_plan(plan_json, true);
         *out = run_filter_groupby(plan, n_rows, columns, n_columns, flags);
+    });
+}
+
+int pmh_execute_filter_groupby_noalloc(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
+                                       uint32_t n_columns, pmh_out_column *outputs, uint32_t n_outputs,
+                                       uint64_t *n_groups) {
+    return guarded([&] {
+        if (!n_groups || (n_outputs && !outputs)) fail(PMH_E_VALUE, "polars_metal: outputs and n_groups are required");
+        *n_groups = 0;
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        if (n_rows > PMH_NOALLOC_MAX_ROWS)
+            fail(PMH_E_VALUE, "polars_metal: allocation-free group-by takes at most " +
+                                  std::to_string(PMH_NOALLOC_MAX_ROWS) + " rows, got " + std::to_string(n_rows));
+        for (uint32_t i = 0; i < n_columns; ++i)
+            if (columns[i].location != PMH_DEVICE)
+                fail(PMH_E_VALUE, "polars_metal: allocation-free group-by takes device-resident columns only");
+        GroupByPlan plan = parse_groupby_plan(plan_json, true);
+        const uint32_t n_out = (uint32_t)(plan.keys.size() + plan.aggs.size());
+        if (n_outputs != n_out)
+            fail(PMH_E_VALUE, "polars_metal: allocation-free group-by needs " + std::to_string(n_out) +
+                                  " output columns (keys, then aggregates), got " + std::to_string(n_outputs));
+        for (uint32_t i = 0; i < n_outputs; ++i)
+            if (!outputs[i].data && outputs[i].data_len)
+                fail(PMH_E_VALUE, "polars_metal: output " + std::to_string(i) + " has no data buffer");
+        pmh_result *r = nullptr;
+        {
+            const ScratchScope scope;
+            r = run_filter_groupby(plan, n_rows, columns, n_columns, PMH_OUT_DEVICE);
+            std::unique_ptr<pmh_result, void (*)(pmh_result *)> own(r, pmh_result_free);
+            *n_groups = r->n_rows;
+            const uint64_t ng = r->n_rows, vbytes = bitmap_bytes_padded(ng);
+            for (uint32_t i = 0; i < n_outputs; ++i) {
+                const pmh_out_column &c = r->columns[i];
+                const uint64_t need = c.tag == PMH_BOOL ? vbytes : ng * (uint64_t)tag_size(c.tag);
+                if (outputs[i].data_len < need || (outputs[i].valid && outputs[i].valid_len < vbytes))
+                    fail(PMH_E_VALUE, "polars_metal: output " + std::to_string(i) + " (" + c.name + ") holds fewer than " +
+                                          std::to_string(ng) + " groups (needs " + std::to_string(need) +
+                                          " data bytes and " + std::to_string(vbytes) + " validity bytes)");
+            }
+            copy_outputs_to_caller(*r, outputs, n_outputs);
+        }
     });
 }
 

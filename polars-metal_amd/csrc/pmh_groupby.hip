@@ -314,6 +314,19 @@ struct TableState {
 };
 TableState g_table;
 
+// g_table describes the "gb:table" slab; an allocation-free call's table is
+// in the scratch array, starts re-initialised and leaves g_table unchanged.
+struct TableScratchGuard {
+    TableState saved;
+    bool on;
+    explicit TableScratchGuard(bool scratch) : saved(g_table), on(scratch) {
+        if (on) g_table.clean = false;
+    }
+    ~TableScratchGuard() {
+        if (on) g_table = saved;
+    }
+};
+
 int agg_vkind(int tag) {
     if (tag == PMH_F64 || tag == PMH_F32) return VK_F;
     if (tag == PMH_I64) return VK_I64;
@@ -598,6 +611,7 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
 
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     Runtime &R = rt();
+    const TableScratchGuard table_guard(R.scratch_mode);
     const uint64_t n_rows = P.n_rows;
     const bool wide = P.wide;
     const int n_out = P.n_out;
@@ -616,6 +630,7 @@ pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
     uint64_t hint = plan.opts.capacity_hint;
     uint64_t cap;
+    if (R.scratch_mode) hint = std::max<uint64_t>(n_rows, 1);  // every row its own group fits: no resize
     if (hint > 0) cap = next_pow2(std::max<uint64_t>((uint64_t)(hint / 0.5), 64));
     else cap = next_pow2(std::min<uint64_t>(std::max<uint64_t>(n_rows / 64, 4096), 1ull << 21));
     const uint64_t initial_cap = cap;
@@ -665,9 +680,8 @@ pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     Launch *dL = (Launch *)slab("gb:launch", sizeof(Launch));
     uint32_t ctrl_h[8];
     int passes = 0;
-    hipEvent_t ev0, ev1;
-    PMH_HIP(hipEventCreate(&ev0));
-    PMH_HIP(hipEventCreate(&ev1));
+    PooledEvent pe0, pe1;
+    const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
     PMH_HIP(hipEventRecord(ev0, R.stream));
     for (;;) {
         ++passes;
@@ -745,8 +759,6 @@ pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     PMH_HIP(hipEventSynchronize(ev1));
     float dev_ms = 0.f;
     (void)hipEventElapsedTime(&dev_ms, ev0, ev1);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
 
     uint64_t ng = std::min<uint64_t>(ctrl_h[0], cap + 2);
     pmh_result *res = new_result(n_out);

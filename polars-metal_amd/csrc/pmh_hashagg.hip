@@ -1474,6 +1474,21 @@ struct FallbackState {
 };
 FallbackState g_fb;
 
+// g_fb describes the fallback table in the "hs:fb_table" slab.  An
+// allocation-free call builds its table in the scratch array instead: it
+// starts from an unknown (so re-initialised) table and leaves g_fb as it
+// found it.
+struct FallbackScratchGuard {
+    FallbackState saved;
+    bool on;
+    explicit FallbackScratchGuard(bool scratch) : saved(g_fb), on(scratch) {
+        if (on) g_fb.clean = false;
+    }
+    ~FallbackScratchGuard() {
+        if (on) g_fb = saved;
+    }
+};
+
 uint64_t pow2_floor(uint64_t x) {
     uint64_t p = 1;
     while (p * 2 <= x) p *= 2;
@@ -1702,6 +1717,7 @@ void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, cons
 
 pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     Runtime &R = rt();
+    const FallbackScratchGuard fb_guard(R.scratch_mode);
     const QueryDesc &q0 = P.q;
     const uint64_t n = P.n_rows;
     const bool wide = P.wide;
@@ -1874,9 +1890,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     uint32_t *mbox = (uint32_t *)(pin + up_bytes);
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
-    hipEvent_t ev0, ev1;
-    PMH_HIP(hipEventCreate(&ev0));
-    PMH_HIP(hipEventCreate(&ev1));
+    PooledEvent pe0, pe1;
+    const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
     PMH_HIP(hipEventRecord(ev0, R.stream));
     auto launch_check = [](void) { PMH_HIP(hipGetLastError()); };
     hipLaunchKernelGGL(hs_reset, dim3(1), dim3(64), 0, R.stream, dL);
@@ -1911,8 +1926,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     uint32_t sk[ST_N];
     std::vector<uint32_t> hist(1024);
     PMH_HIP(hipMemcpyAsync(mbox, L.stats, (ST_N + 1024) * 4, hipMemcpyDeviceToHost, R.stream));
-    hipEvent_t ev_sketch;
-    PMH_HIP(hipEventCreateWithFlags(&ev_sketch, hipEventDisableTiming));
+    PooledEvent pe_sketch;
+    const hipEvent_t ev_sketch = pe_sketch.ev;
     PMH_HIP(hipEventRecord(ev_sketch, R.stream));
     auto launch_scan = [&]() {
         const bool cached = q0.n_cols <= 4;
@@ -1949,9 +1964,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     };
     if (!opt.sketch_only) launch_scan();
     {
-        const hipError_t e = hipEventSynchronize(ev_sketch);
-        (void)hipEventDestroy(ev_sketch);
-        PMH_HIP(e);
+        PMH_HIP(hipEventSynchronize(ev_sketch));
     }
     memcpy(sk, mbox, sizeof(sk));
     memcpy(hist.data(), mbox + ST_N, 1024 * 4);
@@ -1962,8 +1975,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     sizer_features(sketch, sk[ST_SAMPLE_NULLKEY], q0.total_bits, P.feat);
     if (opt.sketch_only) {
         collect_kernel_times();
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
         pmh_result *res = new_result(0);
         res->stats.method = PMH_METHOD_HASH;
         res->stats.sample_rows = sk[ST_SAMPLE_SEL];
@@ -1974,7 +1985,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         res->stats.distinct_estimate = d_est;
         return res;
     }
-    if (opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && sizer_installed()) {
+    if (opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && sizer_installed() && !R.scratch_mode) {
         // the learned sizer picks the engine and, for hash, the cardinality
         // the partitions are sized for; overflow still falls back exactly
         float pred[2];
@@ -1983,8 +1994,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             P.route = 1;
             P.sizer_choice = 2;
             collect_kernel_times();
-            (void)hipEventDestroy(ev0);
-            (void)hipEventDestroy(ev1);
             return nullptr;
         }
         P.sizer_choice = 1;
@@ -2171,8 +2180,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     cursor = mbox[8 + ST_N];
     float dev_ms = 0.f;
     (void)hipEventElapsedTime(&dev_ms, ev0, ev1);
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
     if (host_stats[ST_ERROR]) {
         g_fb.clean = false;
         fail(PMH_E_COMPUTE, "polars_metal: partitioned group-by detected an inconsistent table state");

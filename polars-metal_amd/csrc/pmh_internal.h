@@ -202,11 +202,47 @@ struct Runtime {
     std::map<std::string, std::pair<int64_t, double>> ktimes;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     std::vector<hipEvent_t> event_pool;
+    // Allocation-free calls carve every slab from the library's static
+    // device array `scratch` (its device address, resolved once at set-up)
+    // and use the mailbox and events made at set-up.
+    uint8_t *scratch = nullptr;
+    size_t scratch_cap = 0;
+    bool scratch_mode = false;
+    size_t scratch_used = 0;
+    std::map<std::string, std::pair<size_t, size_t>> scratch_slabs;  // name -> (offset, bytes)
 };
 
 Runtime &rt();  // initialises the device on first use (throws PMH_E_COMPUTE)
 void *slab(const std::string &name, size_t bytes);  // grow-only named device buffer
 void *pinned_host(size_t bytes);
+
+// Bytes of the static scratch array, and the most rows an allocation-free
+// call takes (its slabs at that size fit the array for every engine).
+constexpr size_t kScratchBytes = PMH_NOALLOC_SCRATCH_BYTES;
+constexpr uint64_t kNoAllocMaxRows = PMH_NOALLOC_MAX_ROWS;
+
+// While alive, slabs come from the scratch array, bump-allocated by name
+// (a name asked again with no more bytes gets its earlier range), and a
+// request beyond the array fails with PMH_E_VALUE.  Restores slab() on exit.
+struct ScratchScope {
+    ScratchScope();
+    ~ScratchScope();
+};
+
+// Copies an engine's device result into caller buffers (data, then the
+// validity bitmap padded to 4 bytes, all ones where the result has none)
+// and waits for the copies.  Sizes are checked by the caller.
+void copy_outputs_to_caller(const pmh_result &r, pmh_out_column *outs, uint32_t n_outs);
+
+// An event from the runtime's pool (made at set-up), returned on scope exit;
+// with the pool empty a normal call makes one, an allocation-free call fails.
+struct PooledEvent {
+    PooledEvent();
+    ~PooledEvent();
+    PooledEvent(const PooledEvent &) = delete;
+    PooledEvent &operator=(const PooledEvent &) = delete;
+    hipEvent_t ev = nullptr;
+};
 // Launch bracketing for the device-time instrumentation.
 struct KernelTimer {
     KernelTimer(const char *name);

@@ -19,6 +19,17 @@ const char *last_error() { return g_last_error.c_str(); }
 
 static Runtime *g_rt = nullptr;
 
+// Device memory of the allocation-free entry point: one array per device,
+// held until the process ends.  Host code never uses this name as an
+// address; Runtime::scratch holds the device address hipGetSymbolAddress
+// returns for it.
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+// Pinned mailbox bytes and pooled events made at set-up, enough for every
+// single-query engine, so an allocation-free call needs no more.
+constexpr size_t kSetupPinnedBytes = size_t(1) << 20;
+constexpr int kSetupEvents = 8;
+
 Runtime &rt() {
     if (g_rt) return *g_rt;
     int n = 0;
@@ -30,13 +41,67 @@ Runtime &rt() {
     PMH_HIP(hipGetDeviceProperties(&r->prop, r->device));
     // Default priority: streams of different priorities are not used.
     PMH_HIP(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    // the scratch array's device address on this (the current) device
+    void *sp = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !sp) {
+        (void)hipGetLastError();
+        fail(PMH_E_COMPUTE, std::string("polars_metal: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(se));
+    }
+    r->scratch = (uint8_t *)sp;
+    r->scratch_cap = kScratchBytes;
+    for (int i = 0; i < kSetupEvents; ++i) {
+        hipEvent_t ev;
+        PMH_HIP(hipEventCreate(&ev));
+        r->event_pool.push_back(ev);
+    }
     g_rt = r;
+    (void)pinned_host(kSetupPinnedBytes);
     return *r;
+}
+
+ScratchScope::ScratchScope() {
+    Runtime &r = rt();
+    r.scratch_mode = true;
+    r.scratch_used = 0;
+    r.scratch_slabs.clear();
+}
+
+ScratchScope::~ScratchScope() {
+    Runtime &r = *g_rt;
+    r.scratch_mode = false;
+    r.scratch_slabs.clear();
+}
+
+PooledEvent::PooledEvent() {
+    Runtime &r = rt();
+    if (!r.event_pool.empty()) {
+        ev = r.event_pool.back();
+        r.event_pool.pop_back();
+        return;
+    }
+    if (r.scratch_mode) fail(PMH_E_RUNTIME, "polars_metal: allocation-free call ran out of pooled events");
+    PMH_HIP(hipEventCreate(&ev));
+}
+
+PooledEvent::~PooledEvent() {
+    if (ev) g_rt->event_pool.push_back(ev);
 }
 
 void *slab(const std::string &name, size_t bytes) {
     Runtime &r = rt();
     if (bytes == 0) bytes = 256;
+    if (r.scratch_mode) {
+        auto it = r.scratch_slabs.find(name);
+        if (it != r.scratch_slabs.end() && it->second.second >= bytes) return r.scratch + it->second.first;
+        const size_t off = (r.scratch_used + 255) & ~size_t(255);
+        if (off + bytes > r.scratch_cap)
+            fail(PMH_E_VALUE, "polars_metal: allocation-free call needs more than its " +
+                                  std::to_string(r.scratch_cap) + "-byte scratch buffer (at " + name + ")");
+        r.scratch_used = off + bytes;
+        r.scratch_slabs[name] = {off, bytes};
+        return r.scratch + off;
+    }
     auto it = r.slabs.find(name);
     if (it != r.slabs.end() && it->second.second >= bytes) return it->second.first;
     if (it != r.slabs.end()) {
@@ -55,6 +120,7 @@ void *slab(const std::string &name, size_t bytes) {
 void *pinned_host(size_t bytes) {
     Runtime &r = rt();
     if (r.pinned_size >= bytes && r.pinned) return r.pinned;
+    if (r.scratch_mode) fail(PMH_E_RUNTIME, "polars_metal: allocation-free call needs a larger pinned mailbox");
     if (r.pinned) {
         PMH_HIP(hipStreamSynchronize(r.stream));
         PMH_HIP(hipHostFree(r.pinned));
@@ -68,7 +134,7 @@ void *pinned_host(size_t bytes) {
 
 KernelTimer::KernelTimer(const char *n) : name(n) {
     Runtime &r = rt();
-    if (!r.profiling) return;
+    if (!r.profiling || r.scratch_mode) return;
     auto take = [&]() {
         hipEvent_t ev;
         if (!r.event_pool.empty()) { ev = r.event_pool.back(); r.event_pool.pop_back(); }
@@ -170,6 +236,52 @@ std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint6
         out.push_back(col);
     }
     return out;
+}
+
+// Bitmap bytes [0, vbytes) of a group count ng: all ones for rows < ng
+// (ones), or the copied bits kept; every bit from row ng on cleared.
+__global__ void tidy_bitmap(uint8_t *bm, uint64_t ng, uint64_t vbytes, int ones) {
+    const uint64_t full = ng >> 3;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < vbytes; i += (uint64_t)gridDim.x * blockDim.x) {
+        if (i < full) {
+            if (ones) bm[i] = 0xFF;
+        } else if (i == full && (ng & 7)) {
+            const uint8_t keep = (uint8_t)((1u << (ng & 7)) - 1);
+            bm[i] = ones ? keep : (uint8_t)(bm[i] & keep);
+        } else {
+            bm[i] = 0;
+        }
+    }
+}
+
+void copy_outputs_to_caller(const pmh_result &r, pmh_out_column *outs, uint32_t n_outs) {
+    Runtime &R = rt();
+    const uint64_t ng = r.n_rows;
+    const uint64_t vbytes = bitmap_bytes_padded(ng);
+    const uint64_t packed = (ng + 7) / 8;
+    for (uint32_t i = 0; i < n_outs; ++i) {
+        const pmh_out_column &src = r.columns[i];
+        pmh_out_column &dst = outs[i];
+        if (src.location != PMH_DEVICE) fail(PMH_E_RUNTIME, "polars_metal: allocation-free call got host outputs");
+        const bool bits = src.tag == PMH_BOOL;
+        const uint64_t need = bits ? vbytes : ng * (uint64_t)tag_size(src.tag);
+        if (need && src.data_len < (bits ? packed : need))
+            fail(PMH_E_RUNTIME, "polars_metal: allocation-free call: engine output shorter than its group count");
+        if (need) PMH_HIP(hipMemcpyAsync(dst.data, src.data, bits ? packed : need, hipMemcpyDeviceToDevice, R.stream));
+        if (bits && need) hipLaunchKernelGGL(tidy_bitmap, dim3(1), dim3(256), 0, R.stream, (uint8_t *)dst.data, ng, vbytes, 0);
+        if (dst.valid) {
+            if (src.valid && packed)
+                PMH_HIP(hipMemcpyAsync(dst.valid, src.valid, packed, hipMemcpyDeviceToDevice, R.stream));
+            hipLaunchKernelGGL(tidy_bitmap, dim3((uint32_t)std::min<uint64_t>(1024, vbytes / 256 + 1)), dim3(256), 0,
+                               R.stream, dst.valid, ng, vbytes, src.valid ? 0 : 1);
+        }
+        PMH_HIP(hipGetLastError());
+        dst.tag = src.tag;
+        dst.location = PMH_DEVICE;
+        dst.data_len = need;
+        dst.valid_len = dst.valid ? vbytes : 0;
+    }
+    PMH_HIP(hipStreamSynchronize(R.stream));
 }
 
 }  // namespace pmh

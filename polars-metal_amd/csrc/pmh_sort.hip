@@ -1144,9 +1144,8 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         L.fast = aligned && L.q.n_keys <= 4 && plan.opts.fast_scan != 0 && lower_fast_scan(L.q, L.fs) ? 1 : 0;
     }
     SLaunch *dL = (SLaunch *)slab("so:launch", sizeof(SLaunch));
-    hipEvent_t ev0, ev1;
-    PMH_HIP(hipEventCreate(&ev0));
-    PMH_HIP(hipEventCreate(&ev1));
+    PooledEvent pe0, pe1;
+    const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
     PMH_HIP(hipEventRecord(ev0, R.stream));
 
     // ---- 1. varying lane bits and the selected-row count
@@ -1299,8 +1298,6 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     PMH_HIP(hipEventSynchronize(ev1));
     float dev_ms = 0;
     PMH_HIP(hipEventElapsedTime(&dev_ms, ev0, ev1));
-    PMH_HIP(hipEventDestroy(ev0));
-    PMH_HIP(hipEventDestroy(ev1));
 
     pmh_result *res = new_result(P.n_out);
     res->n_rows = ng;

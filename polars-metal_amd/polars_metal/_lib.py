@@ -16,6 +16,7 @@ LIB_PATH = Path(os.environ.get("PMH_LIBRARY", _HERE / "libpmh.so"))
 PMH_OK, PMH_E_VALUE, PMH_E_KEY, PMH_E_NOTIMPL, PMH_E_RUNTIME, PMH_E_COMPUTE = range(6)
 PMH_HOST, PMH_DEVICE = 0, 1
 PMH_OUT_HOST, PMH_OUT_DEVICE = 0, 1
+PMH_NOALLOC_MAX_ROWS = 262144  # include/pmh.h
 
 TAGS = {
     "I64": 0, "F64": 1, "Bool": 2, "I32": 3, "F32": 4, "I8": 5, "I16": 6,
@@ -120,6 +121,9 @@ SIGNATURES = {
     "pmh_execute_filter_groupby": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                                   ctypes.c_uint32, ctypes.c_uint32,
                                                   ctypes.POINTER(ctypes.POINTER(PmhResult))]),
+    "pmh_execute_filter_groupby_noalloc": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
+                                                          ctypes.c_uint32, ctypes.POINTER(PmhOutColumn), ctypes.c_uint32,
+                                                          ctypes.POINTER(ctypes.c_uint64)]),
     "pmh_execute_filter_compact": (ctypes.c_int, [_U8P, _U8P, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                                   ctypes.c_uint32, ctypes.POINTER(ctypes.POINTER(PmhResult))]),
     "pmh_cmp_i64_col_scalar": (ctypes.c_int, [_P, _U8P, ctypes.c_int64, ctypes.c_int, ctypes.c_uint64, _U8P, _U8P]),

@@ -28,7 +28,8 @@ from ._lib import PMH_DEVICE, PMH_HOST, PMH_OUT_DEVICE, PMH_OUT_HOST, TAGS, TAG_
 
 __all__ = [
     "version_string", "device_name", "execute_groupby", "execute_filter_groupby",
-    "execute_filter_groupby_device", "execute_filter_groupby_ptrs", "execute_filter_compact", "cmp_i64_col_scalar",
+    "execute_filter_groupby_device", "execute_filter_groupby_ptrs", "execute_filter_groupby_noalloc",
+    "execute_filter_compact", "cmp_i64_col_scalar",
     "cmp_i64_col_col", "cmp_f64_col_scalar", "cmp_f64_col_col", "bool_and_dispatch",
     "bool_or_dispatch", "compute_lifting_plan", "warmup_common_fused_signatures",
     "set_profiling", "kernel_times", "last_stats",
@@ -361,6 +362,31 @@ def execute_filter_groupby_ptrs(plan_dict: dict, n_rows: int, columns: Iterable[
         return _collect_host_result(res_p, {})
     finally:
         lib.pmh_result_free(res_p)
+
+
+def execute_filter_groupby_noalloc(plan_dict: dict, n_rows: int, columns: Iterable[tuple], outputs: Sequence[tuple]):
+    """Allocation-free fused call (pmh_execute_filter_groupby_noalloc).
+    ``columns`` as for execute_filter_groupby_device; ``outputs`` one
+    ``(data_ptr, data_len, valid_ptr, valid_len)`` caller-owned device
+    buffer per output column (keys, then aggregates; valid_ptr 0 for none).
+    Returns ``(n_groups, [(tag, data_len, valid_len), ...])``."""
+    lib = _lib.load()
+    columns = list(columns)
+    arr, keep = _device_column_array(columns)
+    outs = (_lib.PmhOutColumn * max(1, len(outputs)))()
+    for i, (dp, dl, vp, vl) in enumerate(outputs):
+        outs[i].data, outs[i].data_len = dp or None, dl
+        outs[i].valid, outs[i].valid_len = vp or None, vl if vp else 0
+    ng = ctypes.c_uint64(0)
+    st = lib.pmh_execute_filter_groupby_noalloc(_plan_json(plan_dict), n_rows, arr, len(columns), outs, len(outputs),
+                                                ctypes.byref(ng))
+    if st != _lib.PMH_OK:
+        try:
+            check(st)
+        except ValueError as e:
+            e.n_groups = ng.value
+            raise
+    return ng.value, [(TAG_NAMES[outs[i].tag], outs[i].data_len, outs[i].valid_len) for i in range(len(outputs))]
 
 
 def execute_filter_compact(pred_data: bytes, pred_valid: bytes, n_rows: int, columns: Sequence[tuple]) -> list:

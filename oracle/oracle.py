@@ -74,6 +74,9 @@ def lib() -> ctypes.CDLL:
         L.orc_groupby.restype = ctypes.c_void_p
         L.orc_groupby.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcCol),
                                   ctypes.c_int, ctypes.POINTER(OrcCol), ctypes.c_int]
+        L.orc_groupby_par.restype = ctypes.c_void_p
+        L.orc_groupby_par.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcCol),
+                                      ctypes.c_int, ctypes.POINTER(OrcCol), ctypes.c_int, ctypes.c_int]
         L.orc_gb_n_groups.restype = ctypes.c_uint64
         L.orc_gb_n_groups.argtypes = [ctypes.c_void_p]
         L.orc_gb_lanes.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
@@ -233,11 +236,12 @@ def encode_keys(frame: dict, keys: list):
 # ---------------------------------------------------------------------------
 # group-by
 # ---------------------------------------------------------------------------
-def groupby(plan: dict, frame: dict, sum_mode: int = 0) -> dict:
+def groupby(plan: dict, frame: dict, sum_mode: int = 0, threads: int | None = None) -> dict:
     """Run plan (keys, aggs, optional predicate) on frame.  Returns an ordered
     dict ``name -> (tag, values, valid)`` with groups in first-occurrence
     order, plus the keys '__n_groups__' and '__ops__' (output alias -> agg
-    op, which the comparison helpers read)."""
+    op, which the comparison helpers read).  ``threads`` runs the all-core
+    form (orc_groupby_par): the same bits, groups in partition order."""
     n = frame_rows(frame)
     keep_mask = None
     if plan.get("predicate"):
@@ -252,8 +256,11 @@ def groupby(plan: dict, frame: dict, sum_mode: int = 0) -> dict:
     kcols = (OrcCol * max(1, len(keys)))(*[k.col(*frame[nm]) for nm in keys])
     vcols = (OrcCol * max(1, len(val_names)))(*[k.col(*frame[nm]) for nm in val_names])
     L = lib()
-    g = L.orc_groupby(n, keep_mask.ctypes.data if keep_mask is not None else None, len(keys), kcols,
-                      len(val_names), vcols, sum_mode)
+    km = keep_mask.ctypes.data if keep_mask is not None else None
+    if threads is None:
+        g = L.orc_groupby(n, km, len(keys), kcols, len(val_names), vcols, sum_mode)
+    else:
+        g = L.orc_groupby_par(n, km, len(keys), kcols, len(val_names), vcols, sum_mode, threads)
     if not g:
         raise ValueError("oracle: group-by rejected (key too wide?)")
     try:

@@ -49,6 +49,10 @@
 
 #include "pmh_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 /* ------------------------------------------------------------------ */
 /* column access                                                       */
 /* ------------------------------------------------------------------ */
@@ -133,14 +137,16 @@ static int apply_op(int op, int c) {
     }
 }
 
-int orc_eval_predicate(const orc_node *nodes, int n_nodes, const orc_col *cols,
-                       uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid) {
+/* Row r of a postfix predicate program -> (*d, *v); -1 on a malformed
+ * program (which fails on every row alike). */
+static int eval_row(const orc_node *nodes, int n_nodes, const orc_col *cols, uint64_t r,
+                    uint8_t *d, uint8_t *v) {
     slot_t st[32];
-    for (uint64_t r = 0; r < n_rows; r++) {
+    {
         int sp = 0;
         for (int k = 0; k < n_nodes; k++) {
             const orc_node *nd = &nodes[k];
-            slot_t s; memset(&s, 0, sizeof s);
+            slot_t s = {0, 0, 0.0, 0, 0};
             switch (nd->kind) {
             case ORC_N_COLUMN: {
                 const orc_col *c = &cols[nd->col];
@@ -186,9 +192,23 @@ int orc_eval_predicate(const orc_node *nodes, int n_nodes, const orc_col *cols,
             if (sp >= 32) return -1;
         }
         if (sp != 1) return -1;
-        out_valid[r] = (uint8_t)st[0].valid;
-        out_data[r] = (uint8_t)(st[0].valid && st[0].b);
+        *v = (uint8_t)st[0].valid;
+        *d = (uint8_t)(st[0].valid && st[0].b);
     }
+    return 0;
+}
+
+/* Rows are independent: with OpenMP the row loop runs on every host core
+ * (the CPU baseline); each row's result is the same either way. */
+int orc_eval_predicate(const orc_node *nodes, int n_nodes, const orc_col *cols,
+                       uint64_t n_rows, uint8_t *out_data, uint8_t *out_valid) {
+    if (n_rows == 0) return 0;
+    uint8_t d0, v0;
+    if (eval_row(nodes, n_nodes, cols, 0, &d0, &v0) != 0) return -1;
+    out_data[0] = d0;
+    out_valid[0] = v0;
+#pragma omp parallel for schedule(static)
+    for (uint64_t r = 1; r < n_rows; r++) (void)eval_row(nodes, n_nodes, cols, r, &out_data[r], &out_valid[r]);
     return 0;
 }
 
@@ -296,6 +316,7 @@ int orc_encode_keys(int n_keys, const orc_col *keys, uint64_t n_rows,
         off += 1 + b;
         if (off > 128) return -2; /* KeyEncodeError::TooWide */
     }
+#pragma omp parallel for schedule(static)
     for (uint64_t r = 0; r < n_rows; r++) {
         uint64_t l = 0, h = 0;
         for (int k = 0; k < n_keys; k++) {
@@ -500,4 +521,203 @@ void orc_gb_free(orc_gb *g) {
         free(g->min_i[j]); free(g->max_i[j]); free(g->min_f[j]); free(g->max_f[j]);
     }
     free(g);
+}
+
+/* ------------------------------------------------------------------ */
+/* all-core group-by (the CPU baseline)                                */
+/* ------------------------------------------------------------------ */
+
+/* The same group-by on every host core, bit-identical to orc_groupby in
+ * the same sum_mode: selected rows are hash-partitioned by their lane with
+ * a stable scatter (thread chunks in row order), so each partition holds
+ * its rows in row order, and each partition then folds its groups exactly
+ * as orc_groupby does -- every group's f64 sum adds the same values in the
+ * same order.  Groups come out partition by partition, each partition's in
+ * first-occurrence order.  n_threads <= 0 takes OpenMP's default. */
+orc_gb *orc_groupby_par(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_col *keys,
+                        int n_vals, const orc_col *vals, int sum_mode, int n_threads) {
+#ifndef _OPENMP
+    return orc_groupby(n_rows, keep, n_keys, keys, n_vals, vals, sum_mode);
+#else
+    if (n_vals > ORC_MAX_VALS || n_vals < 0) return NULL;
+    if (n_threads <= 0) n_threads = omp_get_max_threads();
+    const int T = n_threads;
+    enum { PBITS = 12, NP = 1 << PBITS };
+    const uint64_t na = n_rows ? n_rows : 1;
+    uint64_t *lo = (uint64_t *)malloc(sizeof(uint64_t) * na);
+    uint64_t *hi = (uint64_t *)malloc(sizeof(uint64_t) * na);
+    if (n_keys > 0) {
+        if (orc_encode_keys(n_keys, keys, n_rows, lo, hi) < 0) { free(lo); free(hi); return NULL; }
+    } else {
+        memset(lo, 0, sizeof(uint64_t) * na); memset(hi, 0, sizeof(uint64_t) * na);
+    }
+    uint64_t *cnt = (uint64_t *)calloc((size_t)T * NP, sizeof(uint64_t));
+    const uint64_t per = (n_rows + T - 1) / T;
+#define ORC_PART(r) (mix64(lo[r] ^ mix64(hi[r] + 0x9e3779b97f4a7c15ULL)) >> (64 - PBITS))
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t r0 = (uint64_t)t * per, r1 = r0 + per < n_rows ? r0 + per : n_rows;
+        uint64_t *c = cnt + (size_t)t * NP;
+        for (uint64_t r = r0; r < r1; r++)
+            if (!keep || keep[r]) c[ORC_PART(r)]++;
+    }
+    /* partition-major offsets: partition p's rows, thread chunk by chunk */
+    uint64_t *pstart = (uint64_t *)malloc(sizeof(uint64_t) * (NP + 1));
+    uint64_t run = 0;
+    for (int p = 0; p < NP; p++) {
+        pstart[p] = run;
+        for (int t = 0; t < T; t++) {
+            const uint64_t c = cnt[(size_t)t * NP + p];
+            cnt[(size_t)t * NP + p] = run;
+            run += c;
+        }
+    }
+    pstart[NP] = run;
+    const uint64_t n_sel = run;
+    uint32_t *idx = (uint32_t *)malloc(sizeof(uint32_t) * (n_sel ? n_sel : 1));
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const uint64_t r0 = (uint64_t)t * per, r1 = r0 + per < n_rows ? r0 + per : n_rows;
+        uint64_t *c = cnt + (size_t)t * NP;
+        for (uint64_t r = r0; r < r1; r++)
+            if (!keep || keep[r]) idx[c[ORC_PART(r)]++] = (uint32_t)r;
+    }
+#undef ORC_PART
+    free(cnt);
+    /* pass 1: group ids local to each partition, first rows in place */
+    uint32_t *lgid = (uint32_t *)malloc(sizeof(uint32_t) * (n_sel ? n_sel : 1));
+    uint32_t *firsts = (uint32_t *)malloc(sizeof(uint32_t) * (n_sel ? n_sel : 1));
+    uint64_t *png = (uint64_t *)calloc(NP + 1, sizeof(uint64_t));
+#pragma omp parallel num_threads(T)
+    {
+        uint64_t cap = 0;
+        uint64_t *tk_lo = NULL, *tk_hi = NULL;
+        uint32_t *tg = NULL;
+        uint8_t *used = NULL;
+#pragma omp for schedule(dynamic, 16)
+        for (int p = 0; p < NP; p++) {
+            const uint64_t s0 = pstart[p], s1 = pstart[p + 1];
+            uint64_t want = 16;
+            while (want < 2 * (s1 - s0)) want <<= 1;
+            /* grow by distinct keys: a hot partition of few keys stays small */
+            uint64_t tcap = want < 1024 ? want : 1024;
+            if (tcap > cap) {
+                free(tk_lo); free(tk_hi); free(tg); free(used);
+                cap = tcap;
+                tk_lo = (uint64_t *)malloc(8 * cap); tk_hi = (uint64_t *)malloc(8 * cap);
+                tg = (uint32_t *)malloc(4 * cap); used = (uint8_t *)malloc(cap);
+            }
+            uint64_t mask = tcap - 1;
+            memset(used, 0, tcap);
+            uint32_t ng = 0;
+            for (uint64_t i = s0; i < s1; i++) {
+                const uint32_t r = idx[i];
+                uint64_t h = mix64(lo[r] ^ mix64(hi[r] + 0x51afd7ed558ccdULL)) & mask;
+                for (;;) {
+                    if (!used[h]) {
+                        used[h] = 1; tk_lo[h] = lo[r]; tk_hi[h] = hi[r]; tg[h] = ng;
+                        firsts[s0 + ng] = r; lgid[i] = ng++; break;
+                    }
+                    if (tk_lo[h] == lo[r] && tk_hi[h] == hi[r]) { lgid[i] = tg[h]; break; }
+                    h = (h + 1) & mask;
+                }
+                if (2 * (uint64_t)ng > tcap && tcap < want) {
+                    /* rehash into twice the slots */
+                    const uint64_t ncap = tcap * 2;
+                    uint64_t *nlo = (uint64_t *)malloc(8 * ncap), *nhi = (uint64_t *)malloc(8 * ncap);
+                    uint32_t *ng_ = (uint32_t *)malloc(4 * ncap);
+                    uint8_t *nu = (uint8_t *)calloc(ncap, 1);
+                    for (uint64_t j = 0; j < tcap; j++) {
+                        if (!used[j]) continue;
+                        uint64_t q = mix64(tk_lo[j] ^ mix64(tk_hi[j] + 0x51afd7ed558ccdULL)) & (ncap - 1);
+                        while (nu[q]) q = (q + 1) & (ncap - 1);
+                        nu[q] = 1; nlo[q] = tk_lo[j]; nhi[q] = tk_hi[j]; ng_[q] = tg[j];
+                    }
+                    free(tk_lo); free(tk_hi); free(tg); free(used);
+                    tk_lo = nlo; tk_hi = nhi; tg = ng_; used = nu;
+                    tcap = cap = ncap;
+                    mask = tcap - 1;
+                }
+            }
+            png[p + 1] = ng;
+        }
+        free(tk_lo); free(tk_hi); free(tg); free(used);
+    }
+    for (int p = 0; p < NP; p++) png[p + 1] += png[p];
+    const uint64_t ng = png[NP];
+    orc_gb *g = (orc_gb *)calloc(1, sizeof(orc_gb));
+    g->n_groups = ng;
+    const uint64_t ga = ng ? ng : 1;
+    g->lane_lo = (uint64_t *)malloc(sizeof(uint64_t) * ga);
+    g->lane_hi = (uint64_t *)malloc(sizeof(uint64_t) * ga);
+    g->first_row = (uint32_t *)malloc(sizeof(uint32_t) * ga);
+    g->len = (uint32_t *)calloc(ga, sizeof(uint32_t));
+    g->n_vals = n_vals;
+    for (int j = 0; j < n_vals; j++) {
+        const int fl = is_float_tag(vals[j].tag);
+        g->val_tags[j] = vals[j].tag;
+        g->count[j] = (uint32_t *)calloc(ga, sizeof(uint32_t));
+        if (fl) {
+            g->sum_f[j] = (double *)calloc(ga, sizeof(double));
+            g->min_f[j] = (double *)malloc(sizeof(double) * ga);
+            g->max_f[j] = (double *)malloc(sizeof(double) * ga);
+        } else {
+            g->sum_i[j] = (int64_t *)calloc(ga, sizeof(int64_t));
+            g->min_i[j] = (int64_t *)malloc(sizeof(int64_t) * ga);
+            g->max_i[j] = (int64_t *)malloc(sizeof(int64_t) * ga);
+        }
+    }
+    /* pass 2: fold each partition's rows, in row order, into its groups */
+#pragma omp parallel for num_threads(T) schedule(dynamic, 16)
+    for (int p = 0; p < NP; p++) {
+        const uint64_t s0 = pstart[p], s1 = pstart[p + 1], g0 = png[p], gn = png[p + 1] - png[p];
+        for (uint64_t k = 0; k < gn; k++) {
+            const uint32_t r = firsts[s0 + k];
+            g->first_row[g0 + k] = r; g->lane_lo[g0 + k] = lo[r]; g->lane_hi[g0 + k] = hi[r];
+        }
+        for (uint64_t i = s0; i < s1; i++) g->len[g0 + lgid[i]]++;
+        for (int j = 0; j < n_vals; j++) {
+            const orc_col *c = &vals[j];
+            const int fl = is_float_tag(c->tag);
+            if (fl) for (uint64_t k = 0; k < gn; k++) { g->min_f[j][g0 + k] = INFINITY; g->max_f[j][g0 + k] = -INFINITY; }
+            else for (uint64_t k = 0; k < gn; k++) { g->min_i[j][g0 + k] = INT64_MAX; g->max_i[j][g0 + k] = INT64_MIN; }
+            xsum_t *xs = NULL;
+            if (fl && sum_mode == 0) {
+                xs = (xsum_t *)malloc(sizeof(xsum_t) * (gn ? gn : 1));
+                for (uint64_t k = 0; k < gn; k++) xsum_init(&xs[k]);
+            }
+            for (uint64_t i = s0; i < s1; i++) {
+                const uint32_t r = idx[i];
+                if (!valid_at(c->valid, r)) continue;
+                const uint64_t gi = g0 + lgid[i];
+                g->count[j][gi]++;
+                if (fl) {
+                    const double v = load_f64(c->tag, c->data, r);
+                    if (xs) xsum_add(&xs[lgid[i]], v); else g->sum_f[j][gi] += v;
+                    if (isnan(v) || isnan(g->min_f[j][gi])) g->min_f[j][gi] = canon_nan();
+                    else if (zero_signed_lt(v, g->min_f[j][gi])) g->min_f[j][gi] = v;
+                    if (isnan(v) || isnan(g->max_f[j][gi])) g->max_f[j][gi] = canon_nan();
+                    else if (zero_signed_lt(g->max_f[j][gi], v)) g->max_f[j][gi] = v;
+                } else {
+                    const int64_t v = load_i64(c->tag, c->data, r);
+                    g->sum_i[j][gi] = (int64_t)((uint64_t)g->sum_i[j][gi] + (uint64_t)v);
+                    if (v < g->min_i[j][gi]) g->min_i[j][gi] = v;
+                    if (v > g->max_i[j][gi]) g->max_i[j][gi] = v;
+                }
+            }
+            if (xs) {
+                for (uint64_t k = 0; k < gn; k++) {
+                    const double rr = xsum_result(&xs[k]);
+                    g->sum_f[j][g0 + k] = rr == 0.0 ? 0.0 : rr;
+                    xsum_free(&xs[k]);
+                }
+                free(xs);
+            }
+        }
+    }
+    free(lo); free(hi); free(idx); free(lgid); free(firsts); free(png); free(pstart);
+    return g;
+#endif
 }

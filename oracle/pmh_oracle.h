@@ -53,6 +53,8 @@ int orc_encode_keys(int n_keys, const orc_col *keys, uint64_t n_rows,
                     uint64_t *lo, uint64_t *hi);
 orc_gb *orc_groupby(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_col *keys,
                     int n_vals, const orc_col *vals, int sum_mode);
+orc_gb *orc_groupby_par(uint64_t n_rows, const uint8_t *keep, int n_keys, const orc_col *keys,
+                        int n_vals, const orc_col *vals, int sum_mode, int n_threads);
 uint64_t orc_gb_n_groups(const orc_gb *g);
 void orc_gb_lanes(const orc_gb *g, uint64_t *lo, uint64_t *hi, uint32_t *first_row, uint32_t *len);
 void orc_gb_val(const orc_gb *g, int j, uint32_t *count, int64_t *sum_i, int64_t *min_i,

@@ -245,3 +245,34 @@ def test_sequential_sum_mode_differs_only_by_rounding():
     exact = orc.groupby(plan, frame, sum_mode=0)["s"][1][0]
     seq = orc.groupby(plan, frame, sum_mode=1)["s"][1][0]
     assert exact == 100.0 and seq != exact and abs(seq - exact) < 1e-10
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_all_core_groupby_is_bit_identical(threads):
+    """The all-core oracle (the CPU baseline) against the one-thread oracle,
+    in both sum modes: same groups, same bits (its partitions keep every
+    group's rows in row order, so even row-order f64 sums agree)."""
+    from polars_metal import synth
+    from helpers import assert_results_equal
+    frame, plan = synth.zipf(300_000, seed=3)
+    frame["v"][1][::101] = np.nan
+    frame["v"][1][::103] = -0.0
+    for mode in (0, 1):
+        a = orc.groupby(plan, frame, sum_mode=mode)
+        b = orc.groupby(plan, frame, sum_mode=mode, threads=threads)
+        assert_results_equal(b, a, ["key"], label=f"par{threads}/mode{mode}")
+    hf, hp = synth.highcard(200_000, seed=4, pool=60_000)
+    a = orc.groupby(hp, hf, sum_mode=1)
+    b = orc.groupby(hp, hf, sum_mode=1, threads=threads)
+    assert a["__n_groups__"] > 40_000
+    assert_results_equal(b, a, ["a", "b"], label=f"par{threads}/highcard")
+    # one hot key: a partition far larger than the rest, few distinct keys
+    n = 100_000
+    f = {"k": ("I64", np.where(np.arange(n) % 3 == 0, 7, np.arange(n)), None),
+         "v": ("F64", np.random.default_rng(1).standard_normal(n), None)}
+    p = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op}
+                                          for op in ("Sum", "Min", "Max", "Count")]}
+    assert_results_equal(orc.groupby(p, f, sum_mode=1, threads=threads), orc.groupby(p, f, sum_mode=1), ["k"])
+    # empty input and a predicate that keeps nothing
+    e = {k: (t, v[:0], None) for k, (t, v, _) in frame.items()}
+    assert orc.groupby(plan, e, threads=threads)["__n_groups__"] == 0

@@ -473,6 +473,13 @@ __device__ __forceinline__ double limbs_to_double(const int64_t *L, double res) 
     return res != 0.0 ? q + res : q;
 }
 
+// Sum of one value per lane over the wave (every lane gets the sum).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
 // Exclusive scan of one value per thread across a block of at most 1024
 // threads (whole waves); s_wave holds 16 words.  Every thread of the block
 // must call it.

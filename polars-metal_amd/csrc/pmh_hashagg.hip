@@ -100,6 +100,7 @@ struct HLaunch {
     uint32_t *tile_hist;   // [kRadix][max_tiles]
     uint32_t *digit_base;  // [kRadix]
     uint64_t *part_start;  // [partitions + 1]: first spill row of each partition after the passes
+    uint32_t *tile_reg;    // [max_tiles]: scan region of each first-pass tile
     uint32_t *xtot;        // one-pass partitioning: [8][kWideRadix] rows per (XCD slot, digit)
     uint64_t *xcur;        // [8][kWideRadix]: next free row of each (XCD slot, digit) sub-region
     uint32_t max_tiles, pbits, sortbits, passes;
@@ -302,20 +303,29 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
     uint32_t drops = 0, sidekeys = 0;
-    // sampled keys in batches of kSampleU rows per thread, every load of a
-    // batch issued before the first probe
+    // sampled keys in batches of kSampleU rows per thread; the next batch's
+    // loads are issued before the current batch is probed, so the table
+    // work hides the sample's memory latency
     constexpr int kSampleU = 8;
+    uint64_t nlo[kSampleU], nhi[kSampleU];
+    uint8_t nsp[kSampleU];
+    auto load_batch = [&](uint32_t j0) __attribute__((always_inline)) {
+#pragma unroll
+        for (int u = 0; u < kSampleU; ++u) {
+            const uint32_t i = j0 + u * bs;
+            const uint32_t ii = i < nsel ? i : 0;
+            nlo[u] = gld<uint64_t>(L->samp_lo, ii);
+            nhi[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
+            nsp[u] = gld<uint8_t>(L->samp_sp, ii);
+        }
+    };
+    if (tid < nsel) load_batch(tid);
     for (uint32_t i0 = tid; i0 < nsel; i0 += bs * kSampleU) {
         uint64_t lo[kSampleU], hi[kSampleU];
         uint8_t sp[kSampleU];
 #pragma unroll
-        for (int u = 0; u < kSampleU; ++u) {
-            const uint32_t i = i0 + u * bs;
-            const uint32_t ii = i < nsel ? i : 0;
-            lo[u] = gld<uint64_t>(L->samp_lo, ii);
-            hi[u] = wide ? gld<uint64_t>(L->samp_hi, ii) : 0;
-            sp[u] = gld<uint8_t>(L->samp_sp, ii);
-        }
+        for (int u = 0; u < kSampleU; ++u) { lo[u] = nlo[u]; hi[u] = nhi[u]; sp[u] = nsp[u]; }
+        if (i0 + bs * kSampleU < nsel) load_batch(i0 + bs * kSampleU);
 #pragma unroll
         for (int u = 0; u < kSampleU; ++u) {
             if (i0 + u * bs >= nsel) break;
@@ -344,18 +354,28 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     if (drops) atomicAdd(&s_drop, drops);
     if (sidekeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], sidekeys);
     __syncthreads();
+    // counts 1 and 2 (most of the keys) are tallied in registers and give
+    // their histogram buckets once: thousands of LDS atomics on hist[1] and
+    // hist[2] would serialise on one address each
     uint32_t f1 = 0, f2 = 0, dist = 0;
     for (uint32_t i = tid; i < SS; i += bs) {
         const uint32_t c = s_cnt[i];
         if (!c) continue;
         ++dist;
         if (c == 1) ++f1;
-        if (c == 2) ++f2;
-        atomicAdd(&hist[min(c, 1023u)], 1u);
+        else if (c == 2) ++f2;
+        else atomicAdd(&hist[min(c, 1023u)], 1u);
     }
-    atomicAdd(&s_f1, f1);
-    atomicAdd(&s_f2, f2);
-    atomicAdd(&s_distinct, dist);
+    f1 = wave_sum(f1);
+    f2 = wave_sum(f2);
+    dist = wave_sum(dist);
+    if ((tid & 63) == 0) {
+        atomicAdd(&s_f1, f1);
+        atomicAdd(&s_f2, f2);
+        atomicAdd(&s_distinct, dist);
+    }
+    __syncthreads();
+    if (tid == 0) { hist[1] = s_f1; hist[2] = s_f2; }
     __syncthreads();
     for (uint32_t i = tid; i < 1024; i += bs) L->sketch_hist[i] = hist[i];
     {
@@ -534,12 +554,19 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
             if (!c) continue;
             ++gd;
             if (c == 1) ++g1;
-            if (c == 2) ++g2;
-            atomicAdd(&hist[min(c, 1023u)], 1u);
+            else if (c == 2) ++g2;
+            else atomicAdd(&hist[min(c, 1023u)], 1u);
         }
-        atomicAdd(&s_f1, g1);
-        atomicAdd(&s_f2, g2);
-        atomicAdd(&s_distinct, gd);
+        g1 = wave_sum(g1);
+        g2 = wave_sum(g2);
+        gd = wave_sum(gd);
+        if ((tid & 63) == 0) {
+            atomicAdd(&s_f1, g1);
+            atomicAdd(&s_f2, g2);
+            atomicAdd(&s_distinct, gd);
+        }
+        __syncthreads();
+        if (tid == 0) { hist[1] = s_f1; hist[2] = s_f2; }
         __syncthreads();
         for (uint32_t i = tid; i < 1024; i += bs) L->sketch_hist[i] = hist[i] * 8;
         if (tid == 0) {
@@ -742,7 +769,9 @@ __global__ __launch_bounds__(1024) void hs_regions(const HLaunch *__restrict__ L
         }
         if (b < L->nb) {
             L->region_off[b] = carry_a + s_a[threadIdx.x] - x;
-            L->tile_off[b] = carry_b + s_b[threadIdx.x] - y;
+            const uint32_t t0 = carry_b + s_b[threadIdx.x] - y;
+            L->tile_off[b] = t0;
+            for (uint32_t j = 0; j < y; ++j) L->tile_reg[t0 + j] = b;
         }
         __syncthreads();
         if (threadIdx.x == 1023) { carry_a += s_a[1023]; carry_b += s_b[1023]; }
@@ -775,13 +804,7 @@ __device__ __forceinline__ bool tile_span(const HLaunch *__restrict__ L, bool fi
         ts.rows = (uint32_t)min<uint64_t>(kPartTile, total - ts.src_begin);
         return true;
     }
-    // binary search: last region b with tile_off[b] <= t
-    uint32_t lo = 0, hi = L->nb;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) / 2;
-        if (L->tile_off[mid] <= t) lo = mid;
-        else hi = mid;
-    }
+    const uint32_t lo = L->tile_reg[t];  // the region holding tile t
     const uint32_t j = t - L->tile_off[lo];
     const uint32_t n_b = L->tail_n[lo];
     ts.src_begin = (uint64_t)lo * L->chunk + (uint64_t)j * kPartTile;
@@ -987,20 +1010,21 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     }
 }
 
-// One-pass partitioning, for at most 1024 partitions.  Rows only need to be
-// grouped by partition, not kept in a stable order, so a tile ranks its rows
-// with LDS counters and a single pass over the scan's tail replaces the
-// stable five-bit passes.  The digit is the partition: the low pbits of the
-// spill hash.
+// One pass, up to 1024 partitions.  Rows only need to be grouped by
+// partition, not kept in a stable order, so a tile ranks its rows with LDS
+// counters; the digit is the partition, the low pbits of the spill hash.
 //
 // The grid is a multiple of 8 and block b takes tiles b, b + grid, ..., so
 // all tiles of one "XCD slot" x = b mod 8 run on one XCD (workgroups are
 // dispatched round-robin over the eight XCDs).  Each partition is laid out
 // as eight sub-regions, one per slot, and a tile claims its runs in its
-// slot's sub-region with one atomic per digit: the runs that meet in one
-// sub-region are written through one XCD's L2, which merges their partial
-// lines before they reach HBM.  Placement decides only speed: both passes
-// use the same tile -> block -> slot mapping.
+// slot's sub-region with one atomic per digit: runs written at about the
+// same time by one XCD's tiles land next to each other and are written
+// through that XCD's L2, which completes their lines before they reach HBM.
+// (Exact per-block offsets, with no atomic, measured slower: a block's runs
+// then fill their lines only tile after tile, and the partial lines of every
+// block outgrow the L2.)  Placement decides only speed: both kernels use the
+// same tile -> block -> slot mapping.
 __global__ __launch_bounds__(kPartThreads) void rp_hist1(const HLaunch *__restrict__ L, const Spill *src_p,
                                                         uint32_t nd) {
     __shared__ uint32_t hist[kWideRadix];
@@ -1080,8 +1104,8 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
     uint16_t *st_d = st_fl + kPartTile;
     uint8_t *st_v = (uint8_t *)(st_d + kPartTile);
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        TileSpan ts;
-        tile_span(L, true, t, ntiles, total, ts);
+        TileSpan tsc;
+        tile_span(L, true, t, ntiles, total, tsc);
         run[tid] = 0;
         __syncthreads();
         // the first two value columns load with the keys, so the tile waits
@@ -1092,7 +1116,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + tid;
-            const uint64_t s = ts.src_begin + (i < ts.rows ? i : 0);
+            const uint64_t s = tsc.src_begin + (i < tsc.rows ? i : 0);
             klo[k] = gld_nt<uint64_t>(src.klo, s);
             khi[k] = wide ? gld_nt<uint64_t>(src.khi, s) : 0;
             fl[k] = gld_nt<uint16_t>(src.flags, s);
@@ -1102,7 +1126,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             dg[k] = (uint32_t)spill_hash(klo[k], khi[k], fl[k]) & (nd - 1);
-            rk[k] = k * kPartThreads + tid < ts.rows ? atomicAdd(&run[dg[k]], 1u) : 0xFFFFFFFFu;
+            rk[k] = k * kPartThreads + tid < tsc.rows ? atomicAdd(&run[dg[k]], 1u) : 0xFFFFFFFFu;
         }
         __syncthreads();
         {
@@ -1115,7 +1139,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             if (rk[k] == 0xFFFFFFFFu) continue;
-            const uint64_t s = ts.src_begin + k * kPartThreads + tid;
+            const uint64_t s = tsc.src_begin + k * kPartThreads + tid;
             const uint32_t sp = dstart[dg[k]] + rk[k];
             st_lo[sp] = klo[k];
             if (wide) st_hi[sp] = khi[k];
@@ -1130,7 +1154,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
         }
         __syncthreads();
         // write runs: consecutive staged rows -> consecutive positions
-        for (uint32_t sp = tid; sp < ts.rows; sp += kPartThreads) {
+        for (uint32_t sp = tid; sp < tsc.rows; sp += kPartThreads) {
             const uint32_t d = st_d[sp];
             const uint64_t g = gbase[d] + (sp - dstart[d]);
             gst<uint64_t>(dst.klo, g, st_lo[sp]);
@@ -1834,6 +1858,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)std::max(kRadix, kWideRadix) * max_tiles * 4 + 64);
     L.digit_base = (uint32_t *)slab("hs:digit_base", std::max(kRadix, kWideRadix) * 4 + 64);
     L.part_start = (uint64_t *)slab("hs:part_start", ((1ull << 15) + 2) * 8 + 64);
+    const uint32_t pgrid = (uint32_t)((cus * 2 + 7) / 8 * 8);  // partition kernels: a multiple of 8 (XCD slots)
+    L.tile_reg = (uint32_t *)slab("hs:tile_reg", (uint64_t)max_tiles * 4 + 64);
     L.xtot = (uint32_t *)slab("hs:xtot", 8 * kWideRadix * 4 + 64);
     L.xcur = (uint64_t *)slab("hs:xcur", 8 * kWideRadix * 8 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
@@ -2058,7 +2084,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
     launch_check();
-    const int pgrid = (cus * 2 + 7) / 8 * 8;  // a multiple of 8: the one-pass XCD slots
+
     const size_t stage_lds = (size_t)kPartTile * rec_bytes;
     PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)stage_lds));
     const Spill *bufs[3] = {&dL->spill, &dL->ping, &dL->pong};

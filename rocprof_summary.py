@@ -14,6 +14,9 @@ the copy (which moves exactly its size each way):
 
 The counters are KiB per dispatch (summed over the device's channels).
 
+``traffic_all OUT "title" "commands"`` prints the same scaled counters for
+every kernel of the run as a markdown table.
+
 Kernel traces: ``python3 rocprof_summary.py trace DB_DIR "title" "command"`` prints
 a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database."""
 from __future__ import annotations
@@ -89,6 +92,32 @@ def summarize(out: Path, kernel: str) -> dict:
                             "write_factor": write_factor}}
 
 
+def traffic_table(out: Path, steps: int, title: str, command: str) -> str:
+    """Markdown table of every kernel's FETCH_SIZE and WRITE_SIZE per
+    launch, each scaled by what the counter reads for the 1 GiB copy."""
+    f = counter_kib(out / "f", "FETCH_SIZE")
+    w = counter_kib(out / "w", "WRITE_SIZE")
+    cf = counter_kib(out / "cf", "FETCH_SIZE")
+    cw = counter_kib(out / "cw", "WRITE_SIZE")
+    ff = COPY_BYTES / (cf[COPY_KERNEL] * 1024.0)
+    wf = COPY_BYTES / (cw[COPY_KERNEL] * 1024.0)
+    lines = [f"# {title}", "", "Commands (from `/tmp`, with `REPO` the repository root, on one MI355X; each counter "
+             "in its own run):", "", "```", command, "```", "",
+             f"Counters are KiB per dispatch; FETCH_SIZE is scaled by {ff:.3f} and WRITE_SIZE by {wf:.3f}, the "
+             f"factors that make a {COPY_BYTES} B device copy read and write its size.", "",
+             "| kernel | fetch MB / launch | write MB / launch | total MB |", "|---|---|---|---|"]
+    tot_f = tot_w = 0.0
+    for k in sorted(set(f) | set(w), key=lambda k: -(f.get(k, 0) + w.get(k, 0))):
+        if k == COPY_KERNEL:
+            continue
+        fb, wb = f.get(k, 0.0) * 1024 * ff, w.get(k, 0.0) * 1024 * wf
+        tot_f += fb
+        tot_w += wb
+        lines.append(f"| `{k}` | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} |")
+    lines.append(f"| **all** | {tot_f / 1e6:.1f} | {tot_w / 1e6:.1f} | {(tot_f + tot_w) / 1e6:.1f} |")
+    return "\n".join(lines) + "\n"
+
+
 def short_name(name: str) -> str:
     n = name.replace("pmh::(anonymous namespace)::", "").replace("pmh::", "").replace("void ", "")
     return n.split("(")[0].strip()
@@ -115,7 +144,7 @@ def trace_markdown(db_dir: Path, title: str, command: str) -> str:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("zipf100m", "copy", "traffic", "trace"))
+    ap.add_argument("mode", choices=("zipf100m", "copy", "traffic", "traffic_all", "trace"))
     ap.add_argument("args", nargs="*")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--rows", type=int, default=100_000_000)
@@ -124,6 +153,8 @@ def main(argv=None) -> int:
         run_zipf100m(a.steps, a.rows)
     elif a.mode == "copy":
         run_copy(a.steps)
+    elif a.mode == "traffic_all":
+        print(traffic_table(Path(a.args[0]), a.steps, a.args[1], a.args[2]), end="")
     elif a.mode == "traffic":
         print(json.dumps(summarize(Path(a.args[0]), a.args[1] if len(a.args) > 1 else "hs_scan_fast"), indent=1))
     else:

@@ -94,52 +94,64 @@ def log(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def sizer_ab(n_jobs: int, seed: int = 777) -> dict:
-    """Live A/B of the learned sizer: each fresh job (the training stream's
-    distribution, another seed) runs under method "auto" without and with
-    the shipped model; both are warmed once per job, then timed once each,
-    alternating which goes first.  Results are identical by construction
-    (the engines' fallbacks keep every output exact); group counts are
-    checked."""
-    from polars_metal import _native, learned, synth
+def stream_ab(n_jobs: int, seed: int = 777, rows_hi: float = 1e7) -> dict:
+    """Learned sizing plus batching against the plain path on one stream of
+    fresh jobs from the sizer's training distribution (another seed): the
+    plain leg runs one call per job under method "auto" without a model;
+    the other runs stream.run (jobs of one plan template batched into one
+    captured launch, the rest alone, the shipped model installed).  Each leg
+    is warmed once and then timed end to end; every job's groups are
+    checked to be the same in both."""
+    from polars_metal import _native, learned, stream, synth
     from polars_metal.buffer import device_columns
-    specs = synth.sizer_job_specs(n_jobs, seed, rows_hi=1e7)
+    specs = synth.sizer_job_specs(n_jobs, seed, rows_hi=rows_hi)
     meta = learned.install_shipped()
     learned.install(None)
-    t_plain = t_learned = 0.0
-    n_rows = 0
-    picks = {"hash": 0, "sort": 0}
-    for i, spec in enumerate(specs):
+    jobs, bufs, names = [], [], []
+    for spec in specs:
         frame, plan = synth.sizer_job(spec)
-        cols, bufs = device_columns(frame)
-        del frame
-        n = spec["rows"]
-        n_rows += n
-        p = dict(plan, options={"method": "auto"})
-        times, groups = {}, {}
-        for mode in (("plain", "learned") if i % 2 == 0 else ("learned", "plain")):
-            if mode == "learned":
-                learned.install_shipped()
-            for rep in range(2):
-                t0 = time.perf_counter()
-                r = _native.execute_filter_groupby_device(p, n, cols)
-                dt = time.perf_counter() - t0
-                groups[mode] = r.stats["n_groups"]
-                if mode == "learned":
-                    picks[r.stats["method"]] = picks.get(r.stats["method"], 0) + (rep == 1)
-                r.free()
-            times[mode] = dt
-            learned.install(None)
-        if groups["plain"] != groups["learned"]:
-            raise AssertionError(f"sizer A/B job {i}: group counts differ")
-        t_plain += times["plain"]
-        t_learned += times["learned"]
-        del bufs
-    return {"speedup": t_plain / t_learned, "jobs": n_jobs, "rows": n_rows, "plain_ms": t_plain * 1e3,
-            "learned_ms": t_learned * 1e3, "learned_picks": picks,
-            "sample": f"{n_jobs} fresh jobs (seed {seed}): rows log-uniform [1e3, 1e7], distinct log-uniform "
-                      f"[1, rows], key power law s~U[0,1.5], i32/i64/u128 keys, nulls U[0,0.2]; wall time per "
-                      f"synchronous call after one warm call, method auto without vs with the shipped model",
+        cols, b = device_columns(frame)
+        jobs.append((plan, spec["rows"], cols))
+        bufs.append(b)
+        names.append([k[0] for k in plan["keys"]] + [a["output_alias"] for a in plan["aggs"]])
+    n_rows = sum(s_["rows"] for s_ in specs)
+    # warm both legs, then time each end to end; a device consumer would use
+    # each result in place, so the timed legs hand results over and free them
+    stream.run_plain(jobs)
+    stream.run(jobs)
+    t0 = time.perf_counter()
+    stream.run_plain(jobs)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    info = stream.run(jobs)
+    t_fast = time.perf_counter() - t0
+    # untimed: every job's groups, by an order-independent checksum, in both legs
+    sums = {"plain": {}, "fast": {}}
+    stream.run_plain(jobs, consume=lambda i, r: sums["plain"].__setitem__(i, group_checksum(r, names[i])))
+    stream.run(jobs, consume=lambda i, r: sums["fast"].__setitem__(i, group_checksum(r, names[i])))
+    if sums["plain"] != sums["fast"] or len(sums["plain"]) != n_jobs:
+        raise AssertionError("stream: results differ between the plain and the batched legs")
+    # the sizer alone: the same jobs one call each, with the model
+    learned.install_shipped()
+    try:
+        t0 = time.perf_counter()
+        picks = {"hash": 0, "sort": 0}
+        for plan, n, cols in jobs:
+            r = _native.execute_filter_groupby_device(dict(plan, options={"method": "auto"}), n, cols)
+            picks[r.stats["method"]] += 1
+            r.free()
+        t_sizer = time.perf_counter() - t0
+    finally:
+        learned.install(None)
+    del bufs
+    return {"speedup": t_plain / t_fast, "jobs": n_jobs, "rows": n_rows, "plain_ms": t_plain * 1e3,
+            "learned_batched_ms": t_fast * 1e3, "batched_jobs": info["batched"], "single_jobs": info["single"],
+            "batches": info["batches"], "sizer_only_speedup": t_plain / t_sizer, "sizer_picks": picks,
+            "sample": f"{n_jobs} fresh jobs (seed {seed}): rows log-uniform [1e3, {rows_hi:.0e}], distinct log-uniform "
+                      f"[1, rows], key power law s~U[0,1.5], i32/i64/98-bit keys, nulls U[0,0.2]; wall time of the "
+                      f"whole stream after one warm run of each leg; plain = one call per job, method auto, no "
+                      f"model; learned = jobs of one plan template in one batched launch (98-bit keys alone), "
+                      f"shipped model installed; every job's groups checked equal",
             "model_held_out_gain": meta.get("report", {}).get("held_out", {}).get("gain")}
 
 
@@ -391,8 +403,8 @@ def main() -> int:
         out["cpu_baseline"] = cpu_baseline(frame, plan, min(n, args.cpu_rows))
     del frame
     if not args.no_sizer:
-        log(f"learned sizer A/B on {args.sizer_jobs} fresh jobs")
-        out["learned_vs_plain"] = sizer_ab(args.sizer_jobs)
+        log(f"learned sizing + batching vs plain on {args.sizer_jobs} fresh jobs")
+        out["learned_vs_plain"] = stream_ab(args.sizer_jobs)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -757,7 +757,12 @@ struct BatchGraph {
     hipGraphExec_t exec = nullptr;
     std::string key;  // buffer addresses and launch shapes
 };
-BatchGraph g_graph;
+// The chains captured for recent batch shapes, most recent first: a stream
+// that alternates plan templates replays each one's graph instead of
+// capturing again.  A graph is replayed only when every buffer address and
+// launch shape it was captured with is the current one.
+constexpr size_t kGraphCache = 8;
+std::vector<BatchGraph> g_graphs;
 
 }  // namespace
 
@@ -1014,11 +1019,15 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
     snprintf(keybuf, sizeof(keybuf), "%p %p %p %p %p %p %p %zu %u %u %u %u %d %zu", (void *)dL, (void *)L.rk,
              (void *)L.part_cnt, (void *)L.part_off, (void *)L.t.klo, (void *)L.out[0].data, (void *)d_lds_off, lds_agg,
              grid, (unsigned)P.n_out, tile_block, agg_block, sig, scat_lds);
-    bool reused = g_graph.exec && g_graph.key == keybuf;
-    if (!reused) {
-        if (g_graph.exec) {
-            PMH_HIP(hipGraphExecDestroy(g_graph.exec));
-            g_graph.exec = nullptr;
+    size_t gi = 0;
+    while (gi < g_graphs.size() && g_graphs[gi].key != keybuf) ++gi;
+    const bool reused = gi < g_graphs.size();
+    if (reused) {
+        std::rotate(g_graphs.begin(), g_graphs.begin() + gi, g_graphs.begin() + gi + 1);
+    } else {
+        if (g_graphs.size() >= kGraphCache) {
+            PMH_HIP(hipGraphExecDestroy(g_graphs.back().exec));
+            g_graphs.pop_back();
         }
         hipGraph_t graph;
         PMH_HIP(hipStreamBeginCapture(R.stream, hipStreamCaptureModeThreadLocal));
@@ -1038,13 +1047,16 @@ void run_batch(const GroupByPlan &tmpl, const std::vector<BatchQueryIn> &qs, uin
         hipLaunchKernelGGL(bt_fb_emit, dim3(grid), dim3(256), 0, R.stream, dL);
         hipLaunchKernelGGL(bt_pack, dim3(grid), dim3(1024), 0, R.stream, dL);
         PMH_HIP(hipStreamEndCapture(R.stream, &graph));
-        PMH_HIP(hipGraphInstantiate(&g_graph.exec, graph, nullptr, nullptr, 0));
+        BatchGraph bg;
+        bg.key = keybuf;
+        const hipError_t ie = hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0);
         PMH_HIP(hipGraphDestroy(graph));
-        g_graph.key = keybuf;
+        PMH_HIP(ie);
+        g_graphs.insert(g_graphs.begin(), bg);
     }
     {
         KernelTimer kt("bt_graph");
-        PMH_HIP(hipGraphLaunch(g_graph.exec, R.stream));
+        PMH_HIP(hipGraphLaunch(g_graphs.front().exec, R.stream));
     }
     std::vector<uint32_t> cursor(nq), qsel(nq), qfb(nq);
     uint32_t fb_ctrl[2];

@@ -74,3 +74,35 @@ def test_sizer_job_at_1e8_rows(native):
         # every v is in [0, 1): each group's sum lies in [0, len)
         assert (sums >= 0).all() and (sums < lens).all()
         r.free()
+
+
+def test_stream_of_jobs_batched_with_the_model(native):
+    """stream.run: jobs of one plan template go through one batched launch
+    (98-bit keys alone), with the shipped model installed; every job
+    against the oracle, bit for bit, in job order."""
+    from polars_metal import stream
+    from polars_metal.buffer import device_columns
+    specs = synth.sizer_job_specs(36, seed=2029, rows_lo=1e3, rows_hi=3e5)
+    jobs, frames, bufs = [], [], []
+    for i, spec in enumerate(specs):
+        frame, plan = synth.sizer_job(spec)
+        if i % 4 == 0:  # a predicate on some jobs: the batch takes each query's own
+            plan = dict(plan, predicate=synth.pred_gt_f64("v", 0.25))
+        cols, b = device_columns(frame)
+        jobs.append((plan, spec["rows"], cols))
+        frames.append(frame)
+        bufs.append(b)
+    from test_batch import _fetch
+    seen = []
+
+    def check(i, r):
+        plan, n, _ = jobs[i]
+        assert_results_equal(_fetch(r), orc.groupby(plan, frames[i]), [k[0] for k in plan["keys"]], label=f"stream{i}")
+        seen.append(i)
+
+    info = stream.run(jobs, consume=check, tune=False)
+    assert info["batched"] >= 20 and info["single"] >= 1 and info["batches"] == 2
+    assert sorted(seen) == list(range(len(jobs)))
+    seen.clear()
+    stream.run_plain(jobs, consume=check)
+    assert sorted(seen) == list(range(len(jobs)))

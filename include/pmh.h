@@ -271,6 +271,29 @@ typedef struct pmh_mlp pmh_mlp;
  * sketch fields of *stats.  No counterpart in the reference. */
 int pmh_sketch_features(const char *plan_json, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols,
                         float *features, pmh_exec_stats *stats);
+/* The same sketch whole: its counts, the PMH_MLP_IN features and plain
+ * distinct estimate the sketch block derives from them, and the installed
+ * model's outputs as the sketch block computed them on the device.  Counts
+ * of a hash-subsampled recount (a sample whose fingerprint table dropped
+ * more than an eighth of it) are scaled back by 8.  No counterpart in the
+ * reference. */
+typedef struct pmh_sketch_info {
+    uint32_t sample_size;     /* rows sampled */
+    uint32_t sample_rows;     /* sampled rows that passed the predicate */
+    uint32_t distinct;        /* distinct keys counted among them (side keys excluded) */
+    uint32_t f1, f2;          /* keys counted once, twice */
+    uint32_t dropped;         /* sampled rows whose key found no fingerprint slot */
+    uint32_t null_keys;       /* sampled rows with a side key (a null or all-ones lane) */
+    uint32_t hot_keys;        /* keys placed in the hot table */
+    uint32_t lane_bits;       /* bits of the packed group key */
+    uint32_t scale;           /* 1, or 8 after a hash-subsampled recount */
+    uint32_t hist[1024];      /* keys counted c times (c clipped at 1023; hist[0] unused) */
+    float features[PMH_MLP_IN];
+    float prediction[2];      /* NaN, NaN with no model installed */
+    double distinct_estimate; /* the plain estimate the partitions are sized from without a model */
+} pmh_sketch_info;
+int pmh_sketch_inspect(const char *plan_json, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols,
+                       pmh_sketch_info *info);
 
 /* A device-resident model initialised from PMH_MLP_PARAMS host floats. */
 int pmh_mlp_new(const float *init_params, pmh_mlp **out);

@@ -301,6 +301,17 @@ int pmh_sketch_features(const char *plan_json, uint64_t n_rows, const pmh_column
     });
 }
 
+int pmh_sketch_inspect(const char *plan_json, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols,
+                       pmh_sketch_info *info) {
+    return guarded([&] {
+        if (!info) fail(PMH_E_VALUE, "polars_metal: info is NULL");
+        if (!plan_json) fail(PMH_E_VALUE, "polars_metal: plan parse error: plan is NULL");
+        *info = pmh_sketch_info{};
+        GroupByPlan plan = parse_groupby_plan(plan_json, true);
+        run_sketch(plan, n_rows, cols, n_cols, nullptr, nullptr, info);
+    });
+}
+
 struct pmh_mlp {
     pmh::Mlp *m;
 };

@@ -849,18 +849,20 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
 }
 
 // The partitioned engine's sample sketch alone: the learned sizer's
-// features and the sketch fields of the stats.
+// features and the sketch fields of the stats, or (info) the whole sketch
+// with the installed model's outputs.
 void run_sketch(GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols, float *feat,
-                pmh_exec_stats *stats) {
+                pmh_exec_stats *stats, pmh_sketch_info *info) {
     plan.opts.method = "hash";
     plan.opts.sketch_only = true;
     Prep P;
     prepare_query(P, plan, n_rows, cols, n_cols);
+    P.sketch_out = info;
     pmh_result *r = run_hash_partitioned(P, plan, 0);
     if (!r) fail(PMH_E_NOTIMPL, "polars_metal: no sample sketch for this query shape");
     if (stats) *stats = r->stats;
     pmh_result_free(r);
-    memcpy(feat, P.feat, sizeof(P.feat));
+    if (feat) memcpy(feat, P.feat, sizeof(P.feat));
 }
 
 void lower_predicate_into(QueryDesc &q, const GroupByPlan &plan, const Prep &P) {

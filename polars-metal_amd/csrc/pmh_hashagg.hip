@@ -46,6 +46,7 @@
 #include <cstring>
 
 #include "pmh_mlp.h"
+#include "pmh_mlp_dev.h"
 #include "pmh_table.h"
 
 namespace pmh {
@@ -72,6 +73,15 @@ enum StatIdx : int32_t {
     ST_SAMPLE_F2 = 5, ST_HOT = 6, ST_HOT_SAMPLE_ROWS = 7, ST_TILES0 = 8, ST_SAMPLE_NULLKEY = 9,
     ST_SAMPLE_DROPPED = 10, ST_ERROR = 11, ST_N = 16
 };
+// The sizer block follows the stats (words): the sketch's features, the
+// installed model's two outputs and the plain distinct estimate (f64), so
+// the host plans from one read-back of ST_N + SZ_N words.  The count
+// histogram follows it.
+enum SizerIdx : int32_t {
+    SZ_FEAT = 0, SZ_PRED = PMH_MLP_IN, SZ_EST = PMH_MLP_IN + 2, SZ_HAVE = PMH_MLP_IN + 4, SZ_SCALE = PMH_MLP_IN + 5,
+    SZ_N = 32
+};
+static_assert(SZ_SCALE < SZ_N && (ST_N + SZ_EST) % 2 == 0, "sizer block layout");
 
 struct HLaunch {
     QueryDesc q;
@@ -112,8 +122,10 @@ struct HLaunch {
     // outputs
     OutCol out[kOutMax];
     uint32_t *out_cursor;  // [0] groups written by hs_hot_emit and hs_part_agg
-    uint32_t *stats;       // StatIdx
+    uint32_t *stats;       // StatIdx, then the sizer block (SizerIdx)
     uint32_t *sketch_hist; // [1024]: sampled keys seen c times (c clipped at 1023)
+    const float *sizer;    // the installed sizer's parameters, or null: no forward in the sketch block
+    int32_t lane_bits, pad4;
     FastScan fs;
 };
 
@@ -281,6 +293,74 @@ __device__ __forceinline__ uint32_t hot_side_slot(int special, uint32_t CL) { re
 // key placed within kSampleProbe probes is found again within as many.
 constexpr int kSampleProbe = 8;
 
+// The end of the sketch block: the Shlosser sums over the count histogram
+// (thread t takes bucket t), then, on wave 0, the plain distinct estimate and
+// the sizer's features (lane i evaluates feature i, so the wave pays for one
+// logarithm and one division), then, with a model installed, its forward
+// pass on parameters staged in LDS while the sums ran.  hist holds the
+// counts divided by scale (8 after a hash-subsampled recount; buckets 1 and
+// 2 included).  Writes the sizer block after the stats.
+static_assert((PMH_MLP_IN & (PMH_MLP_IN - 1)) == 0 && PMH_MLP_IN <= 64, "one wave-0 lane per feature");
+__device__ void sketch_sizer(const HLaunch *__restrict__ L, uint32_t s_sel, uint32_t d, uint32_t f1, uint32_t f2,
+                             uint32_t dropped, uint32_t side_keys, const uint32_t *hist, uint32_t scale,
+                             float *s_par) {
+    __shared__ double s_red[3][16];
+    const uint32_t tid = threadIdx.x, bs = blockDim.x, lane = tid & 63;
+    if (L->sizer)  // uniform across the block
+        for (uint32_t i = tid; i < PMH_MLP_PARAMS; i += bs) s_par[i] = L->sizer[i];
+    Sketch k = sketch_head(s_sel, L->n_sample, L->q.n_rows, d, f1, f2, dropped);
+    const double qf = sketch_qf(k);
+    double F2 = 0, num = 0, den = 0;
+    for (uint32_t c = tid; c < 1024; c += bs)
+        if (c >= 1) sketch_bucket((int)c, (double)hist[c] * scale + (c == 1 ? (double)dropped : 0.0), qf, F2, num, den);
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        F2 += __shfl_xor(F2, o);
+        num += __shfl_xor(num, o);
+        den += __shfl_xor(den, o);
+    }
+    if (lane == 0) {
+        s_red[0][tid / 64] = F2;
+        s_red[1][tid / 64] = num;
+        s_red[2][tid / 64] = den;
+    }
+    __syncthreads();
+    if (tid >= 64) return;
+    const uint32_t nw = bs / 64;
+    F2 = lane < nw ? s_red[0][lane] : 0.0;
+    num = lane < nw ? s_red[1][lane] : 0.0;
+    den = lane < nw ? s_red[2][lane] : 0.0;
+#pragma unroll
+    for (int o = 32; o; o >>= 1) {
+        F2 += __shfl_xor(F2, o);
+        num += __shfl_xor(num, o);
+        den += __shfl_xor(den, o);
+    }
+    k.F2 = F2;
+    k.shl_num = num;
+    k.shl_den = den;
+    const double gee = gee_estimate(k), shl = shlosser_estimate(k), plain = plain_from(k, gee, shl);
+    double a, b;
+    const int kind = sizer_feature_arg((int)(lane & (PMH_MLP_IN - 1)), k, side_keys, L->lane_bits, gee, shl, plain, a, b);
+    const float f = sizer_feature_value(kind, a, b);
+    uint32_t *blk = L->stats + ST_N;
+    if (lane < PMH_MLP_IN) ((float *)blk)[SZ_FEAT + lane] = f;
+    if (lane == 0) {
+        *(double *)(blk + SZ_EST) = plain;
+        blk[SZ_HAVE] = L->sizer ? 1u : 0u;
+        blk[SZ_SCALE] = scale;
+    }
+    if (!L->sizer) return;
+    float x[PMH_MLP_IN];
+#pragma unroll
+    for (int i = 0; i < PMH_MLP_IN; ++i) x[i] = __shfl(f, i);
+    const mlpdev::Acts o = mlpdev::forward_lane(s_par, x, (int)lane);
+    if (lane == 0) {
+        ((float *)blk)[SZ_PRED + 0] = o.o0;
+        ((float *)blk)[SZ_PRED + 1] = o.o1;
+    }
+}
+
 __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restrict__ L) {
     extern __shared__ __align__(16) uint8_t smem[];
     const QueryDesc *__restrict__ q = &L->q;
@@ -293,13 +373,13 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     uint64_t *h_hi = h_lo + L->hot_slots;
     __shared__ uint32_t s_thresh, s_take_eq, s_hot, s_f1, s_f2, s_distinct, s_drop, s_hotrows;
     __shared__ uint32_t s_wave[16];
-    __shared__ uint32_t s_eq_used, s_npick;
+    __shared__ uint32_t s_eq_used, s_npick, s_side;
     const uint32_t tid = threadIdx.x, bs = blockDim.x;
     const uint32_t CL = L->hot_slots;
     for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
     for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
     for (uint32_t i = tid; i < CL; i += bs) { h_lo[i] = kEmpty; if (wide) h_hi[i] = kEmpty; }
-    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; s_eq_used = 0; s_npick = 0; }
+    if (tid == 0) { s_hot = 0; s_f1 = 0; s_f2 = 0; s_distinct = 0; s_drop = 0; s_take_eq = 0; s_hotrows = 0; s_eq_used = 0; s_npick = 0; s_side = 0; }
     __syncthreads();
     const uint32_t nsel = min(L->stats[ST_SAMPLE_SEL], L->n_sample);
     uint32_t drops = 0, sidekeys = 0;
@@ -352,7 +432,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         }
     }
     if (drops) atomicAdd(&s_drop, drops);
-    if (sidekeys) atomicAdd(&L->stats[ST_SAMPLE_NULLKEY], sidekeys);
+    if (sidekeys) atomicAdd(&s_side, sidekeys);
     __syncthreads();
     // counts 1 and 2 (most of the keys) are tallied in registers and give
     // their histogram buckets once: thousands of LDS atomics on hist[1] and
@@ -399,6 +479,7 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
         L->stats[ST_SAMPLE_F1] = s_f1;
         L->stats[ST_SAMPLE_F2] = s_f2;
         L->stats[ST_SAMPLE_DROPPED] = s_drop;
+        L->stats[ST_SAMPLE_NULLKEY] = s_side;
     }
     __syncthreads();
     // Picks, one slot per thread: keys with count > T, and keys with count
@@ -507,7 +588,8 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     // three bits clear, an eighth of the key space, with all of their sampled
     // rows, counted again in the cleared table.  Their counts times eight
     // estimate the whole sample's, and the estimators are linear in them.
-    if (s_drop * 8 > nsel) {
+    const bool recount = s_drop * 8 > nsel;  // uniform: s_drop is final since the count's barrier
+    if (recount) {
         __syncthreads();
         for (uint32_t i = tid; i < SS; i += bs) { s_fp[i] = 0; s_cnt[i] = 0; }
         for (uint32_t i = tid; i < 1024; i += bs) hist[i] = 0;
@@ -576,6 +658,12 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
             L->stats[ST_SAMPLE_DROPPED] = s_drop * 8;
         }
     }
+    __syncthreads();  // the LDS tables are free from here
+    const uint32_t sc = recount ? 8 : 1;
+    // the model's parameters (21 KB) are staged over the fingerprint array
+    // (at least 32 KB), away from the histogram the sums read
+    static_assert(PMH_MLP_PARAMS * 4 <= 8192 * 4, "sizer parameters fit the smallest fingerprint array");
+    sketch_sizer(L, nsel, s_distinct * sc, s_f1 * sc, s_f2 * sc, s_drop * sc, s_side, hist, sc, (float *)smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -1569,92 +1657,19 @@ size_t state_bytes(const Prep &P, uint64_t nslots) {
 
 Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
                    uint32_t dropped, const std::vector<uint32_t> &hist) {
-    Sketch k;
-    k.sample_rows = s_sel;
-    k.sample_size = n_sample;
-    k.n_rows = n;
-    // A dropped row's key found no slot within the probe cap, so none of
-    // that key's rows were counted.  Keys arriving late meet a fuller table,
-    // so dropped rows are mostly rare keys: each counts as a singleton.
-    k.distinct = d + dropped;
-    k.f1 = f1 + dropped;
-    k.f2 = f2;
-    k.dropped = dropped;
-    k.n_sel = n_sample ? (double)n * s_sel / n_sample : 0.0;
-    // F2 moment of the sample frequencies, and the Shlosser sums
-    const double qf = k.n_sel > 0 ? std::min(1.0, s_sel / k.n_sel) : 1.0;
+    Sketch k = sketch_head(s_sel, n_sample, n, d, f1, f2, dropped);
+    const double qf = sketch_qf(k);
     double F2 = 0, num = 0, den = 0;
     // hist[c]: sampled keys seen c times (c < 1024; a shorter histogram is
     // zero past its end)
     const int nc = (int)std::min<size_t>(hist.size(), 1024);
-    for (int c = 1; c < nc; ++c) {
-        const double h = (double)hist[c] + (c == 1 ? (double)dropped : 0.0);
-        if (h == 0) continue;
-        F2 += (double)c * c * h;
-        num += std::pow(1.0 - qf, c) * h;
-        den += c * qf * std::pow(1.0 - qf, c - 1) * h;
-    }
+    for (int c = 1; c < nc; ++c)
+        sketch_bucket(c, (double)hist[c] + (c == 1 ? (double)dropped : 0.0), qf, F2, num, den);
     k.F2 = F2;
     k.shl_num = num;
     k.shl_den = den;
     return k;
 }
-
-// Distinct keys among the selected rows: the geometric mean of GEE
-// (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
-// estimator (Haas et al., 1995; it overshoots), floored at the sample's own
-// distinct count and at Chao's lower bound (Chao, 1984).  A sample of singletons with no key seen twice (a
-// near-unique selection), or a fingerprint table that still dropped most of
-// it, extrapolates linearly; a few doubletons already say that keys repeat.
-double gee_estimate(const Sketch &k) {
-    if (k.sample_rows == 0) return 0.0;
-    const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
-    return std::sqrt(std::max(k.n_sel, s) / s) * f1 + (d - f1);
-}
-double shlosser_estimate(const Sketch &k) {
-    if (k.sample_rows == 0) return 0.0;
-    return k.shl_den > 0 ? k.distinct + k.f1 * k.shl_num / k.shl_den : gee_estimate(k);
-}
-double plain_distinct_estimate(const Sketch &k) {
-    if (k.sample_rows == 0) return 0.0;
-    const double s = k.sample_rows, d = k.distinct;
-    if (2.0 * k.dropped > s || (k.f2 == 0 && k.f1 >= 0.95 * s)) return std::max(d, k.n_sel * d / s);
-    if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
-    const double gee = gee_estimate(k), shl = shlosser_estimate(k);
-    // Chao's lower bound (bias-corrected Chao1) lifts the mean where keys
-    // repeat a few times each across a selection the sample barely covers
-    // (there GEE undershoots far more than Shlosser overshoots)
-    const double chao = std::min(k.n_sel, d + (double)k.f1 * (k.f1 > 0 ? k.f1 - 1.0 : 0.0) / (2.0 * (k.f2 + 1.0)));
-    return std::max({d, std::sqrt(std::max(gee, 1.0) * std::max(shl, 1.0)), chao});
-}
-
-// Features of the learned sizer: scale-free quantities of the sketch, each
-// roughly in [0, 1].
-void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f) {
-    for (int i = 0; i < PMH_MLP_IN; ++i) f[i] = 0.f;
-    const double s = std::max<double>(1.0, k.sample_rows);
-    auto lg = [](double x) { return (float)(std::log2(std::max(0.0, x) + 1.0) / 32.0); };
-    f[0] = lg((double)k.n_rows);
-    f[1] = k.sample_size ? (float)((double)k.sample_rows / k.sample_size) : 0.f;
-    f[2] = lg(k.distinct);
-    f[3] = (float)(k.f1 / s);
-    f[4] = (float)(k.f2 / s);
-    f[5] = (float)(k.distinct / s);
-    f[6] = lg(gee_estimate(k));
-    f[7] = lg(shlosser_estimate(k));
-    f[8] = lg(plain_distinct_estimate(k));
-    f[9] = (float)(k.F2 / (s * s));
-    f[10] = (float)(null_keys / s);
-    f[11] = (float)(k.dropped / s);
-    f[12] = lane_bits <= 33 ? 1.f : 0.f;
-    f[13] = lane_bits > 33 && lane_bits <= 65 ? 1.f : 0.f;
-    f[14] = lane_bits > 65 ? 1.f : 0.f;
-    f[15] = lg(k.n_sel);
-}
-
-namespace {
-
-}  // namespace
 
 // Lower the predicate for hs_scan_fast: an AND tree whose leaves compare one
 // column with a literal (any operator but Ne), test a Boolean column, or are
@@ -1864,8 +1879,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.xcur = (uint64_t *)slab("hs:xcur", 8 * kWideRadix * 8 + 64);
     L.out_cursor = (uint32_t *)slab("hs:cursor", 64);
     // stats and the sketch histogram are adjacent: one read-back
-    L.stats = (uint32_t *)slab("hs:stats", (ST_N + 1024) * 4 + 64);
-    L.sketch_hist = L.stats + ST_N;
+    L.stats = (uint32_t *)slab("hs:stats", (ST_N + SZ_N + 1024) * 4 + 64);
+    L.sketch_hist = L.stats + ST_N + SZ_N;
+    // the learned sizer runs inside the sketch block when "auto" will use it
+    const bool use_sizer = opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && !R.scratch_mode;
+    L.sizer = sizer_installed() && (use_sizer || opt.sketch_only) ? sizer_device_params() : nullptr;
+    L.lane_bits = q0.total_bits;
     // fallback table
     {
         GTable &t = L.t;
@@ -1911,7 +1930,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // of pageable memory.  The query ends synchronised, so a later query
     // never overwrites a staged copy still being read.
     const size_t up_bytes = (sizeof(HLaunch) + 255) & ~size_t(255);
-    uint8_t *pin = (uint8_t *)pinned_host(up_bytes + (ST_N + 1024 + 64) * 4);
+    uint8_t *pin = (uint8_t *)pinned_host(up_bytes + (ST_N + SZ_N + 1024 + 64) * 4);
     HLaunch *up = (HLaunch *)pin;
     uint32_t *mbox = (uint32_t *)(pin + up_bytes);
     memcpy(up, &L, sizeof(HLaunch));
@@ -1950,8 +1969,8 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // behind it in stream order.  A query the learned sizer then routes to
     // the sort engine has run one scan for nothing (its results are unused).
     uint32_t sk[ST_N];
-    std::vector<uint32_t> hist(1024);
-    PMH_HIP(hipMemcpyAsync(mbox, L.stats, (ST_N + 1024) * 4, hipMemcpyDeviceToHost, R.stream));
+    const size_t back_words = ST_N + SZ_N + (opt.sketch_only ? 1024 : 0);
+    PMH_HIP(hipMemcpyAsync(mbox, L.stats, back_words * 4, hipMemcpyDeviceToHost, R.stream));
     PooledEvent pe_sketch;
     const hipEvent_t ev_sketch = pe_sketch.ev;
     PMH_HIP(hipEventRecord(ev_sketch, R.stream));
@@ -1993,13 +2012,30 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         PMH_HIP(hipEventSynchronize(ev_sketch));
     }
     memcpy(sk, mbox, sizeof(sk));
-    memcpy(hist.data(), mbox + ST_N, 1024 * 4);
+    const uint32_t *blk = mbox + ST_N;
     const double tail_cap = 0.5 * TS;  // target keys per partition
-    Sketch sketch = sketch_from(sk[ST_SAMPLE_SEL], n_sample, n, sk[ST_SAMPLE_DISTINCT], sk[ST_SAMPLE_F1],
-                                sk[ST_SAMPLE_F2], sk[ST_SAMPLE_DROPPED], hist);
-    double d_est = plain_distinct_estimate(sketch);
-    sizer_features(sketch, sk[ST_SAMPLE_NULLKEY], q0.total_bits, P.feat);
+    double d_est;
+    memcpy(&d_est, blk + SZ_EST, 8);
+    memcpy(P.feat, blk + SZ_FEAT, sizeof(P.feat));
+    float pred[2] = {NAN, NAN};
+    if (blk[SZ_HAVE]) memcpy(pred, blk + SZ_PRED, sizeof(pred));
     if (opt.sketch_only) {
+        if (pmh_sketch_info *o = P.sketch_out) {
+            o->sample_size = n_sample;
+            o->sample_rows = sk[ST_SAMPLE_SEL];
+            o->distinct = sk[ST_SAMPLE_DISTINCT];
+            o->f1 = sk[ST_SAMPLE_F1];
+            o->f2 = sk[ST_SAMPLE_F2];
+            o->dropped = sk[ST_SAMPLE_DROPPED];
+            o->null_keys = sk[ST_SAMPLE_NULLKEY];
+            o->hot_keys = sk[ST_HOT];
+            o->lane_bits = (uint32_t)q0.total_bits;
+            o->scale = blk[SZ_SCALE];
+            memcpy(o->hist, blk + SZ_N, sizeof(o->hist));
+            memcpy(o->features, P.feat, sizeof(o->features));
+            memcpy(o->prediction, pred, sizeof(o->prediction));
+            o->distinct_estimate = d_est;
+        }
         collect_kernel_times();
         pmh_result *res = new_result(0);
         res->stats.method = PMH_METHOD_HASH;
@@ -2011,11 +2047,10 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         res->stats.distinct_estimate = d_est;
         return res;
     }
-    if (opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && sizer_installed() && !R.scratch_mode) {
-        // the learned sizer picks the engine and, for hash, the cardinality
-        // the partitions are sized for; overflow still falls back exactly
-        float pred[2];
-        sizer_predict(P.feat, pred);
+    if (L.sizer) {
+        // the learned sizer (evaluated in the sketch block) picks the engine
+        // and, for hash, the cardinality the partitions are sized for;
+        // overflow still falls back exactly
         if (pred[1] > 0.f) {
             P.route = 1;
             P.sizer_choice = 2;
@@ -2043,13 +2078,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         // projected from the sample) per partition, up to two radix passes
         const double hot_frac =
             sk[ST_SAMPLE_SEL] ? std::min(1.0, (double)sk[ST_HOT_SAMPLE_ROWS] / sk[ST_SAMPLE_SEL]) : 0.0;
-        const double tail_rows = sketch.n_sel * (1.0 - hot_frac);
+        const double n_sel = n_sample ? (double)n * sk[ST_SAMPLE_SEL] / n_sample : 0.0;
+        const double tail_rows = n_sel * (1.0 - hot_frac);
         P_parts = std::max<uint64_t>(P_parts, std::min<uint64_t>(1024, pow2_floor((uint64_t)(tail_rows / 8192) + 1)));
     }
     P_parts = std::min<uint64_t>(P_parts, 1u << 15);
     const int pbits = log2u(P_parts);
-    // up to 1024 partitions: one unstable pass on the partition digit, when
-    // its LDS staging (16-bit digits) fits
     // up to 1024 partitions: one unstable pass on the partition digit; more:
     // an unstable pass on the low pbits - 7 bits, then one stable 7-bit pass
     // (LSD order needs stability from the second pass on), when the unstable

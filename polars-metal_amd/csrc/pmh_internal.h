@@ -278,7 +278,7 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
 
 // sample sketch only: the learned sizer's features (pmh_groupby.hip)
 void run_sketch(GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols, float *feat,
-                pmh_exec_stats *stats);
+                pmh_exec_stats *stats, pmh_sketch_info *info = nullptr);
 
 // filter entries (pmh_filter.hip)
 pmh_result *run_filter_compact(const uint8_t *pd, const uint8_t *pv, uint64_t n_rows,

@@ -22,7 +22,10 @@ float mlp_train(Mlp *M, uint32_t epochs, uint32_t batch, float lr, float lambda,
 // The model "auto" consults after its sample sketch (nullptr uninstalls).
 void sizer_install(const float *params);
 bool sizer_installed();
-void sizer_predict(const float *feat, float *out);  // out[0]: log2(groups+1)/32, out[1]: logit(sort faster)
+// The installed parameters in device memory (nullptr: none).  The partitioned
+// engine's sketch block runs one query's forward pass on them; its outputs
+// are out[0] = log2(groups + 1) / 32, out[1] = logit(sort engine faster).
+const float *sizer_device_params();
 void sizer_predict_many(const float *feat, uint32_t n, float *out);  // n rows of features, 2n outputs
 
 }  // namespace pmh

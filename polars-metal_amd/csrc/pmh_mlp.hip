@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "pmh_mlp.h"
+#include "pmh_mlp_dev.h"
 
 namespace pmh {
 
@@ -28,11 +29,15 @@ namespace {
 
 constexpr int H = kMlpHidden;
 constexpr int F = kMlpIn;
-constexpr int O_W1 = 0, O_B1 = O_W1 + F * H, O_W2 = O_B1 + H, O_B2 = O_W2 + H * H, O_W3 = O_B2 + H,
-              O_B3 = O_W3 + H * 2, O_END = O_B3 + 2;
-static_assert(O_END == kMlpParams, "parameter layout");
-
-__device__ __forceinline__ float relu(float x) { return x > 0.f ? x : 0.f; }
+using mlpdev::O_W1;
+using mlpdev::O_B1;
+using mlpdev::O_W2;
+using mlpdev::O_B2;
+using mlpdev::O_W3;
+using mlpdev::O_B3;
+using mlpdev::relu;
+using Acts = mlpdev::Acts;
+static_assert(mlpdev::O_END == kMlpParams, "parameter layout");
 
 // Loads the parameters into LDS (every thread of the block helps).
 __device__ __forceinline__ void load_params(float *sp, const float *__restrict__ p) {
@@ -40,30 +45,8 @@ __device__ __forceinline__ void load_params(float *sp, const float *__restrict__
     __syncthreads();
 }
 
-struct Acts {
-    float h1, h2;  // this lane's hidden units
-    float o0, o1;  // outputs (uniform across the wave)
-};
-
 __device__ __forceinline__ Acts forward_one(const float *sp, const float *__restrict__ x, int lane) {
-    Acts a;
-    float z = sp[O_B1 + lane];
-#pragma unroll
-    for (int i = 0; i < F; ++i) z += sp[O_W1 + i * H + lane] * x[i];
-    a.h1 = relu(z);
-    z = sp[O_B2 + lane];
-#pragma unroll 8
-    for (int k = 0; k < H; ++k) z += sp[O_W2 + k * H + lane] * __shfl(a.h1, k);
-    a.h2 = relu(z);
-    float p0 = sp[O_W3 + lane * 2 + 0] * a.h2, p1 = sp[O_W3 + lane * 2 + 1] * a.h2;
-#pragma unroll
-    for (int o = 32; o; o >>= 1) {
-        p0 += __shfl_xor(p0, o);
-        p1 += __shfl_xor(p1, o);
-    }
-    a.o0 = p0 + sp[O_B3 + 0];
-    a.o1 = p1 + sp[O_B3 + 1];
-    return a;
+    return mlpdev::forward_lane(sp, x, lane);
 }
 
 __global__ __launch_bounds__(256) void mlp_forward(const float *__restrict__ params, const float *__restrict__ x,
@@ -334,31 +317,7 @@ void sizer_install(const float *params) {
 
 bool sizer_installed() { return g_sizer.on; }
 
-// One query's prediction, evaluated on the host: a single 16-64-64-2
-// forward is about 5K multiply-adds, far less than a device round trip
-// (copy in, launch, copy out, synchronise) on the query's critical path.
-// Same function as mlp_forward, up to the order of float additions.
-void sizer_predict(const float *feat, float *out) {
-    const float *p = g_sizer.host;
-    float h1[H], h2[H];
-    for (int j = 0; j < H; ++j) {
-        float z = p[O_B1 + j];
-        for (int i = 0; i < F; ++i) z += p[O_W1 + i * H + j] * feat[i];
-        h1[j] = z > 0.f ? z : 0.f;
-    }
-    for (int j = 0; j < H; ++j) {
-        float z = p[O_B2 + j];
-        for (int k = 0; k < H; ++k) z += p[O_W2 + k * H + j] * h1[k];
-        h2[j] = z > 0.f ? z : 0.f;
-    }
-    float o0 = p[O_B3 + 0], o1 = p[O_B3 + 1];
-    for (int j = 0; j < H; ++j) {
-        o0 += p[O_W3 + j * 2 + 0] * h2[j];
-        o1 += p[O_W3 + j * 2 + 1] * h2[j];
-    }
-    out[0] = o0;
-    out[1] = o1;
-}
+const float *sizer_device_params() { return g_sizer.on ? g_sizer.dev : nullptr; }
 
 void sizer_predict_many(const float *feat, uint32_t n, float *out) {
     if (!n) return;

@@ -512,6 +512,7 @@ struct Prep {
     uint64_t n_rows = 0;
     std::vector<Column> dcols;    // the query's columns, in QueryDesc.cols order
     float feat[PMH_MLP_IN] = {};  // the learned sizer's features, once the sketch ran
+    pmh_sketch_info *sketch_out = nullptr;  // sketch-only runs: filled with the whole sketch
     int route = 0;                // 1: the learned sizer sent the query to the sort engine
     int sizer_choice = 0;         // pmh_exec_stats::sizer_choice
     bool overflowed = false;      // the partitioned engine's fallback table overflowed
@@ -519,7 +520,9 @@ struct Prep {
     double d_retry = 0;           // > 0: size the partitioned engine for this many keys
 };
 
-// Sample sketch of the partitioned engine (host side).
+// Sample sketch of the partitioned engine.  The estimators and features
+// below are shared by the host (the batch plans from them) and the device
+// (the single-query sketch block evaluates them next to its counts).
 struct Sketch {
     uint32_t sample_rows = 0, sample_size = 0, distinct = 0, f1 = 0, f2 = 0, dropped = 0;
     uint64_t n_rows = 0;
@@ -527,11 +530,123 @@ struct Sketch {
 };
 Sketch sketch_from(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d, uint32_t f1, uint32_t f2,
                    uint32_t dropped, const std::vector<uint32_t> &hist);
-double plain_distinct_estimate(const Sketch &k);
-double gee_estimate(const Sketch &k);
-double shlosser_estimate(const Sketch &k);
-// The learned sizer's PMH_MLP_IN features of a sketch.
-void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f);
+
+// The sketch fields that do not need the count histogram.  A dropped row's
+// key found no slot within the probe cap, so none of that key's rows were
+// counted.  Keys arriving late meet a fuller table, so dropped rows are
+// mostly rare keys: each counts as a singleton.
+__host__ __device__ inline Sketch sketch_head(uint32_t s_sel, uint32_t n_sample, uint64_t n, uint32_t d,
+                                              uint32_t f1, uint32_t f2, uint32_t dropped) {
+    Sketch k;
+    k.sample_rows = s_sel;
+    k.sample_size = n_sample;
+    k.n_rows = n;
+    k.distinct = d + dropped;
+    k.f1 = f1 + dropped;
+    k.f2 = f2;
+    k.dropped = dropped;
+    k.n_sel = n_sample ? (double)n * s_sel / n_sample : 0.0;
+    return k;
+}
+// Fraction of the selection the sample covers.
+__host__ __device__ inline double sketch_qf(const Sketch &k) {
+    return k.n_sel > 0 ? fmin(1.0, k.sample_rows / k.n_sel) : 1.0;
+}
+// Bucket c of the count histogram (h keys seen c times, dropped rows
+// included in c = 1) adds to the F2 moment and the Shlosser sums.
+// (1 - q)^c by squaring: c < 1024 takes ten steps, against a general pow's
+// range reduction and polynomial on the sketch block's critical path.
+__host__ __device__ inline double sketch_ipow(double b, int e) {
+    double r = 1.0;
+    for (; e; e >>= 1, b *= b)
+        if (e & 1) r *= b;
+    return r;
+}
+__host__ __device__ inline void sketch_bucket(int c, double h, double qf, double &F2, double &num, double &den) {
+    if (h == 0) return;
+    const double t = sketch_ipow(1.0 - qf, c - 1);  // (1 - q)^(c - 1)
+    F2 += (double)c * c * h;
+    num += t * (1.0 - qf) * h;
+    den += c * qf * t * h;
+}
+
+// Distinct keys among the selected rows: the geometric mean of GEE
+// (Charikar et al., 2000; it undershoots on skewed keys) and Shlosser's
+// estimator (Haas et al., 1995; it overshoots), floored at the sample's own
+// distinct count and at Chao's lower bound (Chao, 1984).  A sample of
+// singletons with no key seen twice (a near-unique selection), or a
+// fingerprint table that still dropped most of it, extrapolates linearly; a
+// few doubletons already say that keys repeat.
+__host__ __device__ inline double gee_estimate(const Sketch &k) {
+    if (k.sample_rows == 0) return 0.0;
+    const double s = k.sample_rows, d = k.distinct, f1 = k.f1;
+    return sqrt(fmax(k.n_sel, s) / s) * f1 + (d - f1);
+}
+__host__ __device__ inline double shlosser_estimate(const Sketch &k) {
+    if (k.sample_rows == 0) return 0.0;
+    return k.shl_den > 0 ? k.distinct + k.f1 * k.shl_num / k.shl_den : gee_estimate(k);
+}
+// The estimate from GEE and Shlosser's values (gee, shl) of the same sketch.
+__host__ __device__ inline double plain_from(const Sketch &k, double gee, double shl) {
+    if (k.sample_rows == 0) return 0.0;
+    const double s = k.sample_rows, d = k.distinct;
+    if (2.0 * k.dropped > s || (k.f2 == 0 && k.f1 >= 0.95 * s)) return fmax(d, k.n_sel * d / s);
+    if (k.n_sel <= s * 1.0000001) return d;  // the sample is the whole selection
+    // Chao's lower bound (bias-corrected Chao1) lifts the mean where keys
+    // repeat a few times each across a selection the sample barely covers
+    // (there GEE undershoots far more than Shlosser overshoots)
+    const double chao = fmin(k.n_sel, d + (double)k.f1 * (k.f1 > 0 ? k.f1 - 1.0 : 0.0) / (2.0 * (k.f2 + 1.0)));
+    return fmax(fmax(d, sqrt(fmax(gee, 1.0) * fmax(shl, 1.0))), chao);
+}
+__host__ __device__ inline double plain_distinct_estimate(const Sketch &k) {
+    return plain_from(k, gee_estimate(k), shlosser_estimate(k));
+}
+
+// The learned sizer's PMH_MLP_IN features of a sketch: scale-free
+// quantities, each roughly in [0, 1].  Feature i is lg(a) = log2(a + 1) / 32,
+// a ratio a / b, or a constant a; sizer_feature_arg gives its kind and
+// operands, so the device evaluates all of them at once, one per lane.
+enum SizerFeatureKind : int { SF_LG = 0, SF_RATIO = 1, SF_CONST = 2 };
+__host__ __device__ inline int sizer_feature_arg(int i, const Sketch &k, uint32_t null_keys, int lane_bits,
+                                                 double gee, double shl, double plain, double &a, double &b) {
+    const double s = fmax(1.0, (double)k.sample_rows);
+    b = s;
+    switch (i) {
+    case 0: a = (double)k.n_rows; return SF_LG;
+    case 1:
+        a = (double)k.sample_rows;
+        b = (double)k.sample_size;
+        if (!k.sample_size) { a = 0.0; return SF_CONST; }
+        return SF_RATIO;
+    case 2: a = k.distinct; return SF_LG;
+    case 3: a = k.f1; return SF_RATIO;
+    case 4: a = k.f2; return SF_RATIO;
+    case 5: a = k.distinct; return SF_RATIO;
+    case 6: a = gee; return SF_LG;
+    case 7: a = shl; return SF_LG;
+    case 8: a = plain; return SF_LG;
+    case 9: a = k.F2; b = s * s; return SF_RATIO;
+    case 10: a = null_keys; return SF_RATIO;
+    case 11: a = k.dropped; return SF_RATIO;
+    case 12: a = lane_bits <= 33 ? 1.0 : 0.0; return SF_CONST;
+    case 13: a = lane_bits > 33 && lane_bits <= 65 ? 1.0 : 0.0; return SF_CONST;
+    case 14: a = lane_bits > 65 ? 1.0 : 0.0; return SF_CONST;
+    case 15: a = k.n_sel; return SF_LG;
+    default: a = 0.0; return SF_CONST;
+    }
+}
+__host__ __device__ inline float sizer_lg(double x) { return (float)(log2(fmax(0.0, x) + 1.0) / 32.0); }
+__host__ __device__ inline float sizer_feature_value(int kind, double a, double b) {
+    return kind == SF_LG ? sizer_lg(a) : kind == SF_RATIO ? (float)(a / b) : (float)a;
+}
+inline void sizer_features(const Sketch &k, uint32_t null_keys, int lane_bits, float *f) {
+    const double gee = gee_estimate(k), shl = shlosser_estimate(k), plain = plain_from(k, gee, shl);
+    for (int i = 0; i < PMH_MLP_IN; ++i) {
+        double a, b;
+        const int kind = sizer_feature_arg(i, k, null_keys, lane_bits, gee, shl, plain, a, b);
+        f[i] = sizer_feature_value(kind, a, b);
+    }
+}
 
 void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols, uint32_t n_cols);
 pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags);

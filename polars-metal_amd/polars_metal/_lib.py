@@ -75,6 +75,25 @@ class PmhExecStats(ctypes.Structure):
     ]
 
 
+class PmhSketchInfo(ctypes.Structure):
+    _fields_ = [
+        ("sample_size", ctypes.c_uint32),
+        ("sample_rows", ctypes.c_uint32),
+        ("distinct", ctypes.c_uint32),
+        ("f1", ctypes.c_uint32),
+        ("f2", ctypes.c_uint32),
+        ("dropped", ctypes.c_uint32),
+        ("null_keys", ctypes.c_uint32),
+        ("hot_keys", ctypes.c_uint32),
+        ("lane_bits", ctypes.c_uint32),
+        ("scale", ctypes.c_uint32),
+        ("hist", ctypes.c_uint32 * 1024),
+        ("features", ctypes.c_float * 16),
+        ("prediction", ctypes.c_float * 2),
+        ("distinct_estimate", ctypes.c_double),
+    ]
+
+
 class PmhBatchQuery(ctypes.Structure):
     _fields_ = [
         ("n_rows", ctypes.c_uint64),
@@ -146,6 +165,8 @@ SIGNATURES = {
     "pmh_kernel_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "pmh_sketch_features": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                            ctypes.c_uint32, _FP, ctypes.POINTER(PmhExecStats)]),
+    "pmh_sketch_inspect": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
+                                          ctypes.c_uint32, ctypes.POINTER(PmhSketchInfo)]),
     "pmh_mlp_new": (ctypes.c_int, [_FP, ctypes.POINTER(_P)]),
     "pmh_mlp_free": (None, [_P]),
     "pmh_mlp_params": (ctypes.c_int, [_P, _FP]),

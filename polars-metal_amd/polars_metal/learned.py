@@ -146,6 +146,22 @@ def sketch_features(plan: dict, n_rows: int, columns: list) -> tuple[np.ndarray,
     return feat, _stats_dict(st)
 
 
+def sketch_inspect(plan: dict, n_rows: int, columns: list) -> dict:
+    """The whole sample sketch of a query: its counts and histogram, the
+    features, the plain distinct estimate, and the installed model's outputs
+    as the device's sketch block computed them (NaN without a model)."""
+    lib = _lib.load()
+    arr, keep = _device_column_array(list(columns))
+    info = _lib.PmhSketchInfo()
+    _lib.check(lib.pmh_sketch_inspect(json.dumps(plan).encode(), n_rows, arr, len(columns), ctypes.byref(info)))
+    del keep
+    out = {name: getattr(info, name) for name, _ in _lib.PmhSketchInfo._fields_}
+    out["hist"] = np.ctypeslib.as_array(info.hist).copy()
+    out["features"] = np.ctypeslib.as_array(info.features).copy()
+    out["prediction"] = np.ctypeslib.as_array(info.prediction).copy()
+    return out
+
+
 def targets(n_groups: int, t_hash_ms: float | None, t_sort_ms: float | None) -> np.ndarray:
     y1 = -1.0 if t_hash_ms is None or t_sort_ms is None else (1.0 if t_sort_ms < t_hash_ms else 0.0)
     return np.array([np.log2(n_groups + 1.0) / 32.0, y1], np.float32)

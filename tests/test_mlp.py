@@ -1,11 +1,14 @@
 """The learned sizer's network: the CPU restatement (oracle/mlp.py) pinned by
 finite differences and Adam's closed form, then the device kernels
 (pmh_mlp_*) checked against it, trained, and used for routing."""
+import math
+
 import numpy as np
 import pytest
 
 import mlp as mo
 import oracle as orc
+import sizer as osz
 from helpers import assert_results_equal, run_gpu
 
 
@@ -110,6 +113,63 @@ def test_device_training_fits_a_known_function(native):
 
 
 # ---------------------------------------------------------------------------
+# CPU: pin the restatement of the sketch estimators and features
+# ---------------------------------------------------------------------------
+def _hist(**f):
+    h = np.zeros(1024, np.int64)
+    for c, v in f.items():
+        h[int(c[1:])] = v
+    return h
+
+
+def test_sketch_estimators_hand_worked():
+    """f1 = 4, f2 = 2, f3 = 1 (d = 7, s = 11 rows) sampled from 22 rows
+    (q = 1/2): every estimator worked out by hand."""
+    k = osz.sketch(_hist(c1=4, c2=2, c3=1), s_sel=11, n_sample=11, n_rows=22, d=7, f1=4, f2=2, dropped=0)
+    assert k["n_sel"] == 22.0
+    assert k["F2"] == 1 * 4 + 4 * 2 + 9 * 1
+    # sum (1-q)^c f_c = 4/2 + 2/4 + 1/8; sum c q (1-q)^(c-1) f_c = 4/2 + 2*2/4 + 3/8
+    assert math.isclose(k["shl_num"], 2.625) and math.isclose(k["shl_den"], 3.375)
+    assert math.isclose(osz.shlosser(k), 7 + 4 * 2.625 / 3.375)
+    assert math.isclose(osz.gee(k), math.sqrt(2.0) * 4 + 3)
+    assert math.isclose(osz.chao1(k), 7 + 4 * 3 / (2 * 3))
+    want = max(7.0, math.sqrt((math.sqrt(2.0) * 4 + 3) * (7 + 4 * 2.625 / 3.375)), 9.0)
+    assert math.isclose(osz.plain_estimate(k), want)
+
+
+def test_sketch_estimator_regimes():
+    # the sample is the whole selection (q = 1): Shlosser adds nothing, the estimate is d
+    k = osz.sketch(_hist(c1=5, c2=3), s_sel=11, n_sample=11, n_rows=11, d=8, f1=5, f2=3, dropped=0)
+    assert osz.shlosser(k) == 8 and osz.plain_estimate(k) == 8
+    # GEE scales the singletons by sqrt(n_sel / s): sqrt(100) * 40 + 20
+    k = osz.sketch(_hist(c1=40, c2=20, c3=20), s_sel=100, n_sample=100, n_rows=10_000, d=60, f1=40, f2=20,
+                   dropped=0)
+    assert math.isclose(osz.gee(k), 420.0)
+    # a sample of singletons extrapolates linearly: n_sel * d / s
+    k = osz.sketch(_hist(c1=1000), s_sel=1000, n_sample=2000, n_rows=1_000_000, d=1000, f1=1000, f2=0, dropped=0)
+    assert math.isclose(osz.plain_estimate(k), 500_000 * 1000 / 1000)
+    # dropped rows count as singletons, and dropping most of the sample extrapolates linearly
+    k = osz.sketch(_hist(c1=10, c2=10), s_sel=100, n_sample=100, n_rows=1000, d=20, f1=10, f2=10, dropped=70)
+    assert k["d"] == 90 and k["f1"] == 80
+    assert math.isclose(osz.plain_estimate(k), 1000 * 90 / 100)
+    # nothing selected
+    k = osz.sketch(np.zeros(1024), s_sel=0, n_sample=100, n_rows=1000, d=0, f1=0, f2=0, dropped=0)
+    assert osz.plain_estimate(k) == 0.0 and osz.gee(k) == 0.0
+
+
+def test_sketch_features_closed_forms():
+    k = osz.sketch(_hist(c1=4, c2=2, c3=1), s_sel=11, n_sample=16, n_rows=1 << 20, d=7, f1=4, f2=2, dropped=0)
+    for bits, hot in [(17, 12), (33, 12), (34, 13), (65, 13), (66, 14), (97, 14)]:
+        f = osz.features(k, null_keys=3, lane_bits=bits)
+        assert f[12:15].sum() == 1 and f[hot] == 1
+    f = osz.features(k, null_keys=3, lane_bits=65)
+    assert math.isclose(f[0] * 32, math.log2((1 << 20) + 1))
+    assert f[1] == 11 / 16 and f[3] == 4 / 11 and f[4] == 2 / 11 and f[5] == 7 / 11
+    assert f[9] == 21 / 121 and f[10] == 3 / 11 and f[11] == 0
+    assert math.isclose(2 ** (f[8] * 32) - 1, osz.plain_estimate(k))
+
+
+# ---------------------------------------------------------------------------
 # GPU: features and routing
 # ---------------------------------------------------------------------------
 @pytest.mark.gpu
@@ -124,6 +184,80 @@ def test_sketch_features(native):
     assert abs(f[1] - 0.3) < 0.02                   # sample selectivity of (w >= 200) & (w < 500)
     assert f[13] == 1.0 and f[12] == 0.0 and f[14] == 0.0  # one i64 key: the 65-bit lane class
     assert abs(f[0] * 32 - np.log2(500_001)) < 1e-4
+
+
+def _inspect_case(kind):
+    from polars_metal import synth
+    rng = np.random.default_rng(11)
+    if kind == "zipf":
+        return synth.zipf(500_000, seed=3) + (500_000,)
+    if kind == "uniform_recount":  # ~60K distinct sampled keys: the fingerprint table drops, then recounts
+        n = 4_000_000
+        frame = {"key": ("I64", rng.integers(1, n // 10 + 1, n).astype(np.int64), None), "v": ("F64", rng.random(n), None)}
+        plan = {"keys": [["key", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum",
+                                                     "output_alias": "s"}]}
+        return frame, plan, n
+    if kind == "wide_nulls":  # 98-bit lanes: a null key is a key value of the lane
+        frame, plan = synth.sizer_job({"rows": 800_000, "distinct": 30_000, "s": 0.9, "dtype": "U128",
+                                       "null_frac": 0.15, "seed": 5})
+        return frame, plan, 800_000
+    if kind == "i64_nulls":  # 65-bit lanes: null keys are side keys, counted apart
+        frame, plan = synth.sizer_job({"rows": 900_000, "distinct": 200_000, "s": 0.4, "dtype": "I64",
+                                       "null_frac": 0.12, "seed": 7})
+        return frame, plan, 900_000
+    if kind == "i32_small":  # fewer rows than the sample: the sample is the selection
+        frame, plan = synth.sizer_job({"rows": 20_000, "distinct": 3_000, "s": 1.2, "dtype": "I32",
+                                       "null_frac": 0.05, "seed": 6})
+        return frame, plan, 20_000
+    frame, plan = synth.zipf(200_000, seed=4)  # "none": a predicate no row passes
+    plan = dict(plan, predicate=synth.pred_gt_f64("v", 1e9))  # v is standard normal
+    return frame, plan, 200_000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["zipf", "uniform_recount", "wide_nulls", "i64_nulls", "i32_small", "none"])
+def test_sketch_block_sizer_matches_restatement(native, kind):
+    """The sketch block computes the sizer's features, the plain distinct
+    estimate and (model installed) the forward pass on the device; the CPU
+    restatement of the same sketch gives the same numbers."""
+    from polars_metal import learned
+    from polars_metal.buffer import device_columns
+    frame, plan, n = _inspect_case(kind)
+    cols, bufs = device_columns(frame)
+    params = learned.load_model()
+    learned.install(params)
+    try:
+        got = learned.sketch_inspect(plan, n, cols)
+    finally:
+        learned.install(None)
+    bare = learned.sketch_inspect(plan, n, cols)
+    print(kind, {k: v for k, v in got.items() if k not in ("hist", "features")}, flush=True)
+    for sk in (got, bare):
+        h = sk["hist"].astype(np.int64)
+        assert sk["sample_size"] == min(n, 1 << 16)
+        if kind in ("uniform_recount", "wide_nulls", "i64_nulls"):  # more sampled keys than fingerprint slots
+            assert sk["scale"] == 8 and sk["dropped"] % 8 == 0 and (h % 8 == 0).all()
+        else:
+            assert sk["scale"] == 1
+        if kind == "wide_nulls":
+            assert sk["lane_bits"] > 65
+        if kind == "i64_nulls":
+            assert sk["lane_bits"] == 65 and sk["null_keys"] > 0
+        if kind == "none":
+            assert sk["sample_rows"] == 0 and sk["distinct_estimate"] == 0
+        assert h[1:].sum() == sk["distinct"] and h[1] == sk["f1"] and h[2] == sk["f2"]
+        if sk["scale"] == 1:  # the counts cover the counted rows (counts clip at 1023)
+            assert (np.arange(1024) * h)[1:].sum() <= sk["sample_rows"] - sk["null_keys"] - sk["dropped"]
+        k = osz.sketch(h, sk["sample_rows"], sk["sample_size"], n, sk["distinct"], sk["f1"], sk["f2"],
+                       sk["dropped"])
+        want = osz.features(k, sk["null_keys"], sk["lane_bits"])
+        np.testing.assert_allclose(sk["features"], want, rtol=2e-6, atol=1e-7)
+        assert math.isclose(sk["distinct_estimate"], osz.plain_estimate(k), rel_tol=1e-12, abs_tol=1e-12)
+    o, _ = mo.forward(params, got["features"][None, :])
+    np.testing.assert_allclose(got["prediction"], o[0], rtol=0, atol=2e-5)
+    # without a model there is no prediction (the sketch itself varies from
+    # call to call only in which keys win contended fingerprint slots)
+    assert np.isnan(bare["prediction"]).all()
 
 
 @pytest.mark.gpu

@@ -241,6 +241,67 @@ int pmh_buffer_free(void *device_ptr);
  * copy that calibrates HBM traffic counters. */
 int pmh_buffer_copy(void *dst_device, const void *src_device, uint64_t bytes);
 
+/* ------------------------------------------------------------------------
+ * Host columns onto the device.  Every host-to-device copy of the library
+ * (host columns of the fused entry, pmh_buffer_from_host, cache misses)
+ * goes through one pinned staging ring: slot-sized pieces copied by a pool
+ * of host threads, each DMA overlapping the next piece's copy.  Replaces
+ * the reference's StagingPool (crates/polars-metal-buffer/src/staging.rs:
+ * 24-58).
+ *
+ * The device column cache replaces its zero-copy Arrow bridge
+ * (crates/polars-metal-buffer/src/bridge.rs:56-64, which on unified memory
+ * reads host bytes in place): a column is uploaded once and found again by
+ * its tag, its chunks' host addresses, row counts and bit offsets, and the
+ * caller's stamp, so a second query on the same buffers copies nothing.  The
+ * library cannot see host writes: a caller whose bytes at an address change,
+ * or whose buffer is freed and the address reused, passes a new stamp.
+ * A column of several chunks (the reference concatenates Arrow chunks on the
+ * host, python/polars_metal/_udf.py:1441-1456) is assembled on the device.
+ * ---------------------------------------------------------------------- */
+typedef struct pmh_chunk {
+    uint64_t n_rows;
+    const void *data;          /* host: the chunk's first value; Bool: the byte holding its first bit */
+    const uint8_t *valid;      /* host validity bitmap, or NULL: every row of the chunk valid */
+    uint32_t data_bit_offset;  /* Bool only: bit of data[0] holding the first row (0..7) */
+    uint32_t valid_bit_offset; /* bit of valid[0] holding the first row (0..7) */
+} pmh_chunk;
+
+typedef struct pmh_resident_column {
+    uint64_t entry;           /* handle for pmh_column_release / pmh_column_evict */
+    uint64_t n_rows;
+    const void *data;         /* device: the chunks' values, concatenated */
+    uint64_t data_len;
+    const uint8_t *valid;     /* device bitmap (row i at bit i), or NULL: no chunk had one */
+    uint64_t valid_len;
+    int32_t hit;              /* 1: found resident, no bytes copied */
+    int32_t pad;
+} pmh_resident_column;
+
+typedef struct pmh_colcache_info {
+    uint64_t entries;
+    uint64_t bytes_resident;
+    uint64_t capacity;        /* bytes; unheld entries beyond it are evicted least recently used */
+    uint64_t hits, misses;
+    uint64_t bytes_uploaded;  /* host bytes the cache's misses copied */
+    uint64_t h2d_bytes;       /* host bytes every copy of the library moved to the device */
+} pmh_colcache_info;
+
+/* The device copy of a host column given as chunks, uploaded on a miss.  The
+ * entry is held (never evicted) until pmh_column_release; its device
+ * pointers stay valid while it is held.  Pass them to the fused entry as
+ * PMH_DEVICE columns. */
+int pmh_column_resident(int32_t tag, const pmh_chunk *chunks, uint32_t n_chunks, uint64_t stamp,
+                        pmh_resident_column *out);
+int pmh_column_release(uint64_t entry);
+/* Frees an unheld entry now (an entry already evicted is not an error). */
+int pmh_column_evict(uint64_t entry);
+/* Frees every unheld entry. */
+int pmh_colcache_clear(void);
+/* Default: a quarter of device memory. */
+int pmh_colcache_set_capacity(uint64_t bytes);
+int pmh_colcache_stats(pmh_colcache_info *out);
+
 void pmh_result_free(pmh_result *r);
 void pmh_string_free(char *s);
 

@@ -216,7 +216,7 @@ int pmh_buffer_from_host(const void *src, uint64_t bytes, void **out) {
         Runtime &r = rt();
         *out = nullptr;
         PMH_HIP(hipMalloc(out, bytes ? bytes : 1));
-        if (bytes) PMH_HIP(hipMemcpyAsync(*out, src, bytes, hipMemcpyHostToDevice, r.stream));
+        upload_host(*out, src, bytes);
         PMH_HIP(hipStreamSynchronize(r.stream));
     });
 }
@@ -246,6 +246,26 @@ int pmh_buffer_free(void *dev) {
             PMH_HIP(hipStreamSynchronize(r.stream));
             PMH_HIP(hipFree(dev));
         }
+    });
+}
+
+int pmh_column_resident(int32_t tag, const pmh_chunk *chunks, uint32_t n_chunks, uint64_t stamp,
+                        pmh_resident_column *out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        column_resident(tag, chunks, n_chunks, stamp, out);
+    });
+}
+
+int pmh_column_release(uint64_t entry) { return guarded([&] { column_release(entry); }); }
+int pmh_column_evict(uint64_t entry) { return guarded([&] { column_evict(entry); }); }
+int pmh_colcache_clear(void) { return guarded([&] { colcache_clear(); }); }
+int pmh_colcache_set_capacity(uint64_t bytes) { return guarded([&] { colcache_set_capacity(bytes); }); }
+
+int pmh_colcache_stats(pmh_colcache_info *out) {
+    return guarded([&] {
+        if (!out) fail(PMH_E_VALUE, "polars_metal: out is NULL");
+        colcache_stats(out);
     });
 }
 

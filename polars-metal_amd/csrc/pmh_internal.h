@@ -268,6 +268,20 @@ void check_column(const pmh_column &c, uint64_t n_rows, const std::string &nm);
 std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
                                   const std::vector<std::string> &needed);
 
+// Host bytes onto the device through the pinned staging ring, on the
+// library's stream (pmh_ingest.hip).  Returns once src is no longer read;
+// the DMA completes in stream order.
+void upload_host(void *dst, const void *src, size_t bytes);
+uint64_t h2d_bytes();  // bytes upload_host has moved, since the process began
+// The device column cache (pmh_ingest.hip).
+void column_resident(int32_t tag, const pmh_chunk *chunks, uint32_t n_chunks, uint64_t stamp,
+                     pmh_resident_column *out);
+void column_release(uint64_t entry);
+void column_evict(uint64_t entry);
+void colcache_clear();
+void colcache_set_capacity(uint64_t bytes);
+void colcache_stats(pmh_colcache_info *out);
+
 // result helpers
 pmh_result *new_result(size_t n_cols);
 void set_out_name(pmh_out_column &c, const std::string &name);

@@ -3,7 +3,8 @@
 // reference's persistent BuildScratch / StagingPool,
 // crates/polars-metal-kernels/src/groupby_build_partitioned/scratch.rs:29-145,
 // crates/polars-metal-buffer/src/staging.rs:24-58), column staging and the
-// optional per-kernel HIP-event timers.
+// optional per-kernel HIP-event timers (host columns upload through the
+// staging ring of pmh_ingest.hip).
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -223,12 +224,12 @@ std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint6
             // Device copies are padded to 16 bytes so vector loads never run
             // past the allocation.
             void *d = slab("in:" + std::to_string(staged), need + 16);
-            if (need) PMH_HIP(hipMemcpyAsync(d, c->data, need, hipMemcpyHostToDevice, r.stream));
+            upload_host(d, c->data, need);
             col.d_data = d;
             col.d_valid = nullptr;
             if (c->valid) {
                 void *dv = slab("inv:" + std::to_string(staged), vneed + 16);
-                if (vneed) PMH_HIP(hipMemcpyAsync(dv, c->valid, vneed, hipMemcpyHostToDevice, r.stream));
+                upload_host(dv, c->valid, vneed);
                 col.d_valid = (const uint8_t *)dv;
             }
             ++staged;

@@ -94,6 +94,34 @@ class PmhSketchInfo(ctypes.Structure):
     ]
 
 
+class PmhChunk(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64),
+        ("data", ctypes.c_void_p),
+        ("valid", ctypes.c_void_p),
+        ("data_bit_offset", ctypes.c_uint32),
+        ("valid_bit_offset", ctypes.c_uint32),
+    ]
+
+
+class PmhResidentColumn(ctypes.Structure):
+    _fields_ = [
+        ("entry", ctypes.c_uint64),
+        ("n_rows", ctypes.c_uint64),
+        ("data", ctypes.c_void_p),
+        ("data_len", ctypes.c_uint64),
+        ("valid", ctypes.c_void_p),
+        ("valid_len", ctypes.c_uint64),
+        ("hit", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
+class PmhColcacheInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("entries", "bytes_resident", "capacity", "hits", "misses", "bytes_uploaded", "h2d_bytes")]
+
+
 class PmhBatchQuery(ctypes.Structure):
     _fields_ = [
         ("n_rows", ctypes.c_uint64),
@@ -165,6 +193,13 @@ SIGNATURES = {
     "pmh_kernel_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "pmh_sketch_features": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                            ctypes.c_uint32, _FP, ctypes.POINTER(PmhExecStats)]),
+    "pmh_column_resident": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(PmhChunk), ctypes.c_uint32,
+                                           ctypes.c_uint64, ctypes.POINTER(PmhResidentColumn)]),
+    "pmh_column_release": (ctypes.c_int, [ctypes.c_uint64]),
+    "pmh_column_evict": (ctypes.c_int, [ctypes.c_uint64]),
+    "pmh_colcache_clear": (ctypes.c_int, []),
+    "pmh_colcache_set_capacity": (ctypes.c_int, [ctypes.c_uint64]),
+    "pmh_colcache_stats": (ctypes.c_int, [ctypes.POINTER(PmhColcacheInfo)]),
     "pmh_sketch_inspect": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                           ctypes.c_uint32, ctypes.POINTER(PmhSketchInfo)]),
     "pmh_mlp_new": (ctypes.c_int, [_FP, ctypes.POINTER(_P)]),

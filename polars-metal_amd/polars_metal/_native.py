@@ -316,15 +316,17 @@ def _device_to_host(ptr: int, nbytes: int) -> bytes:
 def _device_column_array(columns: list, location: int = PMH_DEVICE):
     """PmhColumn array for ``(name, tag, data_ptr, data_len, valid_ptr,
     valid_len)`` columns (device pointers, or host addresses with
-    ``location=PMH_HOST``), and the objects that must outlive it."""
+    ``location=PMH_HOST``; a seventh element gives one column's own
+    location), and the objects that must outlive it."""
     arr = (_lib.PmhColumn * max(1, len(columns)))()
     keep = []
-    for i, (name, tag, dp, dl, vp, vl) in enumerate(columns):
+    for i, c in enumerate(columns):
+        name, tag, dp, dl, vp, vl = c[:6]
         nb = name.encode()
         keep.append(nb)
         arr[i].name = nb
         arr[i].tag = _tag(tag)
-        arr[i].location = location
+        arr[i].location = c[6] if len(c) > 6 else location
         arr[i].data = dp
         arr[i].data_len = dl
         arr[i].valid = vp or None

@@ -15,14 +15,15 @@ differ, each on purpose:
     (``Scan["selection"]``, which the reference's walker refuses at
     _walker.py:195-196) is lowered the same way.
   * Columns are not copied into ``bytes`` (_udf.py:1068-1084).  Each Arrow
-    buffer is passed by address; the backend reads it in place (or serves it
-    from its device column cache, ``cache=True``).  A column without a
+    buffer is passed by address; the backend uploads it through its pinned
+    staging ring (or, ``cache=True``, serves it from its device column
+    cache, so a repeated query copies nothing).  A column without a
     validity buffer is passed with none, which the backend reads as all
     valid (_udf.py:1574-1582 materialises all-ones bytes instead).
-  * A column of several chunks is passed chunk by chunk to the device
-    column cache, which concatenates on the device; without the cache it is
-    combined on the host, as the reference's ``_materialize_arrow`` does
-    (_udf.py:1441-1456).
+  * A column of several chunks is passed chunk by chunk and assembled on
+    the device (kept resident with ``cache=True``, dropped after the query
+    otherwise), where the reference's ``_materialize_arrow`` concatenates
+    on the host (_udf.py:1441-1456).
 
 Shapes this module does not lower (expression aggregates, HStack, other
 upstream nodes) raise NotImplementedError; ``polars_metal.install`` then
@@ -32,7 +33,7 @@ from __future__ import annotations
 
 from typing import Any
 
-from . import _lib, _native
+from . import _lib, _native, colcache
 
 __all__ = ["dispatch_groupby", "dispatch_groupby_pydf", "resolve_upstream", "arrow_tag"]
 
@@ -200,26 +201,31 @@ def dispatch_groupby(table, wire_plan: dict, cache: bool = False):
     needed = _needed_columns(plan, predicate)
     n_rows = table.num_rows
     cols = _Columns()
-    for name in needed:
-        if name not in table.column_names:
-            raise RuntimeError(f"polars_metal: column {name!r} required by GroupBy not in upstream df")
-        col = table.column(name)
-        tag = arrow_tag(col.type)
-        if tag is None:
-            raise RuntimeError(f"polars_metal: unsupported dtype {col.type} on column {name!r}")
-        if tag == "Utf8":
-            cols.add_utf8(name, _single(col))
-        elif cache:
-            cols.cols.append(_cached_column(name, tag, col))
-        else:
-            cols.add_fixed(name, tag, _single(col))
-    for k in plan["keys"]:
-        if k[1] in ("Utf8", "String"):
-            k[1] = "U32"
-        elif k[1] == "Date":
-            k[1] = "I32"
-    location = _lib.PMH_DEVICE if cache else _lib.PMH_HOST
-    out = _native.execute_filter_groupby_ptrs(plan, n_rows, cols.cols, location)
+    handles: list = []
+    try:
+        for name in needed:
+            if name not in table.column_names:
+                raise RuntimeError(f"polars_metal: column {name!r} required by GroupBy not in upstream df")
+            col = table.column(name)
+            tag = arrow_tag(col.type)
+            if tag is None:
+                raise RuntimeError(f"polars_metal: unsupported dtype {col.type} on column {name!r}")
+            if tag == "Utf8":
+                cols.add_utf8(name, _single(col))
+            elif cache or col.num_chunks > 1:
+                c, h = colcache.device_column(name, tag, col, transient=not cache)
+                cols.cols.append(c)
+                handles.append(h)
+            else:
+                cols.add_fixed(name, tag, _single(col))
+        for k in plan["keys"]:
+            if k[1] in ("Utf8", "String"):
+                k[1] = "U32"
+            elif k[1] == "Date":
+                k[1] = "I32"
+        out = _native.execute_filter_groupby_ptrs(plan, n_rows, cols.cols, _lib.PMH_HOST)
+    finally:
+        colcache.release(handles)
     n = 0
     for _, tag, data, _ in out:
         if tag != "Bool":
@@ -227,11 +233,6 @@ def dispatch_groupby(table, wire_plan: dict, cache: bool = False):
             break
     arrays = [_out_array(tag, data, valid, n, cols.dicts.get(name)) for name, tag, data, valid in out]
     return pa.table(arrays, names=[c[0] for c in out])
-
-
-def _cached_column(name: str, tag: str, col) -> tuple:
-    from . import colcache
-    return colcache.device_column(name, tag, col)
 
 
 def dispatch_groupby_pydf(df_pydf: Any, wire_plan: dict):

@@ -787,9 +787,9 @@ pmh_result *run_hash_global(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             c.location = PMH_HOST;
             c.data = malloc(c.data_len ? c.data_len : 1);
             c.valid = (uint8_t *)malloc(vbytes);
-            if (c.data_len) PMH_HIP(hipMemcpyAsync(c.data, oc.data, c.data_len, hipMemcpyDeviceToHost, R.stream));
+            if (c.data_len) download_host(c.data, oc.data, c.data_len);
             if (oc.valid) {
-                PMH_HIP(hipMemcpyAsync(c.valid, oc.valid, vbytes, hipMemcpyDeviceToHost, R.stream));
+                download_host(c.valid, oc.valid, vbytes);
             } else {
                 memset(c.valid, 0xFF, vbytes);  // counts / lens are never null (groupby/mod.rs:695-706)
             }

@@ -2296,9 +2296,9 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             memset(c.valid, 0, vbytes);
             if (is_bool) memset(c.data, 0, c.data_len);
             const uint64_t nb_copy = is_bool ? packed : c.data_len;
-            if (nb_copy) PMH_HIP(hipMemcpyAsync(c.data, dsrc, nb_copy, hipMemcpyDeviceToHost, R.stream));
+            if (nb_copy) download_host(c.data, dsrc, nb_copy);
             if (oc.valid) {
-                if (packed) PMH_HIP(hipMemcpyAsync(c.valid, oc.valid, packed, hipMemcpyDeviceToHost, R.stream));
+                if (packed) download_host(c.valid, oc.valid, packed);
             } else {
                 memset(c.valid, 0xFF, vbytes);  // counts / lens are never null
             }

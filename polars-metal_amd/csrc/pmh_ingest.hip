@@ -123,7 +123,7 @@ CopyPool &pool() {
 // ---------------------------------------------------------------------------
 // staging ring
 // ---------------------------------------------------------------------------
-constexpr size_t kRingSlot = size_t(8) << 20;
+constexpr size_t kRingSlot = size_t(32) << 20;
 constexpr int kRingSlots = 4;
 
 struct Ring {
@@ -168,6 +168,36 @@ void upload_host(void *dst, const void *src, size_t bytes) {
 }
 
 uint64_t h2d_bytes() { return g_h2d_bytes; }
+
+void download_host(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return;
+    Runtime &R = rt();
+    if (bytes < (size_t(1) << 20)) {  // small: one pageable copy costs less than the ring's round trips
+        PMH_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, R.stream));
+        return;
+    }
+    Ring &g = ring();
+    const size_t pieces = ceil_div(bytes, kRingSlot);
+    // piece k uses slot k % kRingSlots: up to kRingSlots DMAs in flight while
+    // the host copies the oldest one out
+    auto issue = [&](size_t k) {
+        const int i = (int)(k % kRingSlots);
+        if (g.pending[i]) PMH_HIP(hipEventSynchronize(g.done[i]));
+        const size_t off = k * kRingSlot, n = std::min(kRingSlot, bytes - off);
+        PMH_HIP(hipMemcpyAsync(g.slot[i], (const uint8_t *)src + off, n, hipMemcpyDeviceToHost, R.stream));
+        PMH_HIP(hipEventRecord(g.done[i], R.stream));
+        g.pending[i] = true;
+    };
+    for (size_t k = 0; k < std::min<size_t>(pieces, kRingSlots); ++k) issue(k);
+    for (size_t k = 0; k < pieces; ++k) {
+        const int i = (int)(k % kRingSlots);
+        PMH_HIP(hipEventSynchronize(g.done[i]));
+        g.pending[i] = false;
+        const size_t off = k * kRingSlot, n = std::min(kRingSlot, bytes - off);
+        pool().copy((uint8_t *)dst + off, g.slot[i], n);
+        if (k + kRingSlots < pieces) issue(k + kRingSlots);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // bit assembly

@@ -273,6 +273,10 @@ std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint6
 // the DMA completes in stream order.
 void upload_host(void *dst, const void *src, size_t bytes);
 uint64_t h2d_bytes();  // bytes upload_host has moved, since the process began
+// Device bytes into host memory (after the work queued on the stream): a
+// copy of 1 MiB or more goes through the same ring and is complete on
+// return; a smaller one completes by the next synchronisation of the stream.
+void download_host(void *dst, const void *src, size_t bytes);
 // The device column cache (pmh_ingest.hip).
 void column_resident(int32_t tag, const pmh_chunk *chunks, uint32_t n_chunks, uint64_t stamp,
                      pmh_resident_column *out);

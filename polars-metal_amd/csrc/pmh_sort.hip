@@ -1330,9 +1330,9 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             memset(col.valid, 0, vbytes);
             if (is_bool) memset(col.data, 0, col.data_len);
             const uint64_t nb_copy = is_bool ? packed : col.data_len;
-            if (nb_copy) PMH_HIP(hipMemcpyAsync(col.data, dsrc, nb_copy, hipMemcpyDeviceToHost, R.stream));
+            if (nb_copy) download_host(col.data, dsrc, nb_copy);
             if (oc.valid) {
-                if (packed) PMH_HIP(hipMemcpyAsync(col.valid, oc.valid, packed, hipMemcpyDeviceToHost, R.stream));
+                if (packed) download_host(col.valid, oc.valid, packed);
             } else {
                 memset(col.valid, 0xFF, vbytes);  // counts / lens are never null
             }

@@ -242,6 +242,50 @@ int pmh_buffer_free(void *device_ptr);
 int pmh_buffer_copy(void *dst_device, const void *src_device, uint64_t bytes);
 
 /* ------------------------------------------------------------------------
+ * Utf8 keys: the dictionary built and read on the device.  Replaces the
+ * reference's host-side build_dict_nullable (crates/polars-metal-buffer/src/
+ * dict.rs:40-66, called from crates/polars-metal-core/src/udf/groupby/
+ * mod.rs:818-936) and decode_dict (dict.rs:69-74) with its packed output
+ * form (mod.rs:646-670).  A key column's codes go to the fused entry as a
+ * PMH_DEVICE column of tag PMH_UTF8; the key column of its result decodes
+ * back to strings.
+ * ---------------------------------------------------------------------- */
+typedef struct pmh_utf8_column {
+    uint64_t n_rows;
+    int32_t location;           /* PMH_HOST or PMH_DEVICE: offsets, bytes and valid alike */
+    int32_t large;              /* 0: int32 offsets (Arrow string); 1: int64 (large_string) */
+    const void *offsets;        /* n_rows + 1 entries; offsets[0] may be nonzero (a slice) */
+    const uint8_t *bytes;       /* the string buffer */
+    uint64_t bytes_len;         /* bytes readable at `bytes`; offsets[n_rows] must not exceed it */
+    const uint8_t *valid;       /* NULL: every row valid */
+    uint32_t valid_bit_offset;  /* bit of valid[0] holding row 0 (0..7) */
+    uint32_t pad;
+} pmh_utf8_column;
+
+typedef struct pmh_utf8_dict pmh_utf8_dict;
+
+/* Dictionary-encodes a Utf8 column on the device.  Code k is the k-th
+ * distinct string in row order (first seen wins, as build_dict_nullable);
+ * null rows get code 0 and stay null; "" is a value of its own.  A valid
+ * row with decreasing or out-of-buffer offsets, or bytes that are not
+ * well-formed UTF-8, fails with PMH_E_VALUE naming the first such row (as
+ * mod.rs:884-911).  Rows: below 2^31. */
+int pmh_utf8_encode(const pmh_utf8_column *col, pmh_utf8_dict **out);
+/* The device codes (u32 per row) and their validity bitmap (row i at bit i;
+ * NULL when the column had none), both valid until pmh_utf8_dict_free: the
+ * PMH_DEVICE key column for the fused entry.  Also the rows and the
+ * distinct strings. */
+int pmh_utf8_dict_codes(const pmh_utf8_dict *d, const uint32_t **codes_device, const uint8_t **valid_device,
+                        uint64_t *n_rows, uint64_t *n_distinct);
+/* The strings of n codes (with their validity bitmap, NULL: all valid; both
+ * at `location`) in the packed wire form: u32 n, int32 offsets[n + 1], the
+ * bytes; a null row has length 0.  *out_packed is the library's (release it
+ * with pmh_string_free). */
+int pmh_utf8_decode(const pmh_utf8_dict *d, const uint32_t *codes, const uint8_t *valid, uint64_t n,
+                    int32_t location, uint8_t **out_packed, uint64_t *out_len);
+void pmh_utf8_dict_free(pmh_utf8_dict *d);
+
+/* ------------------------------------------------------------------------
  * Host columns onto the device.  Every host-to-device copy of the library
  * (host columns of the fused entry, pmh_buffer_from_host, cache misses)
  * goes through one pinned staging ring: slot-sized pieces copied by a pool

@@ -15,9 +15,16 @@ outputs with the reference's dtype rules:
   * min / max: input dtype, null for all-null groups, NaN poisons floats
     (groupby.rs:1467-1469)
   * keys: the value at the group's first row (groupby.rs:1833-1985)
+  * Utf8 keys: grouped on dictionary codes in first-seen row order, null
+    rows code 0 and null, "" a value of its own (build_dict_nullable,
+    crates/polars-metal-buffer/src/dict.rs:40-66; ``utf8_dict``); the key
+    column comes back as the group's first string, and on the wire in the
+    packed form u32 n, i32 offsets[n + 1], bytes, a null row empty
+    (crates/polars-metal-core/src/udf/groupby/mod.rs:646-670)
 
 Frames are dicts ``name -> (tag, values, valid)``; ``values`` is a numpy
-array (bool array for Bool), ``valid`` a bool array or None (all valid).
+array (bool array for Bool, an object array of str for Utf8), ``valid`` a
+bool array or None (all valid).
 """
 from __future__ import annotations
 
@@ -126,11 +133,53 @@ def frame_rows(frame: dict) -> int:
     return len(next(iter(frame.values()))[1])
 
 
+def utf8_dict(values, valid) -> tuple:
+    """The reference's build_dict_nullable (dict.rs:40-66): (distinct strings
+    in first-seen order, u32 codes, validity); a null row has code 0."""
+    n = len(values)
+    ok = np.ones(n, bool) if valid is None else np.asarray(valid, bool)
+    dictionary: list = []
+    seen: dict = {}
+    codes = np.zeros(n, np.uint32)
+    for i in range(n):
+        if not ok[i]:
+            continue
+        s = values[i]
+        c = seen.get(s)
+        if c is None:
+            c = seen[s] = len(dictionary)
+            dictionary.append(s)
+        codes[i] = c
+    return dictionary, codes, ok
+
+
+def utf8_packed(values, valid) -> bytes:
+    """Strings -> the packed wire form (a null row has length 0)."""
+    n = len(values)
+    ok = np.ones(n, bool) if valid is None else np.asarray(valid, bool)
+    blobs = [values[i].encode("utf-8") if ok[i] else b"" for i in range(n)]
+    offs = np.zeros(n + 1, np.int32)
+    np.cumsum([len(b) for b in blobs], out=offs[1:])
+    return np.uint32(n).tobytes() + offs.tobytes() + b"".join(blobs)
+
+
+def utf8_unpacked(data: bytes) -> np.ndarray:
+    n = int(np.frombuffer(data, np.uint32, 1)[0])
+    offs = np.frombuffer(data, np.int32, n + 1, 4)
+    blob = data[4 + 4 * (n + 1):]
+    return np.array([blob[offs[i]:offs[i + 1]].decode("utf-8") for i in range(n)], object)
+
+
 def to_wire(frame: dict, names=None) -> list:
     """Frame -> the reference's ``(name, tag, data, valid)`` tuples."""
     out = []
     for name in (names or frame.keys()):
         tag, values, valid = frame[name]
+        if tag == "Utf8":
+            n = len(values)
+            out.append((name, tag, utf8_packed(values, valid),
+                        pack_bits(np.ones(n, bool) if valid is None else valid).tobytes()))
+            continue
         if tag == "Bool":
             data = pack_bits(values).tobytes()
         else:
@@ -253,7 +302,15 @@ def groupby(plan: dict, frame: dict, sum_mode: int = 0, threads: int | None = No
         if a.get("op") != "Len" and a.get("kind") != "Length" and a["input_col"] not in val_names:
             val_names.append(a["input_col"])
     k = _Keep()
-    kcols = (OrcCol * max(1, len(keys)))(*[k.col(*frame[nm]) for nm in keys])
+
+    def key_col(nm):
+        tag, values, valid = frame[nm]
+        if tag == "Utf8":
+            _, codes, ok = utf8_dict(values, valid)
+            return k.col("U32", codes, ok)
+        return k.col(tag, values, valid)
+
+    kcols = (OrcCol * max(1, len(keys)))(*[key_col(nm) for nm in keys])
     vcols = (OrcCol * max(1, len(val_names)))(*[k.col(*frame[nm]) for nm in val_names])
     L = lib()
     km = keep_mask.ctypes.data if keep_mask is not None else None
@@ -291,7 +348,9 @@ def groupby(plan: dict, frame: dict, sum_mode: int = 0, threads: int | None = No
         tag, values, valid = frame[nm]
         kv = np.asarray(values)[first] if ng else np.asarray(values)[:0]
         vv = (np.ones(ng, bool) if valid is None else np.asarray(valid)[first]) if ng else np.zeros(0, bool)
-        kv = np.where(vv, kv, np.zeros_like(kv))
+        kv = np.where(vv, kv, "" if tag == "Utf8" else np.zeros_like(kv))
+        if tag == "Utf8":
+            kv = kv.astype(object)
         out[nm] = (tag, kv, vv)
     with np.errstate(invalid="ignore", divide="ignore"):
         for a in plan["aggs"]:
@@ -347,7 +406,8 @@ def from_wire(outputs: list) -> dict:
         if tag == "Bool":
             continue
         if tag == "Utf8":
-            continue
+            n = int(np.frombuffer(data, np.uint32, 1)[0])
+            break
         arr = np.frombuffer(data, dtype=NP_DTYPE[tag])
         n = len(arr)
         break
@@ -357,7 +417,7 @@ def from_wire(outputs: list) -> dict:
         if tag == "Bool":
             values = unpack_bits(data, n)
         elif tag == "Utf8":
-            values = np.asarray(data)
+            values = utf8_unpacked(data)
         else:
             values = np.frombuffer(data, dtype=NP_DTYPE[tag]).copy()
         res[name] = (tag, values, unpack_bits(valid, n))
@@ -376,7 +436,7 @@ def canonical(result: dict, keys: list) -> dict:
         vv = np.asarray(v)
         if vv.dtype.kind == "f":
             vv = vv.view(np.int64 if vv.dtype.itemsize == 8 else np.int32)
-        sort_cols.append(np.where(ok, vv, 0))
+        sort_cols.append(np.where(ok, vv, "" if vv.dtype == object else 0))
         sort_cols.append(ok.astype(np.int8))
     order = np.lexsort(sort_cols)
     out = {}

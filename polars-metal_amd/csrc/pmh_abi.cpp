@@ -269,6 +269,45 @@ int pmh_colcache_stats(pmh_colcache_info *out) {
     });
 }
 
+int pmh_utf8_encode(const pmh_utf8_column *col, pmh_utf8_dict **out) {
+    return guarded([&] {
+        if (!col || !out) fail(PMH_E_VALUE, "polars_metal: utf8_encode needs a column and an out pointer");
+        *out = nullptr;
+        *out = utf8_encode(*col);
+    });
+}
+
+int pmh_utf8_dict_codes(const pmh_utf8_dict *d, const uint32_t **codes, const uint8_t **valid, uint64_t *n_rows,
+                        uint64_t *n_distinct) {
+    return guarded([&] {
+        if (!d) fail(PMH_E_VALUE, "polars_metal: dictionary is NULL");
+        utf8_dict_codes(*d, codes, valid, n_rows, n_distinct);
+    });
+}
+
+int pmh_utf8_decode(const pmh_utf8_dict *d, const uint32_t *codes, const uint8_t *valid, uint64_t n, int32_t location,
+                    uint8_t **out_packed, uint64_t *out_len) {
+    return guarded([&] {
+        if (!d || !out_packed || !out_len) fail(PMH_E_VALUE, "polars_metal: utf8_decode needs a dictionary and outputs");
+        if (n && !codes) fail(PMH_E_VALUE, "polars_metal: codes is NULL");
+        *out_packed = nullptr;
+        *out_len = 0;
+        std::vector<uint8_t> v = utf8_decode(*d, codes, valid, n, location);
+        auto *p = (uint8_t *)malloc(v.size() ? v.size() : 1);
+        if (!p) fail(PMH_E_RUNTIME, "polars_metal: host allocation failed");
+        memcpy(p, v.data(), v.size());
+        *out_packed = p;
+        *out_len = v.size();
+    });
+}
+
+void pmh_utf8_dict_free(pmh_utf8_dict *d) {
+    try {
+        utf8_dict_free(d);
+    } catch (...) {
+    }
+}
+
 void pmh_result_free(pmh_result *r) {
     if (!r) return;
     for (uint32_t i = 0; i < r->n_columns; ++i) {

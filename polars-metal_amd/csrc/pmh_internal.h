@@ -286,6 +286,14 @@ void colcache_clear();
 void colcache_set_capacity(uint64_t bytes);
 void colcache_stats(pmh_colcache_info *out);
 
+// Utf8 dictionaries on the device (pmh_utf8.hip).
+pmh_utf8_dict *utf8_encode(const pmh_utf8_column &c);
+std::vector<uint8_t> utf8_decode(const pmh_utf8_dict &d, const uint32_t *codes, const uint8_t *valid, uint64_t n,
+                                 int location);
+void utf8_dict_free(pmh_utf8_dict *d);
+void utf8_dict_codes(const pmh_utf8_dict &d, const uint32_t **codes, const uint8_t **valid, uint64_t *n_rows,
+                     uint64_t *n_distinct);
+
 // result helpers
 pmh_result *new_result(size_t n_cols);
 void set_out_name(pmh_out_column &c, const std::string &name);

@@ -122,6 +122,20 @@ class PmhColcacheInfo(ctypes.Structure):
                 ("entries", "bytes_resident", "capacity", "hits", "misses", "bytes_uploaded", "h2d_bytes")]
 
 
+class PmhUtf8Column(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64),
+        ("location", ctypes.c_int32),
+        ("large", ctypes.c_int32),
+        ("offsets", ctypes.c_void_p),
+        ("bytes", ctypes.c_void_p),
+        ("bytes_len", ctypes.c_uint64),
+        ("valid", ctypes.c_void_p),
+        ("valid_bit_offset", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
 class PmhBatchQuery(ctypes.Structure):
     _fields_ = [
         ("n_rows", ctypes.c_uint64),
@@ -193,6 +207,14 @@ SIGNATURES = {
     "pmh_kernel_times": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "pmh_sketch_features": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_uint64, ctypes.POINTER(PmhColumn),
                                            ctypes.c_uint32, _FP, ctypes.POINTER(PmhExecStats)]),
+    "pmh_utf8_encode": (ctypes.c_int, [ctypes.POINTER(PmhUtf8Column), ctypes.POINTER(ctypes.c_void_p)]),
+    "pmh_utf8_dict_codes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p),
+                                           ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "pmh_utf8_decode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p),
+                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "pmh_utf8_dict_free": (None, [ctypes.c_void_p]),
     "pmh_column_resident": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(PmhChunk), ctypes.c_uint32,
                                            ctypes.c_uint64, ctypes.POINTER(PmhResidentColumn)]),
     "pmh_column_release": (ctypes.c_int, [ctypes.c_uint64]),

@@ -69,50 +69,92 @@ def _plan_json(plan: dict) -> bytes:
 # ---------------------------------------------------------------------------
 # Utf8 keys: the reference builds a dictionary on the host
 # (crates/polars-metal-core/src/udf/groupby/mod.rs:818-936) and groups on u32
-# codes; the same happens here, in Python, before the device call.
+# codes; here the library builds it on the device (pmh_utf8_encode) and the
+# key column of the result decodes there too (pmh_utf8_decode).
 # ---------------------------------------------------------------------------
-def _utf8_codes(data: bytes, valid: bytes | None, n_rows: int):
-    """Packed wire form ``[n u32][(n+1) x i32 offsets][bytes]``
-    (python/polars_metal/_udf.py:1497-1548) -> (u32 codes, dictionary).
-    Codes are first-seen dictionary indices, null rows code 0 (the
-    reference's build_dict_nullable, crates/polars-metal-buffer/src/
-    dict.rs:40-66); the dictionary is built by Arrow's C++ hash table."""
-    import numpy as np
-    import pyarrow as pa
-    import pyarrow.compute as pc
-    (n,) = struct.unpack_from("<I", data, 0)
-    if n != n_rows:
-        raise ValueError(f"polars_metal: Utf8 column has {n} rows, expected {n_rows}")
-    if len(data) < 4 + 4 * (n + 1):
-        raise ValueError("polars_metal: Utf8 column truncated in its offsets")
-    offs = pa.py_buffer(data[4:4 + 4 * (n + 1)])
-    blob = pa.py_buffer(data[4 + 4 * (n + 1):])
-    vb = pa.py_buffer(valid) if valid else None
-    arr = pa.Array.from_buffers(pa.string(), n, [vb, offs, blob], null_count=-1)
-    enc = pc.dictionary_encode(arr)
-    codes = np.ascontiguousarray(enc.indices.fill_null(0).cast(pa.uint32()).to_numpy(zero_copy_only=False))
-    return codes.tobytes(), enc.dictionary
+class Utf8Dict:
+    """A Utf8 column's device dictionary: ``column(name)`` is its codes as a
+    PMH_DEVICE key column (tag Utf8), ``decode`` turns result codes back
+    into the packed wire form ``[n u32][(n+1) x i32 offsets][bytes]``."""
 
+    def __init__(self, col: "_lib.PmhUtf8Column", keep=None):
+        self._keep = keep
+        h = ctypes.c_void_p()
+        check(_lib.load().pmh_utf8_encode(ctypes.byref(col), ctypes.byref(h)))
+        self._h = h
+        codes, valid = ctypes.c_void_p(), ctypes.c_void_p()
+        nr, nd = ctypes.c_uint64(), ctypes.c_uint64()
+        check(_lib.load().pmh_utf8_dict_codes(h, ctypes.byref(codes), ctypes.byref(valid), ctypes.byref(nr),
+                                              ctypes.byref(nd)))
+        self.codes, self.valid = codes.value or 0, valid.value or 0
+        self.n_rows, self.n_distinct = nr.value, nd.value
+        self._keep = None  # the library holds its own device copy from here on
 
-def _encode_utf8(dictionary, codes: bytes, valid: bytes, n: int) -> bytes:
-    """u32 codes + dictionary -> the packed wire form (nulls as empty)."""
-    import numpy as np
-    import pyarrow as pa
-    if n == 0:
-        return struct.pack("<I", 0) + struct.pack("<i", 0)
-    ok = np.unpackbits(np.frombuffer(valid, np.uint8), bitorder="little")[:n].astype(bool)
-    c = np.frombuffer(codes, np.uint32, count=n)
-    if len(dictionary) == 0:
-        strs = pa.array([""] * n, pa.string())
-    else:
-        strs = dictionary.take(pa.array(np.where(ok, c, 0)))
-    lens = np.where(ok, np.diff(np.frombuffer(strs.buffers()[1], np.int32, count=n + 1, offset=4 * strs.offset)), 0)
-    offs = np.zeros(n + 1, np.int32)
-    np.cumsum(lens, out=offs[1:])
-    src = np.frombuffer(strs.buffers()[2], np.uint8) if strs.buffers()[2] is not None else np.zeros(0, np.uint8)
-    so = np.frombuffer(strs.buffers()[1], np.int32, count=n + 1, offset=4 * strs.offset)
-    blob = b"".join(bytes(src[so[i]:so[i + 1]]) for i in np.flatnonzero(ok)) if ok.any() else b""
-    return struct.pack("<I", n) + offs.tobytes() + blob
+    @classmethod
+    def from_packed(cls, data: bytes, valid: bytes | None, n_rows: int) -> "Utf8Dict":
+        """The reference's packed wire form (python/polars_metal/_udf.py:1497-1548),
+        checked as the reference checks it (mod.rs:818-875)."""
+        if len(data) < 4:
+            raise ValueError(f"polars_metal: Utf8 column wire payload too short ({len(data)} B; need >= 4 for header)")
+        (n,) = struct.unpack_from("<I", data, 0)
+        if n != n_rows:
+            raise ValueError(f"polars_metal: Utf8 column wire n_rows={n} disagrees with column n_rows={n_rows}")
+        if len(data) < 4 + 4 * (n + 1):
+            raise ValueError("polars_metal: Utf8 column wire payload truncated in offsets")
+        if valid is not None and len(valid) < (n + 7) // 8:
+            raise ValueError(f"polars_metal: Utf8 column validity buffer is {len(valid)} B, need at least "
+                             f"{(n + 7) // 8} B for {n} rows")
+        base = ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p).value
+        col = _lib.PmhUtf8Column()
+        col.n_rows, col.location, col.large = n, PMH_HOST, 0
+        col.offsets, col.bytes = base + 4, base + 4 + 4 * (n + 1)
+        col.bytes_len = len(data) - 4 - 4 * (n + 1)
+        col.valid = ctypes.cast(ctypes.c_char_p(valid), ctypes.c_void_p).value if valid else None
+        return cls(col, (data, valid))
+
+    @classmethod
+    def from_arrow(cls, arr) -> "Utf8Dict":
+        """A pyarrow string or large_string Array, read in place (its slice
+        offset folded into the offsets pointer and a validity bit offset)."""
+        import pyarrow as pa
+        large = pa.types.is_large_string(arr.type)
+        ow = 8 if large else 4
+        vbuf, obuf, dbuf = arr.buffers()
+        off = arr.offset
+        col = _lib.PmhUtf8Column()
+        col.n_rows, col.location, col.large = len(arr), PMH_HOST, int(large)
+        col.offsets = obuf.address + ow * off
+        col.bytes = dbuf.address if dbuf is not None else None
+        col.bytes_len = dbuf.size if dbuf is not None else 0
+        if vbuf is not None and arr.null_count:
+            col.valid, col.valid_bit_offset = vbuf.address + off // 8, off % 8
+        return cls(col, arr)
+
+    def column(self, name: str) -> tuple:
+        return (name, "Utf8", self.codes, self.n_rows * 4, self.valid, (self.n_rows + 7) // 8 if self.valid else 0,
+                PMH_DEVICE)
+
+    def decode(self, codes: bytes, valid: bytes | None, n: int) -> bytes:
+        """Host codes (and validity) of n result rows -> packed wire form."""
+        out, ln = ctypes.c_void_p(), ctypes.c_uint64()
+        cp = ctypes.cast(ctypes.c_char_p(codes), ctypes.c_void_p).value if n else None
+        vp = ctypes.cast(ctypes.c_char_p(valid), ctypes.c_void_p).value if valid else None
+        check(_lib.load().pmh_utf8_decode(self._h, cp, vp, n, PMH_HOST, ctypes.byref(out), ctypes.byref(ln)))
+        try:
+            return ctypes.string_at(out.value, ln.value)
+        finally:
+            _lib.load().pmh_string_free(out)
+
+    def free(self) -> None:
+        if self._h:
+            _lib.load().pmh_utf8_dict_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -133,7 +175,14 @@ def _prepare_columns(columns: Sequence[tuple], n_rows: int) -> _Prepared:
         tag = TAG_ALIASES.get(tag, tag)
         t = _tag(tag)
         if tag == "Utf8":
-            data, dicts[name] = _utf8_codes(bytes(data), bytes(valid) if valid else None, n_rows)
+            d = Utf8Dict.from_packed(bytes(data), bytes(valid) if valid else None, n_rows)
+            dicts[name] = d
+            nb = name.encode("utf-8")
+            keep.append(nb)
+            arr[i].name, arr[i].tag, arr[i].location = nb, t, PMH_DEVICE
+            arr[i].data, arr[i].data_len = d.codes or None, n_rows * 4
+            arr[i].valid, arr[i].valid_len = (d.valid or None), ((n_rows + 7) // 8 if d.valid else 0)
+            continue
         data = bytes(data) if not isinstance(data, bytes) else data
         valid = bytes(valid) if valid is not None and not isinstance(valid, bytes) else valid
         nb = name.encode("utf-8")
@@ -194,7 +243,7 @@ def _collect_host_result(res_p, dicts: dict) -> list:
         data = ctypes.string_at(c.data, c.data_len) if c.data_len else b""
         valid = ctypes.string_at(c.valid, c.valid_len) if c.valid_len else b""
         if tag == "Utf8":
-            data = _encode_utf8(dicts[name], data, valid, n)
+            data = dicts[name].decode(data, valid, n)
         out.append((name, tag, data, valid))
     return out
 
@@ -347,11 +396,13 @@ def execute_filter_groupby_device(plan_dict: dict, n_rows: int, columns: Iterabl
 
 
 def execute_filter_groupby_ptrs(plan_dict: dict, n_rows: int, columns: Iterable[tuple],
-                                location: int = PMH_HOST) -> list:
+                                location: int = PMH_HOST, dicts: dict | None = None) -> list:
     """The fused call over columns given by address, ``(name, tag, data_ptr,
     data_len, valid_ptr, valid_len)`` (host memory by default, device
     memory with ``location=PMH_DEVICE``); outputs come back as the
-    reference's host tuples.  Nothing is copied on the Python side."""
+    reference's host tuples, Utf8 keys decoded through ``dicts`` (name ->
+    Utf8Dict) into the packed wire form.  Nothing is copied on the Python
+    side."""
     lib = _lib.load()
     columns = list(columns)
     arr, keep = _device_column_array(columns, location)
@@ -361,7 +412,7 @@ def execute_filter_groupby_ptrs(plan_dict: dict, n_rows: int, columns: Iterable[
     try:
         _LAST_STATS.clear()
         _LAST_STATS.update(_stats_dict(res_p.contents.stats))
-        return _collect_host_result(res_p, {})
+        return _collect_host_result(res_p, dicts or {})
     finally:
         lib.pmh_result_free(res_p)
 

@@ -128,24 +128,15 @@ class _Columns:
         self.cols.append((name, tag, dp, dl, vp, vl))
 
     def add_utf8(self, name: str, arr) -> None:
-        """Utf8 keys group on first-seen dictionary codes (the reference's
-        build_dict_nullable, crates/polars-metal-buffer/src/dict.rs:40-66):
-        null rows get code 0 and stay null; "" is a value of its own."""
-        import numpy as np
-        import pyarrow as pa
-        import pyarrow.compute as pc
-        if arr.type != pa.string():
-            arr = arr.cast(pa.string())
-        enc = pc.dictionary_encode(arr)
-        codes = enc.indices.fill_null(0).cast(pa.uint32())
-        codes = np.ascontiguousarray(codes.to_numpy(zero_copy_only=False))
-        vbuf = None
-        if arr.null_count:
-            vbuf = pa.concat_arrays([arr]).buffers()[0]
-        self.keep += [codes, vbuf, arr]
-        self.dicts[name] = enc.dictionary
-        vp, vl = (vbuf.address, (len(arr) + 7) // 8) if vbuf is not None else (0, 0)
-        self.cols.append((name, "U32", codes.ctypes.data, codes.nbytes, vp, vl))
+        """Utf8 keys group on first-seen dictionary codes built on the
+        device (the reference's build_dict_nullable,
+        crates/polars-metal-buffer/src/dict.rs:40-66, runs on the host):
+        null rows get code 0 and stay null; "" is a value of its own.  The
+        Arrow buffers are read in place."""
+        d = _native.Utf8Dict.from_arrow(arr)
+        self.keep.append(d)
+        self.dicts[name] = d
+        self.cols.append(d.column(name))
 
 
 def _single(col):
@@ -156,20 +147,17 @@ def _single(col):
     return col
 
 
-def _out_array(tag: str, data: bytes, valid: bytes, n: int, dictionary=None):
+def _out_array(tag: str, data: bytes, valid: bytes, n: int):
     """One output column as a pyarrow array (the reference's
-    ``_assemble_series``, _udf.py:1585-1669, minus the Polars wrap)."""
-    import numpy as np
+    ``_assemble_series``, _udf.py:1585-1669, minus the Polars wrap); a Utf8
+    key arrives in the packed wire form and is wrapped without a copy."""
     import pyarrow as pa
     nb = (n + 7) // 8
     vb = pa.py_buffer(valid[:nb]) if n else None
-    if dictionary is not None:
-        if len(dictionary) == 0:  # every key null
-            return pa.nulls(n, pa.string())
-        codes = pa.array(np.frombuffer(data, np.uint32, count=n))
-        arr = dictionary.take(codes)
-        mask = pa.Array.from_buffers(pa.bool_(), n, [None, vb]) if n else pa.array([], pa.bool_())
-        return pc_if_else(mask, arr)
+    if tag == "Utf8":
+        buf = pa.py_buffer(data)
+        return pa.Array.from_buffers(pa.string(), n, [vb, buf.slice(4, 4 * (n + 1)), buf.slice(4 + 4 * (n + 1))],
+                                     null_count=-1)
     pa_type = {"I64": pa.int64(), "F64": pa.float64(), "Bool": pa.bool_(), "I32": pa.int32(),
                "F32": pa.float32(), "I8": pa.int8(), "I16": pa.int16(), "U8": pa.uint8(), "U16": pa.uint16(),
                "U32": pa.uint32(), "U64": pa.uint64()}[tag]
@@ -178,12 +166,6 @@ def _out_array(tag: str, data: bytes, valid: bytes, n: int, dictionary=None):
     width = _lib.TAG_SIZE[tag]
     db = pa.py_buffer(data[:nb] if tag == "Bool" else data[:n * width])
     return pa.Array.from_buffers(pa_type, n, [vb, db], null_count=-1)
-
-
-def pc_if_else(mask, arr):
-    import pyarrow as pa
-    import pyarrow.compute as pc
-    return pc.if_else(mask, arr, pa.scalar(None, type=arr.type))
 
 
 def dispatch_groupby(table, wire_plan: dict, cache: bool = False):
@@ -219,19 +201,22 @@ def dispatch_groupby(table, wire_plan: dict, cache: bool = False):
             else:
                 cols.add_fixed(name, tag, _single(col))
         for k in plan["keys"]:
-            if k[1] in ("Utf8", "String"):
-                k[1] = "U32"
+            if k[1] == "String":
+                k[1] = "Utf8"
             elif k[1] == "Date":
                 k[1] = "I32"
-        out = _native.execute_filter_groupby_ptrs(plan, n_rows, cols.cols, _lib.PMH_HOST)
+        out = _native.execute_filter_groupby_ptrs(plan, n_rows, cols.cols, _lib.PMH_HOST, cols.dicts)
     finally:
         colcache.release(handles)
     n = 0
     for _, tag, data, _ in out:
+        if tag == "Utf8":
+            n = int.from_bytes(data[:4], "little")
+            break
         if tag != "Bool":
             n = len(data) // _lib.TAG_SIZE[tag]
             break
-    arrays = [_out_array(tag, data, valid, n, cols.dicts.get(name)) for name, tag, data, valid in out]
+    arrays = [_out_array(tag, data, valid, n) for name, tag, data, valid in out]
     return pa.table(arrays, names=[c[0] for c in out])
 
 

@@ -276,3 +276,64 @@ def test_all_core_groupby_is_bit_identical(threads):
     # empty input and a predicate that keeps nothing
     e = {k: (t, v[:0], None) for k, (t, v, _) in frame.items()}
     assert orc.groupby(plan, e, threads=threads)["__n_groups__"] == 0
+
+
+# ---------------------------------------------------------------------------
+# Utf8 keys: the dictionary restatement against the reference's own KATs
+# (crates/polars-metal-buffer/tests/test_dict.rs) and numpy
+# ---------------------------------------------------------------------------
+def test_utf8_dict_reference_kats():
+    d, codes, ok = orc.utf8_dict(np.array(["b", "a", "b", "c", "a"], object), None)
+    assert d == ["b", "a", "c"] and codes.tolist() == [0, 1, 0, 2, 1]  # test_dict.rs:20-28
+    d, codes, ok = orc.utf8_dict(np.array(["a", "x", "", "a", "y"], object),
+                                 np.array([True, False, True, True, False]))
+    assert len(d) == 2 and ok.tolist() == [True, False, True, True, False]  # test_dict.rs:39-49
+    assert codes[0] == 0 and codes[2] == 1 and codes[3] == 0
+    d, codes, ok = orc.utf8_dict(np.array(["p", "q", "r"], object), np.zeros(3, bool))
+    assert d == [] and codes.tolist() == [0, 0, 0] and not ok.any()  # test_dict.rs:52-58
+    d, codes, _ = orc.utf8_dict(np.array([], object), None)
+    assert d == [] and codes.size == 0  # test_dict.rs:31-36
+
+
+def test_utf8_dict_first_seen_order_vs_numpy():
+    rng = np.random.default_rng(4)
+    words = np.array(["", "a", "ab", "Zürich", "x" * 30, "ab ", "日本"], object)
+    vals = words[rng.integers(0, len(words), 5000)]
+    ok = rng.random(5000) < 0.9
+    d, codes, _ = orc.utf8_dict(vals, ok)
+    uniq, first, inv = np.unique(vals[ok].astype(str), return_index=True, return_inverse=True)
+    order = np.argsort(first)  # distinct strings by first valid row
+    rank = np.empty(len(uniq), np.int64)
+    rank[order] = np.arange(len(uniq))
+    assert d == [str(x) for x in uniq[order]]
+    np.testing.assert_array_equal(codes[ok], rank[inv])
+    assert (codes[~ok] == 0).all()
+
+
+def test_utf8_packed_form_round_trip():
+    vals = np.array(["ab", "", "日本", "zz"], object)
+    ok = np.array([True, True, True, False])
+    p = orc.utf8_packed(vals, ok)
+    assert p[:4] == (4).to_bytes(4, "little")
+    assert np.frombuffer(p, np.int32, 5, 4).tolist() == [0, 2, 2, 8, 8]  # the null row is empty
+    assert orc.utf8_unpacked(p).tolist() == ["ab", "", "日本", ""]
+
+
+def test_utf8_key_groupby_vs_python():
+    rng = np.random.default_rng(6)
+    words = np.array(["", "a", "b", "ab"], object)
+    s = words[rng.integers(0, 4, 400)]
+    ok = rng.random(400) < 0.85
+    k2 = rng.integers(0, 3, 400)
+    v = rng.integers(-9, 9, 400)
+    frame = {"s": ("Utf8", s, ok), "k2": ("I64", k2, None), "v": ("I64", v, None)}
+    plan = {"keys": [["s", "Utf8"], ["k2", "I64"]],
+            "aggs": [{"kind": "Simple", "input_col": "v", "op": "Sum", "output_alias": "t"}]}
+    got = orc.groupby(plan, frame)
+    want: dict = {}
+    for i in range(400):
+        key = (s[i] if ok[i] else None, int(k2[i]))
+        want[key] = want.get(key, 0) + int(v[i])
+    gs, gk, gt = got["s"], got["k2"], got["t"]
+    seen = {((gs[1][j] if gs[2][j] else None), int(gk[1][j])): int(gt[1][j]) for j in range(got["__n_groups__"])}
+    assert seen == want

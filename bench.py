@@ -368,6 +368,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sizer", action="store_true")
     ap.add_argument("--no-host-path", action="store_true")
+    ap.add_argument("--no-highcard", action="store_true", help="skip the 200M-row composite-key leg")
     ap.add_argument("--options", type=str, default="", help="JSON plan options (launch shape overrides)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -456,6 +457,14 @@ def main() -> int:
     if not args.no_sizer:
         log(f"learned sizing + batching vs plain on {args.sizer_jobs} fresh jobs")
         out["learned_vs_plain"] = stream_ab(args.sizer_jobs)
+    if not args.no_highcard:
+        log("high-cardinality leg: 200M rows, ~49M groups, hash and sort engines")
+        hc = bench_highcard(argparse.Namespace(steps=5, warmup=2))
+        out["highcard"] = {"ms_per_query": {m: e["ms_per_query"] for m, e in hc["config"]["engines"].items()},
+                           "kernel_ms": {m: e["kernel_ms"] for m, e in hc["config"]["engines"].items()},
+                           "groups": hc["config"]["engines"]["hash"]["groups"], "agree": hc["config"]["agree"],
+                           "rows_per_s": hc["value"], "roofline": hc["roofline"], "config": hc["config"]["workload"],
+                           "steps": 5, "warmup": 2}
     print(json.dumps(out), flush=True)
     return 0
 

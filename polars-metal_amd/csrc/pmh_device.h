@@ -235,6 +235,21 @@ __device__ __forceinline__ void or_shift128(uint64_t &lo, uint64_t &hi, uint64_t
     }
 }
 
+// The order-preserving unsigned image of a key value's raw bits under a
+// narrowed field's transform.
+__host__ __device__ __forceinline__ uint64_t key_image(int xform, int tag, uint64_t raw) {
+    if (xform != KX_SIGNED) return raw;
+    const int w = tag_data_bits(tag);
+    const uint64_t s = w >= 64 ? raw : (uint64_t)((int64_t)(raw << (64 - w)) >> (64 - w));
+    return s ^ (1ull << 63);
+}
+// The data bits of a field for a valid value's raw bits.
+__device__ __forceinline__ uint64_t key_data(const KeyField &f, uint64_t raw) {
+    if (f.xform == KX_RAW) return raw;
+    const uint64_t d = key_image(f.xform, f.tag, raw) - f.bias;
+    return f.bits >= 64 ? d : (d & ((1ull << f.bits) - 1));
+}
+
 template <class Row>
 __device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, const Row &row) {
     Key k{0, 0, 0};
@@ -249,8 +264,11 @@ __device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, const R
     }
     for (int i = 0; i < q->n_keys; ++i) {
         const KeyField &f = q->keys[i];
-        if (!row.valid(f.col)) or_shift128(k.lo, k.hi, 1, f.null_off);
-        else or_shift128(k.lo, k.hi, row.bits(f.col), f.data_off);
+        if (!row.valid(f.col)) {
+            if (f.null_off >= 0) or_shift128(k.lo, k.hi, 1, f.null_off);
+        } else {
+            or_shift128(k.lo, k.hi, key_data(f, row.bits(f.col)), f.data_off);
+        }
     }
     if (q->allones_possible && k.lo == kEmpty && (mode == KM_LANE64 || k.hi == kEmpty)) k.special = 1;
     return k;

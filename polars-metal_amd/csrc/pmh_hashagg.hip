@@ -58,6 +58,11 @@ constexpr int kPartRows = 4;               // rows per thread per partition tile
 constexpr int kPartTile = kPartThreads * kPartRows;
 constexpr int kWideRadixBits = 10;  // one-pass partitioning: up to 1024 partitions
 constexpr int kWideRadix = 1 << kWideRadixBits;
+// At most 2^17 tail partitions: two passes (an unstable 10-bit one, then a
+// stable 7-bit one) reach them, and their hash bits stay below bit 20, where
+// the per-partition table's slot bits begin.
+constexpr int kMaxPartBits = kWideRadixBits + kRadixBits;
+static_assert(kMaxPartBits <= 20, "partition bits and slot bits of the hash are disjoint");
 static_assert(kWideRadix == kPartThreads, "one thread per digit in the block scans");
 constexpr uint16_t kFlagAllOnes = 1u << 8, kFlagNull = 1u << 9;
 constexpr int kProbe = 16;
@@ -1743,7 +1748,102 @@ bool lower_fast_scan(const QueryDesc &q, FastScan &fs) {
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// key narrowing: a lane wider than 64 bits whose columns' value ranges fit
+// 64 bits runs as a narrow lane (8-byte keys through the whole engine)
+// ---------------------------------------------------------------------------
+// kr[3k]: smallest image of key k, kr[3k + 1]: largest, kr[3k + 2]: nonzero
+// if a row of key k is null; over every row (the predicate is not applied,
+// so the ranges cover any selection).  kr starts at (~0, 0, 0).
+__global__ __launch_bounds__(256) void hs_key_range(const QueryDesc *__restrict__ q, unsigned long long *kr) {
+    const uint64_t n = q->n_rows;
+    for (int k = 0; k < q->n_keys; ++k) {
+        const KeyField &f = q->keys[k];
+        const DevCol &c = q->cols[f.col];
+        const int xf = tag_signed(f.tag) ? KX_SIGNED : KX_UNSIGNED;
+        uint64_t lo = ~0ull, hi = 0;
+        uint32_t nul = 0;
+        for (uint64_t r = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; r < n; r += (uint64_t)gridDim.x * blockDim.x) {
+            if (!dvalid(c.valid, r)) {
+                nul = 1;
+                continue;
+            }
+            const uint64_t m = key_image(xf, f.tag, dload_bits(c, r));
+            lo = m < lo ? m : lo;
+            hi = m > hi ? m : hi;
+        }
+#pragma unroll
+        for (int o = 32; o; o >>= 1) {
+            const uint64_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+            nul |= __shfl_xor(nul, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (lo != ~0ull) atomicMin(&kr[3 * k], (unsigned long long)lo);
+            if (hi) atomicMax(&kr[3 * k + 1], (unsigned long long)hi);
+            if (nul) atomicOr(&kr[3 * k + 2], 1ull);
+        }
+    }
+}
+
+__global__ void hs_key_range_init(unsigned long long *kr, int n_keys) {
+    const int i = threadIdx.x;
+    if (i < 3 * n_keys) kr[i] = i % 3 == 0 ? ~0ull : 0ull;
+}
+
 namespace {
+
+// Rows from which the narrowing pass (one read of the key columns and a
+// host synchronisation) is worth trying.
+constexpr uint64_t kNarrowMinRows = uint64_t(1) << 20;
+
+// Builds in qn the lane of q narrowed to the key columns' value ranges:
+// each column's null bit only if it holds a null, its data as the image
+// minus the smallest image, in the bits the range needs.  False (qn
+// unchanged) when the narrowed lane is still wider than 64 bits.
+bool narrow_lane(const QueryDesc &q, QueryDesc &qn) {
+    Runtime &R = rt();
+    const int nk = q.n_keys;
+    auto *dq = (QueryDesc *)slab("hs:narrow_q", sizeof(QueryDesc));
+    auto *kr = (unsigned long long *)slab("hs:narrow_kr", 3 * kMaxKeys * 8);
+    auto *pin = (unsigned long long *)pinned_host(3 * kMaxKeys * 8 + sizeof(QueryDesc));
+    memcpy(pin + 3 * kMaxKeys, &q, sizeof(QueryDesc));
+    PMH_HIP(hipMemcpyAsync(dq, pin + 3 * kMaxKeys, sizeof(QueryDesc), hipMemcpyHostToDevice, R.stream));
+    hipLaunchKernelGGL(hs_key_range_init, dim3(1), dim3(64), 0, R.stream, kr, nk);
+    PMH_HIP(hipGetLastError());
+    const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
+    {
+        KernelTimer kt("hs_key_range");
+        hipLaunchKernelGGL(hs_key_range, dim3(cus * 8), dim3(256), 0, R.stream, dq, kr);
+        PMH_HIP(hipGetLastError());
+    }
+    PMH_HIP(hipMemcpyAsync(pin, kr, 3 * nk * 8, hipMemcpyDeviceToHost, R.stream));
+    PMH_HIP(hipStreamSynchronize(R.stream));
+    int off = 0;
+    KeyField keys[kMaxKeys];
+    for (int k = 0; k < nk; ++k) {
+        KeyField f = q.keys[k];
+        const uint64_t lo = pin[3 * k], hi = pin[3 * k + 1];
+        const bool nul = pin[3 * k + 2] != 0;
+        const bool any = lo <= hi;  // some valid row
+        const int w = !any || hi == lo ? 0 : 64 - __builtin_clzll(hi - lo);
+        f.xform = tag_signed(f.tag) ? KX_SIGNED : KX_UNSIGNED;
+        f.bias = any ? lo : 0;
+        f.bits = w;
+        f.null_off = nul ? off++ : -1;
+        f.data_off = w ? off : 0;
+        off += w;
+        keys[k] = f;
+    }
+    if (off > 64) return false;
+    qn = q;
+    for (int k = 0; k < nk; ++k) qn.keys[k] = keys[k];
+    qn.total_bits = off;
+    qn.key_mode = KM_LANE64;
+    qn.allones_possible = off == 64;
+    return true;
+}
 
 void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, const int32_t *vw, int nv) {
     S.klo = (uint64_t *)slab(tag + ":klo", cap * 8 + 64);
@@ -1757,9 +1857,15 @@ void alloc_spill(const std::string &tag, Spill &S, uint64_t cap, bool wide, cons
 pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     Runtime &R = rt();
     const FallbackScratchGuard fb_guard(R.scratch_mode);
-    const QueryDesc &q0 = P.q;
     const uint64_t n = P.n_rows;
-    const bool wide = P.wide;
+    const Options &opt0 = plan.opts;
+    // a wide lane whose key columns' ranges fit 64 bits runs narrow here
+    // (the other engines, and this one's fallbacks, keep P.q's lane)
+    QueryDesc qn;
+    bool narrowed = false;
+    if (P.wide && n >= kNarrowMinRows && !R.scratch_mode && opt0.narrow_keys != 0) narrowed = narrow_lane(P.q, qn);
+    const QueryDesc &q0 = narrowed ? qn : P.q;
+    const bool wide = narrowed ? false : P.wide;
     const int nv = q0.n_vals;
     const Options &opt = plan.opts;
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
@@ -1872,7 +1978,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     alloc_spill("hs:pong", L.pong, cap, wide, L.vw, nv);
     L.tile_hist = (uint32_t *)slab("hs:tile_hist", (uint64_t)std::max(kRadix, kWideRadix) * max_tiles * 4 + 64);
     L.digit_base = (uint32_t *)slab("hs:digit_base", std::max(kRadix, kWideRadix) * 4 + 64);
-    L.part_start = (uint64_t *)slab("hs:part_start", ((1ull << 15) + 2) * 8 + 64);
+    L.part_start = (uint64_t *)slab("hs:part_start", ((1ull << kMaxPartBits) + 2) * 8 + 64);
     const uint32_t pgrid = (uint32_t)((cus * 2 + 7) / 8 * 8);  // partition kernels: a multiple of 8 (XCD slots)
     L.tile_reg = (uint32_t *)slab("hs:tile_reg", (uint64_t)max_tiles * 4 + 64);
     L.xtot = (uint32_t *)slab("hs:xtot", 8 * kWideRadix * 4 + 64);
@@ -1884,7 +1990,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // the learned sizer runs inside the sketch block when "auto" will use it
     const bool use_sizer = opt.method == "auto" && opt.distinct_hint <= 0 && P.d_retry <= 0 && !R.scratch_mode;
     L.sizer = sizer_installed() && (use_sizer || opt.sketch_only) ? sizer_device_params() : nullptr;
-    L.lane_bits = q0.total_bits;
+    L.lane_bits = P.q.total_bits;  // the sizer's features use the plan's lane width
     // fallback table
     {
         GTable &t = L.t;
@@ -2029,7 +2135,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
             o->dropped = sk[ST_SAMPLE_DROPPED];
             o->null_keys = sk[ST_SAMPLE_NULLKEY];
             o->hot_keys = sk[ST_HOT];
-            o->lane_bits = (uint32_t)q0.total_bits;
+            o->lane_bits = (uint32_t)P.q.total_bits;
             o->scale = blk[SZ_SCALE];
             memcpy(o->hist, blk + SZ_N, sizeof(o->hist));
             memcpy(o->features, P.feat, sizeof(o->features));
@@ -2082,7 +2188,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         const double tail_rows = n_sel * (1.0 - hot_frac);
         P_parts = std::max<uint64_t>(P_parts, std::min<uint64_t>(1024, pow2_floor((uint64_t)(tail_rows / 8192) + 1)));
     }
-    P_parts = std::min<uint64_t>(P_parts, 1u << 15);
+    P_parts = std::min<uint64_t>(P_parts, 1u << kMaxPartBits);
     const int pbits = log2u(P_parts);
     // up to 1024 partitions: one unstable pass on the partition digit; more:
     // an unstable pass on the low pbits - 7 bits, then one stable 7-bit pass

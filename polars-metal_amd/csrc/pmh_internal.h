@@ -99,9 +99,20 @@ enum KeyMode : int32_t {
     KM_LANE128 = 3  // encoded lane up to 128 bits
 };
 
+// A key column's field of the lane: its null bit at null_off (-1: the
+// column holds no null, so no bit), its `bits` data bits at data_off.  With
+// xform 0 the data bits are the value's raw bits; a narrowed lane (xform 1:
+// raw bits as unsigned, 2: signed, sign-extended with the sign bit flipped)
+// stores the value's image minus bias, the image of the column's smallest
+// value, in as many bits as the column's range needs.
 struct KeyField {
-    int32_t col, tag, bits, null_off, data_off, pad;
+    int32_t col, tag, bits, null_off, data_off, xform;
+    uint64_t bias;
 };
+enum KeyXform : int32_t { KX_RAW = 0, KX_UNSIGNED = 1, KX_SIGNED = 2 };
+__host__ __device__ inline bool tag_signed(int tag) {
+    return tag == PMH_I64 || tag == PMH_I32 || tag == PMH_I16 || tag == PMH_I8;
+}
 
 enum ValKind : int32_t { VK_F = 0, VK_I64 = 1, VK_I32 = 2 };
 enum ValNeed : int32_t { NEED_SUM = 1, NEED_MINMAX = 2 };
@@ -169,6 +180,7 @@ struct Options {
     int64_t fast_scan = 1;        // partitioned engine: 1 use the specialised scan where the query fits it, 0 never
     int64_t scan_rows = 4;        // specialised scan: rows per lane per step (4 or 8)
     int64_t scan_blocks = 1;      // partitioned scan: resident blocks per CU (the hot table shares LDS)
+    int64_t narrow_keys = 1;      // partitioned engine: 1 narrow a lane wider than 64 bits to its key ranges, 0 never
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 

@@ -254,7 +254,8 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                     for (int i = 0; i < 4; ++i) {
                         if (i >= nk) continue;
                         const bool ok = !kc[i].vstride || cut(kv[i], bpos(kc[i], row), 1);
-                        key |= ok ? cut(kw[i], vpos(kc[i], row), kc[i].mask) << kdata[i] : 1ull << knull[i];
+                        const uint64_t raw = cut(kw[i], vpos(kc[i], row), kc[i].mask);
+                        key |= ok ? key_data(q->keys[i], raw) << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
                     }
                     special = (allones && key == kEmpty) ? 1 : 0;
                 }

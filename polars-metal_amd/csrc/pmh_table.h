@@ -366,13 +366,17 @@ __device__ __forceinline__ void key_field(const QueryDesc *__restrict__ q, int f
     if (special == 1) { lo = kEmpty; hi = mode == KM_LANE128 ? kEmpty : 0; }
     const KeyField &f = q->keys[fi];
     const int no = f.null_off;
-    valid = ((no < 64 ? (lo >> no) : (hi >> (no - 64))) & 1) == 0;
+    valid = no < 0 || ((no < 64 ? (lo >> no) : (hi >> (no - 64))) & 1) == 0;
     const int off = f.data_off;
     uint64_t v;
     if (off == 0) v = lo;
     else if (off < 64) v = (lo >> off) | (hi << (64 - off));
     else v = hi >> (off - 64);
     if (f.bits < 64) v &= (1ull << f.bits) - 1;
+    if (f.xform != KX_RAW) {  // a narrowed field: back to the raw bits
+        v += f.bias;
+        if (f.xform == KX_SIGNED) v ^= 1ull << 63;
+    }
     bits = valid ? v : 0;
 }
 

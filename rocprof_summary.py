@@ -15,7 +15,8 @@ the copy (which moves exactly its size each way):
 The counters are KiB per dispatch (summed over the device's channels).
 
 ``traffic_all OUT "title" "commands"`` prints the same scaled counters for
-every kernel of the run as a markdown table.
+every kernel of the run as a markdown table.  ``highcard --method hash|sort``
+runs the 200M-row composite-key workload instead of ``zipf100m``.
 
 Kernel traces: ``python3 rocprof_summary.py trace DB_DIR "title" "command"`` prints
 a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database."""
@@ -46,6 +47,20 @@ def run_zipf100m(steps: int, rows: int) -> None:
     for i in range(steps):
         r = _native.execute_filter_groupby_device(plan, rows, cols)
         print(f"step {i}: {r.stats['n_groups']} groups", flush=True)
+        r.free()
+
+
+def run_highcard(steps: int, method: str) -> None:
+    from polars_metal import _native, synth
+    from polars_metal.buffer import device_columns
+    n = 200_000_000
+    frame, plan = synth.highcard(n, seed=3)
+    cols, bufs = device_columns(frame)
+    del frame
+    plan = dict(plan, options={"method": method})
+    for i in range(steps):
+        r = _native.execute_filter_groupby_device(plan, n, cols)
+        print(f"step {i}: {r.stats['n_groups']} groups ({r.stats['method']})", flush=True)
         r.free()
 
 
@@ -144,13 +159,16 @@ def trace_markdown(db_dir: Path, title: str, command: str) -> str:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("zipf100m", "copy", "traffic", "traffic_all", "trace"))
+    ap.add_argument("mode", choices=("zipf100m", "highcard", "copy", "traffic", "traffic_all", "trace"))
     ap.add_argument("args", nargs="*")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--rows", type=int, default=100_000_000)
+    ap.add_argument("--method", default="hash", help="highcard: the engine")
     a = ap.parse_args(argv)
     if a.mode == "zipf100m":
         run_zipf100m(a.steps, a.rows)
+    elif a.mode == "highcard":
+        run_highcard(a.steps, a.method)
     elif a.mode == "copy":
         run_copy(a.steps)
     elif a.mode == "traffic_all":

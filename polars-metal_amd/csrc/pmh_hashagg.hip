@@ -1300,8 +1300,15 @@ __global__ __launch_bounds__(256) void hs_part_bounds(const HLaunch *__restrict_
 
 // A tail row whose key found no slot in its partition's LDS table folds
 // into the global fallback table: rare, so kept out of line.
-__device__ __noinline__ void part_fallback(const HLaunch *__restrict__ L, Key k, uint64_t h, SpillRowV row) {
+// Scalar arguments only: structures passed by value to an out-of-line call
+// go through the stack, which gave every instantiation of the caller 64
+// bytes of scratch per lane.
+__device__ __noinline__ void part_fallback(const HLaunch *__restrict__ L, const Spill *S, uint64_t i, uint32_t fl,
+                                           uint64_t v0, uint64_t v1, uint64_t klo, uint64_t khi, int special,
+                                           uint64_t h) {
     GSink gs{L->t.arr, L->t.mask + 3};
+    const Key k{klo, khi, special};
+    const SpillRowV row{L, S, i, (uint16_t)fl, v0, v1};
     const int64_t g = L->wide ? g_find128(L->t, k, h) : g_find64(L->t, k, h);
     if (g >= 0) add_row(gs, &L->q, (uint64_t)g, row);
 }
@@ -1417,7 +1424,7 @@ __global__ __launch_bounds__(1024) void hs_part_agg(const HLaunch *__restrict__ 
                     if constexpr (SIG >= 0) lds_agg2_add_sig<SIG>(A, (uint32_t)slot, fl, v0b[u], v1b[u]);
                     else add_row(ls, q, (uint32_t)slot, row);
                 } else {
-                    part_fallback(L, k, h, row);
+                    part_fallback(L, &S, i, fl, v0b[u], v1b[u], k.lo, k.hi, k.special, h);
                 }
             }
         }

@@ -876,38 +876,62 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
             for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = 0;
         }
     };
-    // one lane writes the totals of one run of group gid
-    auto flush = [&](uint64_t gid, const Carry &x) {
-        if (x.len) g.add32(lay.a_len, gid, x.len);
+    // one lane writes the totals of one run of group gid: with plain stores
+    // when the run is the group's only one (it starts and ends inside this
+    // tile, so no other tile touches the group; the arrays hold identities),
+    // else with atomics
+    auto flush = [&](uint64_t gid, const Carry &x, bool only) {
+        auto w32 = [&](int a, uint32_t v) {
+            if (only) gst<uint32_t>(L->arr[a], gid, v);
+            else g.add32(a, gid, v);
+        };
+        if (x.len) w32(lay.a_len, x.len);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             if (j >= nv || !x.cnt[j]) continue;
             const ValSpec &S = vs[j];
-            g.add32(S.a_cnt, gid, x.cnt[j]);
+            w32(S.a_cnt, x.cnt[j]);
             if (S.need & NEED_SUM) {
                 if (S.kind == VK_F) {
 #pragma unroll
-                    for (int z = 0; z < kLimbs; ++z)
-                        if (x.limb[j][z]) g.addlimb(S.a_limb, z, gid, x.limb[j][z]);
+                    for (int z = 0; z < kLimbs; ++z) {
+                        if (!x.limb[j][z]) continue;
+                        if (only) gst<int64_t>(L->arr[S.a_limb], (uint64_t)z * L->gcap + gid, x.limb[j][z]);
+                        else g.addlimb(S.a_limb, z, gid, x.limb[j][z]);
+                    }
+                } else if (only) {
+                    gst<int64_t>(L->arr[S.a_sum], gid, x.sum[j]);
                 } else {
                     g.add64(S.a_sum, gid, x.sum[j]);
                 }
             }
             if (S.need & NEED_MINMAX) {
                 if (S.kind == VK_I32) {
-                    if (x.mn[j] != INT64_MAX) g.min32(S.a_min, gid, (int32_t)x.mn[j]);
-                    if (x.mx[j] != INT64_MIN) g.max32(S.a_max, gid, (int32_t)x.mx[j]);
+                    if (only) {
+                        gst<int32_t>(L->arr[S.a_min], gid, (int32_t)x.mn[j]);
+                        gst<int32_t>(L->arr[S.a_max], gid, (int32_t)x.mx[j]);
+                    } else {
+                        if (x.mn[j] != INT64_MAX) g.min32(S.a_min, gid, (int32_t)x.mn[j]);
+                        if (x.mx[j] != INT64_MIN) g.max32(S.a_max, gid, (int32_t)x.mx[j]);
+                    }
+                } else if (only) {
+                    gst<int64_t>(L->arr[S.a_min], gid, x.mn[j]);
+                    gst<int64_t>(L->arr[S.a_max], gid, x.mx[j]);
                 } else {
                     if (x.mn[j] != INT64_MAX) g.min64(S.a_min, gid, x.mn[j]);
                     if (x.mx[j] != INT64_MIN) g.max64(S.a_max, gid, x.mx[j]);
                 }
-                if (S.kind == VK_F && x.nan[j]) g.or32(S.a_nan, gid, 1u);
+                if (S.kind == VK_F && x.nan[j]) {
+                    if (only) gst<uint32_t>(L->arr[S.a_nan], gid, 1u);
+                    else g.or32(S.a_nan, gid, 1u);
+                }
             }
         }
     };
     reset();
     const uint64_t tb = (uint64_t)t * kSortTile;
     uint64_t before = 0;  // group starts seen in this tile before the current iteration
+    bool c_own = false;   // the carried run started inside this tile
     for (uint32_t k = 0; k < kSortTile; k += 64) {
         const uint64_t i = tb + k + lane;
         const bool live = i < L->n_sel;
@@ -923,7 +947,7 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
         }
         // the open run of the previous iteration ends where this one starts
         if (have && (sm & 1ull)) {
-            if (lane == 0) flush(cgid, c);
+            if (lane == 0) flush(cgid, c, c_own);
             have = false;
             reset();
         }
@@ -1018,8 +1042,12 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
                 for (int z = 0; z < kLimbs; ++z) x.limb[j][z] += c.limb[j][z];
             }
         }
-        // closed runs go out; the last run of the iteration becomes the carry
-        if (seg_last && lane != last_live) flush(gid, x);
+        // closed runs go out; the last run of the iteration becomes the carry.
+        // A run owns its group unless it began at lane 0 as the carried run
+        // of a tile whose first records continue the previous tile's group.
+        const bool own = seg_start > 0 || (sm & 1ull) || (have && c_own);
+        if (seg_last && lane != last_live) flush(gid, x, own);
+        c_own = __shfl(own ? 1 : 0, last_live) != 0;
         cgid = __shfl(gid, last_live);
         c.len = __shfl(x.len, last_live);
 #pragma unroll
@@ -1034,7 +1062,11 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
         }
         have = true;
     }
-    if (have && lane == 0) flush(cgid, c);
+    // the tile's last run is the group's only one if the next tile starts a
+    // new group
+    const uint64_t nx = tb + kSortTile;
+    const bool closed = nx >= L->n_sel || is_start(klo, khi, two, nx);
+    if (have && lane == 0) flush(cgid, c, c_own && closed);
 }
 
 // ---------------------------------------------------------------------------

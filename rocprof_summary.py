@@ -19,7 +19,8 @@ every kernel of the run as a markdown table.  ``highcard --method hash|sort``
 runs the 200M-row composite-key workload instead of ``zipf100m``.
 
 Kernel traces: ``python3 rocprof_summary.py trace DB_DIR "title" "command"`` prints
-a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database."""
+a markdown table of a ``rocprofv3 --kernel-trace --stats`` result database;
+``counters "title" "command" DIR...`` tabulates every counter of ``--pmc`` runs."""
 from __future__ import annotations
 
 import argparse
@@ -133,6 +134,38 @@ def traffic_table(out: Path, steps: int, title: str, command: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+def counters_table(dirs: list, title: str, command: str) -> str:
+    """Markdown table of every counter of the given ``--pmc`` runs, mean
+    per dispatch, per kernel."""
+    per = collections.defaultdict(dict)
+    names = []
+    for d in dirs:
+        db = glob.glob(str(Path(d) / "*.db"))[0]
+        c = sqlite3.connect(db)
+        cols = [r[1] for r in c.execute("pragma table_info(counters_collection)")]
+        ki, ni, vi = cols.index("kernel_name"), cols.index("counter_name"), cols.index("value")
+        di = cols.index("dispatch_id")
+        tot = collections.defaultdict(float)
+        disp = collections.defaultdict(set)
+        for r in c.execute("select * from counters_collection"):
+            k = r[ki].split("(")[0].split("::")[-1].split("<")[0].strip()
+            tot[(k, r[ni])] += r[vi]
+            disp[(k, r[ni])].add(r[di])
+        for (k, n), v in tot.items():
+            per[k][n] = v / len(disp[(k, n)])
+            if n not in names:
+                names.append(n)
+    lines = [f"# {title}", "", "Commands (from `/tmp`, with `REPO` the repository root, on one MI355X; each counter "
+             "group in its own run):", "", "```", command, "```", "",
+             "Mean per dispatch (rocprofv3 `--pmc`, summed over the device):", "",
+             "| kernel | " + " | ".join(names) + " |", "|---|" + "---|" * len(names)]
+    for k in sorted(per, key=lambda k: -per[k].get("SQ_WAVE_CYCLES", 0.0)):
+        if k == COPY_KERNEL:
+            continue
+        lines.append(f"| `{k}` | " + " | ".join(f"{per[k].get(n, 0.0):.0f}" for n in names) + " |")
+    return "\n".join(lines) + "\n"
+
+
 def short_name(name: str) -> str:
     n = name.replace("pmh::(anonymous namespace)::", "").replace("pmh::", "").replace("void ", "")
     return n.split("(")[0].strip()
@@ -159,7 +192,7 @@ def trace_markdown(db_dir: Path, title: str, command: str) -> str:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("zipf100m", "highcard", "copy", "traffic", "traffic_all", "trace"))
+    ap.add_argument("mode", choices=("zipf100m", "highcard", "copy", "traffic", "traffic_all", "trace", "counters"))
     ap.add_argument("args", nargs="*")
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--rows", type=int, default=100_000_000)
@@ -173,6 +206,8 @@ def main(argv=None) -> int:
         run_copy(a.steps)
     elif a.mode == "traffic_all":
         print(traffic_table(Path(a.args[0]), a.steps, a.args[1], a.args[2]), end="")
+    elif a.mode == "counters":
+        print(counters_table(a.args[2:], a.args[0], a.args[1]), end="")
     elif a.mode == "traffic":
         print(json.dumps(summarize(Path(a.args[0]), a.args[1] if len(a.args) > 1 else "hs_scan_fast"), indent=1))
     else:

@@ -143,7 +143,14 @@ int pmh_device_name(char *buf, size_t buf_len);
  * plan_json: {"keys": [[name, tag], ...], "aggs": [{"kind": "Simple"|"Length",
  * "input_col": ..., "op": "Sum|Mean|Count|Min|Max|Len", "output_alias": ...}],
  * "options"?: {...}}.  Outputs: keys in `keys` order, then aggs in `aggs`
- * order (mod.rs:1109-1143). */
+ * order (mod.rs:1109-1143).
+ * An agg of kind "Expression" carries an AggExpr tree under "expr"
+ * (Column / LiteralF64 / LiteralI64 / Binary Add|Sub|Mul|Div, at most four
+ * binary levels; mod.rs:278-360, plan/mod.rs:133-178); the tree is evaluated
+ * per row on the device with Polars' arithmetic dtypes and folded by "op",
+ * where the reference evaluates it through Polars on the host first
+ * (python/polars_metal/_udf.py:1164-1209).  Operand dtypes Polars types
+ * outside the rules restated in pmh_expr.hip return PMH_E_NOTIMPL. */
 int pmh_execute_groupby(const char *plan_json, uint64_t n_rows, const pmh_column *columns,
                         uint32_t n_columns, pmh_result **out);
 

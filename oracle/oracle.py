@@ -283,6 +283,152 @@ def encode_keys(frame: dict, keys: list):
 
 
 # ---------------------------------------------------------------------------
+# expression aggregates
+# ---------------------------------------------------------------------------
+# The reference evaluates an Expression agg's AggExpr tree
+# (crates/polars-metal-core/src/plan/mod.rs:133-142) through Polars before
+# the group-by call and folds the result as a Simple agg
+# (python/polars_metal/_udf.py:1164-1209).  Restated here with Polars'
+# arithmetic rules: Add/Sub/Mul in the numeric supertype (integers wrap,
+# floats round once per operation), Div true division (Float64 for two
+# integers), literals typed by the other operand, nulls propagated from every
+# column read.  Shapes Polars types outside these rules raise
+# NotImplementedError, as the backend does.
+_INT_BITS = {"I8": 8, "I16": 16, "I32": 32, "I64": 64, "U8": 8, "U16": 16, "U32": 32, "U64": 64}
+_FLOATS = ("F32", "F64")
+
+
+def _supertype(a: str, b: str) -> str:
+    if a == b:
+        return a
+    if a in _FLOATS and b in _FLOATS:
+        return "F64"
+    if a in _FLOATS or b in _FLOATS:
+        f, i = (a, b) if a in _FLOATS else (b, a)
+        return "F64" if f == "F64" or _INT_BITS[i] > 16 else "F32"
+    sa, sb = a.startswith("I"), b.startswith("I")
+    if sa == sb:
+        return a if _INT_BITS[a] >= _INT_BITS[b] else b
+    si, ui = (a, b) if sa else (b, a)
+    if _INT_BITS[si] > _INT_BITS[ui]:
+        return si
+    if ui == "U64":
+        raise NotImplementedError("signed integer with UInt64")
+    return "I" + str(2 * _INT_BITS[ui])
+
+
+def _lit_value(lit, tag: str):
+    """An open literal ('int'|'float', value) as a numpy scalar of tag."""
+    kind, v = lit
+    if tag in _FLOATS:
+        return NP_DTYPE[tag](float(v) if kind == "float" else int(v))
+    if kind == "float":
+        raise NotImplementedError("float literal as an integer")
+    info = np.iinfo(NP_DTYPE[tag])
+    if not info.min <= int(v) <= info.max:
+        raise NotImplementedError(f"literal {v} does not fit {tag}")
+    return NP_DTYPE[tag](int(v))
+
+
+def _expr(e: dict, frame: dict, used: list):
+    """-> ('col', tag, values) or ('lit', (kind, value))."""
+    k = e["kind"]
+    if k == "Column":
+        tag, values, _ = frame[e["name"]]
+        if tag not in _INT_BITS and tag not in _FLOATS:
+            raise NotImplementedError(f"{tag} in arithmetic")
+        if e["name"] not in used:
+            used.append(e["name"])
+        return ("col", tag, np.asarray(values, dtype=NP_DTYPE[tag]))
+    if k == "LiteralI64":
+        return ("lit", ("int", int(e["value"])))
+    if k == "LiteralF64":
+        return ("lit", ("float", float(e["value"])))
+    if k != "Binary":
+        raise ValueError(k)
+    op = e["op"]
+    lhs, rhs = _expr(e["lhs"], frame, used), _expr(e["rhs"], frame, used)
+    if lhs[0] == "lit" and rhs[0] == "lit":
+        (ka, a), (kb, b) = lhs[1], rhs[1]
+        if ka == kb == "int" and op != "Div":
+            r = {"Add": a + b, "Sub": a - b, "Mul": a * b}[op]
+            r = (r + 2 ** 63) % 2 ** 64 - 2 ** 63
+            return ("lit", ("int", r))
+        a, b = float(a), float(b)
+        with np.errstate(all="ignore"):
+            r = {"Add": np.float64(a) + b, "Sub": np.float64(a) - b, "Mul": np.float64(a) * b,
+                 "Div": np.float64(a) / b}[op]
+        return ("lit", ("float", float(r)))
+
+    def side(me, other):
+        if me[0] == "col":
+            return me[1]
+        return other[1] if me[1][0] == "int" else ("F32" if other[1] == "F32" else "F64")
+
+    tl, tr = side(lhs, rhs), side(rhs, lhs)
+    if op == "Div" and tl in _INT_BITS and tr in _INT_BITS:
+        ct = "F64"
+    else:
+        ct = _supertype(tl, tr)
+
+    def operand(x, t):
+        if x[0] == "lit":
+            _lit_value(x[1], t)  # must hold its operand type
+            return _lit_value(x[1], ct)
+        return x[2].astype(NP_DTYPE[ct])
+
+    a, b = operand(lhs, tl), operand(rhs, tr)
+    with np.errstate(all="ignore"):
+        if op == "Add":
+            r = a + b
+        elif op == "Sub":
+            r = a - b
+        elif op == "Mul":
+            r = a * b
+        else:
+            r = a / b
+    n = frame_rows(frame)
+    return ("col", ct, np.broadcast_to(np.asarray(r, dtype=NP_DTYPE[ct]), (n,)).copy())
+
+
+def eval_expr(expr: dict, frame: dict) -> tuple:
+    """AggExpr dict -> (tag, values, valid) for every row of frame."""
+    used: list = []
+    r = _expr(expr, frame, used)
+    if r[0] == "lit":
+        raise NotImplementedError("a tree of literals only")
+    tag, values = r[1], r[2]
+    valid = np.ones(len(values), bool)
+    for nm in used:
+        v = frame[nm][2]
+        if v is not None:
+            valid &= np.asarray(v, bool)
+    values = np.where(valid, values, np.zeros(1, values.dtype))
+    return tag, values, valid
+
+
+def materialize_expressions(plan: dict, frame: dict) -> tuple:
+    """The reference's _materialize_expression_aggs (_udf.py:1173-1209): each
+    Expression agg becomes a Simple agg over a column holding its values."""
+    if not any(a.get("kind") == "Expression" for a in plan["aggs"]):
+        return plan, frame
+    import json
+    frame = dict(frame)
+    aggs = []
+    names: dict = {}  # one column per distinct tree
+    for a in plan["aggs"]:
+        if a.get("kind") != "Expression" or a.get("op") == "Len":
+            aggs.append(a)
+            continue
+        text = json.dumps(a["expr"], sort_keys=True)
+        if text not in names:
+            names[text] = f"__pm_expr_agg_{len(names)}__"
+            frame[names[text]] = eval_expr(a["expr"], frame)
+        aggs.append({"kind": "Simple", "input_col": names[text], "op": a["op"], "output_alias": a["output_alias"]})
+    return dict(plan, aggs=aggs), frame
+
+
+# ---------------------------------------------------------------------------
 # group-by
 # ---------------------------------------------------------------------------
 def groupby(plan: dict, frame: dict, sum_mode: int = 0, threads: int | None = None) -> dict:
@@ -291,6 +437,7 @@ def groupby(plan: dict, frame: dict, sum_mode: int = 0, threads: int | None = No
     order, plus the keys '__n_groups__' and '__ops__' (output alias -> agg
     op, which the comparison helpers read).  ``threads`` runs the all-core
     form (orc_groupby_par): the same bits, groups in partition order."""
+    plan, frame = materialize_expressions(plan, frame)
     n = frame_rows(frame)
     keep_mask = None
     if plan.get("predicate"):

@@ -440,6 +440,10 @@ int pmh_execute_batch(const char *template_plan_json, const pmh_batch_query *que
         for (uint32_t i = 0; i < n_queries; ++i) results[i] = nullptr;
         GroupByPlan tmpl = parse_groupby_plan(template_plan_json, true);
         tmpl.predicate.reset();
+        for (auto &a : tmpl.aggs)
+            if (a.expr)
+                fail(PMH_E_NOTIMPL, "polars_metal: a batch folds columns only; run expression aggregates "
+                                    "through pmh_execute_filter_groupby");
         std::vector<BatchQueryIn> qs(n_queries);
         for (uint32_t i = 0; i < n_queries; ++i)
             qs[i] = BatchQueryIn{queries[i].n_rows, queries[i].columns, queries[i].n_columns, queries[i].predicate_json};

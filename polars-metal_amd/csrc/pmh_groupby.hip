@@ -442,8 +442,11 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
     // ---- columns referenced by the plan
     std::vector<std::string> needed;
     for (auto &k : plan.keys) needed.push_back(k.first);
-    for (auto &a : plan.aggs)
-        if (a.op != AGG_LEN) needed.push_back(a.input_col);
+    for (auto &a : plan.aggs) {
+        if (a.op == AGG_LEN) continue;
+        if (a.expr) collect_expr_cols(*a.expr, needed);
+        else needed.push_back(a.input_col);
+    }
     if (plan.predicate) collect_pred_cols(*plan.predicate, needed);
     for (auto &c : plan.extra_cols) needed.push_back(c);
     // dtype checks before any device work (the same failures the reference
@@ -460,16 +463,33 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
         if (t != pt) fail(PMH_E_VALUE, "polars_metal: key " + k.first + " dtype mismatch between plan and column");
         if (tag_size(c->tag) < 0) fail(PMH_E_NOTIMPL, "polars_metal: unsupported key dtype");
     }
+    // an expression aggregate folds a column of the expression's dtype
+    std::function<int(const std::string &)> host_tag = [&](const std::string &nm) { return find_host(nm)->tag; };
     for (auto &a : plan.aggs) {
         if (a.op == AGG_LEN) continue;
-        const pmh_column *c = find_host(a.input_col);
-        if (agg_vkind(c->tag) < 0)
+        const int tag = a.expr ? expr_result_tag(*a.expr, host_tag) : find_host(a.input_col)->tag;
+        if (agg_vkind(tag) < 0)
             fail(PMH_E_VALUE, std::string("polars_metal: unsupported (agg_op, dtype) combination: ") + op_name(a.op) +
-                                  " / " + tag_name(c->tag));
+                                  " / " + tag_name(tag));
     }
     for (auto &nm : needed) check_column(*find_host(nm), n_rows, nm);
     rt();
     std::vector<Column> dcols = stage_columns(cols, n_cols, n_rows, needed);
+    // expression aggregates: each distinct tree evaluated once into a
+    // device column
+    std::vector<std::string> agg_input(plan.aggs.size());
+    for (size_t ai = 0; ai < plan.aggs.size(); ++ai) {
+        const PlanAgg &a = plan.aggs[ai];
+        agg_input[ai] = a.input_col;
+        if (a.op == AGG_LEN || !a.expr) continue;
+        for (size_t aj = 0; aj < ai && agg_input[ai] == a.input_col; ++aj)
+            if (plan.aggs[aj].expr && plan.aggs[aj].op != AGG_LEN && expr_equal(*plan.aggs[aj].expr, *a.expr))
+                agg_input[ai] = agg_input[aj];
+        if (agg_input[ai] != a.input_col) continue;
+        const std::string nm = "expr:" + std::to_string(ai);
+        dcols.push_back(materialize_expr(*a.expr, dcols, n_rows, nm));
+        agg_input[ai] = nm;
+    }
     auto col_index = [&](const std::string &nm) -> int {
         for (size_t i = 0; i < dcols.size(); ++i)
             if (dcols[i].name == nm) return (int)i;
@@ -533,7 +553,7 @@ void prepare_query(Prep &P, const GroupByPlan &plan, uint64_t n_rows, const pmh_
     for (size_t ai = 0; ai < plan.aggs.size(); ++ai) {
         const PlanAgg &a = plan.aggs[ai];
         if (a.op == AGG_LEN) continue;
-        int ci = col_index(a.input_col);
+        int ci = col_index(agg_input[ai]);
         int vj = -1;
         for (int j = 0; j < q.n_vals; ++j)
             if (q.vals[j].col == ci) vj = j;

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -132,6 +133,22 @@ struct Layout {
     int32_t limbs_contig;  // limb arrays of one value column are consecutive
 };
 
+// Expression program (pmh_expr.hip): nodes in postfix order; each node's
+// value has dtype `tag`; a binary node converts both operands (of dtypes
+// ta, tb) to `ctag` first and computes in it.
+enum ExprKind : int32_t { EX_COL = 0, EX_LIT = 1, EX_ADD = 2, EX_SUB = 3, EX_MUL = 4, EX_DIV = 5 };
+constexpr int kMaxExprNodes = 31;  // a binary tree four levels deep
+constexpr int kMaxExprCols = 16;
+struct ExprNode {
+    int32_t kind, tag, ctag, ta, tb, col;
+    uint64_t lit;  // EX_LIT: the literal's bits in dtype `tag`
+};
+struct ExprProg {
+    int32_t n_nodes, n_cols;
+    DevCol cols[kMaxExprCols];
+    ExprNode nodes[kMaxExprNodes];
+};
+
 struct QueryDesc {
     uint64_t n_rows;
     int32_t n_cols;
@@ -150,11 +167,30 @@ struct QueryDesc {
 // ---------------------------------------------------------------------------
 enum AggOpHost { AGG_SUM, AGG_MEAN, AGG_COUNT, AGG_MIN, AGG_MAX, AGG_LEN };
 
+// Parsed AggExpr (crates/polars-metal-core/src/plan/mod.rs:133-142): a
+// column, an f64 or i64 literal, or Add/Sub/Mul/Div of two subtrees, at most
+// four binary levels deep (plan/mod.rs:169-178).
+struct PlanExpr {
+    enum Kind { COLUMN, LIT_F64, LIT_I64, BINARY } kind = COLUMN;
+    std::string name;  // COLUMN
+    int op = 0;        // BINARY: ExprKind EX_ADD .. EX_DIV
+    int64_t ival = 0;  // LIT_I64
+    double fval = 0;   // LIT_F64
+    std::shared_ptr<PlanExpr> lhs, rhs;
+};
+constexpr int kMaxExprDepth = 4;
+
 struct PlanAgg {
     std::string input_col;
     int op;
     std::string alias;
+    // kind "Expression": the per-row expression the reducer `op` folds
+    std::shared_ptr<PlanExpr> expr;
 };
+
+// Every column an expression reads, in tree order.
+void collect_expr_cols(const PlanExpr &e, std::vector<std::string> &out);
+bool expr_equal(const PlanExpr &a, const PlanExpr &b);  // same tree, literals by bits
 
 struct PlanPredNode {  // parsed PredicateAst
     std::string kind;
@@ -309,6 +345,15 @@ void utf8_dict_codes(const pmh_utf8_dict &d, const uint32_t **codes, const uint8
 // result helpers
 pmh_result *new_result(size_t n_cols);
 void set_out_name(pmh_out_column &c, const std::string &name);
+
+// Expression aggregates (pmh_expr.hip).  expr_result_tag: the dtype of the
+// expression's value under Polars' arithmetic rules, given the dtype of each
+// column (PMH_E_NOTIMPL for a shape whose rules are not restated here).
+// materialize_expr: evaluates it for every row into a device column (slab
+// `slab_name`); a row is null where any column it reads is null.
+int expr_result_tag(const PlanExpr &e, const std::function<int(const std::string &)> &col_tag);
+Column materialize_expr(const PlanExpr &e, const std::vector<Column> &cols, uint64_t n_rows,
+                        const std::string &slab_name);
 
 // group-by entry (pmh_groupby.hip)
 pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const pmh_column *cols,

@@ -242,7 +242,67 @@ std::unique_ptr<PlanPredNode> parse_pred(const JVal &v, int depth) {
     return n;
 }
 
+// parse_agg_expr_dict (crates/polars-metal-core/src/udf/groupby/mod.rs:278-360)
+// with AggExpr::validate's depth cap of four binary levels (plan/mod.rs:169-178).
+std::shared_ptr<PlanExpr> parse_expr(const JVal &v, int depth) {
+    if (v.t != JVal::OBJ) parse_err("wrong type for field expr(dict)");
+    auto e = std::make_shared<PlanExpr>();
+    const JVal *k = v.get("kind");
+    if (!k || k->t != JVal::STR) parse_err("wrong type for field expr.kind");
+    if (k->s == "Column") {
+        const JVal *nm = v.get("name");
+        if (!nm || nm->t != JVal::STR) parse_err("wrong type for field expr.name");
+        e->kind = PlanExpr::COLUMN;
+        e->name = nm->s;
+    } else if (k->s == "LiteralF64") {
+        const JVal *x = v.get("value");
+        if (!x || !x->is_num()) parse_err("wrong type for field expr.value(f64)");
+        e->kind = PlanExpr::LIT_F64;
+        e->fval = x->num();
+    } else if (k->s == "LiteralI64") {
+        const JVal *x = v.get("value");
+        if (!x || x->t != JVal::INT) parse_err("wrong type for field expr.value(i64)");
+        e->kind = PlanExpr::LIT_I64;
+        e->ival = x->i;
+    } else if (k->s == "Binary") {
+        const JVal *op = v.get("op");
+        if (!op || op->t != JVal::STR) parse_err("wrong type for field expr.op");
+        e->kind = PlanExpr::BINARY;
+        if (op->s == "Add") e->op = EX_ADD;
+        else if (op->s == "Sub") e->op = EX_SUB;
+        else if (op->s == "Mul") e->op = EX_MUL;
+        else if (op->s == "Div") e->op = EX_DIV;
+        else parse_err("unknown agg op binary op " + op->s);
+        if (depth >= kMaxExprDepth) parse_err("wrong type for field expr(too deep)");
+        const JVal *l = v.get("lhs");
+        const JVal *r = v.get("rhs");
+        if (!l) parse_err("wrong type for field expr.lhs");
+        if (!r) parse_err("wrong type for field expr.rhs");
+        e->lhs = parse_expr(*l, depth + 1);
+        e->rhs = parse_expr(*r, depth + 1);
+    } else {
+        parse_err("wrong type for field expr.kind");
+    }
+    return e;
+}
+
 }  // namespace
+
+bool expr_equal(const PlanExpr &a, const PlanExpr &b) {
+    if (a.kind != b.kind) return false;
+    switch (a.kind) {
+    case PlanExpr::COLUMN: return a.name == b.name;
+    case PlanExpr::LIT_I64: return a.ival == b.ival;
+    case PlanExpr::LIT_F64: return memcmp(&a.fval, &b.fval, sizeof(double)) == 0;
+    default: return a.op == b.op && expr_equal(*a.lhs, *b.lhs) && expr_equal(*a.rhs, *b.rhs);
+    }
+}
+
+void collect_expr_cols(const PlanExpr &e, std::vector<std::string> &out) {
+    if (e.kind == PlanExpr::COLUMN) out.push_back(e.name);
+    if (e.lhs) collect_expr_cols(*e.lhs, out);
+    if (e.rhs) collect_expr_cols(*e.rhs, out);
+}
 
 const char *tag_name(int tag) {
     static const char *names[] = {"I64", "F64", "Bool", "I32", "F32", "I8", "I16", "U8", "U16", "U32", "U64", "Utf8"};
@@ -309,9 +369,17 @@ GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate) {
             a.op = parse_op(op->s);
             if (a.op < 0) parse_err("unknown agg op " + op->s);
         } else if (kind == "Expression") {
-            // Expression aggs are evaluated by Polars before the call in the
-            // reference (python/polars_metal/_udf.py:1164-1209).
-            fail(PMH_E_NOTIMPL, "polars_metal: Expression aggs are not lowered by this backend");
+            // The reference evaluates the expression through Polars on the
+            // host before the call (python/polars_metal/_udf.py:1164-1209);
+            // here it is evaluated on the device (pmh_expr.hip).
+            const JVal *op = e.get("op");
+            if (!op || op->t != JVal::STR) parse_err("wrong type for field op");
+            a.op = parse_op(op->s);
+            if (a.op < 0) parse_err("unknown agg op " + op->s);
+            const JVal *ex = e.get("expr");
+            if (!ex) parse_err("wrong type for field expr");
+            a.expr = parse_expr(*ex, 0);
+            if (a.op == AGG_LEN) a.expr.reset();
         } else {
             parse_err("unknown agg op kind=" + kind);
         }

@@ -198,7 +198,7 @@ void check_column(const pmh_column &c, uint64_t n_rows, const std::string &nm) {
 
 std::vector<Column> stage_columns(const pmh_column *cols, uint32_t n_cols, uint64_t n_rows,
                                   const std::vector<std::string> &needed) {
-    Runtime &r = rt();
+    rt();  // device set up before any staging
     std::vector<Column> out;
     std::set<std::string> seen;
     int staged = 0;

@@ -25,9 +25,13 @@ differ, each on purpose:
     otherwise), where the reference's ``_materialize_arrow`` concatenates
     on the host (_udf.py:1441-1456).
 
-Shapes this module does not lower (expression aggregates, HStack, other
-upstream nodes) raise NotImplementedError; ``polars_metal.install`` then
-hands them to the reference's own dispatch.
+  * Expression aggregates (``sum(a * (1 - b))``) are not evaluated through
+    Polars into temporary columns (_udf.py:1164-1209); the AggExpr tree goes
+    into the plan and the device evaluates it inside the same call.
+
+Shapes this module does not lower (HStack, other upstream nodes) raise
+NotImplementedError; ``polars_metal.install`` then hands them to the
+reference's own dispatch.
 """
 from __future__ import annotations
 
@@ -83,12 +87,6 @@ def _needed_columns(plan: dict, predicate: dict | None) -> list:
 
     for k in plan["keys"]:
         add(k[0])
-    for a in plan["aggs"]:
-        if a.get("kind") == "Expression":
-            raise NotImplementedError("polars_metal: Expression aggregates are evaluated by the reference's hook")
-        if a.get("kind") != "Length" and a.get("op") != "Len":
-            add(a.get("input_col"))
-
     def walk(p):
         if p is None:
             return
@@ -97,6 +95,13 @@ def _needed_columns(plan: dict, predicate: dict | None) -> list:
         walk(p.get("lhs"))
         walk(p.get("rhs"))
 
+    for a in plan["aggs"]:
+        if a.get("kind") == "Length" or a.get("op") == "Len":
+            continue
+        if a.get("kind") == "Expression":
+            walk(a.get("expr"))  # evaluated on the device inside the call
+        else:
+            add(a.get("input_col"))
     walk(predicate)
     return needed
 

@@ -32,6 +32,19 @@ Each case is named for what it pins:
                     an f32 running sum gives 1); min / max with NaN, +-0.0,
                     +-inf
   f32_nan_keys_f64  F32 values under a predicate on an F64 column
+  expr_*            Expression aggs (AggExpr trees, crates/polars-metal-core/
+                    src/plan/mod.rs:133-142): each Binary node of a case
+                    carries its compute dtype ("ct"), assigned by hand from
+                    Polars' arithmetic rules (the reference evaluates the
+                    tree through Polars, python/polars_metal/_udf.py:
+                    1164-1209), and the per-row values are computed here
+                    with Python integers wrapped to that width, Python
+                    floats, and f32 rounding through struct after every
+                    operation; "ct" is stripped from the written plan.
+                    expr_q1_disc and expr_ab_sum are the reference's
+                    tests/python_integration/test_walker_expression_
+                    unfolding.py:17-56 shapes, sum(a * (1 - b)) and
+                    sum(a * b), with nulls added.
 
 Float min/max rule: NaN anywhere in the group (among non-null rows) gives
 the canonical quiet NaN; otherwise the extreme value, with -0.0 ordered
@@ -145,6 +158,52 @@ def wrap(x, bits):
     return x - (1 << bits) if x >= 1 << (bits - 1) else x
 
 
+INT_BITS = {"I8": 8, "I16": 16, "I32": 32, "I64": 64, "U8": 8, "U16": 16, "U32": 32, "U64": 64}
+
+
+def fdiv(a, b):
+    """IEEE division (Python raises on a zero divisor)."""
+    if b == 0.0:
+        if a == 0.0 or math.isnan(a):
+            return float("nan")
+        return math.copysign(float("inf"), a) * math.copysign(1.0, b)
+    return a / b
+
+
+def eval_expr_row(e, cols, r):
+    """Value of an annotated AggExpr tree at row r, or None when a column it
+    reads is null there."""
+    k = e["kind"]
+    if k == "Column":
+        return cols[e["name"]]["values"][r]
+    if k.startswith("Literal"):
+        return e["value"]
+    a, b = eval_expr_row(e["lhs"], cols, r), eval_expr_row(e["rhs"], cols, r)
+    if a is None or b is None:
+        # null propagates; both sides are still evaluated (a column read
+        # anywhere in the tree nulls the row)
+        return None
+    ct, op = e["ct"], e["op"]
+    if ct in INT_BITS:
+        a, b = int(a), int(b)
+        x = {"Add": a + b, "Sub": a - b, "Mul": a * b}[op]
+        bits = INT_BITS[ct]
+        return wrap(x, bits) if ct.startswith("I") else x & ((1 << bits) - 1)
+    a, b = float(a), float(b)
+    if ct == "F32":
+        a, b = f32r(a), f32r(b)
+    x = {"Add": lambda: a + b, "Sub": lambda: a - b, "Mul": lambda: a * b, "Div": lambda: fdiv(a, b)}[op]()
+    return f32r(x) if ct == "F32" else x
+
+
+def strip_ct(e):
+    e = {k: v for k, v in e.items() if k != "ct"}
+    for side in ("lhs", "rhs"):
+        if side in e:
+            e[side] = strip_ct(e[side])
+    return e
+
+
 def expected(plan, cols, n):
     keys = [k[0] for k in plan["keys"]]
     groups = {}
@@ -172,9 +231,14 @@ def expected(plan, cols, n):
             if a["op"] == "Len":
                 out[a["output_alias"]].append(len(rows))
                 continue
-            c = cols[a["input_col"]]
-            vals = [c["values"][r] for r in rows if c["values"][r] is not None]
-            tag = c["tag"]
+            if a.get("kind") == "Expression":
+                ev = [eval_expr_row(a["expr"], cols, r) for r in rows]
+                vals = [x for x in ev if x is not None]
+                tag = a["tag"]
+            else:
+                c = cols[a["input_col"]]
+                vals = [c["values"][r] for r in rows if c["values"][r] is not None]
+                tag = c["tag"]
             op = a["op"]
             if op == "Count":
                 out[a["output_alias"]].append(len(vals))
@@ -301,6 +365,100 @@ def cases():
     cs.append(("f32_under_f64_pred", {"keys": [["key", "I64"]], "predicate": gt0("p"),
                                       "aggs": [agg("v", "Sum"), agg("v", "Min"), agg("v", "Max"), agg("v", "Mean")]},
                {"key": col("I64", [1, 1, 1, 2, 2, 2, 1, 2]), "v": col("F32", gv), "p": col("F64", gp)}, 8))
+    cs.extend(expression_cases())
+    return cs
+
+
+def col_(name):
+    return {"kind": "Column", "name": name}
+
+
+def lit(v):
+    return {"kind": "LiteralF64" if isinstance(v, float) else "LiteralI64", "value": v}
+
+
+def binop(op, lhs, rhs, ct):
+    return {"kind": "Binary", "op": op, "lhs": lhs, "rhs": rhs, "ct": ct}
+
+
+def xagg(expr, op, alias, tag):
+    """An Expression agg; `tag` (the tree's result dtype) is for this
+    generator only and is stripped from the written plan."""
+    return {"kind": "Expression", "op": op, "output_alias": alias, "expr": expr, "tag": tag}
+
+
+def expression_cases():
+    cs = []
+    # sum(a * (1 - b)) and friends on the reference's Q1 shape: F64 columns,
+    # an integer literal 1 taking Float64 beside b
+    k = [0, 0, 1, 1] * 6 + [2, 2]
+    a = [10.0, 20.0, 30.0, 40.0] * 6 + [None, 5.5]
+    b = [0.05, 0.1, 0.15, 0.2] * 6 + [0.5, None]
+    disc = binop("Mul", col_("a"), binop("Sub", lit(1), col_("b"), "F64"), "F64")
+    cs.append(("expr_q1_disc", {"keys": [["k", "I64"]],
+                                "aggs": [xagg(disc, "Sum", "disc", "F64"), xagg(disc, "Mean", "disc_mean", "F64"),
+                                         xagg(disc, "Count", "disc_n", "F64"), xagg(disc, "Min", "disc_min", "F64"),
+                                         xagg(disc, "Max", "disc_max", "F64"), agg("a", "Sum")]},
+               {"k": col("I64", k), "a": col("F64", a), "b": col("F64", b)}, len(k)))
+    k = [0, 0, 1, 1, 2, 2] * 5
+    a = [1.0, 2.0, 3.0, 4.0, 5.0, 6.0] * 5
+    b = [0.1, 0.2, 0.3, 0.4, 0.5, 0.6] * 5
+    cs.append(("expr_ab_sum", {"keys": [["k", "I64"]],
+                               "aggs": [xagg(binop("Mul", col_("a"), col_("b"), "F64"), "Sum", "sum_ab", "F64")]},
+               {"k": col("I64", k), "a": col("F64", a), "b": col("F64", b)}, len(k)))
+    # i32 * i32 wraps in 32 bits, then the i32 sum wraps too; + literal 7
+    x = [70000, -70000, 46341, 1 << 30, None, 3, -(1 << 31), 12345]
+    y = [70000, 70000, 46341, 4, 9, None, -1, -3]
+    prod = binop("Add", binop("Mul", col_("x"), col_("y"), "I32"), lit(7), "I32")
+    cs.append(("expr_i32_wrap", {"keys": [["k", "I32"]],
+                                 "aggs": [xagg(prod, "Sum", "s", "I32"), xagg(prod, "Min", "mn", "I32"),
+                                          xagg(prod, "Max", "mx", "I32"), xagg(prod, "Mean", "m", "I32"),
+                                          xagg(prod, "Count", "c", "I32")]},
+               {"k": col("I32", [1, 1, 2, 2, 1, 2, 3, 3]), "x": col("I32", x), "y": col("I32", y)}, 8))
+    # true division: two integers give Float64 (x / 0 -> +-inf, 0 / 0 -> NaN)
+    p = [7, -7, 0, 1 << 62, 9, None, 5, -12]
+    q = [2, 0, 0, 3, -4, 1, 0, 5]
+    cs.append(("expr_int_div", {"keys": [["k", "I64"]],
+                                "aggs": [xagg(binop("Div", col_("p"), col_("q"), "F64"), "Sum", "s", "F64"),
+                                         xagg(binop("Div", col_("p"), col_("q"), "F64"), "Max", "mx", "F64"),
+                                         xagg(binop("Div", col_("w"), lit(3), "F64"), "Sum", "w3", "F64"),
+                                         xagg(binop("Div", col_("w"), lit(3), "F64"), "Min", "w3min", "F64")]},
+               {"k": col("I64", [0, 0, 1, 1, 2, 2, 3, 3]), "p": col("I64", p), "q": col("I64", q),
+                "w": col("I32", [1, 2, 3, 4, -5, 6, 7, None])}, 8))
+    # Float32: a float literal beside F32 is F32; I16 with F32 gives F32;
+    # every operation rounds to f32
+    f = [f32r(v) for v in [0.1, 1.0, 3.3, -2.5, 1e7, 0.7, 16777216.0, 2.0]]
+    h = [3, -4, 100, 32767, -32768, 0, 1, None]
+    fexpr = binop("Add", binop("Mul", col_("f"), lit(0.1), "F32"), col_("h"), "F32")
+    cs.append(("expr_f32", {"keys": [["k", "I64"]],
+                            "aggs": [xagg(fexpr, "Sum", "s", "F32"), xagg(fexpr, "Min", "mn", "F32"),
+                                     xagg(fexpr, "Max", "mx", "F32"), xagg(fexpr, "Mean", "m", "F32"),
+                                     xagg(binop("Div", col_("f"), lit(3), "F32"), "Sum", "f3", "F32")]},
+               {"k": col("I64", [0, 0, 0, 1, 1, 1, 2, 2]), "f": col("F32", f), "h": col("I16", h)}, 8))
+    # promotions: I32 + U32 -> I64; (U8 - 1) wraps in U8, then with I32 -> I32;
+    # I32 * F32 -> F64; I8 * U16 -> I32
+    u = [0, 4294967295, 7, 100, 2, 3]
+    s32 = [-1, -1, 5, -200, None, 8]
+    u8 = [0, 1, 255, 9, 3, None]
+    f2 = [f32r(v) for v in [0.3, -1.5, 2.25, 1e-3, 8.0, 3.0]]
+    i8 = [-128, 127, -1, 5, 0, 9]
+    u16 = [65535, 65535, 2, 3, 1, 0]
+    cs.append(("expr_promote", {"keys": [["k", "I64"]],
+                                "aggs": [xagg(binop("Add", col_("s32"), col_("u"), "I64"), "Sum", "su", "I64"),
+                                         xagg(binop("Add", binop("Sub", col_("u8"), lit(1), "U8"), col_("s32"), "I32"),
+                                              "Sum", "u8s", "I32"),
+                                         xagg(binop("Mul", col_("s32"), col_("f2"), "F64"), "Sum", "sf", "F64"),
+                                         xagg(binop("Mul", col_("i8"), col_("u16"), "I32"), "Max", "iu", "I32")]},
+               {"k": col("I64", [0, 1, 0, 1, 0, 1]), "u": col("U32", u), "s32": col("I32", s32),
+                "u8": col("U8", u8), "f2": col("F32", f2), "i8": col("I8", i8), "u16": col("U16", u16)}, 6))
+    # four binary levels (the reference's depth cap), nulls of w propagate
+    deep = binop("Div", binop("Sub", binop("Mul", binop("Add", col_("v"), lit(1.0), "F64"), lit(2), "F64"),
+                                       col_("w"), "F64"), lit(4), "F64")
+    cs.append(("expr_depth4", {"keys": [["k", "I64"]], "predicate": gt0("v"),
+                               "aggs": [xagg(deep, "Sum", "s", "F64"), xagg(deep, "Count", "c", "F64"),
+                                        xagg(deep, "Mean", "m", "F64"), agg("v", "Count")]},
+               {"k": col("I64", [1, 2, 1, 2, 1, 2, 1, 2]), "v": col("F64", [0.5, 1.5, -2.0, 3.25, 9.0, None, 0.125, 7.0]),
+                "w": col("I32", [1, None, 3, 4, -5, 6, 7, 8])}, 8))
     return cs
 
 
@@ -309,7 +467,19 @@ def main():
     for name, plan, cols, n in cases():
         exp, ng = expected(plan, cols, n)
         enc_cols = {k: {"tag": c["tag"], "values": [fenc(v) for v in c["values"]]} for k, c in cols.items()}
-        out.append({"name": name, "plan": plan, "n_rows": n, "columns": enc_cols, "expected": exp, "n_groups": ng})
+        expr_tags = {}
+        aggs = []
+        for a in plan["aggs"]:
+            if a.get("kind") == "Expression":
+                expr_tags[a["output_alias"]] = a["tag"]
+                a = {k: v for k, v in a.items() if k != "tag"}
+                a["expr"] = strip_ct(a["expr"])
+            aggs.append(a)
+        plan = dict(plan, aggs=aggs)
+        case = {"name": name, "plan": plan, "n_rows": n, "columns": enc_cols, "expected": exp, "n_groups": ng}
+        if expr_tags:
+            case["expr_tags"] = expr_tags
+        out.append(case)
     OUT.write_text(json.dumps(out, indent=1))
     print(f"wrote {len(out)} cases to {OUT}")
 

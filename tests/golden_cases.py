@@ -49,7 +49,8 @@ def expected_of(case, frame):
         if op in ("Len", "Count"):
             tag = "U32"
         else:
-            src = case["columns"][a["input_col"]]["tag"]
+            src = (case["expr_tags"][alias] if a.get("kind") == "Expression"
+                   else case["columns"][a["input_col"]]["tag"])
             tag = "F64" if op == "Mean" and src != "F32" else src
         arr = np.array([0 if v is None else v for v in vals], dtype=orc.NP_DTYPE[tag])
         out[alias] = (tag, arr, valid)

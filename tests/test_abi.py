@@ -66,6 +66,48 @@ def test_unsupported_agg_dtype_is_value_error():
         _native.execute_groupby(plan, 1, cols)
 
 
+def _xplan(expr, op="Sum"):
+    return {"keys": [["k", "I64"]], "aggs": [{"kind": "Expression", "op": op, "output_alias": "e", "expr": expr}]}
+
+
+def _col(n):
+    return {"kind": "Column", "name": n}
+
+
+def _bin(op, a, b):
+    return {"kind": "Binary", "op": op, "lhs": a, "rhs": b}
+
+
+XCOLS = COLS + [("i8", "I8", b"\x00", b"\x01"), ("u64", "U64", b"\x00" * 8, b"\x01"),
+                ("b", "Bool", b"\x00", b"\x01"), ("i32", "I32", b"\x00" * 4, b"\x01")]
+
+
+@pytest.mark.parametrize("expr,op,exc,msg", [
+    # the reference's parser: depth cap of four binary levels (plan/mod.rs:169-178), op names
+    (_bin("Add", _bin("Add", _bin("Add", _bin("Add", _bin("Add", _col("v"), {"kind": "LiteralF64", "value": 1.0}),
+                                                  {"kind": "LiteralF64", "value": 1.0}),
+                                      {"kind": "LiteralF64", "value": 1.0}),
+                          {"kind": "LiteralF64", "value": 1.0}), {"kind": "LiteralF64", "value": 1.0}),
+     "Sum", ValueError, "too deep"),
+    (_bin("Pow", _col("v"), _col("v")), "Sum", ValueError, "unknown agg op binary op Pow"),
+    ({"kind": "Sqrt"}, "Sum", ValueError, "expr.kind"),
+    ({"kind": "LiteralI64", "value": 1.5}, "Sum", ValueError, "expr.value(i64)"),
+    # dtype rules refused before any device work
+    (_bin("Add", _col("b"), _col("v")), "Sum", NotImplementedError, "Bool"),
+    (_bin("Add", _col("k"), _col("u64")), "Sum", NotImplementedError, "UInt64"),
+    (_bin("Add", _col("i8"), {"kind": "LiteralI64", "value": 1000}), "Sum", NotImplementedError, "does not fit I8"),
+    (_bin("Add", {"kind": "LiteralI64", "value": 1}, {"kind": "LiteralF64", "value": 2.0}), "Sum",
+     NotImplementedError, "literals only"),
+    # an I8 result is not a summable dtype (as a Simple agg over I8 is not)
+    (_bin("Mul", _col("i8"), {"kind": "LiteralI64", "value": 2}), "Sum", ValueError, "Sum / I8"),
+    (_col("nope"), "Sum", KeyError, "column not found"),
+])
+def test_expression_agg_errors(expr, op, exc, msg):
+    from polars_metal import _native
+    with pytest.raises(exc, match=re.escape(msg)):
+        _native.execute_groupby(_xplan(expr, op), 1, XCOLS)
+
+
 def test_legacy_len_without_kind():
     # groupby/mod.rs:420-434: a missing "kind" with op Len reads as Length;
     # the plan parses and fails only at the device (none here) or runs.

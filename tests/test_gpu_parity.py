@@ -65,8 +65,11 @@ def _conjunction(p):
 
 def _batchable(case):
     """The batch takes one key lane of at most 64 bits, at most two
-    aggregated columns and conjunctions of compares."""
+    aggregated columns (no expression aggregates) and conjunctions of
+    compares."""
     keys = case["plan"]["keys"]
+    if any(a.get("kind") == "Expression" for a in case["plan"]["aggs"]):
+        return False
     vals = {a["input_col"] for a in case["plan"]["aggs"] if a.get("op") != "Len"}
     return (len(keys) == 1 and keys[0][1] != "F32" and len(vals) <= 2 and case["n_rows"] > 0
             and _conjunction(case["plan"].get("predicate")))
@@ -584,3 +587,48 @@ def test_scan_signatures_hot_and_tail(native, sig, two_pred, key):
     assert st["scan_kernel"] == "specialised"
     assert st["hot_keys"] > 0 and st["tail_rows"] > 0
     assert_results_equal(got, want, ["k"], label=f"sig{sig}/{two_pred}/{key}")
+
+
+EXPR_ENGINES = [{"method": "hash"}, {"method": "hash", "fast_scan": 0}, {"method": "sort"},
+                {"method": "hash_global"}, {"method": "auto"}]
+
+
+@pytest.mark.parametrize("opts", EXPR_ENGINES, ids=[str(o) for o in EXPR_ENGINES])
+def test_expression_aggs_random(native, opts):
+    """Expression aggregates over mixed dtypes with nulls, under a filter,
+    on every engine: bit-exact against the oracle's evaluation."""
+    rng = np.random.default_rng(21)
+    n = 400_000
+    frame = {"key": ("I64", rng.integers(0, 5000, n), rng.random(n) < 0.98),
+             "a": ("F64", rng.standard_normal(n) * 100, rng.random(n) < 0.95),
+             "b": ("F64", rng.random(n), None),
+             "i": ("I32", rng.integers(-2**31, 2**31 - 1, n).astype(np.int32), rng.random(n) < 0.9),
+             "j": ("I32", rng.integers(-50000, 50000, n).astype(np.int32), None),
+             "f": ("F32", rng.standard_normal(n).astype(np.float32), rng.random(n) < 0.97),
+             "h": ("I16", rng.integers(-30000, 30000, n).astype(np.int16), None),
+             "u": ("U32", rng.integers(0, 2**32 - 1, n, dtype=np.uint64).astype(np.uint32), None)}
+    col = lambda x: {"kind": "Column", "name": x}  # noqa: E731
+    b = lambda op, x, y: {"kind": "Binary", "op": op, "lhs": x, "rhs": y}  # noqa: E731
+    exprs = {
+        "disc": b("Mul", col("a"), b("Sub", {"kind": "LiteralI64", "value": 1}, col("b"))),  # F64
+        "iwrap": b("Add", b("Mul", col("i"), col("j")), {"kind": "LiteralI64", "value": 7}),  # I32, wraps
+        "idiv": b("Div", col("i"), col("j")),                                                 # F64
+        "f32": b("Add", b("Mul", col("f"), {"kind": "LiteralF64", "value": 0.1}), col("h")),  # F32
+        "iu": b("Sub", col("i"), col("u")),                                                   # I64
+        "deep": b("Div", b("Sub", b("Mul", b("Add", col("a"), col("f")), col("j")), col("b")),
+                  {"kind": "LiteralI64", "value": 3}),                                        # F64, depth 4
+    }
+    aggs = []
+    for name, e in exprs.items():
+        for op in ("Sum", "Min", "Max", "Mean", "Count"):
+            aggs.append({"kind": "Expression", "op": op, "output_alias": f"{name}_{op.lower()}", "expr": e})
+    pred = {"kind": "Compare", "op": "Gt", "dtype": "F64", "lhs": {"kind": "Column", "name": "b", "dtype": "F64"},
+            "rhs": {"kind": "LiteralF64", "value": 0.3}}
+    want = orc.groupby({"keys": [["key", "I64"]], "aggs": aggs, "predicate": pred}, frame)
+    print(f"expression aggs {opts}", flush=True)
+    for lo in range(0, len(aggs), 10):  # at most eight aggregated columns per query
+        part = aggs[lo:lo + 10]
+        plan = {"keys": [["key", "I64"]], "aggs": part, "predicate": pred, "options": opts}
+        got = run_gpu(native, plan, frame)
+        sub = {k: v for k, v in want.items() if k == "key" or k.startswith("__") or k in {a["output_alias"] for a in part}}
+        assert_results_equal(got, sub, ["key"], label=f"expr/{opts}/{lo}")

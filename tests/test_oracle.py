@@ -337,3 +337,62 @@ def test_utf8_key_groupby_vs_python():
     gs, gk, gt = got["s"], got["k2"], got["t"]
     seen = {((gs[1][j] if gs[2][j] else None), int(gk[1][j])): int(gt[1][j]) for j in range(got["__n_groups__"])}
     assert seen == want
+
+
+# Polars' numeric supertypes for arithmetic, stated per pair (polars-core
+# get_supertype): the expression restatement must type each pair this way.
+SUPERTYPE_KATS = [
+    ("I32", "I64", "I64"), ("I8", "I16", "I16"), ("U8", "U32", "U32"), ("I8", "U8", "I16"),
+    ("I16", "U16", "I32"), ("I32", "U32", "I64"), ("I8", "U32", "I64"), ("I16", "U8", "I16"),
+    ("I64", "U32", "I64"), ("I8", "F32", "F32"), ("U16", "F32", "F32"), ("I32", "F32", "F64"),
+    ("U64", "F32", "F64"), ("I64", "F64", "F64"), ("F32", "F64", "F64"), ("F32", "F32", "F32"),
+]
+
+
+def _two_cols(a, b):
+    return {"x": (a, np.array([3], orc.NP_DTYPE[a]), None), "y": (b, np.array([2], orc.NP_DTYPE[b]), None)}
+
+
+@pytest.mark.parametrize("a,b,want", SUPERTYPE_KATS)
+def test_expression_supertypes(a, b, want):
+    add = {"kind": "Binary", "op": "Add", "lhs": {"kind": "Column", "name": "x"}, "rhs": {"kind": "Column", "name": "y"}}
+    for e in (add, dict(add, lhs=add["rhs"], rhs=add["lhs"])):
+        tag, vals, ok = orc.eval_expr(e, _two_cols(a, b))
+        assert tag == want and vals[0] == 5 and ok.all()
+
+
+def test_expression_division_and_literal_typing():
+    def ev(expr, frame):
+        return orc.eval_expr(expr, frame)
+
+    col = lambda n: {"kind": "Column", "name": n}  # noqa: E731
+    li = lambda v: {"kind": "LiteralI64", "value": v}  # noqa: E731
+    lf = lambda v: {"kind": "LiteralF64", "value": v}  # noqa: E731
+    b = lambda op, x, y: {"kind": "Binary", "op": op, "lhs": x, "rhs": y}  # noqa: E731
+    fr = {"i": ("I32", np.array([7, -7, 0], np.int32), None), "f": ("F32", np.array([1.5, 0.1, -2], np.float32), None),
+          "h": ("I16", np.array([3, 0, 1], np.int16), None), "u": ("U8", np.array([0, 1, 255], np.uint8), None)}
+    # true division of integers is Float64, with IEEE infinities
+    tag, v, _ = ev(b("Div", col("i"), col("h")), fr)
+    assert tag == "F64" and v[0] == 7 / 3 and v[1] == -math.inf and v[2] == 0.0
+    tag, v, _ = ev(b("Div", col("i"), li(2)), fr)
+    assert tag == "F64" and v[0] == 3.5
+    # F32 with I16 or with a literal stays F32, rounded once per operation
+    tag, v, _ = ev(b("Div", col("f"), col("h")), fr)
+    assert tag == "F32" and v[0] == np.float32(0.5)
+    tag, v, _ = ev(b("Mul", col("f"), lf(0.1)), fr)
+    assert tag == "F32" and v[1] == np.float32(np.float32(0.1) * np.float32(0.1))
+    # an integer literal takes the column's integer type and wraps in it
+    tag, v, _ = ev(b("Sub", col("u"), li(1)), fr)
+    assert tag == "U8" and list(v) == [255, 0, 254]
+    tag, v, _ = ev(b("Mul", col("i"), lf(0.5)), fr)
+    assert tag == "F64" and v[0] == 3.5
+    tag, v, _ = ev(b("Add", col("i"), b("Mul", li(2), li(3))), fr)  # literal subtree folded
+    assert tag == "I32" and v[0] == 13
+    with pytest.raises(NotImplementedError):
+        ev(b("Add", col("u"), li(300)), fr)
+    with pytest.raises(NotImplementedError):
+        ev(b("Add", col("i"), {"kind": "Column", "name": "q"}), dict(fr, q=("U64", np.zeros(3, np.uint64), None)))
+    # nulls of every column read propagate
+    fr2 = dict(fr, h=("I16", np.array([3, 0, 1], np.int16), np.array([True, False, True])))
+    _, _, ok = ev(b("Add", col("i"), b("Mul", col("h"), li(0))), fr2)
+    assert list(ok) == [True, False, True]

@@ -39,13 +39,36 @@ def test_resolve_upstream_shapes():
         udf.resolve_upstream({"kind": "HStack", "input": scan})
 
 
-def test_expression_aggs_go_to_the_reference_hook():
-    plan = {"kind": "GroupBy", "input": {"kind": "Scan"}, "keys": [["k", "I64"]],
-            "aggs": [{"kind": "Expression", "op": "Sum", "output_alias": "s",
-                      "expr": {"kind": "Binary", "op": "Mul", "lhs": {"kind": "Column", "name": "v"},
-                               "rhs": {"kind": "LiteralF64", "value": 2.0}}}]}
-    with pytest.raises(NotImplementedError):
-        udf.dispatch_groupby(pa.table({"k": [1], "v": [1.0]}), plan)
+DISC = {"kind": "Binary", "op": "Mul", "lhs": {"kind": "Column", "name": "v"},
+        "rhs": {"kind": "Binary", "op": "Sub", "lhs": {"kind": "LiteralI64", "value": 1},
+                "rhs": {"kind": "Column", "name": "w"}}}
+
+
+def test_expression_aggs_stage_the_columns_they_read():
+    # the AggExpr tree's columns are passed to the fused call, which
+    # evaluates the tree on the device (no temporary host columns)
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Expression", "op": "Sum", "output_alias": "s", "expr": DISC},
+                                             {"kind": "Simple", "input_col": "k", "op": "Count", "output_alias": "c"}]}
+    assert udf._needed_columns(plan, _cmp("Gt", "x", "F64", 0.0)) == ["k", "v", "w", "x"]
+
+
+@pytest.mark.gpu
+def test_dispatch_groupby_expression_aggs(native):
+    """sum(v * (1 - w)) and friends (the reference's Q1 shape,
+    tests/python_integration/test_walker_expression_unfolding.py:38-56)
+    under a pushed-down selection, against the oracle's evaluation."""
+    rng = np.random.default_rng(11)
+    n = 50_000
+    frame = _table(rng, n)
+    table = _pa(frame)
+    aggs = [{"kind": "Expression", "op": op, "output_alias": f"d_{op.lower()}", "expr": DISC}
+            for op in ("Sum", "Mean", "Min", "Max", "Count")] + AGGS[:2]
+    sel = _cmp("Gt", "v", "F64", -0.5)
+    plan = {"kind": "GroupBy", "input": {"kind": "Scan", "selection": sel}, "keys": [["k", "I64"]], "aggs": aggs}
+    got = _from_table(udf.dispatch_groupby(table, plan))
+    want = orc.groupby({"keys": [["k", "I64"]], "aggs": aggs, "predicate": sel}, frame)
+    assert got["d_sum"][0] == "F64"
+    assert_results_equal(got, want, ["k"], label="expr")
 
 
 def test_install_rebinds_the_reference_hook():

@@ -51,9 +51,11 @@ def run_plain(jobs: Sequence[tuple], consume: Callable | None = None) -> None:
 # A single call carries a fixed cost of roughly 0.15 ms (its sample sketch,
 # about ten launches, two host synchronisations) and then streams at about
 # 11 us per million rows; the batch streams at about 44 us per million rows
-# but shares one sketch launch and one graph among all its queries.  Jobs
-# above this many rows measured faster alone on a C4-distribution stream.
-BATCH_MAX_ROWS = 1 << 18
+# but shares one sketch launch and one graph among all its queries.  On 120
+# jobs of the C4 distribution (rows log-uniform in [1e3, 1e7]) the stream
+# measured 1.16x over one plain call per job with this limit at 2^18 rows,
+# 1.25x at 2^19, 1.13x at 2^20 and 1.09x at 2^21.
+BATCH_MAX_ROWS = 1 << 19
 
 
 def run(jobs: Sequence[tuple], consume: Callable | None = None, min_batch: int = 2, tune: bool = True,

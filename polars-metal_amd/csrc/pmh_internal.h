@@ -217,6 +217,8 @@ struct Options {
     int64_t scan_rows = 4;        // specialised scan: rows per lane per step (4 or 8)
     int64_t scan_blocks = 1;      // partitioned scan: resident blocks per CU (the hot table shares LDS)
     int64_t narrow_keys = 1;      // partitioned engine: 1 narrow a lane wider than 64 bits to its key ranges, 0 never
+    int64_t sort_buckets = 1;     // sort engine: 1 radix passes over the key's top bits, then LDS-sorted buckets
+                                  // where that saves passes; 0 radix passes over the whole key
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 

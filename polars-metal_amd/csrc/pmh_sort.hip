@@ -53,7 +53,16 @@ struct Run {
     uint32_t pos, len, at, pad;
 };
 
-enum SortCtr : int32_t { SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, SC_CURSOR, SC_GROUPS, SC_N = 8 };
+enum SortCtr : int32_t { SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, SC_CURSOR, SC_GROUPS, SC_BIG, SC_N = 8 };
+
+// Bucket finish (MSD over the top bits, then LDS sorts): a bucket of at most
+// kBucketCap records is sorted in one block's LDS; the radix passes first
+// order the records by the key's top tbits bits, which split them into
+// buckets of about kBucketMean records.
+constexpr uint32_t kBucketCap = 4096;   // keys of one word (2048 for two-word keys)
+constexpr uint32_t kBucketMean = 3072;  // (1536 for two-word keys)
+__host__ __device__ constexpr uint32_t bucket_cap(bool two) { return two ? kBucketCap / 2 : kBucketCap; }
+constexpr int kBucketThreads = 1024;
 
 struct SLaunch {
     QueryDesc q;
@@ -232,6 +241,10 @@ __global__ __launch_bounds__(1024) void so_bits_final(const SLaunch *__restrict_
         L->ctr[SC_AND_HI] = s[3];
         L->ctr[SC_SEL] = carry;
     }
+}
+
+__global__ void so_reset_big(const SLaunch *__restrict__ L) {
+    if (threadIdx.x == 0) L->ctr[SC_BIG] = 0;
 }
 
 __global__ void so_reset(const SLaunch *__restrict__ L) {
@@ -673,6 +686,245 @@ __global__ void so_init_state(const SLaunch *__restrict__ L, uint64_t ng) {
             if (mo_size(op) == 4) gst<uint32_t>(L->arr[a], i, (uint32_t)mo_identity(op));
             else gst<uint64_t>(L->arr[a], i, mo_identity(op));
         }
+}
+
+// ---------------------------------------------------------------------------
+// 3b. bucket finish: after the passes over the top tbits key bits the records
+// are grouped by those bits in order (buckets); each bucket is sorted on the
+// whole key in LDS and written back, which completes the sort.
+// ---------------------------------------------------------------------------
+// The key's bits [shift, shift + 64) as a 64-bit value.
+__device__ __forceinline__ uint64_t key_bits_from(uint64_t lo, uint64_t hi, int shift) {
+    return shift == 0 ? lo : shift < 64 ? (lo >> shift) | (hi << (64 - shift)) : hi >> (shift - 64);
+}
+
+// Start of every bucket: bstart[b] = first record whose top bits are >= b,
+// by a binary search per bucket over the records, which the passes left in
+// bucket order (bstart[nb] = n_sel).
+__global__ __launch_bounds__(256) void so_bucket_bounds(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                      const uint64_t *khi, uint64_t *bstart, uint32_t nb,
+                                                      int shift) {
+    const bool two = L->kw == 2;
+    const uint64_t n = L->n_sel;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b <= nb; b += gridDim.x * blockDim.x) {
+        uint64_t lo = 0, hi = b == nb ? n : 0;
+        if (b < nb) {
+            hi = n;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) / 2;
+                const uint64_t top = key_bits_from(gld<uint64_t>(klo, mid), two ? gld<uint64_t>(khi, mid) : 0, shift);
+                if (top < b) lo = mid + 1;
+                else hi = mid;
+            }
+        }
+        bstart[b] = b == nb ? n : lo;
+    }
+}
+
+// Skew probe before the top-bit passes: the largest count of one bucket
+// among up to kProbeSample evenly spaced records (any order: the records
+// are still in row order) into ctr[SC_BIG].  A bucket far above the LDS
+// capacity shows as a large count; buckets near it may not (they still
+// fall back after the bucket sort flags them).
+constexpr uint32_t kProbeSample = 8192;
+__global__ __launch_bounds__(1024) void so_bucket_probe(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                      const uint64_t *khi, int shift) {
+    constexpr uint32_t kSlots = 2 * kProbeSample;
+    __shared__ uint32_t s_id[kSlots], s_cnt[kSlots];
+    __shared__ uint32_t s_max;
+    const bool two = L->kw == 2;
+    const uint64_t n = L->n_sel;
+    const uint32_t S = (uint32_t)min<uint64_t>(n, kProbeSample);
+    for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) { s_id[i] = 0; s_cnt[i] = 0; }
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    uint32_t mx = 0;
+    for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
+        const uint64_t r = (uint64_t)i * (n / S) + ((uint64_t)i * (n % S)) / S;
+        const uint64_t top = key_bits_from(gld<uint64_t>(klo, r), two ? gld<uint64_t>(khi, r) : 0, shift);
+        const uint32_t id = (uint32_t)top + 1u;  // buckets are fewer than 2^32 - 1
+        uint32_t h = (uint32_t)(((uint64_t)id * 0x9E3779B97F4A7C15ull) >> 49) & (kSlots - 1);
+        for (uint32_t p = 0; p < kSlots; ++p) {  // at most S < kSlots ids: a free slot exists
+            const uint32_t cur = atomicCAS(&s_id[h], 0u, id);
+            if (cur == 0u || cur == id) {
+                mx = max(mx, atomicAdd(&s_cnt[h], 1u) + 1u);
+                break;
+            }
+            h = (h + 1) & (kSlots - 1);
+        }
+    }
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) L->ctr[SC_BIG] = s_max;
+}
+
+__device__ __forceinline__ bool rec_less(uint64_t ahi, uint64_t alo, uint32_t arow, uint64_t bhi, uint64_t blo,
+                                         uint32_t brow) {
+    if (ahi != bhi) return ahi < bhi;
+    if (alo != blo) return alo < blo;
+    return arow < brow;
+}
+
+// Bitonic sort of m (a power of two) records in LDS on (key, row).
+template <bool TWO>
+__device__ __forceinline__ void lds_bitonic(uint64_t *s_lo, uint64_t *s_hi, uint32_t *s_row, uint32_t m) {
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < m / 2; t += blockDim.x) {
+                const uint32_t a = 2 * j * (t / j) + (t % j), c = a + j;
+                const bool up = (a & k) == 0;
+                const uint64_t alo = s_lo[a], clo = s_lo[c];
+                const uint64_t ahi = TWO ? s_hi[a] : 0, chi = TWO ? s_hi[c] : 0;
+                const uint32_t arow = s_row[a], crow = s_row[c];
+                if (rec_less(chi, clo, crow, ahi, alo, arow) == up) {
+                    s_lo[a] = clo;
+                    s_lo[c] = alo;
+                    if (TWO) { s_hi[a] = chi; s_hi[c] = ahi; }
+                    s_row[a] = crow;
+                    s_row[c] = arow;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// One bucket per block iteration (its records share the key's top bits):
+// a counting sort on the next kBinBits bits places the records by bin in a
+// second LDS buffer, then one thread per bin orders its few records by an
+// insertion sort (equal keys always share a bin).  A bin of more than
+// kBinSortMax records sends the bucket to a bitonic sort instead.  The
+// sorted bucket is written back.  A bucket larger than kBucketCap is left as
+// it is and flags SC_BIG: the host then sorts with whole-key LSD passes.
+constexpr int kBinBits = 12;
+constexpr uint32_t kBins = 1u << kBinBits;
+constexpr uint32_t kBinSortMax = 64;
+template <bool TWO>
+__global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *__restrict__ L, uint64_t *klo,
+                                                               uint64_t *khi, uint32_t *krow,
+                                                               const uint64_t *bstart, uint32_t nb, int bin_shift,
+                                                               int bin_bits) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr uint32_t cap = bucket_cap(TWO);
+    // two record buffers (keys, [hi], rows), then the bin counters
+    uint64_t *a_lo = (uint64_t *)smem;
+    uint64_t *a_hi = a_lo + cap;
+    uint32_t *a_row = (uint32_t *)(TWO ? a_hi + cap : a_hi);
+    uint64_t *b_lo = (uint64_t *)(a_row + cap);
+    uint64_t *b_hi = b_lo + cap;
+    uint32_t *b_row = (uint32_t *)(TWO ? b_hi + cap : b_hi);
+    uint32_t *bin = b_row + cap;  // kBins: counts, then starts
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_crowded;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbins = 1u << bin_bits;
+    const uint64_t bmask = nbins - 1;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint64_t beg = bstart[b], end = bstart[b + 1];
+        const uint64_t n64 = end - beg;  // uniform across the block
+        if (n64 <= 1) continue;
+        if (n64 > cap) {
+            if (tid == 0) atomicAdd((unsigned long long *)&L->ctr[SC_BIG], 1ull);
+            continue;
+        }
+        const uint32_t n = (uint32_t)n64;
+        for (uint32_t i = tid; i < nbins; i += kBucketThreads) bin[i] = 0;
+        if (tid == 0) s_crowded = 0;
+        __syncthreads();
+        // load and count; a record's rank in its bin comes from the count
+        constexpr uint32_t kPer = cap / kBucketThreads;
+        uint32_t rk[kPer], bn[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t i = k * kBucketThreads + tid;
+            rk[k] = ~0u;
+            bn[k] = 0;
+            if (i < n) {
+                const uint64_t lo = gld_nt<uint64_t>(klo, beg + i);
+                const uint64_t hi = TWO ? gld_nt<uint64_t>(khi, beg + i) : 0;
+                a_lo[i] = lo;
+                if (TWO) a_hi[i] = hi;
+                a_row[i] = gld_nt<uint32_t>(krow, beg + i);
+                bn[k] = (uint32_t)(key_bits_from(lo, hi, bin_shift) & bmask);
+                rk[k] = atomicAdd(&bin[bn[k]], 1u);
+            }
+        }
+        __syncthreads();
+        // bin starts: each thread scans its consecutive run of bins
+        {
+            constexpr uint32_t kRun = kBins / kBucketThreads;
+            uint32_t c[kRun], tot = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kRun; ++r) {
+                const uint32_t i = tid * kRun + r;
+                c[r] = i < nbins ? bin[i] : 0;
+                tot += c[r];
+                if (c[r] > kBinSortMax) s_crowded = 1;
+            }
+            uint32_t base = block_excl_scan(tot, s_wave);
+#pragma unroll
+            for (uint32_t r = 0; r < kRun; ++r) {
+                const uint32_t i = tid * kRun + r;
+                if (i < nbins) bin[i] = base;
+                base += c[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t i = k * kBucketThreads + tid;
+            if (rk[k] == ~0u) continue;
+            const uint32_t d = bin[bn[k]] + rk[k];
+            b_lo[d] = a_lo[i];
+            if (TWO) b_hi[d] = a_hi[i];
+            b_row[d] = a_row[i];
+        }
+        __syncthreads();
+        if (s_crowded) {
+            // a crowded bin: sort the whole bucket (padded) instead
+            uint32_t m = 2;
+            while (m < n) m <<= 1;
+            for (uint32_t i = n + tid; i < m; i += kBucketThreads) {
+                b_lo[i] = ~0ull;
+                if (TWO) b_hi[i] = ~0ull;
+                b_row[i] = ~0u;
+            }
+            __syncthreads();
+            lds_bitonic<TWO>(b_lo, b_hi, b_row, m);
+        } else {
+            // inside each bin, a record's place is the number of the bin's
+            // records that order before it (keys, then rows: all distinct),
+            // counted by every record at once; the bucket goes back to A
+#pragma unroll
+            for (uint32_t k = 0; k < kPer; ++k) {
+                if (rk[k] == ~0u) continue;
+                const uint32_t s0 = bin[bn[k]], s1 = bn[k] + 1 < nbins ? bin[bn[k] + 1] : n;
+                const uint32_t me = s0 + rk[k];
+                const uint64_t lo = b_lo[me], hi = TWO ? b_hi[me] : 0;
+                const uint32_t row = b_row[me];
+                uint32_t r = 0;
+                for (uint32_t x = s0; x < s1; ++x)
+                    r += rec_less(TWO ? b_hi[x] : 0, b_lo[x], b_row[x], hi, lo, row) ? 1u : 0u;
+                a_lo[s0 + r] = lo;
+                if (TWO) a_hi[s0 + r] = hi;
+                a_row[s0 + r] = row;
+            }
+            __syncthreads();
+            for (uint32_t i = tid; i < n; i += kBucketThreads) {
+                gst<uint64_t>(klo, beg + i, a_lo[i]);
+                if (TWO) gst<uint64_t>(khi, beg + i, a_hi[i]);
+                gst<uint32_t>(krow, beg + i, a_row[i]);
+            }
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = tid; i < n; i += kBucketThreads) {
+            gst<uint64_t>(klo, beg + i, b_lo[i]);
+            if (TWO) gst<uint64_t>(khi, beg + i, b_hi[i]);
+            gst<uint32_t>(krow, beg + i, b_row[i]);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1210,7 +1462,27 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     L.const_hi = c[SC_AND_HI] & ~vhi;
     uint32_t width = 0;
     for (int i = 0; i < L.n_runs; ++i) width += L.runs[i].len;
-    const int passes = (int)((width + kDigitBits - 1) / kDigitBits);
+    const int full_passes = (int)((width + kDigitBits - 1) / kDigitBits);
+    // MSD finish: when the records split into buckets of about kBucketMean
+    // by their top tbits bits with fewer passes than the whole key needs,
+    // the passes order only those bits and LDS sorts finish each bucket
+    int tbits = 0;
+    const uint64_t bmean = L.kw == 2 ? kBucketMean / 2 : kBucketMean;
+    if (n_sel >= 2 * bmean && plan.opts.sort_buckets != 0) {
+        int t = 1;
+        while (t < 40 && (n_sel >> t) > bmean) ++t;
+        t = std::max(t, kDigitBits);
+        if ((t + kDigitBits - 1) / kDigitBits < full_passes) tbits = t;
+    }
+    std::vector<int> shifts;
+    if (tbits) {
+        const int np = (tbits + kDigitBits - 1) / kDigitBits;
+        for (int p = 0; p < np; ++p)
+            shifts.push_back(std::min<int>((int)width - tbits + p * kDigitBits, (int)width - kDigitBits));
+    } else {
+        for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
+    }
+    int passes = (int)shifts.size();
 
     // ---- buffers
     const uint64_t cap = std::max<uint64_t>(n_sel, 1);
@@ -1243,36 +1515,98 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         const size_t dyn = L.kw == 2 ? (size_t)kSortTile * 8 : 0;
         PMH_HIP(hipFuncSetAttribute((const void *)so_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)std::max<size_t>(dyn, 16)));
-        for (int ps = 0; ps < passes; ++ps) {
-            const int shift = ps * kDigitBits;
-            {
-                KernelTimer kt("so_hist");
-                hipLaunchKernelGGL(so_hist, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, shift);
-                launch_check();
+        auto run_passes = [&](const std::vector<int> &sh) {
+            for (const int shift : sh) {
+                {
+                    KernelTimer kt("so_hist");
+                    hipLaunchKernelGGL(so_hist, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, shift);
+                    launch_check();
+                }
+                {
+                    KernelTimer kt("so_scan");
+                    hipLaunchKernelGGL(so_scan, dim3(kDigits), dim3(1024), 0, R.stream, dL);
+                    launch_check();
+                }
+                {
+                    KernelTimer kt("so_scatter");
+                    hipLaunchKernelGGL(so_scatter, dim3(pgrid), dim3(kSortThreads), std::max<size_t>(dyn, 16),
+                                       R.stream, dL, slo, shi, srow, dlo, dhi, drow, shift);
+                    launch_check();
+                }
+                std::swap(slo, dlo);
+                std::swap(shi, dhi);
+                std::swap(srow, drow);
             }
-            {
-                KernelTimer kt("so_scan");
-                hipLaunchKernelGGL(so_scan, dim3(kDigits), dim3(1024), 0, R.stream, dL);
-                launch_check();
+        };
+        if (tbits) {
+            // skewed keys would overflow the buckets: probe a sample first
+            hipLaunchKernelGGL(so_bucket_probe, dim3(1), dim3(1024), 0, R.stream, dL, slo, shi, (int)width - tbits);
+            launch_check();
+            uint64_t pc[SC_N];
+            PMH_HIP(hipMemcpyAsync(pc, L.ctr, sizeof(pc), hipMemcpyDeviceToHost, R.stream));
+            PMH_HIP(hipStreamSynchronize(R.stream));
+            const uint64_t S = std::min<uint64_t>(n_sel, kProbeSample);
+            const double expect = (double)S * bucket_cap(L.kw == 2) / (double)n_sel;  // sampled rows of a full bucket
+            if ((double)pc[SC_BIG] > std::max(16.0, 4.0 * expect)) {
+                tbits = 0;
+                shifts.clear();
+                for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
+                passes = full_passes;
             }
-            {
-                KernelTimer kt("so_scatter");
-                hipLaunchKernelGGL(so_scatter, dim3(pgrid), dim3(kSortThreads), std::max<size_t>(dyn, 16), R.stream,
-                                   dL, slo, shi, srow, dlo, dhi, drow, shift);
-                launch_check();
-            }
-            std::swap(slo, dlo);
-            std::swap(shi, dhi);
-            std::swap(srow, drow);
-        }
-        // ---- 4. group starts
-        {
-            KernelTimer kt("so_bound");
-            hipLaunchKernelGGL(so_bound, dim3(pgrid), dim3(256), 0, R.stream, dL, slo, shi);
+            hipLaunchKernelGGL(so_reset_big, dim3(1), dim3(64), 0, R.stream, dL);
             launch_check();
         }
-        hipLaunchKernelGGL(so_bscan, dim3(1), dim3(1024), 0, R.stream, dL);
-        launch_check();
+        run_passes(shifts);
+        if (tbits) {
+            // ---- 3b. bucket finish
+            const uint32_t nb = 1u << tbits;
+            uint64_t *bstart = (uint64_t *)slab("so:bstart", ((uint64_t)nb + 1) * 8 + 64);
+            {
+                KernelTimer kt("so_bucket_bounds");
+                hipLaunchKernelGGL(so_bucket_bounds, dim3((uint32_t)ceil_div((uint64_t)nb + 1, 256)), dim3(256), 0,
+                                   R.stream, dL, slo, shi, bstart, nb, (int)width - tbits);
+                launch_check();
+            }
+            {
+                KernelTimer kt("so_bucket_sort");
+                const bool two = L.kw == 2;
+                const size_t lds = (size_t)bucket_cap(two) * 2 * (two ? 20 : 12) + (size_t)kBins * 4;
+                const void *fn = two ? (const void *)so_bucket_sort<true> : (const void *)so_bucket_sort<false>;
+                PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+                const uint32_t grid = (uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * (lds * 2 <= 160 * 1024 ? 2 : 1));
+                // the bins: the kBinBits key bits below the bucket's top bits
+                // (fewer when the key has fewer bits left)
+                int bin_bits = std::min<int>(kBinBits, (int)width - tbits);
+                int bin_shift = (int)width - tbits - bin_bits;
+                void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&srow, (void *)&bstart,
+                                (void *)const_cast<uint32_t *>(&nb), (void *)&bin_shift, (void *)&bin_bits};
+                PMH_HIP(hipLaunchKernel(fn, dim3(grid), dim3(kBucketThreads), args, lds, R.stream));
+                launch_check();
+            }
+        }
+        // ---- 4. group starts
+        auto run_bounds = [&]() {
+            {
+                KernelTimer kt("so_bound");
+                hipLaunchKernelGGL(so_bound, dim3(pgrid), dim3(256), 0, R.stream, dL, slo, shi);
+                launch_check();
+            }
+            hipLaunchKernelGGL(so_bscan, dim3(1), dim3(1024), 0, R.stream, dL);
+            launch_check();
+        };
+        run_bounds();
+        PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
+        PMH_HIP(hipStreamSynchronize(R.stream));
+        if (c[SC_BIG]) {
+            // a bucket outgrew the LDS sort (keys concentrated in a few top
+            // bit patterns): the whole key by LSD passes after all
+            std::vector<int> all;
+            for (int p = 0; p < full_passes; ++p) all.push_back(p * kDigitBits);
+            run_passes(all);
+            passes += full_passes;
+            tbits = 0;
+            run_bounds();
+        }
     }
     PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
     PMH_HIP(hipStreamSynchronize(R.stream));

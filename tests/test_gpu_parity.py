@@ -632,3 +632,51 @@ def test_expression_aggs_random(native, opts):
         got = run_gpu(native, plan, frame)
         sub = {k: v for k, v in want.items() if k == "key" or k.startswith("__") or k in {a["output_alias"] for a in part}}
         assert_results_equal(got, sub, ["key"], label=f"expr/{opts}/{lo}")
+
+
+SORT_BUCKET_CASES = ["highcard", "wide", "zipf_skew", "i32_small"]
+
+
+@pytest.mark.parametrize("case", SORT_BUCKET_CASES)
+def test_sort_bucket_finish(native, case):
+    """The sort engine's MSD finish (radix passes over the key's top bits,
+    then each bucket sorted in LDS) against the oracle and against whole-key
+    LSD passes; skewed keys overflow a bucket and fall back to the LSD
+    passes."""
+    rng = np.random.default_rng(SORT_BUCKET_CASES.index(case) + 40)
+    if case == "highcard":
+        frame, plan = synth.highcard(1_500_000, seed=41, pool=700_000)
+        keys = ["a", "b"]
+    elif case == "wide":  # two random full-range halves: a compressed key of more than 64 bits
+        n = 600_000
+        frame = {"a": ("I64", rng.integers(-2**62, 2**62, n), rng.random(n) < 0.97),
+                 "b": ("I32", rng.integers(-2**31, 2**31 - 1, n).astype(np.int32), None),
+                 "v": ("F64", rng.standard_normal(n), rng.random(n) < 0.9)}
+        plan = {"keys": [["a", "I64"], ["b", "I32"]], "aggs": [
+            {"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in ("Sum", "Count", "Min", "Max")]}
+        keys = ["a", "b"]
+    elif case == "zipf_skew":
+        frame, plan = synth.zipf(1_000_000, seed=42)
+        keys = ["key"]
+    else:
+        n = 400_000
+        # no null keys: the null group (one key, 5% of the rows) would fill
+        # one bucket past the LDS sort, and the skew probe keeps whole-key passes
+        frame = {"k": ("I32", rng.integers(-(1 << 30), 1 << 30, n).astype(np.int32), None),
+                 "v": ("I64", rng.integers(-1 << 40, 1 << 40, n), None)}
+        plan = {"keys": [["k", "I32"]], "aggs": [
+            {"kind": "Simple", "input_col": "v", "op": op, "output_alias": op} for op in ("Sum", "Min", "Mean")]}
+        keys = ["k"]
+    want = orc.groupby(plan, frame)
+    print(f"sort bucket finish {case}", flush=True)
+    got = run_gpu(native, dict(plan, options={"method": "sort"}), frame)
+    st_b = native.last_stats()
+    got_lsd = run_gpu(native, dict(plan, options={"method": "sort", "sort_buckets": 0}), frame)
+    st_l = native.last_stats()
+    assert st_b["method"] == st_l["method"] == "sort"
+    if case != "zipf_skew":
+        assert st_b["n_passes"] < st_l["n_passes"], (st_b, st_l)  # fewer radix passes with the bucket finish
+    else:
+        assert st_b["n_passes"] == st_l["n_passes"]  # the skew probe keeps hot buckets out: whole-key passes
+    assert_results_equal(got, want, keys, label=f"sortb/{case}")
+    assert_results_equal(got_lsd, want, keys, label=f"sortl/{case}")

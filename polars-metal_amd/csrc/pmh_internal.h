@@ -219,6 +219,8 @@ struct Options {
     int64_t narrow_keys = 1;      // partitioned engine: 1 narrow a lane wider than 64 bits to its key ranges, 0 never
     int64_t sort_buckets = 1;     // sort engine: 1 radix passes over the key's top bits, then LDS-sorted buckets
                                   // where that saves passes; 0 radix passes over the whole key
+    int64_t sort_carry = 1;       // sort engine: 1 records carry the aggregated values (no gather through
+                                  // the row in the fold) where the fast scan encodes them; 0 rows
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 

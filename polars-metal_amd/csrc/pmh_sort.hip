@@ -75,6 +75,12 @@ struct SLaunch {
     uint32_t ntiles, pad2;
     uint64_t *ka_lo, *ka_hi, *kb_lo, *kb_hi;
     uint32_t *ra, *rb;
+    // carry mode: the records hold the value bits of the (at most two)
+    // aggregated columns and their validity instead of the row index, so
+    // the fold streams them instead of gathering through the row
+    int32_t carry, pad5;
+    uint64_t *va[2], *vb[2];  // value bits, ping and pong
+    uint8_t *fa, *fb;         // bit j: value j valid
     uint32_t *tile_hist;  // [kDigits][ntiles]
     uint32_t *digit_tot;  // [kDigits]
     uint32_t *tile_cnt;   // [ntiles + 1]: group starts per tile, then exclusive prefix
@@ -290,6 +296,11 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
         kdata[i] = i < nk ? q->keys[i].data_off : 0;
     }
     const bool two = L->kw == 2;
+    const bool carry = ENCODE && L->carry;
+    const int nv = carry ? q->n_vals : 0;
+    RoleCol vc[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) vc[j] = role_col(q, q->vals[j < q->n_vals ? j : 0].col);
     const int lane = threadIdx.x & 63;
     const uint64_t n = q->n_rows;
     const uint64_t per = (uint64_t)gridDim.x * blockDim.x * U;
@@ -306,7 +317,18 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
             sel |= (uint32_t)(row[u] < n) << u;
             row[u] = row[u] < n ? row[u] : n - 1;
         }
-        uint64_t pw[2][U], pv[2][U], kw[4][U], kv[4][U];
+        uint64_t pw[2][U], pv[2][U], kw[4][U], kv[4][U], vw[2][U], vv[2][U];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) vw[j][u] = vv[j][u] = 0;
+            if (j >= nv) continue;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                vw[j][u] = word_at(vc[j].base, vpos(vc[j], row[u]));
+                vv[j][u] = word_at(vc[j].vbase, bpos(vc[j], row[u]));
+            }
+        }
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             if (r >= nr) continue;
@@ -387,7 +409,18 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
                 const uint64_t pos = b + (uint64_t)__popcll(m & ((1ull << lane) - 1));
                 gst<uint64_t>(L->ka_lo, pos, clo);
                 if (two) gst<uint64_t>(L->ka_hi, pos, chi);
-                gst<uint32_t>(L->ra, pos, (uint32_t)row[u]);
+                if (carry) {
+                    uint32_t f = 0;
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        if (j >= nv) continue;
+                        gst<uint64_t>(L->va[j], pos, cut(vw[j][u], vpos(vc[j], row[u]), vc[j].mask));
+                        f |= (uint32_t)cut(vv[j][u], bpos(vc[j], row[u]), 1) << j;
+                    }
+                    gst<uint8_t>(L->fa, pos, (uint8_t)f);
+                } else {
+                    gst<uint32_t>(L->ra, pos, (uint32_t)row[u]);
+                }
             }
         }
     }
@@ -515,9 +548,19 @@ __device__ __forceinline__ uint32_t block_excl_digits(uint32_t x, uint32_t *tmp)
     return r;
 }
 
+// The payload a record carries beside its key: the row index, or (carry
+// mode) the value bits of up to two columns and their validity bits.
+struct Payload {
+    uint32_t *row;
+    uint64_t *v[2];
+    uint8_t *f;
+};
+
+// CARRY: the records carry values (Payload v, f) instead of rows.
+template <bool CARRY>
 __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__restrict__ L, const uint64_t *slo,
-                                                         const uint64_t *shi, const uint32_t *srow, uint64_t *dlo,
-                                                         uint64_t *dhi, uint32_t *drow, int shift) {
+                                                         const uint64_t *shi, const Payload sp_, uint64_t *dlo,
+                                                         uint64_t *dhi, const Payload dp_, int shift) {
     __shared__ uint32_t wcnt[kWaves][kDigits];
     __shared__ uint32_t run[kDigits];     // rows of each digit ranked so far in this tile
     __shared__ uint32_t dstart[kDigits];  // LDS start of each digit run
@@ -525,11 +568,19 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
     __shared__ uint32_t dbase[kDigits];   // global start of each digit
     __shared__ uint32_t tmp[kDigits];
     __shared__ uint64_t st_lo[kSortTile];
-    __shared__ uint32_t st_row[kSortTile];
     __shared__ uint8_t st_d[kSortTile];
+    // dynamic: [hi keys when kw == 2], then rows, or values and validity
     extern __shared__ __align__(16) uint8_t smem[];
-    uint64_t *st_hi = (uint64_t *)smem;  // kSortTile entries when kw == 2
     const bool two = L->kw == 2;
+    const int nv = CARRY ? L->q.n_vals : 0;
+    uint64_t *st_hi = (uint64_t *)smem;
+    uint8_t *dyn = smem + (two ? (size_t)kSortTile * 8 : 0);
+    uint32_t *st_row = (uint32_t *)dyn;
+    uint64_t *st_v0 = (uint64_t *)dyn;
+    uint64_t *st_v1 = st_v0 + kSortTile;
+    uint8_t *st_f = (uint8_t *)(st_v0 + (size_t)kSortTile * (nv > 1 ? 2 : 1));
+    const uint32_t *srow = sp_.row;
+    uint32_t *drow = dp_.row;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1;
     {
@@ -546,6 +597,8 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
         uint32_t my_d[kSortRows], my_pos[kSortRows];
         uint64_t my_lo[kSortRows], my_hi[kSortRows];
         uint32_t my_row[kSortRows];
+        uint64_t my_v0[kSortRows], my_v1[kSortRows];
+        uint8_t my_f[kSortRows];
         // every record of the tile loads before the first rank (one wait on
         // memory per tile), non-temporally: each is read once, and the
         // tile's runs are still being merged in L2
@@ -555,7 +608,16 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
             const uint64_t src = b + (i < rows ? i : 0);  // rows >= 1
             my_lo[k] = gld_nt<uint64_t>(slo, src);
             my_hi[k] = two ? gld_nt<uint64_t>(shi, src) : 0;
-            my_row[k] = gld_nt<uint32_t>(srow, src);
+            if (CARRY) {
+                my_row[k] = 0;
+                my_v0[k] = gld_nt<uint64_t>(sp_.v[0], src);
+                my_v1[k] = nv > 1 ? gld_nt<uint64_t>(sp_.v[1], src) : 0;
+                my_f[k] = gld_nt<uint8_t>(sp_.f, src);
+            } else {
+                my_row[k] = gld_nt<uint32_t>(srow, src);
+                my_v0[k] = my_v1[k] = 0;
+                my_f[k] = 0;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kSortRows; ++k) {
@@ -604,7 +666,13 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
             const uint32_t sp = dstart[my_d[k]] + my_pos[k];
             st_lo[sp] = my_lo[k];
             if (two) st_hi[sp] = my_hi[k];
-            st_row[sp] = my_row[k];
+            if (CARRY) {
+                st_v0[sp] = my_v0[k];
+                if (nv > 1) st_v1[sp] = my_v1[k];
+                st_f[sp] = my_f[k];
+            } else {
+                st_row[sp] = my_row[k];
+            }
             st_d[sp] = (uint8_t)my_d[k];
         }
         __syncthreads();
@@ -613,7 +681,13 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
             const uint64_t g = (uint64_t)gbase[d] + (sp - dstart[d]);
             gst<uint64_t>(dlo, g, st_lo[sp]);
             if (two) gst<uint64_t>(dhi, g, st_hi[sp]);
-            gst<uint32_t>(drow, g, st_row[sp]);
+            if (CARRY) {
+                gst<uint64_t>(dp_.v[0], g, st_v0[sp]);
+                if (nv > 1) gst<uint64_t>(dp_.v[1], g, st_v1[sp]);
+                gst<uint8_t>(dp_.f, g, st_f[sp]);
+            } else {
+                gst<uint32_t>(drow, g, st_row[sp]);
+            }
         }
         __syncthreads();
     }
@@ -765,6 +839,28 @@ __device__ __forceinline__ bool rec_less(uint64_t ahi, uint64_t alo, uint32_t ar
     return arow < brow;
 }
 
+// A sorted bucket out to the destination buffer: the keys, and the row or
+// (CARRY) the values and validity found at the record's place in the
+// source bucket.
+template <bool TWO, bool CARRY>
+__device__ __forceinline__ void bucket_out(const SLaunch *__restrict__ L, const Payload &sp_, uint64_t *dlo,
+                                           uint64_t *dhi, const Payload &dp_, uint64_t beg, uint32_t n,
+                                           const uint64_t *s_lo, const uint64_t *s_hi, const uint32_t *s_row) {
+    const int nv = CARRY ? L->q.n_vals : 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBucketThreads) {
+        gst<uint64_t>(dlo, beg + i, s_lo[i]);
+        if (TWO) gst<uint64_t>(dhi, beg + i, s_hi[i]);
+        if (CARRY) {
+            const uint64_t from = beg + s_row[i];
+            gst<uint64_t>(dp_.v[0], beg + i, gld<uint64_t>(sp_.v[0], from));
+            if (nv > 1) gst<uint64_t>(dp_.v[1], beg + i, gld<uint64_t>(sp_.v[1], from));
+            gst<uint8_t>(dp_.f, beg + i, gld<uint8_t>(sp_.f, from));
+        } else {
+            gst<uint32_t>(dp_.row, beg + i, s_row[i]);
+        }
+    }
+}
+
 // Bitonic sort of m (a power of two) records in LDS on (key, row).
 template <bool TWO>
 __device__ __forceinline__ void lds_bitonic(uint64_t *s_lo, uint64_t *s_hi, uint32_t *s_row, uint32_t m) {
@@ -799,9 +895,13 @@ __device__ __forceinline__ void lds_bitonic(uint64_t *s_lo, uint64_t *s_hi, uint
 constexpr int kBinBits = 12;
 constexpr uint32_t kBins = 1u << kBinBits;
 constexpr uint32_t kBinSortMax = 64;
-template <bool TWO>
-__global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *__restrict__ L, uint64_t *klo,
-                                                               uint64_t *khi, uint32_t *krow,
+// The sorted bucket goes to the other buffer (keys and payload): CARRY
+// records sort on (key, place in the bucket) and take their values from
+// that place; row records sort on (key, row).
+template <bool TWO, bool CARRY>
+__global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                               const uint64_t *khi, const Payload sp_,
+                                                               uint64_t *dlo, uint64_t *dhi, const Payload dp_,
                                                                const uint64_t *bstart, uint32_t nb, int bin_shift,
                                                                int bin_bits) {
     extern __shared__ __align__(16) uint8_t smem[];
@@ -844,7 +944,7 @@ __global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *
                 const uint64_t hi = TWO ? gld_nt<uint64_t>(khi, beg + i) : 0;
                 a_lo[i] = lo;
                 if (TWO) a_hi[i] = hi;
-                a_row[i] = gld_nt<uint32_t>(krow, beg + i);
+                a_row[i] = CARRY ? i : gld_nt<uint32_t>(sp_.row, beg + i);
                 bn[k] = (uint32_t)(key_bits_from(lo, hi, bin_shift) & bmask);
                 rk[k] = atomicAdd(&bin[bn[k]], 1u);
             }
@@ -910,19 +1010,11 @@ __global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *
                 a_row[s0 + r] = row;
             }
             __syncthreads();
-            for (uint32_t i = tid; i < n; i += kBucketThreads) {
-                gst<uint64_t>(klo, beg + i, a_lo[i]);
-                if (TWO) gst<uint64_t>(khi, beg + i, a_hi[i]);
-                gst<uint32_t>(krow, beg + i, a_row[i]);
-            }
+            bucket_out<TWO, CARRY>(L, sp_, dlo, dhi, dp_, beg, n, a_lo, a_hi, a_row);
             __syncthreads();
             continue;
         }
-        for (uint32_t i = tid; i < n; i += kBucketThreads) {
-            gst<uint64_t>(klo, beg + i, b_lo[i]);
-            if (TWO) gst<uint64_t>(khi, beg + i, b_hi[i]);
-            gst<uint32_t>(krow, beg + i, b_row[i]);
-        }
+        bucket_out<TWO, CARRY>(L, sp_, dlo, dhi, dp_, beg, n, b_lo, b_hi, b_row);
         __syncthreads();
     }
 }
@@ -1092,8 +1184,10 @@ struct Carry {
     int64_t limb[2][kLimbs], sum[2], mn[2], mx[2];
 };
 
+template <bool CARRY>
 __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict__ L, const uint64_t *klo,
-                                                     const uint64_t *khi, const uint32_t *krow) {
+                                                     const uint64_t *khi, const Payload pl) {
+    const uint32_t *krow = pl.row;
     const QueryDesc *__restrict__ q = &L->q;
     const Layout &lay = q->layout;
     const bool two = L->kw == 2;
@@ -1207,15 +1301,21 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
         const int seg_start = 63 - __builtin_clzll(heads & ((2ull << lane) - 1));
         const int last_live = 63 - __builtin_clzll(lv);
         const bool seg_last = live && (lane == last_live || ((heads >> (lane + 1)) & 1));
-        // the row's contributions: every word first
-        const uint64_t row = live ? gld<uint32_t>(krow, i) : 0;
+        // the row's contributions: every word first (carried values are
+        // read in record order, with no gather through the row)
+        const uint64_t row = !CARRY && live ? gld<uint32_t>(krow, i) : 0;
         uint64_t w[2], vb[2];
+        const uint32_t fl = CARRY && live ? gld<uint8_t>(pl.f, i) : 0;
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             w[j] = vb[j] = 0;
             if (j >= nv) continue;
-            w[j] = word_at(vc[j].base, vpos(vc[j], row));
-            vb[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+            if (CARRY) {
+                w[j] = live ? gld<uint64_t>(pl.v[j], i) : 0;
+            } else {
+                w[j] = word_at(vc[j].base, vpos(vc[j], row));
+                vb[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+            }
         }
         Carry x;
         x.len = seg_scan<uint32_t>(live ? 1u : 0u, seg_start, lane, addu);
@@ -1229,8 +1329,8 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
             for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = 0;
             if (j >= nv) continue;
             const ValSpec &S = vs[j];
-            const uint64_t raw = cut(w[j], vpos(vc[j], row), vc[j].mask);
-            const bool ok = live && cut(vb[j], bpos(vc[j], row), 1);
+            const uint64_t raw = CARRY ? w[j] : cut(w[j], vpos(vc[j], row), vc[j].mask);
+            const bool ok = live && (CARRY ? ((fl >> j) & 1) != 0 : cut(vb[j], bpos(vc[j], row), 1) != 0);
             x.cnt[j] = seg_scan<uint32_t>(ok ? 1u : 0u, seg_start, lane, addu);
             if (S.kind == VK_F) {
                 const double v = vc[j].tag == PMH_F64 ? __longlong_as_double((long long)raw)
@@ -1427,6 +1527,9 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         }
         L.fast = aligned && L.q.n_keys <= 4 && plan.opts.fast_scan != 0 && lower_fast_scan(L.q, L.fs) ? 1 : 0;
     }
+    // the records carry the values where the fast scan encodes them and
+    // the fast fold reads them (one or two value columns)
+    L.carry = L.fast && P.q.n_vals >= 1 && P.q.n_vals <= 2 && plan.opts.sort_carry != 0 ? 1 : 0;
     SLaunch *dL = (SLaunch *)slab("so:launch", sizeof(SLaunch));
     PooledEvent pe0, pe1;
     const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
@@ -1491,8 +1594,17 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     L.kb_lo = (uint64_t *)slab("so:kb_lo", cap * 8 + 64);
     L.ka_hi = L.kw == 2 ? (uint64_t *)slab("so:ka_hi", cap * 8 + 64) : nullptr;
     L.kb_hi = L.kw == 2 ? (uint64_t *)slab("so:kb_hi", cap * 8 + 64) : nullptr;
-    L.ra = (uint32_t *)slab("so:ra", cap * 4 + 64);
-    L.rb = (uint32_t *)slab("so:rb", cap * 4 + 64);
+    if (L.carry) {
+        for (int j = 0; j < P.q.n_vals; ++j) {
+            L.va[j] = (uint64_t *)slab("so:va" + std::to_string(j), cap * 8 + 64);
+            L.vb[j] = (uint64_t *)slab("so:vb" + std::to_string(j), cap * 8 + 64);
+        }
+        L.fa = (uint8_t *)slab("so:fa", cap + 64);
+        L.fb = (uint8_t *)slab("so:fb", cap + 64);
+    } else {
+        L.ra = (uint32_t *)slab("so:ra", cap * 4 + 64);
+        L.rb = (uint32_t *)slab("so:rb", cap * 4 + 64);
+    }
     L.tile_hist = (uint32_t *)slab("so:tile_hist", (uint64_t)kDigits * L.ntiles * 4 + 64);
     L.digit_tot = (uint32_t *)slab("so:digit_tot", kDigits * 4 + 64);
     L.tile_cnt = (uint32_t *)slab("so:tile_cnt", ((uint64_t)L.ntiles + 1) * 4 + 64);
@@ -1501,7 +1613,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     // ---- 2. records, 3. radix passes
     const int pgrid = cus * 2;
     uint64_t *slo = L.ka_lo, *shi = L.ka_hi, *dlo = L.kb_lo, *dhi = L.kb_hi;
-    uint32_t *srow = L.ra, *drow = L.rb;
+    Payload spl{L.ra, {L.va[0], L.va[1]}, L.fa}, dpl{L.rb, {L.vb[0], L.vb[1]}, L.fb};
     if (n_sel) {
         if (L.fast) {
             KernelTimer kt("so_encode_fast");
@@ -1512,9 +1624,10 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             hipLaunchKernelGGL(so_encode, dim3(cus * 8), dim3(256), 0, R.stream, dL);
             launch_check();
         }
-        const size_t dyn = L.kw == 2 ? (size_t)kSortTile * 8 : 0;
-        PMH_HIP(hipFuncSetAttribute((const void *)so_scatter, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)std::max<size_t>(dyn, 16)));
+        const size_t dyn = (L.kw == 2 ? (size_t)kSortTile * 8 : 0) +
+                           (L.carry ? (size_t)kSortTile * (8 * (size_t)P.q.n_vals + 1) : (size_t)kSortTile * 4);
+        const void *scatter_fn = L.carry ? (const void *)so_scatter<true> : (const void *)so_scatter<false>;
+        PMH_HIP(hipFuncSetAttribute(scatter_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
         auto run_passes = [&](const std::vector<int> &sh) {
             for (const int shift : sh) {
                 {
@@ -1529,13 +1642,15 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 }
                 {
                     KernelTimer kt("so_scatter");
-                    hipLaunchKernelGGL(so_scatter, dim3(pgrid), dim3(kSortThreads), std::max<size_t>(dyn, 16),
-                                       R.stream, dL, slo, shi, srow, dlo, dhi, drow, shift);
+                    int sh = shift;
+                    void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl,
+                                    (void *)&dlo, (void *)&dhi, (void *)&dpl, (void *)&sh};
+                    PMH_HIP(hipLaunchKernel(scatter_fn, dim3(pgrid), dim3(kSortThreads), args, dyn, R.stream));
                     launch_check();
                 }
                 std::swap(slo, dlo);
                 std::swap(shi, dhi);
-                std::swap(srow, drow);
+                std::swap(spl, dpl);
             }
         };
         if (tbits) {
@@ -1571,18 +1686,26 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 KernelTimer kt("so_bucket_sort");
                 const bool two = L.kw == 2;
                 const size_t lds = (size_t)bucket_cap(two) * 2 * (two ? 20 : 12) + (size_t)kBins * 4;
-                const void *fn = two ? (const void *)so_bucket_sort<true> : (const void *)so_bucket_sort<false>;
+                static const void *const kBucket[2][2] = {
+                    {(const void *)so_bucket_sort<false, false>, (const void *)so_bucket_sort<false, true>},
+                    {(const void *)so_bucket_sort<true, false>, (const void *)so_bucket_sort<true, true>}};
+                const void *fn = kBucket[two ? 1 : 0][L.carry ? 1 : 0];
                 PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
                 const uint32_t grid = (uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * (lds * 2 <= 160 * 1024 ? 2 : 1));
                 // the bins: the kBinBits key bits below the bucket's top bits
                 // (fewer when the key has fewer bits left)
                 int bin_bits = std::min<int>(kBinBits, (int)width - tbits);
                 int bin_shift = (int)width - tbits - bin_bits;
-                void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&srow, (void *)&bstart,
-                                (void *)const_cast<uint32_t *>(&nb), (void *)&bin_shift, (void *)&bin_bits};
+                void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl, (void *)&dlo, (void *)&dhi,
+                                (void *)&dpl, (void *)&bstart, (void *)const_cast<uint32_t *>(&nb),
+                                (void *)&bin_shift, (void *)&bin_bits};
                 PMH_HIP(hipLaunchKernel(fn, dim3(grid), dim3(kBucketThreads), args, lds, R.stream));
                 launch_check();
             }
+            // the sorted records are in the other buffer
+            std::swap(slo, dlo);
+            std::swap(shi, dhi);
+            std::swap(spl, dpl);
         }
         // ---- 4. group starts
         auto run_bounds = [&]() {
@@ -1599,7 +1722,12 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         PMH_HIP(hipStreamSynchronize(R.stream));
         if (c[SC_BIG]) {
             // a bucket outgrew the LDS sort (keys concentrated in a few top
-            // bit patterns): the whole key by LSD passes after all
+            // bit patterns): the whole key by LSD passes after all, from the
+            // buffer the bucket sort read (it skipped the large buckets, so
+            // only that buffer holds every record)
+            std::swap(slo, dlo);
+            std::swap(shi, dhi);
+            std::swap(spl, dpl);
             std::vector<int> all;
             for (int p = 0; p < full_passes; ++p) all.push_back(p * kDigitBits);
             run_passes(all);
@@ -1644,11 +1772,16 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         }
         if (L.fast && P.q.n_vals <= 2) {
             KernelTimer kt("so_reduce_fast");
-            hipLaunchKernelGGL(so_reduce_fast, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo, shi, srow);
+            if (L.carry)
+                hipLaunchKernelGGL(so_reduce_fast<true>, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo, shi,
+                                   spl);
+            else
+                hipLaunchKernelGGL(so_reduce_fast<false>, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo,
+                                   shi, spl);
             launch_check();
         } else {
             KernelTimer kt("so_reduce");
-            hipLaunchKernelGGL(so_reduce, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, srow);
+            hipLaunchKernelGGL(so_reduce, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, spl.row);
             launch_check();
         }
         {

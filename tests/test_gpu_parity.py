@@ -680,3 +680,7 @@ def test_sort_bucket_finish(native, case):
         assert st_b["n_passes"] == st_l["n_passes"]  # the skew probe keeps hot buckets out: whole-key passes
     assert_results_equal(got, want, keys, label=f"sortb/{case}")
     assert_results_equal(got_lsd, want, keys, label=f"sortl/{case}")
+    # records that carry row indices (the fold gathers the values) instead of values
+    got_rows = run_gpu(native, dict(plan, options={"method": "sort", "sort_carry": 0}), frame)
+    assert native.last_stats()["n_passes"] == st_b["n_passes"]
+    assert_results_equal(got_rows, want, keys, label=f"sortr/{case}")

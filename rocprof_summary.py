@@ -86,7 +86,7 @@ def counter_kib(db_dir: Path, counter: str) -> dict:
     for r in c.execute("select * from counters_collection"):
         if r[ni] != counter:
             continue
-        name = r[ki].split("(")[0].split("::")[-1].split("<")[0].strip()
+        name = r[ki].replace("(anonymous namespace)::", "").split("(")[0].split("::")[-1].split("<")[0].strip()
         tot[name] += r[vi]
         disp[name].add(r[di])
     return {k: tot[k] / len(disp[k]) for k in tot}
@@ -148,7 +148,7 @@ def counters_table(dirs: list, title: str, command: str) -> str:
         tot = collections.defaultdict(float)
         disp = collections.defaultdict(set)
         for r in c.execute("select * from counters_collection"):
-            k = r[ki].split("(")[0].split("::")[-1].split("<")[0].strip()
+            k = r[ki].replace("(anonymous namespace)::", "").split("(")[0].split("::")[-1].split("<")[0].strip()
             tot[(k, r[ni])] += r[vi]
             disp[(k, r[ni])].add(r[di])
         for (k, n), v in tot.items():

@@ -561,12 +561,15 @@ template <bool CARRY>
 __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__restrict__ L, const uint64_t *slo,
                                                          const uint64_t *shi, const Payload sp_, uint64_t *dlo,
                                                          uint64_t *dhi, const Payload dp_, int shift) {
-    __shared__ uint32_t wcnt[kWaves][kDigits];
-    __shared__ uint32_t run[kDigits];     // rows of each digit ranked so far in this tile
+    // per (row step, wave, digit): the wave's rows of that digit in that
+    // step, then (in place) the rows of that digit before them in the tile
+    __shared__ uint16_t wcnt[kSortRows][kWaves][kDigits];
+    __shared__ uint32_t run[kDigits];     // rows of each digit in this tile
     __shared__ uint32_t dstart[kDigits];  // LDS start of each digit run
     __shared__ uint32_t gbase[kDigits];   // global start of each digit run of this tile
     __shared__ uint32_t dbase[kDigits];   // global start of each digit
     __shared__ uint32_t tmp[kDigits];
+    __shared__ uint32_t s_wave[16];
     __shared__ uint64_t st_lo[kSortTile];
     __shared__ uint8_t st_d[kSortTile];
     // dynamic: [hi keys when kw == 2], then rows, or values and validity
@@ -592,8 +595,6 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
     for (uint32_t t = blockIdx.x; t < L->ntiles; t += gridDim.x) {
         const uint64_t b = (uint64_t)t * kSortTile;
         const uint32_t rows = (uint32_t)min<uint64_t>(kSortTile, L->n_sel - b);
-        if (threadIdx.x < kDigits) run[threadIdx.x] = 0;
-        __syncthreads();
         uint32_t my_d[kSortRows], my_pos[kSortRows];
         uint64_t my_lo[kSortRows], my_hi[kSortRows];
         uint32_t my_row[kSortRows];
@@ -619,41 +620,51 @@ __global__ __launch_bounds__(kSortThreads) void so_scatter(const SLaunch *__rest
                 my_f[k] = 0;
             }
         }
+        // every step's multisplit first (lanes holding the same digit),
+        // then one count per (step, wave, digit) and one scan per digit in
+        // (step, wave) order, which is row order: the rank stays stable
+        uint32_t my_rank[kSortRows];
+        for (uint32_t x = threadIdx.x; x < kSortRows * kWaves * kDigits / 2; x += kSortThreads)
+            ((uint32_t *)&wcnt[0][0][0])[x] = 0;
 #pragma unroll
         for (int k = 0; k < kSortRows; ++k) {
             const uint32_t i = k * kSortThreads + threadIdx.x;
             const bool live = i < rows;
             const uint32_t d = live ? digit_of(my_lo[k], my_hi[k], shift) : 0;
-            // multisplit: lanes holding the same digit
             unsigned long long peers = __ballot(live);
 #pragma unroll
             for (int bit = 0; bit < kDigitBits; ++bit) {
                 const unsigned long long m = __ballot((d >> bit) & 1);
                 peers &= ((d >> bit) & 1) ? m : ~m;
             }
-            const uint32_t rank = (uint32_t)__popcll(peers & lt);
-            for (uint32_t x = threadIdx.x; x < kWaves * kDigits; x += kSortThreads) (&wcnt[0][0])[x] = 0;
-            __syncthreads();
-            if (live && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            if (threadIdx.x < kDigits) {
-                const uint32_t dd = threadIdx.x;
-                uint32_t acc = run[dd];
+            my_d[k] = d;
+            my_rank[k] = live ? (uint32_t)__popcll(peers & lt) : 0xFFFFFFFFu;
+            my_pos[k] = live ? (uint32_t)__popcll(peers) : 0;  // the peers' count, for the leader
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k)
+            if (my_rank[k] == 0) wcnt[k][wave][my_d[k]] = (uint16_t)my_pos[k];
+        __syncthreads();
+        if (threadIdx.x < kDigits) {
+            const uint32_t dd = threadIdx.x;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < kSortRows; ++k)
                 for (int w = 0; w < kWaves; ++w) {
-                    const uint32_t x = wcnt[w][dd];
-                    wcnt[w][dd] = acc;
+                    const uint32_t x = wcnt[k][w][dd];
+                    wcnt[k][w][dd] = (uint16_t)acc;
                     acc += x;
                 }
-                run[dd] = acc;
-            }
-            __syncthreads();
-            my_d[k] = d;
-            my_pos[k] = live ? wcnt[wave][d] + rank : 0xFFFFFFFFu;
-            __syncthreads();
+            run[dd] = acc;
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k)
+            my_pos[k] = my_rank[k] != 0xFFFFFFFFu ? wcnt[k][wave][my_d[k]] + my_rank[k] : 0xFFFFFFFFu;
         {
             const uint32_t cnt = threadIdx.x < kDigits ? run[threadIdx.x] : 0;
-            const uint32_t e = block_excl_digits(cnt, tmp);
+            const uint32_t e = block_excl_scan(cnt, s_wave);
             if (threadIdx.x < kDigits) {
                 dstart[threadIdx.x] = e;
                 gbase[threadIdx.x] = dbase[threadIdx.x] + L->tile_hist[(uint64_t)threadIdx.x * L->ntiles + t];
@@ -1527,9 +1538,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         }
         L.fast = aligned && L.q.n_keys <= 4 && plan.opts.fast_scan != 0 && lower_fast_scan(L.q, L.fs) ? 1 : 0;
     }
-    // the records carry the values where the fast scan encodes them and
-    // the fast fold reads them (one or two value columns)
-    L.carry = L.fast && P.q.n_vals >= 1 && P.q.n_vals <= 2 && plan.opts.sort_carry != 0 ? 1 : 0;
+
     SLaunch *dL = (SLaunch *)slab("so:launch", sizeof(SLaunch));
     PooledEvent pe0, pe1;
     const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
@@ -1566,6 +1575,14 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     uint32_t width = 0;
     for (int i = 0; i < L.n_runs; ++i) width += L.runs[i].len;
     const int full_passes = (int)((width + kDigitBits - 1) / kDigitBits);
+    // the records carry the values where the fast scan encodes them and
+    // the fast fold reads them (one or two value columns), and where the
+    // scatter's staging of them fits its LDS (not two-word keys with two
+    // value columns)
+    L.carry = L.fast && P.q.n_vals >= 1 && P.q.n_vals <= 2 && !(L.kw == 2 && P.q.n_vals == 2) &&
+                      plan.opts.sort_carry != 0
+                  ? 1
+                  : 0;
     // MSD finish: when the records split into buckets of about kBucketMean
     // by their top tbits bits with fewer passes than the whole key needs,
     // the passes order only those bits and LDS sorts finish each bucket

@@ -53,7 +53,11 @@ struct Run {
     uint32_t pos, len, at, pad;
 };
 
-enum SortCtr : int32_t { SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, SC_CURSOR, SC_GROUPS, SC_BIG, SC_N = 8 };
+enum SortCtr : int32_t {
+    SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, SC_CURSOR, SC_GROUPS, SC_BIG,
+    SC_PROBE_D,  // distinct top-bit patterns in the skew probe's sample
+    SC_N = 9
+};
 
 // Bucket finish (MSD over the top bits, then LDS sorts): a bucket of at most
 // kBucketCap records is sorted in one block's LDS; the radix passes first
@@ -62,7 +66,7 @@ enum SortCtr : int32_t { SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, S
 constexpr uint32_t kBucketCap = 4096;   // keys of one word (2048 for two-word keys)
 constexpr uint32_t kBucketMean = 3072;  // (1536 for two-word keys)
 __host__ __device__ constexpr uint32_t bucket_cap(bool two) { return two ? kBucketCap / 2 : kBucketCap; }
-constexpr int kBucketThreads = 1024;
+constexpr int kBucketThreads = 512;
 
 struct SLaunch {
     QueryDesc q;
@@ -734,34 +738,6 @@ __global__ __launch_bounds__(256) void so_bound(const SLaunch *__restrict__ L, c
     }
 }
 
-__global__ __launch_bounds__(1024) void so_bscan(const SLaunch *__restrict__ L) {
-    __shared__ uint32_t s[1024];
-    __shared__ uint32_t carry;
-    const uint32_t nt = L->ntiles;
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (uint32_t t0 = 0; t0 < nt; t0 += 1024) {
-        const uint32_t t = t0 + threadIdx.x;
-        const uint32_t x = t < nt ? L->tile_cnt[t] : 0;
-        s[threadIdx.x] = x;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            const uint32_t y = threadIdx.x >= (uint32_t)off ? s[threadIdx.x - off] : 0;
-            __syncthreads();
-            s[threadIdx.x] += y;
-            __syncthreads();
-        }
-        if (t < nt) L->tile_cnt[t] = carry + s[threadIdx.x] - x;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += s[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        L->tile_cnt[nt] = carry;
-        L->ctr[SC_GROUPS] = carry;
-    }
-}
-
 __global__ void so_init_state(const SLaunch *__restrict__ L, uint64_t ng) {
     const Layout &lay = L->q.layout;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -808,20 +784,21 @@ __global__ __launch_bounds__(256) void so_bucket_bounds(const SLaunch *__restric
 
 // Skew probe before the top-bit passes: the largest count of one bucket
 // among up to kProbeSample evenly spaced records (any order: the records
-// are still in row order) into ctr[SC_BIG].  A bucket far above the LDS
+// are still in row order) into ctr[SC_BIG], and the number of distinct
+// buckets they fall in into ctr[SC_PROBE_D].  A bucket far above the LDS
 // capacity shows as a large count; buckets near it may not (they still
-// fall back after the bucket sort flags them).
+// fall back after the bucket kernel flags them).
 constexpr uint32_t kProbeSample = 8192;
 __global__ __launch_bounds__(1024) void so_bucket_probe(const SLaunch *__restrict__ L, const uint64_t *klo,
                                                       const uint64_t *khi, int shift) {
     constexpr uint32_t kSlots = 2 * kProbeSample;
     __shared__ uint32_t s_id[kSlots], s_cnt[kSlots];
-    __shared__ uint32_t s_max;
+    __shared__ uint32_t s_max, s_distinct;
     const bool two = L->kw == 2;
     const uint64_t n = L->n_sel;
     const uint32_t S = (uint32_t)min<uint64_t>(n, kProbeSample);
     for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) { s_id[i] = 0; s_cnt[i] = 0; }
-    if (threadIdx.x == 0) s_max = 0;
+    if (threadIdx.x == 0) s_max = s_distinct = 0;
     __syncthreads();
     uint32_t mx = 0;
     for (uint32_t i = threadIdx.x; i < S; i += blockDim.x) {
@@ -832,6 +809,7 @@ __global__ __launch_bounds__(1024) void so_bucket_probe(const SLaunch *__restric
         for (uint32_t p = 0; p < kSlots; ++p) {  // at most S < kSlots ids: a free slot exists
             const uint32_t cur = atomicCAS(&s_id[h], 0u, id);
             if (cur == 0u || cur == id) {
+                if (cur == 0u) atomicAdd(&s_distinct, 1u);
                 mx = max(mx, atomicAdd(&s_cnt[h], 1u) + 1u);
                 break;
             }
@@ -840,7 +818,10 @@ __global__ __launch_bounds__(1024) void so_bucket_probe(const SLaunch *__restric
     }
     atomicMax(&s_max, mx);
     __syncthreads();
-    if (threadIdx.x == 0) L->ctr[SC_BIG] = s_max;
+    if (threadIdx.x == 0) {
+        L->ctr[SC_BIG] = s_max;
+        L->ctr[SC_PROBE_D] = s_distinct;
+    }
 }
 
 __device__ __forceinline__ bool rec_less(uint64_t ahi, uint64_t alo, uint32_t arow, uint64_t bhi, uint64_t blo,
@@ -848,28 +829,6 @@ __device__ __forceinline__ bool rec_less(uint64_t ahi, uint64_t alo, uint32_t ar
     if (ahi != bhi) return ahi < bhi;
     if (alo != blo) return alo < blo;
     return arow < brow;
-}
-
-// A sorted bucket out to the destination buffer: the keys, and the row or
-// (CARRY) the values and validity found at the record's place in the
-// source bucket.
-template <bool TWO, bool CARRY>
-__device__ __forceinline__ void bucket_out(const SLaunch *__restrict__ L, const Payload &sp_, uint64_t *dlo,
-                                           uint64_t *dhi, const Payload &dp_, uint64_t beg, uint32_t n,
-                                           const uint64_t *s_lo, const uint64_t *s_hi, const uint32_t *s_row) {
-    const int nv = CARRY ? L->q.n_vals : 0;
-    for (uint32_t i = threadIdx.x; i < n; i += kBucketThreads) {
-        gst<uint64_t>(dlo, beg + i, s_lo[i]);
-        if (TWO) gst<uint64_t>(dhi, beg + i, s_hi[i]);
-        if (CARRY) {
-            const uint64_t from = beg + s_row[i];
-            gst<uint64_t>(dp_.v[0], beg + i, gld<uint64_t>(sp_.v[0], from));
-            if (nv > 1) gst<uint64_t>(dp_.v[1], beg + i, gld<uint64_t>(sp_.v[1], from));
-            gst<uint8_t>(dp_.f, beg + i, gld<uint8_t>(sp_.f, from));
-        } else {
-            gst<uint32_t>(dp_.row, beg + i, s_row[i]);
-        }
-    }
 }
 
 // Bitonic sort of m (a power of two) records in LDS on (key, row).
@@ -896,19 +855,167 @@ __device__ __forceinline__ void lds_bitonic(uint64_t *s_lo, uint64_t *s_hi, uint
     }
 }
 
-// One bucket per block iteration (its records share the key's top bits):
-// a counting sort on the next kBinBits bits places the records by bin in a
-// second LDS buffer, then one thread per bin orders its few records by an
-// insertion sort (equal keys always share a bin).  A bin of more than
-// kBinSortMax records sends the bucket to a bitonic sort instead.  The
-// sorted bucket is written back.  A bucket larger than kBucketCap is left as
-// it is and flags SC_BIG: the host then sorts with whole-key LSD passes.
 constexpr int kBinBits = 12;
 constexpr uint32_t kBins = 1u << kBinBits;
 constexpr uint32_t kBinSortMax = 64;
-// The sorted bucket goes to the other buffer (keys and payload): CARRY
-// records sort on (key, place in the bucket) and take their values from
-// that place; row records sort on (key, row).
+
+// LDS of the bucket kernels: the bucket's records grouped by bin (keys,
+// [hi], rows), the bin counters, and the sorted order (place -> record).
+// Small enough (72 KB for one-word keys) for two blocks per CU, so one
+// block's loads overlap the other's LDS work.
+template <bool TWO>
+struct BucketBufs {
+    uint64_t *lo, *hi;
+    uint32_t *row;
+    uint32_t *bin;   // kBins: counts, then starts
+    uint16_t *perm;  // sorted place -> record, when the bins were ranked
+};
+template <bool TWO>
+__device__ __forceinline__ BucketBufs<TWO> bucket_bufs(uint8_t *smem) {
+    constexpr uint32_t cap = bucket_cap(TWO);
+    BucketBufs<TWO> B;
+    B.lo = (uint64_t *)smem;
+    B.hi = B.lo + cap;
+    B.row = (uint32_t *)(TWO ? B.hi + cap : B.hi);
+    B.bin = B.row + cap;
+    B.perm = (uint16_t *)(B.bin + kBins);
+    return B;
+}
+__host__ __device__ constexpr size_t bucket_lds_bytes(bool two) {
+    return (size_t)bucket_cap(two) * ((two ? 16 : 8) + 4 + 2) + (size_t)kBins * 4;
+}
+
+// Sorts one bucket, the n (1..cap) records [beg, beg + n) (they share the
+// key's top bits), on (key, row) in LDS: a counting sort on the next
+// bin_bits key bits places the records by bin, then every record finds its
+// place inside its bin by counting the bin's records that order before it
+// (equal keys always share a bin) and writes itself into B.perm at that
+// place.  A bin of more than kBinSortMax records sends the bucket to a
+// bitonic sort of the records themselves instead.  CARRY records sort on
+// (key, place in the bucket) and take their values from that place later;
+// row records sort on (key, row).  Returns true when the sorted order is
+// B.perm, false when the records themselves are sorted; the block is
+// synchronised on return.
+template <bool TWO, bool CARRY>
+__device__ __forceinline__ bool bucket_sort_lds(const uint64_t *klo, const uint64_t *khi, const Payload &sp_,
+                                                uint64_t beg, uint32_t n, const BucketBufs<TWO> &B, uint32_t nbins,
+                                                int bin_shift, uint32_t *s_wave, uint32_t *s_crowded) {
+    constexpr uint32_t cap = bucket_cap(TWO);
+    constexpr uint32_t kPer = cap / kBucketThreads;
+    const uint32_t tid = threadIdx.x;
+    const uint64_t bmask = nbins - 1;
+    for (uint32_t i = tid; i < nbins; i += kBucketThreads) B.bin[i] = 0;
+    if (tid == 0) *s_crowded = 0;
+    // every load is issued before the first is used
+    uint64_t lo[kPer], hi[kPer];
+    uint32_t row[kPer], br[kPer];  // br: bin << 16 | rank in the bin, ~0u: no record
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = k * kBucketThreads + tid;
+        lo[k] = hi[k] = 0;
+        row[k] = 0;
+        if (i < n) {
+            lo[k] = gld_nt<uint64_t>(klo, beg + i);
+            if (TWO) hi[k] = gld_nt<uint64_t>(khi, beg + i);
+            row[k] = CARRY ? i : gld_nt<uint32_t>(sp_.row, beg + i);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = k * kBucketThreads + tid;
+        br[k] = ~0u;
+        if (i < n) {
+            const uint32_t bn = (uint32_t)(key_bits_from(lo[k], hi[k], bin_shift) & bmask);
+            br[k] = bn << 16 | atomicAdd(&B.bin[bn], 1u);
+        }
+    }
+    __syncthreads();
+    // bin starts: each thread scans its consecutive run of bins
+    {
+        constexpr uint32_t kRun = kBins / kBucketThreads;
+        uint32_t c[kRun], tot = 0;
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint32_t i = tid * kRun + r;
+            c[r] = i < nbins ? B.bin[i] : 0;
+            tot += c[r];
+            if (c[r] > kBinSortMax) *s_crowded = 1;
+        }
+        uint32_t base = block_excl_scan(tot, s_wave);
+#pragma unroll
+        for (uint32_t r = 0; r < kRun; ++r) {
+            const uint32_t i = tid * kRun + r;
+            if (i < nbins) B.bin[i] = base;
+            base += c[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        if (br[k] == ~0u) continue;
+        const uint32_t d = B.bin[br[k] >> 16] + (br[k] & 0xFFFFu);
+        B.lo[d] = lo[k];
+        if (TWO) B.hi[d] = hi[k];
+        B.row[d] = row[k];
+    }
+    __syncthreads();
+    if (*s_crowded) {
+        // a crowded bin: sort the whole bucket (padded) instead
+        uint32_t m = 2;
+        while (m < n) m <<= 1;
+        for (uint32_t i = n + tid; i < m; i += kBucketThreads) {
+            B.lo[i] = ~0ull;
+            if (TWO) B.hi[i] = ~0ull;
+            B.row[i] = ~0u;
+        }
+        __syncthreads();
+        lds_bitonic<TWO>(B.lo, B.hi, B.row, m);
+        return false;
+    }
+    // inside each bin, a record's place is the number of the bin's records
+    // that order before it (keys, then rows: all distinct)
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        if (br[k] == ~0u) continue;
+        const uint32_t bn = br[k] >> 16;
+        const uint32_t s0 = B.bin[bn], s1 = bn + 1 < nbins ? B.bin[bn + 1] : n;
+        uint32_t r = 0;
+        for (uint32_t x = s0; x < s1; ++x)
+            r += rec_less(TWO ? B.hi[x] : 0, B.lo[x], B.row[x], hi[k], lo[k], row[k]) ? 1u : 0u;
+        B.perm[s0 + r] = (uint16_t)(s0 + (br[k] & 0xFFFFu));
+    }
+    __syncthreads();
+    return true;
+}
+
+// A sorted bucket out to the destination buffer: the keys, and the row or
+// (CARRY) the values and validity found at the record's place in the
+// source bucket.
+template <bool TWO, bool CARRY>
+__device__ __forceinline__ void bucket_out(const SLaunch *__restrict__ L, const Payload &sp_, uint64_t *dlo,
+                                           uint64_t *dhi, const Payload &dp_, uint64_t beg, uint32_t n,
+                                           const BucketBufs<TWO> &B, bool ranked) {
+    const int nv = CARRY ? L->q.n_vals : 0;
+    for (uint32_t i = threadIdx.x; i < n; i += kBucketThreads) {
+        const uint32_t x = ranked ? B.perm[i] : i;
+        gst<uint64_t>(dlo, beg + i, B.lo[x]);
+        if (TWO) gst<uint64_t>(dhi, beg + i, B.hi[x]);
+        if (CARRY) {
+            const uint64_t from = beg + B.row[x];
+            gst<uint64_t>(dp_.v[0], beg + i, gld<uint64_t>(sp_.v[0], from));
+            if (nv > 1) gst<uint64_t>(dp_.v[1], beg + i, gld<uint64_t>(sp_.v[1], from));
+            gst<uint8_t>(dp_.f, beg + i, gld<uint8_t>(sp_.f, from));
+        } else {
+            gst<uint32_t>(dp_.row, beg + i, B.row[x]);
+        }
+    }
+}
+
+// One bucket per block iteration: sorted in LDS and written, keys and
+// payload, to the other buffer.  A bucket larger than its LDS capacity is
+// left as it is and flags SC_BIG: the host then sorts with whole-key LSD
+// passes.
 template <bool TWO, bool CARRY>
 __global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *__restrict__ L, const uint64_t *klo,
                                                                const uint64_t *khi, const Payload sp_,
@@ -916,116 +1023,21 @@ __global__ __launch_bounds__(kBucketThreads) void so_bucket_sort(const SLaunch *
                                                                const uint64_t *bstart, uint32_t nb, int bin_shift,
                                                                int bin_bits) {
     extern __shared__ __align__(16) uint8_t smem[];
-    constexpr uint32_t cap = bucket_cap(TWO);
-    // two record buffers (keys, [hi], rows), then the bin counters
-    uint64_t *a_lo = (uint64_t *)smem;
-    uint64_t *a_hi = a_lo + cap;
-    uint32_t *a_row = (uint32_t *)(TWO ? a_hi + cap : a_hi);
-    uint64_t *b_lo = (uint64_t *)(a_row + cap);
-    uint64_t *b_hi = b_lo + cap;
-    uint32_t *b_row = (uint32_t *)(TWO ? b_hi + cap : b_hi);
-    uint32_t *bin = b_row + cap;  // kBins: counts, then starts
+    const BucketBufs<TWO> B = bucket_bufs<TWO>(smem);
     __shared__ uint32_t s_wave[16];
     __shared__ uint32_t s_crowded;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t nbins = 1u << bin_bits;
-    const uint64_t bmask = nbins - 1;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const uint64_t beg = bstart[b], end = bstart[b + 1];
         const uint64_t n64 = end - beg;  // uniform across the block
         if (n64 <= 1) continue;
-        if (n64 > cap) {
-            if (tid == 0) atomicAdd((unsigned long long *)&L->ctr[SC_BIG], 1ull);
+        if (n64 > bucket_cap(TWO)) {
+            if (threadIdx.x == 0) atomicAdd((unsigned long long *)&L->ctr[SC_BIG], 1ull);
             continue;
         }
         const uint32_t n = (uint32_t)n64;
-        for (uint32_t i = tid; i < nbins; i += kBucketThreads) bin[i] = 0;
-        if (tid == 0) s_crowded = 0;
-        __syncthreads();
-        // load and count; a record's rank in its bin comes from the count
-        constexpr uint32_t kPer = cap / kBucketThreads;
-        uint32_t rk[kPer], bn[kPer];
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            const uint32_t i = k * kBucketThreads + tid;
-            rk[k] = ~0u;
-            bn[k] = 0;
-            if (i < n) {
-                const uint64_t lo = gld_nt<uint64_t>(klo, beg + i);
-                const uint64_t hi = TWO ? gld_nt<uint64_t>(khi, beg + i) : 0;
-                a_lo[i] = lo;
-                if (TWO) a_hi[i] = hi;
-                a_row[i] = CARRY ? i : gld_nt<uint32_t>(sp_.row, beg + i);
-                bn[k] = (uint32_t)(key_bits_from(lo, hi, bin_shift) & bmask);
-                rk[k] = atomicAdd(&bin[bn[k]], 1u);
-            }
-        }
-        __syncthreads();
-        // bin starts: each thread scans its consecutive run of bins
-        {
-            constexpr uint32_t kRun = kBins / kBucketThreads;
-            uint32_t c[kRun], tot = 0;
-#pragma unroll
-            for (uint32_t r = 0; r < kRun; ++r) {
-                const uint32_t i = tid * kRun + r;
-                c[r] = i < nbins ? bin[i] : 0;
-                tot += c[r];
-                if (c[r] > kBinSortMax) s_crowded = 1;
-            }
-            uint32_t base = block_excl_scan(tot, s_wave);
-#pragma unroll
-            for (uint32_t r = 0; r < kRun; ++r) {
-                const uint32_t i = tid * kRun + r;
-                if (i < nbins) bin[i] = base;
-                base += c[r];
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            const uint32_t i = k * kBucketThreads + tid;
-            if (rk[k] == ~0u) continue;
-            const uint32_t d = bin[bn[k]] + rk[k];
-            b_lo[d] = a_lo[i];
-            if (TWO) b_hi[d] = a_hi[i];
-            b_row[d] = a_row[i];
-        }
-        __syncthreads();
-        if (s_crowded) {
-            // a crowded bin: sort the whole bucket (padded) instead
-            uint32_t m = 2;
-            while (m < n) m <<= 1;
-            for (uint32_t i = n + tid; i < m; i += kBucketThreads) {
-                b_lo[i] = ~0ull;
-                if (TWO) b_hi[i] = ~0ull;
-                b_row[i] = ~0u;
-            }
-            __syncthreads();
-            lds_bitonic<TWO>(b_lo, b_hi, b_row, m);
-        } else {
-            // inside each bin, a record's place is the number of the bin's
-            // records that order before it (keys, then rows: all distinct),
-            // counted by every record at once; the bucket goes back to A
-#pragma unroll
-            for (uint32_t k = 0; k < kPer; ++k) {
-                if (rk[k] == ~0u) continue;
-                const uint32_t s0 = bin[bn[k]], s1 = bn[k] + 1 < nbins ? bin[bn[k] + 1] : n;
-                const uint32_t me = s0 + rk[k];
-                const uint64_t lo = b_lo[me], hi = TWO ? b_hi[me] : 0;
-                const uint32_t row = b_row[me];
-                uint32_t r = 0;
-                for (uint32_t x = s0; x < s1; ++x)
-                    r += rec_less(TWO ? b_hi[x] : 0, b_lo[x], b_row[x], hi, lo, row) ? 1u : 0u;
-                a_lo[s0 + r] = lo;
-                if (TWO) a_hi[s0 + r] = hi;
-                a_row[s0 + r] = row;
-            }
-            __syncthreads();
-            bucket_out<TWO, CARRY>(L, sp_, dlo, dhi, dp_, beg, n, a_lo, a_hi, a_row);
-            __syncthreads();
-            continue;
-        }
-        bucket_out<TWO, CARRY>(L, sp_, dlo, dhi, dp_, beg, n, b_lo, b_hi, b_row);
+        const bool ranked = bucket_sort_lds<TWO, CARRY>(klo, khi, sp_, beg, n, B, 1u << bin_bits, bin_shift, s_wave,
+                                                        &s_crowded);
+        bucket_out<TWO, CARRY>(L, sp_, dlo, dhi, dp_, beg, n, B, ranked);
         __syncthreads();
     }
 }
@@ -1044,6 +1056,27 @@ __device__ __forceinline__ T seg_scan(T x, int seg_start, int lane, Op op) {
     }
     return x;
 }
+
+// seg_scan with nr rounds: complete when every segment of the step has
+// fewer than 2^nr lanes.
+template <class T, class Op>
+__device__ __forceinline__ T seg_scan_n(T x, int seg_start, int lane, Op op, int nr) {
+    for (int r = 0; r < nr; ++r) {
+        const int off = 1 << r;
+        const T y = __shfl_up(x, off);
+        if (lane - off >= seg_start) x = op(x, y);
+    }
+    return x;
+}
+
+// Lane src's value (src uniform across the wave).
+__device__ __forceinline__ uint32_t lane_u32(uint32_t x, int src) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, src);
+}
+__device__ __forceinline__ uint64_t lane_u64(uint64_t x, int src) {
+    return (uint64_t)lane_u32((uint32_t)x, src) | (uint64_t)lane_u32((uint32_t)(x >> 32), src) << 32;
+}
+__device__ __forceinline__ int64_t lane_i64(int64_t x, int src) { return (int64_t)lane_u64((uint64_t)x, src); }
 
 __global__ __launch_bounds__(kSortThreads) void so_reduce(const SLaunch *__restrict__ L, const uint64_t *klo,
                                                         const uint64_t *khi, const uint32_t *krow) {
@@ -1184,55 +1217,213 @@ __global__ __launch_bounds__(kSortThreads) void so_reduce(const SLaunch *__restr
     }
 }
 
-// The common shape (at most two aggregated columns): one wave per tile of
-// kSortTile records, 64 records per iteration.  Runs inside an iteration
-// fold with segmented scans as in so_reduce; the iteration's last run stays
-// open in registers (carry) and joins the next iteration's first run, so a
-// group spanning many records costs one atomic per tile and state array
-// instead of one per wave.
+// The common shape (at most two aggregated columns): runs of one wave step
+// of 64 records fold with segmented scans as in so_reduce; a step's last run
+// stays open in registers (Carry) and joins the next step's first run.
+// res: the values outside the exact window (and Inf/NaN), summed in f64 in
+// record order, as a sequential fold sums them (their sum depends on the
+// order: 1e308 - 1e308 + 5e-324 is 5e-324 only in that order).
 struct Carry {
     uint32_t len, cnt[2], nan[2];
     int64_t limb[2][kLimbs], sum[2], mn[2], mx[2];
+    double res[2];
 };
 
+__device__ __forceinline__ void carry_reset(Carry &c) {
+    c.len = 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        c.cnt[j] = c.nan[j] = 0;
+        c.sum[j] = 0;
+        c.mn[j] = INT64_MAX;
+        c.mx[j] = INT64_MIN;
+        c.res[j] = 0.0;
+#pragma unroll
+        for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = 0;
+    }
+}
+
+// x (this step's run that began at lane 0) continues the run carried in c.
+// EXACT: some column keeps exact f64 sums (limbs and residual); else those
+// fields stay zero and are skipped.  The residual is not joined here: the
+// step's fold starts its first run from the carried one (fold_records).
+template <bool EXACT>
+__device__ __forceinline__ void carry_join(Carry &x, const Carry &c) {
+    x.len += c.len;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        x.cnt[j] += c.cnt[j];
+        x.nan[j] += c.nan[j];
+        x.sum[j] += c.sum[j];
+        x.mn[j] = x.mn[j] < c.mn[j] ? x.mn[j] : c.mn[j];
+        x.mx[j] = x.mx[j] > c.mx[j] ? x.mx[j] : c.mx[j];
+        if (!EXACT) continue;
+#pragma unroll
+        for (int z = 0; z < kLimbs; ++z) x.limb[j][z] += c.limb[j][z];
+    }
+}
+
+// c = lane src's x (every lane; src uniform).
+template <bool EXACT>
+__device__ __forceinline__ void carry_take(Carry &c, const Carry &x, int src) {
+    c.len = lane_u32(x.len, src);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        c.cnt[j] = lane_u32(x.cnt[j], src);
+        c.nan[j] = lane_u32(x.nan[j], src);
+        c.sum[j] = lane_i64(x.sum[j], src);
+        c.mn[j] = lane_i64(x.mn[j], src);
+        c.mx[j] = lane_i64(x.mx[j], src);
+        if (!EXACT) continue;
+        c.res[j] = __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(x.res[j]), src));
+#pragma unroll
+        for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = lane_i64(x.limb[j][z], src);
+    }
+}
+
+// The aggregated columns of the fast fold.
+struct FoldVals {
+    int nv;
+    ValSpec vs[2];
+    RoleCol vc[2];
+};
+__device__ __forceinline__ FoldVals fold_vals(const QueryDesc *__restrict__ q) {
+    FoldVals F;
+    F.nv = q->n_vals;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        F.vs[j] = q->vals[j < F.nv ? j : 0];
+        F.vc[j] = role_col(q, F.vs[j].col);
+    }
+    return F;
+}
+
+// A record's value bits and validity: CARRY from the payload at index at,
+// else from the source columns at row at.  Every word is loaded before any
+// is used.
 template <bool CARRY>
+__device__ __forceinline__ void record_values(const FoldVals &F, const Payload &pl, bool live, uint64_t at,
+                                              uint64_t (&raw)[2], bool (&ok)[2]) {
+    uint64_t w[2], vb[2];
+    const uint32_t fl = CARRY && live ? gld<uint8_t>(pl.f, at) : 0;
+    const uint64_t row = live ? at : 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        w[j] = vb[j] = 0;
+        if (j >= F.nv) continue;
+        if (CARRY) {
+            w[j] = live ? gld<uint64_t>(pl.v[j], at) : 0;
+        } else {
+            w[j] = word_at(F.vc[j].base, vpos(F.vc[j], row));
+            vb[j] = word_at(F.vc[j].vbase, bpos(F.vc[j], row));
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        raw[j] = 0;
+        ok[j] = false;
+        if (j >= F.nv) continue;
+        raw[j] = CARRY ? w[j] : cut(w[j], vpos(F.vc[j], row), F.vc[j].mask);
+        ok[j] = live && (CARRY ? ((fl >> j) & 1) != 0 : cut(vb[j], bpos(F.vc[j], row), 1) != 0);
+    }
+}
+
+// x = each lane's run totals from its segment's first lane (seg_start) up to
+// itself; the segment's last lane holds the run's totals.  res0: the
+// residual the first segment continues from (the carried run's, or zero).
+// Counts come from ballots (no shuffles); sums, limbs and min/max scan
+// with only the rounds the step's longest segment needs.
+template <bool EXACT>
+__device__ __forceinline__ void fold_records(Carry &x, const FoldVals &F, bool live, int seg_start, int lane,
+                                             const uint64_t (&raw)[2], const bool (&ok)[2], const double (&res0)[2]) {
+    auto add64 = [](int64_t a, int64_t b) { return a + b; };
+    auto mn64 = [](int64_t a, int64_t b) { return a < b ? a : b; };
+    auto mx64 = [](int64_t a, int64_t b) { return a > b ? a : b; };
+    carry_reset(x);
+    // the lanes of this lane's segment up to itself
+    const unsigned long long seg = ((2ull << lane) - 1) & ~((1ull << seg_start) - 1);
+    int nr = 0;
+    while (nr < 6 && __ballot(live && lane - seg_start >= (1 << nr))) ++nr;
+    x.len = (uint32_t)__popcll(__ballot(live) & seg);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= F.nv) continue;
+        const ValSpec &S = F.vs[j];
+        x.cnt[j] = (uint32_t)__popcll(__ballot(ok[j]) & seg);
+        if (S.kind == VK_F) {
+            const double v = F.vc[j].tag == PMH_F64 ? __longlong_as_double((long long)raw[j])
+                                                    : (double)__uint_as_float((uint32_t)raw[j]);
+            if (EXACT && (S.need & NEED_SUM)) {
+                int64_t lb[kLimbs] = {0, 0, 0, 0, 0};
+                bool out = false;
+                if (ok[j]) {
+                    const uint64_t bb = (uint64_t)__double_as_longlong(v);
+                    const uint32_t E = (uint32_t)(bb >> 52) & 0x7FFu;
+                    uint64_t m = bb & 0xFFFFFFFFFFFFFull;
+                    if (E) m |= 1ull << 52;
+                    const int e = E ? (int)E - 1075 : -1074;
+                    const int sh0 = -kFixScale - e;
+                    const bool low_ok = m == 0 || sh0 <= 0 || (sh0 < 64 && (m & ((1ull << sh0) - 1)) == 0);
+                    const uint64_t mm = sh0 > 0 ? (sh0 < 64 ? m >> sh0 : 0) : m;
+                    const int p = sh0 > 0 ? 0 : e + kFixScale;
+                    const int msb = mm ? p + 63 - __builtin_clzll(mm) : 0;
+                    if (E != 0x7FFu && low_ok && msb < 32 * kLimbs) {
+                        const int li = p >> 5, sh = p & 31;
+                        const uint64_t lo = mm << sh;
+                        const uint64_t hi = sh ? (mm >> (64 - sh)) : 0;
+                        int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
+                        if (bb >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
+#pragma unroll
+                        for (int z = 0; z < kLimbs; ++z)
+                            lb[z] = (z == li ? c0 : 0) + (z == li + 1 ? c1 : 0) + (z == li + 2 ? c2 : 0);
+                    } else {
+                        out = true;  // outside the window (rare): the residual
+                    }
+                }
+#pragma unroll
+                for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = seg_scan_n<int64_t>(lb[z], seg_start, lane, add64, nr);
+                // the rare residual values one at a time, in record order
+                double r = seg_start == 0 ? res0[j] : 0.0;
+                for (unsigned long long om = __ballot(out); om; om &= om - 1) {
+                    const int src = __builtin_ctzll(om);
+                    const double val = __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(v), src));
+                    const int sseg = (int)lane_u32((uint32_t)seg_start, src);
+                    if (lane >= src && seg_start == sseg) r += val;
+                }
+                x.res[j] = r;
+            }
+            if (S.need & NEED_MINMAX) {
+                const bool nan = ok[j] && v != v;
+                const int64_t kk = f64_order_key(v);
+                x.mn[j] = seg_scan_n<int64_t>(ok[j] && !nan ? kk : INT64_MAX, seg_start, lane, mn64, nr);
+                x.mx[j] = seg_scan_n<int64_t>(ok[j] && !nan ? kk : INT64_MIN, seg_start, lane, mx64, nr);
+                x.nan[j] = (uint32_t)__popcll(__ballot(nan) & seg);
+            }
+        } else {
+            const int64_t v = S.kind == VK_I64 ? (int64_t)raw[j] : (int64_t)(int32_t)(uint32_t)raw[j];
+            if (S.need & NEED_SUM) x.sum[j] = seg_scan_n<int64_t>(ok[j] ? v : 0, seg_start, lane, add64, nr);
+            if (S.need & NEED_MINMAX) {
+                x.mn[j] = seg_scan_n<int64_t>(ok[j] ? v : INT64_MAX, seg_start, lane, mn64, nr);
+                x.mx[j] = seg_scan_n<int64_t>(ok[j] ? v : INT64_MIN, seg_start, lane, mx64, nr);
+            }
+        }
+    }
+}
+
+// One wave per tile of kSortTile sorted records.  A group spanning many
+// records costs one atomic per tile and state array instead of one per wave.
+template <bool CARRY, bool EXACT>
 __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict__ L, const uint64_t *klo,
                                                      const uint64_t *khi, const Payload pl) {
-    const uint32_t *krow = pl.row;
     const QueryDesc *__restrict__ q = &L->q;
     const Layout &lay = q->layout;
     const bool two = L->kw == 2;
     const int lane = threadIdx.x & 63;
     const uint32_t t = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (t >= L->ntiles) return;  // whole waves leave together
-    const int nv = q->n_vals;
-    RoleCol vc[2];
-    ValSpec vs[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-        vs[j] = q->vals[j < nv ? j : 0];
-        vc[j] = role_col(q, vs[j].col);
-    }
+    const FoldVals F = fold_vals(q);
+    const int nv = F.nv;
     GSink g{L->arr, L->gcap};
-    auto addu = [](uint32_t a, uint32_t b) { return a + b; };
-    auto add64 = [](int64_t a, int64_t b) { return a + b; };
-    auto mn64 = [](int64_t a, int64_t b) { return a < b ? a : b; };
-    auto mx64 = [](int64_t a, int64_t b) { return a > b ? a : b; };
-    Carry c{};
-    bool have = false;
-    uint64_t cgid = 0;
-    auto reset = [&]() {
-        c.len = 0;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            c.cnt[j] = c.nan[j] = 0;
-            c.sum[j] = 0;
-            c.mn[j] = INT64_MAX;
-            c.mx[j] = INT64_MIN;
-#pragma unroll
-            for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = 0;
-        }
-    };
     // one lane writes the totals of one run of group gid: with plain stores
     // when the run is the group's only one (it starts and ends inside this
     // tile, so no other tile touches the group; the arrays hold identities),
@@ -1246,15 +1437,20 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             if (j >= nv || !x.cnt[j]) continue;
-            const ValSpec &S = vs[j];
+            const ValSpec &S = F.vs[j];
             w32(S.a_cnt, x.cnt[j]);
             if (S.need & NEED_SUM) {
                 if (S.kind == VK_F) {
+                    if (!EXACT) continue;  // not reached: EXACT is set when a column sums floats
 #pragma unroll
                     for (int z = 0; z < kLimbs; ++z) {
                         if (!x.limb[j][z]) continue;
                         if (only) gst<int64_t>(L->arr[S.a_limb], (uint64_t)z * L->gcap + gid, x.limb[j][z]);
                         else g.addlimb(S.a_limb, z, gid, x.limb[j][z]);
+                    }
+                    if (x.res[j] != 0.0 || x.res[j] != x.res[j]) {
+                        if (only) gst<double>(L->arr[S.a_res], gid, x.res[j]);
+                        else g.addf(S.a_res, gid, x.res[j]);
                     }
                 } else if (only) {
                     gst<int64_t>(L->arr[S.a_sum], gid, x.sum[j]);
@@ -1285,7 +1481,10 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
             }
         }
     };
-    reset();
+    Carry c;
+    carry_reset(c);
+    bool have = false;
+    uint64_t cgid = 0;
     const uint64_t tb = (uint64_t)t * kSortTile;
     uint64_t before = 0;  // group starts seen in this tile before the current iteration
     bool c_own = false;   // the carried run started inside this tile
@@ -1306,123 +1505,27 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
         if (have && (sm & 1ull)) {
             if (lane == 0) flush(cgid, c, c_own);
             have = false;
-            reset();
         }
         const unsigned long long heads = sm | 1ull;
         const int seg_start = 63 - __builtin_clzll(heads & ((2ull << lane) - 1));
         const int last_live = 63 - __builtin_clzll(lv);
         const bool seg_last = live && (lane == last_live || ((heads >> (lane + 1)) & 1));
-        // the row's contributions: every word first (carried values are
-        // read in record order, with no gather through the row)
-        const uint64_t row = !CARRY && live ? gld<uint32_t>(krow, i) : 0;
-        uint64_t w[2], vb[2];
-        const uint32_t fl = CARRY && live ? gld<uint8_t>(pl.f, i) : 0;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            w[j] = vb[j] = 0;
-            if (j >= nv) continue;
-            if (CARRY) {
-                w[j] = live ? gld<uint64_t>(pl.v[j], i) : 0;
-            } else {
-                w[j] = word_at(vc[j].base, vpos(vc[j], row));
-                vb[j] = word_at(vc[j].vbase, bpos(vc[j], row));
-            }
-        }
+        uint64_t raw[2];
+        bool ok[2];
+        record_values<CARRY>(F, pl, live, CARRY ? i : (live ? gld<uint32_t>(pl.row, i) : 0), raw, ok);
         Carry x;
-        x.len = seg_scan<uint32_t>(live ? 1u : 0u, seg_start, lane, addu);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            x.cnt[j] = x.nan[j] = 0;
-            x.sum[j] = 0;
-            x.mn[j] = INT64_MAX;
-            x.mx[j] = INT64_MIN;
-#pragma unroll
-            for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = 0;
-            if (j >= nv) continue;
-            const ValSpec &S = vs[j];
-            const uint64_t raw = CARRY ? w[j] : cut(w[j], vpos(vc[j], row), vc[j].mask);
-            const bool ok = live && (CARRY ? ((fl >> j) & 1) != 0 : cut(vb[j], bpos(vc[j], row), 1) != 0);
-            x.cnt[j] = seg_scan<uint32_t>(ok ? 1u : 0u, seg_start, lane, addu);
-            if (S.kind == VK_F) {
-                const double v = vc[j].tag == PMH_F64 ? __longlong_as_double((long long)raw)
-                                                      : (double)__uint_as_float((uint32_t)raw);
-                if (S.need & NEED_SUM) {
-                    int64_t lb[kLimbs] = {0, 0, 0, 0, 0};
-                    if (ok) {
-                        const uint64_t bb = (uint64_t)__double_as_longlong(v);
-                        const uint32_t E = (uint32_t)(bb >> 52) & 0x7FFu;
-                        uint64_t m = bb & 0xFFFFFFFFFFFFFull;
-                        if (E) m |= 1ull << 52;
-                        const int e = E ? (int)E - 1075 : -1074;
-                        const int sh0 = -kFixScale - e;
-                        const bool low_ok = m == 0 || sh0 <= 0 || (sh0 < 64 && (m & ((1ull << sh0) - 1)) == 0);
-                        const uint64_t mm = sh0 > 0 ? (sh0 < 64 ? m >> sh0 : 0) : m;
-                        const int p = sh0 > 0 ? 0 : e + kFixScale;
-                        const int msb = mm ? p + 63 - __builtin_clzll(mm) : 0;
-                        if (E != 0x7FFu && low_ok && msb < 32 * kLimbs) {
-                            const int li = p >> 5, sh = p & 31;
-                            const uint64_t lo = mm << sh;
-                            const uint64_t hi = sh ? (mm >> (64 - sh)) : 0;
-                            int64_t c0 = (int64_t)(lo & 0xFFFFFFFFull), c1 = (int64_t)(lo >> 32), c2 = (int64_t)hi;
-                            if (bb >> 63) { c0 = -c0; c1 = -c1; c2 = -c2; }
-#pragma unroll
-                            for (int z = 0; z < kLimbs; ++z)
-                                lb[z] = (z == li ? c0 : 0) + (z == li + 1 ? c1 : 0) + (z == li + 2 ? c2 : 0);
-                        } else {
-                            g.addf(S.a_res, gid, v);  // outside the window (rare): straight to the residual
-                        }
-                    }
-#pragma unroll
-                    for (int z = 0; z < kLimbs; ++z) x.limb[j][z] = seg_scan<int64_t>(lb[z], seg_start, lane, add64);
-                }
-                if (S.need & NEED_MINMAX) {
-                    const bool nan = ok && v != v;
-                    const int64_t kk = f64_order_key(v);
-                    x.mn[j] = seg_scan<int64_t>(ok && !nan ? kk : INT64_MAX, seg_start, lane, mn64);
-                    x.mx[j] = seg_scan<int64_t>(ok && !nan ? kk : INT64_MIN, seg_start, lane, mx64);
-                    x.nan[j] = seg_scan<uint32_t>(nan ? 1u : 0u, seg_start, lane, addu);
-                }
-            } else {
-                const int64_t v = S.kind == VK_I64 ? (int64_t)raw : (int64_t)(int32_t)(uint32_t)raw;
-                if (S.need & NEED_SUM) x.sum[j] = seg_scan<int64_t>(ok ? v : 0, seg_start, lane, add64);
-                if (S.need & NEED_MINMAX) {
-                    x.mn[j] = seg_scan<int64_t>(ok ? v : INT64_MAX, seg_start, lane, mn64);
-                    x.mx[j] = seg_scan<int64_t>(ok ? v : INT64_MIN, seg_start, lane, mx64);
-                }
-            }
-        }
+        const double res0[2] = {have ? c.res[0] : 0.0, have ? c.res[1] : 0.0};
+        fold_records<EXACT>(x, F, live, seg_start, lane, raw, ok, res0);
         // a continuing run: the carry joins this iteration's first run
-        if (have && seg_start == 0 && seg_last) {
-            x.len += c.len;
-#pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                x.cnt[j] += c.cnt[j];
-                x.nan[j] += c.nan[j];
-                x.sum[j] += c.sum[j];
-                x.mn[j] = x.mn[j] < c.mn[j] ? x.mn[j] : c.mn[j];
-                x.mx[j] = x.mx[j] > c.mx[j] ? x.mx[j] : c.mx[j];
-#pragma unroll
-                for (int z = 0; z < kLimbs; ++z) x.limb[j][z] += c.limb[j][z];
-            }
-        }
+        if (have && seg_start == 0 && seg_last) carry_join<EXACT>(x, c);
         // closed runs go out; the last run of the iteration becomes the carry.
         // A run owns its group unless it began at lane 0 as the carried run
         // of a tile whose first records continue the previous tile's group.
         const bool own = seg_start > 0 || (sm & 1ull) || (have && c_own);
         if (seg_last && lane != last_live) flush(gid, x, own);
-        c_own = __shfl(own ? 1 : 0, last_live) != 0;
-        cgid = __shfl(gid, last_live);
-        c.len = __shfl(x.len, last_live);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            c.cnt[j] = __shfl(x.cnt[j], last_live);
-            c.nan[j] = __shfl(x.nan[j], last_live);
-            c.sum[j] = __shfl(x.sum[j], last_live);
-            c.mn[j] = __shfl(x.mn[j], last_live);
-            c.mx[j] = __shfl(x.mx[j], last_live);
-#pragma unroll
-            for (int z = 0; z < kLimbs; ++z) c.limb[j][z] = __shfl(x.limb[j][z], last_live);
-        }
+        c_own = lane_u32(own ? 1u : 0u, last_live) != 0;
+        cgid = lane_u64(gid, last_live);
+        carry_take<EXACT>(c, x, last_live);
         have = true;
     }
     // the tile's last run is the group's only one if the next tile starts a
@@ -1432,9 +1535,228 @@ __global__ __launch_bounds__(256) void so_reduce_fast(const SLaunch *__restrict_
     if (have && lane == 0) flush(cgid, c, c_own && closed);
 }
 
+// Every state field of group gid, with plain stores (the fold's state
+// arrays are not initialised: a group's one writer stores all of them).
+template <bool EXACT>
+__device__ __forceinline__ void fold_store(const SLaunch *__restrict__ L, const FoldVals &F, uint64_t gid,
+                                           const Carry &x) {
+    const Layout &lay = L->q.layout;
+    gst<uint32_t>(L->arr[lay.a_len], gid, x.len);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (j >= F.nv) continue;
+        const ValSpec &S = F.vs[j];
+        gst<uint32_t>(L->arr[S.a_cnt], gid, x.cnt[j]);
+        if (S.need & NEED_SUM) {
+            if (S.kind == VK_F) {
+                if (!EXACT) continue;  // not reached: EXACT is set when a column sums floats
+#pragma unroll
+                for (int z = 0; z < kLimbs; ++z) gst<int64_t>(L->arr[S.a_limb + z], gid, x.limb[j][z]);
+                gst<double>(L->arr[S.a_res], gid, x.res[j]);
+            } else {
+                gst<int64_t>(L->arr[S.a_sum], gid, x.sum[j]);
+            }
+        }
+        if (S.need & NEED_MINMAX) {
+            if (S.kind == VK_I32) {
+                gst<int32_t>(L->arr[S.a_min], gid, (int32_t)x.mn[j]);
+                gst<int32_t>(L->arr[S.a_max], gid, (int32_t)x.mx[j]);
+            } else {
+                gst<int64_t>(L->arr[S.a_min], gid, x.mn[j]);
+                gst<int64_t>(L->arr[S.a_max], gid, x.mx[j]);
+            }
+            if (S.kind == VK_F) gst<uint32_t>(L->arr[S.a_nan], gid, x.nan[j] ? 1u : 0u);
+        }
+    }
+}
+
+// Bucket fold: each bucket is sorted in LDS as in so_bucket_sort and its
+// groups are folded straight from LDS; no sorted record is written back.  A
+// bucket holds whole groups (its records share the key's top bits), and
+// each wave folds whole groups: wave w takes the groups that start in
+// [w n / W, (w + 1) n / W) of the bucket's n sorted records (W waves).  A
+// group's state (the fast fold's shape, at most two aggregated columns) and
+// compressed key go to sparse index bstart[b] + (its rank among the
+// bucket's groups), with plain stores; bg[b] counts the bucket's groups.
+// The emit reads them back in bucket order.  Four waves per SIMD (at most
+// 128 registers): two blocks per CU.
+template <bool TWO, bool CARRY, bool EXACT>
+__global__ __launch_bounds__(kBucketThreads, 4) void so_bucket_fold(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                               const uint64_t *khi, const Payload sp_,
+                                                               const uint64_t *bstart, uint32_t nb, int bin_shift,
+                                                               int bin_bits, uint32_t *bg) {
+    constexpr int kWB = kBucketThreads / 64;
+    constexpr uint32_t cap = bucket_cap(TWO);
+    extern __shared__ __align__(16) uint8_t smem[];
+    const BucketBufs<TWO> B = bucket_bufs<TWO>(smem);
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_crowded;
+    __shared__ uint32_t s_split[kWB], s_gcnt[kWB];
+    __shared__ unsigned long long s_start[cap / 64];  // bit i: sorted place i starts a group
+    const FoldVals F = fold_vals(&L->q);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint64_t beg = bstart[b], n64 = bstart[b + 1] - beg;  // uniform across the block
+        if (n64 == 0 || n64 > cap) {
+            if (threadIdx.x == 0) {
+                bg[b] = 0;
+                if (n64) atomicAdd((unsigned long long *)&L->ctr[SC_BIG], 1ull);
+            }
+            continue;
+        }
+        const uint32_t n = (uint32_t)n64;
+        const bool ranked = bucket_sort_lds<TWO, CARRY>(klo, khi, sp_, beg, n, B, 1u << bin_bits, bin_shift, s_wave,
+                                                        &s_crowded);
+        auto rec = [&](uint32_t i) -> uint32_t { return ranked ? B.perm[i] : i; };
+        // group starts, one bit per sorted place
+        for (uint32_t k = (uint32_t)w * 64; k < n; k += kBucketThreads) {
+            const uint32_t i = k + lane;
+            bool st = false;
+            if (i < n) {
+                const uint32_t x = rec(i);
+                st = i == 0;
+                if (!st) {
+                    const uint32_t p = rec(i - 1);
+                    st = B.lo[x] != B.lo[p] || (TWO && B.hi[x] != B.hi[p]);
+                }
+            }
+            const unsigned long long m = __ballot(st);
+            if (lane == 0) s_start[k >> 6] = m;
+        }
+        __syncthreads();
+        auto starts = [&](uint32_t k) -> unsigned long long { return k < n ? s_start[k >> 6] : 0ull; };
+        // the wave's records: from the first group start at or after w n / W
+        // to the next wave's first (n for the last wave)
+        {
+            const uint32_t t0 = (uint32_t)((uint64_t)w * n / kWB);
+            uint32_t s = n;
+            for (uint32_t k = t0 & ~63u; k < n; k += 64) {
+                unsigned long long m = starts(k);
+                if (k < t0) m &= ~0ull << (t0 - k);
+                if (k + 64 > n) m &= (n - k) == 64 ? ~0ull : ((1ull << (n - k)) - 1);
+                if (m) {
+                    s = k + (uint32_t)__builtin_ctzll(m);
+                    break;
+                }
+            }
+            if (lane == 0) s_split[w] = s;
+        }
+        __syncthreads();
+        const uint32_t r0 = s_split[w], r1 = w + 1 < kWB ? s_split[w + 1] : n;
+        // groups of each wave, then of the waves before it
+        {
+            uint32_t cnt = 0;
+            for (uint32_t k = r0 & ~63u; k < r1; k += 64) {
+                unsigned long long m = starts(k);
+                if (k < r0) m &= ~0ull << (r0 - k);
+                if (k + 64 > r1) m &= (r1 - k) == 64 ? ~0ull : ((1ull << (r1 - k)) - 1);
+                cnt += (uint32_t)__popcll(m);
+            }
+            if (lane == 0) s_gcnt[w] = cnt;
+        }
+        __syncthreads();
+        uint32_t gbase = 0, total = 0;
+#pragma unroll
+        for (int v = 0; v < kWB; ++v) {
+            const uint32_t cv = s_gcnt[v];
+            gbase += v < w ? cv : 0;
+            total += cv;
+        }
+        if (threadIdx.x == 0) bg[b] = total;
+        Carry c;
+        carry_reset(c);
+        bool have = false;
+        uint64_t cgid = 0;
+        uint32_t before = 0;
+        for (uint32_t k = r0; k < r1; k += 64) {
+            const uint32_t i = k + lane;
+            const bool live = i < r1;
+            const unsigned long long lv = __ballot(live);
+            // starts among this step's places k .. k + 63 (k need not be a multiple of 64)
+            const uint32_t sh = k & 63u;
+            const unsigned long long w0 = starts(k & ~63u), w1 = starts((k & ~63u) + 64);
+            const unsigned long long sm = (sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0) & lv;
+            const bool start = (sm >> lane) & 1;
+            const uint64_t gid = beg + gbase + before + (uint32_t)__popcll(sm & ((2ull << lane) - 1)) - 1;
+            before += (uint32_t)__popcll(sm);
+            const uint32_t x = live ? rec(i) : 0;
+            if (start) {
+                gst<uint64_t>(L->gk_lo, gid, B.lo[x]);
+                if (TWO) gst<uint64_t>(L->gk_hi, gid, B.hi[x]);
+            }
+            if (have && (sm & 1ull)) {
+                if (lane == 0) fold_store<EXACT>(L, F, cgid, c);
+                have = false;
+            }
+            const unsigned long long heads = sm | 1ull;
+            const int seg_start = 63 - __builtin_clzll(heads & ((2ull << lane) - 1));
+            const int last_live = 63 - __builtin_clzll(lv);
+            const bool seg_last = live && (lane == last_live || ((heads >> (lane + 1)) & 1));
+            uint64_t raw[2];
+            bool ok[2];
+            const uint32_t at = live ? B.row[x] : 0;
+            record_values<CARRY>(F, sp_, live, CARRY ? beg + at : (uint64_t)at, raw, ok);
+            Carry xs;
+            const double res0[2] = {have ? c.res[0] : 0.0, have ? c.res[1] : 0.0};
+            fold_records<EXACT>(xs, F, live, seg_start, lane, raw, ok, res0);
+            if (have && seg_start == 0 && seg_last) carry_join<EXACT>(xs, c);
+            if (seg_last && lane != last_live) fold_store<EXACT>(L, F, gid, xs);
+            cgid = lane_u64(gid, last_live);
+            carry_take<EXACT>(c, xs, last_live);
+            have = true;
+        }
+        if (have && lane == 0) fold_store<EXACT>(L, F, cgid, c);
+        __syncthreads();  // the next bucket reuses the LDS
+    }
+}
+
+// Exclusive prefix of n counts (pre may be cnt: in place) and their total,
+// by one block: each thread sums a consecutive run of counts.
+__global__ __launch_bounds__(1024) void so_scan_counts(const uint32_t *cnt, uint32_t *pre, uint32_t n,
+                                                     uint64_t *total) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint64_t b0 = (uint64_t)threadIdx.x * per;
+    uint32_t s = 0;
+#pragma unroll 8
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint64_t i = b0 + k;
+        if (i < n) s += cnt[i];
+    }
+    uint32_t base = block_excl_scan(s, s_wave);
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint64_t i = b0 + k;
+        if (i < n) {
+            const uint32_t x = cnt[i];
+            pre[i] = base;
+            base += x;
+        }
+    }
+    if (threadIdx.x == 1023) *total = base;
+}
+
 // ---------------------------------------------------------------------------
 // 6. outputs
 // ---------------------------------------------------------------------------
+// The lane key of the group whose compressed key is at gk[s].  One 64-bit
+// key column: the sort lane's encoding back to the raw key (special 2: the
+// null key, 1: the key equal to the empty marker).
+__device__ __forceinline__ void group_key(const SLaunch *__restrict__ L, uint64_t s, bool two, uint64_t &lo,
+                                          uint64_t &hi, int &special) {
+    decompress(L, gld<uint64_t>(L->gk_lo, s), two ? gld<uint64_t>(L->gk_hi, s) : 0, lo, hi);
+    special = 0;
+    if (L->q.key_mode == KM_RAW64) {
+        if (lo & 1) {
+            special = 2;
+            lo = 0;
+        } else {
+            lo = (lo >> 1) | (hi << 63);
+            special = lo == kEmpty ? 1 : 0;
+        }
+        hi = 0;
+    }
+}
+
 __global__ __launch_bounds__(256) void so_emit(const SLaunch *__restrict__ L) {
     const QueryDesc *__restrict__ q = &L->q;
     const uint64_t ng = L->ctr[SC_GROUPS];
@@ -1442,17 +1764,29 @@ __global__ __launch_bounds__(256) void so_emit(const SLaunch *__restrict__ L) {
     const GReader rd{L->arr};
     for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < ng; gi += (uint64_t)gridDim.x * blockDim.x) {
         uint64_t lo, hi;
-        decompress(L, gld<uint64_t>(L->gk_lo, gi), two ? gld<uint64_t>(L->gk_hi, gi) : 0, lo, hi);
-        int special = 0;
-        if (q->key_mode == KM_RAW64) {
-            if (lo & 1) { special = 2; lo = 0; }
-            else {
-                lo = (lo >> 1) | (hi << 63);
-                special = lo == kEmpty ? 1 : 0;
-            }
-            hi = 0;
-        }
+        int special;
+        group_key(L, gi, two, lo, hi, special);
         emit_group(q, L->out, L->n_out, rd, gi, lo, hi, special, gi);
+    }
+}
+
+// Outputs of the bucket fold: bucket b's groups sit at sparse index
+// bstart[b] + j and go to output row bpre[b] + j (bucket order is key order).
+__global__ __launch_bounds__(256) void so_emit_buckets(const SLaunch *__restrict__ L, const uint64_t *bstart,
+                                                     const uint32_t *bg, const uint32_t *bpre, uint32_t nb) {
+    const QueryDesc *__restrict__ q = &L->q;
+    const bool two = L->kw == 2;
+    const GReader rd{L->arr};
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const uint32_t cnt = bg[b];
+        const uint64_t s0 = bstart[b], o0 = bpre[b];
+        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+            const uint64_t s = s0 + j;
+            uint64_t lo, hi;
+            int special;
+            group_key(L, s, two, lo, hi, special);
+            emit_group(q, L->out, L->n_out, rd, s, lo, hi, special, o0 + j);
+        }
     }
 }
 
@@ -1594,14 +1928,19 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         t = std::max(t, kDigitBits);
         if ((t + kDigitBits - 1) / kDigitBits < full_passes) tbits = t;
     }
+    // the top-bit passes for tbits, or the whole key's
     std::vector<int> shifts;
-    if (tbits) {
-        const int np = (tbits + kDigitBits - 1) / kDigitBits;
-        for (int p = 0; p < np; ++p)
-            shifts.push_back(std::min<int>((int)width - tbits + p * kDigitBits, (int)width - kDigitBits));
-    } else {
-        for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
-    }
+    auto plan_shifts = [&]() {
+        shifts.clear();
+        if (tbits) {
+            const int np = (tbits + kDigitBits - 1) / kDigitBits;
+            for (int p = 0; p < np; ++p)
+                shifts.push_back(std::min<int>((int)width - tbits + p * kDigitBits, (int)width - kDigitBits));
+        } else {
+            for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
+        }
+    };
+    plan_shifts();
     int passes = (int)shifts.size();
 
     // ---- buffers
@@ -1626,6 +1965,23 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     L.digit_tot = (uint32_t *)slab("so:digit_tot", kDigits * 4 + 64);
     L.tile_cnt = (uint32_t *)slab("so:tile_cnt", ((uint64_t)L.ntiles + 1) * 4 + 64);
     PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
+
+    // group state and compressed keys for slots group slots
+    auto group_state = [&](uint64_t slots) {
+        L.gcap = std::max<uint64_t>(slots, 1);
+        L.gk_lo = (uint64_t *)slab("so:gk_lo", L.gcap * 8 + 64);
+        L.gk_hi = (uint64_t *)slab("so:gk_hi", L.gcap * 8 + 64);
+        uint8_t *hb = (uint8_t *)slab("so:state", state_bytes(P, L.gcap) + 4096);
+        size_t o = 0;
+        carve_state(P, hb, o, L.gcap, L.arr);
+    };
+    bool exact = false;  // some aggregated column keeps exact f64 sums (limbs and residual)
+    for (int j = 0; j < P.q.n_vals; ++j)
+        if (P.q.vals[j].kind == VK_F && (P.q.vals[j].need & NEED_SUM)) exact = true;
+    bool fused = false;  // the bucket fold ran: group state at sparse indices, emitted by bucket
+    uint64_t *bstart = nullptr;
+    uint32_t *bg = nullptr, *bpre = nullptr;
+    uint32_t nb = 0;
 
     // ---- 2. records, 3. radix passes
     const int pgrid = cus * 2;
@@ -1681,48 +2037,88 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             const double expect = (double)S * bucket_cap(L.kw == 2) / (double)n_sel;  // sampled rows of a full bucket
             if ((double)pc[SC_BIG] > std::max(16.0, 4.0 * expect)) {
                 tbits = 0;
-                shifts.clear();
-                for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
-                passes = full_passes;
+            } else if (S >= 4 * (uint64_t(1) << tbits) && pc[SC_PROBE_D]) {
+                // a sample this dense meets every populated bucket: keys that
+                // fill only part of the top bits' range populate fewer
+                // buckets, each larger than the mean.  More top bits at the
+                // same number of passes split them (about in half per bit).
+                const int np = (tbits + kDigitBits - 1) / kDigitBits;
+                double per = (double)n_sel / (double)pc[SC_PROBE_D];
+                while (per > (double)bmean && tbits + 1 <= (int)width &&
+                       (tbits + kDigitBits) / kDigitBits == np) {
+                    ++tbits;
+                    per /= 2;
+                }
             }
+            plan_shifts();
+            passes = (int)shifts.size();
             hipLaunchKernelGGL(so_reset_big, dim3(1), dim3(64), 0, R.stream, dL);
             launch_check();
         }
         run_passes(shifts);
         if (tbits) {
             // ---- 3b. bucket finish
-            const uint32_t nb = 1u << tbits;
-            uint64_t *bstart = (uint64_t *)slab("so:bstart", ((uint64_t)nb + 1) * 8 + 64);
+            nb = 1u << tbits;
+            bstart = (uint64_t *)slab("so:bstart", ((uint64_t)nb + 1) * 8 + 64);
             {
                 KernelTimer kt("so_bucket_bounds");
                 hipLaunchKernelGGL(so_bucket_bounds, dim3((uint32_t)ceil_div((uint64_t)nb + 1, 256)), dim3(256), 0,
                                    R.stream, dL, slo, shi, bstart, nb, (int)width - tbits);
                 launch_check();
             }
-            {
+            const bool two = L.kw == 2;
+            const size_t lds = bucket_lds_bytes(two);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * (lds * 2 <= 160 * 1024 ? 2 : 1));
+            // the bins: the kBinBits key bits below the bucket's top bits
+            // (fewer when the key has fewer bits left)
+            int bin_bits = std::min<int>(kBinBits, (int)width - tbits);
+            int bin_shift = (int)width - tbits - bin_bits;
+            fused = L.fast && P.q.n_vals <= 2 && plan.opts.sort_fold != 0;
+            if (fused) {
+                // group state and keys at sparse indices: one slot per record
+                group_state(n_sel);
+                PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
+                bg = (uint32_t *)slab("so:bg", (uint64_t)nb * 4 + 64);
+                bpre = (uint32_t *)slab("so:bpre", (uint64_t)nb * 4 + 64);
+                {
+                    KernelTimer kt("so_bucket_fold");
+                    static const void *const kFold[2][2][2] = {
+                        {{(const void *)so_bucket_fold<false, false, false>,
+                          (const void *)so_bucket_fold<false, false, true>},
+                         {(const void *)so_bucket_fold<false, true, false>,
+                          (const void *)so_bucket_fold<false, true, true>}},
+                        {{(const void *)so_bucket_fold<true, false, false>,
+                          (const void *)so_bucket_fold<true, false, true>},
+                         {(const void *)so_bucket_fold<true, true, false>,
+                          (const void *)so_bucket_fold<true, true, true>}}};
+                    const void *fn = kFold[two ? 1 : 0][L.carry ? 1 : 0][exact ? 1 : 0];
+                    const size_t flds = lds;
+                    PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)flds));
+                    void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl, (void *)&bstart,
+                                    (void *)&nb, (void *)&bin_shift, (void *)&bin_bits, (void *)&bg};
+                    const uint32_t fgrid =
+                        (uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * (flds * 2 <= 160 * 1024 ? 2 : 1));
+                    PMH_HIP(hipLaunchKernel(fn, dim3(fgrid), dim3(kBucketThreads), args, flds, R.stream));
+                    launch_check();
+                }
+                hipLaunchKernelGGL(so_scan_counts, dim3(1), dim3(1024), 0, R.stream, bg, bpre, nb, L.ctr + SC_GROUPS);
+                launch_check();
+            } else {
                 KernelTimer kt("so_bucket_sort");
-                const bool two = L.kw == 2;
-                const size_t lds = (size_t)bucket_cap(two) * 2 * (two ? 20 : 12) + (size_t)kBins * 4;
                 static const void *const kBucket[2][2] = {
                     {(const void *)so_bucket_sort<false, false>, (const void *)so_bucket_sort<false, true>},
                     {(const void *)so_bucket_sort<true, false>, (const void *)so_bucket_sort<true, true>}};
                 const void *fn = kBucket[two ? 1 : 0][L.carry ? 1 : 0];
                 PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-                const uint32_t grid = (uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * (lds * 2 <= 160 * 1024 ? 2 : 1));
-                // the bins: the kBinBits key bits below the bucket's top bits
-                // (fewer when the key has fewer bits left)
-                int bin_bits = std::min<int>(kBinBits, (int)width - tbits);
-                int bin_shift = (int)width - tbits - bin_bits;
                 void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl, (void *)&dlo, (void *)&dhi,
-                                (void *)&dpl, (void *)&bstart, (void *)const_cast<uint32_t *>(&nb),
-                                (void *)&bin_shift, (void *)&bin_bits};
+                                (void *)&dpl, (void *)&bstart, (void *)&nb, (void *)&bin_shift, (void *)&bin_bits};
                 PMH_HIP(hipLaunchKernel(fn, dim3(grid), dim3(kBucketThreads), args, lds, R.stream));
                 launch_check();
+                // the sorted records are in the other buffer
+                std::swap(slo, dlo);
+                std::swap(shi, dhi);
+                std::swap(spl, dpl);
             }
-            // the sorted records are in the other buffer
-            std::swap(slo, dlo);
-            std::swap(shi, dhi);
-            std::swap(spl, dpl);
         }
         // ---- 4. group starts
         auto run_bounds = [&]() {
@@ -1731,20 +2127,25 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 hipLaunchKernelGGL(so_bound, dim3(pgrid), dim3(256), 0, R.stream, dL, slo, shi);
                 launch_check();
             }
-            hipLaunchKernelGGL(so_bscan, dim3(1), dim3(1024), 0, R.stream, dL);
+            hipLaunchKernelGGL(so_scan_counts, dim3(1), dim3(1024), 0, R.stream, L.tile_cnt, L.tile_cnt, L.ntiles,
+                               L.ctr + SC_GROUPS);
             launch_check();
         };
-        run_bounds();
+        if (!fused) run_bounds();
         PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
         PMH_HIP(hipStreamSynchronize(R.stream));
         if (c[SC_BIG]) {
             // a bucket outgrew the LDS sort (keys concentrated in a few top
             // bit patterns): the whole key by LSD passes after all, from the
-            // buffer the bucket sort read (it skipped the large buckets, so
-            // only that buffer holds every record)
-            std::swap(slo, dlo);
-            std::swap(shi, dhi);
-            std::swap(spl, dpl);
+            // buffer the bucket kernel read (the bucket sort skipped the
+            // large buckets, so only that buffer holds every record; the
+            // fold writes no records)
+            if (!fused) {
+                std::swap(slo, dlo);
+                std::swap(shi, dhi);
+                std::swap(spl, dpl);
+            }
+            fused = false;
             std::vector<int> all;
             for (int p = 0; p < full_passes; ++p) all.push_back(p * kDigitBits);
             run_passes(all);
@@ -1757,30 +2158,23 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     PMH_HIP(hipStreamSynchronize(R.stream));
     const uint64_t ng = n_sel ? c[SC_GROUPS] : 0;
 
-    // ---- group state and outputs, sized for ng
-    const uint64_t gcap = std::max<uint64_t>(ng, 1);
-    L.gcap = gcap;
-    L.gk_lo = (uint64_t *)slab("so:gk_lo", gcap * 8 + 64);
-    L.gk_hi = (uint64_t *)slab("so:gk_hi", gcap * 8 + 64);
-    {
-        uint8_t *hb = (uint8_t *)slab("so:state", state_bytes(P, gcap) + 4096);
-        size_t o = 0;
-        carve_state(P, hb, o, gcap, L.arr);
-    }
+    // ---- group state (unless the fold placed it) and outputs, sized for ng
+    if (!fused) group_state(ng);
+    const uint64_t ocap = std::max<uint64_t>(ng, 1);
     for (int oi = 0; oi < P.n_out; ++oi) {
         OutCol &oc = L.out[oi];
         oc = P.outs[oi];
         const size_t esz = std::max(1, tag_size(oc.tag));
-        oc.data = slab("so:out" + std::to_string(oi), gcap * esz + 64);
+        oc.data = slab("so:out" + std::to_string(oi), ocap * esz + 64);
         const bool need_valid = !(oc.src == OS_AGG && (oc.op == AGG_LEN || oc.op == AGG_COUNT));
-        oc.vbytes = need_valid ? (uint8_t *)slab("so:outvb" + std::to_string(oi), gcap + 64) : nullptr;
-        oc.valid = need_valid ? (uint64_t *)slab("so:outv" + std::to_string(oi), gcap / 8 + 64) : nullptr;
+        oc.vbytes = need_valid ? (uint8_t *)slab("so:outvb" + std::to_string(oi), ocap + 64) : nullptr;
+        oc.valid = need_valid ? (uint64_t *)slab("so:outv" + std::to_string(oi), ocap / 8 + 64) : nullptr;
         oc.data_bits = (oc.src == OS_KEY && oc.tag == PMH_BOOL)
-                           ? (uint8_t *)slab("so:outbits" + std::to_string(oi), gcap / 8 + 64)
+                           ? (uint8_t *)slab("so:outbits" + std::to_string(oi), ocap / 8 + 64)
                            : nullptr;
     }
     PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
-    if (ng) {
+    if (ng && !fused) {
         {
             KernelTimer kt("so_init_state");
             hipLaunchKernelGGL(so_init_state, dim3((uint32_t)std::min<uint64_t>(ceil_div(ng, 256), cus * 16)),
@@ -1789,19 +2183,26 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         }
         if (L.fast && P.q.n_vals <= 2) {
             KernelTimer kt("so_reduce_fast");
-            if (L.carry)
-                hipLaunchKernelGGL(so_reduce_fast<true>, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo, shi,
-                                   spl);
-            else
-                hipLaunchKernelGGL(so_reduce_fast<false>, dim3((L.ntiles + 3) / 4), dim3(256), 0, R.stream, dL, slo,
-                                   shi, spl);
+            static const void *const kReduce[2][2] = {
+                {(const void *)so_reduce_fast<false, false>, (const void *)so_reduce_fast<false, true>},
+                {(const void *)so_reduce_fast<true, false>, (const void *)so_reduce_fast<true, true>}};
+            const void *fn = kReduce[L.carry ? 1 : 0][exact ? 1 : 0];
+            void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl};
+            PMH_HIP(hipLaunchKernel(fn, dim3((L.ntiles + 3) / 4), dim3(256), args, 0, R.stream));
             launch_check();
         } else {
             KernelTimer kt("so_reduce");
             hipLaunchKernelGGL(so_reduce, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, spl.row);
             launch_check();
         }
-        {
+    }
+    if (ng) {
+        if (fused) {
+            KernelTimer kt("so_emit");
+            hipLaunchKernelGGL(so_emit_buckets, dim3((uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * 16)), dim3(256),
+                               0, R.stream, dL, bstart, bg, bpre, nb);
+            launch_check();
+        } else {
             KernelTimer kt("so_emit");
             hipLaunchKernelGGL(so_emit, dim3((uint32_t)std::min<uint64_t>(ceil_div(ng, 256), cus * 16)), dim3(256), 0,
                                R.stream, dL);

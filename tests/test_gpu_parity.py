@@ -684,3 +684,53 @@ def test_sort_bucket_finish(native, case):
     got_rows = run_gpu(native, dict(plan, options={"method": "sort", "sort_carry": 0}), frame)
     assert native.last_stats()["n_passes"] == st_b["n_passes"]
     assert_results_equal(got_rows, want, keys, label=f"sortr/{case}")
+    # buckets written back sorted and folded by a later pass, not in LDS
+    got_wb = run_gpu(native, dict(plan, options={"method": "sort", "sort_fold": 0}), frame)
+    assert native.last_stats()["n_passes"] == st_b["n_passes"]
+    assert_results_equal(got_wb, want, keys, label=f"sortw/{case}")
+
+
+@pytest.mark.parametrize("carry", [1, 0])
+def test_sort_bucket_fold_edges(native, carry):
+    """The bucket fold (each LDS-sorted bucket's groups folded in place and
+    stored once, with no initialised state) on the values its stores must
+    carry: exact f64 sums with values outside the limb window (+-1e300,
+    +-Inf, NaN, subnormals), signed zeros and NaN in min/max, all-null
+    groups (count 0: sum 0, mean/min/max null), null keys, and groups of
+    one row, against the oracle and the written-back fold."""
+    rng = np.random.default_rng(91 + carry)
+    n = 1_200_000
+    k = rng.integers(0, 300_000, n)
+    # huge values with normal ones, subnormals with eighths: the residual
+    # then decides the correctly rounded sum whatever the order (a group
+    # mixing 1e300, -1e300 and a subnormal would not be)
+    v = rng.standard_normal(n)
+    u = rng.random(n)
+    for lo, hi, x in ((0.000, 0.004, 1e300), (0.004, 0.008, -1e300), (0.008, 0.009, np.inf),
+                      (0.009, 0.010, -np.inf), (0.010, 0.011, np.nan), (0.011, 0.021, 0.0), (0.021, 0.031, -0.0)):
+        v[(u >= lo) & (u < hi)] = x
+    w = rng.integers(-1000, 1000, n).astype(np.float64) / 8
+    u = rng.random(n)
+    w[u < 0.01] = np.nan
+    w[(u >= 0.01) & (u < 0.02)] = 5e-324 * rng.integers(1, 50, int(((u >= 0.01) & (u < 0.02)).sum()))
+    w[(u >= 0.02) & (u < 0.025)] = -1e-310
+    # keys >= 290000 never have a valid v: all-null groups
+    vv = (rng.random(n) < 0.9) & (k < 290_000)
+    # few null keys: the null group sorts into the first bucket, which must
+    # stay within the LDS sort's capacity for the fold to run
+    frame = {"k": ("I64", k, rng.random(n) < 0.999), "v": ("F64", v, vv), "w": ("F64", w, None)}
+    aggs = [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": f"v_{op}"}
+            for op in ("Sum", "Mean", "Min", "Max", "Count")]
+    aggs += [{"kind": "Simple", "input_col": "w", "op": op, "output_alias": f"w_{op}"} for op in ("Sum", "Max", "Mean")]
+    aggs.append({"kind": "Length", "op": "Len", "output_alias": "n"})
+    plan = {"keys": [["k", "I64"]], "aggs": aggs}
+    want = orc.groupby(plan, frame)
+    print(f"sort bucket fold edges carry={carry}", flush=True)
+    got = run_gpu(native, dict(plan, options={"method": "sort", "sort_carry": carry}), frame)
+    st = native.last_stats()
+    got_wb = run_gpu(native, dict(plan, options={"method": "sort", "sort_carry": carry, "sort_fold": 0}), frame)
+    st_wb = native.last_stats()
+    assert st["method"] == st_wb["method"] == "sort" and st["n_passes"] == st_wb["n_passes"]
+    assert st["n_passes"] <= 2  # the top-bit passes and the bucket finish, not the whole key's three
+    assert_results_equal(got, want, ["k"], label=f"fold/{carry}")
+    assert_results_equal(got_wb, want, ["k"], label=f"foldwb/{carry}")

@@ -221,6 +221,8 @@ struct Options {
                                   // where that saves passes; 0 radix passes over the whole key
     int64_t sort_carry = 1;       // sort engine: 1 records carry the aggregated values (no gather through
                                   // the row in the fold) where the fast scan encodes them; 0 rows
+    int64_t sort_sample_bits = 1;  // sort engine: 1 a large query without a predicate takes its varying key bits
+                                   // from a sample (the encode checks every record), 0 a pass over every row
     int64_t sort_fold = 1;        // sort engine: 1 fold each LDS-sorted bucket's groups in place (no sorted
                                   // records written back), 0 write the bucket back and fold in a later pass
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)

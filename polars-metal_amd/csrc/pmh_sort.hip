@@ -56,7 +56,8 @@ struct Run {
 enum SortCtr : int32_t {
     SC_OR_LO = 0, SC_OR_HI, SC_AND_LO, SC_AND_HI, SC_SEL, SC_CURSOR, SC_GROUPS, SC_BIG,
     SC_PROBE_D,  // distinct top-bit patterns in the skew probe's sample
-    SC_N = 9
+    SC_MISS,     // sampled lane bits: a record has a bit outside them
+    SC_N = 10
 };
 
 // Bucket finish (MSD over the top bits, then LDS sorts): a bucket of at most
@@ -74,6 +75,8 @@ struct SLaunch {
     int32_t n_runs, pad;
     Run runs[kMaxRuns];
     uint64_t const_lo, const_hi;  // lane bits that do not vary (outside every run)
+    uint64_t vmask_lo, vmask_hi;  // lane bits inside the runs
+    int32_t sampled, pad6;        // the runs come from a sample: the encode checks every record
     uint64_t *ctr;                // SortCtr
     uint64_t n_sel, gcap;
     uint32_t ntiles, pad2;
@@ -253,6 +256,43 @@ __global__ __launch_bounds__(1024) void so_bits_final(const SLaunch *__restrict_
     }
 }
 
+// Varying lane bits from kBitsSample evenly spaced rows of a query without a
+// predicate (every row is a record; the encode checks each against them),
+// one row per thread, and the encode's per-block first record, which needs
+// no count then: the encode's block b takes 1024 rows of each full round of
+// nblk * 1024 rows and min(1024, max(0, rest - 1024 b)) of the last.  The
+// OR/AND counters start at 0 and all ones (so_reset).
+constexpr uint32_t kBitsSample = 65536;
+__global__ __launch_bounds__(1024) void so_bits_sample(const SLaunch *__restrict__ L) {
+    const QueryDesc *__restrict__ q = &L->q;
+    const uint64_t n = q->n_rows;
+    const uint32_t S = (uint32_t)min<uint64_t>(n, kBitsSample);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t olo = 0, ohi = 0, alo = ~0ull, ahi = ~0ull;
+    if (i < S) {
+        const MemRow row{q, (uint64_t)i * n / S};
+        sort_lane(q, row, olo, ohi);
+        alo = olo;
+        ahi = ohi;
+    }
+    olo = wave_or(olo);
+    ohi = wave_or(ohi);
+    alo = wave_and(alo);
+    ahi = wave_and(ahi);
+    if ((threadIdx.x & 63) == 0) {
+        atomicOr((unsigned long long *)&L->ctr[SC_OR_LO], olo);
+        atomicOr((unsigned long long *)&L->ctr[SC_OR_HI], ohi);
+        atomicAnd((unsigned long long *)&L->ctr[SC_AND_LO], alo);
+        atomicAnd((unsigned long long *)&L->ctr[SC_AND_HI], ahi);
+    }
+    if (blockIdx.x == 0) {
+        const uint64_t per = (uint64_t)L->nblk * 1024, full = n / per, rest = n % per;
+        for (uint32_t b = threadIdx.x; b < L->nblk; b += blockDim.x)
+            L->boff[b] = (uint64_t)b * full * 1024 + min<uint64_t>(rest, (uint64_t)b * 1024);
+        if (threadIdx.x == 0) L->ctr[SC_SEL] = n;
+    }
+}
+
 __global__ void so_reset_big(const SLaunch *__restrict__ L) {
     if (threadIdx.x == 0) L->ctr[SC_BIG] = 0;
 }
@@ -310,6 +350,7 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
     const uint64_t per = (uint64_t)gridDim.x * blockDim.x * U;
     const uint64_t iters = (n + per - 1) / per;  // uniform: ballots see whole waves
     uint64_t olo = 0, ohi = 0, alo = ~0ull, ahi = ~0ull;
+    uint64_t miss = 0;  // ENCODE: lane bits of records outside the runs' and the constant bits
     uint32_t cnt = 0;
     for (uint64_t it = 0; it < iters; ++it) {
         const uint64_t base = it * per + (uint64_t)blockIdx.x * blockDim.x * U + threadIdx.x;
@@ -388,6 +429,7 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
                 }
             }
             const bool s = (sel >> u) & 1;
+            if (ENCODE && s) miss |= ((lo ^ L->const_lo) & ~L->vmask_lo) | ((hi ^ L->const_hi) & ~L->vmask_hi);
             if (!ENCODE) {
                 if (s) {
                     olo |= lo;
@@ -429,6 +471,10 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
         }
     }
     if (!ENCODE) block_bits_out(L, olo, ohi, alo, ahi, cnt);
+    // runs planned from a sample: a record with a bit outside them sends
+    // the query through the exact pass
+    if (ENCODE && L->sampled && __ballot(miss != 0) && lane == 0)
+        atomicOr((unsigned long long *)&L->ctr[SC_MISS], 1ull);
 }
 
 // ---------------------------------------------------------------------------
@@ -1711,17 +1757,24 @@ __global__ __launch_bounds__(kBucketThreads, 4) void so_bucket_fold(const SLaunc
 }
 
 // Exclusive prefix of n counts (pre may be cnt: in place) and their total,
-// by one block: each thread sums a consecutive run of counts.
+// by one block: each thread sums a consecutive run of counts, a multiple of
+// four long, read as 16-byte words (cnt is 16-byte aligned).
 __global__ __launch_bounds__(1024) void so_scan_counts(const uint32_t *cnt, uint32_t *pre, uint32_t n,
                                                      uint64_t *total) {
     __shared__ uint32_t s_wave[16];
-    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t per = (((n + 1023) / 1024) + 3) & ~3u;
     const uint64_t b0 = (uint64_t)threadIdx.x * per;
     uint32_t s = 0;
-#pragma unroll 8
-    for (uint32_t k = 0; k < per; ++k) {
+#pragma unroll 4
+    for (uint32_t k = 0; k < per; k += 4) {
         const uint64_t i = b0 + k;
-        if (i < n) s += cnt[i];
+        if (i + 4 <= n) {
+            const uint4 v = *(const uint4 *)(cnt + i);
+            s += v.x + v.y + v.z + v.w;
+        } else {
+            for (uint32_t r = 0; r < 4; ++r)
+                if (i + r < n) s += cnt[i + r];
+        }
     }
     uint32_t base = block_excl_scan(s, s_wave);
     for (uint32_t k = 0; k < per; ++k) {
@@ -1772,15 +1825,17 @@ __global__ __launch_bounds__(256) void so_emit(const SLaunch *__restrict__ L) {
 
 // Outputs of the bucket fold: bucket b's groups sit at sparse index
 // bstart[b] + j and go to output row bpre[b] + j (bucket order is key order).
+// One wave per bucket, so a block has four buckets in flight.
 __global__ __launch_bounds__(256) void so_emit_buckets(const SLaunch *__restrict__ L, const uint64_t *bstart,
                                                      const uint32_t *bg, const uint32_t *bpre, uint32_t nb) {
     const QueryDesc *__restrict__ q = &L->q;
     const bool two = L->kw == 2;
     const GReader rd{L->arr};
-    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+    const uint32_t lane = threadIdx.x & 63, nw = blockDim.x / 64;
+    for (uint32_t b = blockIdx.x * nw + threadIdx.x / 64; b < nb; b += gridDim.x * nw) {
         const uint32_t cnt = bg[b];
         const uint64_t s0 = bstart[b], o0 = bpre[b];
-        for (uint32_t j = threadIdx.x; j < cnt; j += blockDim.x) {
+        for (uint32_t j = lane; j < cnt; j += 64) {
             const uint64_t s = s0 + j;
             uint64_t lo, hi;
             int special;
@@ -1878,56 +1933,38 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     const hipEvent_t ev0 = pe0.ev, ev1 = pe1.ev;
     PMH_HIP(hipEventRecord(ev0, R.stream));
 
-    // ---- 1. varying lane bits and the selected-row count
-    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
-    hipLaunchKernelGGL(so_reset, dim3(1), dim3(64), 0, R.stream, dL);
-    if (L.fast) {
-        KernelTimer kt("so_bits_fast");
-        hipLaunchKernelGGL(so_scan_fast<false>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
-        launch_check();
-    } else {
-        KernelTimer kt("so_bits");
-        hipLaunchKernelGGL(so_bits, dim3(cus * 8), dim3(256), 0, R.stream, dL);
-        launch_check();
-    }
-    hipLaunchKernelGGL(so_bits_final, dim3(1), dim3(1024), 0, R.stream, dL);
-    launch_check();
+    // ---- 1. varying lane bits and the selected-row count: exactly, by a
+    // pass over every row; or, for a large query without a predicate (every
+    // row is a record), from an evenly spaced sample, which the encode then
+    // checks every record against (a record outside it redoes the exact way)
     uint64_t c[SC_N];
-    PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipStreamSynchronize(R.stream));
-    const uint64_t n_sel = c[SC_SEL];
-    L.n_sel = n_sel;
-    plan_runs(L, c[SC_OR_LO] ^ c[SC_AND_LO], c[SC_OR_HI] ^ c[SC_AND_HI]);
-    uint64_t vlo = 0, vhi = 0;  // bits covered by the runs
-    for (int i = 0; i < L.n_runs; ++i)
-        for (uint32_t b = L.runs[i].pos; b < L.runs[i].pos + L.runs[i].len; ++b) {
-            if (b < 64) vlo |= 1ull << b;
-            else vhi |= 1ull << (b - 64);
+    auto bits_pass = [&](bool sampled) {
+        L.sampled = sampled ? 1 : 0;
+        PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
+        hipLaunchKernelGGL(so_reset, dim3(1), dim3(64), 0, R.stream, dL);
+        if (sampled) {
+            hipLaunchKernelGGL(so_bits_sample, dim3(kBitsSample / 1024), dim3(1024), 0, R.stream, dL);
+            launch_check();
+        } else {
+            if (L.fast) {
+                KernelTimer kt("so_bits_fast");
+                hipLaunchKernelGGL(so_scan_fast<false>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                launch_check();
+            } else {
+                KernelTimer kt("so_bits");
+                hipLaunchKernelGGL(so_bits, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                launch_check();
+            }
+            hipLaunchKernelGGL(so_bits_final, dim3(1), dim3(1024), 0, R.stream, dL);
+            launch_check();
         }
-    L.const_lo = c[SC_AND_LO] & ~vlo;
-    L.const_hi = c[SC_AND_HI] & ~vhi;
+        PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));
+        PMH_HIP(hipStreamSynchronize(R.stream));
+    };
+    uint64_t n_sel = 0;
     uint32_t width = 0;
-    for (int i = 0; i < L.n_runs; ++i) width += L.runs[i].len;
-    const int full_passes = (int)((width + kDigitBits - 1) / kDigitBits);
-    // the records carry the values where the fast scan encodes them and
-    // the fast fold reads them (one or two value columns), and where the
-    // scatter's staging of them fits its LDS (not two-word keys with two
-    // value columns)
-    L.carry = L.fast && P.q.n_vals >= 1 && P.q.n_vals <= 2 && !(L.kw == 2 && P.q.n_vals == 2) &&
-                      plan.opts.sort_carry != 0
-                  ? 1
-                  : 0;
-    // MSD finish: when the records split into buckets of about kBucketMean
-    // by their top tbits bits with fewer passes than the whole key needs,
-    // the passes order only those bits and LDS sorts finish each bucket
-    int tbits = 0;
-    const uint64_t bmean = L.kw == 2 ? kBucketMean / 2 : kBucketMean;
-    if (n_sel >= 2 * bmean && plan.opts.sort_buckets != 0) {
-        int t = 1;
-        while (t < 40 && (n_sel >> t) > bmean) ++t;
-        t = std::max(t, kDigitBits);
-        if ((t + kDigitBits - 1) / kDigitBits < full_passes) tbits = t;
-    }
+    int full_passes = 0, tbits = 0;
+    uint64_t bmean = 0;
     // the top-bit passes for tbits, or the whole key's
     std::vector<int> shifts;
     auto plan_shifts = [&]() {
@@ -1940,31 +1977,70 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             for (int p = 0; p < full_passes; ++p) shifts.push_back(p * kDigitBits);
         }
     };
-    plan_shifts();
-    int passes = (int)shifts.size();
-
-    // ---- buffers
-    const uint64_t cap = std::max<uint64_t>(n_sel, 1);
-    L.ntiles = (uint32_t)std::max<uint64_t>(1, ceil_div(n_sel, kSortTile));
-    L.ka_lo = (uint64_t *)slab("so:ka_lo", cap * 8 + 64);
-    L.kb_lo = (uint64_t *)slab("so:kb_lo", cap * 8 + 64);
-    L.ka_hi = L.kw == 2 ? (uint64_t *)slab("so:ka_hi", cap * 8 + 64) : nullptr;
-    L.kb_hi = L.kw == 2 ? (uint64_t *)slab("so:kb_hi", cap * 8 + 64) : nullptr;
-    if (L.carry) {
-        for (int j = 0; j < P.q.n_vals; ++j) {
-            L.va[j] = (uint64_t *)slab("so:va" + std::to_string(j), cap * 8 + 64);
-            L.vb[j] = (uint64_t *)slab("so:vb" + std::to_string(j), cap * 8 + 64);
+    // the compressed record layout, the passes and the buffers, from the bits
+    auto plan_layout = [&]() {
+        n_sel = c[SC_SEL];
+        L.n_sel = n_sel;
+        plan_runs(L, c[SC_OR_LO] ^ c[SC_AND_LO], c[SC_OR_HI] ^ c[SC_AND_HI]);
+        uint64_t vlo = 0, vhi = 0;  // bits covered by the runs
+        for (int i = 0; i < L.n_runs; ++i)
+            for (uint32_t b = L.runs[i].pos; b < L.runs[i].pos + L.runs[i].len; ++b) {
+                if (b < 64) vlo |= 1ull << b;
+                else vhi |= 1ull << (b - 64);
+            }
+        L.vmask_lo = vlo;
+        L.vmask_hi = vhi;
+        L.const_lo = c[SC_AND_LO] & ~vlo;
+        L.const_hi = c[SC_AND_HI] & ~vhi;
+        width = 0;
+        for (int i = 0; i < L.n_runs; ++i) width += L.runs[i].len;
+        full_passes = (int)((width + kDigitBits - 1) / kDigitBits);
+        // the records carry the values where the fast scan encodes them and
+        // the fast fold reads them (one or two value columns), and where the
+        // scatter's staging of them fits its LDS (not two-word keys with two
+        // value columns)
+        L.carry = L.fast && P.q.n_vals >= 1 && P.q.n_vals <= 2 && !(L.kw == 2 && P.q.n_vals == 2) &&
+                          plan.opts.sort_carry != 0
+                      ? 1
+                      : 0;
+        // MSD finish: when the records split into buckets of about kBucketMean
+        // by their top tbits bits with fewer passes than the whole key needs,
+        // the passes order only those bits and LDS sorts finish each bucket
+        tbits = 0;
+        bmean = L.kw == 2 ? kBucketMean / 2 : kBucketMean;
+        if (n_sel >= 2 * bmean && plan.opts.sort_buckets != 0) {
+            int t = 1;
+            while (t < 40 && (n_sel >> t) > bmean) ++t;
+            t = std::max(t, kDigitBits);
+            if ((t + kDigitBits - 1) / kDigitBits < full_passes) tbits = t;
         }
-        L.fa = (uint8_t *)slab("so:fa", cap + 64);
-        L.fb = (uint8_t *)slab("so:fb", cap + 64);
-    } else {
-        L.ra = (uint32_t *)slab("so:ra", cap * 4 + 64);
-        L.rb = (uint32_t *)slab("so:rb", cap * 4 + 64);
-    }
-    L.tile_hist = (uint32_t *)slab("so:tile_hist", (uint64_t)kDigits * L.ntiles * 4 + 64);
-    L.digit_tot = (uint32_t *)slab("so:digit_tot", kDigits * 4 + 64);
-    L.tile_cnt = (uint32_t *)slab("so:tile_cnt", ((uint64_t)L.ntiles + 1) * 4 + 64);
-    PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
+        plan_shifts();
+        // ---- buffers
+        const uint64_t cap = std::max<uint64_t>(n_sel, 1);
+        L.ntiles = (uint32_t)std::max<uint64_t>(1, ceil_div(n_sel, kSortTile));
+        L.ka_lo = (uint64_t *)slab("so:ka_lo", cap * 8 + 64);
+        L.kb_lo = (uint64_t *)slab("so:kb_lo", cap * 8 + 64);
+        L.ka_hi = L.kw == 2 ? (uint64_t *)slab("so:ka_hi", cap * 8 + 64) : nullptr;
+        L.kb_hi = L.kw == 2 ? (uint64_t *)slab("so:kb_hi", cap * 8 + 64) : nullptr;
+        if (L.carry) {
+            for (int j = 0; j < P.q.n_vals; ++j) {
+                L.va[j] = (uint64_t *)slab("so:va" + std::to_string(j), cap * 8 + 64);
+                L.vb[j] = (uint64_t *)slab("so:vb" + std::to_string(j), cap * 8 + 64);
+            }
+            L.fa = (uint8_t *)slab("so:fa", cap + 64);
+            L.fb = (uint8_t *)slab("so:fb", cap + 64);
+        } else {
+            L.ra = (uint32_t *)slab("so:ra", cap * 4 + 64);
+            L.rb = (uint32_t *)slab("so:rb", cap * 4 + 64);
+        }
+        L.tile_hist = (uint32_t *)slab("so:tile_hist", (uint64_t)kDigits * L.ntiles * 4 + 64);
+        L.digit_tot = (uint32_t *)slab("so:digit_tot", kDigits * 4 + 64);
+        L.tile_cnt = (uint32_t *)slab("so:tile_cnt", ((uint64_t)L.ntiles + 1) * 4 + 64);
+        PMH_HIP(hipMemcpyAsync(dL, &L, sizeof(SLaunch), hipMemcpyHostToDevice, R.stream));
+    };
+    bits_pass(L.fast && L.q.n_pred == 0 && L.q.n_rows >= (uint64_t(1) << 20) && plan.opts.sort_sample_bits != 0);
+    plan_layout();
+    int passes = (int)shifts.size();
 
     // group state and compressed keys for slots group slots
     auto group_state = [&](uint64_t slots) {
@@ -1985,17 +2061,65 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
 
     // ---- 2. records, 3. radix passes
     const int pgrid = cus * 2;
-    uint64_t *slo = L.ka_lo, *shi = L.ka_hi, *dlo = L.kb_lo, *dhi = L.kb_hi;
-    Payload spl{L.ra, {L.va[0], L.va[1]}, L.fa}, dpl{L.rb, {L.vb[0], L.vb[1]}, L.fb};
+    uint64_t *slo = nullptr, *shi = nullptr, *dlo = nullptr, *dhi = nullptr;
+    Payload spl{}, dpl{};
     if (n_sel) {
-        if (L.fast) {
-            KernelTimer kt("so_encode_fast");
-            hipLaunchKernelGGL(so_scan_fast<true>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
-            launch_check();
-        } else {
-            KernelTimer kt("so_encode");
-            hipLaunchKernelGGL(so_encode, dim3(cus * 8), dim3(256), 0, R.stream, dL);
-            launch_check();
+        for (;;) {
+            slo = L.ka_lo;
+            shi = L.ka_hi;
+            dlo = L.kb_lo;
+            dhi = L.kb_hi;
+            spl = Payload{L.ra, {L.va[0], L.va[1]}, L.fa};
+            dpl = Payload{L.rb, {L.vb[0], L.vb[1]}, L.fb};
+            if (L.fast) {
+                KernelTimer kt("so_encode_fast");
+                hipLaunchKernelGGL(so_scan_fast<true>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                launch_check();
+            } else {
+                KernelTimer kt("so_encode");
+                hipLaunchKernelGGL(so_encode, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                launch_check();
+            }
+            // skewed keys would overflow the buckets: probe a sample first
+            if (tbits) {
+                hipLaunchKernelGGL(so_bucket_probe, dim3(1), dim3(1024), 0, R.stream, dL, slo, shi, (int)width - tbits);
+                launch_check();
+            }
+            if (!tbits && !L.sampled) break;
+            uint64_t pc[SC_N];
+            PMH_HIP(hipMemcpyAsync(pc, L.ctr, sizeof(pc), hipMemcpyDeviceToHost, R.stream));
+            PMH_HIP(hipStreamSynchronize(R.stream));
+            if (L.sampled && pc[SC_MISS]) {
+                // a record with a lane bit the sample did not see vary
+                bits_pass(false);
+                plan_layout();
+                passes = (int)shifts.size();
+                continue;
+            }
+            if (tbits) {
+                const uint64_t S = std::min<uint64_t>(n_sel, kProbeSample);
+                const double expect = (double)S * bucket_cap(L.kw == 2) / (double)n_sel;  // sampled rows of a full bucket
+                if ((double)pc[SC_BIG] > std::max(16.0, 4.0 * expect)) {
+                    tbits = 0;
+                } else if (S >= 4 * (uint64_t(1) << tbits) && pc[SC_PROBE_D]) {
+                    // a sample this dense meets every populated bucket: keys that
+                    // fill only part of the top bits' range populate fewer
+                    // buckets, each larger than the mean.  More top bits at the
+                    // same number of passes split them (about in half per bit).
+                    const int np = (tbits + kDigitBits - 1) / kDigitBits;
+                    double per = (double)n_sel / (double)pc[SC_PROBE_D];
+                    while (per > (double)bmean && tbits + 1 <= (int)width &&
+                           (tbits + kDigitBits) / kDigitBits == np) {
+                        ++tbits;
+                        per /= 2;
+                    }
+                }
+                plan_shifts();
+                passes = (int)shifts.size();
+                hipLaunchKernelGGL(so_reset_big, dim3(1), dim3(64), 0, R.stream, dL);
+                launch_check();
+            }
+            break;
         }
         const size_t dyn = (L.kw == 2 ? (size_t)kSortTile * 8 : 0) +
                            (L.carry ? (size_t)kSortTile * (8 * (size_t)P.q.n_vals + 1) : (size_t)kSortTile * 4);
@@ -2026,35 +2150,6 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 std::swap(spl, dpl);
             }
         };
-        if (tbits) {
-            // skewed keys would overflow the buckets: probe a sample first
-            hipLaunchKernelGGL(so_bucket_probe, dim3(1), dim3(1024), 0, R.stream, dL, slo, shi, (int)width - tbits);
-            launch_check();
-            uint64_t pc[SC_N];
-            PMH_HIP(hipMemcpyAsync(pc, L.ctr, sizeof(pc), hipMemcpyDeviceToHost, R.stream));
-            PMH_HIP(hipStreamSynchronize(R.stream));
-            const uint64_t S = std::min<uint64_t>(n_sel, kProbeSample);
-            const double expect = (double)S * bucket_cap(L.kw == 2) / (double)n_sel;  // sampled rows of a full bucket
-            if ((double)pc[SC_BIG] > std::max(16.0, 4.0 * expect)) {
-                tbits = 0;
-            } else if (S >= 4 * (uint64_t(1) << tbits) && pc[SC_PROBE_D]) {
-                // a sample this dense meets every populated bucket: keys that
-                // fill only part of the top bits' range populate fewer
-                // buckets, each larger than the mean.  More top bits at the
-                // same number of passes split them (about in half per bit).
-                const int np = (tbits + kDigitBits - 1) / kDigitBits;
-                double per = (double)n_sel / (double)pc[SC_PROBE_D];
-                while (per > (double)bmean && tbits + 1 <= (int)width &&
-                       (tbits + kDigitBits) / kDigitBits == np) {
-                    ++tbits;
-                    per /= 2;
-                }
-            }
-            plan_shifts();
-            passes = (int)shifts.size();
-            hipLaunchKernelGGL(so_reset_big, dim3(1), dim3(64), 0, R.stream, dL);
-            launch_check();
-        }
         run_passes(shifts);
         if (tbits) {
             // ---- 3b. bucket finish
@@ -2199,7 +2294,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     if (ng) {
         if (fused) {
             KernelTimer kt("so_emit");
-            hipLaunchKernelGGL(so_emit_buckets, dim3((uint32_t)std::min<uint64_t>(nb, (uint64_t)cus * 16)), dim3(256),
+            hipLaunchKernelGGL(so_emit_buckets, dim3((uint32_t)std::min<uint64_t>(ceil_div(nb, 4), (uint64_t)cus * 16)), dim3(256),
                                0, R.stream, dL, bstart, bg, bpre, nb);
             launch_check();
         } else {
@@ -2220,8 +2315,8 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     res->n_rows = ng;
     res->stats.method = PMH_METHOD_SORT;
     res->stats.n_passes = passes;
-    res->stats.initial_capacity = cap;
-    res->stats.final_capacity = cap;
+    res->stats.initial_capacity = std::max<uint64_t>(n_sel, 1);
+    res->stats.final_capacity = std::max<uint64_t>(n_sel, 1);
     res->stats.n_groups = ng;
     res->stats.n_selected = n_sel;
     res->stats.device_ms = dev_ms;

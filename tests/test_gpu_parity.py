@@ -690,6 +690,34 @@ def test_sort_bucket_finish(native, case):
     assert_results_equal(got_wb, want, keys, label=f"sortw/{case}")
 
 
+@pytest.mark.parametrize("case", ["high_bit", "low_bit", "null_key", "none"])
+def test_sort_sampled_bits(native, case):
+    """A large query without a predicate takes its varying key bits from an
+    evenly spaced sample of 65536 rows; the encode checks every record and a
+    key with a bit the sample did not see vary (one row off the sample grid:
+    a high bit, a low bit, a null) sends the query through the exact pass.
+    Results match the oracle and the exact pass either way."""
+    rng = np.random.default_rng(["high_bit", "low_bit", "null_key", "none"].index(case) + 70)
+    n = (1 << 21) + 3
+    k = rng.integers(0, 1 << 18, n) * 2  # even keys: bit 0 of the data is constant
+    kv = np.ones(n, bool)
+    if case == "high_bit":
+        k[5] = 1 << 40
+    elif case == "low_bit":
+        k[7] += 1
+    elif case == "null_key":
+        kv[9] = False
+    frame = {"k": ("I64", k, kv), "v": ("I64", rng.integers(-1000, 1000, n), None)}
+    plan = {"keys": [["k", "I64"]], "aggs": [{"kind": "Simple", "input_col": "v", "op": op, "output_alias": op}
+                                             for op in ("Sum", "Count", "Max")]}
+    want = orc.groupby(plan, frame)
+    print(f"sort sampled bits {case}", flush=True)
+    got = run_gpu(native, dict(plan, options={"method": "sort"}), frame)
+    got_x = run_gpu(native, dict(plan, options={"method": "sort", "sort_sample_bits": 0}), frame)
+    assert_results_equal(got, want, ["k"], label=f"sampled/{case}")
+    assert_results_equal(got_x, want, ["k"], label=f"exact/{case}")
+
+
 @pytest.mark.parametrize("carry", [1, 0])
 def test_sort_bucket_fold_edges(native, carry):
     """The bucket fold (each LDS-sorted bucket's groups folded in place and

@@ -250,6 +250,19 @@ __device__ __forceinline__ uint64_t key_data(const KeyField &f, uint64_t raw) {
     return f.bits >= 64 ? d : (d & ((1ull << f.bits) - 1));
 }
 
+// key_data, and in over the value's bits beyond the field: nonzero when its
+// image lies outside a narrowed field's range [bias, bias + 2^bits).
+__device__ __forceinline__ uint64_t key_data_over(const KeyField &f, uint64_t raw, uint64_t &over) {
+    if (f.xform == KX_RAW) {
+        over = 0;
+        return raw;
+    }
+    const uint64_t d = key_image(f.xform, f.tag, raw) - f.bias;
+    const uint64_t m = f.bits >= 64 ? d : (d & ((1ull << f.bits) - 1));
+    over = d ^ m;
+    return m;
+}
+
 template <class Row>
 __device__ __forceinline__ Key load_key(const QueryDesc *__restrict__ q, const Row &row) {
     Key k{0, 0, 0};

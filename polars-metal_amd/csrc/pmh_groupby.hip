@@ -845,6 +845,13 @@ pmh_result *run_filter_groupby(const GroupByPlan &plan, uint64_t n_rows, const p
         // or its fallback table overflowed; the global-table engine (which
         // resizes) then runs the query
         pmh_result *r = run_hash_partitioned(P, plan, flags);
+        if (!r && P.narrow_miss) {
+            // a key outside the sampled narrowing ranges: narrowed from
+            // every row this time
+            P.narrow_miss = false;
+            P.narrow_exact = true;
+            r = run_hash_partitioned(P, plan, flags);
+        }
         if (r) return r;
         if (P.route == 1) {  // the learned sizer chose the sort engine
             r = run_sort(P, plan, flags);

@@ -76,7 +76,7 @@ struct Spill {
 enum StatIdx : int32_t {
     ST_SELECTED = 0, ST_TAIL = 1, ST_SAMPLE_SEL = 2, ST_SAMPLE_DISTINCT = 3, ST_SAMPLE_F1 = 4,
     ST_SAMPLE_F2 = 5, ST_HOT = 6, ST_HOT_SAMPLE_ROWS = 7, ST_TILES0 = 8, ST_SAMPLE_NULLKEY = 9,
-    ST_SAMPLE_DROPPED = 10, ST_ERROR = 11, ST_N = 16
+    ST_SAMPLE_DROPPED = 10, ST_ERROR = 11, ST_NARROW_MISS = 12, ST_N = 16
 };
 // The sizer block follows the stats (words): the sketch's features, the
 // installed model's two outputs and the plain distinct estimate (f64), so
@@ -1794,6 +1794,42 @@ __global__ __launch_bounds__(256) void hs_key_range(const QueryDesc *__restrict_
     }
 }
 
+// hs_key_range over kNarrowSample evenly spaced rows, one per thread: the
+// ranges the scan then checks every selected key against.
+constexpr uint32_t kNarrowSample = 65536;
+__global__ __launch_bounds__(1024) void hs_key_range_sample(const QueryDesc *__restrict__ q, unsigned long long *kr) {
+    const uint64_t n = q->n_rows;
+    const uint32_t S = (uint32_t)min<uint64_t>(n, kNarrowSample);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t r = i < S ? (uint64_t)i * n / S : 0;
+    for (int k = 0; k < q->n_keys; ++k) {
+        const KeyField &f = q->keys[k];
+        const DevCol &c = q->cols[f.col];
+        const int xf = tag_signed(f.tag) ? KX_SIGNED : KX_UNSIGNED;
+        uint64_t lo = ~0ull, hi = 0;
+        uint32_t nul = 0;
+        if (i < S) {
+            if (!dvalid(c.valid, r)) {
+                nul = 1;
+            } else {
+                lo = hi = key_image(xf, f.tag, dload_bits(c, r));
+            }
+        }
+#pragma unroll
+        for (int o = 32; o; o >>= 1) {
+            const uint64_t a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+            nul |= __shfl_xor(nul, o);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            if (lo != ~0ull) atomicMin(&kr[3 * k], (unsigned long long)lo);
+            if (hi) atomicMax(&kr[3 * k + 1], (unsigned long long)hi);
+            if (nul) atomicOr(&kr[3 * k + 2], 1ull);
+        }
+    }
+}
+
 __global__ void hs_key_range_init(unsigned long long *kr, int n_keys) {
     const int i = threadIdx.x;
     if (i < 3 * n_keys) kr[i] = i % 3 == 0 ? ~0ull : 0ull;
@@ -1808,8 +1844,10 @@ constexpr uint64_t kNarrowMinRows = uint64_t(1) << 20;
 // Builds in qn the lane of q narrowed to the key columns' value ranges:
 // each column's null bit only if it holds a null, its data as the image
 // minus the smallest image, in the bits the range needs.  False (qn
-// unchanged) when the narrowed lane is still wider than 64 bits.
-bool narrow_lane(const QueryDesc &q, QueryDesc &qn) {
+// unchanged) when the narrowed lane is still wider than 64 bits.  sampled:
+// the ranges of an evenly spaced sample, which the specialised scan checks
+// every selected key against (qn.narrow_check).
+bool narrow_lane(const QueryDesc &q, QueryDesc &qn, bool sampled) {
     Runtime &R = rt();
     const int nk = q.n_keys;
     auto *dq = (QueryDesc *)slab("hs:narrow_q", sizeof(QueryDesc));
@@ -1820,7 +1858,11 @@ bool narrow_lane(const QueryDesc &q, QueryDesc &qn) {
     hipLaunchKernelGGL(hs_key_range_init, dim3(1), dim3(64), 0, R.stream, kr, nk);
     PMH_HIP(hipGetLastError());
     const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
-    {
+    if (sampled) {
+        KernelTimer kt("hs_key_range");
+        hipLaunchKernelGGL(hs_key_range_sample, dim3(kNarrowSample / 1024), dim3(1024), 0, R.stream, dq, kr);
+        PMH_HIP(hipGetLastError());
+    } else {
         KernelTimer kt("hs_key_range");
         hipLaunchKernelGGL(hs_key_range, dim3(cus * 8), dim3(256), 0, R.stream, dq, kr);
         PMH_HIP(hipGetLastError());
@@ -1832,23 +1874,46 @@ bool narrow_lane(const QueryDesc &q, QueryDesc &qn) {
     for (int k = 0; k < nk; ++k) {
         KeyField f = q.keys[k];
         const uint64_t lo = pin[3 * k], hi = pin[3 * k + 1];
-        const bool nul = pin[3 * k + 2] != 0;
+        // sampled: a column with a validity bitmap keeps a null bit even if
+        // the sample saw no null
+        const bool nul = pin[3 * k + 2] != 0 || (sampled && q.cols[f.col].valid != nullptr);
         const bool any = lo <= hi;  // some valid row
-        const int w = !any || hi == lo ? 0 : 64 - __builtin_clzll(hi - lo);
+        int w = !any || hi == lo ? 0 : 64 - __builtin_clzll(hi - lo);
+        uint64_t bias = any ? lo : 0;
+        if (sampled && any) {
+            // the sampled range doubled (about its middle, inside the type's
+            // images), or the type's whole range where that is no wider:
+            // values a little beyond the sample's extremes still fit
+            const int W = tag_data_bits(f.tag);
+            const bool sgn = tag_signed(f.tag);
+            const uint64_t tmin = W >= 64 || !sgn ? 0 : (1ull << 63) - (1ull << (W - 1));
+            const uint64_t tmax = W >= 64 ? ~0ull : (sgn ? (1ull << 63) + (1ull << (W - 1)) - 1 : (1ull << W) - 1);
+            if (w + 1 >= W) {
+                w = W;
+                bias = tmin;
+            } else {
+                w += 1;
+                const uint64_t span = (1ull << w) - 1;  // the field covers [bias, bias + span]
+                bias = lo - std::min((span - (hi - lo)) / 2, lo - tmin);
+                if (bias > tmax - span) bias = tmax - span;
+            }
+        }
         f.xform = tag_signed(f.tag) ? KX_SIGNED : KX_UNSIGNED;
-        f.bias = any ? lo : 0;
+        f.bias = bias;
         f.bits = w;
         f.null_off = nul ? off++ : -1;
         f.data_off = w ? off : 0;
         off += w;
         keys[k] = f;
     }
-    if (off > 64) return false;
+    // the margins do not fit: ranges from every row instead
+    if (off > 64) return sampled ? narrow_lane(q, qn, false) : false;
     qn = q;
     for (int k = 0; k < nk; ++k) qn.keys[k] = keys[k];
     qn.total_bits = off;
     qn.key_mode = KM_LANE64;
     qn.allones_possible = off == 64;
+    qn.narrow_check = sampled ? 1 : 0;
     return true;
 }
 
@@ -1868,14 +1933,28 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     const Options &opt0 = plan.opts;
     // a wide lane whose key columns' ranges fit 64 bits runs narrow here
     // (the other engines, and this one's fallbacks, keep P.q's lane)
+    const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
+    // the specialised scan reads values as aligned 8-byte words, which needs
+    // every column naturally aligned (Arrow buffers and numpy arrays are)
+    bool aligned = true;
+    for (int c = 0; c < P.q.n_cols; ++c) {
+        const int w = tag_size(P.q.cols[c].tag);
+        if (w > 1 && ((uintptr_t)P.q.cols[c].data % (uintptr_t)w) != 0) aligned = false;
+    }
+    // the specialised scan runs for the narrowed lane (the key mode aside,
+    // the same conditions as below): its ranges may then come from a sample,
+    // which the scan checks
+    FastScan fs0{};
+    const bool fast_narrow = opt0.fast_scan != 0 && aligned && P.q.n_vals <= 2 && P.q.n_keys <= 4 &&
+                             ceil_div(n, cus) < (1ull << 25) && lower_fast_scan(P.q, fs0);
     QueryDesc qn;
     bool narrowed = false;
-    if (P.wide && n >= kNarrowMinRows && !R.scratch_mode && opt0.narrow_keys != 0) narrowed = narrow_lane(P.q, qn);
+    if (P.wide && n >= kNarrowMinRows && !R.scratch_mode && opt0.narrow_keys != 0)
+        narrowed = narrow_lane(P.q, qn, fast_narrow && opt0.narrow_keys == 1 && !P.narrow_exact);
     const QueryDesc &q0 = narrowed ? qn : P.q;
     const bool wide = narrowed ? false : P.wide;
     const int nv = q0.n_vals;
     const Options &opt = plan.opts;
-    const int cus = R.prop.multiProcessorCount > 0 ? R.prop.multiProcessorCount : 256;
     const size_t lds_max = std::min<size_t>(R.prop.sharedMemPerBlock ? R.prop.sharedMemPerBlock : 65536, 160 * 1024);
     const size_t lds_budget = lds_max - 1024;
 
@@ -1885,18 +1964,16 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     L.q = q0;
     L.wide = wide;
     L.n_out = P.n_out;
-    // the specialised scan reads values as aligned 8-byte words, which needs
-    // every column naturally aligned (Arrow buffers and numpy arrays are)
-    bool aligned = true;
-    for (int c = 0; c < q0.n_cols; ++c) {
-        const int w = tag_size(q0.cols[c].tag);
-        if (w > 1 && ((uintptr_t)q0.cols[c].data % (uintptr_t)w) != 0) aligned = false;
-    }
     if (opt.fast_scan != 0 && aligned && !wide && nv <= 2 && q0.n_keys <= 4 && ceil_div(n, cus) < (1ull << 25) &&
         (q0.key_mode == KM_RAW64 || q0.key_mode == KM_LANE64 || q0.key_mode == KM_NONE) && lower_fast_scan(q0, L.fs))
         L.fs.enabled = 1;
     else
         L.fs = FastScan{};
+    if (q0.narrow_check && !L.fs.enabled) {
+        // sampled ranges that no scan would check: narrow exactly instead
+        P.narrow_miss = true;
+        return nullptr;
+    }
     L.fs.rows = opt.scan_rows == 8 ? 8 : 4;
     size_t rec_bytes = 8 + (wide ? 8 : 0) + 2 + 1;
     for (int j = 0; j < nv; ++j) {
@@ -2092,22 +2169,26 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         const size_t l1 = std::max<size_t>(lds1, 16);
         if (L.fs.enabled) {
             KernelTimer kt("hs_scan_fast");
-            const bool raw = q0.key_mode == KM_RAW64;
+            const int km = q0.key_mode == KM_RAW64 ? kScanRaw64 : q0.narrow_check ? kScanLaneChecked : kScanLane;
             const bool r8 = L.fs.rows == 8;
-            // four rows per lane: one instantiation per aggregate signature
-            // and predicate width; eight: the fold resolved at run time
-#define PMH_SCAN4(R, N)                                                                                 \
-    {(const void *)hs_scan_fast<R, 4, 0, N>, (const void *)hs_scan_fast<R, 4, 1, N>,                     \
-     (const void *)hs_scan_fast<R, 4, 2, N>, (const void *)hs_scan_fast<R, 4, 3, N>,                     \
-     (const void *)hs_scan_fast<R, 4, 4, N>}
-            static const void *const kScan4[2][2][kNumAggSigs] = {{PMH_SCAN4(false, false), PMH_SCAN4(false, true)},
-                                                                    {PMH_SCAN4(true, false), PMH_SCAN4(true, true)}};
+            // four rows per lane: one instantiation per key kind, aggregate
+            // signature and predicate width; eight: the fold resolved at run time
+#define PMH_SCAN4(K, N)                                                                                 \
+    {(const void *)hs_scan_fast<K, 4, 0, N>, (const void *)hs_scan_fast<K, 4, 1, N>,                     \
+     (const void *)hs_scan_fast<K, 4, 2, N>, (const void *)hs_scan_fast<K, 4, 3, N>,                     \
+     (const void *)hs_scan_fast<K, 4, 4, N>}
+            static const void *const kScan4[3][2][kNumAggSigs] = {
+                {PMH_SCAN4(kScanLane, false), PMH_SCAN4(kScanLane, true)},
+                {PMH_SCAN4(kScanRaw64, false), PMH_SCAN4(kScanRaw64, true)},
+                {PMH_SCAN4(kScanLaneChecked, false), PMH_SCAN4(kScanLaneChecked, true)}};
 #undef PMH_SCAN4
+            static const void *const kScan8[3] = {(const void *)hs_scan_fast<kScanLane, 8, 0, true>,
+                                                  (const void *)hs_scan_fast<kScanRaw64, 8, 0, true>,
+                                                  (const void *)hs_scan_fast<kScanLaneChecked, 8, 0, true>};
             static_assert(kNumAggSigs == 5, "one instantiation per signature");
             const int sig = agg_signature(q0);  // the fast scan takes at most two value columns
             const bool nr2 = L.fs.nr > 1;
-            const void *fn = r8 ? (raw ? (const void *)hs_scan_fast<true, 8, 0, true> : (const void *)hs_scan_fast<false, 8, 0, true>)
-                                : kScan4[raw ? 1 : 0][nr2 ? 1 : 0][sig];
+            const void *fn = r8 ? kScan8[km] : kScan4[km][nr2 ? 1 : 0][sig];
             PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
             void *args[] = {(void *)&dL};
             PMH_HIP(hipLaunchKernel(fn, dim3(nb), dim3(1024), args, l1, R.stream));
@@ -2177,6 +2258,13 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     if (opt.distinct_hint > 0) d_est = (double)opt.distinct_hint;
     if (P.d_retry > 0) d_est = P.d_retry;
     const double d_tail = std::max(0.0, d_est - sk[ST_HOT]);
+    // a sample of near-unique keys (nine in ten seen once) estimates the
+    // distinct count from a handful of repeats, and the power-of-two
+    // partition count doubles on a 40% change of it: size the partitions for
+    // half again as many keys, since too few partitions spill their full
+    // tables into the fallback table (the slow side of the error)
+    const bool near_unique = sk[ST_SAMPLE_SEL] && (double)sk[ST_SAMPLE_F1] >= 0.9 * (double)sk[ST_SAMPLE_SEL];
+    const double d_part = near_unique && opt.distinct_hint <= 0 && P.d_retry <= 0 ? 1.5 * d_tail : d_tail;
     uint64_t P_parts;
     if (opt.partitions > 0) {
         P_parts = pow2_floor((uint64_t)opt.partitions);
@@ -2185,7 +2273,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         // runs here: estimates of near-unique samples overshoot, and keys
         // that miss a full partition table go to the fallback table; only
         // its overflow sends the query to the global-table engine
-        P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_tail / tail_cap)));
+        P_parts = pow2_ceil((uint64_t)std::max(1.0, std::ceil(d_part / tail_cap)));
         // and enough partitions for the aggregation to spread over the
         // device: about 8K tail rows (selected rows outside the hot keys,
         // projected from the sample) per partition, up to two radix passes
@@ -2213,7 +2301,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // per-partition table: about twice the expected keys per partition, so
     // the per-partition LDS reset stays proportional to the work
     if (opt.tail_slots <= 0) {
-        const double per = d_tail / (double)P_parts;
+        const double per = d_part / (double)P_parts;
         uint32_t ts = (uint32_t)std::min<uint64_t>(TS, std::max<uint64_t>(256, pow2_ceil((uint64_t)(2.5 * per) + 1)));
         L.tail_slots = ts;
     }
@@ -2356,6 +2444,14 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     if (host_stats[ST_ERROR]) {
         g_fb.clean = false;
         fail(PMH_E_COMPUTE, "polars_metal: partitioned group-by detected an inconsistent table state");
+    }
+    if (host_stats[ST_NARROW_MISS]) {
+        // a selected key outside the sampled narrowing ranges: the caller
+        // re-runs the query narrowed from every row
+        g_fb.clean = false;
+        P.narrow_miss = true;
+        collect_kernel_times();
+        return nullptr;
     }
     if (host_ctrl[1]) {
         // the fallback table overflowed and is dirty: the caller re-runs the

@@ -157,7 +157,8 @@ struct QueryDesc {
     PredNode pred[kMaxPredNodes];
     int32_t key_mode, n_keys, total_bits, allones_possible;
     KeyField keys[kMaxKeys];
-    int32_t n_vals, pad2;
+    int32_t n_vals;
+    int32_t narrow_check;  // the narrowed lane's ranges come from a sample: the scan flags keys outside them
     ValSpec vals[kMaxVals];
     Layout layout;
 };
@@ -216,7 +217,8 @@ struct Options {
     int64_t fast_scan = 1;        // partitioned engine: 1 use the specialised scan where the query fits it, 0 never
     int64_t scan_rows = 4;        // specialised scan: rows per lane per step (4 or 8)
     int64_t scan_blocks = 1;      // partitioned scan: resident blocks per CU (the hot table shares LDS)
-    int64_t narrow_keys = 1;      // partitioned engine: 1 narrow a lane wider than 64 bits to its key ranges, 0 never
+    int64_t narrow_keys = 1;      // partitioned engine: narrow a lane wider than 64 bits to its key ranges, 1 from
+                                  // a sample the specialised scan checks (else every row), 2 from every row, 0 never
     int64_t sort_buckets = 1;     // sort engine: 1 radix passes over the key's top bits, then LDS-sorted buckets
                                   // where that saves passes; 0 radix passes over the whole key
     int64_t sort_carry = 1;       // sort engine: 1 records carry the aggregated values (no gather through

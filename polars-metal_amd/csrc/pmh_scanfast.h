@@ -42,8 +42,14 @@ constexpr uint32_t kRing = 127;
 // time), so the fold of hot rows carries no branch on the plan.  NR2: the
 // predicate may test two columns (else at most one, and the second
 // column's descriptor takes no registers).
-template <bool RAW64, int U, int SIG, bool NR2>
+// KM: the key: kScanLane (a lane of at most 64 bits), kScanRaw64 (one
+// 64-bit column) or kScanLaneChecked (a lane narrowed from a sample: keys
+// outside its ranges are flagged).
+constexpr int kScanLane = 0, kScanRaw64 = 1, kScanLaneChecked = 2;
+template <int KM, int U, int SIG, bool NR2>
 __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__ L) {
+    constexpr bool RAW64 = KM == kScanRaw64;
+    constexpr bool nchk = KM == kScanLaneChecked;
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t s_cursor, s_sel;
     const QueryDesc *__restrict__ q = &L->q;
@@ -142,6 +148,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     const uint32_t steps = (span + U * bs - 1) / (U * bs);
     uint32_t qhead = 0, qn = 0;  // wave-uniform ring state
     uint32_t wave_sel = 0;
+    uint64_t nmiss = 0;  // nchk: bits of selected keys outside the sampled ranges
 
     // A step covers U * 1024 rows; wave w takes the U * 64 contiguous rows
     // from w * U * 64, 64 per sub-step, so a wave's cache lines are its own.
@@ -255,7 +262,14 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                         if (i >= nk) continue;
                         const bool ok = !kc[i].vstride || cut(kv[i], bpos(kc[i], row), 1);
                         const uint64_t raw = cut(kw[i], vpos(kc[i], row), kc[i].mask);
-                        key |= ok ? key_data(q->keys[i], raw) << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
+                        if constexpr (nchk) {
+                            uint64_t over;
+                            const uint64_t kd = key_data_over(q->keys[i], raw, over);
+                            key |= ok ? kd << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
+                            nmiss |= (ok ? over : (uint64_t)(knull[i] < 0)) & (act ? ~0ull : 0ull);
+                        } else {
+                            key |= ok ? key_data(q->keys[i], raw) << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
+                        }
                     }
                     special = (allones && key == kEmpty) ? 1 : 0;
                 }
@@ -300,6 +314,10 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
         }
     }
     if (lane == 0) atomicAdd(&s_sel, wave_sel);
+    // a selected key outside the sampled narrowing ranges: the query runs again
+    if constexpr (nchk) {
+        if (__ballot(nmiss != 0) && lane == 0) atomicOr(&L->stats[ST_NARROW_MISS], 1u);
+    }
     __syncthreads();
     if (tid == 0) {
         L->tail_n[blockIdx.x] = s_cursor;

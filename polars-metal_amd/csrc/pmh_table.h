@@ -520,6 +520,8 @@ struct Prep {
     int route = 0;                // 1: the learned sizer sent the query to the sort engine
     int sizer_choice = 0;         // pmh_exec_stats::sizer_choice
     bool overflowed = false;      // the partitioned engine's fallback table overflowed
+    bool narrow_miss = false;     // a key fell outside the sampled narrowing ranges: run again, narrowed exactly
+    bool narrow_exact = false;    // narrow from a pass over every row, not a sample
     double d_used = 0;            // the distinct estimate that run was sized for
     double d_retry = 0;           // > 0: size the partitioned engine for this many keys
 };

@@ -5,7 +5,9 @@ range needs; a null bit only for a column with nulls).  Every case is checked
 bit for bit against the oracle and against the same engine without
 narrowing; the cases cover signed and unsigned, 8- to 64-bit and float keys,
 nulls, a constant column, and a narrowed lane of exactly 64 bits whose
-all-ones value is a real key."""
+all-ones value is a real key.  Ranges come from a 65536-row sample that
+the specialised scan checks every selected key against (a key outside them
+re-runs the query with ranges from every row), or from every row."""
 import numpy as np
 import pytest
 
@@ -50,7 +52,7 @@ def test_narrowed_lane_matches_oracle(native, name):
     rng = np.random.default_rng(["signed_ranges", "small_ints", "constant_and_float", "full_64"].index(name))
     frame, plan, keys = _case(name, rng)
     want = orc.groupby(plan, frame)
-    for narrow in (1, 0):
+    for narrow in (1, 2, 0):  # ranges from a checked sample, from every row, no narrowing
         native.set_profiling(True)
         try:
             got = run_gpu(native, dict(plan, options={"method": "hash", "narrow_keys": narrow}), frame)
@@ -64,3 +66,32 @@ def test_narrowed_lane_matches_oracle(native, name):
         # narrowed, the query takes the narrow lane's specialised scan
         assert st["scan_kernel"] == ("specialised" if narrow else "generic")
         assert_results_equal(got, want, keys, label=f"{name}-narrow{narrow}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["high", "low", "null"])
+def test_sampled_ranges_miss(native, case):
+    """A key outside the sampled ranges in a row off the sample grid (above
+    them, below them, or a null where the sample saw none) sends the query
+    through ranges from every row; the result matches the oracle and the
+    exact ranges."""
+    rng = np.random.default_rng(["high", "low", "null"].index(case) + 30)
+    n = (1 << 21) + 3
+    a = rng.integers(0, 1000, n).astype(np.int32)
+    b = rng.integers(1 << 40, (1 << 40) + (1 << 20), n)
+    av = None
+    if case == "high":
+        b[5] = 1 << 50
+    elif case == "low":
+        a[7] = -5
+    else:
+        av = np.ones(n, bool)
+        av[9] = False
+    frame = {"a": ("I32", a, av), "b": ("I64", b, None), "v": ("I64", rng.integers(-100, 100, n), None)}
+    plan = {"keys": [["a", "I32"], ["b", "I64"]], "aggs": AGGS}
+    want = orc.groupby(plan, frame)
+    for narrow in (1, 2):
+        print(f"sampled ranges {case} narrow={narrow}", flush=True)
+        got = run_gpu(native, dict(plan, options={"method": "hash", "narrow_keys": narrow}), frame)
+        assert native.last_stats()["method"] == "hash"
+        assert_results_equal(got, want, ["a", "b"], label=f"{case}-narrow{narrow}")

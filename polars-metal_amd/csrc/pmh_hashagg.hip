@@ -972,9 +972,9 @@ __global__ __launch_bounds__(1024) void rp_scan(const HLaunch *__restrict__ L, i
 // consecutive threads write consecutive positions of each digit run.
 __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__restrict__ L, int first,
                                                           const Spill *src_p, const Spill *dst_p, int shift) {
+    constexpr int kW = kPartThreads / 64;
     extern __shared__ __align__(16) uint8_t smem[];
-    __shared__ uint32_t wcnt[kPartThreads / 64][kRadix];
-    __shared__ uint32_t run[kRadix];       // rows of each digit staged so far
+    __shared__ uint32_t run[kRadix];       // rows of each digit in this tile
     __shared__ uint32_t dstart[kRadix];    // LDS start of each digit run
     __shared__ uint32_t gbase[kRadix];     // global start of each digit run for this tile
     __shared__ uint32_t dtot[kRadix];
@@ -995,6 +995,11 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     uint16_t *st_fl = (uint16_t *)p;
     uint8_t *st_d = (uint8_t *)(st_fl + kPartTile);
     uint8_t *st_v = st_d + kPartTile;
+    // per (row step, wave, digit): the wave's rows of that digit in that
+    // step, then (in place) the rows of that digit before them in the tile;
+    // it lives in the staging keys, which are written only after the ranks
+    uint16_t *wc = (uint16_t *)smem;
+    static_assert(kPartRows * kW * kRadix * 2 <= kPartTile * 8, "the counts fit the staged keys");
     // digit totals over all tiles -> exclusive digit bases
     {
         const uint32_t x = threadIdx.x < kRadix ? L->digit_base[threadIdx.x] : 0;
@@ -1005,9 +1010,7 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         TileSpan ts;
         tile_span(L, first, t, ntiles, total, ts);
-        if (threadIdx.x < kRadix) run[threadIdx.x] = 0;
-        __syncthreads();
-        uint32_t my_d[kPartRows], my_pos[kPartRows];
+        uint32_t my_d[kPartRows], my_pos[kPartRows], my_rank[kPartRows];
         // every row of the tile (keys, flags, the first two value columns)
         // loads before the first rank: one wait on memory per tile
         uint64_t klo[kPartRows], khi[kPartRows], v0[kPartRows], v1[kPartRows];
@@ -1022,40 +1025,46 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter(const HLaunch *__rest
             v0[k] = nv > 0 ? gload_w_nt(src.val[0], L->vw[0], s) : 0;
             v1[k] = nv > 1 ? gload_w_nt(src.val[1], L->vw[1], s) : 0;
         }
+        for (uint32_t x = threadIdx.x; x < kPartRows * kW * kRadix / 2; x += kPartThreads) ((uint32_t *)wc)[x] = 0;
+        // every step's multisplit first (lanes holding the same digit), then
+        // one count per (step, wave, digit) and one scan per digit in (step,
+        // wave) order, which is row order: the rank stays stable
 #pragma unroll
         for (int k = 0; k < kPartRows; ++k) {
             const uint32_t i = k * kPartThreads + threadIdx.x;
             const bool live = i < ts.rows;
             const uint32_t d = live ? pass_digit(klo[k], khi[k], fl[k], shift) : 0;
-            // multisplit: lanes with the same digit
             unsigned long long peers = __ballot(live);
 #pragma unroll
             for (int b = 0; b < kRadixBits; ++b) {
                 const unsigned long long m = __ballot((d >> b) & 1);
                 peers &= ((d >> b) & 1) ? m : ~m;
             }
-            const uint32_t rank = (uint32_t)__popcll(peers & lt);
-            for (uint32_t w = threadIdx.x; w < (kPartThreads / 64) * kRadix; w += kPartThreads)
-                wcnt[w / kRadix][w % kRadix] = 0;
-            __syncthreads();
-            if (live && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            // per digit: exclusive scan over waves, then advance the run
-            if (threadIdx.x < kRadix) {
-                const uint32_t dd = threadIdx.x;
-                uint32_t acc = run[dd];
-                for (int w = 0; w < kPartThreads / 64; ++w) {
-                    const uint32_t x = wcnt[w][dd];
-                    wcnt[w][dd] = acc;
+            my_d[k] = d;
+            my_rank[k] = live ? (uint32_t)__popcll(peers & lt) : 0xFFFFFFFFu;
+            my_pos[k] = live ? (uint32_t)__popcll(peers) : 0;  // the peers' count, for the leader
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k)
+            if (my_rank[k] == 0) wc[(k * kW + wave) * kRadix + my_d[k]] = (uint16_t)my_pos[k];
+        __syncthreads();
+        if (threadIdx.x < kRadix) {
+            const uint32_t dd = threadIdx.x;
+            uint32_t acc = 0;
+#pragma unroll
+            for (int k = 0; k < kPartRows; ++k)
+                for (int w = 0; w < kW; ++w) {
+                    const uint32_t x = wc[(k * kW + w) * kRadix + dd];
+                    wc[(k * kW + w) * kRadix + dd] = (uint16_t)acc;
                     acc += x;
                 }
-                run[dd] = acc;
-            }
-            __syncthreads();
-            my_d[k] = d;
-            my_pos[k] = live ? wcnt[wave][d] + rank : 0xFFFFFFFFu;
-            __syncthreads();
+            run[dd] = acc;
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k)
+            my_pos[k] = my_rank[k] != 0xFFFFFFFFu ? wc[(k * kW + wave) * kRadix + my_d[k]] + my_rank[k] : 0xFFFFFFFFu;
         // LDS start of each digit run within the tile
         {
             const uint32_t c = threadIdx.x < kRadix ? run[threadIdx.x] : 0;
@@ -1983,7 +1992,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     }
     // shapes that do not fit the LDS plans go to the global-table engine
     // the stable scatter stages a tile and keeps per-wave digit counts
-    if ((size_t)kPartTile * rec_bytes + (size_t)kRadix * 4 * (kPartThreads / 64 + 4) + 64 > lds_budget) return nullptr;
+    if ((size_t)kPartTile * rec_bytes + (size_t)kRadix * 4 * 4 + 128 > lds_budget) return nullptr;
     size_t state_per_slot = 0;
     for (int a = 0; a < q0.layout.n_arrs; ++a) state_per_slot += q0.layout.esz[a];
     const size_t key_bytes = wide ? 16 : 8;

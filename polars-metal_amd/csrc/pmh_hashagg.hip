@@ -130,7 +130,8 @@ struct HLaunch {
     uint32_t *stats;       // StatIdx, then the sizer block (SizerIdx)
     uint32_t *sketch_hist; // [1024]: sampled keys seen c times (c clipped at 1023)
     const float *sizer;    // the installed sizer's parameters, or null: no forward in the sketch block
-    int32_t lane_bits, pad4;
+    int32_t lane_bits, back_words;  // back_words: stats words the sketch hands the host
+    uint32_t *mbox;        // the host's pinned mailbox (device address): read-backs land here
     FastScan fs;
 };
 
@@ -669,6 +670,11 @@ __global__ __launch_bounds__(1024) void hs_sample_count(const HLaunch *__restric
     // (at least 32 KB), away from the histogram the sums read
     static_assert(PMH_MLP_PARAMS * 4 <= 8192 * 4, "sizer parameters fit the smallest fingerprint array");
     sketch_sizer(L, nsel, s_distinct * sc, s_f1 * sc, s_f2 * sc, s_drop * sc, s_side, hist, sc, (float *)smem);
+    // the words the host plans from go straight into its pinned mailbox
+    // (vector stores over the link; no copy on the stream)
+    __syncthreads();
+    for (uint32_t i = tid; i < (uint32_t)L->back_words; i += bs) L->mbox[i] = L->stats[i];
+    __threadfence_system();
 }
 
 // ---------------------------------------------------------------------------
@@ -1569,8 +1575,17 @@ __global__ void hs_pack(const HLaunch *__restrict__ L) {
     }
 }
 
+// The query's control words, statistics and group count into the host's
+// pinned mailbox, then the control words cleared for the next query (64
+// threads).
 __global__ void hs_clear_ctrl(const HLaunch *__restrict__ L) {
-    if (threadIdx.x < 8) L->t.ctrl[threadIdx.x] = 0;
+    const uint32_t i = threadIdx.x;
+    if (i < 8) L->mbox[i] = L->t.ctrl[i];
+    if (i < ST_N) L->mbox[8 + i] = L->stats[i];
+    if (i == 0) L->mbox[8 + ST_N] = L->out_cursor[0];
+    __threadfence_system();
+    __syncthreads();
+    if (i < 8) L->t.ctrl[i] = 0;
 }
 
 
@@ -2132,6 +2147,12 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     uint8_t *pin = (uint8_t *)pinned_host(up_bytes + (ST_N + SZ_N + 1024 + 64) * 4);
     HLaunch *up = (HLaunch *)pin;
     uint32_t *mbox = (uint32_t *)(pin + up_bytes);
+    {
+        void *dmbox = nullptr;
+        PMH_HIP(hipHostGetDevicePointer(&dmbox, mbox, 0));
+        L.mbox = (uint32_t *)dmbox;
+    }
+    L.back_words = (int32_t)(ST_N + SZ_N + (opt.sketch_only ? 1024 : 0));
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     PooledEvent pe0, pe1;
@@ -2168,8 +2189,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
     // behind it in stream order.  A query the learned sizer then routes to
     // the sort engine has run one scan for nothing (its results are unused).
     uint32_t sk[ST_N];
-    const size_t back_words = ST_N + SZ_N + (opt.sketch_only ? 1024 : 0);
-    PMH_HIP(hipMemcpyAsync(mbox, L.stats, back_words * 4, hipMemcpyDeviceToHost, R.stream));
     PooledEvent pe_sketch;
     const hipEvent_t ev_sketch = pe_sketch.ev;
     PMH_HIP(hipEventRecord(ev_sketch, R.stream));
@@ -2438,9 +2457,6 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         launch_check();
     }
     uint32_t host_ctrl[8], host_stats[ST_N], cursor = 0;
-    PMH_HIP(hipMemcpyAsync(mbox, L.t.ctrl, sizeof(host_ctrl), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipMemcpyAsync(mbox + 8, L.stats, sizeof(host_stats), hipMemcpyDeviceToHost, R.stream));
-    PMH_HIP(hipMemcpyAsync(mbox + 8 + ST_N, L.out_cursor, 4, hipMemcpyDeviceToHost, R.stream));
     hipLaunchKernelGGL(hs_clear_ctrl, dim3(1), dim3(64), 0, R.stream, dL);
     launch_check();
     PMH_HIP(hipEventRecord(ev1, R.stream));

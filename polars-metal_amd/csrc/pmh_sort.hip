@@ -341,6 +341,7 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
     }
     const bool two = L->kw == 2;
     const bool carry = ENCODE && L->carry;
+    const bool all = q->n_pred == 0;  // every row selected
     const int nv = carry ? q->n_vals : 0;
     RoleCol vc[2];
 #pragma unroll
@@ -445,14 +446,21 @@ __global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ 
                 const Run rr = s_runs[k];
                 put_bits(clo, chi, bits_at(lo, hi, rr.pos, rr.len), rr.at);
             }
-            const unsigned long long m = __ballot(s);
-            if (!m) continue;
-            const int leader = __ffsll((long long)m) - 1;
-            uint64_t b = 0;
-            if (lane == leader) b = bbase + atomicAdd(&s_cur, (uint32_t)__popcll(m));
-            b = __shfl(b, leader);
+            uint64_t pos;
+            if (all) {
+                // no predicate: every row is a record, in the block's row order
+                if (!s) continue;
+                pos = bbase + it * (uint64_t)(blockDim.x * U) + (uint64_t)u * blockDim.x + threadIdx.x;
+            } else {
+                const unsigned long long m = __ballot(s);
+                if (!m) continue;
+                const int leader = __ffsll((long long)m) - 1;
+                uint64_t b = 0;
+                if (lane == leader) b = bbase + atomicAdd(&s_cur, (uint32_t)__popcll(m));
+                b = __shfl(b, leader);
+                pos = b + (uint64_t)__popcll(m & ((1ull << lane) - 1));
+            }
             if (s) {
-                const uint64_t pos = b + (uint64_t)__popcll(m & ((1ull << lane) - 1));
                 gst<uint64_t>(L->ka_lo, pos, clo);
                 if (two) gst<uint64_t>(L->ka_hi, pos, chi);
                 if (carry) {

@@ -172,6 +172,91 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
     };
     if (steps) load_pred(0);
 
+    // one dense round: the rows (one per active lane) gathered, hashed,
+    // folded into the hot table or appended to the tail spill
+    auto dense = [&](const uint32_t row, const bool act) __attribute__((always_inline)) {
+        // gathers: every word first, then the cuts
+        // (columns without a validity bitmap load no validity word)
+        uint64_t kw[4], kv[4], vw_[2], vv_[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            kw[i] = 0;
+            kv[i] = ~0ull;
+            if (i >= nk) continue;
+            kw[i] = word_at_nt(kc[i].base, vpos(kc[i], row));
+            if (kc[i].vstride) kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
+        }
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            vw_[j] = 0;
+            vv_[j] = ~0ull;
+            if (j >= nv) continue;
+            vw_[j] = word_at_nt(vc[j].base, vpos(vc[j], row));
+            if (vc[j].vstride) vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
+        }
+        uint64_t key = 0;
+        int special = 0;
+        if (RAW64) {
+            if (kc[0].vstride && !cut(kv[0], bpos(kc[0], row), 1)) special = 2;
+            else {
+                key = kw[0];
+                special = key == kEmpty ? 1 : 0;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i >= nk) continue;
+                const bool ok = !kc[i].vstride || cut(kv[i], bpos(kc[i], row), 1);
+                const uint64_t raw = cut(kw[i], vpos(kc[i], row), kc[i].mask);
+                if constexpr (nchk) {
+                    uint64_t over;
+                    const uint64_t kd = key_data_over(q->keys[i], raw, over);
+                    key |= ok ? kd << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
+                    nmiss |= (ok ? over : (uint64_t)(knull[i] < 0)) & (act ? ~0ull : 0ull);
+                } else {
+                    key |= ok ? key_data(q->keys[i], raw) << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
+                }
+            }
+            special = (allones && key == kEmpty) ? 1 : 0;
+        }
+        uint64_t vr[2] = {0, 0};
+        uint32_t vv = 0;  // bit j: value j valid
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            if (j >= nv) continue;
+            vr[j] = cut(vw_[j], vpos(vc[j], row), vc[j].mask);
+            const bool ok = !vc[j].vstride || cut(vv_[j], bpos(vc[j], row), 1);
+            vv |= (uint32_t)ok << j;
+        }
+        // hot-table probe
+        // the all-ones and null keys fold into the hot table's side
+        // slots instead of spilling (they would otherwise pile up on
+        // one partition slot of the tail)
+        int32_t slot = -1;
+        if (act && special && CL) slot = (int32_t)hot_side_slot(special, CL);
+        if (act && special == 0 && CL) slot = hot_find(lkeys, key, hash_key(key, 0), CL);
+        // hot rows: fold into LDS
+        if (slot >= 0) lds_agg2_add_sig<SIG>(A, (uint32_t)slot, vv, vr[0], vr[1]);
+        // tail rows: wave-contiguous append
+        const bool tail = act && slot < 0;
+        const unsigned long long m = __ballot(tail);
+        if (m) {
+            const int leader = __ffsll((long long)__ballot(1)) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(&s_cursor, (uint32_t)__popcll(m));
+            base = __shfl(base, leader);
+            if (tail) {
+                const uint32_t pos = base + (uint32_t)__popcll(m & below);
+                uint16_t fl = special == 1 ? kFlagAllOnes : (special == 2 ? kFlagNull : 0);
+                fl |= (uint16_t)vv;
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (j < nv) gstore_w(sval[j], vwid[j], pos, vr[j]);
+                gst<uint64_t>(sklo, pos, key);
+                gst<uint16_t>(sflags, pos, fl);
+            }
+        }
+    };
     for (uint32_t it = 0; it <= steps; ++it) {
         const bool drain = it == steps;
         uint32_t selbits = 0;  // bit u: row u of this lane passes
@@ -203,6 +288,17 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
         }
 #pragma unroll 1
         for (int u = 0; u < (drain ? 1 : U); ++u) {
+            if constexpr (!RAW64) {
+                if (nr == 0) {
+                    // no predicate: every row is selected, so each sub-step's
+                    // 64 rows are a dense round as they stand (no ring)
+                    if (drain) break;
+                    const bool s = (selbits >> u) & 1;
+                    wave_sel += (uint32_t)__popcll(__ballot(s));
+                    dense(s ? r0 + u * 64 : 0, s);
+                    continue;
+                }
+            }
             if (!drain) {
                 const bool s = (selbits >> u) & 1;
                 const unsigned long long M = __ballot(s);
@@ -229,87 +325,7 @@ __global__ __launch_bounds__(1024) void hs_scan_fast(const HLaunch *__restrict__
                 const uint32_t row = act ? off : 0;
                 qhead = qhead + cnt >= kRing ? qhead + cnt - kRing : qhead + cnt;
                 qn -= cnt;
-                // gathers: every word first, then the cuts
-                // (columns without a validity bitmap load no validity word)
-                uint64_t kw[4], kv[4], vw_[2], vv_[2];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    kw[i] = 0;
-                    kv[i] = ~0ull;
-                    if (i >= nk) continue;
-                    kw[i] = word_at_nt(kc[i].base, vpos(kc[i], row));
-                    if (kc[i].vstride) kv[i] = word_at(kc[i].vbase, bpos(kc[i], row));
-                }
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    vw_[j] = 0;
-                    vv_[j] = ~0ull;
-                    if (j >= nv) continue;
-                    vw_[j] = word_at_nt(vc[j].base, vpos(vc[j], row));
-                    if (vc[j].vstride) vv_[j] = word_at(vc[j].vbase, bpos(vc[j], row));
-                }
-                uint64_t key = 0;
-                int special = 0;
-                if (RAW64) {
-                    if (kc[0].vstride && !cut(kv[0], bpos(kc[0], row), 1)) special = 2;
-                    else {
-                        key = kw[0];
-                        special = key == kEmpty ? 1 : 0;
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        if (i >= nk) continue;
-                        const bool ok = !kc[i].vstride || cut(kv[i], bpos(kc[i], row), 1);
-                        const uint64_t raw = cut(kw[i], vpos(kc[i], row), kc[i].mask);
-                        if constexpr (nchk) {
-                            uint64_t over;
-                            const uint64_t kd = key_data_over(q->keys[i], raw, over);
-                            key |= ok ? kd << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
-                            nmiss |= (ok ? over : (uint64_t)(knull[i] < 0)) & (act ? ~0ull : 0ull);
-                        } else {
-                            key |= ok ? key_data(q->keys[i], raw) << kdata[i] : (knull[i] >= 0 ? 1ull << knull[i] : 0);
-                        }
-                    }
-                    special = (allones && key == kEmpty) ? 1 : 0;
-                }
-                uint64_t vr[2] = {0, 0};
-                uint32_t vv = 0;  // bit j: value j valid
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    if (j >= nv) continue;
-                    vr[j] = cut(vw_[j], vpos(vc[j], row), vc[j].mask);
-                    const bool ok = !vc[j].vstride || cut(vv_[j], bpos(vc[j], row), 1);
-                    vv |= (uint32_t)ok << j;
-                }
-                // hot-table probe
-                // the all-ones and null keys fold into the hot table's side
-                // slots instead of spilling (they would otherwise pile up on
-                // one partition slot of the tail)
-                int32_t slot = -1;
-                if (act && special && CL) slot = (int32_t)hot_side_slot(special, CL);
-                if (act && special == 0 && CL) slot = hot_find(lkeys, key, hash_key(key, 0), CL);
-                // hot rows: fold into LDS
-                if (slot >= 0) lds_agg2_add_sig<SIG>(A, (uint32_t)slot, vv, vr[0], vr[1]);
-                // tail rows: wave-contiguous append
-                const bool tail = act && slot < 0;
-                const unsigned long long m = __ballot(tail);
-                if (m) {
-                    const int leader = __ffsll((long long)__ballot(1)) - 1;
-                    uint32_t base = 0;
-                    if ((int)lane == leader) base = atomicAdd(&s_cursor, (uint32_t)__popcll(m));
-                    base = __shfl(base, leader);
-                    if (tail) {
-                        const uint32_t pos = base + (uint32_t)__popcll(m & below);
-                        uint16_t fl = special == 1 ? kFlagAllOnes : (special == 2 ? kFlagNull : 0);
-                        fl |= (uint16_t)vv;
-#pragma unroll
-                        for (int j = 0; j < 2; ++j)
-                            if (j < nv) gstore_w(sval[j], vwid[j], pos, vr[j]);
-                        gst<uint64_t>(sklo, pos, key);
-                        gst<uint16_t>(sflags, pos, fl);
-                    }
-                }
+                dense(row, act);
             }
         }
     }

@@ -118,6 +118,12 @@ struct HLaunch {
     uint32_t *tile_reg;    // [max_tiles]: scan region of each first-pass tile
     uint32_t *xtot;        // one-pass partitioning: [8][kWideRadix] rows per (XCD slot, digit)
     uint64_t *xcur;        // [8][kWideRadix]: next free row of each (XCD slot, digit) sub-region
+    // two-pass layout, second pass (MSD inside the first pass's buckets)
+    uint32_t *xtot2;       // [partitions][8]: rows per (partition, XCD slot)
+    uint64_t *xcur2;       // [partitions][8]: next free row of each (partition, XCD slot) sub-region
+    uint64_t *bstart2;     // [buckets + 1]: first-pass bucket starts
+    uint32_t *tile_off2;   // [buckets + 1]: first tile of each bucket
+    uint32_t *tile_reg2;   // [tiles]: bucket of each tile
     uint32_t max_tiles, pbits, sortbits, passes;
     // per-partition aggregation
     uint32_t tail_slots;
@@ -1280,6 +1286,194 @@ __global__ __launch_bounds__(kPartThreads) void rp_scatter1(const HLaunch *__res
 }
 
 // ---------------------------------------------------------------------------
+// Second pass of the two-pass layout, MSD: the first pass grouped the rows
+// by the low pbits - 7 hash bits (bucket b); this one groups each bucket by
+// the next 7 bits (d) inside it, unstably, into partition b * 128 + d.  A
+// tile lies inside one bucket (bucket-aligned tiles), so it ranks 128
+// digits, and its runs claim their place with one atomic per digit on the
+// cursor of (b, d, XCD slot) as the first pass does: runs that meet in one
+// sub-region are written through one L2.  Partitions end up contiguous in
+// (b, d) order, so their starts come from the counts (no search).
+// ---------------------------------------------------------------------------
+// Bucket starts (from the first pass), bucket-aligned tiles, and the
+// (partition, slot) counts cleared; block 0 maps tiles, every block clears.
+__global__ __launch_bounds__(1024) void rp_tiles2(const HLaunch *__restrict__ L, uint32_t nd) {
+    const uint32_t P = nd * kRadix;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P * 8; i += gridDim.x * blockDim.x) L->xtot2[i] = 0;
+    if (blockIdx.x != 0) return;
+    __shared__ uint32_t s_wave[16];
+    const uint32_t b = threadIdx.x;  // nd <= 1024
+    const uint64_t s0 = b < nd ? L->part_start[b] : 0, s1 = b < nd ? L->part_start[b + 1] : 0;
+    const uint32_t nt = b < nd ? (uint32_t)((s1 - s0 + kPartTile - 1) / kPartTile) : 0;
+    const uint32_t t0 = block_excl_scan(nt, s_wave);
+    if (b < nd) {
+        L->bstart2[b] = s0;
+        L->tile_off2[b] = t0;
+        for (uint32_t j = 0; j < nt; ++j) L->tile_reg2[t0 + j] = b;
+    }
+    if (b + 1 == nd) {
+        L->bstart2[nd] = s1;
+        L->tile_off2[nd] = t0 + nt;
+    }
+}
+
+// Tile t of the second pass: its bucket, first row and rows.
+__device__ __forceinline__ void tile2(const HLaunch *__restrict__ L, uint32_t t, uint32_t &b, uint64_t &begin,
+                                      uint32_t &rows) {
+    b = L->tile_reg2[t];
+    begin = L->bstart2[b] + (uint64_t)(t - L->tile_off2[b]) * kPartTile;
+    rows = (uint32_t)min<uint64_t>(kPartTile, L->bstart2[b + 1] - begin);
+}
+
+// Rows per (partition, XCD slot), slot-minor: xtot2[(b * 128 + d) * 8 + slot].
+__global__ __launch_bounds__(kPartThreads) void rp_hist2(const HLaunch *__restrict__ L, const Spill *src_p, uint32_t nd,
+                                                        int shift) {
+    __shared__ uint32_t hist[kRadix];
+    const Spill &src = *src_p;
+    const bool wide = L->wide;
+    const uint32_t tid = threadIdx.x, slot = blockIdx.x & 7;
+    const uint32_t ntiles = L->tile_off2[nd];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t b, rows;
+        uint64_t begin;
+        tile2(L, t, b, begin, rows);
+        if (tid < kRadix) hist[tid] = 0;
+        __syncthreads();
+        uint32_t d[kPartRows];
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + tid;
+            const uint64_t s = begin + (i < rows ? i : 0);
+            d[k] = pass_digit(gld_nt<uint64_t>(src.klo, s), wide ? gld_nt<uint64_t>(src.khi, s) : 0,
+                              gld_nt<uint16_t>(src.flags, s), shift);
+        }
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k)
+            if (k * kPartThreads + tid < rows) atomicAdd(&hist[d[k]], 1u);
+        __syncthreads();
+        if (tid < kRadix && hist[tid]) atomicAdd(&L->xtot2[((uint64_t)b * kRadix + tid) * 8 + slot], hist[tid]);
+        __syncthreads();
+    }
+}
+
+// One block: the exclusive prefix of the (partition, slot) counts in order
+// is each slot's cursor; a partition starts at its slot 0's.  Each thread
+// takes a consecutive run of entries.
+__global__ __launch_bounds__(1024) void rp_scan2(const HLaunch *__restrict__ L, uint32_t nd) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t n = nd * kRadix * 8;
+    const uint32_t per = (n + 1023) / 1024;  // a multiple of 8: n is
+    const uint32_t b0 = threadIdx.x * per;
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < per; ++k)
+        if (b0 + k < n) s += L->xtot2[b0 + k];
+    uint64_t base = block_excl_scan(s, s_wave);
+    for (uint32_t k = 0; k < per; ++k) {
+        const uint32_t i = b0 + k;
+        if (i >= n) break;
+        if ((i & 7) == 0) L->part_start[i >> 3] = base;
+        L->xcur2[i] = base;
+        base += L->xtot2[i];
+    }
+    if (threadIdx.x == 1023) L->part_start[nd * kRadix] = base;
+}
+
+__global__ __launch_bounds__(kPartThreads) void rp_scatter2(const HLaunch *__restrict__ L, const Spill *src_p,
+                                                           const Spill *dst_p, uint32_t nd, int shift) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    __shared__ uint32_t run[kRadix];     // rows of each digit in this tile
+    __shared__ uint32_t dstart[kRadix];  // LDS start of each digit run
+    __shared__ uint64_t gbase[kRadix];   // global start of each digit run for this tile
+    __shared__ uint32_t s_wave[16];
+    const Spill &src = *src_p;
+    const Spill &dst = *dst_p;
+    const QueryDesc *__restrict__ q = &L->q;
+    const int nv = q->n_vals;
+    const bool wide = L->wide;
+    const uint32_t tid = threadIdx.x, slot = blockIdx.x & 7;
+    const uint32_t ntiles = L->tile_off2[nd];
+    // LDS staging: keys, [hi], flags, digits, then value columns
+    uint64_t *st_lo = (uint64_t *)smem;
+    uint64_t *st_hi = st_lo + kPartTile;
+    uint8_t *p = (uint8_t *)(wide ? st_hi + kPartTile : st_hi);
+    uint16_t *st_fl = (uint16_t *)p;
+    uint8_t *st_d = (uint8_t *)(st_fl + kPartTile);
+    uint8_t *st_v = st_d + kPartTile;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t b, rows;
+        uint64_t begin;
+        tile2(L, t, b, begin, rows);
+        if (tid < kRadix) run[tid] = 0;
+        __syncthreads();
+        // the first two value columns load with the keys: one wait on
+        // memory per tile before ranking
+        uint64_t klo[kPartRows], khi[kPartRows], v0[kPartRows], v1[kPartRows];
+        uint16_t fl[kPartRows];
+        uint32_t dg[kPartRows], rk[kPartRows];
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            const uint32_t i = k * kPartThreads + tid;
+            const uint64_t s = begin + (i < rows ? i : 0);
+            klo[k] = gld_nt<uint64_t>(src.klo, s);
+            khi[k] = wide ? gld_nt<uint64_t>(src.khi, s) : 0;
+            fl[k] = gld_nt<uint16_t>(src.flags, s);
+            v0[k] = nv > 0 ? gload_w_nt(src.val[0], L->vw[0], s) : 0;
+            v1[k] = nv > 1 ? gload_w_nt(src.val[1], L->vw[1], s) : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            dg[k] = pass_digit(klo[k], khi[k], fl[k], shift);
+            rk[k] = k * kPartThreads + tid < rows ? atomicAdd(&run[dg[k]], 1u) : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        {
+            const uint32_t c = tid < kRadix ? run[tid] : 0;
+            const uint32_t e = block_excl_scan(c, s_wave);
+            if (tid < kRadix) {
+                dstart[tid] = e;
+                gbase[tid] = c ? atomicAdd((unsigned long long *)&L->xcur2[((uint64_t)b * kRadix + tid) * 8 + slot],
+                                           (unsigned long long)c)
+                               : 0;
+            }
+        }
+        __syncthreads();
+        // stage in digit order
+#pragma unroll
+        for (int k = 0; k < kPartRows; ++k) {
+            if (rk[k] == 0xFFFFFFFFu) continue;
+            const uint64_t s = begin + k * kPartThreads + tid;
+            const uint32_t sp = dstart[dg[k]] + rk[k];
+            st_lo[sp] = klo[k];
+            if (wide) st_hi[sp] = khi[k];
+            st_fl[sp] = fl[k];
+            st_d[sp] = (uint8_t)dg[k];
+            uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                store_w(vp, w, sp, j == 0 ? v0[k] : j == 1 ? v1[k] : gload_w(src.val[j], w, s));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+        // write runs: consecutive staged rows -> consecutive positions
+        for (uint32_t sp = tid; sp < rows; sp += kPartThreads) {
+            const uint32_t d = st_d[sp];
+            const uint64_t g = gbase[d] + (sp - dstart[d]);
+            gst<uint64_t>(dst.klo, g, st_lo[sp]);
+            if (wide) gst<uint64_t>(dst.khi, g, st_hi[sp]);
+            gst<uint16_t>(dst.flags, g, st_fl[sp]);
+            const uint8_t *vp = st_v;
+            for (int j = 0; j < nv; ++j) {
+                const int w = L->vw[j];
+                gstore_w(dst.val[j], w, g, load_w(vp, w, sp));
+                vp += (size_t)w * kPartTile;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // per-partition aggregation
 // ---------------------------------------------------------------------------
 // The passes order rows by the low `sortbits` bits of the hash; a partition
@@ -2343,6 +2537,24 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         L.live_off = with <= lds_budget && per_cu(with) >= per_cu(lds3b) ? (uint32_t)off : 0;
         if (L.live_off) lds3b = with;
     }
+    if (two_pass) {
+        // the second pass's counts, cursors and bucket-aligned tiles
+        const uint32_t nd = 1u << (pbits - kRadixBits);
+        const uint64_t P2 = (uint64_t)nd * kRadix;
+        const uint64_t max_t2 = ceil_div(n, kPartTile) + nd + 1;
+        uint8_t *hb = (uint8_t *)slab("hs:pass2", P2 * 8 * 12 + (nd + 1) * 12 + max_t2 * 4 + 2048);
+        size_t o = 0;
+        auto carve2 = [&](size_t bytes) {
+            void *c = hb + o;
+            o += (bytes + 255) & ~size_t(255);
+            return c;
+        };
+        L.xtot2 = (uint32_t *)carve2(P2 * 8 * 4);
+        L.xcur2 = (uint64_t *)carve2(P2 * 8 * 8);
+        L.bstart2 = (uint64_t *)carve2((size_t)(nd + 1) * 8);
+        L.tile_off2 = (uint32_t *)carve2((size_t)(nd + 1) * 4);
+        L.tile_reg2 = (uint32_t *)carve2(max_t2 * 4);
+    }
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
     hipLaunchKernelGGL(hs_regions, dim3(1), dim3(1024), 0, R.stream, dL);
@@ -2376,21 +2588,29 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         dst = 2;
     }
     if (two_pass) {
+        // MSD inside the first pass's buckets (nd of them)
+        const uint32_t nd = 1u << (pbits - kRadixBits);
+        const uint32_t P2 = nd * kRadix;
         const int shift = pbits - kRadixBits;
+        hipLaunchKernelGGL(rp_tiles2, dim3((uint32_t)std::min<uint64_t>(ceil_div((uint64_t)P2 * 8, 1024), cus * 4)),
+                           dim3(1024), 0, R.stream, dL, nd);
+        launch_check();
         {
             KernelTimer kt("rp_hist");
-            hipLaunchKernelGGL(rp_hist, dim3(pgrid), dim3(kPartThreads), 0, R.stream, dL, 0, bufs[src], shift);
+            hipLaunchKernelGGL(rp_hist2, dim3(pgrid), dim3(kPartThreads), 0, R.stream, dL, bufs[src], nd, shift);
             launch_check();
         }
         {
             KernelTimer kt("rp_scan");
-            hipLaunchKernelGGL(rp_scan, dim3(kRadix), dim3(1024), 0, R.stream, dL, 0);
+            hipLaunchKernelGGL(rp_scan2, dim3(1), dim3(1024), 0, R.stream, dL, nd);
             launch_check();
         }
         {
             KernelTimer kt("rp_scatter");
-            hipLaunchKernelGGL(rp_scatter, dim3(pgrid), dim3(kPartThreads), stage_lds, R.stream, dL, 0, bufs[src],
-                               bufs[dst], shift);
+            PMH_HIP(hipFuncSetAttribute((const void *)rp_scatter2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)stage_lds));
+            hipLaunchKernelGGL(rp_scatter2, dim3(pgrid), dim3(kPartThreads), stage_lds, R.stream, dL, bufs[src],
+                               bufs[dst], nd, shift);
             launch_check();
         }
         src = dst;
@@ -2417,7 +2637,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         src = dst;
         dst = dst == 1 ? 2 : 1;
     }
-    if (!one_pass) {
+    if (!one_pass && !two_pass) {
         KernelTimer kt("hs_part_bounds");
         hipLaunchKernelGGL(hs_part_bounds, dim3((uint32_t)ceil_div(P_parts + 1, 256)), dim3(256), 0, R.stream, dL,
                            bufs[src]);

@@ -124,6 +124,7 @@ struct HLaunch {
     uint64_t *bstart2;     // [buckets + 1]: first-pass bucket starts
     uint32_t *tile_off2;   // [buckets + 1]: first tile of each bucket
     uint32_t *tile_reg2;   // [tiles]: bucket of each tile
+    uint32_t *bsum2;       // [partitions / 1024]: rows per block of 1024 partitions
     uint32_t max_tiles, pbits, sortbits, passes;
     // per-partition aggregation
     uint32_t tail_slots;
@@ -1356,26 +1357,53 @@ __global__ __launch_bounds__(kPartThreads) void rp_hist2(const HLaunch *__restri
     }
 }
 
-// One block: the exclusive prefix of the (partition, slot) counts in order
-// is each slot's cursor; a partition starts at its slot 0's.  Each thread
-// takes a consecutive run of entries.
-__global__ __launch_bounds__(1024) void rp_scan2(const HLaunch *__restrict__ L, uint32_t nd) {
+// Per 1024 partitions (one thread each, its eight slot counts) the rows
+// they hold, into bsum2.
+__global__ __launch_bounds__(1024) void rp_sum2(const HLaunch *__restrict__ L, uint32_t P2) {
     __shared__ uint32_t s_wave[16];
-    const uint32_t n = nd * kRadix * 8;
-    const uint32_t per = (n + 1023) / 1024;  // a multiple of 8: n is
-    const uint32_t b0 = threadIdx.x * per;
+    const uint32_t c = blockIdx.x * 1024 + threadIdx.x;
     uint32_t s = 0;
-    for (uint32_t k = 0; k < per; ++k)
-        if (b0 + k < n) s += L->xtot2[b0 + k];
-    uint64_t base = block_excl_scan(s, s_wave);
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint32_t i = b0 + k;
-        if (i >= n) break;
-        if ((i & 7) == 0) L->part_start[i >> 3] = base;
-        L->xcur2[i] = base;
-        base += L->xtot2[i];
+    if (c < P2) {
+        const uint4 a = *(const uint4 *)(L->xtot2 + (uint64_t)c * 8);
+        const uint4 b = *(const uint4 *)(L->xtot2 + (uint64_t)c * 8 + 4);
+        s = a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
     }
-    if (threadIdx.x == 1023) L->part_start[nd * kRadix] = base;
+    const uint32_t e = block_excl_scan(s, s_wave);
+    if (threadIdx.x == 1023) L->bsum2[blockIdx.x] = e + s;
+}
+
+// Each block of 1024 partitions: its first row (the blocks before it, by
+// one wave), then each partition's start and its slots' cursors (the
+// exclusive prefix of the (partition, slot) counts in order).
+__global__ __launch_bounds__(1024) void rp_scan2(const HLaunch *__restrict__ L, uint32_t P2) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint64_t s_off;
+    if (threadIdx.x < 64) {
+        uint64_t o = 0;
+        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 64) o += L->bsum2[j];
+#pragma unroll
+        for (int k = 32; k; k >>= 1) o += __shfl_xor(o, k);
+        if (threadIdx.x == 0) s_off = o;
+    }
+    __syncthreads();
+    const uint32_t c = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t cnt[8], s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        cnt[k] = c < P2 ? L->xtot2[(uint64_t)c * 8 + k] : 0;
+        s += cnt[k];
+    }
+    const uint64_t base = s_off + block_excl_scan(s, s_wave);
+    if (c < P2) {
+        L->part_start[c] = base;
+        uint64_t o = base;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            L->xcur2[(uint64_t)c * 8 + k] = o;
+            o += cnt[k];
+        }
+    }
+    if (c + 1 == P2) L->part_start[P2] = base + s;
 }
 
 __global__ __launch_bounds__(kPartThreads) void rp_scatter2(const HLaunch *__restrict__ L, const Spill *src_p,
@@ -2542,7 +2570,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         const uint32_t nd = 1u << (pbits - kRadixBits);
         const uint64_t P2 = (uint64_t)nd * kRadix;
         const uint64_t max_t2 = ceil_div(n, kPartTile) + nd + 1;
-        uint8_t *hb = (uint8_t *)slab("hs:pass2", P2 * 8 * 12 + (nd + 1) * 12 + max_t2 * 4 + 2048);
+        uint8_t *hb = (uint8_t *)slab("hs:pass2", P2 * 8 * 12 + (nd + 1) * 12 + max_t2 * 4 + P2 / 256 + 2048);
         size_t o = 0;
         auto carve2 = [&](size_t bytes) {
             void *c = hb + o;
@@ -2554,6 +2582,7 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         L.bstart2 = (uint64_t *)carve2((size_t)(nd + 1) * 8);
         L.tile_off2 = (uint32_t *)carve2((size_t)(nd + 1) * 4);
         L.tile_reg2 = (uint32_t *)carve2(max_t2 * 4);
+        L.bsum2 = (uint32_t *)carve2((size_t)ceil_div(P2, 1024) * 4);
     }
     memcpy(up, &L, sizeof(HLaunch));
     PMH_HIP(hipMemcpyAsync(dL, up, sizeof(HLaunch), hipMemcpyHostToDevice, R.stream));
@@ -2602,7 +2631,10 @@ pmh_result *run_hash_partitioned(Prep &P, const GroupByPlan &plan, uint32_t flag
         }
         {
             KernelTimer kt("rp_scan");
-            hipLaunchKernelGGL(rp_scan2, dim3(1), dim3(1024), 0, R.stream, dL, nd);
+            const uint32_t sb = (uint32_t)ceil_div(P2, 1024);
+            hipLaunchKernelGGL(rp_sum2, dim3(sb), dim3(1024), 0, R.stream, dL, P2);
+            launch_check();
+            hipLaunchKernelGGL(rp_scan2, dim3(sb), dim3(1024), 0, R.stream, dL, P2);
             launch_check();
         }
         {

@@ -227,6 +227,8 @@ struct Options {
                                    // from a sample (the encode checks every record), 0 a pass over every row
     int64_t sort_fold = 1;        // sort engine: 1 fold each LDS-sorted bucket's groups in place (no sorted
                                   // records written back), 0 write the bucket back and fold in a later pass
+    int64_t sort_msd = 1;         // sort engine: 1 the top-bit passes (at most two) run most significant digit
+                                  // first, unstable and merged per XCD; 0 stable LSD passes
     bool sketch_only = false;     // partitioned engine: stop after the sample sketch (feature extraction)
 };
 

@@ -415,6 +415,7 @@ GroupByPlan parse_groupby_plan(const std::string &json, bool allow_predicate) {
                 else if (kv.first == "sort_carry") plan.opts.sort_carry = v.i;
                 else if (kv.first == "sort_fold") plan.opts.sort_fold = v.i;
                 else if (kv.first == "sort_sample_bits") plan.opts.sort_sample_bits = v.i;
+                else if (kv.first == "sort_msd") plan.opts.sort_msd = v.i;
                 else parse_err("unknown option " + kv.first);
             }
         }

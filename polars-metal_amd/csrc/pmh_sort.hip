@@ -836,6 +836,276 @@ __global__ __launch_bounds__(256) void so_bucket_bounds(const SLaunch *__restric
     }
 }
 
+// MSD top-bit passes of the bucket finish (unstable: the bucket fold orders
+// each bucket itself).  The first pass groups the records by the top
+// tbits - 8 key bits, the second groups each of those buckets by the next 8
+// bits inside it; bucket b of the fold is then contiguous, in key order, and
+// its start comes from the counts.  A tile ranks its records by LDS
+// counters and its runs claim their place with one atomic per digit on the
+// cursor of (digit, XCD slot): the grid is a multiple of 8 and block k takes
+// tiles k, k + grid, ..., so runs that meet in one sub-region are written
+// through one L2.  Second-pass tiles lie inside one first-pass bucket.
+struct MsdPass {
+    int shift;               // the digit: key bits [shift, shift + bits)
+    uint32_t nd;             // digits (at most kDigits)
+    uint32_t second;         // second pass: tiles inside the buckets below
+    uint32_t pad;
+    const uint64_t *bst;     // second pass: first-pass bucket starts [buckets + 1]
+    const uint32_t *toff;    // second pass: first tile of each bucket [buckets + 1]
+    const uint32_t *treg;    // second pass: bucket of each tile
+    uint32_t *xtot;          // [(bucket * nd + digit) * 8 + slot]: records
+    uint64_t *xcur;          // same layout: next free record of each sub-region
+    uint32_t ntiles;         // first pass: tiles of kSortTile records
+    uint32_t pad2;
+};
+
+__device__ __forceinline__ void msd_tile(const SLaunch *__restrict__ L, const MsdPass &M, uint32_t t, uint32_t &b,
+                                         uint64_t &begin, uint32_t &rows) {
+    if (M.second) {
+        b = M.treg[t];
+        begin = M.bst[b] + (uint64_t)(t - M.toff[b]) * kSortTile;
+        rows = (uint32_t)min<uint64_t>(kSortTile, M.bst[b + 1] - begin);
+    } else {
+        b = 0;
+        begin = (uint64_t)t * kSortTile;
+        rows = (uint32_t)min<uint64_t>(kSortTile, L->n_sel - begin);
+    }
+}
+
+__device__ __forceinline__ uint32_t msd_ntiles(const MsdPass &M, uint32_t nbk) {
+    return M.second ? M.toff[nbk] : M.ntiles;
+}
+
+// Records per (bucket, digit, XCD slot).  nbk: first-pass buckets (second
+// pass).  Each wave counts into its own LDS histogram; the first pass adds
+// them up over all the block's tiles (one bucket), the second per tile.
+__global__ __launch_bounds__(kSortThreads) void so_msd_hist(const SLaunch *__restrict__ L, const uint64_t *klo,
+                                                          const uint64_t *khi, const MsdPass M, uint32_t nbk) {
+    __shared__ uint32_t hist[kWaves][kDigits];
+    const bool two = L->kw == 2;
+    const uint32_t tid = threadIdx.x, slot = blockIdx.x & 7, wave = tid >> 6;
+    const uint32_t ntiles = msd_ntiles(M, nbk);
+    const uint64_t dmask = M.nd - 1;
+    auto flush = [&](uint32_t b) {
+        if (tid < M.nd) {
+            uint32_t c = 0;
+            for (int w = 0; w < kWaves; ++w) c += hist[w][tid];
+            if (c) atomicAdd(&M.xtot[((uint64_t)b * M.nd + tid) * 8 + slot], c);
+        }
+    };
+    for (uint32_t i = tid; i < kWaves * kDigits; i += kSortThreads) (&hist[0][0])[i] = 0;
+    __syncthreads();
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t b, rows;
+        uint64_t begin;
+        msd_tile(L, M, t, b, begin, rows);
+        uint32_t d[kSortRows];
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k) {
+            const uint32_t i = k * kSortThreads + tid;
+            const uint64_t s = begin + (i < rows ? i : 0);
+            d[k] = (uint32_t)(key_bits_from(gld_nt<uint64_t>(klo, s), two ? gld_nt<uint64_t>(khi, s) : 0, M.shift) &
+                              dmask);
+        }
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k)
+            if (k * kSortThreads + tid < rows) atomicAdd(&hist[wave][d[k]], 1u);
+        if (M.second) {
+            __syncthreads();
+            flush(b);
+            __syncthreads();
+            for (uint32_t i = tid; i < kWaves * kDigits; i += kSortThreads) (&hist[0][0])[i] = 0;
+            __syncthreads();
+        }
+    }
+    if (!M.second) {
+        __syncthreads();
+        flush(0);
+    }
+}
+
+// Per 1024 (bucket, digit) pairs, one thread each, the records they hold.
+__global__ __launch_bounds__(1024) void so_msd_sum(const uint32_t *xtot, uint32_t np, uint32_t *bsum) {
+    __shared__ uint32_t s_wave[16];
+    const uint32_t c = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t s = 0;
+    if (c < np) {
+        const uint4 a = *(const uint4 *)(xtot + (uint64_t)c * 8);
+        const uint4 b = *(const uint4 *)(xtot + (uint64_t)c * 8 + 4);
+        s = a.x + a.y + a.z + a.w + b.x + b.y + b.z + b.w;
+    }
+    const uint32_t e = block_excl_scan(s, s_wave);
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = e + s;
+}
+
+// Each block of 1024 pairs: its first record (the blocks before it, by one
+// wave), then each pair's start (into starts, with starts[np] the total) and
+// its slots' cursors: the exclusive prefix of the counts in order.
+__global__ __launch_bounds__(1024) void so_msd_scan(const uint32_t *xtot, uint64_t *xcur, uint32_t np,
+                                                  const uint32_t *bsum, uint64_t *starts) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint64_t s_off;
+    if (threadIdx.x < 64) {
+        uint64_t o = 0;
+        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 64) o += bsum[j];
+#pragma unroll
+        for (int k = 32; k; k >>= 1) o += __shfl_xor(o, k);
+        if (threadIdx.x == 0) s_off = o;
+    }
+    __syncthreads();
+    const uint32_t c = blockIdx.x * 1024 + threadIdx.x;
+    uint32_t cnt[8], s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        cnt[k] = c < np ? xtot[(uint64_t)c * 8 + k] : 0;
+        s += cnt[k];
+    }
+    const uint64_t base = s_off + block_excl_scan(s, s_wave);
+    if (c < np) {
+        starts[c] = base;
+        uint64_t o = base;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            xcur[(uint64_t)c * 8 + k] = o;
+            o += cnt[k];
+        }
+    }
+    if (c + 1 == np) starts[np] = base + s;
+}
+
+// The second pass's tiles inside the first pass's nbk buckets (nbk < 1024):
+// every block scans the buckets' tile counts, block 0 writes each bucket's
+// first tile, and the blocks share the map tile -> bucket and clearing the
+// second pass's nclear counts.  nbk = 0 only clears.
+__global__ __launch_bounds__(1024) void so_msd_tiles(const uint64_t *bst, uint32_t nbk, uint32_t *toff, uint32_t *treg,
+                                                   uint32_t *xtot, uint32_t nclear) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_toff[1025];
+    const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x, gn = gridDim.x * blockDim.x;
+    for (uint32_t i = gt; i < nclear; i += gn) xtot[i] = 0;
+    if (nbk == 0) return;
+    const uint32_t b = threadIdx.x;
+    const uint32_t nt = b < nbk ? (uint32_t)((bst[b + 1] - bst[b] + kSortTile - 1) / kSortTile) : 0;
+    const uint32_t t0 = block_excl_scan(nt, s_wave);
+    if (b < nbk) s_toff[b] = t0;
+    if (b + 1 == nbk) s_toff[nbk] = t0 + nt;
+    __syncthreads();
+    if (blockIdx.x == 0 && b <= nbk) toff[b] = s_toff[b];
+    const uint32_t total = s_toff[nbk];
+    for (uint32_t t = gt; t < total; t += gn) {
+        // the last bucket whose first tile is at or before t
+        uint32_t lo = 0, hi = nbk;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s_toff[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        treg[t] = lo;
+    }
+}
+
+template <bool CARRY, bool TWO>
+__global__ __launch_bounds__(kSortThreads, TWO ? 4 : 8) void so_msd_scatter(const SLaunch *__restrict__ L, const uint64_t *slo,
+                                                             const uint64_t *shi, const Payload sp_, uint64_t *dlo,
+                                                             uint64_t *dhi, const Payload dp_, const MsdPass M,
+                                                             uint32_t nbk) {
+    __shared__ uint32_t run[kDigits];     // records of each digit in this tile
+    __shared__ uint32_t dstart[kDigits];  // LDS start of each digit run
+    __shared__ uint64_t gbase[kDigits];   // global start of each digit run of this tile
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint64_t st_lo[kSortTile];
+    __shared__ uint8_t st_d[kSortTile];
+    // dynamic: [hi keys when kw == 2], then rows, or values and validity
+    extern __shared__ __align__(16) uint8_t smem[];
+    constexpr bool two = TWO;  // two-word keys
+    const int nv = CARRY ? L->q.n_vals : 0;
+    uint64_t *st_hi = (uint64_t *)smem;
+    uint8_t *dyn = smem + (two ? (size_t)kSortTile * 8 : 0);
+    uint32_t *st_row = (uint32_t *)dyn;
+    uint64_t *st_v0 = (uint64_t *)dyn;
+    uint64_t *st_v1 = st_v0 + kSortTile;
+    uint8_t *st_f = (uint8_t *)(st_v0 + (size_t)kSortTile * (nv > 1 ? 2 : 1));
+    const uint32_t tid = threadIdx.x, slot = blockIdx.x & 7;
+    const uint32_t ntiles = msd_ntiles(M, nbk);
+    const uint64_t dmask = M.nd - 1;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        uint32_t b, rows;
+        uint64_t begin;
+        msd_tile(L, M, t, b, begin, rows);
+        if (tid < kDigits) run[tid] = 0;
+        __syncthreads();
+        // every record of the tile loads before the first rank
+        uint64_t my_lo[kSortRows], my_hi[kSortRows], my_v0[kSortRows], my_v1[kSortRows];
+        uint32_t my_row[kSortRows], my_rk[kSortRows];
+        uint8_t my_f[kSortRows];
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k) {
+            const uint32_t i = k * kSortThreads + tid;
+            const uint64_t src = begin + (i < rows ? i : 0);
+            my_lo[k] = gld_nt<uint64_t>(slo, src);
+            my_hi[k] = two ? gld_nt<uint64_t>(shi, src) : 0;
+            if (CARRY) {
+                my_row[k] = 0;
+                my_v0[k] = gld_nt<uint64_t>(sp_.v[0], src);
+                my_v1[k] = nv > 1 ? gld_nt<uint64_t>(sp_.v[1], src) : 0;
+                my_f[k] = gld_nt<uint8_t>(sp_.f, src);
+            } else {
+                my_row[k] = gld_nt<uint32_t>(sp_.row, src);
+                my_v0[k] = my_v1[k] = 0;
+                my_f[k] = 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k) {
+            const uint32_t d = (uint32_t)(key_bits_from(my_lo[k], my_hi[k], M.shift) & dmask);
+            my_rk[k] = k * kSortThreads + tid < rows ? atomicAdd(&run[d], 1u) : 0xFFFFFFFFu;
+        }
+        __syncthreads();
+        {
+            const uint32_t c = tid < kDigits ? run[tid] : 0;
+            const uint32_t e = block_excl_scan(c, s_wave);
+            if (tid < kDigits) {
+                dstart[tid] = e;
+                gbase[tid] = c ? atomicAdd((unsigned long long *)&M.xcur[((uint64_t)b * M.nd + tid) * 8 + slot],
+                                           (unsigned long long)c)
+                               : 0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortRows; ++k) {
+            if (my_rk[k] == 0xFFFFFFFFu) continue;
+            const uint32_t d = (uint32_t)(key_bits_from(my_lo[k], my_hi[k], M.shift) & dmask);
+            const uint32_t sp = dstart[d] + my_rk[k];
+            st_lo[sp] = my_lo[k];
+            if (two) st_hi[sp] = my_hi[k];
+            if (CARRY) {
+                st_v0[sp] = my_v0[k];
+                if (nv > 1) st_v1[sp] = my_v1[k];
+                st_f[sp] = my_f[k];
+            } else {
+                st_row[sp] = my_row[k];
+            }
+            st_d[sp] = (uint8_t)d;
+        }
+        __syncthreads();
+        for (uint32_t sp = tid; sp < rows; sp += kSortThreads) {
+            const uint32_t d = st_d[sp];
+            const uint64_t g = gbase[d] + (sp - dstart[d]);
+            gst<uint64_t>(dlo, g, st_lo[sp]);
+            if (two) gst<uint64_t>(dhi, g, st_hi[sp]);
+            if (CARRY) {
+                gst<uint64_t>(dp_.v[0], g, st_v0[sp]);
+                if (nv > 1) gst<uint64_t>(dp_.v[1], g, st_v1[sp]);
+                gst<uint8_t>(dp_.f, g, st_f[sp]);
+            } else {
+                gst<uint32_t>(dp_.row, g, st_row[sp]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Skew probe before the top-bit passes: the largest count of one bucket
 // among up to kProbeSample evenly spaced records (any order: the records
 // are still in row order) into ctr[SC_BIG], and the number of distinct
@@ -2158,12 +2428,86 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 std::swap(spl, dpl);
             }
         };
-        run_passes(shifts);
-        if (tbits) {
-            // ---- 3b. bucket finish
+        // MSD top-bit passes (tbits <= 16): the first over the top tbits - 8
+        // bits, the second over the next 8 inside each first-pass bucket, whose
+        // starts are then the buckets' starts
+        const bool msd = tbits && tbits <= 2 * kDigitBits && plan.opts.sort_msd != 0;
+        auto run_msd = [&]() {
             nb = 1u << tbits;
             bstart = (uint64_t *)slab("so:bstart", ((uint64_t)nb + 1) * 8 + 64);
-            {
+            const bool one = tbits <= kDigitBits;
+            const uint32_t nd1 = one ? kDigits : 1u << (tbits - kDigitBits);
+            uint32_t *xtot = (uint32_t *)slab("so:xtot", (uint64_t)nb * 8 * 4 + 64);
+            uint64_t *xcur = (uint64_t *)slab("so:xcur", (uint64_t)nb * 8 * 8 + 64);
+            uint32_t *bsum = (uint32_t *)slab("so:bsum", ceil_div((uint64_t)nb, 1024) * 4 + 64);
+            uint64_t *st1 = one ? bstart : (uint64_t *)slab("so:st1", ((uint64_t)nd1 + 1) * 8 + 64);
+            uint32_t *toff = (uint32_t *)slab("so:toff", ((uint64_t)nd1 + 1) * 4 + 64);
+            uint32_t *treg = (uint32_t *)slab("so:treg", ((uint64_t)L.ntiles + nd1 + 1) * 4 + 64);
+            static const void *const kScatter[2][2] = {
+                {(const void *)so_msd_scatter<false, false>, (const void *)so_msd_scatter<false, true>},
+                {(const void *)so_msd_scatter<true, false>, (const void *)so_msd_scatter<true, true>}};
+            const void *fn = kScatter[L.carry ? 1 : 0][L.kw == 2 ? 1 : 0];
+            PMH_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn));
+            auto pass = [&](MsdPass M, uint32_t nbk, uint64_t *starts) {
+                const uint32_t np = nbk ? nbk * M.nd : M.nd;
+                const uint32_t sblocks = (uint32_t)ceil_div((uint64_t)np, 1024);
+                {
+                    KernelTimer kt("so_msd_hist");
+                    hipLaunchKernelGGL(so_msd_hist, dim3(pgrid), dim3(kSortThreads), 0, R.stream, dL, slo, shi, M,
+                                       nbk);
+                    launch_check();
+                }
+                hipLaunchKernelGGL(so_msd_sum, dim3(sblocks), dim3(1024), 0, R.stream, M.xtot, np, bsum);
+                launch_check();
+                hipLaunchKernelGGL(so_msd_scan, dim3(sblocks), dim3(1024), 0, R.stream, M.xtot, M.xcur, np, bsum,
+                                   starts);
+                launch_check();
+                {
+                    KernelTimer kt("so_msd_scatter");
+                    void *args[] = {(void *)&dL, (void *)&slo, (void *)&shi, (void *)&spl, (void *)&dlo,
+                                    (void *)&dhi, (void *)&dpl, (void *)&M,   (void *)&nbk};
+                    PMH_HIP(hipLaunchKernel(fn, dim3(pgrid), dim3(kSortThreads), args, dyn, R.stream));
+                    launch_check();
+                }
+                std::swap(slo, dlo);
+                std::swap(shi, dhi);
+                std::swap(spl, dpl);
+            };
+            // pass 1: every tile of the records (the counts cleared first)
+            hipLaunchKernelGGL(so_msd_tiles, dim3(1), dim3(1024), 0, R.stream, (const uint64_t *)nullptr, 0u,
+                               (uint32_t *)nullptr, (uint32_t *)nullptr, xtot, nd1 * 8);
+            launch_check();
+            MsdPass M1{};
+            M1.shift = (int)width - (one ? kDigitBits : tbits - kDigitBits);
+            M1.nd = nd1;
+            M1.xtot = xtot;
+            M1.xcur = xcur;
+            M1.ntiles = L.ntiles;
+            pass(M1, 0, st1);
+            if (one) return;
+            // pass 2: tiles inside each first-pass bucket
+            const uint32_t nclear = nb * 8;
+            hipLaunchKernelGGL(so_msd_tiles, dim3((uint32_t)std::min<uint64_t>(1024, ceil_div(nclear, 1024))),
+                               dim3(1024), 0, R.stream, (const uint64_t *)st1, nd1, toff, treg, xtot, nclear);
+            launch_check();
+            MsdPass M2{};
+            M2.shift = (int)width - tbits;
+            M2.nd = kDigits;
+            M2.second = 1;
+            M2.bst = st1;
+            M2.toff = toff;
+            M2.treg = treg;
+            M2.xtot = xtot;
+            M2.xcur = xcur;
+            pass(M2, nd1, bstart);
+        };
+        if (msd) run_msd();
+        else run_passes(shifts);
+        if (tbits) {
+            // ---- 3b. bucket finish
+            if (!msd) {
+                nb = 1u << tbits;
+                bstart = (uint64_t *)slab("so:bstart", ((uint64_t)nb + 1) * 8 + 64);
                 KernelTimer kt("so_bucket_bounds");
                 hipLaunchKernelGGL(so_bucket_bounds, dim3((uint32_t)ceil_div((uint64_t)nb + 1, 256)), dim3(256), 0,
                                    R.stream, dL, slo, shi, bstart, nb, (int)width - tbits);

@@ -2035,35 +2035,50 @@ __global__ __launch_bounds__(kBucketThreads, 4) void so_bucket_fold(const SLaunc
 }
 
 // Exclusive prefix of n counts (pre may be cnt: in place) and their total,
-// by one block: each thread sums a consecutive run of counts, a multiple of
-// four long, read as 16-byte words (cnt is 16-byte aligned).
-__global__ __launch_bounds__(1024) void so_scan_counts(const uint32_t *cnt, uint32_t *pre, uint32_t n,
-                                                     uint64_t *total) {
+// by blocks of 4096 counts, four per thread read as one 16-byte word (cnt is
+// 16-byte aligned): so_count_sum leaves each block's sum in bsum, then
+// so_count_scan takes the sums of the blocks before its own (one wave) and
+// writes its block's prefixes; the last block writes the total.
+__device__ __forceinline__ uint4 count_quad(const uint32_t *cnt, uint32_t n, uint64_t i) {
+    if (i + 4 <= n) return *(const uint4 *)(cnt + i);
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (i < n) v.x = cnt[i];
+    if (i + 1 < n) v.y = cnt[i + 1];
+    if (i + 2 < n) v.z = cnt[i + 2];
+    return v;
+}
+
+__global__ __launch_bounds__(1024) void so_count_sum(const uint32_t *cnt, uint32_t n, uint32_t *bsum) {
     __shared__ uint32_t s_wave[16];
-    const uint32_t per = (((n + 1023) / 1024) + 3) & ~3u;
-    const uint64_t b0 = (uint64_t)threadIdx.x * per;
-    uint32_t s = 0;
-#pragma unroll 4
-    for (uint32_t k = 0; k < per; k += 4) {
-        const uint64_t i = b0 + k;
-        if (i + 4 <= n) {
-            const uint4 v = *(const uint4 *)(cnt + i);
-            s += v.x + v.y + v.z + v.w;
-        } else {
-            for (uint32_t r = 0; r < 4; ++r)
-                if (i + r < n) s += cnt[i + r];
-        }
+    const uint4 v = count_quad(cnt, n, ((uint64_t)blockIdx.x * 1024 + threadIdx.x) * 4);
+    const uint32_t s = v.x + v.y + v.z + v.w;
+    const uint32_t e = block_excl_scan(s, s_wave);
+    if (threadIdx.x == 1023) bsum[blockIdx.x] = e + s;
+}
+
+__global__ __launch_bounds__(1024) void so_count_scan(const uint32_t *cnt, uint32_t *pre, uint32_t n,
+                                                    const uint32_t *bsum, uint64_t *total) {
+    __shared__ uint32_t s_wave[16];
+    __shared__ uint32_t s_off;
+    if (threadIdx.x < 64) {
+        uint32_t o = 0;
+        for (uint32_t j = threadIdx.x; j < blockIdx.x; j += 64) o += bsum[j];
+        o = wave_sum(o);
+        if (threadIdx.x == 0) s_off = o;
     }
-    uint32_t base = block_excl_scan(s, s_wave);
-    for (uint32_t k = 0; k < per; ++k) {
-        const uint64_t i = b0 + k;
-        if (i < n) {
-            const uint32_t x = cnt[i];
-            pre[i] = base;
-            base += x;
-        }
+    __syncthreads();
+    const uint64_t i = ((uint64_t)blockIdx.x * 1024 + threadIdx.x) * 4;
+    const uint4 v = count_quad(cnt, n, i);
+    const uint32_t s = v.x + v.y + v.z + v.w;
+    const uint32_t base = s_off + block_excl_scan(s, s_wave);
+    if (i + 4 <= n) {
+        *(uint4 *)(pre + i) = make_uint4(base, base + v.x, base + v.x + v.y, base + v.x + v.y + v.z);
+    } else {
+        if (i < n) pre[i] = base;
+        if (i + 1 < n) pre[i + 1] = base + v.x;
+        if (i + 2 < n) pre[i + 2] = base + v.x + v.y;
     }
-    if (threadIdx.x == 1023) *total = base;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 1023) *total = base + s;
 }
 
 // ---------------------------------------------------------------------------
@@ -2333,6 +2348,16 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
     for (int j = 0; j < P.q.n_vals; ++j)
         if (P.q.vals[j].kind == VK_F && (P.q.vals[j].need & NEED_SUM)) exact = true;
     bool fused = false;  // the bucket fold ran: group state at sparse indices, emitted by bucket
+    // exclusive prefix of n counts and their total (into SC_GROUPS)
+    auto scan_counts = [&](const uint32_t *cnt, uint32_t *pre, uint32_t n) {
+        const uint32_t blocks = (uint32_t)std::max<uint64_t>(1, ceil_div((uint64_t)n, 4096));
+        uint32_t *bsum = (uint32_t *)slab("so:csum", (uint64_t)blocks * 4 + 64);
+        hipLaunchKernelGGL(so_count_sum, dim3(blocks), dim3(1024), 0, R.stream, cnt, n, bsum);
+        launch_check();
+        hipLaunchKernelGGL(so_count_scan, dim3(blocks), dim3(1024), 0, R.stream, cnt, pre, n, (const uint32_t *)bsum,
+                           L.ctr + SC_GROUPS);
+        launch_check();
+    };
     uint64_t *bstart = nullptr;
     uint32_t *bg = nullptr, *bpre = nullptr;
     uint32_t nb = 0;
@@ -2548,8 +2573,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                     PMH_HIP(hipLaunchKernel(fn, dim3(fgrid), dim3(kBucketThreads), args, flds, R.stream));
                     launch_check();
                 }
-                hipLaunchKernelGGL(so_scan_counts, dim3(1), dim3(1024), 0, R.stream, bg, bpre, nb, L.ctr + SC_GROUPS);
-                launch_check();
+                scan_counts(bg, bpre, nb);
             } else {
                 KernelTimer kt("so_bucket_sort");
                 static const void *const kBucket[2][2] = {
@@ -2574,9 +2598,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
                 hipLaunchKernelGGL(so_bound, dim3(pgrid), dim3(256), 0, R.stream, dL, slo, shi);
                 launch_check();
             }
-            hipLaunchKernelGGL(so_scan_counts, dim3(1), dim3(1024), 0, R.stream, L.tile_cnt, L.tile_cnt, L.ntiles,
-                               L.ctr + SC_GROUPS);
-            launch_check();
+            scan_counts(L.tile_cnt, L.tile_cnt, L.ntiles);
         };
         if (!fused) run_bounds();
         PMH_HIP(hipMemcpyAsync(c, L.ctr, sizeof(c), hipMemcpyDeviceToHost, R.stream));

@@ -303,13 +303,16 @@ __global__ void so_reset(const SLaunch *__restrict__ L) {
 
 // ---------------------------------------------------------------------------
 // 1+2, the common shape: an interval predicate (see FastScan) and at most
-// four key fields, read as aligned words (pmh_words.h), four rows per lane,
+// four key fields, read as aligned words (pmh_words.h), two rows per lane,
 // every load issued before any value is used.  ENCODE false: varying bits
 // (as so_bits); true: records (as so_encode).
 // ---------------------------------------------------------------------------
 template <bool ENCODE>
-__global__ __launch_bounds__(256) void so_scan_fast(const SLaunch *__restrict__ L) {
-    constexpr int U = 4;
+__global__ __launch_bounds__(512) void so_scan_fast(const SLaunch *__restrict__ L) {
+    // 1024 rows per block and round (the encode's per-block first records
+    // count the bits pass's blocks): two rows per lane over 512 threads keeps
+    // the words in flight within four waves per SIMD
+    constexpr int U = 2;
     __shared__ Run s_runs[kMaxRuns];
     __shared__ uint32_t s_cur;
     const QueryDesc *__restrict__ q = &L->q;
@@ -2241,7 +2244,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
         } else {
             if (L.fast) {
                 KernelTimer kt("so_bits_fast");
-                hipLaunchKernelGGL(so_scan_fast<false>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                hipLaunchKernelGGL(so_scan_fast<false>, dim3(cus * 8), dim3(512), 0, R.stream, dL);
                 launch_check();
             } else {
                 KernelTimer kt("so_bits");
@@ -2376,7 +2379,7 @@ pmh_result *run_sort(Prep &P, const GroupByPlan &plan, uint32_t flags) {
             dpl = Payload{L.rb, {L.vb[0], L.vb[1]}, L.fb};
             if (L.fast) {
                 KernelTimer kt("so_encode_fast");
-                hipLaunchKernelGGL(so_scan_fast<true>, dim3(cus * 8), dim3(256), 0, R.stream, dL);
+                hipLaunchKernelGGL(so_scan_fast<true>, dim3(cus * 8), dim3(512), 0, R.stream, dL);
                 launch_check();
             } else {
                 KernelTimer kt("so_encode");

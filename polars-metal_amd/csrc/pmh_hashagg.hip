@@ -1406,7 +1406,7 @@ __global__ __launch_bounds__(1024) void rp_scan2(const HLaunch *__restrict__ L, 
     if (c + 1 == P2) L->part_start[P2] = base + s;
 }
 
-__global__ __launch_bounds__(kPartThreads) void rp_scatter2(const HLaunch *__restrict__ L, const Spill *src_p,
+__global__ __launch_bounds__(kPartThreads, 8) void rp_scatter2(const HLaunch *__restrict__ L, const Spill *src_p,
                                                            const Spill *dst_p, uint32_t nd, int shift) {
     extern __shared__ __align__(16) uint8_t smem[];
     __shared__ uint32_t run[kRadix];     // rows of each digit in this tile

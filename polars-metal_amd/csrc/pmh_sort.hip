@@ -2122,7 +2122,7 @@ __global__ __launch_bounds__(256) void so_emit(const SLaunch *__restrict__ L) {
 // Outputs of the bucket fold: bucket b's groups sit at sparse index
 // bstart[b] + j and go to output row bpre[b] + j (bucket order is key order).
 // One wave per bucket, so a block has four buckets in flight.
-__global__ __launch_bounds__(256) void so_emit_buckets(const SLaunch *__restrict__ L, const uint64_t *bstart,
+__global__ __launch_bounds__(256, 8) void so_emit_buckets(const SLaunch *__restrict__ L, const uint64_t *bstart,
                                                      const uint32_t *bg, const uint32_t *bpre, uint32_t nb) {
     const QueryDesc *__restrict__ q = &L->q;
     const bool two = L->kw == 2;

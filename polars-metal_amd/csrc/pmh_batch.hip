@@ -321,7 +321,7 @@ __global__ void bt_fill(const BLaunch *__restrict__ L) {
 // Tiles of every query: the histogram pass counts selected rows per (query,
 // partition); the scatter pass writes them partition-contiguous as records
 // (key lane, value bits, flags).  A thread takes 16 rows of its tile (at
-// most 8192 rows, 512 threads) in two passes of 8, each pass issuing all its
+// most 8192 rows, 512 threads) in passes of 8 (the count pass: of 4), each issuing all its
 // loads before using any.  The scatter pass ranks the tile's rows within
 // their partitions, then lists them in partition order in LDS, 4 bytes a row
 // (value validity bits << 25 | partition << 13 | row in tile) beside their
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(512) void bt_tiles(const BLaunch *__restrict__ L) {
     __shared__ uint32_t s_sel, s_wave[16];
     // counts; then (scatter) each run's start in the list
     uint32_t *hist = SCATTER ? (uint32_t *)smem : s_hist;
-    constexpr int U = 8, kPass = 2;
+    constexpr int U = SCATTER ? 8 : 4, kPass = 16 / U;  // 16 rows a thread; the count pass holds fewer words
     const int nv = L->q.n_vals;
     const int vtag[2] = {L->q.cols[L->q.vals[0].col].tag, nv > 1 ? L->q.cols[L->q.vals[1].col].tag : 0};
     const uint32_t bs = blockDim.x, tid = threadIdx.x;

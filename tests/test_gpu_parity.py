@@ -640,9 +640,9 @@ SORT_BUCKET_CASES = ["highcard", "wide", "zipf_skew", "i32_small"]
 @pytest.mark.parametrize("case", SORT_BUCKET_CASES)
 def test_sort_bucket_finish(native, case):
     """The sort engine's MSD finish (radix passes over the key's top bits,
-    then each bucket sorted in LDS) against the oracle and against whole-key
-    LSD passes; skewed keys overflow a bucket and fall back to the LSD
-    passes."""
+    most significant digit first or stable LSD, then each bucket sorted in
+    LDS) against the oracle and against whole-key LSD passes; skewed keys
+    overflow a bucket and fall back to the LSD passes."""
     rng = np.random.default_rng(SORT_BUCKET_CASES.index(case) + 40)
     if case == "highcard":
         frame, plan = synth.highcard(1_500_000, seed=41, pool=700_000)
@@ -688,6 +688,10 @@ def test_sort_bucket_finish(native, case):
     got_wb = run_gpu(native, dict(plan, options={"method": "sort", "sort_fold": 0}), frame)
     assert native.last_stats()["n_passes"] == st_b["n_passes"]
     assert_results_equal(got_wb, want, keys, label=f"sortw/{case}")
+    # top-bit passes as stable LSD passes (not most significant digit first)
+    got_s = run_gpu(native, dict(plan, options={"method": "sort", "sort_msd": 0}), frame)
+    assert native.last_stats()["n_passes"] == st_b["n_passes"]
+    assert_results_equal(got_s, want, keys, label=f"sorts/{case}")
 
 
 @pytest.mark.parametrize("case", ["high_bit", "low_bit", "null_key", "none"])

@@ -27,14 +27,19 @@
 //                     rows with LDS counters, and each partition is laid out
 //                     as eight per-XCD sub-regions so the runs that meet in a
 //                     sub-region merge in one XCD's L2.
+//   rp_tiles2/rp_hist2/rp_sum2/rp_scan2/rp_scatter2
+//                     up to 2^17 partitions: the unstable pass takes the low
+//                     bits, then a second unstable pass takes the next 7
+//                     inside each first-pass bucket (tiles never cross one),
+//                     with the same per-XCD sub-regions; its counts give the
+//                     partition starts.
 //   rp_hist/rp_scan/rp_scatter
-//                     more partitions: the unstable pass takes the low bits
-//                     and stable LSD passes (7 bits each) the rest: histogram
-//                     per tile, one scan per digit, then a scatter that ranks
-//                     rows with wave multisplit ballots, stages the tile in
-//                     LDS in digit order and writes each digit run
-//                     contiguously; hs_part_bounds then finds the partition
-//                     starts.
+//                     the general multi-pass form: stable LSD passes (7 bits
+//                     each) after the unstable one: histogram per tile, one
+//                     scan per digit, then a scatter that ranks rows with
+//                     wave multisplit ballots, stages the tile in LDS in
+//                     digit order and writes each digit run contiguously;
+//                     hs_part_bounds then finds the partition starts.
 //   hs_part_agg       one block per partition folds the partition's tail rows
 //                     into an LDS table and writes its groups straight to the
 //                     outputs; a row that finds no LDS slot goes to the global

@@ -19,11 +19,14 @@
 //   3. so_hist / so_scan / so_scatter per 8-bit digit: a stable LSD pass with
 //      per-tile digit counts, a per-digit scan over tiles, and a tile-local
 //      stable rank (wave multisplit ballots + LDS counters) staged in LDS so
-//      each digit run leaves as contiguous writes.
-//   4. so_bound / so_bscan: group starts (compressed key differs from the
-//      previous record), counted per tile and scanned, so a record's group is
-//      its number of preceding starts: groups come out in key order, like
-//      the reference's segment ids.
+//      each digit run leaves as contiguous writes.  Large queries sort only
+//      the key's top bits, most significant digit first where that takes
+//      at most two passes (so_msd_*), and finish each bucket those bits
+//      define in LDS (3b: so_bucket_fold folds its groups there).
+//   4. so_bound / so_count_sum / so_count_scan: group starts (compressed key
+//      differs from the previous record), counted per tile and scanned, so a
+//      record's group is its number of preceding starts: groups come out in
+//      key order, like the reference's segment ids.
 //   5. so_reduce: each record gathers its row's values and folds them into
 //      its group's state arrays; a wave first reduces each run of equal
 //      groups among its 64 lanes (segmented shuffle scans), so one atomic per

@@ -883,25 +883,22 @@ __device__ __forceinline__ uint32_t msd_ntiles(const MsdPass &M, uint32_t nbk) {
 }
 
 // Records per (bucket, digit, XCD slot).  nbk: first-pass buckets (second
-// pass).  Each wave counts into its own LDS histogram; the first pass adds
-// them up over all the block's tiles (one bucket), the second per tile.
+// pass).  The first pass takes tile t on block t mod grid, as the scatter
+// does, and each wave counts into its own LDS histogram over all the block's
+// tiles.  The second pass only has to credit each tile to the slot of the
+// scatter block that takes it ((t mod grid) mod 8), so a block counts a
+// contiguous range of tiles (one or two buckets) into per-slot LDS counts
+// and flushes them when the bucket changes.
 __global__ __launch_bounds__(kSortThreads) void so_msd_hist(const SLaunch *__restrict__ L, const uint64_t *klo,
                                                           const uint64_t *khi, const MsdPass M, uint32_t nbk) {
-    __shared__ uint32_t hist[kWaves][kDigits];
+    __shared__ uint32_t hist[kWaves][kDigits];  // first pass: [wave][digit]; second: [slot][digit]
     const bool two = L->kw == 2;
-    const uint32_t tid = threadIdx.x, slot = blockIdx.x & 7, wave = tid >> 6;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
     const uint32_t ntiles = msd_ntiles(M, nbk);
     const uint64_t dmask = M.nd - 1;
-    auto flush = [&](uint32_t b) {
-        if (tid < M.nd) {
-            uint32_t c = 0;
-            for (int w = 0; w < kWaves; ++w) c += hist[w][tid];
-            if (c) atomicAdd(&M.xtot[((uint64_t)b * M.nd + tid) * 8 + slot], c);
-        }
-    };
     for (uint32_t i = tid; i < kWaves * kDigits; i += kSortThreads) (&hist[0][0])[i] = 0;
     __syncthreads();
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    auto count = [&](uint32_t t, uint32_t row) {
         uint32_t b, rows;
         uint64_t begin;
         msd_tile(L, M, t, b, begin, rows);
@@ -915,19 +912,41 @@ __global__ __launch_bounds__(kSortThreads) void so_msd_hist(const SLaunch *__res
         }
 #pragma unroll
         for (int k = 0; k < kSortRows; ++k)
-            if (k * kSortThreads + tid < rows) atomicAdd(&hist[wave][d[k]], 1u);
-        if (M.second) {
-            __syncthreads();
-            flush(b);
-            __syncthreads();
-            for (uint32_t i = tid; i < kWaves * kDigits; i += kSortThreads) (&hist[0][0])[i] = 0;
-            __syncthreads();
-        }
-    }
+            if (k * kSortThreads + tid < rows) atomicAdd(&hist[row][d[k]], 1u);
+    };
     if (!M.second) {
+        for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) count(t, wave);
         __syncthreads();
-        flush(0);
+        if (tid < M.nd) {
+            uint32_t c = 0;
+            for (int w = 0; w < kWaves; ++w) c += hist[w][tid];
+            if (c) atomicAdd(&M.xtot[tid * 8 + (blockIdx.x & 7)], c);
+        }
+        return;
     }
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = min(ntiles, t0 + per);
+    uint32_t cur = 0xFFFFFFFFu;  // the bucket the counts belong to
+    auto flush = [&]() {
+        __syncthreads();
+        for (uint32_t i = tid; i < 8 * kDigits; i += kSortThreads) {
+            const uint32_t c = (&hist[0][0])[i];
+            if (c) {
+                atomicAdd(&M.xtot[((uint64_t)cur * M.nd + (i % kDigits)) * 8 + i / kDigits], c);
+                (&hist[0][0])[i] = 0;
+            }
+        }
+        __syncthreads();
+    };
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint32_t b = M.treg[t];
+        if (b != cur) {
+            if (cur != 0xFFFFFFFFu) flush();
+            cur = b;
+        }
+        count(t, (t % gridDim.x) & 7);
+    }
+    if (cur != 0xFFFFFFFFu) flush();
 }
 
 // Per 1024 (bucket, digit) pairs, one thread each, the records they hold.
